@@ -1,0 +1,162 @@
+#!/usr/bin/env python3
+"""Benchmark of the step hot path on one MI355X.
+
+Workload (value): 1,048,576 LJ particles, periodic square box at number
+density 0.7, jittered lattice (seed 1), Gaussian velocities T = 1, cutoff
+2.5 sigma, 489x489 chunks, dt = 1e-3, FP64, newton_step! semantics (binning,
+two force evaluations with stale chunks, velocity Verlet, periodic walls).
+The state is resident in HBM when the timed region starts.
+
+Prints ONE JSON line.  `roofline` is for the pair-force kernel (both force
+evaluations of a step are launches of it): achieved = algorithmic bytes per
+launch / average launch duration measured with HIP events on the library's
+stream.  `cpu_baseline` is the CPU oracle (a restatement of the reference's
+loops, OpenMP over chunk columns like the reference's Threaded mode) timed
+on this host on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+for _p in (ROOT / "mavi.jl_amd", ROOT / "oracle"):
+    if str(_p) not in sys.path:
+        sys.path.insert(0, str(_p))
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+# Algorithmic work per particle-step (FP64, SoA, one force evaluation):
+#   bytes: state x,y,vx,vy read+write 64, F_old read + F_new write 32,
+#          binning 40 (read x,y 16, key 4, permutation 4, reordered x,y 16),
+#          force-phase read of reordered x,y 16                 -> 152 B
+#   flops: 6.87 unique pairs within rc (0.5 rho pi rc^2) x 21 + 14 -> 158
+B_STEP = 152.0
+B_FORCE_LAUNCH = (152.0 - 40.0) / 2.0  # the two pair-force launches carry all but binning
+FLOP_STEP = 158.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def read_traffic(name: str):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary."""
+    p = ROOT / "profiles" / name
+    if not p.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(pos, vel, space, dyn, ic, steps: int, threads: int):
+    import oracle
+
+    o = oracle.from_system(len(pos), space, dyn, ic)
+    t0 = time.perf_counter()
+    o.newton_steps(pos, vel, steps, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return len(pos) * steps / dt, dt
+
+
+def host_cores() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nside", type=int, default=1024)
+    ap.add_argument("--cpu-steps", type=int, default=4, help="steps of the CPU sample (all host cores)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--prof-steps", type=int, default=100)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported in this build "
+              "(the multi-GPU decomposition is a design note, not code).", file=sys.stderr)
+        sys.exit(2)
+
+    import mavi_hip as mv
+    from mavi_hip import workloads as wl
+
+    pos, vel, space, dyn, ic = wl.lj_periodic(nside=args.nside)
+    n = len(pos)
+    log(f"[bench] building system: {n} particles, chunks {ic.chunks_cfg.num_cols}x{ic.chunks_cfg.num_rows}")
+    system = mv.System(state=mv.SecondLawState(pos=pos, vel=vel), space_cfg=space, dynamic_cfg=dyn, int_cfg=ic)
+    dev = system.dev
+    log(f"[bench] warmup {args.warmup} steps")
+    dev.step(args.warmup)
+    log(f"[bench] timed {args.steps} steps")
+    t0 = time.perf_counter()
+    dev.step(args.steps)  # returns after the library's stream is synchronised
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    value = n * args.steps / wall
+    ms_per_step = wall / args.steps * 1e3
+
+    log(f"[bench] event-timed pass, {args.prof_steps} steps")
+    dev.prof_enable(True)
+    dev.step(args.prof_steps)
+    f_ms, f_n = dev.prof_get(mv._lib.MAVI_PROF_FORCE)
+    b_ms, b_n = dev.prof_get(mv._lib.MAVI_PROF_BIN)
+    dev.prof_enable(False)
+    f_avg_s = f_ms / max(f_n, 1) / 1e3
+    b_avg_s = b_ms / max(b_n, 1) / 1e3
+    achieved = B_FORCE_LAUNCH * n / f_avg_s / 1e9
+    traffic = read_traffic("pair_force_lj_1M_traffic.txt")
+    step_gbs = B_STEP * n / (wall / args.steps) / 1e9
+    fp64_tflops = FLOP_STEP * n / (wall / args.steps) / 1e12
+
+    cpu = None
+    if not args.no_cpu:
+        threads = host_cores()
+        log(f"[bench] CPU oracle sample: {args.cpu_steps} steps of the same system, {threads} threads")
+        v, dt = cpu_baseline(pos, vel, space, dyn, ic, args.cpu_steps, threads)
+        cpu = {"value": v, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+               "sample": f"{n} particles x {args.cpu_steps} newton_step! ({dt:.1f} s), OpenMP over chunk "
+                         "columns like Threaded mode; restatement, not reference (no Julia on the host)"}
+
+    out = {
+        "metric": "particle-steps/sec on 1 GPU: plain FP64 (1M LJ, periodic, rho=0.7, rc=2.5)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"plain LJ, {n} particles, periodic square box, density 0.7, cutoff 2.5 sigma, "
+                               f"{ic.chunks_cfg.num_cols}x{ic.chunks_cfg.num_rows} chunks, dt 1e-3, "
+                               "newton_step! (velocity Verlet, 2 force evaluations per step)",
+                   "particles": n, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_pair (pair forces + Verlet/wall epilogue)",
+                     "avg_launch_us": f_avg_s * 1e6, "launches": f_n,
+                     "bytes_per_launch_alg": B_FORCE_LAUNCH * n},
+        "step_rates": {"alg_GBs": step_gbs, "fp64_alg_TFLOPs": fp64_tflops,
+                       "fp64_frac": fp64_tflops / FP64_PEAK_TFLOPS, "bin_avg_us": b_avg_s * 1e6},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = value / cpu["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,153 @@
+/*
+ * mavi_hip.h -- C ABI of libmavi_hip.so, the MI355X (gfx950) step hot path of
+ * Mavi.jl.  Plain C types only: host arrays are caller-owned and never kept
+ * after a call returns; device buffers, streams and graphs are owned by the
+ * library.  Every function returns MAVI_OK (0) or an MAVI_E_* status; the
+ * message of the last failure on a context is mavi_last_error(ctx).
+ *
+ * Host array layout is Julia's Vector{SVector{2,Float64}}: x0,y0,x1,y1,...
+ * Particle ids are the slot indices of that vector (0-based here, 1-based in
+ * Julia); for rings, slot = (ring-1)*Nmax + p as in to_scalar_idx
+ * (src/rings/states.jl:137-140).
+ *
+ * Each entry point names the reference function whose semantics it
+ * reproduces (paths relative to the Mavi.jl repository root).
+ */
+#ifndef MAVI_HIP_H
+#define MAVI_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MAVI_ABI_VERSION 1
+
+/* Status codes. */
+#define MAVI_OK 0
+#define MAVI_E_CELL_OVERFLOW 1 /* a chunk holds more than nc particles: the
+                                  BoundsError of src/chunks.jl:144-146 */
+#define MAVI_E_OUT_OF_BOX 2    /* a particle maps outside the chunk grid: the
+                                  BoundsError of src/chunks.jl:144 / the throw
+                                  of src/systems.jl:76-79 */
+#define MAVI_E_NONFINITE 3     /* NaN/Inf position met while binning */
+#define MAVI_E_HIP 4           /* HIP runtime failure */
+#define MAVI_E_BADARG 5        /* invalid descriptor or argument */
+#define MAVI_E_UNSUPPORTED 6   /* configuration outside the GPU path (the
+                                  Julia glue keeps the CPU method for it) */
+
+/* SpaceCfg wall types (src/configs.jl:238-239). */
+#define MAVI_WALL_RIGID 0
+#define MAVI_WALL_PERIODIC 1
+
+/* Built-in DynamicCfg kinds. */
+#define MAVI_POT_LJ 0        /* LenJonesCfg, src/configs.jl:380-397 */
+#define MAVI_POT_HARMTRUNC 1 /* HarmTruncCfg, src/configs.jl:341-368 */
+#define MAVI_POT_RINGS 2     /* RingsCfg + HarmTruncCfg/InteractionMatrix,
+                                src/rings/configs.jl:26-164 */
+
+typedef struct mavi_ctx mavi_ctx;
+typedef struct mavi_sys mavi_sys;
+
+/* Everything the step needs from System/SpaceCfg/DynamicCfg/IntCfg/Chunks. */
+typedef struct {
+    int64_t n;              /* length(state.pos): particle slots */
+    double bottom_left[2];  /* RectangleCfg.bottom_left */
+    double size[2];         /* RectangleCfg.length, .height */
+    int32_t wall;           /* MAVI_WALL_* */
+    int32_t potential;      /* MAVI_POT_* */
+    double pot[4];          /* LJ: sigma, epsilon, rcut (<= 0: none, the
+                               reference's LenJonesCfg); HT: k_rep, k_atr,
+                               dist_eq, dist_max */
+    double dt;              /* IntCfg.dt */
+    int32_t cols, rows;     /* ChunksCfg(num_cols, num_rows); 0,0 = chunks nothing */
+    int32_t nc;             /* chunk capacity (src/chunks.jl:32-34), checked */
+    int32_t reserved0;
+    double radius;          /* particle_radius(dynamic_cfg) for RigidWalls */
+    /* Rings (potential == MAVI_POT_RINGS) */
+    int32_t nmax;           /* num_max_particles(state) */
+    int32_t nrings;         /* size(rings_pos, 2) */
+    int32_t ntypes;         /* RingsCfg.num_types */
+    int32_t reserved1;
+    const int32_t *ring_types; /* state.types (1-based), NULL if untyped */
+    const int32_t *type_np;    /* particles per ring for each type */
+    const double *type_par;    /* per type: p0, relax_time, vo, mobility,
+                                  rot_diff, k_area, k_spring, l_spring */
+    const double *interaction; /* ntypes x ntypes HarmTrunc (k_rep, k_atr,
+                                  dist_eq, dist_max), column-major like
+                                  InteractionMatrix.matrix */
+    uint64_t seed;          /* device rotational-noise stream (Philox4x32-10) */
+} mavi_sys_desc;
+
+/* Library / context ------------------------------------------------------ */
+int mavi_abi_version(void);
+/* One context per host task; it owns one HIP stream on `device`. */
+int mavi_ctx_create(int device, mavi_ctx **out);
+int mavi_ctx_destroy(mavi_ctx *ctx);
+const char *mavi_last_error(const mavi_ctx *ctx);
+
+/* System: replaces System(...) construction, src/systems.jl:73-114 and
+ * RingsSystem, src/rings/rings.jl:231-291 (device side only). */
+int mavi_sys_create(mavi_ctx *ctx, const mavi_sys_desc *desc, mavi_sys **out);
+int mavi_sys_destroy(mavi_sys *sys);
+
+/* State transfer (AoS host <-> device).  vel is SecondLawState.vel, pol is
+ * RingsState.pol (nrings angles); either may be NULL. */
+int mavi_sys_upload(mavi_sys *sys, const double *pos, const double *vel, const double *pol);
+int mavi_sys_download(mavi_sys *sys, double *pos, double *vel, double *pol);
+
+/* update_chunks!(chunks), src/chunks.jl:150-161 (+ src/integration.jl:54-59,
+ * src/rings/integration.jl:18-23).  Bins the current positions; when the
+ * arrays are given, exports chunks.chunk_particles (nc, rows, cols) and
+ * chunks.num_particles_in_chunk (rows, cols), column-major, 1-based ids,
+ * exactly the reference layout. */
+int mavi_bin(mavi_sys *sys, int64_t *chunk_particles, int64_t *counts);
+
+/* clean_forces! + update_chunks! + calc_forces!(system), src/integration.jl:112-226
+ * (plain), forces!(system) src/rings/integration.jl:197-226 (rings: pair +
+ * springs + area).  Writes get_forces(system) to forces (2n doubles). */
+int mavi_forces(mavi_sys *sys, double *forces);
+
+/* get_forces(system) as left by the last step (2n doubles). */
+int mavi_get_forces(mavi_sys *sys, double *forces);
+
+/* nsteps x newton_step! (src/integration.jl:507-515) for LJ/HT systems, or
+ * nsteps x Rings.Integration.step! (src/rings/integration.jl:522-543) for
+ * rings with device rotational noise.  State stays resident. */
+int mavi_step(mavi_sys *sys, int64_t nsteps);
+
+/* Rings step with host-supplied standard-normal draws: noise[step*nrings + r]
+ * replaces randn(system.rng) of src/rings/integration.jl:348.  NULL = zero. */
+int mavi_rings_step(mavi_sys *sys, int64_t nsteps, const double *noise);
+
+/* RingsInfo: areas (nrings), cms (2*nrings), continuos_pos (2*nmax*nrings),
+ * src/rings/rings.jl:118-128; any may be NULL. */
+int mavi_rings_info(mavi_sys *sys, double *areas, double *cms, double *continuous_pos);
+
+/* kinetic_energy(state) and potential_energy(system, ::LenJonesCfg),
+ * src/quantities.jl:12-18, 46-66 (PE over all pairs, as the reference). */
+int mavi_observables(mavi_sys *sys, double *ke, double *pe);
+
+/* nsteps x newton_step!, recording kinetic and potential energy after every
+ * step (ke, pe: nsteps doubles each; pe may be NULL). */
+int mavi_run_record(mavi_sys *sys, int64_t nsteps, double *ke, double *pe);
+
+/* Kernel timing with HIP events on the context's stream.  While enabled,
+ * steps launch kernels directly (no graph) and every launch of the timed
+ * kernel classes is bracketed by events.  mavi_prof_get returns, for kernel
+ * class `which`, the summed duration in ms and the number of launches. */
+#define MAVI_PROF_BIN 0
+#define MAVI_PROF_FORCE 1    /* pair-force kernels (both evaluations) */
+#define MAVI_PROF_RINGS 2    /* per-ring springs/area/overdamped kernel */
+#define MAVI_PROF_NCLASS 3
+int mavi_prof_enable(mavi_sys *sys, int on);
+int mavi_prof_get(mavi_sys *sys, int which, double *ms_total, int64_t *launches);
+
+/* Force-kernel tile shape: 0 = library default; otherwise the workgroup size
+ * (64, 128, 256, 512) used by the pair-force kernels.  Used by the autotuner. */
+int mavi_sys_set_tile(mavi_sys *sys, int32_t workgroup);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

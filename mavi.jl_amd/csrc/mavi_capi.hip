@@ -1,0 +1,500 @@
+// mavi_capi.hip -- the extern "C" boundary of libmavi_hip.so (include/mavi_hip.h).
+#include <math.h>
+#include <string.h>
+
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "mavi_internal.h"
+
+namespace {
+
+int set_err(mavi_ctx *ctx, int code, const std::string &msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                                  \
+    do {                                                                                                   \
+        hipError_t e_ = (expr);                                                                            \
+        if (e_ != hipSuccess)                                                                              \
+            return set_err((ctx), MAVI_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(T **p, int64_t count) {
+    *p = nullptr;
+    if (count <= 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, sizeof(T) * (size_t)count);
+    if (e == hipSuccess) e = hipMemset(*p, 0, sizeof(T) * (size_t)count);
+    return e;
+}
+
+template <typename T>
+void dfree(T *&p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+// Turn device flags raised since the last check into a status.
+int check_flags(mavi_sys *s) {
+    mavi_ctx *ctx = s->ctx;
+    uint32_t h = 0;
+    HIPCHK(ctx, hipMemcpyAsync(&h, s->flags, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (h == 0) return MAVI_OK;
+    HIPCHK(ctx, hipMemsetAsync(s->flags, 0, sizeof(uint32_t), ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (h & MAVI_FLAG_NONFINITE)
+        return set_err(ctx, MAVI_E_NONFINITE, "non-finite particle position met while binning");
+    if (h & MAVI_FLAG_OUT_OF_BOX)
+        return set_err(ctx, MAVI_E_OUT_OF_BOX, "particle outside the chunk grid (BoundsError in update_chunks!)");
+    return set_err(ctx, MAVI_E_CELL_OVERFLOW,
+                   "a chunk holds more particles than its capacity nc (BoundsError in update_chunks!)");
+}
+
+bool is_pow2_wg(int32_t w) { return w == 64 || w == 128 || w == 256 || w == 512 || w == 1024; }
+
+}  // namespace
+
+double exact_gt_threshold(double rc) {
+    if (!(rc > 0.0) || isinf(rc)) return INFINITY;
+    double t = rc * rc;
+    while (sqrt(t) > rc) t = nextafter(t, 0.0);
+    while (sqrt(nextafter(t, INFINITY)) <= rc) t = nextafter(t, INFINITY);
+    return t;
+}
+
+double exact_lt_threshold(double deq) {
+    if (!(deq > 0.0)) return 0.0;
+    double t = deq * deq;
+    while (sqrt(t) < deq) t = nextafter(t, INFINITY);
+    while (sqrt(nextafter(t, 0.0)) >= deq) t = nextafter(t, 0.0);
+    return t;
+}
+
+extern "C" {
+
+int mavi_abi_version(void) { return MAVI_ABI_VERSION; }
+
+int mavi_ctx_create(int device, mavi_ctx **out) {
+    if (!out) return MAVI_E_BADARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return MAVI_E_HIP;
+    if (device < 0 || device >= ndev) return MAVI_E_BADARG;
+    mavi_ctx *ctx = new mavi_ctx();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return MAVI_E_HIP;
+    }
+    *out = ctx;
+    return MAVI_OK;
+}
+
+int mavi_ctx_destroy(mavi_ctx *ctx) {
+    if (!ctx) return MAVI_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return MAVI_OK;
+}
+
+const char *mavi_last_error(const mavi_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int mavi_sys_destroy(mavi_sys *s) {
+    if (!s) return MAVI_OK;
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+    if (s->graph) (void)hipGraphExecDestroy(s->graph);
+    dfree(s->pos); dfree(s->vel); dfree(s->force);
+    dfree(s->key); dfree(s->rank); dfree(s->cnt); dfree(s->start); dfree(s->bsum);
+    dfree(s->perm); dfree(s->scell); dfree(s->ps); dfree(s->ps2); dfree(s->f1);
+    dfree(s->flags); dfree(s->active_ids);
+    dfree(s->pol); dfree(s->areas); dfree(s->cms); dfree(s->cms_next); dfree(s->cpos);
+    dfree(s->rtype); dfree(s->types); dfree(s->inter); dfree(s->noise); dfree(s->red);
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    delete s;
+    return MAVI_OK;
+}
+
+int mavi_sys_create(mavi_ctx *ctx, const mavi_sys_desc *d, mavi_sys **out) {
+    if (!ctx || !d || !out) return MAVI_E_BADARG;
+    *out = nullptr;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (d->n <= 0 || d->n > (int64_t)0x7fffffff / 2)
+        return set_err(ctx, MAVI_E_BADARG, "n must be in [1, 2^30)");
+    if (!(d->size[0] > 0.0) || !(d->size[1] > 0.0)) return set_err(ctx, MAVI_E_BADARG, "box size must be > 0");
+    if (d->wall != MAVI_WALL_RIGID && d->wall != MAVI_WALL_PERIODIC)
+        return set_err(ctx, MAVI_E_UNSUPPORTED, "only RigidWalls and PeriodicWalls rectangles run on the GPU");
+    if (d->potential < MAVI_POT_LJ || d->potential > MAVI_POT_RINGS)
+        return set_err(ctx, MAVI_E_UNSUPPORTED, "unknown potential kind");
+    const bool chunks = d->cols > 0 || d->rows > 0;
+    if (chunks) {
+        if (d->cols <= 0 || d->rows <= 0) return set_err(ctx, MAVI_E_BADARG, "cols and rows must both be > 0");
+        if (d->wall == MAVI_WALL_PERIODIC && (d->cols < 3 || d->rows < 3))
+            return set_err(ctx, MAVI_E_UNSUPPORTED,
+                           "periodic chunk grids need >= 3 chunks per side (the reference stencil repeats "
+                           "neighbours below that)");
+        if ((int64_t)d->cols * d->rows > 0x3fffffff) return set_err(ctx, MAVI_E_BADARG, "chunk grid too large");
+        if (d->nc <= 0) return set_err(ctx, MAVI_E_BADARG, "chunk capacity nc must be > 0");
+    }
+    if (!(d->dt > 0.0) && d->dt != 0.0) return set_err(ctx, MAVI_E_BADARG, "dt must be >= 0");
+
+    mavi_sys *s = new mavi_sys();
+    s->ctx = ctx;
+    SysParams &P = s->P;
+    P.n = (int32_t)d->n;
+    P.cols = chunks ? d->cols : 0;
+    P.rows = chunks ? d->rows : 0;
+    P.ncell = P.cols * P.rows;
+    P.nc = chunks ? d->nc : 0;
+    P.periodic = d->wall == MAVI_WALL_PERIODIC;
+    P.potential = d->potential;
+    P.blx = d->bottom_left[0];
+    P.bly = d->bottom_left[1];
+    P.L = d->size[0];
+    P.H = d->size[1];
+    P.cl = chunks ? d->size[0] / (double)P.cols : 0.0;  // src/chunks.jl:29-30
+    P.ch = chunks ? d->size[1] / (double)P.rows : 0.0;
+    P.halfL = d->size[0] / 2.0;
+    P.halfH = d->size[1] / 2.0;
+    P.dt = d->dt;
+    P.term = d->dt * d->dt / 2.0;
+    P.hdt = d->dt / 2.0;
+    P.radius = d->radius;
+    P.p0 = d->pot[0];
+    P.p1 = d->pot[1];
+    P.p2 = d->pot[2];
+    P.p3 = d->pot[3];
+    P.cut2 = INFINITY;
+    P.eq2 = 0.0;
+    if (d->potential == MAVI_POT_LJ) {
+        P.cut2 = exact_gt_threshold(d->pot[2]);
+        P.lj24eps = 24.0 * d->pot[1];
+        P.ljsig2 = d->pot[0] * d->pot[0];
+    } else if (d->potential == MAVI_POT_HARMTRUNC) {
+        P.cut2 = exact_gt_threshold(d->pot[3]);
+        P.eq2 = exact_lt_threshold(d->pot[2]);
+    }
+
+    std::vector<int32_t> active;
+    std::vector<int32_t> host_rtype;
+    std::vector<RingType> host_types;
+    std::vector<RingInter> host_inter;
+    if (d->potential == MAVI_POT_RINGS) {
+        if (d->wall != MAVI_WALL_PERIODIC) {
+            delete s;
+            return set_err(ctx, MAVI_E_UNSUPPORTED, "rings run on the GPU with PeriodicWalls");
+        }
+        if (d->nmax <= 0 || d->nrings <= 0 || (int64_t)d->nmax * d->nrings != d->n || d->ntypes <= 0 ||
+            !d->type_np || !d->type_par || !d->interaction) {
+            delete s;
+            return set_err(ctx, MAVI_E_BADARG, "rings descriptor: need n == nmax*nrings and type tables");
+        }
+        P.nmax = d->nmax;
+        P.nrings = d->nrings;
+        P.ntypes = d->ntypes;
+        P.typed = d->ring_types != nullptr;
+        if (!P.typed && d->ntypes != 1) {
+            delete s;
+            return set_err(ctx, MAVI_E_BADARG, "untyped rings need ntypes == 1");
+        }
+        for (int t = 0; t < d->ntypes; t++) {
+            RingType T{};
+            const double *q = d->type_par + 8 * t;
+            T.p0 = q[0]; T.relax = q[1]; T.vo = q[2]; T.mu = q[3];
+            T.rot = q[4]; T.karea = q[5]; T.kspring = q[6]; T.lspring = q[7];
+            T.np = d->type_np[t];
+            if (T.np < 1 || T.np > d->nmax) {
+                delete s;
+                return set_err(ctx, MAVI_E_BADARG, "ring particle count outside [1, nmax]");
+            }
+            host_types.push_back(T);
+        }
+        for (int k = 0; k < d->ntypes * d->ntypes; k++) {
+            RingInter I{};
+            const double *q = d->interaction + 4 * k;
+            I.k_rep = q[0]; I.k_atr = q[1]; I.dist_eq = q[2]; I.dist_max = q[3];
+            I.max2 = exact_gt_threshold(q[3]);
+            I.eq2 = exact_lt_threshold(q[2]);
+            host_inter.push_back(I);
+        }
+        host_rtype.resize(d->nrings);
+        for (int r = 0; r < d->nrings; r++) {
+            int t = P.typed ? d->ring_types[r] - 1 : 0;
+            if (t < 0 || t >= d->ntypes) {
+                delete s;
+                return set_err(ctx, MAVI_E_BADARG, "ring type outside [1, ntypes]");
+            }
+            host_rtype[r] = t;
+            // FixRingsIds.p_ids (src/rings/states.jl:46-62): the first
+            // num_particles[type] slots of every ring, ascending.
+            for (int i = 0; i < host_types[t].np; i++) active.push_back(r * d->nmax + i);
+        }
+        P.na = (int32_t)active.size();
+    } else {
+        P.na = P.n;
+    }
+
+    hipError_t e = hipSuccess;
+#define ALLOC(ptr, cnt) \
+    if (e == hipSuccess) e = dalloc(&(ptr), (cnt))
+    ALLOC(s->pos, P.n);
+    ALLOC(s->vel, P.n);
+    ALLOC(s->force, P.n);
+    ALLOC(s->key, P.na);
+    ALLOC(s->rank, P.na);
+    ALLOC(s->perm, P.na);
+    ALLOC(s->scell, P.na);
+    ALLOC(s->ps, P.na);
+    ALLOC(s->ps2, P.na);
+    ALLOC(s->f1, P.na);
+    ALLOC(s->flags, 1);
+    ALLOC(s->cnt, (int64_t)P.ncell + 1);
+    ALLOC(s->start, (int64_t)P.ncell + 1);
+    s->nscan_blocks = (P.ncell + 1 + 1023) / 1024;
+    ALLOC(s->bsum, s->nscan_blocks);
+    if (P.potential == MAVI_POT_RINGS) {
+        ALLOC(s->active_ids, P.na);
+        ALLOC(s->pol, P.nrings);
+        ALLOC(s->areas, P.nrings);
+        ALLOC(s->cms, P.nrings);
+        ALLOC(s->cms_next, P.nrings);
+        ALLOC(s->cpos, P.n);
+        ALLOC(s->rtype, P.nrings);
+        ALLOC(s->types, P.ntypes);
+        ALLOC(s->inter, (int64_t)P.ntypes * P.ntypes);
+    }
+#undef ALLOC
+    if (e == hipSuccess) e = hipEventCreate(&s->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&s->ev1);
+    if (e == hipSuccess && P.potential == MAVI_POT_RINGS) {
+        e = hipMemcpy(s->active_ids, active.data(), sizeof(int32_t) * active.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(s->rtype, host_rtype.data(), sizeof(int32_t) * host_rtype.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(s->types, host_types.data(), sizeof(RingType) * host_types.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(s->inter, host_inter.data(), sizeof(RingInter) * host_inter.size(), hipMemcpyHostToDevice);
+    }
+    s->seed = d->seed;
+    if (e != hipSuccess) {
+        std::string m = std::string("allocation/upload failed: ") + hipGetErrorString(e);
+        mavi_sys_destroy(s);
+        return set_err(ctx, MAVI_E_HIP, m);
+    }
+    *out = s;
+    return MAVI_OK;
+}
+
+int mavi_sys_upload(mavi_sys *s, const double *pos, const double *vel, const double *pol) {
+    if (!s || !pos) return MAVI_E_BADARG;
+    mavi_ctx *ctx = s->ctx;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const size_t bytes = sizeof(double2) * (size_t)s->P.n;
+    HIPCHK(ctx, hipMemcpyAsync(s->pos, pos, bytes, hipMemcpyHostToDevice, ctx->stream));
+    if (vel) HIPCHK(ctx, hipMemcpyAsync(s->vel, vel, bytes, hipMemcpyHostToDevice, ctx->stream));
+    if (pol && s->P.potential == MAVI_POT_RINGS)
+        HIPCHK(ctx, hipMemcpyAsync(s->pol, pol, sizeof(double) * s->P.nrings, hipMemcpyHostToDevice, ctx->stream));
+    if (s->P.potential == MAVI_POT_RINGS) HIPCHK(ctx, mavi::launch_rings_info(s, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    s->has_state = true;
+    s->binned = false;
+    return MAVI_OK;
+}
+
+int mavi_sys_download(mavi_sys *s, double *pos, double *vel, double *pol) {
+    if (!s) return MAVI_E_BADARG;
+    mavi_ctx *ctx = s->ctx;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const size_t bytes = sizeof(double2) * (size_t)s->P.n;
+    if (pos) HIPCHK(ctx, hipMemcpyAsync(pos, s->pos, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    if (vel) HIPCHK(ctx, hipMemcpyAsync(vel, s->vel, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    if (pol && s->P.potential == MAVI_POT_RINGS)
+        HIPCHK(ctx, hipMemcpyAsync(pol, s->pol, sizeof(double) * s->P.nrings, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return MAVI_OK;
+}
+
+int mavi_bin(mavi_sys *s, int64_t *chunk_particles, int64_t *counts) {
+    if (!s) return MAVI_E_BADARG;
+    mavi_ctx *ctx = s->ctx;
+    if (s->P.cols == 0) return set_err(ctx, MAVI_E_BADARG, "system has no chunks");
+    if (!s->has_state) return set_err(ctx, MAVI_E_BADARG, "no state uploaded");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, mavi::launch_bin(s, ctx->stream));
+    int st = check_flags(s);
+    if (st != MAVI_OK) return st;
+    s->binned = true;
+    if (!chunk_particles && !counts) return MAVI_OK;
+    const int64_t ncell = s->P.ncell, nc = s->P.nc;
+    int64_t *dcp = nullptr, *dcnt = nullptr;
+    HIPCHK(ctx, dalloc(&dcp, nc * ncell));
+    hipError_t e = dalloc(&dcnt, ncell);
+    if (e != hipSuccess) { dfree(dcp); HIPCHK(ctx, e); }
+    if (chunk_particles) e = hipMemcpy(dcp, chunk_particles, sizeof(int64_t) * nc * ncell, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = mavi::launch_export_chunks(s, dcp, dcnt, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess && chunk_particles)
+        e = hipMemcpy(chunk_particles, dcp, sizeof(int64_t) * nc * ncell, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && counts) e = hipMemcpy(counts, dcnt, sizeof(int64_t) * ncell, hipMemcpyDeviceToHost);
+    dfree(dcp);
+    dfree(dcnt);
+    HIPCHK(ctx, e);
+    return MAVI_OK;
+}
+
+int mavi_forces(mavi_sys *s, double *forces) {
+    if (!s) return MAVI_E_BADARG;
+    mavi_ctx *ctx = s->ctx;
+    if (!s->has_state) return set_err(ctx, MAVI_E_BADARG, "no state uploaded");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (s->P.potential == MAVI_POT_RINGS) {
+        HIPCHK(ctx, mavi::launch_rings_forces(s, ctx->stream));
+    } else {
+        HIPCHK(ctx, mavi::launch_bin(s, ctx->stream));
+        HIPCHK(ctx, mavi::launch_forces_only(s, ctx->stream));
+    }
+    int st = check_flags(s);
+    if (st != MAVI_OK) return st;
+    if (forces)
+        HIPCHK(ctx, hipMemcpy(forces, s->force, sizeof(double2) * (size_t)s->P.n, hipMemcpyDeviceToHost));
+    return MAVI_OK;
+}
+
+int mavi_get_forces(mavi_sys *s, double *forces) {
+    if (!s || !forces) return MAVI_E_BADARG;
+    mavi_ctx *ctx = s->ctx;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipMemcpy(forces, s->force, sizeof(double2) * (size_t)s->P.n, hipMemcpyDeviceToHost));
+    return MAVI_OK;
+}
+
+static int steps_impl(mavi_sys *s, int64_t nsteps, const double *noise_host) {
+    mavi_ctx *ctx = s->ctx;
+    if (nsteps < 0) return set_err(ctx, MAVI_E_BADARG, "nsteps < 0");
+    if (!s->has_state) return set_err(ctx, MAVI_E_BADARG, "no state uploaded");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (s->P.potential != MAVI_POT_RINGS) {
+        for (int64_t k = 0; k < nsteps; k++) HIPCHK(ctx, mavi::launch_verlet_step(s, ctx->stream));
+    } else {
+        const int64_t nr = s->P.nrings;
+        if (noise_host) {
+            const int64_t need = nsteps * nr;
+            if (need > s->noise_cap) {
+                dfree(s->noise);
+                s->noise_cap = 0;
+                HIPCHK(ctx, dalloc(&s->noise, need));
+                s->noise_cap = need;
+            }
+            HIPCHK(ctx, hipMemcpyAsync(s->noise, noise_host, sizeof(double) * need, hipMemcpyHostToDevice,
+                                       ctx->stream));
+        }
+        for (int64_t k = 0; k < nsteps; k++) {
+            const double *nz = noise_host ? s->noise + k * nr : nullptr;
+            HIPCHK(ctx, mavi::launch_rings_step(s, nz, s->rng_step, ctx->stream));
+            s->rng_step++;
+        }
+    }
+    s->binned = false;
+    return check_flags(s);
+}
+
+int mavi_step(mavi_sys *s, int64_t nsteps) {
+    if (!s) return MAVI_E_BADARG;
+    return steps_impl(s, nsteps, nullptr);
+}
+
+int mavi_rings_step(mavi_sys *s, int64_t nsteps, const double *noise) {
+    if (!s) return MAVI_E_BADARG;
+    if (s->P.potential != MAVI_POT_RINGS) return set_err(s->ctx, MAVI_E_BADARG, "not a rings system");
+    if (!noise) {
+        // zero noise: stage zeros
+        mavi_ctx *ctx = s->ctx;
+        std::vector<double> z((size_t)(nsteps * s->P.nrings), 0.0);
+        (void)ctx;
+        return steps_impl(s, nsteps, z.data());
+    }
+    return steps_impl(s, nsteps, noise);
+}
+
+int mavi_rings_info(mavi_sys *s, double *areas, double *cms, double *continuous_pos) {
+    if (!s) return MAVI_E_BADARG;
+    mavi_ctx *ctx = s->ctx;
+    if (s->P.potential != MAVI_POT_RINGS) return set_err(ctx, MAVI_E_BADARG, "not a rings system");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (areas) HIPCHK(ctx, hipMemcpy(areas, s->areas, sizeof(double) * s->P.nrings, hipMemcpyDeviceToHost));
+    if (cms) HIPCHK(ctx, hipMemcpy(cms, s->cms, sizeof(double2) * s->P.nrings, hipMemcpyDeviceToHost));
+    if (continuous_pos)
+        HIPCHK(ctx, hipMemcpy(continuous_pos, s->cpos, sizeof(double2) * s->P.n, hipMemcpyDeviceToHost));
+    return MAVI_OK;
+}
+
+int mavi_observables(mavi_sys *s, double *ke, double *pe) {
+    if (!s) return MAVI_E_BADARG;
+    mavi_ctx *ctx = s->ctx;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const int nb = (s->P.n + 255) / 256;
+    if (s->red_cap < 2 * nb) {
+        dfree(s->red);
+        s->red_cap = 0;
+        HIPCHK(ctx, dalloc(&s->red, 2 * nb));
+        s->red_cap = 2 * nb;
+    }
+    HIPCHK(ctx, mavi::launch_observables(s, s->red, ctx->stream));
+    std::vector<double> h(2 * nb);
+    HIPCHK(ctx, hipMemcpyAsync(h.data(), s->red, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    double k = 0.0, p = 0.0;
+    for (int b = 0; b < nb; b++) {
+        k += h[2 * b];
+        p += h[2 * b + 1];
+    }
+    if (ke) *ke = k / 2.0;
+    if (pe) *pe = (s->P.potential == MAVI_POT_LJ) ? p * (4.0 * s->P.p1) : NAN;
+    return MAVI_OK;
+}
+
+int mavi_run_record(mavi_sys *s, int64_t nsteps, double *ke, double *pe) {
+    if (!s || !ke || s->P.potential == MAVI_POT_RINGS) return MAVI_E_BADARG;
+    for (int64_t k = 0; k < nsteps; k++) {
+        int st = mavi_step(s, 1);
+        if (st != MAVI_OK) return st;
+        double p = 0.0;
+        st = mavi_observables(s, &ke[k], &p);
+        if (st != MAVI_OK) return st;
+        if (pe) pe[k] = p;
+    }
+    return MAVI_OK;
+}
+
+int mavi_prof_enable(mavi_sys *s, int on) {
+    if (!s) return MAVI_E_BADARG;
+    s->prof = on != 0;
+    for (auto &p : s->profs) p = ProfSlot{};
+    return MAVI_OK;
+}
+
+int mavi_prof_get(mavi_sys *s, int which, double *ms_total, int64_t *launches) {
+    if (!s || which < 0 || which >= MAVI_PROF_NCLASS) return MAVI_E_BADARG;
+    if (ms_total) *ms_total = s->profs[which].ms;
+    if (launches) *launches = s->profs[which].launches;
+    return MAVI_OK;
+}
+
+int mavi_sys_set_tile(mavi_sys *s, int32_t workgroup) {
+    if (!s) return MAVI_E_BADARG;
+    if (workgroup == 0) workgroup = 256;
+    if (!is_pow2_wg(workgroup)) return set_err(s->ctx, MAVI_E_BADARG, "workgroup must be 64..1024, power of two");
+    s->wg = workgroup;
+    return MAVI_OK;
+}
+
+}  // extern "C"

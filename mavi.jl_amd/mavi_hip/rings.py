@@ -1,0 +1,235 @@
+"""Mirror of Mavi.Rings for the GPU path (src/rings/*.jl).
+
+  RingsCfg, HarmTruncCfg, InteractionMatrix   src/rings/configs.jl:26-164
+  RingsState                                  src/rings/states.jl:74-124
+  RingsSystem                                 src/rings/rings.jl:231-291
+  step(system)                                Rings.Integration.step!, src/rings/integration.jl:522-543
+  rectangular_grid, create_circle, random_pol src/rings/init_states.jl
+  get_ring_radius                             src/rings/utils.jl:5-7
+
+Host layout: ``rings_pos`` is an (nrings, nmax, 2) array, i.e. Julia's
+Matrix{SVector{2,T}}(nmax, nrings) in memory order, and ``pos`` is its
+(nrings*nmax, 2) reshape sharing memory (slot = ring*nmax + p).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from .configs import GPU, PeriodicWalls, RectangleCfg, SpaceCfg, chunk_capacity
+from .systems import Chunks, TimeInfo, base_desc, check_inside, wall_kind
+
+
+@dataclass
+class HarmTruncCfg:  # src/rings/configs.jl:26-57
+    k_rep: float
+    k_atr: float
+    dist_eq: float
+    dist_max: float
+
+    def params(self):
+        return [float(self.k_rep), float(self.k_atr), float(self.dist_eq), float(self.dist_max)]
+
+    def particle_radius(self):
+        return self.dist_eq / 2.0
+
+
+@dataclass
+class InteractionMatrix:  # src/rings/configs.jl:61-63; matrix[t1][t2]
+    matrix: list
+
+
+_FIELDS = ("p0", "relax_time", "vo", "mobility", "rot_diff", "k_area", "k_spring", "l_spring")
+
+
+class RingsCfg:  # src/rings/configs.jl:95-150
+    def __init__(self, *, p0, relax_time, vo, mobility, rot_diff, k_area, k_spring, l_spring,
+                 interaction_finder, num_particles=-1):
+        args = dict(p0=p0, relax_time=relax_time, vo=vo, mobility=mobility, rot_diff=rot_diff,
+                    k_area=k_area, k_spring=k_spring, l_spring=l_spring)
+        lens = [len(v) if np.ndim(v) else 1 for v in args.values()]
+        self.num_types = max(lens)
+        self.typed = self.num_types > 1 or any(np.ndim(v) for v in args.values())
+        for k, v in args.items():
+            arr = np.atleast_1d(np.asarray(v, dtype=np.float64))
+            if arr.size == 1 and self.num_types > 1:
+                arr = np.full(self.num_types, float(arr[0]))
+            setattr(self, k, arr)
+        self.num_particles = num_particles
+        self.interaction_finder = interaction_finder
+
+    def type_table(self) -> np.ndarray:
+        return np.stack([getattr(self, f) for f in _FIELDS], axis=1).astype(np.float64)  # (ntypes, 8)
+
+    def interaction_table(self) -> np.ndarray:
+        f = self.interaction_finder
+        nt = self.num_types
+        tab = np.zeros((nt, nt, 4))  # [t2, t1, :] so that memory is column-major [t1, t2]
+        for t1 in range(nt):
+            for t2 in range(nt):
+                cfg = f.matrix[t1][t2] if isinstance(f, InteractionMatrix) else f
+                tab[t2, t1] = cfg.params()
+        return tab
+
+    def particle_radius(self, type_=1):
+        f = self.interaction_finder
+        cfg = f.matrix[type_ - 1][type_ - 1] if isinstance(f, InteractionMatrix) else f
+        return cfg.particle_radius()
+
+    def particle_radii(self):
+        return [self.particle_radius(t + 1) for t in range(self.num_types)]
+
+
+class RingsState:  # src/rings/states.jl:74-124
+    def __init__(self, *, rings_pos, pol, num_particles=None, types=None):
+        rp = np.asarray(rings_pos, dtype=np.float64)
+        if rp.ndim == 3 and rp.shape[0] == 2 and rp.shape[2] != 2:
+            rp = np.transpose(rp, (2, 1, 0))  # Julia (2, nmax, nrings) -> (nrings, nmax, 2)
+        self.rings_pos = np.ascontiguousarray(rp)
+        self.num_rings, self.nmax = self.rings_pos.shape[:2]
+        self.pos = self.rings_pos.reshape(self.num_rings * self.nmax, 2)
+        self.pol = np.ascontiguousarray(np.asarray(pol, dtype=np.float64)).copy()
+        if num_particles is None:
+            num_particles = self.nmax
+        self.num_particles = num_particles
+        self.types = None if types is None else np.asarray(types, dtype=np.int32)
+
+    def ring_num_particles(self, ring_id):  # 1-based ring id
+        if self.types is None:
+            return int(self.num_particles)
+        return int(np.asarray(self.num_particles)[self.types[ring_id - 1] - 1])
+
+    def particle_ids(self) -> np.ndarray:
+        """get_particles_ids: FixRingsIds.p_ids (0-based slots)."""
+        if self.types is None:
+            return np.arange(len(self.pos))
+        ids = [r * self.nmax + i for r in range(self.num_rings) for i in range(self.ring_num_particles(r + 1))]
+        return np.asarray(ids)
+
+
+class RingsSystem:
+    """GPU-backed RingsSystem (src/rings/rings.jl:231-291)."""
+
+    def __init__(self, *, state: RingsState, space_cfg: SpaceCfg, dynamic_cfg: RingsCfg, int_cfg,
+                 time_info: TimeInfo | None = None, seed: int = 0, ctx: _lib.Context | None = None):
+        if dynamic_cfg.typed != (state.types is not None):
+            raise ValueError("DynamicCfg types and state.types disagree")
+        if not isinstance(space_cfg.wall_type, PeriodicWalls) or not isinstance(int_cfg.device, GPU):
+            raise _lib.UnsupportedOnGPU(_lib.MAVI_E_UNSUPPORTED,
+                                        "rings run on the GPU with PeriodicWalls rectangles")
+        ids = state.particle_ids()
+        ok, out_ids = check_inside(state.pos, space_cfg.geometry_cfg, ids)
+        if not ok:
+            raise ValueError(f"Particles with ids={list(out_ids + 1)} outside space.")
+        self.state, self.space_cfg, self.dynamic_cfg, self.int_cfg = state, space_cfg, dynamic_cfg, int_cfg
+        self.time_info = time_info or TimeInfo()
+        r = min(dynamic_cfg.particle_radii())  # minimum(particle_radius(dynamic_cfg)), src/systems.jl:26
+        self.chunks = None
+        if int_cfg.chunks_cfg is not None:
+            self.chunks = Chunks(int_cfg.chunks_cfg.num_cols, int_cfg.chunks_cfg.num_rows, space_cfg.geometry_cfg, r)
+        d = base_desc(len(state.pos), space_cfg, dynamic_cfg, int_cfg, self.chunks, r)
+        d.potential = _lib.MAVI_POT_RINGS
+        d.nmax, d.nrings, d.ntypes = state.nmax, state.num_rings, dynamic_cfg.num_types
+        np_types = np.atleast_1d(np.asarray(state.num_particles, dtype=np.int32))
+        self._tables = (
+            np.ascontiguousarray(np_types),
+            np.ascontiguousarray(dynamic_cfg.type_table()),
+            np.ascontiguousarray(dynamic_cfg.interaction_table()),
+            None if state.types is None else np.ascontiguousarray(state.types, dtype=np.int32),
+        )
+        import ctypes as C
+        d.type_np = self._tables[0].ctypes.data_as(C.POINTER(C.c_int32))
+        d.type_par = self._tables[1].ctypes.data_as(C.POINTER(C.c_double))
+        d.interaction = self._tables[2].ctypes.data_as(C.POINTER(C.c_double))
+        if self._tables[3] is not None:
+            d.ring_types = self._tables[3].ctypes.data_as(C.POINTER(C.c_int32))
+        d.seed = int(seed)
+        self.ctx = ctx or _lib.Context(int_cfg.device.device)
+        self.dev = _lib.DeviceSystem(self.ctx, d, keep=self._tables)
+        self.forces = np.zeros_like(state.pos)
+        self.upload()
+        # the constructor's init sequence (src/rings/rings.jl:280-288) ends with forces!
+        self.forces[:] = self.dev.forces()
+        self.areas, self.cms, self.continuos_pos = self.dev.rings_info()
+
+    def upload(self):
+        self.dev.upload(self.state.pos, None, self.state.pol)
+
+    def download(self):
+        self.dev.download(self.state.pos, None, self.state.pol)
+        self.forces[:] = self.dev.get_forces()
+        self.areas, self.cms, self.continuos_pos = self.dev.rings_info()
+
+    def advance(self, nsteps: int, noise=None):
+        if noise is None:
+            self.dev.step(nsteps)
+        else:
+            self.dev.rings_step(nsteps, noise)
+        self.time_info.num_steps += int(nsteps)
+        for _ in range(int(nsteps)):
+            self.time_info.time += self.int_cfg.dt
+
+    def calc_forces(self):
+        self.forces[:] = self.dev.forces()
+        return self.forces
+
+    def update_chunks(self):
+        cp, counts = self.dev.bin(export=True)
+        self.chunks.chunk_particles = cp
+        self.chunks.num_particles_in_chunk = counts
+        return cp, counts
+
+
+def step(system: RingsSystem, noise=None):
+    """Rings.Integration.step!; ``noise`` optionally supplies the nrings
+    standard-normal draws of this step (randn(system.rng) per ring)."""
+    system.upload()
+    system.advance(1, noise)
+    system.download()
+
+
+# --- init states (src/rings/init_states.jl, src/rings/utils.jl) ------------
+def get_ring_radius(p_radius, num_particles):
+    return (p_radius * 2) / (2 * (1 - math.cos(2 * math.pi / num_particles))) ** 0.5
+
+
+def create_circle(center, radius, num_p):
+    theta = 2 * math.pi / num_p
+    out = np.empty((num_p, 2))
+    for i in range(num_p):
+        out[i, 0] = radius * math.cos(i * theta) + center[0]
+        out[i, 1] = radius * math.sin(i * theta) + center[1]
+    return out
+
+
+def rectangular_grid(*, num_cols, num_rows, num_particles, p_radius, types=None, pad_x=0.0, pad_y=0.0,
+                     radius_k=1.0):
+    """Ring lattice, src/rings/init_states.jl:27-72; returns (rings_pos, geometry_cfg)."""
+    if types is None:
+        num_particles = [num_particles]
+        p_radius = [p_radius]
+        types = [1] * (num_cols * num_rows)
+    ring_r = [get_ring_radius(pr, n) for pr, n in zip(p_radius, num_particles)]
+    ring_length = [2 * (rr + pr) for rr, pr in zip(ring_r, p_radius)]
+    max_ring_r, max_ring_length, max_np = max(ring_r), max(ring_length), max(num_particles)
+    pad_x, pad_y = pad_x * max_ring_r, pad_y * max_ring_r
+    rings_pos = np.zeros((num_cols * num_rows, max_np, 2))
+    idx = 0
+    for col_id in range(num_cols):
+        for row_id in range(num_rows):
+            cx = pad_x / 2 + col_id * (max_ring_length + pad_x) + max_ring_length / 2
+            cy = pad_y / 2 + row_id * (max_ring_length + pad_x) + max_ring_length / 2
+            t = types[idx]
+            n = num_particles[t - 1]
+            rings_pos[idx, :n] = create_circle((cx, cy), ring_r[t - 1] * radius_k, n)
+            idx += 1
+    geometry_cfg = RectangleCfg(length=num_cols * (pad_x + max_ring_length),
+                                height=num_rows * (pad_y + max_ring_length))
+    return rings_pos, geometry_cfg
+
+
+def random_pol(num_rings, rng: np.random.Generator):
+    return rng.random(num_rings) * 2 * math.pi

@@ -1,0 +1,182 @@
+"""Mirror of Mavi.States / Mavi.Systems / Mavi.ChunksMod for the GPU path.
+
+System(...) (src/systems.jl:73-114) checks that every particle is inside the
+geometry, builds the chunk grid (src/systems.jl:14-28, src/chunks.jl:26-40),
+creates the device system and uploads the state.  State arrays live on the
+host as (N, 2) float64 arrays, the layout of Vector{SVector{2,Float64}}.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from .configs import (
+    GPU,
+    HarmTruncCfg,
+    LenJonesCfg,
+    PeriodicWalls,
+    RectangleCfg,
+    RigidWalls,
+    SpaceCfg,
+    as_points,
+    chunk_capacity,
+    particle_radius,
+)
+
+
+@dataclass
+class SecondLawState:  # src/states.jl:233-259
+    pos: np.ndarray
+    vel: np.ndarray
+
+    def __post_init__(self):
+        self.pos = as_points(self.pos).copy()
+        self.vel = as_points(self.vel).copy()
+        if self.pos.shape != self.vel.shape:
+            raise ValueError("pos and vel must have the same number of particles")
+
+
+@dataclass
+class TimeInfo:  # src/systems.jl:30-33
+    num_steps: int = 0
+    time: float = 0.0
+
+
+class Chunks:
+    """Chunk grid geometry (src/chunks.jl:10-40); the chunk arrays themselves
+    live on the device and are exported on request (update_chunks!)."""
+
+    def __init__(self, num_cols, num_rows, geometry_cfg: RectangleCfg, particle_r: float):
+        self.num_cols = int(num_cols)
+        self.num_rows = int(num_rows)
+        self.chunk_length = geometry_cfg.length / self.num_cols
+        self.chunk_height = geometry_cfg.height / self.num_rows
+        self.nc = chunk_capacity(self.chunk_length, self.chunk_height, particle_r)
+        self.geometry_cfg = geometry_cfg
+        self._dev = None
+        self.chunk_particles = None
+        self.num_particles_in_chunk = None
+
+
+def check_inside(pos: np.ndarray, geometry_cfg: RectangleCfg, ids=None):
+    """src/space_checks.jl:9-20, 53-60."""
+    bl = np.asarray(geometry_cfg.bottom_left)
+    tr = bl + np.asarray((geometry_cfg.length, geometry_cfg.height))
+    p = pos if ids is None else pos[ids]
+    out = np.nonzero(np.any(p < bl, axis=1) | np.any(p > tr, axis=1))[0]
+    if ids is not None:
+        out = np.asarray(ids)[out]
+    return len(out) == 0, out
+
+
+def gpu_kind(dynamic_cfg):
+    """The trait that routes a DynamicCfg to the GPU: built-in potentials only."""
+    if isinstance(dynamic_cfg, LenJonesCfg):
+        return _lib.MAVI_POT_LJ
+    if isinstance(dynamic_cfg, HarmTruncCfg):
+        return _lib.MAVI_POT_HARMTRUNC
+    return None
+
+
+def wall_kind(space_cfg: SpaceCfg):
+    if not isinstance(space_cfg.geometry_cfg, RectangleCfg):
+        return None
+    if isinstance(space_cfg.wall_type, RigidWalls):
+        return _lib.MAVI_WALL_RIGID
+    if isinstance(space_cfg.wall_type, PeriodicWalls):
+        return _lib.MAVI_WALL_PERIODIC
+    return None
+
+
+def base_desc(n, space_cfg, dynamic_cfg, int_cfg, chunks: Chunks | None, radius) -> _lib.SysDesc:
+    g = space_cfg.geometry_cfg
+    d = _lib.SysDesc()
+    d.n = int(n)
+    d.bottom_left[0], d.bottom_left[1] = g.bottom_left
+    d.size[0], d.size[1] = g.length, g.height
+    d.wall = wall_kind(space_cfg)
+    d.dt = float(int_cfg.dt)
+    if chunks is not None:
+        d.cols, d.rows, d.nc = chunks.num_cols, chunks.num_rows, chunks.nc
+    d.radius = float(radius)
+    return d
+
+
+class System:
+    """GPU-backed System (src/systems.jl:45-114)."""
+
+    def __init__(self, *, state: SecondLawState, space_cfg: SpaceCfg, dynamic_cfg, int_cfg,
+                 time_info: TimeInfo | None = None, ctx: _lib.Context | None = None):
+        ok, out_ids = check_inside(state.pos, space_cfg.geometry_cfg)
+        if not ok:
+            raise ValueError(f"Particles with ids={list(out_ids + 1)} outside space.")
+        kind = gpu_kind(dynamic_cfg)
+        wall = wall_kind(space_cfg)
+        if kind is None or wall is None or not isinstance(int_cfg.device, GPU):
+            raise _lib.UnsupportedOnGPU(
+                _lib.MAVI_E_UNSUPPORTED,
+                f"{type(dynamic_cfg).__name__} / {type(space_cfg.wall_type).__name__} / "
+                f"{type(int_cfg.device).__name__} is not on the GPU path; Mavi's own CPU "
+                "methods run it (the Julia integration dispatches there)",
+            )
+        self.state = state
+        self.space_cfg = space_cfg
+        self.dynamic_cfg = dynamic_cfg
+        self.int_cfg = int_cfg
+        self.time_info = time_info or TimeInfo()
+        r = particle_radius(dynamic_cfg)
+        self.chunks = None
+        if int_cfg.chunks_cfg is not None:
+            self.chunks = Chunks(int_cfg.chunks_cfg.num_cols, int_cfg.chunks_cfg.num_rows,
+                                 space_cfg.geometry_cfg, r)
+        d = base_desc(len(state.pos), space_cfg, dynamic_cfg, int_cfg, self.chunks, r)
+        d.potential = kind
+        if kind == _lib.MAVI_POT_LJ:
+            d.pot[0], d.pot[1] = dynamic_cfg.sigma, dynamic_cfg.epsilon
+            d.pot[2] = dynamic_cfg.cutoff if dynamic_cfg.cutoff else 0.0
+        else:
+            d.pot[0], d.pot[1] = dynamic_cfg.k_rep, dynamic_cfg.k_atr
+            d.pot[2], d.pot[3] = dynamic_cfg.dist_eq, dynamic_cfg.dist_max
+        self.ctx = ctx or _lib.Context(int_cfg.device.device)
+        self.dev = _lib.DeviceSystem(self.ctx, d)
+        self.forces = np.zeros_like(state.pos)
+        self.upload()
+        if self.chunks is not None:
+            self.chunks._dev = self.dev
+            self.update_chunks()
+
+    # --- host <-> device ---------------------------------------------------
+    def upload(self):
+        self.dev.upload(self.state.pos, self.state.vel)
+
+    def download(self):
+        self.dev.download(self.state.pos, self.state.vel)
+        self.forces[:] = self.dev.get_forces()
+
+    # --- reference operations ----------------------------------------------
+    def update_chunks(self):
+        """update_chunks!(system.chunks): exports the reference arrays."""
+        cp, counts = self.dev.bin(export=True)
+        self.chunks.chunk_particles = cp
+        self.chunks.num_particles_in_chunk = counts
+        return cp, counts
+
+    def calc_forces(self):
+        """clean_forces! + update_chunks! + calc_forces!(system)."""
+        self.forces[:] = self.dev.forces()
+        return self.forces
+
+    def get_forces(self):
+        return self.forces
+
+    def get_num_total_particles(self):
+        return len(self.state.pos)
+
+    def advance(self, nsteps: int):
+        """nsteps newton_step! with the state resident on the device."""
+        self.dev.step(nsteps)
+        self.time_info.num_steps += int(nsteps)
+        for _ in range(int(nsteps)):
+            self.time_info.time += self.int_cfg.dt

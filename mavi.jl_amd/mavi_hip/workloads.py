@@ -1,0 +1,96 @@
+"""Deterministic synthetic inputs for the benchmark workloads.
+
+Each generator states its RNG (numpy PCG64 via default_rng(seed)).  They
+return plain arrays plus the configs, so the same input feeds the GPU path
+and the CPU oracle.
+
+  lj_gas_1024        1024 LJ particles (32x32 lattice, offset 0.4), rigid
+                     walls, speed 0.2 at a uniform angle (seed 0), 17x17
+                     chunks, LJ cut at 2.5 sigma (cutoff=None gives the
+                     reference's stencil-ranged LJ), dt = 1e-3
+  lj_periodic        1024x1024 LJ particles, periodic square box at number
+                     density 0.7, jittered lattice (+-0.1 sigma, seed 1),
+                     Gaussian velocities T = 1 with zero net momentum,
+                     cutoff 2.5 sigma, chunks of side >= 2.5, dt = 1e-3
+  rings_mixture      active rings: nrings x 20 particles, periodic box at
+                     ring packing fraction 0.8 (nrings * pi * R_out^2 / L^2,
+                     R_out = regular 20-gon circumradius + particle radius),
+                     springs k = 20, l = 1, HarmTrunc repulsion k = 10 with
+                     interaction radius 1 (dist_eq = dist_max = 1), two types
+                     50/50 (seed 2) whose target areas are 1.0 and 0.9 of the
+                     regular n-gon area, v0 = 1, D_r = 0.1, dt = 1e-3
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .configs import ChunksCfg, GPU, IntCfg, LenJonesCfg, PeriodicWalls, RectangleCfg, RigidWalls, SpaceCfg, particle_radius
+from .init_states import rectangular_grid
+
+
+def lj_gas_1024(nx=32, ny=32, cutoff=2.5):
+    dyn = LenJonesCfg(sigma=1.0, epsilon=1.0, cutoff=cutoff)
+    r = particle_radius(dyn)
+    pos, geom = rectangular_grid(nx, ny, 0.4, r)
+    rng = np.random.default_rng(0)
+    ang = rng.random(len(pos)) * 2 * math.pi
+    vel = np.stack([0.2 * np.cos(ang), 0.2 * np.sin(ang)], axis=1)
+    cols = int(geom.length // 2.5)  # chunk side >= 2.5 sigma (17 for 32x32)
+    rows = int(geom.height // 2.5)
+    space = SpaceCfg(wall_type=RigidWalls(), geometry_cfg=geom)
+    int_cfg = IntCfg(dt=1e-3, chunks_cfg=ChunksCfg(num_cols=cols, num_rows=rows), device=GPU())
+    return pos, vel, space, dyn, int_cfg
+
+
+def lj_periodic(nside=1024, density=0.7, rc=2.5, seed=1, cutoff=True):
+    n = nside * nside
+    L = math.sqrt(n / density)
+    a = L / nside
+    rng = np.random.default_rng(seed)
+    ix, iy = np.meshgrid(np.arange(nside), np.arange(nside), indexing="xy")
+    pos = np.stack([(ix.ravel() + 0.5) * a, (iy.ravel() + 0.5) * a], axis=1)
+    pos += rng.uniform(-0.1, 0.1, size=pos.shape)
+    vel = rng.standard_normal(size=pos.shape)
+    vel -= vel.mean(axis=0)
+    cols = int(L // rc)
+    dyn = LenJonesCfg(sigma=1.0, epsilon=1.0, cutoff=rc if cutoff else None)
+    space = SpaceCfg(wall_type=PeriodicWalls(), geometry_cfg=RectangleCfg(length=L, height=L))
+    int_cfg = IntCfg(dt=1e-3, chunks_cfg=ChunksCfg(num_cols=cols, num_rows=cols), device=GPU())
+    return pos, vel, space, dyn, int_cfg
+
+
+def rings_mixture(nrings=40000, nper=20, phi=0.8, seed=2, rot_diff=0.1, k_area=1.0):
+    """Binary active-ring mixture (see the module docstring).  The target area
+    of a ring is A0 = (n l / p0)^2 (src/rings/integration.jl:170); the
+    contraction c of a type maps to p0 = n l / sqrt(c * A_ngon(n, l))."""
+    from .rings import HarmTruncCfg, RingsCfg, RingsState
+
+    rng = np.random.default_rng(seed)
+    types = rng.integers(1, 3, size=nrings).astype(np.int32)
+    l0 = 1.0
+    p_r = 0.5
+    area_ngon = nper * l0 * l0 / (4 * math.tan(math.pi / nper))
+    p0_a = nper * l0 / math.sqrt(area_ngon)
+    p0_b = nper * l0 / math.sqrt(0.9 * area_ngon)
+    inter = HarmTruncCfg(k_rep=10.0, k_atr=0.0, dist_eq=2 * p_r, dist_max=2 * p_r)
+    cfg = RingsCfg(p0=[p0_a, p0_b], relax_time=[1.0, 1.0], vo=[1.0, 1.0], mobility=[1.0, 1.0],
+                   rot_diff=[rot_diff, rot_diff], k_area=[k_area, k_area], k_spring=[20.0, 20.0],
+                   l_spring=[l0, l0], interaction_finder=inter, num_particles=[nper, nper])
+    ring_r = l0 / (2 * math.sin(math.pi / nper))  # circumradius of the regular n-gon of side l0
+    r_out = ring_r + p_r
+    L = math.sqrt(nrings * math.pi * r_out * r_out / phi)
+    m = int(math.ceil(math.sqrt(nrings)))
+    spacing = L / m
+    ang = np.arange(nper) * 2 * math.pi / nper
+    circle = np.stack([np.cos(ang), np.sin(ang)], axis=1) * (ring_r * 0.97)
+    k = np.arange(nrings)
+    centers = np.stack([(k % m + 0.5) * spacing, (k // m + 0.5) * spacing], axis=1)
+    rings_pos = np.mod(centers[:, None, :] + circle[None, :, :], L)
+    pol = rng.random(nrings) * 2 * math.pi
+    state = RingsState(rings_pos=rings_pos, pol=pol, num_particles=[nper, nper], types=types)
+    cols = int(L // (2 * p_r))
+    space = SpaceCfg(wall_type=PeriodicWalls(), geometry_cfg=RectangleCfg(length=L, height=L))
+    int_cfg = IntCfg(dt=1e-3, chunks_cfg=ChunksCfg(num_cols=cols, num_rows=cols), device=GPU())
+    return state, space, cfg, int_cfg

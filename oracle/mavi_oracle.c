@@ -1,0 +1,656 @@
+/*
+ * mavi_oracle.c -- CPU restatement of Mavi.jl's step hot path (TEST
+ * INFRASTRUCTURE ONLY: the checker for libmavi_hip.so, never the product).
+ *
+ * Compiled with -ffp-contract=off: Julia does not contract a*b+c into an FMA
+ * unless @fastmath/@muladd is used, so every expression here is evaluated as
+ * the Julia source writes it, left to right, one rounding per operation.
+ *
+ * How each part is pinned (tests/test_oracle_*.py), independently of the
+ * formulas restated here:
+ *   or_jl_div / or_cell        exact rational arithmetic (fractions.Fraction)
+ *                              on adversarial inputs: exact multiples of the
+ *                              cell size, +-1 ulp, the top/right edge clamp.
+ *   or_chunk_capacity          the capacities the reference's own configs give
+ *                              (32 for 17x17 over 1024 LJ particles etc.).
+ *   or_bin                     invariants: every active id exactly once,
+ *                              ascending inside a cell, counts sum to N, each
+ *                              id in the cell recomputed exactly.
+ *   or_neighbors               every unordered pair of 8-adjacent cells
+ *                              appears exactly once (grids >= 3x3), brute
+ *                              force enumeration.
+ *   or_pair_force (LJ, HT)     closed forms: LJ zero at 2^(1/6) sigma and
+ *                              24 eps/sigma at sigma, minus the derivative of
+ *                              the LJ energy (finite differences); HT
+ *                              piecewise values; min-image at +-L/2.
+ *   or_calc_forces             brute-force all pairs == cell list for finite
+ *                              range potentials (the reference's own
+ *                              check_chunks technique), Newton's third law
+ *                              (sum F = 0), Threaded == Sequencial.
+ *   or_newton_steps            free flight x + v dt exactly, two-body harmonic
+ *                              oscillation against the analytic solution,
+ *                              energy conservation O(dt^2), time reversal,
+ *                              wall reflection / wrap closed forms.
+ *   or_kinetic/or_potential    brute force on tiny inputs in Python floats.
+ *   rings                      regular n-gon area n l^2/(4 tan(pi/n)), spring
+ *                              force zero at rest length, area force zero at
+ *                              A0 and equal to -dE/dx by finite differences,
+ *                              unwrap continuity, overdamped drift closed form
+ *                              with rot_diff = 0.
+ */
+#include "mavi_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Julia Base arithmetic used by the reference                              */
+/* ------------------------------------------------------------------------ */
+
+/* div(a, b) for Float64 in Julia Base (RoundToZero): the remainder is taken
+ * exactly with rem (= C fmod), the rest divided and rounded to nearest.
+ * Used by update_particle_chunk!, src/chunks.jl:129-130. */
+double or_jl_div(double a, double b) { return nearbyint((a - fmod(a, b)) / b); }
+
+/* src/chunks.jl:32-34:
+ *   nc = (ceil(0.5*chunk_length/particle_r) + 1) * ((ceil(0.5*chunk_height/particle_r) + 1))
+ *   nc = trunc(Int, ceil(nc*2)) */
+int64_t or_chunk_capacity(double cl, double ch, double r) {
+    double nc = (ceil(0.5 * cl / r) + 1.0) * (ceil(0.5 * ch / r) + 1.0);
+    return (int64_t)trunc(ceil(nc * 2.0));
+}
+
+/* src/chunks.jl:120-142 */
+int or_cell(const or_sys *s, double x, double y, int64_t *row, int64_t *col) {
+    double H = s->size[1];
+    double qr = or_jl_div((-y + s->bl[1]) + H, s->ch);
+    double qc = or_jl_div(x - s->bl[0], s->cl);
+    if (!isfinite(qr) || !isfinite(qc)) return OR_E_OUT_OF_BOX; /* trunc(Int, NaN) throws */
+    int64_t r = (int64_t)qr + 1;
+    int64_t c = (int64_t)qc + 1;
+    r -= (r == (int64_t)s->rows + 1) ? 1 : 0;
+    c -= (c == (int64_t)s->cols + 1) ? 1 : 0;
+    *row = r;
+    *col = c;
+    if (r < 1 || r > s->rows || c < 1 || c > s->cols) return OR_E_OUT_OF_BOX; /* BoundsError */
+    return OR_OK;
+}
+
+/* get_particles_ids: all slots for plain states (src/states.jl:213-228 with
+ * active_state = nothing), FixRingsIds.p_ids for rings (src/rings/states.jl:46-62):
+ * untyped rings use eachindex(pos); typed rings list the first
+ * num_particles[type] slots of every ring. */
+static int32_t ring_np(const or_sys *s, int32_t ring /*1-based*/) {
+    if (s->types == NULL) return s->np_type[0];
+    return s->np_type[s->types[ring - 1] - 1];
+}
+
+int64_t or_particle_ids(const or_sys *s, int64_t *ids) {
+    if (s->kind != OR_KIND_RINGS || s->types == NULL) {
+        if (ids)
+            for (int64_t i = 0; i < s->n; i++) ids[i] = i + 1;
+        return s->n;
+    }
+    int64_t k = 0;
+    for (int32_t r = 1; r <= s->nrings; r++) {
+        int64_t base = (int64_t)(r - 1) * s->nmax + 1; /* to_scalar_idx(r, 1), src/rings/states.jl:137-139 */
+        for (int32_t i = 1; i <= ring_np(s, r); i++) {
+            if (ids) ids[k] = base + i - 1;
+            k++;
+        }
+    }
+    return k;
+}
+
+/* update_chunks!: src/chunks.jl:150-161 (+ :144-146 append), visiting
+ * get_particles_ids in ascending order (src/integration.jl:54-59,
+ * src/rings/integration.jl:18-23). */
+int or_bin(const or_sys *s, const double *pos, int64_t *chunk_particles, int64_t *counts) {
+    const int64_t rows = s->rows, cols = s->cols, nc = s->nc;
+    memset(counts, 0, sizeof(int64_t) * rows * cols);
+    int64_t nids = or_particle_ids(s, NULL);
+    int64_t *ids = (int64_t *)malloc(sizeof(int64_t) * (nids > 0 ? nids : 1));
+    or_particle_ids(s, ids);
+    int err = OR_OK;
+    for (int64_t k = 0; k < nids; k++) {
+        int64_t i = ids[k];
+        int64_t r, c;
+        int e = or_cell(s, pos[2 * (i - 1)], pos[2 * (i - 1) + 1], &r, &c);
+        if (e) { err = e; break; }
+        int64_t cell = (r - 1) + rows * (c - 1);
+        int64_t p = counts[cell] + 1;
+        if (p > nc) { err = OR_E_CELL_OVERFLOW; break; } /* BoundsError on chunk_particles */
+        chunk_particles[(p - 1) + nc * cell] = i;
+        counts[cell] += 1;
+    }
+    free(ids);
+    return err;
+}
+
+/* get_neighbors: periodic src/chunks.jl:61-87, bounded src/chunks.jl:89-118. */
+static int64_t wrap_id(int64_t x, int64_t n) { /* get_id, src/chunks.jl:64-73 */
+    if (x == 0) return n;
+    if (x % (n + 1) == 0) return 1;
+    return x;
+}
+
+void or_neighbors(int32_t rows, int32_t cols, int32_t periodic, int64_t *nb, int32_t *nnb) {
+#define SETNB(r_, c_, e_, R_, C_)                                  \
+    do {                                                           \
+        int64_t k_ = (int64_t)((c_)-1) * rows + ((r_)-1);          \
+        nb[(k_ * 4 + (e_)) * 2 + 0] = (R_);                        \
+        nb[(k_ * 4 + (e_)) * 2 + 1] = (C_);                        \
+    } while (0)
+    if (periodic) {
+        for (int64_t i = 1; i <= rows; i++)
+            for (int64_t j = 1; j <= cols; j++) {
+                SETNB(i, j, 0, wrap_id(i + 1, rows), wrap_id(j, cols));
+                SETNB(i, j, 1, wrap_id(i + 1, rows), wrap_id(j + 1, cols));
+                SETNB(i, j, 2, wrap_id(i, rows), wrap_id(j + 1, cols));
+                SETNB(i, j, 3, wrap_id(i - 1, rows), wrap_id(j + 1, cols));
+                nnb[(j - 1) * rows + (i - 1)] = 4;
+            }
+        return;
+    }
+    for (int64_t i = 1; i <= rows - 1; i++) {
+        SETNB(i, 1, 0, i + 1, 1);
+        SETNB(i, 1, 1, i + 1, 2);
+        SETNB(i, 1, 2, i, 2);
+        nnb[(1 - 1) * rows + (i - 1)] = 3;
+        for (int64_t j = 2; j <= cols - 1; j++) {
+            SETNB(i, j, 0, i + 1, j - 1);
+            SETNB(i, j, 1, i + 1, j);
+            SETNB(i, j, 2, i + 1, j + 1);
+            SETNB(i, j, 3, i, j + 1);
+            nnb[(j - 1) * rows + (i - 1)] = 4;
+        }
+        SETNB(i, cols, 0, i + 1, cols - 1);
+        SETNB(i, cols, 1, i + 1, cols);
+        nnb[(cols - 1) * rows + (i - 1)] = 2;
+    }
+    for (int64_t j = 1; j <= cols - 1; j++) {
+        SETNB(rows, j, 0, rows, j + 1);
+        nnb[(j - 1) * rows + (rows - 1)] = 1;
+    }
+    nnb[(cols - 1) * rows + (rows - 1)] = 0;
+#undef SETNB
+}
+
+/* calc_diff, src/integration.jl:37-48.  Periodic rectangle:
+ *   dr = dr - (abs.(dr) .> (size_vec / 2)) .* copysign.(size_vec, dr) */
+void or_calc_diff(const or_sys *s, const double *r1, const double *r2, double *out) {
+    for (int d = 0; d < 2; d++) {
+        double dr = r1[d] - r2[d];
+        if (s->periodic) {
+            double half = s->size[d] / 2.0;
+            double mask = (fabs(dr) > half) ? 1.0 : 0.0;
+            dr = dr - mask * copysign(s->size[d], dr);
+        }
+        out[d] = dr;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Pair forces                                                              */
+/* ------------------------------------------------------------------------ */
+
+/* potential_force(dr, dist, ::LenJonesCfg), src/configs.jl:389-397:
+ *   fmod = 4*epsilon*(12*sigma^12/dist^13 - 6*sigma^6/dist^7); fmod/dist * dr
+ * The optional cutoff (rcut > 0) returns zero(dr) for dist > rcut, the form
+ * the reference uses for its truncated LJ (RunTumbleCfg, src/integration.jl:96-103). */
+static void lj_force(const double *p, const double *dr, double *f) {
+    double sigma = p[0], eps = p[1], rcut = p[2];
+    double dist = sqrt(dr[0] * dr[0] + dr[1] * dr[1]);
+    if (rcut > 0.0 && dist > rcut) { f[0] = 0.0; f[1] = 0.0; return; }
+    double fmod = 4.0 * eps * (12.0 * pow(sigma, 12.0) / pow(dist, 13.0) - 6.0 * pow(sigma, 6.0) / pow(dist, 7.0));
+    double q = fmod / dist;
+    f[0] = q * dr[0];
+    f[1] = q * dr[1];
+}
+
+/* potential_force(dr, dist, ::HarmTruncCfg), src/configs.jl:354-368. */
+static void ht_force(const double *p, const double *dr, double *f) {
+    double k_rep = p[0], k_atr = p[1], dist_eq = p[2], dist_max = p[3];
+    double dist = sqrt(dr[0] * dr[0] + dr[1] * dr[1]);
+    if (dist > dist_max) { f[0] = 0.0; f[1] = 0.0; return; }
+    double fmod;
+    if (dist < dist_eq) fmod = -k_rep * (dist / dist_eq - 1.0);
+    else fmod = -k_atr * (dist / dist_eq - 1.0);
+    double q = fmod / dist;
+    f[0] = q * dr[0];
+    f[1] = q * dr[1];
+}
+
+static inline int32_t ring_of(const or_sys *s, int64_t idx) { /* get_ring_id, src/rings/states.jl:142 */
+    return (int32_t)((idx - 1) / s->nmax + 1);
+}
+
+static const double *inter_of(const or_sys *s, int32_t r1, int32_t r2) {
+    /* get_interaction_cfg, src/rings/configs.jl:79-87 */
+    if (s->types == NULL) return s->inter;
+    int32_t t1 = s->types[r1 - 1], t2 = s->types[r2 - 1];
+    return s->inter + 4 * ((t1 - 1) + (int64_t)s->ntypes * (t2 - 1));
+}
+
+/* Rings calc_interaction + calc_interaction_force, src/rings/integration.jl:32-77. */
+static void rings_force(const or_sys *s, const double *pos, int64_t i, int64_t j, double *f) {
+    int32_t ri = ring_of(s, i), rj = ring_of(s, j);
+    const double *ic = inter_of(s, ri, rj);
+    double dr[2];
+    or_calc_diff(s, pos + 2 * (i - 1), pos + 2 * (j - 1), dr);
+    double dist = sqrt(dr[0] * dr[0] + dr[1] * dr[1]);
+    f[0] = 0.0;
+    f[1] = 0.0;
+    if (dist > ic[3]) return;
+    if (ri == rj) {
+        int64_t diff = i > j ? i - j : j - i;
+        int64_t np = ring_np(s, ri);
+        if (diff == 1 || diff == np - 1) return;
+    }
+    double dist_eq = ic[2];
+    double fmod;
+    if (dist < dist_eq) {
+        fmod = -ic[0] * (dist / dist_eq - 1.0);
+    } else {
+        if (ri == rj) fmod = 0.0;
+        else fmod = -ic[1] * (dist / dist_eq - 1.0);
+    }
+    double q = fmod / dist;
+    f[0] = q * dr[0];
+    f[1] = q * dr[1];
+}
+
+void or_pair_force(const or_sys *s, const double *pos, int64_t i, int64_t j, double *f) {
+    if (s->kind == OR_KIND_RINGS) { rings_force(s, pos, i, j, f); return; }
+    double dr[2];
+    or_calc_diff(s, pos + 2 * (i - 1), pos + 2 * (j - 1), dr);
+    if (s->kind == OR_KIND_LJ) lj_force(s->p, dr, f);
+    else ht_force(s->p, dr, f);
+}
+
+/* ------------------------------------------------------------------------ */
+/* calc_forces!                                                             */
+/* ------------------------------------------------------------------------ */
+
+static inline void acc_pair(const or_sys *s, const double *pos, int64_t p1, int64_t p2,
+                            double *forces, double *absum) {
+    double f[2];
+    or_pair_force(s, pos, p1, p2, f);
+    /* forces[p1_id] += f1; forces[p2_id] -= f1 (src/integration.jl:131-132) */
+    forces[2 * (p1 - 1)] += f[0];
+    forces[2 * (p1 - 1) + 1] += f[1];
+    forces[2 * (p2 - 1)] -= f[0];
+    forces[2 * (p2 - 1) + 1] -= f[1];
+    if (absum) {
+        double a = fabs(f[0]) + fabs(f[1]);
+        absum[p1 - 1] += a;
+        absum[p2 - 1] += a;
+    }
+}
+
+/* One column of the Sequencial loop body, src/integration.jl:116-156. */
+static void force_column(const or_sys *s, const double *pos, const int64_t *cp, const int64_t *counts,
+                         const int64_t *nb, const int32_t *nnb, int64_t col, double *forces, double *absum) {
+    const int64_t rows = s->rows, nc = s->nc;
+    for (int64_t row = 1; row <= rows; row++) {
+        int64_t k = (col - 1) * rows + (row - 1);
+        int64_t np = counts[k];
+        const int64_t *chunk = cp + nc * k;
+        for (int64_t i = 0; i < np; i++) {
+            int64_t p1 = chunk[i];
+            for (int64_t j = i + 1; j < np; j++) acc_pair(s, pos, p1, chunk[j], forces, absum);
+            for (int e = 0; e < nnb[k]; e++) {
+                int64_t nr = nb[(k * 4 + e) * 2], ncol = nb[(k * 4 + e) * 2 + 1];
+                int64_t nk = (ncol - 1) * rows + (nr - 1);
+                int64_t nei_np = counts[nk];
+                const int64_t *nei = cp + nc * nk;
+                for (int64_t j = 0; j < nei_np; j++) acc_pair(s, pos, p1, nei[j], forces, absum);
+            }
+        }
+    }
+}
+
+static void alloc_neighbors(const or_sys *s, int64_t **nb, int32_t **nnb) {
+    int64_t ncell = (int64_t)s->rows * s->cols;
+    *nb = (int64_t *)calloc((size_t)ncell * 8, sizeof(int64_t));
+    *nnb = (int32_t *)calloc((size_t)ncell, sizeof(int32_t));
+    or_neighbors(s->rows, s->cols, s->periodic, *nb, *nnb);
+}
+
+int or_calc_forces(const or_sys *s, const double *pos, const int64_t *cp, const int64_t *counts,
+                   double *forces, double *absum) {
+    memset(forces, 0, sizeof(double) * 2 * s->n);
+    if (absum) memset(absum, 0, sizeof(double) * s->n);
+    if (s->cols <= 0) {
+        /* all pairs over get_particles_ids, src/integration.jl:212-223 */
+        int64_t nids = or_particle_ids(s, NULL);
+        int64_t *ids = (int64_t *)malloc(sizeof(int64_t) * (nids > 0 ? nids : 1));
+        or_particle_ids(s, ids);
+        for (int64_t i = 0; i < nids; i++)
+            for (int64_t j = i + 1; j < nids; j++) acc_pair(s, pos, ids[i], ids[j], forces, absum);
+        free(ids);
+        return OR_OK;
+    }
+    int64_t *nb;
+    int32_t *nnb;
+    alloc_neighbors(s, &nb, &nnb);
+    for (int64_t col = 1; col <= s->cols; col++) force_column(s, pos, cp, counts, nb, nnb, col, forces, absum);
+    free(nb);
+    free(nnb);
+    return OR_OK;
+}
+
+int or_calc_forces_threaded(const or_sys *s, const double *pos, const int64_t *cp, const int64_t *counts,
+                            double *forces, int nthreads) {
+    if (nthreads <= 1 || s->cols <= 0) return or_calc_forces(s, pos, cp, counts, forces, NULL);
+    int64_t *nb;
+    int32_t *nnb;
+    alloc_neighbors(s, &nb, &nnb);
+    /* system.forces: one slice per thread (src/systems.jl:89-97) */
+    double *slices = (double *)calloc((size_t)nthreads * 2 * s->n, sizeof(double));
+    const int64_t cols = s->cols;
+#pragma omp parallel num_threads(nthreads)
+    {
+#ifdef _OPENMP
+        int t = omp_get_thread_num();
+        int nt = omp_get_num_threads();
+#else
+        int t = 0, nt = 1;
+#endif
+        /* Threads.@threads :static: contiguous, near-equal column blocks */
+        int64_t lo = (cols * t) / nt + 1, hi = (cols * (t + 1)) / nt;
+        double *sl = slices + (size_t)t * 2 * s->n;
+        for (int64_t col = lo; col <= hi; col++) force_column(s, pos, cp, counts, nb, nnb, col, sl, NULL);
+    }
+    /* get_forces(system) .= sum(system.forces)  (src/integration.jl:193) */
+    memcpy(forces, slices, sizeof(double) * 2 * s->n);
+    for (int t = 1; t < nthreads; t++) {
+        const double *sl = slices + (size_t)t * 2 * s->n;
+        for (int64_t k = 0; k < 2 * s->n; k++) forces[k] += sl[k];
+    }
+    free(slices);
+    free(nb);
+    free(nnb);
+    return OR_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Plain Newton step                                                        */
+/* ------------------------------------------------------------------------ */
+
+double or_kinetic_energy(int64_t n, const double *vel) {
+    /* src/quantities.jl:12-18: ke += sum(abs2, vel) per particle, then / 2 */
+    double ke = 0.0;
+    for (int64_t i = 0; i < n; i++) ke += vel[2 * i] * vel[2 * i] + vel[2 * i + 1] * vel[2 * i + 1];
+    return ke / 2.0;
+}
+
+double or_potential_energy_lj(const or_sys *s, const double *pos) {
+    /* src/quantities.jl:46-66: 4 eps * sum_{i<j} ((sigma/d)^12 - (sigma/d)^6), all pairs */
+    double sigma = s->p[0], eps = s->p[1];
+    double pot = 0.0;
+    for (int64_t i = 0; i < s->n; i++)
+        for (int64_t j = i + 1; j < s->n; j++) {
+            double dr[2];
+            or_calc_diff(s, pos + 2 * i, pos + 2 * j, dr);
+            double dist = sqrt(dr[0] * dr[0] + dr[1] * dr[1]);
+            pot += (pow(sigma / dist, 12.0) - pow(sigma / dist, 6.0));
+        }
+    pot *= 4.0 * eps;
+    return pot;
+}
+
+/* walls!, rigid rectangle src/integration.jl:271-285 and periodic rectangle
+ * src/integration.jl:309-324, over all slots (plain states: every id). */
+static void walls_rigid(const or_sys *s, const double *pos, double *vel, int64_t i) {
+    const double r = s->radius;
+    int out[2];
+    for (int d = 0; d < 2; d++) {
+        double rel = pos[2 * i + d] - s->bl[d];
+        out[d] = ((rel + r) > s->size[d]) || ((rel - r) < 0.0);
+    }
+    if (out[0] || out[1]) {
+        for (int d = 0; d < 2; d++) vel[2 * i + d] = vel[2 * i + d] * (double)(-2 * out[d] + 1);
+    }
+}
+
+static inline double jl_sign(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : x); }
+
+static void walls_periodic(const or_sys *s, double *pos, int64_t i) {
+    double center[2], half[2], diff[2];
+    int out[2];
+    for (int d = 0; d < 2; d++) {
+        center[d] = s->bl[d] + s->size[d] / 2.0;
+        half[d] = s->size[d] / 2.0;
+        diff[d] = pos[2 * i + d] - center[d];
+        out[d] = fabs(diff[d]) > half[d];
+    }
+    if (out[0] || out[1]) {
+        /* pos .- sign.(diff) .* (half_size * 2) .* out_bounds */
+        for (int d = 0; d < 2; d++)
+            pos[2 * i + d] = pos[2 * i + d] - jl_sign(diff[d]) * (half[d] * 2.0) * (double)out[d];
+    }
+}
+
+int or_newton_steps(const or_sys *s, double *pos, double *vel, double *forces, int64_t nsteps,
+                    double *ke, double *pe, int nthreads) {
+    if (s->kind == OR_KIND_RINGS) return OR_E_BADARG;
+    const int64_t n = s->n;
+    int64_t ncell = (int64_t)s->rows * s->cols;
+    int64_t *cp = NULL, *counts = NULL;
+    if (s->cols > 0) {
+        cp = (int64_t *)malloc(sizeof(int64_t) * (size_t)s->nc * (ncell > 0 ? ncell : 1));
+        counts = (int64_t *)malloc(sizeof(int64_t) * (ncell > 0 ? ncell : 1));
+    }
+    double *old = (double *)malloc(sizeof(double) * 2 * n);
+    const double dt = s->dt;
+    const double term = dt * dt / 2.0; /* dt^2/2, src/integration.jl:423 */
+    int err = OR_OK;
+    for (int64_t step = 0; step < nsteps; step++) {
+        /* newton_step!, src/integration.jl:507-515 */
+        if (s->cols > 0) {
+            err = or_bin(s, pos, cp, counts); /* update_chunks! */
+            if (err) break;
+        }
+        or_calc_forces_threaded(s, pos, cp, counts, forces, nthreads); /* clean + calc_forces! */
+        /* calc_walls_forces!: no-op for Rigid/Periodic walls (src/integration.jl:228) */
+        /* update_verlet!, src/integration.jl:415-431 */
+        memcpy(old, forces, sizeof(double) * 2 * n);
+        for (int64_t k = 0; k < 2 * n; k++) pos[k] = pos[k] + (vel[k] * dt + forces[k] * term);
+        or_calc_forces_threaded(s, pos, cp, counts, forces, nthreads); /* same (stale) chunks */
+        for (int64_t k = 0; k < 2 * n; k++) vel[k] = vel[k] + (dt / 2.0) * (forces[k] + old[k]);
+        /* walls! */
+        for (int64_t i = 0; i < n; i++) {
+            if (s->periodic) walls_periodic(s, pos, i);
+            else walls_rigid(s, pos, vel, i);
+        }
+        if (ke) ke[step] = or_kinetic_energy(n, vel);
+        if (pe) pe[step] = (s->kind == OR_KIND_LJ) ? or_potential_energy_lj(s, pos) : 0.0;
+    }
+    free(old);
+    free(cp);
+    free(counts);
+    return err;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Rings                                                                    */
+/* ------------------------------------------------------------------------ */
+
+static const double *rpar(const or_sys *s, int32_t ring) {
+    if (s->types == NULL) return s->ring_par;
+    return s->ring_par + 8 * (s->types[ring - 1] - 1);
+}
+enum { RP_P0 = 0, RP_RELAX = 1, RP_VO = 2, RP_MU = 3, RP_DR = 4, RP_KAREA = 5, RP_KSPRING = 6, RP_LSPRING = 7 };
+
+static inline int64_t sidx(const or_sys *s, int32_t ring, int32_t i) { /* to_scalar_idx, 1-based */
+    return (int64_t)(ring - 1) * s->nmax + i;
+}
+
+void or_rings_continuous_pos(const or_sys *s, const double *pos, double *cpos) {
+    const int64_t nslots = (int64_t)s->nmax * s->nrings;
+    /* continuos_pos[:, ring_id] .= rings_pos[:, ring_id] (all Nmax slots) */
+    memcpy(cpos, pos, sizeof(double) * 2 * nslots);
+    if (!s->periodic) return; /* update_continuos_pos!(system, wall_type) is a no-op */
+    for (int32_t r = 1; r <= s->nrings; r++) {
+        int32_t np = ring_np(s, r);
+        for (int32_t i = 2; i <= np; i++) {
+            int64_t a = sidx(s, r, i) - 1, b = sidx(s, r, i - 1) - 1;
+            double dr[2];
+            or_calc_diff(s, pos + 2 * a, pos + 2 * b, dr);
+            cpos[2 * a] = cpos[2 * b] + dr[0];
+            cpos[2 * a + 1] = cpos[2 * b + 1] + dr[1];
+        }
+    }
+}
+
+double or_rings_area(const or_sys *s, const double *cpos, int32_t ring) {
+    /* calc_area, src/rings/integration.jl:103-116, over get_continuos_pos */
+    int32_t np = ring_np(s, ring);
+    const double *c = cpos + 2 * (sidx(s, ring, 1) - 1);
+    double area = 0.0;
+    for (int32_t i = 0; i < np - 1; i++) area += c[2 * i] * c[2 * (i + 1) + 1] - c[2 * i + 1] * c[2 * (i + 1)];
+    area += c[2 * (np - 1)] * c[1] - c[2 * (np - 1) + 1] * c[0];
+    return area / 2.0;
+}
+
+int or_rings_forces(const or_sys *s, const double *pos, const double *cpos, double *forces, double *areas) {
+    /* forces!, src/rings/integration.jl:197-226 */
+    int64_t ncell = (int64_t)s->rows * s->cols;
+    int64_t *cp = NULL, *counts = NULL;
+    int err = OR_OK;
+    if (s->cols > 0) {
+        cp = (int64_t *)malloc(sizeof(int64_t) * (size_t)s->nc * ncell);
+        counts = (int64_t *)malloc(sizeof(int64_t) * ncell);
+        err = or_bin(s, pos, cp, counts);
+        if (err) { free(cp); free(counts); return err; }
+    }
+    or_calc_forces(s, pos, cp, counts, forces, NULL);
+    free(cp);
+    free(counts);
+    /* springs, :205-223 */
+    for (int32_t r = 1; r <= s->nrings; r++) {
+        int32_t np = ring_np(s, r);
+        const double *rp = rpar(s, r);
+        double k = rp[RP_KSPRING], l = rp[RP_LSPRING];
+        for (int32_t sp = 1; sp <= np; sp++) {
+            int32_t second = (sp == np) ? 1 : sp + 1;
+            int64_t p1 = sidx(s, r, sp) - 1, p2 = sidx(s, r, second) - 1;
+            double dr[2];
+            or_calc_diff(s, pos + 2 * p1, pos + 2 * p2, dr);
+            double dist = sqrt(dr[0] * dr[0] + dr[1] * dr[1]);
+            double fmod = -k * (dist - l); /* springs_force, :79-97 */
+            double q = fmod / dist;
+            double f0 = q * dr[0], f1 = q * dr[1];
+            forces[2 * p1] += f0;
+            forces[2 * p1 + 1] += f1;
+            forces[2 * p2] -= f0;
+            forces[2 * p2 + 1] -= f1;
+        }
+    }
+    /* area_forces!, :140-195 */
+    for (int32_t r = 1; r <= s->nrings; r++) {
+        int32_t np = ring_np(s, r);
+        const double *rp = rpar(s, r);
+        double area = or_rings_area(s, cpos, r);
+        if (areas) areas[r - 1] = area;
+        double area0 = ((double)np * rp[RP_LSPRING] / rp[RP_P0]);
+        area0 = area0 * area0;
+        for (int32_t i = 1; i <= np; i++) {
+            double fmod = rp[RP_KAREA] * (area - area0);
+            int32_t id1 = i - 1 == 0 ? np : i - 1;
+            int32_t id2 = i + 1 == np + 1 ? 1 : i + 1;
+            int64_t a = sidx(s, r, id1) - 1, b = sidx(s, r, id2) - 1, c = sidx(s, r, i) - 1;
+            double dr[2];
+            or_calc_diff(s, pos + 2 * b, pos + 2 * a, dr);
+            double ad0 = dr[1] / 2.0, ad1 = -dr[0] / 2.0; /* SVector(dr.y, -dr.x) / 2 */
+            forces[2 * c] -= fmod * ad0;
+            forces[2 * c + 1] -= fmod * ad1;
+        }
+    }
+    return OR_OK;
+}
+
+static void rings_cms(const or_sys *s, const double *cpos, double *cms) {
+    /* update_cms!, src/rings/integration.jl:366-372 */
+    for (int32_t r = 1; r <= s->nrings; r++) {
+        int32_t np = ring_np(s, r);
+        const double *c = cpos + 2 * (sidx(s, r, 1) - 1);
+        double sx = 0.0, sy = 0.0;
+        for (int32_t i = 0; i < np; i++) { sx += c[2 * i]; sy += c[2 * i + 1]; }
+        cms[2 * (r - 1)] = sx / (double)np;
+        cms[2 * (r - 1) + 1] = sy / (double)np;
+    }
+}
+
+int or_rings_steps(const or_sys *s, double *pos, double *pol, int64_t nsteps, const double *noise,
+                   double *forces_out, double *areas_out, double *cms_out) {
+    if (s->kind != OR_KIND_RINGS) return OR_E_BADARG;
+    const int64_t nslots = (int64_t)s->nmax * s->nrings;
+    double *cpos = (double *)malloc(sizeof(double) * 2 * nslots);
+    double *forces = (double *)malloc(sizeof(double) * 2 * nslots);
+    double *areas = (double *)malloc(sizeof(double) * s->nrings);
+    double *cms = (double *)malloc(sizeof(double) * 2 * s->nrings);
+    const double dt = s->dt;
+    int err = OR_OK;
+    /* The constructor leaves continuos_pos computed (src/rings/rings.jl:280);
+     * step! uses it in update_cms! before recomputing it. */
+    or_rings_continuous_pos(s, pos, cpos);
+    for (int64_t step = 0; step < nsteps; step++) {
+        rings_cms(s, cpos, cms);                 /* update_cms! */
+        or_rings_continuous_pos(s, pos, cpos);   /* update_chunks_all! then update_continuos_pos! */
+        err = or_rings_forces(s, pos, cpos, forces, areas); /* cleaning! + forces! (bins inside) */
+        if (err) break;
+        /* update!, src/rings/integration.jl:300-351 */
+        for (int32_t r = 1; r <= s->nrings; r++) {
+            int32_t np = ring_np(s, r);
+            const double *rp = rpar(s, r);
+            double vo = rp[RP_VO], relax = rp[RP_RELAX], mu = rp[RP_MU], drot = rp[RP_DR];
+            double theta = pol[r - 1];
+            double px = cos(theta), py = sin(theta);
+            double vcx = 0.0, vcy = 0.0;
+            for (int32_t i = 1; i <= np; i++) {
+                int64_t p = sidx(s, r, i) - 1;
+                double vx = vo * px + mu * forces[2 * p];
+                double vy = vo * py + mu * forces[2 * p + 1];
+                vcx += vx;
+                vcy += vy;
+                pos[2 * p] = pos[2 * p] + vx * dt;
+                pos[2 * p + 1] = pos[2 * p + 1] + vy * dt;
+            }
+            vcx = vcx / (double)np;
+            vcy = vcy / (double)np;
+            double speed = sqrt(vcx * vcx + vcy * vcy);
+            double cr;
+            if (speed == 0.0) {
+                cr = 0.0;
+            } else {
+                cr = (px * vcy - py * vcx) / speed;
+                if (fabs(cr) > 1.0) cr = jl_sign(cr);
+            }
+            double z = noise ? noise[step * s->nrings + (r - 1)] : 0.0;
+            double d_theta = 1.0 / relax * asin(cr) * dt + sqrt(2.0 * drot * dt) * z;
+            pol[r - 1] += d_theta;
+        }
+        /* walls!: periodic wrap over get_particles_ids */
+        if (s->periodic) {
+            int64_t nids = or_particle_ids(s, NULL);
+            int64_t *ids = (int64_t *)malloc(sizeof(int64_t) * nids);
+            or_particle_ids(s, ids);
+            for (int64_t k = 0; k < nids; k++) walls_periodic(s, pos, ids[k] - 1);
+            free(ids);
+        }
+    }
+    if (forces_out) memcpy(forces_out, forces, sizeof(double) * 2 * nslots);
+    if (areas_out) memcpy(areas_out, areas, sizeof(double) * s->nrings);
+    if (cms_out) memcpy(cms_out, cms, sizeof(double) * 2 * s->nrings);
+    free(cpos);
+    free(forces);
+    free(areas);
+    free(cms);
+    return err;
+}

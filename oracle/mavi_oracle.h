@@ -1,0 +1,134 @@
+/*
+ * mavi_oracle.h -- CPU restatement of Mavi.jl's step hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP
+ * library is compared against; it is never linked into, loaded by or called
+ * from the product path (libmavi_hip.so / the mavi_hip Python package).
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
+ *
+ * Every function restates a Julia function of the reference (Mavi.jl v1.0.1);
+ * the citation is given next to each declaration as path:line relative to the
+ * reference repository root.  Particle ids are 1-based, as in Julia.  Host
+ * positions/velocities/forces are interleaved x0,y0,x1,y1,... exactly like
+ * Julia's Vector{SVector{2,Float64}}.
+ *
+ * Pinning (what checks each function independently of its own formula) is
+ * listed in the header comment of mavi_oracle.c.
+ */
+#ifndef MAVI_ORACLE_H
+#define MAVI_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OR_KIND_LJ = 0, OR_KIND_HARMTRUNC = 1, OR_KIND_RINGS = 2 };
+enum { OR_OK = 0, OR_E_CELL_OVERFLOW = 1, OR_E_OUT_OF_BOX = 2, OR_E_BADARG = 5 };
+
+/* One system.  Mirrors what Mavi's System holds for the hot path:
+ *   space_cfg  (src/configs.jl:285-288) -> bl, size, periodic
+ *   dynamic_cfg (src/configs.jl:341-397, src/rings/configs.jl:95-107) -> kind, p, ring_*
+ *   int_cfg    (src/configs.jl:482-487) -> dt, cols/rows (ChunksCfg)
+ *   Chunks     (src/chunks.jl:10-40)    -> cols, rows, cl, ch, nc
+ */
+typedef struct {
+    int64_t n;          /* particle slots (length(state.pos)) */
+    double bl[2];       /* geometry_cfg.bottom_left */
+    double size[2];     /* geometry_cfg.length, geometry_cfg.height */
+    int32_t periodic;   /* 1: PeriodicWalls, 0: RigidWalls */
+    int32_t kind;       /* OR_KIND_* */
+    double p[4];        /* LJ: sigma, epsilon, rcut (<=0: no cutoff, reference semantics)
+                           HT: k_rep, k_atr, dist_eq, dist_max */
+    double dt;
+    int32_t cols, rows; /* ChunksCfg; cols == 0 -> chunks = nothing (all pairs) */
+    int32_t nc;         /* chunk capacity (src/chunks.jl:32-34) */
+    int32_t pad0;
+    double cl, ch;      /* chunk_length, chunk_height (src/chunks.jl:29-30) */
+    double radius;      /* particle_radius(dynamic_cfg) (src/configs.jl:418-421) */
+    /* Rings (kind == OR_KIND_RINGS) */
+    int32_t nmax, nrings, ntypes, pad1;
+    const int32_t *types;    /* per ring, 1-based type; NULL -> untyped (one type) */
+    const int32_t *np_type;  /* particles per ring, per type */
+    const double *ring_par;  /* per type, 8 doubles: p0, relax_time, vo, mobility,
+                                rot_diff, k_area, k_spring, l_spring */
+    const double *inter;     /* ntypes x ntypes HarmTrunc params, 4 doubles each,
+                                index (t1-1) + ntypes*(t2-1) (Julia column-major) */
+} or_sys;
+
+/* Julia Base `div(a::Float64, b::Float64)` (RoundToZero): round((a - rem(a,b))/b). */
+double or_jl_div(double a, double b);
+
+/* Chunk capacity nc, src/chunks.jl:32-34. */
+int64_t or_chunk_capacity(double chunk_length, double chunk_height, double particle_r);
+
+/* Cell of one position, src/chunks.jl:120-142 (1-based row, col). */
+int or_cell(const or_sys *s, double x, double y, int64_t *row, int64_t *col);
+
+/* Active particle ids in ascending order (get_particles_ids):
+ * src/states.jl:186-228 (all ids), src/rings/states.jl:46-62 (FixRingsIds).
+ * Returns the count; ids (1-based) written when ids != NULL. */
+int64_t or_particle_ids(const or_sys *s, int64_t *ids);
+
+/* update_chunks!, src/chunks.jl:150-161 with src/integration.jl:54-59 and
+ * src/rings/integration.jl:18-23.  chunk_particles is (nc, rows, cols)
+ * column-major, counts is (rows, cols) column-major, ids 1-based. */
+int or_bin(const or_sys *s, const double *pos, int64_t *chunk_particles, int64_t *counts);
+
+/* get_neighbors, src/chunks.jl:61-118.  For cell (r,c) (1-based) the list is
+ * stored at k = (c-1)*rows + (r-1): nnb[k] entries, nb[(k*4 + e)*2 + {0,1}] = (row, col). */
+void or_neighbors(int32_t rows, int32_t cols, int32_t periodic, int64_t *nb, int32_t *nnb);
+
+/* calc_diff, src/integration.jl:37-48. */
+void or_calc_diff(const or_sys *s, const double *r1, const double *r2, double *out);
+
+/* calc_interaction(i, j) for the configured potential:
+ * src/integration.jl:62-66 + src/configs.jl:321-323, 354-368, 389-397 (LJ/HT),
+ * src/rings/integration.jl:32-77 (rings).  Ids 1-based. */
+void or_pair_force(const or_sys *s, const double *pos, int64_t i, int64_t j, double *f);
+
+/* calc_forces!: Sequencial chunk loop (src/integration.jl:112-157) when
+ * s->cols > 0, all-pairs (src/integration.jl:197-224) otherwise.  Zeroes
+ * forces first (clean_forces!, src/systems.jl:119-123).  absum (nullable)
+ * receives per particle sum_j |f_ij| (both components' magnitudes), the
+ * scale used by the parity metric. */
+int or_calc_forces(const or_sys *s, const double *pos, const int64_t *chunk_particles,
+                   const int64_t *counts, double *forces, double *absum);
+
+/* Threaded calc_forces!, src/integration.jl:159-194: static column split over
+ * nthreads, per-thread force slices, slices summed in order. */
+int or_calc_forces_threaded(const or_sys *s, const double *pos, const int64_t *chunk_particles,
+                            const int64_t *counts, double *forces, int nthreads);
+
+/* newton_step! x nsteps, src/integration.jl:415-431, 507-515 with walls!
+ * :271-285 / :309-324.  pos/vel updated in place; forces receives
+ * get_forces(system) after the last step; ke/pe (nullable) receive
+ * kinetic_energy / potential_energy after every step (src/quantities.jl).
+ * nthreads > 1 uses the Threaded force loop. */
+int or_newton_steps(const or_sys *s, double *pos, double *vel, double *forces,
+                    int64_t nsteps, double *ke, double *pe, int nthreads);
+
+/* kinetic_energy, src/quantities.jl:12-18. */
+double or_kinetic_energy(int64_t n, const double *vel);
+/* potential_energy(::LenJonesCfg), src/quantities.jl:46-66 (all pairs, no cutoff). */
+double or_potential_energy_lj(const or_sys *s, const double *pos);
+
+/* Rings.  cpos = info.continuos_pos (2*nmax*nrings), areas (nrings), cms (2*nrings). */
+/* update_continuos_pos!, src/rings/integration.jl:118-136 (periodic), copy otherwise. */
+void or_rings_continuous_pos(const or_sys *s, const double *pos, double *cpos);
+/* calc_area, src/rings/integration.jl:103-116 over ring r (1-based). */
+double or_rings_area(const or_sys *s, const double *cpos, int32_t ring);
+/* forces! (pair + springs + area), src/rings/integration.jl:140-226. */
+int or_rings_forces(const or_sys *s, const double *pos, const double *cpos,
+                    double *forces, double *areas);
+/* Rings.Integration.step! x nsteps, src/rings/integration.jl:522-543.
+ * noise (nullable) holds nsteps*nrings standard normal draws consumed in
+ * ring order per step (the randn(system.rng) of :348); NULL means zero noise.
+ * forces/areas/cms (nullable) receive the last step's values. */
+int or_rings_steps(const or_sys *s, double *pos, double *pol, int64_t nsteps,
+                   const double *noise, double *forces, double *areas, double *cms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

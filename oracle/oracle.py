@@ -1,0 +1,316 @@
+"""ctypes front end of the CPU oracle (liboracle.so, built from mavi_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as the checker.  The product (mavi_hip /
+libmavi_hip.so) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+
+KIND_LJ, KIND_HT, KIND_RINGS = 0, 1, 2
+
+
+def build() -> Path:
+    r = subprocess.run(["make", "-B", "-C", str(HERE)], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"oracle build failed:\n{r.stdout}\n{r.stderr}")
+    return LIB
+
+
+class OrSys(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64),
+        ("bl", C.c_double * 2),
+        ("size", C.c_double * 2),
+        ("periodic", C.c_int32),
+        ("kind", C.c_int32),
+        ("p", C.c_double * 4),
+        ("dt", C.c_double),
+        ("cols", C.c_int32),
+        ("rows", C.c_int32),
+        ("nc", C.c_int32),
+        ("pad0", C.c_int32),
+        ("cl", C.c_double),
+        ("ch", C.c_double),
+        ("radius", C.c_double),
+        ("nmax", C.c_int32),
+        ("nrings", C.c_int32),
+        ("ntypes", C.c_int32),
+        ("pad1", C.c_int32),
+        ("types", C.POINTER(C.c_int32)),
+        ("np_type", C.POINTER(C.c_int32)),
+        ("ring_par", C.POINTER(C.c_double)),
+        ("inter", C.POINTER(C.c_double)),
+    ]
+
+
+_D = C.POINTER(C.c_double)
+_I64 = C.POINTER(C.c_int64)
+_I32 = C.POINTER(C.c_int32)
+_S = C.POINTER(OrSys)
+
+_SIG = {
+    "or_jl_div": (C.c_double, [C.c_double, C.c_double]),
+    "or_chunk_capacity": (C.c_int64, [C.c_double, C.c_double, C.c_double]),
+    "or_cell": (C.c_int, [_S, C.c_double, C.c_double, _I64, _I64]),
+    "or_particle_ids": (C.c_int64, [_S, _I64]),
+    "or_bin": (C.c_int, [_S, _D, _I64, _I64]),
+    "or_neighbors": (None, [C.c_int32, C.c_int32, C.c_int32, _I64, _I32]),
+    "or_calc_diff": (None, [_S, _D, _D, _D]),
+    "or_pair_force": (None, [_S, _D, C.c_int64, C.c_int64, _D]),
+    "or_calc_forces": (C.c_int, [_S, _D, _I64, _I64, _D, _D]),
+    "or_calc_forces_threaded": (C.c_int, [_S, _D, _I64, _I64, _D, C.c_int]),
+    "or_newton_steps": (C.c_int, [_S, _D, _D, _D, C.c_int64, _D, _D, C.c_int]),
+    "or_kinetic_energy": (C.c_double, [C.c_int64, _D]),
+    "or_potential_energy_lj": (C.c_double, [_S, _D]),
+    "or_rings_continuous_pos": (None, [_S, _D, _D]),
+    "or_rings_area": (C.c_double, [_S, _D, C.c_int32]),
+    "or_rings_forces": (C.c_int, [_S, _D, _D, _D, _D]),
+    "or_rings_steps": (C.c_int, [_S, _D, _D, C.c_int64, _D, _D, _D, _D]),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = C.CDLL(str(LIB))
+        for k, (res, args) in _SIG.items():
+            f = getattr(L, k)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(_D)
+
+
+def _i64(a):
+    return None if a is None else a.ctypes.data_as(_I64)
+
+
+def jl_div(a: float, b: float) -> float:
+    return lib().or_jl_div(a, b)
+
+
+def chunk_capacity(cl, ch, r) -> int:
+    return int(lib().or_chunk_capacity(cl, ch, r))
+
+
+class Sys:
+    """One oracle system.  Positions are (N, 2) float64 arrays; ids 1-based."""
+
+    def __init__(self, *, n, bl, size, periodic, kind, p, dt=0.0, cols=0, rows=0, nc=0, radius=0.0,
+                 nmax=0, nrings=0, types=None, np_type=None, ring_par=None, inter=None):
+        s = OrSys()
+        s.n = n
+        s.bl[0], s.bl[1] = bl
+        s.size[0], s.size[1] = size
+        s.periodic = 1 if periodic else 0
+        s.kind = kind
+        for i, v in enumerate(p):
+            s.p[i] = v
+        s.dt = dt
+        s.cols, s.rows = cols, rows
+        if cols > 0:
+            s.cl = size[0] / cols
+            s.ch = size[1] / rows
+        s.nc = nc
+        s.radius = radius
+        self._keep = []
+        if kind == KIND_RINGS:
+            s.nmax, s.nrings = nmax, nrings
+            np_type = np.ascontiguousarray(np.atleast_1d(np_type), dtype=np.int32)
+            ring_par = np.ascontiguousarray(ring_par, dtype=np.float64).reshape(-1, 8)
+            inter = np.ascontiguousarray(inter, dtype=np.float64)
+            s.ntypes = len(np_type)
+            s.np_type = np_type.ctypes.data_as(_I32)
+            s.ring_par = ring_par.ctypes.data_as(_D)
+            s.inter = inter.ctypes.data_as(_D)
+            self._keep += [np_type, ring_par, inter]
+            if types is not None:
+                types = np.ascontiguousarray(types, dtype=np.int32)
+                s.types = types.ctypes.data_as(_I32)
+                self._keep.append(types)
+        self.s = s
+        self.n = n
+        self.cols, self.rows, self.nc = cols, rows, nc
+        self.nrings, self.nmax = nrings, nmax
+
+    @property
+    def ptr(self):
+        return C.byref(self.s)
+
+    def cell(self, x, y):
+        r, c = C.c_int64(), C.c_int64()
+        err = lib().or_cell(self.ptr, x, y, C.byref(r), C.byref(c))
+        return err, r.value, c.value
+
+    def particle_ids(self):
+        k = lib().or_particle_ids(self.ptr, None)
+        ids = np.zeros(k, dtype=np.int64)
+        lib().or_particle_ids(self.ptr, _i64(ids))
+        return ids
+
+    def bin(self, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        cp = np.zeros((self.cols, self.rows, self.nc), dtype=np.int64)
+        counts = np.zeros((self.cols, self.rows), dtype=np.int64)
+        err = lib().or_bin(self.ptr, _d(pos), _i64(cp), _i64(counts))
+        return err, cp.transpose(2, 1, 0), counts.transpose(1, 0)
+
+    def _bin_raw(self, pos):
+        cp = np.zeros((self.cols, self.rows, self.nc), dtype=np.int64)
+        counts = np.zeros((self.cols, self.rows), dtype=np.int64)
+        err = lib().or_bin(self.ptr, _d(pos), _i64(cp), _i64(counts))
+        if err:
+            raise RuntimeError(f"oracle binning error {err}")
+        return cp, counts
+
+    def forces(self, pos, with_absum=False):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        cp = counts = None
+        if self.cols > 0:
+            cp, counts = self._bin_raw(pos)
+        f = np.zeros((self.n, 2))
+        ab = np.zeros(self.n) if with_absum else None
+        lib().or_calc_forces(self.ptr, _d(pos), _i64(cp), _i64(counts), _d(f), _d(ab))
+        return (f, ab) if with_absum else f
+
+    def forces_threaded(self, pos, nthreads):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        cp, counts = self._bin_raw(pos)
+        f = np.zeros((self.n, 2))
+        lib().or_calc_forces_threaded(self.ptr, _d(pos), _i64(cp), _i64(counts), _d(f), nthreads)
+        return f
+
+    def pair_force(self, pos, i, j):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        f = np.zeros(2)
+        lib().or_pair_force(self.ptr, _d(pos), i, j, _d(f))
+        return f
+
+    def calc_diff(self, r1, r2):
+        a = np.ascontiguousarray(r1, dtype=np.float64)
+        b = np.ascontiguousarray(r2, dtype=np.float64)
+        out = np.zeros(2)
+        lib().or_calc_diff(self.ptr, _d(a), _d(b), _d(out))
+        return out
+
+    def newton_steps(self, pos, vel, nsteps, record=False, nthreads=1):
+        pos = np.array(pos, dtype=np.float64, order="C")
+        vel = np.array(vel, dtype=np.float64, order="C")
+        f = np.zeros((self.n, 2))
+        ke = np.zeros(nsteps) if record else None
+        pe = np.zeros(nsteps) if record else None
+        err = lib().or_newton_steps(self.ptr, _d(pos), _d(vel), _d(f), nsteps, _d(ke), _d(pe), nthreads)
+        if err:
+            raise RuntimeError(f"oracle newton_steps error {err}")
+        return pos, vel, f, ke, pe
+
+    def kinetic_energy(self, vel):
+        vel = np.ascontiguousarray(vel, dtype=np.float64)
+        return lib().or_kinetic_energy(len(vel), _d(vel))
+
+    def potential_energy_lj(self, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        return lib().or_potential_energy_lj(self.ptr, _d(pos))
+
+    # rings
+    def rings_continuous_pos(self, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        out = np.zeros_like(pos)
+        lib().or_rings_continuous_pos(self.ptr, _d(pos), _d(out))
+        return out
+
+    def rings_area(self, cpos, ring):
+        cpos = np.ascontiguousarray(cpos, dtype=np.float64)
+        return lib().or_rings_area(self.ptr, _d(cpos), ring)
+
+    def rings_forces(self, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        cpos = self.rings_continuous_pos(pos)
+        f = np.zeros((self.n, 2))
+        areas = np.zeros(self.nrings)
+        err = lib().or_rings_forces(self.ptr, _d(pos), _d(cpos), _d(f), _d(areas))
+        if err:
+            raise RuntimeError(f"oracle rings_forces error {err}")
+        return f, areas
+
+    def rings_steps(self, pos, pol, nsteps, noise=None):
+        pos = np.array(pos, dtype=np.float64, order="C")
+        pol = np.array(pol, dtype=np.float64, order="C")
+        if noise is not None:
+            noise = np.ascontiguousarray(noise, dtype=np.float64)
+        f = np.zeros((self.n, 2))
+        areas = np.zeros(self.nrings)
+        cms = np.zeros((self.nrings, 2))
+        err = lib().or_rings_steps(self.ptr, _d(pos), _d(pol), nsteps, _d(noise), _d(f), _d(areas), _d(cms))
+        if err:
+            raise RuntimeError(f"oracle rings_steps error {err}")
+        return pos, pol, f, areas, cms
+
+
+def neighbors(rows, cols, periodic):
+    nb = np.zeros(rows * cols * 8, dtype=np.int64)
+    nnb = np.zeros(rows * cols, dtype=np.int32)
+    lib().or_neighbors(rows, cols, 1 if periodic else 0, _i64(nb), nnb.ctypes.data_as(_I32))
+    out = {}
+    for c in range(1, cols + 1):
+        for r in range(1, rows + 1):
+            k = (c - 1) * rows + (r - 1)
+            out[(r, c)] = [(int(nb[(k * 4 + e) * 2]), int(nb[(k * 4 + e) * 2 + 1])) for e in range(nnb[k])]
+    return out
+
+
+# --- build an oracle system from the mavi_hip host objects -----------------
+def from_system(pos_n, space_cfg, dynamic_cfg, int_cfg, chunks=None, radius=None):
+    """Oracle Sys mirroring a plain GPU System configuration."""
+    from mavi_hip.configs import HarmTruncCfg, LenJonesCfg, PeriodicWalls, particle_radius
+
+    g = space_cfg.geometry_cfg
+    if isinstance(dynamic_cfg, LenJonesCfg):
+        kind, p = KIND_LJ, [dynamic_cfg.sigma, dynamic_cfg.epsilon, dynamic_cfg.cutoff or 0.0, 0.0]
+    elif isinstance(dynamic_cfg, HarmTruncCfg):
+        kind, p = KIND_HT, [dynamic_cfg.k_rep, dynamic_cfg.k_atr, dynamic_cfg.dist_eq, dynamic_cfg.dist_max]
+    else:
+        raise TypeError(type(dynamic_cfg))
+    cols = rows = nc = 0
+    if int_cfg.chunks_cfg is not None:
+        cols, rows = int_cfg.chunks_cfg.num_cols, int_cfg.chunks_cfg.num_rows
+        r = particle_radius(dynamic_cfg)
+        nc = chunk_capacity(g.length / cols, g.height / rows, r)
+    return Sys(n=pos_n, bl=g.bottom_left, size=(g.length, g.height),
+               periodic=isinstance(space_cfg.wall_type, PeriodicWalls), kind=kind, p=p, dt=int_cfg.dt,
+               cols=cols, rows=rows, nc=nc, radius=radius if radius is not None else particle_radius(dynamic_cfg))
+
+
+def from_rings(state, space_cfg, cfg, int_cfg):
+    """Oracle Sys mirroring a rings GPU configuration."""
+    from mavi_hip.configs import PeriodicWalls
+
+    g = space_cfg.geometry_cfg
+    cols = rows = nc = 0
+    r = min(cfg.particle_radii())
+    if int_cfg.chunks_cfg is not None:
+        cols, rows = int_cfg.chunks_cfg.num_cols, int_cfg.chunks_cfg.num_rows
+        nc = chunk_capacity(g.length / cols, g.height / rows, r)
+    inter = cfg.interaction_table()
+    return Sys(n=len(state.pos), bl=g.bottom_left, size=(g.length, g.height),
+               periodic=isinstance(space_cfg.wall_type, PeriodicWalls), kind=KIND_RINGS, p=[0, 0, 0, 0],
+               dt=int_cfg.dt, cols=cols, rows=rows, nc=nc, radius=r, nmax=state.nmax, nrings=state.num_rings,
+               types=state.types, np_type=np.atleast_1d(state.num_particles), ring_par=cfg.type_table(),
+               inter=inter.reshape(-1))
