@@ -22,12 +22,15 @@ int set_err(mavi_ctx *ctx, int code, const std::string &msg) {
             return set_err((ctx), MAVI_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));           \
     } while (0)
 
+// Allocate and zero on the context's stream: the library's stream is
+// non-blocking, so a zero fill on the null stream would not be ordered
+// before the stream's own work on the buffer.
 template <typename T>
-hipError_t dalloc(T **p, int64_t count) {
+hipError_t dalloc(T **p, int64_t count, hipStream_t st) {
     *p = nullptr;
     if (count <= 0) count = 1;
     hipError_t e = hipMalloc((void **)p, sizeof(T) * (size_t)count);
-    if (e == hipSuccess) e = hipMemset(*p, 0, sizeof(T) * (size_t)count);
+    if (e == hipSuccess) e = hipMemsetAsync(*p, 0, sizeof(T) * (size_t)count, st);
     return e;
 }
 
@@ -243,7 +246,7 @@ int mavi_sys_create(mavi_ctx *ctx, const mavi_sys_desc *d, mavi_sys **out) {
 
     hipError_t e = hipSuccess;
 #define ALLOC(ptr, cnt) \
-    if (e == hipSuccess) e = dalloc(&(ptr), (cnt))
+    if (e == hipSuccess) e = dalloc(&(ptr), (cnt), ctx->stream)
     ALLOC(s->pos, P.n);
     ALLOC(s->vel, P.n);
     ALLOC(s->force, P.n);
@@ -274,14 +277,15 @@ int mavi_sys_create(mavi_ctx *ctx, const mavi_sys_desc *d, mavi_sys **out) {
     if (e == hipSuccess) e = hipEventCreate(&s->ev0);
     if (e == hipSuccess) e = hipEventCreate(&s->ev1);
     if (e == hipSuccess && P.potential == MAVI_POT_RINGS) {
-        e = hipMemcpy(s->active_ids, active.data(), sizeof(int32_t) * active.size(), hipMemcpyHostToDevice);
-        if (e == hipSuccess)
-            e = hipMemcpy(s->rtype, host_rtype.data(), sizeof(int32_t) * host_rtype.size(), hipMemcpyHostToDevice);
-        if (e == hipSuccess)
-            e = hipMemcpy(s->types, host_types.data(), sizeof(RingType) * host_types.size(), hipMemcpyHostToDevice);
-        if (e == hipSuccess)
-            e = hipMemcpy(s->inter, host_inter.data(), sizeof(RingInter) * host_inter.size(), hipMemcpyHostToDevice);
+        auto up = [&](void *dst, const void *src, size_t bytes) {
+            return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+        };
+        e = up(s->active_ids, active.data(), sizeof(int32_t) * active.size());
+        if (e == hipSuccess) e = up(s->rtype, host_rtype.data(), sizeof(int32_t) * host_rtype.size());
+        if (e == hipSuccess) e = up(s->types, host_types.data(), sizeof(RingType) * host_types.size());
+        if (e == hipSuccess) e = up(s->inter, host_inter.data(), sizeof(RingInter) * host_inter.size());
     }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     s->seed = d->seed;
     if (e != hipSuccess) {
         std::string m = std::string("allocation/upload failed: ") + hipGetErrorString(e);
@@ -334,10 +338,11 @@ int mavi_bin(mavi_sys *s, int64_t *chunk_particles, int64_t *counts) {
     if (!chunk_particles && !counts) return MAVI_OK;
     const int64_t ncell = s->P.ncell, nc = s->P.nc;
     int64_t *dcp = nullptr, *dcnt = nullptr;
-    HIPCHK(ctx, dalloc(&dcp, nc * ncell));
-    hipError_t e = dalloc(&dcnt, ncell);
+    HIPCHK(ctx, dalloc(&dcp, nc * ncell, ctx->stream));
+    hipError_t e = dalloc(&dcnt, ncell, ctx->stream);
     if (e != hipSuccess) { dfree(dcp); HIPCHK(ctx, e); }
-    if (chunk_particles) e = hipMemcpy(dcp, chunk_particles, sizeof(int64_t) * nc * ncell, hipMemcpyHostToDevice);
+    if (chunk_particles)
+        e = hipMemcpyAsync(dcp, chunk_particles, sizeof(int64_t) * nc * ncell, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess) e = mavi::launch_export_chunks(s, dcp, dcnt, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e == hipSuccess && chunk_particles)
@@ -390,7 +395,7 @@ static int steps_impl(mavi_sys *s, int64_t nsteps, const double *noise_host) {
             if (need > s->noise_cap) {
                 dfree(s->noise);
                 s->noise_cap = 0;
-                HIPCHK(ctx, dalloc(&s->noise, need));
+                HIPCHK(ctx, dalloc(&s->noise, need, ctx->stream));
                 s->noise_cap = need;
             }
             HIPCHK(ctx, hipMemcpyAsync(s->noise, noise_host, sizeof(double) * need, hipMemcpyHostToDevice,
@@ -445,7 +450,7 @@ int mavi_observables(mavi_sys *s, double *ke, double *pe) {
     if (s->red_cap < 2 * nb) {
         dfree(s->red);
         s->red_cap = 0;
-        HIPCHK(ctx, dalloc(&s->red, 2 * nb));
+        HIPCHK(ctx, dalloc(&s->red, 2 * nb, ctx->stream));
         s->red_cap = 2 * nb;
     }
     HIPCHK(ctx, mavi::launch_observables(s, s->red, ctx->stream));

@@ -179,11 +179,18 @@ class Context:
         self.handle = h
         self.device = device
 
+    _per_device: dict = {}
+
     @classmethod
     def default(cls) -> "Context":
-        if cls._default is None:
-            cls._default = Context(int(os.environ.get("MAVI_HIP_DEVICE", "0")))
-        return cls._default
+        return cls.for_device(int(os.environ.get("MAVI_HIP_DEVICE", "0")))
+
+    @classmethod
+    def for_device(cls, device: int) -> "Context":
+        """One shared context (and stream) per device for this process."""
+        if device not in cls._per_device:
+            cls._per_device[device] = Context(device)
+        return cls._per_device[device]
 
     def __del__(self):
         try:

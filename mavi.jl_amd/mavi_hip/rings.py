@@ -147,7 +147,7 @@ class RingsSystem:
         if self._tables[3] is not None:
             d.ring_types = self._tables[3].ctypes.data_as(C.POINTER(C.c_int32))
         d.seed = int(seed)
-        self.ctx = ctx or _lib.Context(int_cfg.device.device)
+        self.ctx = ctx or _lib.Context.for_device(int_cfg.device.device)
         self.dev = _lib.DeviceSystem(self.ctx, d, keep=self._tables)
         self.forces = np.zeros_like(state.pos)
         self.upload()

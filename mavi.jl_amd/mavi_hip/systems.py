@@ -139,7 +139,7 @@ class System:
         else:
             d.pot[0], d.pot[1] = dynamic_cfg.k_rep, dynamic_cfg.k_atr
             d.pot[2], d.pot[3] = dynamic_cfg.dist_eq, dynamic_cfg.dist_max
-        self.ctx = ctx or _lib.Context(int_cfg.device.device)
+        self.ctx = ctx or _lib.Context.for_device(int_cfg.device.device)
         self.dev = _lib.DeviceSystem(self.ctx, d)
         self.forces = np.zeros_like(state.pos)
         self.upload()

@@ -18,8 +18,13 @@ def force_rel_err(f_gpu, f_ref, absum):
     return float(r.max()) if len(r) else 0.0
 
 
-def traj_err(pos_g, vel_g, pos_r, vel_r, L):
-    ep = float(np.abs(pos_g - pos_r).max()) / L
+def traj_err(pos_g, vel_g, pos_r, vel_r, L, periodic=True):
+    """Position error with the minimum image (a particle wrapped on one side
+    and not on the other is the same point), relative to the box size."""
+    d = np.asarray(pos_g) - np.asarray(pos_r)
+    if periodic:
+        d = d - L * np.round(d / L)
+    ep = float(np.abs(d).max()) / L
     ev = 0.0
     if vel_r is not None:
         ev = float(np.abs(vel_g - vel_r).max()) / max(float(np.abs(vel_r).max()), 1e-300)
