@@ -144,8 +144,29 @@ int mavi_prof_enable(mavi_sys *sys, int on);
 int mavi_prof_get(mavi_sys *sys, int which, double *ms_total, int64_t *launches);
 
 /* Force-kernel tile shape: 0 = library default; otherwise the workgroup size
- * (64, 128, 256, 512) used by the pair-force kernels.  Used by the autotuner. */
+ * (64, 128, 256, 512, 1024) used by the pair-force kernels.  Used by the
+ * autotuner. */
 int mavi_sys_set_tile(mavi_sys *sys, int32_t workgroup);
+
+/* Execution options (no effect on results beyond floating-point summation
+ * order; none changes the reference semantics):
+ *   MAVI_OPT_TWO_PHASE  1: finite-range interactions first keep the in-range
+ *                       candidates, then evaluate forces on them (default 0)
+ *   MAVI_OPT_GRAPHS     1 (default): multi-step calls replay a captured
+ *                       two-step hipGraph
+ *   MAVI_OPT_LDS_TILES  1: pair kernels stage chunk-row segments in LDS;
+ *                       0 (default): per-particle gathers from global memory */
+#define MAVI_OPT_TWO_PHASE 0
+#define MAVI_OPT_GRAPHS 1
+#define MAVI_OPT_LDS_TILES 2
+int mavi_sys_set_option(mavi_sys *sys, int32_t option, int32_t value);
+
+/* Autotuner: times `trial_steps` steps of the system's own dynamics for every
+ * pair-kernel shape (workgroup 64..512 x LDS tiles x two-phase list), keeps
+ * the fastest and restores the state it started from.  best_us (nullable)
+ * receives the chosen shape's time per step; shape (nullable, 3 ints)
+ * receives {workgroup, lds_tiles, two_phase}. */
+int mavi_autotune(mavi_sys *sys, int32_t trial_steps, double *best_us, int32_t *shape);
 
 #ifdef __cplusplus
 }

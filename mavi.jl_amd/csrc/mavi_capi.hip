@@ -112,11 +112,16 @@ int mavi_sys_destroy(mavi_sys *s) {
     if (!s) return MAVI_OK;
     (void)hipSetDevice(s->ctx->device);
     (void)hipStreamSynchronize(s->ctx->stream);
-    if (s->graph) (void)hipGraphExecDestroy(s->graph);
-    dfree(s->pos); dfree(s->vel); dfree(s->force);
-    dfree(s->key); dfree(s->rank); dfree(s->cnt); dfree(s->start); dfree(s->bsum);
-    dfree(s->perm); dfree(s->scell); dfree(s->ps); dfree(s->ps2); dfree(s->f1);
-    dfree(s->flags); dfree(s->active_ids);
+    if (s->graph2) (void)hipGraphExecDestroy(s->graph2);
+    for (int c = 0; c < 2; c++) {
+        dfree(s->P_[c]);
+        dfree(s->V_[c]);
+        dfree(s->ID_[c]);
+    }
+    dfree(s->F); dfree(s->P2); dfree(s->F1);
+    dfree(s->key); dfree(s->rank); dfree(s->src); dfree(s->scell); dfree(s->slot_of);
+    dfree(s->cnt); dfree(s->start); dfree(s->scan_status); dfree(s->flags); dfree(s->counter);
+    dfree(s->active_ids); dfree(s->pos_id); dfree(s->vel_id); dfree(s->force_id);
     dfree(s->pol); dfree(s->areas); dfree(s->cms); dfree(s->cms_next); dfree(s->cpos);
     dfree(s->rtype); dfree(s->types); dfree(s->inter); dfree(s->noise); dfree(s->red);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
@@ -178,8 +183,9 @@ int mavi_sys_create(mavi_ctx *ctx, const mavi_sys_desc *d, mavi_sys **out) {
     P.eq2 = 0.0;
     if (d->potential == MAVI_POT_LJ) {
         P.cut2 = exact_gt_threshold(d->pot[2]);
-        P.lj24eps = 24.0 * d->pot[1];
-        P.ljsig2 = d->pot[0] * d->pot[0];
+        const double s2 = d->pot[0] * d->pot[0], s6 = s2 * s2 * s2;
+        P.lj_c12 = 48.0 * d->pot[1] * s6 * s6;
+        P.lj_c6 = 24.0 * d->pot[1] * s6;
     } else if (d->potential == MAVI_POT_HARMTRUNC) {
         P.cut2 = exact_gt_threshold(d->pot[3]);
         P.eq2 = exact_lt_threshold(d->pot[2]);
@@ -219,12 +225,14 @@ int mavi_sys_create(mavi_ctx *ctx, const mavi_sys_desc *d, mavi_sys **out) {
             }
             host_types.push_back(T);
         }
+        s->list2 = 0.0;
         for (int k = 0; k < d->ntypes * d->ntypes; k++) {
             RingInter I{};
             const double *q = d->interaction + 4 * k;
             I.k_rep = q[0]; I.k_atr = q[1]; I.dist_eq = q[2]; I.dist_max = q[3];
             I.max2 = exact_gt_threshold(q[3]);
             I.eq2 = exact_lt_threshold(q[2]);
+            if (I.max2 > s->list2) s->list2 = I.max2;
             host_inter.push_back(I);
         }
         host_rtype.resize(d->nrings);
@@ -247,22 +255,30 @@ int mavi_sys_create(mavi_ctx *ctx, const mavi_sys_desc *d, mavi_sys **out) {
     hipError_t e = hipSuccess;
 #define ALLOC(ptr, cnt) \
     if (e == hipSuccess) e = dalloc(&(ptr), (cnt), ctx->stream)
-    ALLOC(s->pos, P.n);
-    ALLOC(s->vel, P.n);
-    ALLOC(s->force, P.n);
+    const bool rings = P.potential == MAVI_POT_RINGS;
+    for (int c = 0; c < 2; c++) {
+        ALLOC(s->P_[c], P.na);
+        if (!rings) ALLOC(s->V_[c], P.na);
+        ALLOC(s->ID_[c], P.na);
+    }
+    ALLOC(s->F, P.na);
+    ALLOC(s->P2, P.na);
+    ALLOC(s->F1, P.na);
     ALLOC(s->key, P.na);
     ALLOC(s->rank, P.na);
-    ALLOC(s->perm, P.na);
+    ALLOC(s->src, P.na);
     ALLOC(s->scell, P.na);
-    ALLOC(s->ps, P.na);
-    ALLOC(s->ps2, P.na);
-    ALLOC(s->f1, P.na);
+    ALLOC(s->slot_of, P.n);
     ALLOC(s->flags, 1);
-    ALLOC(s->cnt, (int64_t)P.ncell + 1);
+    ALLOC(s->counter, 1);
+    ALLOC(s->cnt, (int64_t)P.ncell + 2);  // + the scan's tile counter
     ALLOC(s->start, (int64_t)P.ncell + 1);
     s->nscan_blocks = (P.ncell + 1 + 1023) / 1024;
-    ALLOC(s->bsum, s->nscan_blocks);
-    if (P.potential == MAVI_POT_RINGS) {
+    ALLOC(s->scan_status, s->nscan_blocks);
+    ALLOC(s->pos_id, P.n);
+    if (!rings) ALLOC(s->vel_id, P.n);
+    ALLOC(s->force_id, P.n);
+    if (rings) {
         ALLOC(s->active_ids, P.na);
         ALLOC(s->pol, P.nrings);
         ALLOC(s->areas, P.nrings);
@@ -301,14 +317,23 @@ int mavi_sys_upload(mavi_sys *s, const double *pos, const double *vel, const dou
     mavi_ctx *ctx = s->ctx;
     HIPCHK(ctx, hipSetDevice(ctx->device));
     const size_t bytes = sizeof(double2) * (size_t)s->P.n;
-    HIPCHK(ctx, hipMemcpyAsync(s->pos, pos, bytes, hipMemcpyHostToDevice, ctx->stream));
-    if (vel) HIPCHK(ctx, hipMemcpyAsync(s->vel, vel, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(s->pos_id, pos, bytes, hipMemcpyHostToDevice, ctx->stream));
+    if (vel && s->vel_id) HIPCHK(ctx, hipMemcpyAsync(s->vel_id, vel, bytes, hipMemcpyHostToDevice, ctx->stream));
     if (pol && s->P.potential == MAVI_POT_RINGS)
         HIPCHK(ctx, hipMemcpyAsync(s->pol, pol, sizeof(double) * s->P.nrings, hipMemcpyHostToDevice, ctx->stream));
-    if (s->P.potential == MAVI_POT_RINGS) HIPCHK(ctx, mavi::launch_rings_info(s, ctx->stream));
+    s->cur = 0;
+    HIPCHK(ctx, mavi::launch_iota_state(s, ctx->stream));
+    if (s->P.potential == MAVI_POT_RINGS) HIPCHK(ctx, mavi::launch_rings_init(s, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     s->has_state = true;
-    s->binned = false;
+    s->sorted = false;
+    return MAVI_OK;
+}
+
+// Bring the id-order staging copies up to date with the sorted state.
+static int sync_staging(mavi_sys *s, bool forces) {
+    mavi_ctx *ctx = s->ctx;
+    HIPCHK(ctx, mavi::launch_unpermute(s, true, forces, ctx->stream));
     return MAVI_OK;
 }
 
@@ -317,8 +342,8 @@ int mavi_sys_download(mavi_sys *s, double *pos, double *vel, double *pol) {
     mavi_ctx *ctx = s->ctx;
     HIPCHK(ctx, hipSetDevice(ctx->device));
     const size_t bytes = sizeof(double2) * (size_t)s->P.n;
-    if (pos) HIPCHK(ctx, hipMemcpyAsync(pos, s->pos, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    if (vel) HIPCHK(ctx, hipMemcpyAsync(vel, s->vel, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    if (pos) HIPCHK(ctx, hipMemcpyAsync(pos, s->pos_id, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    if (vel && s->vel_id) HIPCHK(ctx, hipMemcpyAsync(vel, s->vel_id, bytes, hipMemcpyDeviceToHost, ctx->stream));
     if (pol && s->P.potential == MAVI_POT_RINGS)
         HIPCHK(ctx, hipMemcpyAsync(pol, s->pol, sizeof(double) * s->P.nrings, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -332,9 +357,9 @@ int mavi_bin(mavi_sys *s, int64_t *chunk_particles, int64_t *counts) {
     if (!s->has_state) return set_err(ctx, MAVI_E_BADARG, "no state uploaded");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     HIPCHK(ctx, mavi::launch_bin(s, ctx->stream));
-    int st = check_flags(s);
+    int st = sync_staging(s, false);  // binning moves no force
+    if (st == MAVI_OK) st = check_flags(s);
     if (st != MAVI_OK) return st;
-    s->binned = true;
     if (!chunk_particles && !counts) return MAVI_OK;
     const int64_t ncell = s->P.ncell, nc = s->P.nc;
     int64_t *dcp = nullptr, *dcnt = nullptr;
@@ -365,10 +390,11 @@ int mavi_forces(mavi_sys *s, double *forces) {
         HIPCHK(ctx, mavi::launch_bin(s, ctx->stream));
         HIPCHK(ctx, mavi::launch_forces_only(s, ctx->stream));
     }
-    int st = check_flags(s);
+    int st = sync_staging(s, true);
+    if (st == MAVI_OK) st = check_flags(s);
     if (st != MAVI_OK) return st;
     if (forces)
-        HIPCHK(ctx, hipMemcpy(forces, s->force, sizeof(double2) * (size_t)s->P.n, hipMemcpyDeviceToHost));
+        HIPCHK(ctx, hipMemcpy(forces, s->force_id, sizeof(double2) * (size_t)s->P.n, hipMemcpyDeviceToHost));
     return MAVI_OK;
 }
 
@@ -377,8 +403,34 @@ int mavi_get_forces(mavi_sys *s, double *forces) {
     mavi_ctx *ctx = s->ctx;
     HIPCHK(ctx, hipSetDevice(ctx->device));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    HIPCHK(ctx, hipMemcpy(forces, s->force, sizeof(double2) * (size_t)s->P.n, hipMemcpyDeviceToHost));
+    HIPCHK(ctx, hipMemcpy(forces, s->force_id, sizeof(double2) * (size_t)s->P.n, hipMemcpyDeviceToHost));
     return MAVI_OK;
+}
+
+// One step of the system's dynamics (newton_step! or Rings step!) with the
+// device rotational noise.
+static hipError_t launch_step(mavi_sys *s, hipStream_t st) {
+    if (s->P.potential == MAVI_POT_RINGS) return mavi::launch_rings_step(s, nullptr, st);
+    return mavi::launch_verlet_step(s, st);
+}
+
+// Two steps captured once into a graph (state starting in copy 0, so the
+// ping-pong returns to copy 0) and replayed: one launch per two steps.
+static hipError_t ensure_graph(mavi_sys *s, hipStream_t st) {
+    if (s->graph2) return hipSuccess;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    const int cur0 = s->cur;
+    hipError_t e1 = launch_step(s, st);
+    hipError_t e2 = (e1 == hipSuccess) ? launch_step(s, st) : e1;
+    e = hipStreamEndCapture(st, &g);
+    s->cur = cur0;
+    if (e1 != hipSuccess) e = e1;
+    if (e == hipSuccess && e2 != hipSuccess) e = e2;
+    if (e == hipSuccess) e = hipGraphInstantiate(&s->graph2, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    return e;
 }
 
 static int steps_impl(mavi_sys *s, int64_t nsteps, const double *noise_host) {
@@ -386,28 +438,33 @@ static int steps_impl(mavi_sys *s, int64_t nsteps, const double *noise_host) {
     if (nsteps < 0) return set_err(ctx, MAVI_E_BADARG, "nsteps < 0");
     if (!s->has_state) return set_err(ctx, MAVI_E_BADARG, "no state uploaded");
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    if (s->P.potential != MAVI_POT_RINGS) {
-        for (int64_t k = 0; k < nsteps; k++) HIPCHK(ctx, mavi::launch_verlet_step(s, ctx->stream));
-    } else {
+    hipStream_t st = ctx->stream;
+    if (noise_host) {
         const int64_t nr = s->P.nrings;
-        if (noise_host) {
-            const int64_t need = nsteps * nr;
-            if (need > s->noise_cap) {
-                dfree(s->noise);
-                s->noise_cap = 0;
-                HIPCHK(ctx, dalloc(&s->noise, need, ctx->stream));
-                s->noise_cap = need;
-            }
-            HIPCHK(ctx, hipMemcpyAsync(s->noise, noise_host, sizeof(double) * need, hipMemcpyHostToDevice,
-                                       ctx->stream));
+        const int64_t need = nsteps * nr;
+        if (need > s->noise_cap) {
+            dfree(s->noise);
+            s->noise_cap = 0;
+            HIPCHK(ctx, dalloc(&s->noise, need, st));
+            s->noise_cap = need;
         }
-        for (int64_t k = 0; k < nsteps; k++) {
-            const double *nz = noise_host ? s->noise + k * nr : nullptr;
-            HIPCHK(ctx, mavi::launch_rings_step(s, nz, s->rng_step, ctx->stream));
-            s->rng_step++;
+        HIPCHK(ctx, hipMemcpyAsync(s->noise, noise_host, sizeof(double) * need, hipMemcpyHostToDevice, st));
+        for (int64_t k = 0; k < nsteps; k++) HIPCHK(ctx, mavi::launch_rings_step(s, s->noise + k * nr, st));
+    } else {
+        int64_t k = 0;
+        const bool graphs = s->use_graphs && !s->prof && nsteps >= 4;
+        if (graphs && s->cur != 0) {
+            HIPCHK(ctx, launch_step(s, st));
+            k++;
         }
+        if (graphs) {
+            HIPCHK(ctx, ensure_graph(s, st));
+            for (; k + 2 <= nsteps; k += 2) HIPCHK(ctx, hipGraphLaunch(s->graph2, st));
+        }
+        for (; k < nsteps; k++) HIPCHK(ctx, launch_step(s, st));
     }
-    s->binned = false;
+    int rc = sync_staging(s, true);
+    if (rc != MAVI_OK) return rc;
     return check_flags(s);
 }
 
@@ -498,7 +555,124 @@ int mavi_sys_set_tile(mavi_sys *s, int32_t workgroup) {
     if (!s) return MAVI_E_BADARG;
     if (workgroup == 0) workgroup = 256;
     if (!is_pow2_wg(workgroup)) return set_err(s->ctx, MAVI_E_BADARG, "workgroup must be 64..1024, power of two");
+    if (workgroup != s->wg && s->graph2) {
+        (void)hipStreamSynchronize(s->ctx->stream);
+        (void)hipGraphExecDestroy(s->graph2);
+        s->graph2 = nullptr;
+    }
     s->wg = workgroup;
+    return MAVI_OK;
+}
+
+int mavi_sys_set_option(mavi_sys *s, int32_t option, int32_t value) {
+    if (!s) return MAVI_E_BADARG;
+    bool changed = false;
+    switch (option) {
+        case MAVI_OPT_TWO_PHASE:
+            changed = s->two_phase != (value != 0);
+            s->two_phase = value != 0;
+            break;
+        case MAVI_OPT_GRAPHS:
+            s->use_graphs = value != 0;
+            break;
+        case MAVI_OPT_LDS_TILES:
+            changed = s->lds_tiles != (value != 0);
+            s->lds_tiles = value != 0;
+            break;
+        default:
+            return set_err(s->ctx, MAVI_E_BADARG, "unknown option");
+    }
+    if (changed && s->graph2) {  // the captured graph holds the old kernels
+        (void)hipStreamSynchronize(s->ctx->stream);
+        (void)hipGraphExecDestroy(s->graph2);
+        s->graph2 = nullptr;
+    }
+    return MAVI_OK;
+}
+
+int mavi_autotune(mavi_sys *s, int32_t trial_steps, double *best_us, int32_t *shape) {
+    if (!s || trial_steps <= 0) return MAVI_E_BADARG;
+    mavi_ctx *ctx = s->ctx;
+    if (!s->has_state) return set_err(ctx, MAVI_E_BADARG, "no state uploaded");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    // snapshot of the id-order state; every trial starts from it
+    int rc = sync_staging(s, true);
+    if (rc != MAVI_OK) return rc;
+    const size_t bytes = sizeof(double2) * (size_t)s->P.n;
+    double2 *pos0 = nullptr, *vel0 = nullptr;
+    double *pol0 = nullptr;
+    HIPCHK(ctx, dalloc(&pos0, s->P.n, st));
+    hipError_t e = hipMemcpyAsync(pos0, s->pos_id, bytes, hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess && s->vel_id) {
+        e = dalloc(&vel0, s->P.n, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(vel0, s->vel_id, bytes, hipMemcpyDeviceToDevice, st);
+    }
+    if (e == hipSuccess && s->pol) {
+        e = dalloc(&pol0, s->P.nrings, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(pol0, s->pol, sizeof(double) * s->P.nrings, hipMemcpyDeviceToDevice, st);
+    }
+    const unsigned long long *counter0 = nullptr;
+    (void)counter0;
+    auto restore = [&]() -> hipError_t {
+        hipError_t r = hipMemcpyAsync(s->pos_id, pos0, bytes, hipMemcpyDeviceToDevice, st);
+        if (r == hipSuccess && vel0) r = hipMemcpyAsync(s->vel_id, vel0, bytes, hipMemcpyDeviceToDevice, st);
+        if (r == hipSuccess && pol0)
+            r = hipMemcpyAsync(s->pol, pol0, sizeof(double) * s->P.nrings, hipMemcpyDeviceToDevice, st);
+        s->cur = 0;
+        if (r == hipSuccess) r = mavi::launch_iota_state(s, st);
+        if (r == hipSuccess && s->P.potential == MAVI_POT_RINGS) r = mavi::launch_rings_init(s, st);
+        if (r == hipSuccess) r = hipStreamSynchronize(st);
+        return r;
+    };
+    const int wgs[4] = {64, 128, 256, 512};
+    double best = 1e300;
+    int bw = s->wg, bl = s->lds_tiles, bt = s->two_phase;
+    const bool saved_prof = s->prof;
+    s->prof = false;
+    for (int lds = 0; lds < 2 && e == hipSuccess; lds++) {
+        if (lds && s->P.cols == 0) continue;
+        for (int tp = 0; tp < 2 && e == hipSuccess; tp++) {
+            for (int wi = 0; wi < 4 && e == hipSuccess; wi++) {
+                if (lds && wi > 0) continue;  // the LDS form has a fixed workgroup
+                if (mavi_sys_set_tile(s, wgs[wi]) != MAVI_OK) continue;
+                mavi_sys_set_option(s, MAVI_OPT_LDS_TILES, lds);
+                mavi_sys_set_option(s, MAVI_OPT_TWO_PHASE, tp);
+                if ((e = restore()) != hipSuccess) break;
+                if (steps_impl(s, 2, nullptr) != MAVI_OK) continue;  // warm-up (and graph capture)
+                hipEvent_t a = s->ev0, b = s->ev1;
+                e = hipEventRecord(a, st);
+                if (e == hipSuccess && steps_impl(s, trial_steps, nullptr) != MAVI_OK) continue;
+                if (e == hipSuccess) e = hipEventRecord(b, st);
+                if (e == hipSuccess) e = hipEventSynchronize(b);
+                float ms = 0.f;
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+                const double us = 1e3 * ms / trial_steps;
+                if (e == hipSuccess && us < best) {
+                    best = us;
+                    bw = wgs[wi];
+                    bl = lds;
+                    bt = tp;
+                }
+            }
+        }
+    }
+    s->prof = saved_prof;
+    mavi_sys_set_tile(s, bw);
+    mavi_sys_set_option(s, MAVI_OPT_LDS_TILES, bl);
+    mavi_sys_set_option(s, MAVI_OPT_TWO_PHASE, bt);
+    if (e == hipSuccess) e = restore();
+    dfree(pos0);
+    dfree(vel0);
+    dfree(pol0);
+    HIPCHK(ctx, e);
+    if (best_us) *best_us = best;
+    if (shape) {
+        shape[0] = bw;
+        shape[1] = bl;
+        shape[2] = bt;
+    }
     return MAVI_OK;
 }
 

@@ -1,18 +1,23 @@
 // mavi_internal.h -- shared types and device helpers of libmavi_hip.so.
 //
-// Device data layout (per system, all in HBM, allocated once at creation):
-//   pos, vel, force        double2[n]      id order (slot order of state.pos)
-//   key, rank              int32[n]        cell of each id / arrival rank in it
-//   cnt, start             int32[ncell+1]  particles per cell / exclusive scan
-//   perm                   int32[na]       sorted slot -> id  (cells row-major,
-//                                          ids ascending inside a cell)
-//   scell                  int32[na]       cell of each sorted slot
-//   ps, ps2                double2[na]     sorted positions (x_n, drifted x')
-//   f1                     double2[na]     first-evaluation force, sorted order
-// where na = number of active ids (get_particles_ids) and the cell index is
+// Device data layout (per system, HBM, allocated once at creation).
+// The particle state is kept SORTED BY CHUNK between steps: every step's
+// binning permutes it into the new chunk order, so all kernels read and
+// write it with unit stride.  Two copies ping-pong (index `cur`).
+//   P[2], V[2]   double2[na]   positions / velocities, sorted slots
+//   ID[2]        int32[na]     slot -> particle id (id = slot of state.pos)
+//   F            double2[na]   forces of the last evaluation, sorted slots
+//   P2, F1       double2[na]   drifted positions x' and first-evaluation
+//                              forces inside a Verlet step
+//   key, rank    int32[na]     new chunk of each old slot / rank in it
+//   src, scell   int32[na]     new slot -> old slot / chunk of new slot
+//   slot_of      int32[n]      particle id -> sorted slot (rings)
+//   cnt, start   int32[ncell+1] particles per chunk / exclusive scan
+//   pos_id, vel_id, force_id   double2[n] id-order staging for the host
+// na = number of active ids (get_particles_ids); the chunk index is
 // row0*cols + col0 with row0 = reference row - 1 (row 1 = top of the box,
-// src/chunks.jl:129) and col0 = reference col - 1.  Row-major cells make the
-// three cells (row, col-1..col+1) one contiguous range of sorted slots.
+// src/chunks.jl:129) and col0 = reference col - 1.  Row-major chunks make the
+// three chunks (row, col-1..col+1) one contiguous range of sorted slots.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -24,6 +29,7 @@
 #define MAVI_FLAG_OVERFLOW 1u
 #define MAVI_FLAG_OUT_OF_BOX 2u
 #define MAVI_FLAG_NONFINITE 4u
+#define MAVI_FLAG_SCAN_TIMEOUT 8u
 
 struct mavi_ctx {
     int device = 0;
@@ -47,13 +53,13 @@ struct SysParams {
     double p0, p1, p2, p3;
     double cut2;       // exact threshold: dist > rcut  <=>  r2 > cut2
     double eq2;        // exact threshold: dist < dist_eq <=> r2 < eq2
-    double lj24eps, ljsig2;
+    double lj_c12, lj_c6;  // 48 eps sigma^12, 24 eps sigma^6
     // rings
     int32_t nmax, nrings, ntypes;
     int32_t typed;
 };
 
-// Per ring type parameters (device constant-like array).
+// Per ring type parameters.
 struct RingType {
     double p0, relax, vo, mu, rot, karea, kspring, lspring;
     int32_t np;
@@ -76,33 +82,39 @@ struct mavi_sys {
     mavi_ctx *ctx = nullptr;
     SysParams P{};
     int32_t wg = 256;  // pair-force workgroup size
+    bool two_phase = false; // candidate list for finite-range interactions
+    bool lds_tiles = false; // LDS-staged chunk-row segments for the pair kernels
+    double list2 = INFINITY; // rings: widest interaction range squared
+    int cur = 0;       // which ping-pong copy holds the state
     // device buffers
-    double2 *pos = nullptr, *vel = nullptr, *force = nullptr;
-    int32_t *key = nullptr, *rank = nullptr, *cnt = nullptr, *start = nullptr;
-    int32_t *bsum = nullptr;  // scan block sums
-    int32_t *perm = nullptr, *scell = nullptr;
-    double2 *ps = nullptr, *ps2 = nullptr, *f1 = nullptr;
+    double2 *P_[2] = {nullptr, nullptr}, *V_[2] = {nullptr, nullptr};
+    int32_t *ID_[2] = {nullptr, nullptr};
+    double2 *F = nullptr, *P2 = nullptr, *F1 = nullptr;
+    int32_t *key = nullptr, *rank = nullptr, *src = nullptr, *scell = nullptr, *slot_of = nullptr;
+    int32_t *cnt = nullptr, *start = nullptr;
+    unsigned long long *scan_status = nullptr;
     uint32_t *flags = nullptr;
-    int32_t *active_ids = nullptr;  // na ids ascending (rings with padding)
+    unsigned long long *counter = nullptr;  // device step counter (rotational noise)
+    int32_t *active_ids = nullptr;          // na ids ascending
+    double2 *pos_id = nullptr, *vel_id = nullptr, *force_id = nullptr;
+    int32_t nscan_blocks = 0;
     // rings
     double *pol = nullptr, *areas = nullptr;
     double2 *cms = nullptr, *cms_next = nullptr, *cpos = nullptr;
-    int32_t *rtype = nullptr;   // per ring, 0-based
-    RingType *types = nullptr;  // ntypes
-    RingInter *inter = nullptr; // ntypes*ntypes
-    double *noise = nullptr;    // host-fed noise staging
+    int32_t *rtype = nullptr;
+    RingType *types = nullptr;
+    RingInter *inter = nullptr;
+    double *noise = nullptr;
     int64_t noise_cap = 0;
     uint64_t seed = 0;
-    uint64_t rng_step = 0;      // device noise counter (steps taken)
     // observables scratch
     double *red = nullptr;
     int32_t red_cap = 0;
-    int32_t nscan_blocks = 0;
     bool has_state = false;
-    bool binned = false;        // cells valid for current positions
-    // graph of G steps
-    hipGraphExec_t graph = nullptr;
-    int64_t graph_steps = 0;
+    bool sorted = false;        // a binning has put the state in chunk order
+    // graphs of two steps (state starting in copy 0)
+    hipGraphExec_t graph2 = nullptr;
+    bool use_graphs = true;
     // profiling
     bool prof = false;
     ProfSlot profs[MAVI_PROF_NCLASS];
@@ -134,15 +146,18 @@ __device__ __forceinline__ double min_image(double dr, double size, double half)
     return dr - m * copysign(size, dr);
 }
 
-// Host-side launchers (mavi_kernels.hip)
+// Host-side launchers (mavi_kernels.hip).  Each works on the state copy
+// s->cur and leaves the result in s->cur (binning flips it).
 namespace mavi {
-hipError_t launch_bin(mavi_sys *s, hipStream_t st);
+hipError_t launch_iota_state(mavi_sys *s, hipStream_t st);          // id-order staging -> state copy cur
+hipError_t launch_unpermute(mavi_sys *s, bool vel, bool force, hipStream_t st);  // state -> staging
+hipError_t launch_bin(mavi_sys *s, hipStream_t st);                 // binning + permutation (cur flips)
 hipError_t launch_export_chunks(mavi_sys *s, int64_t *d_cp, int64_t *d_counts, hipStream_t st);
 hipError_t launch_forces_only(mavi_sys *s, hipStream_t st);
 hipError_t launch_verlet_step(mavi_sys *s, hipStream_t st);
-hipError_t launch_rings_step(mavi_sys *s, const double *noise_dev, uint64_t step_index, hipStream_t st);
+hipError_t launch_rings_step(mavi_sys *s, const double *noise_dev, hipStream_t st);
 hipError_t launch_rings_forces(mavi_sys *s, hipStream_t st);
-hipError_t launch_rings_info(mavi_sys *s, hipStream_t st);
+hipError_t launch_rings_init(mavi_sys *s, hipStream_t st);
 hipError_t launch_observables(mavi_sys *s, double *d_out, hipStream_t st);
 hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t st);
 }  // namespace mavi

@@ -1,22 +1,25 @@
 // mavi_kernels.hip -- gfx950 kernels of the step hot path.
 //
-//   K1 binning      k_bin_count -> k_scan_* -> k_scatter -> k_cell_finalize
-//                   (counting sort by chunk; ids ascending inside a chunk, as
-//                   update_chunks! visits them, src/chunks.jl:150-161)
-//   K2 pair forces  k_pair<MODE, POT>: gather over the 3x3 chunk block around
-//                   each particle; the stencil of src/chunks.jl:61-118 visits
-//                   each adjacent chunk pair once, the gather visits it from
+//   K1 binning      k_bin_count (wave-aggregated chunk counts) -> k_scan
+//                   (single-pass look-back scan) -> k_scatter -> k_finalize
+//                   (ids ascending inside a chunk, as update_chunks! appends
+//                   them, src/chunks.jl:150-161; the state is permuted into
+//                   the new chunk order)
+//   K2 pair forces  k_pair<MODE, POT>: each particle gathers over the 3x3
+//                   chunk block around its chunk.  The stencil of
+//                   src/chunks.jl:61-118 visits each adjacent chunk pair once
+//                   and applies f and -f; the gather visits every pair from
 //                   both sides, so the interaction sets are equal for grids
 //                   >= 3x3 (src/integration.jl:112-157).
 //   K3 Verlet+walls MODE_VERLET1/2 epilogues of k_pair (src/integration.jl:
 //                   415-431 with walls! :271-285 / :309-324).
-//   K4 rings        k_rings<UPDATE>: per-ring unwrap, shoelace area,
-//                   springs, area force and overdamped update
+//   K4 rings        k_rings<UPDATE>: per-ring unwrap, shoelace area, springs,
+//                   area force and overdamped update
 //                   (src/rings/integration.jl:79-351).
 //   K7 observables  k_observables (src/quantities.jl:12-18, 46-66).
 //
 // The library is compiled with -ffp-contract=off: a*b+c is only fused where
-// the code calls __builtin_fma, so the arithmetic that must follow Julia's
+// the code calls __builtin_fma, so the arithmetic that follows Julia's
 // rounding (positions, velocities, HarmTrunc forces, ring updates) does.
 #include "mavi_internal.h"
 
@@ -35,8 +38,8 @@ __device__ __forceinline__ int xcd_block() {
 }
 
 __device__ __forceinline__ double rcp_nr(double x) {
-    // v_rcp_f64 followed by two Newton steps: |error| ~ 1 ulp, enough for
-    // the 1e-12 force tolerance of the LJ path.
+    // v_rcp_f64 followed by two Newton steps: ~1 ulp, far inside the 1e-12
+    // force tolerance of the LJ path.
     double y = __builtin_amdgcn_rcp(x);
     double e = __builtin_fma(-x, y, 1.0);
     y = __builtin_fma(y, e, y);
@@ -45,9 +48,42 @@ __device__ __forceinline__ double rcp_nr(double x) {
     return y;
 }
 
+__device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
+
+__global__ void k_counter_inc(unsigned long long *counter) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] += 1ull;
+}
+
 __global__ void k_fill_i32(int32_t *p, int32_t v, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// State staging (host id order <-> sorted state)
+// ---------------------------------------------------------------------------
+
+__global__ void k_iota_state(int32_t na, const int32_t *__restrict__ active, const double2 *__restrict__ pos_id,
+                             const double2 *__restrict__ vel_id, double2 *__restrict__ P, double2 *__restrict__ V,
+                             int32_t *__restrict__ ID, int32_t *__restrict__ slot_of) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= na) return;
+    int id = active ? active[i] : i;
+    P[i] = pos_id[id];
+    if (V) V[i] = vel_id[id];
+    ID[i] = id;
+    if (slot_of) slot_of[id] = i;
+}
+
+__global__ void k_unpermute(int32_t na, const int32_t *__restrict__ ID, const double2 *__restrict__ P,
+                            const double2 *__restrict__ V, const double2 *__restrict__ F, double2 *__restrict__ pos_id,
+                            double2 *__restrict__ vel_id, double2 *__restrict__ force_id) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= na) return;
+    int id = ID[t];
+    if (pos_id) pos_id[id] = P[t];
+    if (vel_id) vel_id[id] = V[t];
+    if (force_id) force_id[id] = F[t];
 }
 
 // ---------------------------------------------------------------------------
@@ -74,23 +110,51 @@ __device__ __forceinline__ int32_t cell_of(const SysParams &P, double x, double 
     return (int32_t)(r * P.cols + c);
 }
 
-__global__ void k_bin_count(SysParams P, const double2 *__restrict__ pos, const int32_t *__restrict__ active,
-                            int32_t *__restrict__ key, int32_t *__restrict__ rank, int32_t *__restrict__ cnt,
-                            uint32_t *__restrict__ flags) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.na) return;
-    int id = active ? active[i] : i;
-    double2 p = pos[id];
+// Chunk of every particle and its arrival rank in that chunk.  Lanes of a
+// wave that land in the same chunk (most of them: the state is sorted by the
+// previous step's chunks) are counted by one atomic of their leader lane.
+__global__ void k_bin_count(SysParams P, const double2 *__restrict__ pos, int32_t *__restrict__ key,
+                            int32_t *__restrict__ rank, int32_t *__restrict__ cnt, uint32_t *__restrict__ flags) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = s < P.na;
     uint32_t flag = 0;
-    int32_t c = cell_of(P, p.x, p.y, &flag);
+    int32_t k = -1;
+    if (valid) {
+        const double2 p = pos[s];
+        k = cell_of(P, p.x, p.y, &flag);
+    }
     if (flag) atomicOr(flags, flag);
-    key[i] = c;
-    rank[i] = atomicAdd(&cnt[c], 1);
+    const unsigned lane = lane_id();
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    unsigned long long remaining = __ballot(valid);
+    int my_leader = (int)lane, my_pos = 0, grp_n = 0;
+    while (remaining) {
+        const int leader = __ffsll((unsigned long long)remaining) - 1;
+        const int lkey = __shfl(k, leader, 64);
+        const unsigned long long grp = __ballot(valid && k == lkey);
+        if (valid && k == lkey) {
+            my_leader = leader;
+            my_pos = __popcll(grp & lt);
+        }
+        if ((int)lane == leader) grp_n = __popcll(grp);
+        remaining &= ~grp;
+    }
+    int base = 0;
+    if (valid && (int)lane == my_leader) base = atomicAdd(&cnt[k], grp_n);
+    base = __shfl(base, my_leader, 64);
+    if (valid) {
+        key[s] = k;
+        rank[s] = base + my_pos;
+    }
 }
 
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 4;
 constexpr int kScanTile = kScanThreads * kScanItems;
+constexpr unsigned long long kStA = 1ull << 62;  // tile aggregate published
+constexpr unsigned long long kStP = 2ull << 62;  // inclusive prefix published
+constexpr unsigned long long kStMask = 0xffffffffull;
+constexpr int kSpinLimit = 1 << 22;
 
 // Block-wide exclusive scan of one int per thread (256 threads, wave64).
 __device__ __forceinline__ int32_t block_excl_scan(int32_t v, int32_t *total) {
@@ -114,111 +178,130 @@ __device__ __forceinline__ int32_t block_excl_scan(int32_t v, int32_t *total) {
     return woff + incl - v;
 }
 
-__global__ void k_scan_local(const int32_t *__restrict__ in, int32_t *__restrict__ out, int32_t *__restrict__ bsum,
-                             int32_t n) {
-    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+// Single-pass exclusive scan of cnt[0..n) with decoupled look-back.  Tiles
+// are workgroups in dispatch order (workgroups start in increasing id order,
+// so a tile only waits on tiles that have started); the wait is bounded
+// (kSpinLimit) and raises MAVI_FLAG_SCAN_TIMEOUT instead of spinning forever.
+__global__ void k_scan(const int32_t *__restrict__ in, int32_t *__restrict__ out,
+                       unsigned long long *__restrict__ status, int32_t *__restrict__ tile_counter, int32_t n,
+                       uint32_t *__restrict__ flags) {
+    __shared__ int32_t s_prefix;
+    const int b = blockIdx.x;
+    (void)tile_counter;
+    const int base = b * kScanTile + threadIdx.x * kScanItems;
     int32_t v[kScanItems];
-    int32_t s = 0;
+    int32_t sum = 0;
 #pragma unroll
     for (int k = 0; k < kScanItems; k++) {
         v[k] = (base + k < n) ? in[base + k] : 0;
-        s += v[k];
+        sum += v[k];
     }
     int32_t total;
-    int32_t off = block_excl_scan(s, &total);
+    int32_t off = block_excl_scan(sum, &total);
+    if (threadIdx.x < 64) {
+        // first wave: publish the tile aggregate, then look back 64 tiles at a
+        // time for the nearest tile with a published inclusive prefix
+        const int lane = threadIdx.x;
+        int32_t exclusive = 0;
+        if (b == 0) {
+            if (lane == 0)
+                __hip_atomic_store(&status[0], kStP | (unsigned long long)(uint32_t)total, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            if (lane == 0)
+                __hip_atomic_store(&status[b], kStA | (unsigned long long)(uint32_t)total, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            int window_end = b - 1;
+            int spins = 0;
+            while (true) {
+                const int p = window_end - lane;
+                // tiles before tile 0 read as a published prefix of 0
+                const unsigned long long st =
+                    (p >= 0) ? __hip_atomic_load(&status[p], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : kStP;
+                const unsigned long long fl = st & (3ull << 62);
+                const unsigned long long pmask = __ballot(fl == kStP);
+                const unsigned long long zmask = __ballot(fl == 0);
+                const int firstP = pmask ? __ffsll(pmask) - 1 : 64;
+                const unsigned long long need = (firstP >= 63) ? ~0ull : ((2ull << firstP) - 1ull);
+                if (zmask & need) {  // a tile in front of the nearest prefix has not published yet
+                    if (++spins > kSpinLimit) {
+                        if (lane == 0) atomicOr(flags, MAVI_FLAG_SCAN_TIMEOUT);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                int32_t val = (lane <= firstP) ? (int32_t)(st & kStMask) : 0;
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) val += __shfl_xor(val, o, 64);
+                exclusive += val;
+                if (firstP < 64) break;
+                window_end -= 64;
+            }
+            if (lane == 0)
+                __hip_atomic_store(&status[b], kStP | (unsigned long long)(uint32_t)(exclusive + total),
+                                   __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (lane == 0) s_prefix = exclusive;
+    }
+    __syncthreads();
+    off += s_prefix;
 #pragma unroll
     for (int k = 0; k < kScanItems; k++) {
         if (base + k < n) out[base + k] = off;
         off += v[k];
     }
-    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
 }
 
-// Exclusive scan of the block sums in one workgroup, tile by tile.
-__global__ void k_scan_bsum(int32_t *__restrict__ bsum, int32_t nb) {
-    __shared__ int32_t carry;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    for (int t0 = 0; t0 < nb; t0 += kScanThreads) {
-        int i = t0 + threadIdx.x;
-        int32_t v = (i < nb) ? bsum[i] : 0;
-        int32_t total;
-        int32_t ex = block_excl_scan(v, &total);
-        int32_t c = carry;
-        if (i < nb) bsum[i] = ex + c;
-        __syncthreads();
-        if (threadIdx.x == 0) carry = c + total;
-        __syncthreads();
+// Place every particle at start[chunk] + arrival rank; reset the counts, the
+// scan state and the tile counter for the next binning; advance the step
+// counter of the device rotational noise.
+__global__ void k_scatter(SysParams P, const int32_t *__restrict__ key, const int32_t *__restrict__ rank,
+                          const int32_t *__restrict__ start, int32_t *__restrict__ src, int32_t *__restrict__ scell,
+                          int32_t *__restrict__ cnt, unsigned long long *__restrict__ status, int32_t nstatus,
+                          int32_t *__restrict__ tile_counter, unsigned long long *__restrict__ counter) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < P.na) {
+        const int k = key[i];
+        const int t = start[k] + rank[i];
+        src[t] = i;
+        scell[t] = k;
+    }
+    if (i <= P.ncell) cnt[i] = 0;
+    if (i < nstatus) status[i] = 0ull;
+    if (i == 0) {
+        *tile_counter = 0;
+        counter[0] += 1ull;
     }
 }
 
-__global__ void k_scan_add(int32_t *__restrict__ out, const int32_t *__restrict__ bsum, int32_t n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] += bsum[i / kScanTile];
-}
-
-__global__ void k_scatter(SysParams P, const int32_t *__restrict__ active, const int32_t *__restrict__ key,
-                          const int32_t *__restrict__ rank, const int32_t *__restrict__ start,
-                          int32_t *__restrict__ perm) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.na) return;
-    int id = active ? active[i] : i;
-    perm[start[key[i]] + rank[i]] = id;
-}
-
-// One thread per chunk: restore ascending id order inside the chunk (the
-// order update_chunks! appends them in), check the capacity nc, and gather
-// the chunk's positions into sorted order.
-__global__ void k_cell_finalize(SysParams P, const int32_t *__restrict__ start, int32_t *__restrict__ perm,
-                                const double2 *__restrict__ pos, double2 *__restrict__ ps,
-                                int32_t *__restrict__ scell, uint32_t *__restrict__ flags) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= P.ncell) return;
-    const int b = start[c], e = start[c + 1], m = e - b;
-    if (m == 0) return;
-    if (m > P.nc) atomicOr(flags, MAVI_FLAG_OVERFLOW);
-    constexpr int R = 16;
-    if (m <= R) {
-        int32_t v[R];
-#pragma unroll
-        for (int k = 0; k < R; k++) v[k] = (k < m) ? perm[b + k] : 0x7fffffff;
-        // insertion sort in registers (fully unrolled, fixed network)
-#pragma unroll
-        for (int k = 1; k < R; k++) {
-#pragma unroll
-            for (int q = k; q > 0; q--) {
-                int32_t lo = min(v[q - 1], v[q]), hi = max(v[q - 1], v[q]);
-                v[q - 1] = lo;
-                v[q] = hi;
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < R; k++)
-            if (k < m) {
-                perm[b + k] = v[k];
-                ps[b + k] = pos[v[k]];
-                scell[b + k] = c;
-            }
-    } else {
-        for (int k = b + 1; k < e; k++) {
-            int32_t x = perm[k];
-            int q = k - 1;
-            while (q >= b && perm[q] > x) {
-                perm[q + 1] = perm[q];
-                q--;
-            }
-            perm[q + 1] = x;
-        }
-        for (int k = b; k < e; k++) {
-            ps[k] = pos[perm[k]];
-            scell[k] = c;
-        }
-    }
+// One thread per new slot: the final slot inside the chunk is the number of
+// chunk members with a smaller id (update_chunks! appends ids ascending);
+// move the particle's state there.
+__global__ void k_finalize(SysParams P, const int32_t *__restrict__ start, const int32_t *__restrict__ src,
+                           const int32_t *__restrict__ scell, const double2 *__restrict__ Pold,
+                           const double2 *__restrict__ Vold, const int32_t *__restrict__ IDold,
+                           double2 *__restrict__ Pnew, double2 *__restrict__ Vnew, int32_t *__restrict__ IDnew,
+                           int32_t *__restrict__ slot_of, uint32_t *__restrict__ flags) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P.na) return;
+    const int c = scell[t];
+    const int b = start[c], e = start[c + 1];
+    const int so = src[t];
+    const int myid = IDold[so];
+    int r = 0;
+    for (int q = b; q < e; q++) r += (IDold[src[q]] < myid) ? 1 : 0;
+    const int t2 = b + r;
+    Pnew[t2] = Pold[so];
+    if (Vnew) Vnew[t2] = Vold[so];
+    IDnew[t2] = myid;
+    if (slot_of) slot_of[myid] = t2;
+    if (t == b && e - b > P.nc) atomicOr(flags, MAVI_FLAG_OVERFLOW);
 }
 
 // chunk_particles (nc, rows, cols) and num_particles_in_chunk (rows, cols),
 // column-major, 1-based: the reference layout of src/chunks.jl:23-24.
-__global__ void k_export_chunks(SysParams P, const int32_t *__restrict__ start, const int32_t *__restrict__ perm,
+__global__ void k_export_chunks(SysParams P, const int32_t *__restrict__ start, const int32_t *__restrict__ ID,
                                 int64_t *__restrict__ cp, int64_t *__restrict__ counts) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= P.ncell) return;
@@ -227,17 +310,7 @@ __global__ void k_export_chunks(SysParams P, const int32_t *__restrict__ start, 
     int b = start[c], m = start[c + 1] - b;
     counts[k] = m;
     int lim = m < P.nc ? m : P.nc;
-    for (int q = 0; q < lim; q++) cp[(int64_t)P.nc * k + q] = (int64_t)perm[b + q] + 1;
-}
-
-// Gather for chunks = nothing: sorted order is the active id order.
-__global__ void k_gather_active(SysParams P, const int32_t *__restrict__ active, const double2 *__restrict__ pos,
-                                int32_t *__restrict__ perm, double2 *__restrict__ ps) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.na) return;
-    int id = active ? active[i] : i;
-    perm[i] = id;
-    ps[i] = pos[id];
+    for (int q = 0; q < lim; q++) cp[(int64_t)P.nc * k + q] = (int64_t)ID[b + q] + 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -248,7 +321,7 @@ enum { MODE_FORCES = 0, MODE_VERLET1 = 1, MODE_VERLET2 = 2 };
 enum { POT_LJ = 0, POT_HT = 1, POT_RINGS = 2 };
 
 struct RingCtx {
-    const int32_t *perm;
+    const int32_t *ids;  // sorted slot -> id
     const int32_t *rtype;
     const RingType *types;
     const RingInter *inter;
@@ -271,18 +344,18 @@ __device__ __forceinline__ int32_t ring_of(uint32_t id, const RingCtx &R) {
 template <int POT>
 struct PairAcc {
     double fx = 0.0, fy = 0.0;
-    // pair context for rings
-    int32_t id_i = 0, ring_i = 0, type_i = 0;
+    int32_t id_i = 0, ring_i = 0, type_i = 0;  // rings only
 
+    // id_j: particle id of the candidate (read by the rings interaction only)
     __device__ __forceinline__ void add(const SysParams &P, const RingCtx &R, double dx, double dy, bool self_ok,
-                                        int j) {
+                                        int id_j) {
         const double r2 = dx * dx + dy * dy;  // sum(dr.^2): two roundings, no FMA
         if constexpr (POT == POT_LJ) {
+            // fmod/dist = 4 eps (12 s^12/d^14 - 6 s^6/d^8) = inv^4 (c12 inv^3 - c6), inv = 1/d^2
             const bool ok = self_ok && (r2 <= P.cut2);
             const double inv = rcp_nr(r2);
-            const double s2 = P.ljsig2 * inv;
-            const double s6 = s2 * s2 * s2;
-            double fr = P.lj24eps * s6 * __builtin_fma(2.0, s6, -1.0) * inv;
+            const double inv3 = inv * inv * inv;
+            double fr = inv * inv3 * __builtin_fma(P.lj_c12, inv3, -P.lj_c6);
             fr = ok ? fr : 0.0;
             fx = __builtin_fma(fr, dx, fx);
             fy = __builtin_fma(fr, dy, fy);
@@ -298,7 +371,6 @@ struct PairAcc {
         } else {
             // rings calc_interaction_force, src/rings/integration.jl:47-77
             if (!self_ok) return;
-            const int32_t id_j = R.perm[j];
             const int32_t ring_j = ring_of((uint32_t)id_j, R);
             const RingInter *ic = R.inter;
             if (P.typed) ic += type_i + P.ntypes * R.rtype[ring_j];
@@ -320,10 +392,82 @@ struct PairAcc {
     }
 };
 
+// Candidates of one particle as up to three contiguous slot ranges, walked
+// as one loop (fewer idle lanes than one loop per range) with the next
+// position loaded before the current one is used.
+template <int POT, bool WRAP>
+__device__ __forceinline__ void scan3(const SysParams &P, const RingCtx &R, PairAcc<POT> &acc,
+                                      const double2 *__restrict__ pin, int s, double xi, double yi, int a0, int l0,
+                                      int a1, int l1, int a2, int l2) {
+    const int tot = l0 + l1 + l2;
+    if (tot <= 0) return;
+    const int b1 = l0, b2 = l0 + l1;
+    const int o0 = a0, o1 = a1 - l0, o2 = a2 - l0 - l1;
+    int j = (0 < b1) ? o0 : ((0 < b2) ? o1 : o2);
+    double2 pj = pin[j];
+    for (int k = 0; k < tot; k++) {
+        const int kn = k + 1;
+        const int jn = (kn < tot) ? kn + ((kn < b1) ? o0 : ((kn < b2) ? o1 : o2)) : j;
+        const double2 pn = pin[jn];
+        double dx = xi - pj.x, dy = yi - pj.y;
+        if constexpr (WRAP) {
+            dx = min_image(dx, P.L, P.halfL);
+            dy = min_image(dy, P.H, P.halfH);
+        }
+        acc.add(P, R, dx, dy, j != s, POT == POT_RINGS ? R.ids[j] : 0);
+        j = jn;
+        pj = pn;
+    }
+}
+
+// Two-phase form for finite-range interactions: a first pass over the
+// candidates only computes r^2 and keeps the in-range candidate numbers
+// (uint16, in LDS, column per lane); the FP64 force math then runs on the
+// kept candidates only, so lanes of a wave stay busy on pairs that matter.
+constexpr int kListCap = 32;
+
+template <int POT, bool WRAP>
+__device__ __forceinline__ bool scan3_listed(const SysParams &P, const RingCtx &R, PairAcc<POT> &acc,
+                                             const double2 *__restrict__ pin, int s, double xi, double yi, int a0,
+                                             int l0, int a1, int l1, int a2, int l2, double list2,
+                                             uint16_t *__restrict__ lst, int stride) {
+    const int tot = l0 + l1 + l2;
+    const int b1 = l0, b2 = l0 + l1;
+    const int o0 = a0, o1 = a1 - l0, o2 = a2 - l0 - l1;
+    int cnt = 0;
+    for (int k = 0; k < tot; k++) {
+        const int j = k + ((k < b1) ? o0 : ((k < b2) ? o1 : o2));
+        const double2 pj = pin[j];
+        double dx = xi - pj.x, dy = yi - pj.y;
+        if constexpr (WRAP) {
+            dx = min_image(dx, P.L, P.halfL);
+            dy = min_image(dy, P.H, P.halfH);
+        }
+        const double r2 = dx * dx + dy * dy;
+        if (j != s && r2 <= list2) {
+            if (cnt < kListCap) lst[cnt * stride] = (uint16_t)k;
+            cnt++;
+        }
+    }
+    const int m = cnt < kListCap ? cnt : kListCap;
+    for (int q = 0; q < m; q++) {
+        const int k = lst[q * stride];
+        const int j = k + ((k < b1) ? o0 : ((k < b2) ? o1 : o2));
+        const double2 pj = pin[j];
+        double dx = xi - pj.x, dy = yi - pj.y;
+        if constexpr (WRAP) {
+            dx = min_image(dx, P.L, P.halfL);
+            dy = min_image(dy, P.H, P.halfH);
+        }
+        acc.add(P, R, dx, dy, true, POT == POT_RINGS ? R.ids[j] : 0);
+    }
+    return cnt > kListCap;
+}
+
 template <int POT>
 __device__ __forceinline__ void scan_range(const SysParams &P, const RingCtx &R, PairAcc<POT> &acc,
-                                           const double2 *__restrict__ pin, int s, double xi, double yi,
-                                           bool wrap, int a, int b) {
+                                           const double2 *__restrict__ pin, int s, double xi, double yi, bool wrap,
+                                           int a, int b) {
     for (int j = a; j < b; j++) {
         const double2 pj = pin[j];
         double dx = xi - pj.x, dy = yi - pj.y;
@@ -331,72 +475,109 @@ __device__ __forceinline__ void scan_range(const SysParams &P, const RingCtx &R,
             dx = min_image(dx, P.L, P.halfL);
             dy = min_image(dy, P.H, P.halfH);
         }
-        acc.add(P, R, dx, dy, j != s, j);
+        acc.add(P, R, dx, dy, j != s, POT == POT_RINGS ? R.ids[j] : 0);
     }
 }
 
 template <int MODE, int POT, bool ALLPAIRS>
 __global__ void k_pair(SysParams P, RingCtx R, const double2 *__restrict__ pin, const int32_t *__restrict__ scell,
-                       const int32_t *__restrict__ start, const int32_t *__restrict__ perm,
-                       double2 *__restrict__ pos, double2 *__restrict__ vel, double2 *__restrict__ force,
-                       double2 *__restrict__ ps2, double2 *__restrict__ f1) {
+                       const int32_t *__restrict__ start, double2 *__restrict__ Pst, double2 *__restrict__ Vst,
+                       double2 *__restrict__ Fout, double2 *__restrict__ P2, double2 *__restrict__ F1,
+                       double list2) {
+    extern __shared__ uint16_t s_list[];
     const int s = xcd_block() * blockDim.x + threadIdx.x;
-    if (s >= P.na) return;
-    const double2 pi = pin[s];
+    const bool valid = s < P.na;
+    const int sv = valid ? s : 0;
+    const double2 pi = pin[sv];
     PairAcc<POT> acc;
     if constexpr (POT == POT_RINGS) {
-        acc.id_i = perm[s];
+        acc.id_i = R.ids[sv];
         acc.ring_i = ring_of((uint32_t)acc.id_i, R);
         acc.type_i = P.typed ? R.rtype[acc.ring_i] : 0;
     }
     if constexpr (ALLPAIRS) {
-        scan_range<POT>(P, R, acc, pin, s, pi.x, pi.y, P.periodic != 0, 0, P.na);
+        if (valid) scan_range<POT>(P, R, acc, pin, s, pi.x, pi.y, P.periodic != 0, 0, P.na);
     } else {
-        const int c = scell[s];
+        const int c = scell[sv];
         const int row = c / P.cols, col = c - row * P.cols;
-        const bool wrap = P.periodic && (row == 0 || row == P.rows - 1 || col == 0 || col == P.cols - 1);
-        for (int d = -1; d <= 1; d++) {
-            int r = row + d;
-            if (P.periodic) r = (r < 0) ? r + P.rows : (r >= P.rows ? r - P.rows : r);
-            else if (r < 0 || r >= P.rows) continue;
-            const int base = r * P.cols;
-            int a0, a1, b0 = 0, b1 = 0;
-            if (!P.periodic) {
-                const int c0 = col > 0 ? col - 1 : 0, c1 = col < P.cols - 1 ? col + 1 : P.cols - 1;
-                a0 = start[base + c0];
-                a1 = start[base + c1 + 1];
-            } else if (col == 0) {
-                a0 = start[base];
-                a1 = start[base + 2];
-                b0 = start[base + P.cols - 1];
-                b1 = start[base + P.cols];
-            } else if (col == P.cols - 1) {
-                a0 = start[base + P.cols - 2];
-                a1 = start[base + P.cols];
-                b0 = start[base];
-                b1 = start[base + 1];
-            } else {
-                a0 = start[base + col - 1];
-                a1 = start[base + col + 2];
+        const bool edge_col = col == 0 || col == P.cols - 1;
+        const bool wrap = valid && P.periodic && (row == 0 || row == P.rows - 1 || edge_col);
+        const bool split = valid && P.periodic && edge_col;
+        if (!__any(split)) {
+            // every lane: one slot range per chunk row
+            const int c0 = col > 0 ? col - 1 : 0, c1 = col < P.cols - 1 ? col + 1 : P.cols - 1;
+            auto row_range = [&](int d, int &a, int &l) {
+                int r = row + d;
+                bool ok = valid;
+                if (P.periodic) r = (r < 0) ? r + P.rows : (r >= P.rows ? r - P.rows : r);
+                else ok = ok && r >= 0 && r < P.rows;
+                const int base = (ok ? r : 0) * P.cols;
+                a = ok ? start[base + c0] : 0;
+                l = ok ? start[base + c1 + 1] - a : 0;
+            };
+            int a0, l0, a1, l1, a2, l2;
+            row_range(-1, a0, l0);
+            row_range(0, a1, l1);
+            row_range(1, a2, l2);
+            const bool wrapw = __any(wrap);
+            // list entries are 16-bit candidate numbers
+            bool direct = !(list2 < INFINITY) || __any(l0 + l1 + l2 > 65535);
+            if (!direct) {
+                uint16_t *lst = s_list + threadIdx.x;
+                const bool over =
+                    wrapw ? scan3_listed<POT, true>(P, R, acc, pin, s, pi.x, pi.y, a0, l0, a1, l1, a2, l2, list2, lst,
+                                                    blockDim.x)
+                          : scan3_listed<POT, false>(P, R, acc, pin, s, pi.x, pi.y, a0, l0, a1, l1, a2, l2, list2, lst,
+                                                     blockDim.x);
+                if (__any(over)) {  // a lane had more in-range candidates than the list holds
+                    acc.fx = 0.0;
+                    acc.fy = 0.0;
+                    direct = true;
+                }
             }
-            scan_range<POT>(P, R, acc, pin, s, pi.x, pi.y, wrap, a0, a1);
-            if (b1 > b0) scan_range<POT>(P, R, acc, pin, s, pi.x, pi.y, wrap, b0, b1);
+            if (direct) {
+                if (wrapw) scan3<POT, true>(P, R, acc, pin, s, pi.x, pi.y, a0, l0, a1, l1, a2, l2);
+                else scan3<POT, false>(P, R, acc, pin, s, pi.x, pi.y, a0, l0, a1, l1, a2, l2);
+            }
+        } else if (valid) {
+            // waves touching the periodic left/right edge: split row windows
+            for (int d = -1; d <= 1; d++) {
+                int r = row + d;
+                r = (r < 0) ? r + P.rows : (r >= P.rows ? r - P.rows : r);
+                const int base = r * P.cols;
+                int a0, a1, b0 = 0, b1 = 0;
+                if (col == 0) {
+                    a0 = start[base];
+                    a1 = start[base + 2];
+                    b0 = start[base + P.cols - 1];
+                    b1 = start[base + P.cols];
+                } else if (col == P.cols - 1) {
+                    a0 = start[base + P.cols - 2];
+                    a1 = start[base + P.cols];
+                    b0 = start[base];
+                    b1 = start[base + 1];
+                } else {
+                    a0 = start[base + col - 1];
+                    a1 = start[base + col + 2];
+                }
+                scan_range<POT>(P, R, acc, pin, s, pi.x, pi.y, wrap, a0, a1);
+                if (b1 > b0) scan_range<POT>(P, R, acc, pin, s, pi.x, pi.y, wrap, b0, b1);
+            }
         }
     }
+    if (!valid) return;
     const double fx = acc.fx, fy = acc.fy;
     if constexpr (MODE == MODE_FORCES) {
-        force[perm[s]] = make_double2(fx, fy);
+        Fout[s] = make_double2(fx, fy);
     } else if constexpr (MODE == MODE_VERLET1) {
         // update_verlet!: pos += vel*dt + forces*term (src/integration.jl:424)
-        const int id = perm[s];
-        const double2 v = vel[id];
-        ps2[s] = make_double2(pi.x + (v.x * P.dt + fx * P.term), pi.y + (v.y * P.dt + fy * P.term));
-        f1[s] = make_double2(fx, fy);
+        const double2 v = Vst[s];
+        P2[s] = make_double2(pi.x + (v.x * P.dt + fx * P.term), pi.y + (v.y * P.dt + fy * P.term));
+        F1[s] = make_double2(fx, fy);
     } else {
         // vel += dt/2 * (forces + old_forces) (src/integration.jl:430), then walls!
-        const int id = perm[s];
-        const double2 v = vel[id];
-        const double2 fo = f1[s];
+        const double2 v = Vst[s];
+        const double2 fo = F1[s];
         double vx = v.x + P.hdt * (fx + fo.x);
         double vy = v.y + P.hdt * (fy + fo.y);
         double x = pi.x, y = pi.y;
@@ -404,10 +585,8 @@ __global__ void k_pair(SysParams P, RingCtx R, const double2 *__restrict__ pin, 
             // src/integration.jl:309-324
             const double dxc = x - (P.blx + P.halfL), dyc = y - (P.bly + P.halfH);
             const bool ox = fabs(dxc) > P.halfL, oy = fabs(dyc) > P.halfH;
-            if (ox || oy) {
-                if (ox) x = x - copysign(P.halfL * 2.0, dxc);
-                if (oy) y = y - copysign(P.halfH * 2.0, dyc);
-            }
+            if (ox) x = x - copysign(P.halfL * 2.0, dxc);
+            if (oy) y = y - copysign(P.halfH * 2.0, dyc);
         } else {
             // src/integration.jl:271-285
             const double rx = x - P.blx, ry = y - P.bly;
@@ -416,9 +595,223 @@ __global__ void k_pair(SysParams P, RingCtx R, const double2 *__restrict__ pin, 
             if (ox) vx = -vx;
             if (oy) vy = -vy;
         }
-        pos[id] = make_double2(x, y);
-        vel[id] = make_double2(vx, vy);
-        force[id] = make_double2(fx, fy);
+        Pst[s] = make_double2(x, y);
+        Vst[s] = make_double2(vx, vy);
+        Fout[s] = make_double2(fx, fy);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K2/K3, LDS-tiled form.  One wave owns a segment of kSegCells consecutive
+// chunks of one chunk row.  It stages the particles of the three neighbouring
+// chunk rows over the segment's columns +-1 (about 3 * 14 * 4.4 = 185 at
+// density 0.7) into LDS, column by column in unwrapped column order, with an
+// offset table per (row, column).  A lane's 3x3 chunk block is then exactly
+// three contiguous LDS ranges, at the periodic edges too.  Global memory is
+// read in bulk; the candidate loops only touch LDS.
+// ---------------------------------------------------------------------------
+constexpr int kSegCells = 12;
+constexpr int kSegWaves = 4;
+constexpr int kStageCap = 416;
+constexpr int kSegCols = kSegCells + 2;        // unwrapped columns staged per row
+constexpr int kSegOff = 3 * kSegCols + 1;      // offset table entries
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int POT, bool LISTED>
+__device__ __forceinline__ bool seg_accumulate(const SysParams &P, const RingCtx &R, PairAcc<POT> &acc,
+                                               const double2 *__restrict__ stg, const int32_t *__restrict__ off,
+                                               int iu, int me, double xi, double yi, bool wrap, double list2,
+                                               uint16_t *__restrict__ lst, int lane, const int32_t *__restrict__ sid) {
+    // ranges: row d covers unwrapped columns iu-1..iu+1
+    const int a0 = off[iu - 1], e0 = off[iu + 2];
+    const int a1 = off[kSegCols + iu - 1], e1 = off[kSegCols + iu + 2];
+    const int a2 = off[2 * kSegCols + iu - 1], e2 = off[2 * kSegCols + iu + 2];
+    const int l0 = e0 - a0, l1 = e1 - a1, l2 = e2 - a2;
+    const int tot = l0 + l1 + l2;
+    const int b1 = l0, b2 = l0 + l1;
+    const int o0 = a0, o1 = a1 - l0, o2 = a2 - l0 - l1;
+    if constexpr (!LISTED) {
+        for (int k = 0; k < tot; k++) {
+            const int j = k + ((k < b1) ? o0 : ((k < b2) ? o1 : o2));
+            const double2 pj = stg[j];
+            double dx = xi - pj.x, dy = yi - pj.y;
+            if (wrap) {
+                dx = min_image(dx, P.L, P.halfL);
+                dy = min_image(dy, P.H, P.halfH);
+            }
+            acc.add(P, R, dx, dy, j != me, POT == POT_RINGS ? sid[j] : 0);
+        }
+        return false;
+    } else {
+        int cnt = 0;
+        for (int k = 0; k < tot; k++) {
+            const int j = k + ((k < b1) ? o0 : ((k < b2) ? o1 : o2));
+            const double2 pj = stg[j];
+            double dx = xi - pj.x, dy = yi - pj.y;
+            if (wrap) {
+                dx = min_image(dx, P.L, P.halfL);
+                dy = min_image(dy, P.H, P.halfH);
+            }
+            const double r2 = dx * dx + dy * dy;
+            if (j != me && r2 <= list2) {
+                if (cnt < kListCap) lst[cnt * 64 + lane] = (uint16_t)j;
+                cnt++;
+            }
+        }
+        const int m = cnt < kListCap ? cnt : kListCap;
+        for (int q = 0; q < m; q++) {
+            const int j = lst[q * 64 + lane];
+            const double2 pj = stg[j];
+            double dx = xi - pj.x, dy = yi - pj.y;
+            if (wrap) {
+                dx = min_image(dx, P.L, P.halfL);
+                dy = min_image(dy, P.H, P.halfH);
+            }
+            acc.add(P, R, dx, dy, true, POT == POT_RINGS ? sid[j] : 0);
+        }
+        return cnt > kListCap;
+    }
+}
+
+template <int MODE, int POT, bool LISTED>
+__global__ __launch_bounds__(64 * kSegWaves) void k_pair_seg(
+    SysParams P, RingCtx R, const double2 *__restrict__ pin, const int32_t *__restrict__ scell,
+    const int32_t *__restrict__ start, double2 *__restrict__ Pst, double2 *__restrict__ Vst,
+    double2 *__restrict__ Fout, double2 *__restrict__ P2, double2 *__restrict__ F1, double list2, int nsegc,
+    int nseg) {
+    __shared__ double2 s_stage[kSegWaves][kStageCap];
+    __shared__ int32_t s_off[kSegWaves][kSegOff];
+    __shared__ int32_t s_sid[kSegWaves][POT == POT_RINGS ? kStageCap : 1];  // staged particle ids (rings)
+    __shared__ uint16_t s_lst[kSegWaves][LISTED ? kListCap * 64 : 1];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = xcd_block() * kSegWaves + w;
+    if (g >= nseg) return;  // whole wave; no workgroup barrier in this kernel
+    const int R0 = g / nsegc;
+    const int c0 = (g - R0 * nsegc) * kSegCells;
+    const int c1 = min(c0 + kSegCells, P.cols) - 1;
+    const int nu = c1 - c0 + 3;  // unwrapped columns c0-1 .. c1+1
+    double2 *stg = s_stage[w];
+    int32_t *off = s_off[w];
+
+    // per (row d, unwrapped column iu) chunk: global start and count
+    int gstart = 0, gcount = 0;
+    const int li = lane;  // li = d * kSegCols + iu
+    const int d = li / kSegCols, iu = li - d * kSegCols;
+    if (li < 3 * kSegCols && iu < nu) {
+        int rr = R0 + d - 1;
+        bool ok = true;
+        if (P.periodic) rr = (rr < 0) ? rr + P.rows : (rr >= P.rows ? rr - P.rows : rr);
+        else ok = rr >= 0 && rr < P.rows;
+        int u = c0 - 1 + iu;
+        if (P.periodic) u = (u < 0) ? u + P.cols : (u >= P.cols ? u - P.cols : u);
+        else ok = ok && u >= 0 && u < P.cols;
+        if (ok) {
+            const int cell = rr * P.cols + u;
+            gstart = start[cell];
+            gcount = start[cell + 1] - gstart;
+        }
+    }
+    // exclusive scan of the counts over the 3*kSegCols table slots
+    int incl = gcount;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    const int excl = incl - gcount;
+    const int total = __shfl(incl, 3 * kSegCols - 1, 64);
+    if (lane < 3 * kSegCols) off[lane] = excl;
+    if (lane == 0) off[3 * kSegCols] = total;
+    const bool staged = total <= kStageCap;
+    if (staged) {
+        for (int q = 0; q < gcount; q++) {
+            stg[excl + q] = pin[gstart + q];
+            if constexpr (POT == POT_RINGS) s_sid[w][excl + q] = R.ids[gstart + q];
+        }
+    }
+    wave_sync();
+
+    const int p0 = start[R0 * P.cols + c0], p1 = start[R0 * P.cols + c1 + 1];
+    const bool edge_rows = P.periodic && (R0 == 0 || R0 == P.rows - 1);
+    const bool edge_cols = P.periodic && (c0 == 0 || c1 == P.cols - 1);
+    for (int base = p0; base < p1; base += 64) {
+        const int s = base + lane;
+        const bool valid = s < p1;
+        const int sv = valid ? s : p0;
+        const int c = scell[sv];
+        const int col = c - R0 * P.cols;
+        const int iu_self = col - c0 + 1;
+        const int me = off[kSegCols + iu_self] + (sv - start[c]);
+        const double2 pi = pin[sv];
+        PairAcc<POT> acc;
+        if constexpr (POT == POT_RINGS) {
+            acc.id_i = R.ids[sv];
+            acc.ring_i = ring_of((uint32_t)acc.id_i, R);
+            acc.type_i = P.typed ? R.rtype[acc.ring_i] : 0;
+        }
+        const bool wrap = edge_rows || edge_cols;
+        if (staged) {
+            if (valid) {
+                const int32_t *sid = s_sid[w];
+                const bool over = seg_accumulate<POT, LISTED>(P, R, acc, stg, off, iu_self, me, pi.x, pi.y, wrap,
+                                                             list2, s_lst[w], lane, sid);
+                if (over) {  // more in-range candidates than the list holds: evaluate all
+                    acc.fx = 0.0;
+                    acc.fy = 0.0;
+                    seg_accumulate<POT, false>(P, R, acc, stg, off, iu_self, me, pi.x, pi.y, wrap, list2, s_lst[w],
+                                               lane, sid);
+                }
+            }
+        } else if (valid) {
+            // staging overflow (unusually dense region): read candidates from global memory
+            for (int dd = -1; dd <= 1; dd++) {
+                int r = R0 + dd;
+                if (P.periodic) r = (r < 0) ? r + P.rows : (r >= P.rows ? r - P.rows : r);
+                else if (r < 0 || r >= P.rows) continue;
+                for (int du = -1; du <= 1; du++) {
+                    int u = col + du;
+                    if (P.periodic) u = (u < 0) ? u + P.cols : (u >= P.cols ? u - P.cols : u);
+                    else if (u < 0 || u >= P.cols) continue;
+                    const int cell = r * P.cols + u;
+                    scan_range<POT>(P, R, acc, pin, s, pi.x, pi.y, wrap, start[cell], start[cell + 1]);
+                }
+            }
+        }
+        if (!valid) continue;
+        const double fx = acc.fx, fy = acc.fy;
+        if constexpr (MODE == MODE_FORCES) {
+            Fout[s] = make_double2(fx, fy);
+        } else if constexpr (MODE == MODE_VERLET1) {
+            const double2 v = Vst[s];
+            P2[s] = make_double2(pi.x + (v.x * P.dt + fx * P.term), pi.y + (v.y * P.dt + fy * P.term));
+            F1[s] = make_double2(fx, fy);
+        } else {
+            const double2 v = Vst[s];
+            const double2 fo = F1[s];
+            double vx = v.x + P.hdt * (fx + fo.x);
+            double vy = v.y + P.hdt * (fy + fo.y);
+            double x = pi.x, y = pi.y;
+            if (P.periodic) {
+                const double dxc = x - (P.blx + P.halfL), dyc = y - (P.bly + P.halfH);
+                const bool ox = fabs(dxc) > P.halfL, oy = fabs(dyc) > P.halfH;
+                if (ox) x = x - copysign(P.halfL * 2.0, dxc);
+                if (oy) y = y - copysign(P.halfH * 2.0, dyc);
+            } else {
+                const double rx = x - P.blx, ry = y - P.bly;
+                const bool ox = ((rx + P.radius) > P.L) || ((rx - P.radius) < 0.0);
+                const bool oy = ((ry + P.radius) > P.H) || ((ry - P.radius) < 0.0);
+                if (ox) vx = -vx;
+                if (oy) vy = -vy;
+            }
+            Pst[s] = make_double2(x, y);
+            Vst[s] = make_double2(vx, vy);
+            Fout[s] = make_double2(fx, fy);
+        }
     }
 }
 
@@ -458,57 +851,63 @@ __device__ double philox_normal(uint64_t seed, uint64_t step, uint32_t ring) {
         k0 += 0x9E3779B9u;
         k1 += 0xBB67AE85u;
     }
-    const double u1 = ((double)(((uint64_t)c0 << 21) ^ (uint64_t)c1 >> 11) + 1.0) * 0x1p-53;  // (0,1]
+    const double u1 = ((double)(((uint64_t)c0 << 21) ^ ((uint64_t)c1 >> 11)) + 1.0) * 0x1p-53;  // (0,1]
     const double u2 = (double)((((uint64_t)c2 << 21) ^ ((uint64_t)c3 >> 11)) & ((1ull << 53) - 1)) * 0x1p-53;
     return sqrt(-2.0 * log(u1)) * cospi(2.0 * u2);
 }
 
+// Unwrap (update_continuos_pos!, :118-136), centre of mass and shoelace area
+// of ring r from positions x(i) (calc_area, :103-116).
+template <typename PosFn>
+__device__ __forceinline__ double ring_geometry(const SysParams &P, int np, PosFn x, double2 *__restrict__ c,
+                                                double2 *cm) {
+    for (int i = 0; i < P.nmax; i++) c[i] = x(i);
+    if (P.periodic) {
+        for (int i = 1; i < np; i++) {
+            const double2 d = cdiff(P, x(i), x(i - 1));
+            c[i] = make_double2(c[i - 1].x + d.x, c[i - 1].y + d.y);
+        }
+    }
+    double sx = 0.0, sy = 0.0;
+    for (int i = 0; i < np; i++) {
+        sx += c[i].x;
+        sy += c[i].y;
+    }
+    *cm = make_double2(sx / (double)np, sy / (double)np);
+    double area = 0.0;
+    for (int i = 0; i < np - 1; i++) area += c[i].x * c[i + 1].y - c[i].y * c[i + 1].x;
+    area += c[np - 1].x * c[0].y - c[np - 1].y * c[0].x;
+    return area / 2.0;
+}
+
 template <bool UPDATE>
 __global__ void k_rings(SysParams P, const int32_t *__restrict__ rtype, const RingType *__restrict__ types,
-                        double2 *__restrict__ pos, double2 *__restrict__ force, double *__restrict__ pol,
-                        double *__restrict__ areas, double2 *__restrict__ cms, double2 *__restrict__ cms_next,
-                        double2 *__restrict__ cpos, const double *__restrict__ noise, uint64_t seed,
-                        uint64_t step_index) {
+                        const int32_t *__restrict__ slot_of, double2 *__restrict__ Pst, double2 *__restrict__ F,
+                        double *__restrict__ pol, double *__restrict__ areas, double2 *__restrict__ cms,
+                        double2 *__restrict__ cms_next, double2 *__restrict__ cpos, const double *__restrict__ noise,
+                        uint64_t seed, const unsigned long long *__restrict__ counter) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= P.nrings) return;
     const RingType T = types[P.typed ? rtype[r] : 0];
     const int np = T.np;
-    const int64_t base = (int64_t)r * P.nmax;
-    double2 *x = pos + base;
-    double2 *F = force + base;
-    double2 *c = cpos + base;
+    const int32_t *slot = slot_of + (int64_t)r * P.nmax;
+    auto X = [&](int i) { return Pst[slot[i]]; };
+    double2 *c = cpos + (int64_t)r * P.nmax;
 
     // update_cms! (src/rings/integration.jl:366-372) reads the continuous
     // positions of the previous update_continuos_pos!
     if constexpr (UPDATE) cms[r] = cms_next[r];
-
-    // update_continuos_pos!, :118-136 (copies every slot, unwraps the ring)
-    for (int i = 0; i < P.nmax; i++) c[i] = x[i];
-    if (P.periodic) {
-        for (int i = 1; i < np; i++) {
-            const double2 d = cdiff(P, x[i], x[i - 1]);
-            c[i] = make_double2(c[i - 1].x + d.x, c[i - 1].y + d.y);
-        }
-    }
-    {
-        double sx = 0.0, sy = 0.0;
-        for (int i = 0; i < np; i++) {
-            sx += c[i].x;
-            sy += c[i].y;
-        }
-        cms_next[r] = make_double2(sx / (double)np, sy / (double)np);
-    }
-    // calc_area, :103-116
-    double area = 0.0;
-    for (int i = 0; i < np - 1; i++) area += c[i].x * c[i + 1].y - c[i].y * c[i + 1].x;
-    area += c[np - 1].x * c[0].y - c[np - 1].y * c[0].x;
-    area = area / 2.0;
+    double2 cm;
+    // slots past np (padding of typed rings) keep their stored positions
+    const double area = ring_geometry(
+        P, np, [&](int i) { return i < np ? X(i) : c[i]; }, c, &cm);
+    cms_next[r] = cm;
     areas[r] = area;
 
     // springs (forces!, :205-223) in the reference's accumulation order:
     // particle 1 gets +f_1 then -f_n, particle i > 1 gets -f_{i-1} then +f_i.
     auto spring = [&](int a, int b) {
-        const double2 d = cdiff(P, x[a], x[b]);
+        const double2 d = cdiff(P, X(a), X(b));
         const double dist = sqrt(d.x * d.x + d.y * d.y);
         const double fmod = -T.kspring * (dist - T.lspring);
         const double q = fmod / dist;
@@ -521,7 +920,7 @@ __global__ void k_rings(SysParams P, const int32_t *__restrict__ rtype, const Ri
     double2 f_prev = f_last;
     for (int i = 0; i < np; i++) {
         const double2 fi = (i == np - 1) ? f_last : spring(i, i + 1);
-        double2 Fi = F[i];
+        double2 Fi = F[slot[i]];
         if (i == 0) {
             Fi.x += fi.x;
             Fi.y += fi.y;
@@ -534,17 +933,17 @@ __global__ void k_rings(SysParams P, const int32_t *__restrict__ rtype, const Ri
             Fi.y += fi.y;
         }
         f_prev = fi;
-        F[i] = Fi;
+        F[slot[i]] = Fi;
     }
     // area_forces!, :157-194 (after all springs, as in the reference)
     for (int i = 0; i < np; i++) {
         const int i1 = (i == 0) ? np - 1 : i - 1;
         const int i2 = (i == np - 1) ? 0 : i + 1;
-        const double2 d = cdiff(P, x[i2], x[i1]);
-        double2 Fi = F[i];
+        const double2 d = cdiff(P, X(i2), X(i1));
+        double2 Fi = F[slot[i]];
         Fi.x -= fa * (d.y / 2.0);
         Fi.y -= fa * (-d.x / 2.0);
-        F[i] = Fi;
+        F[slot[i]] = Fi;
     }
     if constexpr (!UPDATE) return;
 
@@ -554,19 +953,20 @@ __global__ void k_rings(SysParams P, const int32_t *__restrict__ rtype, const Ri
     sincos(theta, &sn, &cs);
     double vcx = 0.0, vcy = 0.0;
     for (int i = 0; i < np; i++) {
-        const double2 Fi = F[i];
+        const double2 Fi = F[slot[i]];
         const double vx = T.vo * cs + T.mu * Fi.x;
         const double vy = T.vo * sn + T.mu * Fi.y;
         vcx += vx;
         vcy += vy;
-        double px = x[i].x + vx * P.dt, py = x[i].y + vy * P.dt;
+        const double2 xi = X(i);
+        double px = xi.x + vx * P.dt, py = xi.y + vy * P.dt;
         if (P.periodic) {  // walls! after update!, applied per particle
             const double dxc = px - (P.blx + P.halfL), dyc = py - (P.bly + P.halfH);
             const bool ox = fabs(dxc) > P.halfL, oy = fabs(dyc) > P.halfH;
             if (ox) px = px - copysign(P.halfL * 2.0, dxc);
             if (oy) py = py - copysign(P.halfH * 2.0, dyc);
         }
-        x[i] = make_double2(px, py);
+        Pst[slot[i]] = make_double2(px, py);
     }
     vcx = vcx / (double)np;
     vcy = vcy / (double)np;
@@ -576,36 +976,21 @@ __global__ void k_rings(SysParams P, const int32_t *__restrict__ rtype, const Ri
         cr = (cs * vcy - sn * vcx) / speed;
         if (fabs(cr) > 1.0) cr = cr > 0.0 ? 1.0 : -1.0;
     }
-    const double z = noise ? noise[r] : (T.rot != 0.0 ? philox_normal(seed, step_index, (uint32_t)r) : 0.0);
+    const double z = noise ? noise[r] : (T.rot != 0.0 ? philox_normal(seed, counter[0], (uint32_t)r) : 0.0);
     const double d_theta = 1.0 / T.relax * asin(cr) * P.dt + sqrt(2.0 * T.rot * P.dt) * z;
     pol[r] = theta + d_theta;
 }
 
 __global__ void k_rings_init(SysParams P, const int32_t *__restrict__ rtype, const RingType *__restrict__ types,
-                             const double2 *__restrict__ pos, double2 *__restrict__ cms,
+                             const double2 *__restrict__ pos_id, double2 *__restrict__ cms,
                              double2 *__restrict__ cms_next, double2 *__restrict__ cpos, double *__restrict__ areas) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= P.nrings) return;
     const int np = types[P.typed ? rtype[r] : 0].np;
-    const int64_t base = (int64_t)r * P.nmax;
-    const double2 *x = pos + base;
-    double2 *c = cpos + base;
-    for (int i = 0; i < P.nmax; i++) c[i] = x[i];
-    if (P.periodic)
-        for (int i = 1; i < np; i++) {
-            const double2 d = cdiff(P, x[i], x[i - 1]);
-            c[i] = make_double2(c[i - 1].x + d.x, c[i - 1].y + d.y);
-        }
-    double sx = 0.0, sy = 0.0;
-    for (int i = 0; i < np; i++) {
-        sx += c[i].x;
-        sy += c[i].y;
-    }
-    cms[r] = cms_next[r] = make_double2(sx / (double)np, sy / (double)np);
-    double area = 0.0;
-    for (int i = 0; i < np - 1; i++) area += c[i].x * c[i + 1].y - c[i].y * c[i + 1].x;
-    area += c[np - 1].x * c[0].y - c[np - 1].y * c[0].x;
-    areas[r] = area / 2.0;
+    const double2 *x = pos_id + (int64_t)r * P.nmax;
+    double2 cm;
+    areas[r] = ring_geometry(P, np, [&](int i) { return x[i]; }, cpos + (int64_t)r * P.nmax, &cm);
+    cms[r] = cms_next[r] = cm;
 }
 
 // ---------------------------------------------------------------------------
@@ -631,7 +1016,7 @@ __global__ void k_observables(SysParams P, const double2 *__restrict__ pos, cons
                               int with_pe, double *__restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double ke = 0.0, pe = 0.0;
-    if (i < P.n) {
+    if (i < P.na) {
         if (vel) {
             const double2 v = vel[i];
             ke = v.x * v.x + v.y * v.y;
@@ -639,7 +1024,7 @@ __global__ void k_observables(SysParams P, const double2 *__restrict__ pos, cons
         if (with_pe) {
             const double2 pi = pos[i];
             const double sig = P.p0;
-            for (int j = i + 1; j < P.n; j++) {
+            for (int j = i + 1; j < P.na; j++) {
                 const double2 pj = pos[j];
                 double dx = pi.x - pj.x, dy = pi.y - pj.y;
                 if (P.periodic) {
@@ -665,7 +1050,7 @@ inline int nblk(int64_t n, int t) { return (int)((n + t - 1) / t); }
 
 RingCtx ring_ctx(mavi_sys *s) {
     RingCtx R{};
-    R.perm = s->perm;
+    R.ids = s->ID_[s->cur];
     R.rtype = s->rtype;
     R.types = s->types;
     R.inter = s->inter;
@@ -674,11 +1059,38 @@ RingCtx ring_ctx(mavi_sys *s) {
     return R;
 }
 
+// Squared range of the candidate list: every pair that can interact lies
+// within it (rings: the widest interaction of the type matrix).  Infinite
+// for the reference's LJ without cutoff: every stencil pair interacts.
+double list_range2(const mavi_sys *s) {
+    if (s->P.potential != MAVI_POT_RINGS) return s->P.cut2;
+    return s->list2;
+}
+
 template <int MODE, int POT, bool ALL>
 hipError_t launch_pair_t(mavi_sys *s, const double2 *pin, hipStream_t st) {
     const int wg = s->wg;
-    hipLaunchKernelGGL((k_pair<MODE, POT, ALL>), dim3(nblk(s->P.na, wg)), dim3(wg), 0, st, s->P, ring_ctx(s), pin,
-                       s->scell, s->start, s->perm, s->pos, s->vel, s->force, s->ps2, s->f1);
+    const int c = s->cur;
+    const double l2 = s->two_phase ? list_range2(s) : INFINITY;
+    if constexpr (!ALL) {
+        if (s->lds_tiles) {
+            const int nsegc = (s->P.cols + kSegCells - 1) / kSegCells;
+            const int nseg = nsegc * s->P.rows;
+            const int nb = (nseg + kSegWaves - 1) / kSegWaves;
+            if (l2 < INFINITY)
+                hipLaunchKernelGGL((k_pair_seg<MODE, POT, true>), dim3(nb), dim3(64 * kSegWaves), 0, st, s->P,
+                                   ring_ctx(s), pin, s->scell, s->start, s->P_[c], s->V_[c], s->F, s->P2, s->F1, l2,
+                                   nsegc, nseg);
+            else
+                hipLaunchKernelGGL((k_pair_seg<MODE, POT, false>), dim3(nb), dim3(64 * kSegWaves), 0, st, s->P,
+                                   ring_ctx(s), pin, s->scell, s->start, s->P_[c], s->V_[c], s->F, s->P2, s->F1, l2,
+                                   nsegc, nseg);
+            return hipGetLastError();
+        }
+    }
+    const size_t lds = (l2 < INFINITY) ? sizeof(uint16_t) * kListCap * (size_t)wg : 0;
+    hipLaunchKernelGGL((k_pair<MODE, POT, ALL>), dim3(nblk(s->P.na, wg)), dim3(wg), lds, st, s->P, ring_ctx(s), pin,
+                       s->scell, s->start, s->P_[c], s->V_[c], s->F, s->P2, s->F1, l2);
     return hipGetLastError();
 }
 
@@ -726,39 +1138,54 @@ hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t st) {
     return hipGetLastError();
 }
 
+hipError_t launch_iota_state(mavi_sys *s, hipStream_t st) {
+    const int c = s->cur;
+    const bool has_vel = s->P.potential != MAVI_POT_RINGS;
+    hipLaunchKernelGGL(k_iota_state, dim3(nblk(s->P.na, 256)), dim3(256), 0, st, s->P.na, s->active_ids, s->pos_id,
+                       has_vel ? s->vel_id : nullptr, s->P_[c], has_vel ? s->V_[c] : nullptr, s->ID_[c], s->slot_of);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpermute(mavi_sys *s, bool vel, bool force, hipStream_t st) {
+    const int c = s->cur;
+    const bool has_vel = vel && s->P.potential != MAVI_POT_RINGS;
+    hipLaunchKernelGGL(k_unpermute, dim3(nblk(s->P.na, 256)), dim3(256), 0, st, s->P.na, s->ID_[c], s->P_[c],
+                       has_vel ? s->V_[c] : nullptr, s->F, s->pos_id, has_vel ? s->vel_id : nullptr,
+                       force ? s->force_id : nullptr);
+    return hipGetLastError();
+}
+
 hipError_t launch_bin(mavi_sys *s, hipStream_t st) {
     const SysParams &P = s->P;
-    if (P.cols == 0) {
-        hipLaunchKernelGGL(k_gather_active, dim3(nblk(P.na, 256)), dim3(256), 0, st, P, s->active_ids, s->pos, s->perm,
-                           s->ps);
-        return hipGetLastError();
-    }
+    if (P.cols == 0) return hipSuccess;  // chunks = nothing: the state order stays the active id order
     ProfScope ps(s, MAVI_PROF_BIN, st);
-    hipError_t e;
-    if ((e = launch_fill_i32(s->cnt, 0, (int64_t)P.ncell + 1, st)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_bin_count, dim3(nblk(P.na, 256)), dim3(256), 0, st, P, s->pos, s->active_ids, s->key,
-                       s->rank, s->cnt, s->flags);
+    const int c = s->cur, nc = 1 - c;
+    hipLaunchKernelGGL(k_bin_count, dim3(nblk(P.na, 256)), dim3(256), 0, st, P, s->P_[c], s->key, s->rank, s->cnt,
+                       s->flags);
     const int n = P.ncell + 1;
-    const int nb = nblk(n, kScanTile);
-    hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(kScanThreads), 0, st, s->cnt, s->start, s->bsum, n);
-    hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(kScanThreads), 0, st, s->bsum, nb);
-    hipLaunchKernelGGL(k_scan_add, dim3(nblk(n, 256)), dim3(256), 0, st, s->start, s->bsum, n);
-    hipLaunchKernelGGL(k_scatter, dim3(nblk(P.na, 256)), dim3(256), 0, st, P, s->active_ids, s->key, s->rank,
-                       s->start, s->perm);
-    hipLaunchKernelGGL(k_cell_finalize, dim3(nblk(P.ncell, 256)), dim3(256), 0, st, P, s->start, s->perm, s->pos,
-                       s->ps, s->scell, s->flags);
+    hipLaunchKernelGGL(k_scan, dim3(nblk(n, kScanTile)), dim3(kScanThreads), 0, st, s->cnt, s->start,
+                       s->scan_status, s->cnt + P.ncell + 1, n, s->flags);
+    const int grid = nblk(P.na > n ? P.na : n, 256);
+    hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(256), 0, st, P, s->key, s->rank, s->start, s->src, s->scell,
+                       s->cnt, s->scan_status, s->nscan_blocks, s->cnt + P.ncell + 1, s->counter);
+    const bool has_vel = P.potential != MAVI_POT_RINGS;
+    hipLaunchKernelGGL(k_finalize, dim3(nblk(P.na, 256)), dim3(256), 0, st, P, s->start, s->src, s->scell, s->P_[c],
+                       has_vel ? s->V_[c] : nullptr, s->ID_[c], s->P_[nc], has_vel ? s->V_[nc] : nullptr, s->ID_[nc],
+                       s->slot_of, s->flags);
+    s->cur = nc;
+    s->sorted = true;
     return hipGetLastError();
 }
 
 hipError_t launch_export_chunks(mavi_sys *s, int64_t *d_cp, int64_t *d_counts, hipStream_t st) {
-    hipLaunchKernelGGL(k_export_chunks, dim3(nblk(s->P.ncell, 256)), dim3(256), 0, st, s->P, s->start, s->perm,
-                       d_cp, d_counts);
+    hipLaunchKernelGGL(k_export_chunks, dim3(nblk(s->P.ncell, 256)), dim3(256), 0, st, s->P, s->start,
+                       s->ID_[s->cur], d_cp, d_counts);
     return hipGetLastError();
 }
 
 hipError_t launch_forces_only(mavi_sys *s, hipStream_t st) {
     ProfScope ps(s, MAVI_PROF_FORCE, st);
-    return launch_pair<MODE_FORCES>(s, s->ps, st);
+    return launch_pair<MODE_FORCES>(s, s->P_[s->cur], st);
 }
 
 hipError_t launch_verlet_step(mavi_sys *s, hipStream_t st) {
@@ -766,10 +1193,24 @@ hipError_t launch_verlet_step(mavi_sys *s, hipStream_t st) {
     if (e != hipSuccess) return e;
     {
         ProfScope ps(s, MAVI_PROF_FORCE, st);
-        if ((e = launch_pair<MODE_VERLET1>(s, s->ps, st)) != hipSuccess) return e;
+        if ((e = launch_pair<MODE_VERLET1>(s, s->P_[s->cur], st)) != hipSuccess) return e;
     }
     ProfScope ps(s, MAVI_PROF_FORCE, st);
-    return launch_pair<MODE_VERLET2>(s, s->ps2, st);
+    return launch_pair<MODE_VERLET2>(s, s->P2, st);
+}
+
+static hipError_t launch_rings_kernel(mavi_sys *s, bool update, const double *noise_dev, hipStream_t st) {
+    ProfScope ps(s, MAVI_PROF_RINGS, st);
+    const int c = s->cur;
+    if (update)
+        hipLaunchKernelGGL(k_rings<true>, dim3(nblk(s->P.nrings, 64)), dim3(64), 0, st, s->P, s->rtype, s->types,
+                           s->slot_of, s->P_[c], s->F, s->pol, s->areas, s->cms, s->cms_next, s->cpos, noise_dev,
+                           s->seed, s->counter);
+    else
+        hipLaunchKernelGGL(k_rings<false>, dim3(nblk(s->P.nrings, 64)), dim3(64), 0, st, s->P, s->rtype, s->types,
+                           s->slot_of, s->P_[c], s->F, s->pol, s->areas, s->cms, s->cms_next, s->cpos, nullptr,
+                           s->seed, s->counter);
+    return hipGetLastError();
 }
 
 hipError_t launch_rings_forces(mavi_sys *s, hipStream_t st) {
@@ -777,38 +1218,35 @@ hipError_t launch_rings_forces(mavi_sys *s, hipStream_t st) {
     if (e != hipSuccess) return e;
     {
         ProfScope ps(s, MAVI_PROF_FORCE, st);
-        if ((e = launch_pair<MODE_FORCES>(s, s->ps, st)) != hipSuccess) return e;
+        if ((e = launch_pair<MODE_FORCES>(s, s->P_[s->cur], st)) != hipSuccess) return e;
     }
-    ProfScope ps(s, MAVI_PROF_RINGS, st);
-    hipLaunchKernelGGL(k_rings<false>, dim3(nblk(s->P.nrings, 64)), dim3(64), 0, st, s->P, s->rtype, s->types,
-                       s->pos, s->force, s->pol, s->areas, s->cms, s->cms_next, s->cpos, nullptr, s->seed,
-                       (uint64_t)0);
-    return hipGetLastError();
+    return launch_rings_kernel(s, false, nullptr, st);
 }
 
-hipError_t launch_rings_step(mavi_sys *s, const double *noise_dev, uint64_t step_index, hipStream_t st) {
+hipError_t launch_rings_step(mavi_sys *s, const double *noise_dev, hipStream_t st) {
     hipError_t e = launch_bin(s, st);
     if (e != hipSuccess) return e;
+    if (s->P.cols == 0) {  // no binning kernel to advance the noise step counter
+        hipLaunchKernelGGL(k_counter_inc, dim3(1), dim3(64), 0, st, s->counter);
+    }
     {
         ProfScope ps(s, MAVI_PROF_FORCE, st);
-        if ((e = launch_pair<MODE_FORCES>(s, s->ps, st)) != hipSuccess) return e;
+        if ((e = launch_pair<MODE_FORCES>(s, s->P_[s->cur], st)) != hipSuccess) return e;
     }
-    ProfScope ps(s, MAVI_PROF_RINGS, st);
-    hipLaunchKernelGGL(k_rings<true>, dim3(nblk(s->P.nrings, 64)), dim3(64), 0, st, s->P, s->rtype, s->types, s->pos,
-                       s->force, s->pol, s->areas, s->cms, s->cms_next, s->cpos, noise_dev, s->seed, step_index);
-    return hipGetLastError();
+    return launch_rings_kernel(s, true, noise_dev, st);
 }
 
-hipError_t launch_rings_info(mavi_sys *s, hipStream_t st) {
-    hipLaunchKernelGGL(k_rings_init, dim3(nblk(s->P.nrings, 64)), dim3(64), 0, st, s->P, s->rtype, s->types, s->pos,
-                       s->cms, s->cms_next, s->cpos, s->areas);
+hipError_t launch_rings_init(mavi_sys *s, hipStream_t st) {
+    hipLaunchKernelGGL(k_rings_init, dim3(nblk(s->P.nrings, 64)), dim3(64), 0, st, s->P, s->rtype, s->types,
+                       s->pos_id, s->cms, s->cms_next, s->cpos, s->areas);
     return hipGetLastError();
 }
 
 hipError_t launch_observables(mavi_sys *s, double *d_out, hipStream_t st) {
     const int with_pe = (s->P.potential == MAVI_POT_LJ) ? 1 : 0;
-    hipLaunchKernelGGL(k_observables, dim3(nblk(s->P.n, kObsThreads)), dim3(kObsThreads), 0, st, s->P, s->pos,
-                       s->vel, with_pe, d_out);
+    const bool has_vel = s->P.potential != MAVI_POT_RINGS;
+    hipLaunchKernelGGL(k_observables, dim3(nblk(s->P.na, kObsThreads)), dim3(kObsThreads), 0, st, s->P,
+                       s->P_[s->cur], has_vel ? s->V_[s->cur] : nullptr, with_pe, d_out);
     return hipGetLastError();
 }
 

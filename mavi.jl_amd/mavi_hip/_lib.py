@@ -113,7 +113,13 @@ SIGNATURES = {
     "mavi_prof_enable": (C.c_int, [_P, C.c_int]),
     "mavi_prof_get": (C.c_int, [_P, C.c_int, _DP, _I64P]),
     "mavi_sys_set_tile": (C.c_int, [_P, C.c_int32]),
+    "mavi_sys_set_option": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "mavi_autotune": (C.c_int, [_P, C.c_int32, _DP, C.POINTER(C.c_int32)]),
 }
+
+MAVI_OPT_TWO_PHASE = 0
+MAVI_OPT_GRAPHS = 1
+MAVI_OPT_LDS_TILES = 2
 
 _lib = None
 
@@ -285,6 +291,16 @@ class DeviceSystem:
 
     def set_tile(self, wg: int):
         self._call("mavi_sys_set_tile", C.c_int32(int(wg)))
+
+    def autotune(self, trial_steps: int = 20):
+        """Pick the fastest pair-kernel shape for this system (state restored)."""
+        best = C.c_double()
+        shape = (C.c_int32 * 3)()
+        self._call("mavi_autotune", C.c_int32(int(trial_steps)), C.byref(best), shape)
+        return best.value, {"workgroup": shape[0], "lds_tiles": shape[1], "two_phase": shape[2]}
+
+    def set_option(self, option: int, value: int):
+        self._call("mavi_sys_set_option", C.c_int32(int(option)), C.c_int32(int(value)))
 
     def __del__(self):
         try:

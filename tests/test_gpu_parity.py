@@ -296,3 +296,46 @@ def test_full_size_force_sample_vs_oracle(gpu_ctx, oracle_mod):
     e = force_rel_err(f, f_ref, ab)
     say(f"  rel err {e:.3e}")
     assert e <= FORCE_RTOL
+
+
+SHAPES = [(256, 0, 0), (128, 0, 1), (256, 1, 0), (256, 1, 1), (64, 0, 0), (512, 0, 1)]
+
+
+@pytest.mark.parametrize("name", ["lj_gas_1024_ref", "lj_periodic_small_cut", "ht_rigid", "ht_periodic"])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_kernel_shapes_parity(gpu_ctx, oracle_mod, name, shape):
+    """Every pair-kernel shape the autotuner can pick (workgroup, LDS tiles,
+    two-phase list) meets the force and 100-step trajectory bars."""
+    from mavi_hip import _lib
+
+    wg, lds, tp = shape
+    say(f"[shape] {name} wg={wg} lds={lds} two_phase={tp}")
+    pos, vel, space, dyn, ic = plain_case(name)
+    sys_ = make_plain(pos, vel, space, dyn, ic)
+    sys_.dev.set_tile(wg)
+    sys_.dev.set_option(_lib.MAVI_OPT_LDS_TILES, lds)
+    sys_.dev.set_option(_lib.MAVI_OPT_TWO_PHASE, tp)
+    o = oracle_mod.from_system(len(pos), space, dyn, ic)
+    f_ref, ab = o.forces(pos, with_absum=True)
+    assert force_rel_err(sys_.calc_forces(), f_ref, ab) <= FORCE_RTOL
+    sys_.advance(100)
+    sys_.download()
+    p_ref, v_ref, _, _, _ = o.newton_steps(pos, vel, 100)
+    ep, ev = traj_err(sys_.state.pos, sys_.state.vel, p_ref, v_ref, space.geometry_cfg.length)
+    assert ep <= TRAJ_RTOL and ev <= TRAJ_RTOL
+
+
+def test_autotune_restores_state(gpu_ctx, oracle_mod):
+    say("[autotune]")
+    pos, vel, space, dyn, ic = plain_case("lj_periodic_small_cut")
+    sys_ = make_plain(pos, vel, space, dyn, ic)
+    best, shape = sys_.dev.autotune(10)
+    assert best > 0 and shape["workgroup"] in (64, 128, 256, 512)
+    sys_.download()
+    assert np.array_equal(sys_.state.pos, pos) and np.array_equal(sys_.state.vel, vel)
+    sys_.advance(50)
+    sys_.download()
+    o = oracle_mod.from_system(len(pos), space, dyn, ic)
+    p_ref, v_ref, _, _, _ = o.newton_steps(pos, vel, 50)
+    ep, ev = traj_err(sys_.state.pos, sys_.state.vel, p_ref, v_ref, space.geometry_cfg.length)
+    assert ep <= TRAJ_RTOL and ev <= TRAJ_RTOL

@@ -181,3 +181,25 @@ def test_rings_chunks_equal_nochunks(gpu_ctx):
     b.download()
     d = float(((a.state.pos - b.state.pos) ** 2).sum())
     assert d < 1e-4
+
+
+@pytest.mark.parametrize("shape", [(256, 1, 0), (256, 1, 1), (128, 0, 1)])
+@pytest.mark.parametrize("name", ["normal", "types"])
+def test_rings_kernel_shapes(gpu_ctx, oracle_mod, name, shape):
+    from mavi_hip import _lib
+
+    wg, lds, tp = shape
+    say(f"[rings shape] {name} {shape}")
+    sys_, pos0, pol0, (state, space, cfg, ic) = build(name)
+    sys_.dev.set_tile(wg)
+    sys_.dev.set_option(_lib.MAVI_OPT_LDS_TILES, lds)
+    sys_.dev.set_option(_lib.MAVI_OPT_TWO_PHASE, tp)
+    rng = np.random.default_rng(99)
+    noise = rng.standard_normal((50, state.num_rings))
+    sys_.advance(50, noise)
+    sys_.download()
+    o = oracle_mod.from_rings(state, space, cfg, ic)
+    p_ref, pol_ref, _, _, _ = o.rings_steps(pos0, pol0, 50, noise)
+    ids = state.particle_ids()
+    ep, _ = traj_err(sys_.state.pos[ids], None, p_ref[ids], None, space.geometry_cfg.length)
+    assert ep <= TRAJ_RTOL and float(np.abs(sys_.state.pol - pol_ref).max()) <= 1e-9
