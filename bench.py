@@ -64,6 +64,45 @@ def cpu_baseline(pos, vel, space, dyn, ic, steps: int, threads: int):
     return len(pos) * steps / dt, dt
 
 
+def bench_rings(args):
+    """Active-ring mixture (40,000 rings x 20 particles): Rings step! with
+    device rotational noise, timed like the plain path."""
+    import mavi_hip as mv
+    from mavi_hip import rings, workloads as wl
+
+    st, sp, cfg, ic = wl.rings_mixture(nrings=args.rings)
+    pos0, pol0 = st.pos.copy(), st.pol.copy()
+    n = int(len(st.pos))
+    log(f"[bench] rings: {args.rings} rings x 20 = {n} particles, chunks {ic.chunks_cfg.num_cols}^2")
+    system = rings.RingsSystem(state=st, space_cfg=sp, dynamic_cfg=cfg, int_cfg=ic, seed=2)
+    dev = system.dev
+    dev.step(args.warmup)
+    t0 = time.perf_counter()
+    dev.step(args.rings_steps)
+    wall = time.perf_counter() - t0
+    dev.prof_enable(True)
+    dev.step(20)
+    prof = {name: dev.prof_get(k) for k, name in ((0, "bin"), (1, "pair"), (2, "ring"))}
+    dev.prof_enable(False)
+    out = {"workload": f"active rings binary mixture, {args.rings} rings x 20 particles, periodic, ring packing "
+                       "fraction 0.8, springs k=20 l=1, HarmTrunc repulsion k=10 radius 1, overdamped v0=1 "
+                       "D_r=0.1, dt 1e-3, device Philox noise",
+           "particles": n, "steps": args.rings_steps, "value": n * args.rings_steps / wall,
+           "unit": "particle-steps/s", "ms_per_step": wall / args.rings_steps * 1e3,
+           "kernel_avg_us": {k: v[0] / max(v[1], 1) * 1e3 for k, v in prof.items()}}
+    if not args.no_cpu:
+        import oracle
+
+        o = oracle.from_rings(st, sp, cfg, ic)
+        t0 = time.perf_counter()
+        o.rings_steps(pos0, pol0, 2)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": n * 2 / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+                               "sample": f"{n} particles x 2 Rings step! on one core ({dt:.1f} s)"}
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    return out
+
+
 def host_cores() -> int:
     try:
         n = len(os.sched_getaffinity(0))
@@ -81,6 +120,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=4, help="steps of the CPU sample (all host cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--prof-steps", type=int, default=100)
+    ap.add_argument("--rings", type=int, default=40000, help="rings in the rings form (0 = skip)")
+    ap.add_argument("--rings-steps", type=int, default=500)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported in this build "
@@ -155,6 +196,12 @@ def main():
     }
     if cpu:
         out["speedup_vs_cpu"] = value / cpu["value"]
+    forms = {}
+    if args.rings > 0:
+        del system, dev
+        forms["rings"] = bench_rings(args)
+    if forms:
+        out["forms"] = forms
     print(json.dumps(out), flush=True)
 
 

@@ -120,7 +120,7 @@ int mavi_sys_destroy(mavi_sys *s) {
     }
     dfree(s->F); dfree(s->P2); dfree(s->F1);
     dfree(s->key); dfree(s->rank); dfree(s->src); dfree(s->scell); dfree(s->slot_of);
-    dfree(s->cnt); dfree(s->start); dfree(s->scan_status); dfree(s->flags); dfree(s->counter);
+    dfree(s->cnt); dfree(s->start); dfree(s->tile_sum); dfree(s->flags); dfree(s->counter);
     dfree(s->active_ids); dfree(s->pos_id); dfree(s->vel_id); dfree(s->force_id);
     dfree(s->pol); dfree(s->areas); dfree(s->cms); dfree(s->cms_next); dfree(s->cpos);
     dfree(s->rtype); dfree(s->types); dfree(s->inter); dfree(s->noise); dfree(s->red);
@@ -271,10 +271,10 @@ int mavi_sys_create(mavi_ctx *ctx, const mavi_sys_desc *d, mavi_sys **out) {
     ALLOC(s->slot_of, P.n);
     ALLOC(s->flags, 1);
     ALLOC(s->counter, 1);
-    ALLOC(s->cnt, (int64_t)P.ncell + 2);  // + the scan's tile counter
+    ALLOC(s->cnt, (int64_t)P.ncell + 1);
     ALLOC(s->start, (int64_t)P.ncell + 1);
-    s->nscan_blocks = (P.ncell + 1 + 1023) / 1024;
-    ALLOC(s->scan_status, s->nscan_blocks);
+    s->nscan_blocks = (P.ncell + 1 + 4095) / 4096;
+    ALLOC(s->tile_sum, s->nscan_blocks);
     ALLOC(s->pos_id, P.n);
     if (!rings) ALLOC(s->vel_id, P.n);
     ALLOC(s->force_id, P.n);

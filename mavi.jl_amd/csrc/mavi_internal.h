@@ -13,6 +13,7 @@
 //   src, scell   int32[na]     new slot -> old slot / chunk of new slot
 //   slot_of      int32[n]      particle id -> sorted slot (rings)
 //   cnt, start   int32[ncell+1] particles per chunk / exclusive scan
+//   tile_sum     int32[ntiles]  scan tile sums (4096 chunks per tile)
 //   pos_id, vel_id, force_id   double2[n] id-order staging for the host
 // na = number of active ids (get_particles_ids); the chunk index is
 // row0*cols + col0 with row0 = reference row - 1 (row 1 = top of the box,
@@ -29,7 +30,6 @@
 #define MAVI_FLAG_OVERFLOW 1u
 #define MAVI_FLAG_OUT_OF_BOX 2u
 #define MAVI_FLAG_NONFINITE 4u
-#define MAVI_FLAG_SCAN_TIMEOUT 8u
 
 struct mavi_ctx {
     int device = 0;
@@ -92,7 +92,7 @@ struct mavi_sys {
     double2 *F = nullptr, *P2 = nullptr, *F1 = nullptr;
     int32_t *key = nullptr, *rank = nullptr, *src = nullptr, *scell = nullptr, *slot_of = nullptr;
     int32_t *cnt = nullptr, *start = nullptr;
-    unsigned long long *scan_status = nullptr;
+    int32_t *tile_sum = nullptr;            // scan tile sums
     uint32_t *flags = nullptr;
     unsigned long long *counter = nullptr;  // device step counter (rotational noise)
     int32_t *active_ids = nullptr;          // na ids ascending

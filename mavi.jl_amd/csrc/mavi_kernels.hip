@@ -1,7 +1,7 @@
 // mavi_kernels.hip -- gfx950 kernels of the step hot path.
 //
-//   K1 binning      k_bin_count (wave-aggregated chunk counts) -> k_scan
-//                   (single-pass look-back scan) -> k_scatter -> k_finalize
+//   K1 binning      k_bin_count (wave-aggregated chunk counts) -> k_tile_*
+//                   (reduce-then-scan) -> k_scatter -> k_finalize
 //                   (ids ascending inside a chunk, as update_chunks! appends
 //                   them, src/chunks.jl:150-161; the state is permuted into
 //                   the new chunk order)
@@ -13,9 +13,10 @@
 //                   >= 3x3 (src/integration.jl:112-157).
 //   K3 Verlet+walls MODE_VERLET1/2 epilogues of k_pair (src/integration.jl:
 //                   415-431 with walls! :271-285 / :309-324).
-//   K4 rings        k_rings<UPDATE>: per-ring unwrap, shoelace area, springs,
-//                   area force and overdamped update
-//                   (src/rings/integration.jl:79-351).
+//   K4 rings        k_rings_lanes<UPDATE> (one ring per 32 lanes, rings of
+//                   up to 32 particles) and k_rings<UPDATE> (one ring per
+//                   thread, any size): unwrap, shoelace area, springs, area
+//                   force and overdamped update (src/rings/integration.jl:79-351).
 //   K7 observables  k_observables (src/quantities.jl:12-18, 46-66).
 //
 // The library is compiled with -ffp-contract=off: a*b+c is only fused where
@@ -151,10 +152,6 @@ __global__ void k_bin_count(SysParams P, const double2 *__restrict__ pos, int32_
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 4;
 constexpr int kScanTile = kScanThreads * kScanItems;
-constexpr unsigned long long kStA = 1ull << 62;  // tile aggregate published
-constexpr unsigned long long kStP = 2ull << 62;  // inclusive prefix published
-constexpr unsigned long long kStMask = 0xffffffffull;
-constexpr int kSpinLimit = 1 << 22;
 
 // Block-wide exclusive scan of one int per thread (256 threads, wave64).
 __device__ __forceinline__ int32_t block_excl_scan(int32_t v, int32_t *total) {
@@ -178,88 +175,66 @@ __device__ __forceinline__ int32_t block_excl_scan(int32_t v, int32_t *total) {
     return woff + incl - v;
 }
 
-// Single-pass exclusive scan of cnt[0..n) with decoupled look-back.  Tiles
-// are workgroups in dispatch order (workgroups start in increasing id order,
-// so a tile only waits on tiles that have started); the wait is bounded
-// (kSpinLimit) and raises MAVI_FLAG_SCAN_TIMEOUT instead of spinning forever.
-__global__ void k_scan(const int32_t *__restrict__ in, int32_t *__restrict__ out,
-                       unsigned long long *__restrict__ status, int32_t *__restrict__ tile_counter, int32_t n,
-                       uint32_t *__restrict__ flags) {
-    __shared__ int32_t s_prefix;
-    const int b = blockIdx.x;
-    (void)tile_counter;
-    const int base = b * kScanTile + threadIdx.x * kScanItems;
-    int32_t v[kScanItems];
+// Exclusive scan of cnt[0..n) in three passes with a fixed cost: tile sums,
+// a one-workgroup scan of the tile sums, then each tile's local scan plus
+// its offset.  Tiles hold kScanTile2 = 4096 counts.
+constexpr int kScanItems2 = 16;
+constexpr int kScanTile2 = kScanThreads * kScanItems2;
+
+__global__ void k_tile_reduce(const int32_t *__restrict__ in, int32_t n, int32_t *__restrict__ tile_sum) {
+    const int base = blockIdx.x * kScanTile2;
     int32_t sum = 0;
 #pragma unroll
-    for (int k = 0; k < kScanItems; k++) {
+    for (int k = 0; k < kScanItems2; k++) {
+        const int i = base + k * kScanThreads + threadIdx.x;  // coalesced
+        sum += (i < n) ? in[i] : 0;
+    }
+    int32_t total;
+    (void)block_excl_scan(sum, &total);
+    if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
+}
+
+__global__ void k_tile_scan(int32_t *__restrict__ tile_sum, int32_t ntiles) {
+    __shared__ int32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int t0 = 0; t0 < ntiles; t0 += kScanThreads) {
+        const int i = t0 + threadIdx.x;
+        const int32_t v = (i < ntiles) ? tile_sum[i] : 0;
+        int32_t total;
+        const int32_t ex = block_excl_scan(v, &total);
+        const int32_t c = carry;
+        if (i < ntiles) tile_sum[i] = ex + c;
+        __syncthreads();
+        if (threadIdx.x == 0) carry = c + total;
+        __syncthreads();
+    }
+}
+
+__global__ void k_tile_apply(const int32_t *__restrict__ in, int32_t n, const int32_t *__restrict__ tile_sum,
+                             int32_t *__restrict__ out) {
+    const int base = blockIdx.x * kScanTile2 + threadIdx.x * kScanItems2;
+    int32_t v[kScanItems2];
+    int32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems2; k++) {
         v[k] = (base + k < n) ? in[base + k] : 0;
         sum += v[k];
     }
     int32_t total;
-    int32_t off = block_excl_scan(sum, &total);
-    if (threadIdx.x < 64) {
-        // first wave: publish the tile aggregate, then look back 64 tiles at a
-        // time for the nearest tile with a published inclusive prefix
-        const int lane = threadIdx.x;
-        int32_t exclusive = 0;
-        if (b == 0) {
-            if (lane == 0)
-                __hip_atomic_store(&status[0], kStP | (unsigned long long)(uint32_t)total, __ATOMIC_RELEASE,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            if (lane == 0)
-                __hip_atomic_store(&status[b], kStA | (unsigned long long)(uint32_t)total, __ATOMIC_RELEASE,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-            int window_end = b - 1;
-            int spins = 0;
-            while (true) {
-                const int p = window_end - lane;
-                // tiles before tile 0 read as a published prefix of 0
-                const unsigned long long st =
-                    (p >= 0) ? __hip_atomic_load(&status[p], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) : kStP;
-                const unsigned long long fl = st & (3ull << 62);
-                const unsigned long long pmask = __ballot(fl == kStP);
-                const unsigned long long zmask = __ballot(fl == 0);
-                const int firstP = pmask ? __ffsll(pmask) - 1 : 64;
-                const unsigned long long need = (firstP >= 63) ? ~0ull : ((2ull << firstP) - 1ull);
-                if (zmask & need) {  // a tile in front of the nearest prefix has not published yet
-                    if (++spins > kSpinLimit) {
-                        if (lane == 0) atomicOr(flags, MAVI_FLAG_SCAN_TIMEOUT);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                int32_t val = (lane <= firstP) ? (int32_t)(st & kStMask) : 0;
+    int32_t off = block_excl_scan(sum, &total) + tile_sum[blockIdx.x];
 #pragma unroll
-                for (int o = 32; o > 0; o >>= 1) val += __shfl_xor(val, o, 64);
-                exclusive += val;
-                if (firstP < 64) break;
-                window_end -= 64;
-            }
-            if (lane == 0)
-                __hip_atomic_store(&status[b], kStP | (unsigned long long)(uint32_t)(exclusive + total),
-                                   __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        if (lane == 0) s_prefix = exclusive;
-    }
-    __syncthreads();
-    off += s_prefix;
-#pragma unroll
-    for (int k = 0; k < kScanItems; k++) {
+    for (int k = 0; k < kScanItems2; k++) {
         if (base + k < n) out[base + k] = off;
         off += v[k];
     }
 }
 
-// Place every particle at start[chunk] + arrival rank; reset the counts, the
-// scan state and the tile counter for the next binning; advance the step
-// counter of the device rotational noise.
+// Place every particle at start[chunk] + arrival rank; reset the counts for
+// the next binning; advance the step counter of the device rotational noise.
 __global__ void k_scatter(SysParams P, const int32_t *__restrict__ key, const int32_t *__restrict__ rank,
                           const int32_t *__restrict__ start, int32_t *__restrict__ src, int32_t *__restrict__ scell,
-                          int32_t *__restrict__ cnt, unsigned long long *__restrict__ status, int32_t nstatus,
-                          int32_t *__restrict__ tile_counter, unsigned long long *__restrict__ counter) {
+                          int32_t *__restrict__ cnt, unsigned long long *__restrict__ counter) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < P.na) {
         const int k = key[i];
@@ -268,11 +243,7 @@ __global__ void k_scatter(SysParams P, const int32_t *__restrict__ key, const in
         scell[t] = k;
     }
     if (i <= P.ncell) cnt[i] = 0;
-    if (i < nstatus) status[i] = 0ull;
-    if (i == 0) {
-        *tile_counter = 0;
-        counter[0] += 1ull;
-    }
+    if (i == 0) counter[0] += 1ull;
 }
 
 // One thread per new slot: the final slot inside the chunk is the number of
@@ -981,6 +952,134 @@ __global__ void k_rings(SysParams P, const int32_t *__restrict__ rtype, const Ri
     pol[r] = theta + d_theta;
 }
 
+// One ring per 32 lanes (lane = particle index inside the ring).  Every
+// reduction the reference does as a sequential loop (unwrap, centre of mass,
+// shoelace area, mean velocity) is done here as the same left-to-right loop
+// over values broadcast from the owning lanes, so the result is bit-identical
+// to the one-thread-per-ring form.
+template <bool UPDATE>
+__global__ __launch_bounds__(256) void k_rings_lanes(
+    SysParams P, const int32_t *__restrict__ rtype, const RingType *__restrict__ types,
+    const int32_t *__restrict__ slot_of, double2 *__restrict__ Pst, double2 *__restrict__ F, double *__restrict__ pol,
+    double *__restrict__ areas, double2 *__restrict__ cms, double2 *__restrict__ cms_next, double2 *__restrict__ cpos,
+    const double *__restrict__ noise, uint64_t seed, const unsigned long long *__restrict__ counter) {
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = gt >> 5;
+    const int i = gt & 31;
+    const bool ring_ok = r < P.nrings;
+    const int rr = ring_ok ? r : 0;
+    const RingType T = types[P.typed ? rtype[rr] : 0];
+    const int np = T.np;
+    const bool act = ring_ok && i < np;
+    const int64_t base = (int64_t)rr * P.nmax;
+    const int slot = act ? slot_of[base + i] : 0;
+    const double2 x = act ? Pst[slot] : make_double2(0.0, 0.0);
+    auto bc = [](double v, int src) { return __shfl(v, src, 32); };
+
+    if constexpr (UPDATE) {
+        if (ring_ok && i == 0) cms[r] = cms_next[r];
+    }
+    // neighbours along the ring
+    const int inext = (i == np - 1) ? 0 : i + 1;
+    const int iprev = (i == 0) ? np - 1 : i - 1;
+    const double2 xn = make_double2(bc(x.x, inext), bc(x.y, inext));
+    const double2 xp = make_double2(bc(x.x, iprev), bc(x.y, iprev));
+
+    // update_continuos_pos!: c_0 = x_0, c_i = c_{i-1} + calc_diff(x_i, x_{i-1})
+    const double2 d = (i == 0) ? x : cdiff(P, x, xp);
+    double cx = 0.0, cy = 0.0;
+    for (int k = 0; k < np; k++) {
+        const double dx = bc(d.x, k), dy = bc(d.y, k);
+        if (k == 0) {
+            cx = dx;
+            cy = dy;
+        } else if (k <= i) {
+            cx = cx + dx;
+            cy = cy + dy;
+        }
+    }
+    if (!P.periodic) {  // update_continuos_pos! is a no-op: continuous = stored positions
+        cx = x.x;
+        cy = x.y;
+    }
+    if (act) cpos[base + i] = make_double2(cx, cy);
+    // centre of mass and shoelace area, sums in ring order
+    const double cnx = bc(cx, inext), cny = bc(cy, inext);
+    const double term = cx * cny - cy * cnx;
+    double sx = 0.0, sy = 0.0, area = 0.0;
+    for (int k = 0; k < np; k++) {
+        sx += bc(cx, k);
+        sy += bc(cy, k);
+        area += bc(term, k);
+    }
+    area = area / 2.0;
+    if (ring_ok && i == 0) {
+        cms_next[r] = make_double2(sx / (double)np, sy / (double)np);
+        areas[r] = area;
+    }
+
+    // springs: lane i holds f_i = spring(i, i+1); particle i gets +f_1 then
+    // -f_n (i = 0) or -f_{i-1} then +f_i, then the area term
+    const double2 ds = cdiff(P, x, xn);
+    const double dist = sqrt(ds.x * ds.x + ds.y * ds.y);
+    const double q = -T.kspring * (dist - T.lspring) / dist;
+    const double2 fi = make_double2(q * ds.x, q * ds.y);
+    const double2 fprev = make_double2(bc(fi.x, iprev), bc(fi.y, iprev));
+    double area0 = (double)np * T.lspring / T.p0;
+    area0 = area0 * area0;
+    const double fa = T.karea * (area - area0);
+    const double2 da = cdiff(P, xn, xp);
+    double2 Fi = act ? F[slot] : make_double2(0.0, 0.0);
+    if (i == 0) {
+        Fi.x += fi.x;
+        Fi.y += fi.y;
+        Fi.x -= fprev.x;
+        Fi.y -= fprev.y;
+    } else {
+        Fi.x -= fprev.x;
+        Fi.y -= fprev.y;
+        Fi.x += fi.x;
+        Fi.y += fi.y;
+    }
+    Fi.x -= fa * (da.y / 2.0);
+    Fi.y -= fa * (-da.x / 2.0);
+    if (act) F[slot] = Fi;
+    if constexpr (!UPDATE) return;
+
+    // update!: v = vo pol + mu F, x += v dt (then walls!), polarisation relaxation
+    const double theta = ring_ok ? pol[rr] : 0.0;
+    double sn, cs;
+    sincos(theta, &sn, &cs);
+    const double vx = T.vo * cs + T.mu * Fi.x;
+    const double vy = T.vo * sn + T.mu * Fi.y;
+    double vcx = 0.0, vcy = 0.0;
+    for (int k = 0; k < np; k++) {
+        vcx += bc(vx, k);
+        vcy += bc(vy, k);
+    }
+    double px = x.x + vx * P.dt, py = x.y + vy * P.dt;
+    if (P.periodic) {
+        const double dxc = px - (P.blx + P.halfL), dyc = py - (P.bly + P.halfH);
+        const bool ox = fabs(dxc) > P.halfL, oy = fabs(dyc) > P.halfH;
+        if (ox) px = px - copysign(P.halfL * 2.0, dxc);
+        if (oy) py = py - copysign(P.halfH * 2.0, dyc);
+    }
+    if (act) Pst[slot] = make_double2(px, py);
+    if (ring_ok && i == 0) {
+        vcx = vcx / (double)np;
+        vcy = vcy / (double)np;
+        const double speed = sqrt(vcx * vcx + vcy * vcy);
+        double cr = 0.0;
+        if (speed != 0.0) {
+            cr = (cs * vcy - sn * vcx) / speed;
+            if (fabs(cr) > 1.0) cr = cr > 0.0 ? 1.0 : -1.0;
+        }
+        const double z = noise ? noise[r] : (T.rot != 0.0 ? philox_normal(seed, counter[0], (uint32_t)r) : 0.0);
+        const double d_theta = 1.0 / T.relax * asin(cr) * P.dt + sqrt(2.0 * T.rot * P.dt) * z;
+        pol[r] = theta + d_theta;
+    }
+}
+
 __global__ void k_rings_init(SysParams P, const int32_t *__restrict__ rtype, const RingType *__restrict__ types,
                              const double2 *__restrict__ pos_id, double2 *__restrict__ cms,
                              double2 *__restrict__ cms_next, double2 *__restrict__ cpos, double *__restrict__ areas) {
@@ -1163,11 +1262,13 @@ hipError_t launch_bin(mavi_sys *s, hipStream_t st) {
     hipLaunchKernelGGL(k_bin_count, dim3(nblk(P.na, 256)), dim3(256), 0, st, P, s->P_[c], s->key, s->rank, s->cnt,
                        s->flags);
     const int n = P.ncell + 1;
-    hipLaunchKernelGGL(k_scan, dim3(nblk(n, kScanTile)), dim3(kScanThreads), 0, st, s->cnt, s->start,
-                       s->scan_status, s->cnt + P.ncell + 1, n, s->flags);
+    const int ntiles = nblk(n, kScanTile2);
+    hipLaunchKernelGGL(k_tile_reduce, dim3(ntiles), dim3(kScanThreads), 0, st, s->cnt, n, s->tile_sum);
+    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(kScanThreads), 0, st, s->tile_sum, ntiles);
+    hipLaunchKernelGGL(k_tile_apply, dim3(ntiles), dim3(kScanThreads), 0, st, s->cnt, n, s->tile_sum, s->start);
     const int grid = nblk(P.na > n ? P.na : n, 256);
     hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(256), 0, st, P, s->key, s->rank, s->start, s->src, s->scell,
-                       s->cnt, s->scan_status, s->nscan_blocks, s->cnt + P.ncell + 1, s->counter);
+                       s->cnt, s->counter);
     const bool has_vel = P.potential != MAVI_POT_RINGS;
     hipLaunchKernelGGL(k_finalize, dim3(nblk(P.na, 256)), dim3(256), 0, st, P, s->start, s->src, s->scell, s->P_[c],
                        has_vel ? s->V_[c] : nullptr, s->ID_[c], s->P_[nc], has_vel ? s->V_[nc] : nullptr, s->ID_[nc],
@@ -1202,10 +1303,23 @@ hipError_t launch_verlet_step(mavi_sys *s, hipStream_t st) {
 static hipError_t launch_rings_kernel(mavi_sys *s, bool update, const double *noise_dev, hipStream_t st) {
     ProfScope ps(s, MAVI_PROF_RINGS, st);
     const int c = s->cur;
+    const double *nz = update ? noise_dev : nullptr;
+    if (s->P.nmax <= 32) {
+        const int nb = nblk((int64_t)s->P.nrings * 32, 256);
+        if (update)
+            hipLaunchKernelGGL(k_rings_lanes<true>, dim3(nb), dim3(256), 0, st, s->P, s->rtype, s->types, s->slot_of,
+                               s->P_[c], s->F, s->pol, s->areas, s->cms, s->cms_next, s->cpos, nz, s->seed,
+                               s->counter);
+        else
+            hipLaunchKernelGGL(k_rings_lanes<false>, dim3(nb), dim3(256), 0, st, s->P, s->rtype, s->types,
+                               s->slot_of, s->P_[c], s->F, s->pol, s->areas, s->cms, s->cms_next, s->cpos, nz,
+                               s->seed, s->counter);
+        return hipGetLastError();
+    }
     if (update)
         hipLaunchKernelGGL(k_rings<true>, dim3(nblk(s->P.nrings, 64)), dim3(64), 0, st, s->P, s->rtype, s->types,
-                           s->slot_of, s->P_[c], s->F, s->pol, s->areas, s->cms, s->cms_next, s->cpos, noise_dev,
-                           s->seed, s->counter);
+                           s->slot_of, s->P_[c], s->F, s->pol, s->areas, s->cms, s->cms_next, s->cpos, nz, s->seed,
+                           s->counter);
     else
         hipLaunchKernelGGL(k_rings<false>, dim3(nblk(s->P.nrings, 64)), dim3(64), 0, st, s->P, s->rtype, s->types,
                            s->slot_of, s->P_[c], s->F, s->pol, s->areas, s->cms, s->cms_next, s->cpos, nullptr,

@@ -62,6 +62,7 @@ def types_case(use_chunks=True, seed=7):
 
 
 CASES = {
+    "big_rings": lambda: normal_case(True, n=40, cols=5, rows=4),  # > 32 particles: one-thread-per-ring kernel
     "normal": lambda: normal_case(True),
     "normal_nochunks": lambda: normal_case(False),
     "types": lambda: types_case(True),
