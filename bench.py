@@ -122,11 +122,25 @@ def main():
     ap.add_argument("--prof-steps", type=int, default=100)
     ap.add_argument("--rings", type=int, default=40000, help="rings in the rings form (0 = skip)")
     ap.add_argument("--rings-steps", type=int, default=500)
+    ap.add_argument("--calib-copy", type=int, default=0,
+                    help="MiB copied device-to-device 4 times before the run (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported in this build "
               "(the multi-GPU decomposition is a design note, not code).", file=sys.stderr)
         sys.exit(2)
+
+    if args.calib_copy > 0:
+        import torch
+
+        nbytes = args.calib_copy << 20
+        a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(4):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy: {nbytes} bytes read + {nbytes} bytes written, 4 launches")
+        del a, b
 
     import mavi_hip as mv
     from mavi_hip import workloads as wl

@@ -1,0 +1,220 @@
+"""
+    MaviHIP
+
+Julia side of the MI355X step hot path of Mavi.jl.  Adds the device mode
+`GPU <: Mavi.Configs.DeviceMode` and methods on Mavi's own generic functions,
+so that `System(...; int_cfg=IntCfg(dt=..., chunks_cfg=..., device=GPU()))`
+runs `newton_step!` / `Rings.Integration.step!` through libmavi_hip.so
+(`ccall`, C ABI in include/mavi_hip.h).  Built-in potentials (LenJonesCfg,
+HarmTruncCfg, RingsCfg with HarmTruncCfg/InteractionMatrix) run on the GPU;
+any other DynamicCfg (user forces, Szabo, RunTumble) and non-rectangular
+geometries keep Mavi's Sequencial CPU methods.
+
+Reference dispatch points used (Mavi.jl repository paths):
+  calc_forces!(system, chunks, device)      src/integration.jl:112, 159, 197, 226
+  update_chunks!(chunks)                     src/chunks.jl:150-161
+  get_step_function(sys_type, system)        src/integration.jl:537-548, src/rings/integration.jl:545
+  run_system(system; tf, num_steps)          src/run_system.jl:7-23
+  kinetic_energy / potential_energy          src/quantities.jl:12-18, 46-66
+"""
+module MaviHIP
+
+using Mavi
+using Mavi.Configs
+using Mavi.Systems
+using Mavi.ChunksMod
+using Mavi.Integration
+using StaticArrays
+
+export GPU
+
+const LIB = joinpath(@__DIR__, "..", "mavi_hip", "libmavi_hip.so")
+
+struct GPU <: Mavi.Configs.DeviceMode
+    device::Int
+end
+GPU() = GPU(0)
+
+# --- C ABI ---------------------------------------------------------------
+
+const MAVI_OK = Cint(0)
+const MAVI_WALL_RIGID = Cint(0)
+const MAVI_WALL_PERIODIC = Cint(1)
+const MAVI_POT_LJ = Cint(0)
+const MAVI_POT_HARMTRUNC = Cint(1)
+const MAVI_POT_RINGS = Cint(2)
+
+struct SysDesc                      # mavi_sys_desc
+    n::Int64
+    bottom_left::NTuple{2,Float64}
+    size::NTuple{2,Float64}
+    wall::Int32
+    potential::Int32
+    pot::NTuple{4,Float64}
+    dt::Float64
+    cols::Int32
+    rows::Int32
+    nc::Int32
+    reserved0::Int32
+    radius::Float64
+    nmax::Int32
+    nrings::Int32
+    ntypes::Int32
+    reserved1::Int32
+    ring_types::Ptr{Int32}
+    type_np::Ptr{Int32}
+    type_par::Ptr{Float64}
+    interaction::Ptr{Float64}
+    seed::UInt64
+end
+
+mutable struct DeviceSys
+    ctx::Ptr{Cvoid}
+    sys::Ptr{Cvoid}
+    host_dirty::Bool
+end
+
+function check(ctx, st)
+    st == MAVI_OK && return nothing
+    msg = unsafe_string(ccall((:mavi_last_error, LIB), Cstring, (Ptr{Cvoid},), ctx))
+    error("libmavi_hip status $st: $msg")
+end
+
+# One context per device, one device system per Mavi System (side table keyed
+# by objectid: System is immutable, its arrays are not).
+const CONTEXTS = Dict{Int,Ptr{Cvoid}}()
+const SYSTEMS = Dict{UInt,DeviceSys}()
+
+function context(dev::Int)
+    get!(CONTEXTS, dev) do
+        out = Ref{Ptr{Cvoid}}(C_NULL)
+        st = ccall((:mavi_ctx_create, LIB), Cint, (Cint, Ref{Ptr{Cvoid}}), dev, out)
+        st == MAVI_OK || error("mavi_ctx_create failed ($st)")
+        out[]
+    end
+end
+
+# --- which configurations take the GPU path --------------------------------
+
+gpu_kind(::Any) = nothing
+gpu_kind(::LenJonesCfg) = MAVI_POT_LJ
+gpu_kind(::HarmTruncCfg) = MAVI_POT_HARMTRUNC
+gpu_kind(::Mavi.Rings.Configs.RingsCfg{U,T,F}) where {U,T<:Mavi.Rings.Configs.HarmTruncCfg,F} = MAVI_POT_RINGS
+
+wall_kind(::SpaceCfg{RigidWalls,<:RectangleCfg}) = MAVI_WALL_RIGID
+wall_kind(::SpaceCfg{PeriodicWalls,<:RectangleCfg}) = MAVI_WALL_PERIODIC
+wall_kind(::Any) = nothing
+
+on_gpu(system) = system.int_cfg.device isa GPU &&
+                 gpu_kind(system.dynamic_cfg) !== nothing && wall_kind(system.space_cfg) !== nothing
+
+function device_system(system)
+    key = objectid(system)
+    haskey(SYSTEMS, key) && return SYSTEMS[key]
+    ctx = context(system.int_cfg.device.device)
+    g = system.space_cfg.geometry_cfg
+    dyn = system.dynamic_cfg
+    chunks = system.chunks
+    cols = isnothing(chunks) ? 0 : chunks.num_cols
+    rows = isnothing(chunks) ? 0 : chunks.num_rows
+    nc = isnothing(chunks) ? 0 : size(chunks.chunk_particles, 1)
+    pot = dyn isa LenJonesCfg ? (dyn.sigma, dyn.epsilon, 0.0, 0.0) :
+          dyn isa HarmTruncCfg ? (dyn.k_rep, dyn.k_atr, dyn.dist_eq, dyn.dist_max) : (0.0, 0.0, 0.0, 0.0)
+    desc = SysDesc(length(system.state.pos), Tuple(g.bottom_left), (g.length, g.height),
+                   wall_kind(system.space_cfg), gpu_kind(dyn), pot, system.int_cfg.dt,
+                   cols, rows, nc, 0, minimum(particle_radius(dyn)),
+                   0, 0, 0, 0, C_NULL, C_NULL, C_NULL, C_NULL, UInt64(0))
+    # (RingsSystem fills nmax/nrings/ntypes and the type tables the same way,
+    #  from RingsCfg fields p0, relax_time, vo, mobility, rot_diff, k_area,
+    #  k_spring, l_spring, num_particles and interaction_finder.)
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ctx, ccall((:mavi_sys_create, LIB), Cint, (Ptr{Cvoid}, Ref{SysDesc}, Ref{Ptr{Cvoid}}), ctx, desc, out))
+    d = DeviceSys(ctx, out[], true)
+    SYSTEMS[key] = d
+    finalizer(system.state) do _
+        ccall((:mavi_sys_destroy, LIB), Cint, (Ptr{Cvoid},), d.sys)
+        delete!(SYSTEMS, key)
+    end
+    return d
+end
+
+function upload!(system, d::DeviceSys)
+    pos = system.state.pos
+    vel = hasproperty(system.state, :vel) ? system.state.vel : nothing
+    GC.@preserve pos vel begin
+        check(d.ctx, ccall((:mavi_sys_upload, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                           d.sys, pointer(pos), isnothing(vel) ? C_NULL : pointer(vel), C_NULL))
+    end
+    d.host_dirty = false
+end
+
+function download!(system, d::DeviceSys)
+    pos = system.state.pos
+    vel = hasproperty(system.state, :vel) ? system.state.vel : nothing
+    forces = get_forces(system)
+    GC.@preserve pos vel forces begin
+        check(d.ctx, ccall((:mavi_sys_download, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                           d.sys, pointer(pos), isnothing(vel) ? C_NULL : pointer(vel), C_NULL))
+        check(d.ctx, ccall((:mavi_get_forces, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), d.sys, pointer(forces)))
+    end
+end
+
+# --- methods on Mavi's generic functions ------------------------------------
+
+function Integration.calc_forces!(system::System, chunks, device::GPU)
+    on_gpu(system) || return Integration.calc_forces!(system, chunks, Sequencial())
+    d = device_system(system)
+    upload!(system, d)
+    forces = get_forces(system)
+    GC.@preserve forces check(d.ctx, ccall((:mavi_forces, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), d.sys,
+                                           pointer(forces)))
+    return nothing
+end
+
+"newton_step! on the GPU: upload, one step, download (per-step sync: user code may read the state)."
+function gpu_newton_step!(system)
+    d = device_system(system)
+    upload!(system, d)
+    check(d.ctx, ccall((:mavi_step, LIB), Cint, (Ptr{Cvoid}, Int64), d.sys, 1))
+    download!(system, d)
+    system.time_info.time += system.int_cfg.dt
+    system.time_info.num_steps += 1
+end
+
+function Integration.get_step_function(t::StandardSys, system::System)
+    if on_gpu(system) && !(system.dynamic_cfg isa SzaboCfg || system.dynamic_cfg isa RunTumbleCfg)
+        return gpu_newton_step!
+    end
+    return Integration.get_step_function(t, system, Val(:cpu))
+end
+# the reference's selection, src/integration.jl:538-548
+function Integration.get_step_function(::StandardSys, system, ::Val{:cpu})
+    cfg = system.dynamic_cfg
+    cfg isa SzaboCfg && return Integration.szabo_step!
+    cfg isa RunTumbleCfg && return Integration.rtp_step!
+    return Integration.newton_step!
+end
+
+"run_system with a resident fast path: upload once, all steps on the device, download once."
+function run_gpu!(system; tf=nothing, num_steps=nothing)
+    n = if isnothing(tf)
+        num_steps
+    else
+        t, k = system.time_info.time, 0
+        while t < tf
+            t += system.int_cfg.dt
+            k += 1
+        end
+        k
+    end
+    d = device_system(system)
+    upload!(system, d)
+    check(d.ctx, ccall((:mavi_step, LIB), Cint, (Ptr{Cvoid}, Int64), d.sys, n))
+    download!(system, d)
+    for _ in 1:n
+        system.time_info.time += system.int_cfg.dt
+    end
+    system.time_info.num_steps += n
+end
+
+end # module
