@@ -95,12 +95,95 @@ def bench_rings(args):
 
         o = oracle.from_rings(st, sp, cfg, ic)
         t0 = time.perf_counter()
-        o.rings_steps(pos0, pol0, 2)
+        o.rings_steps(pos0, pol0, args.rings_cpu_steps)
         dt = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": n * 2 / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-                               "sample": f"{n} particles x 2 Rings step! on one core ({dt:.1f} s)"}
+        out["cpu_baseline"] = {"value": n * args.rings_cpu_steps / dt, "unit": "particle-steps/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"{n} particles x {args.rings_cpu_steps} Rings step! on one core ({dt:.1f} s)"}
         out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     return out
+
+
+def bench_batched(args, plain_value):
+    """Replica batch: 512 replicas (log-uniform 1k-64k particles, half LJ
+    gases, half active-ring mixtures), every phase one launch over all
+    replicas, 100-step graphs.  Compared with the plain path stepping the
+    same replicas one after another (mavi_step per replica, its own 2-step
+    graphs), and with the 1M plain figure."""
+    import mavi_hip as mv
+    from mavi_hip import _lib, rings, systems
+    from mavi_hip import batch as mb
+    from mavi_hip import workloads as wl
+
+    ens = wl.batch_ensemble(nrep=args.batch)
+    ctx = _lib.Context.for_device(0)
+    b = mb.Batch(ctx=ctx)
+    inits = []
+    for kind, case, seed in ens:
+        if kind == "plain":
+            pos, vel, space, dyn, ic = case
+            b.add(mv.SecondLawState(pos=pos, vel=vel), space, dyn, ic)
+            inits.append((pos, vel, None))
+        else:
+            st, space, cfg, ic = case
+            b.add(st, space, cfg, ic, seed=seed)
+            inits.append((st.pos, None, st.pol))
+    b.finalize()
+    b.set_option(mb.MAVI_BOPT_GRAPH_STEPS, args.batch_graph)
+    for i, (p, v, pol) in enumerate(inits):
+        b.upload(i, p, v, pol)
+    info = b.info()
+    n = info["particles"]
+    log(f"[bench] batched: {info}")
+    best_us, shape = b.autotune(trial_steps=4)
+    log(f"[bench] batched autotune: {best_us:.1f} us/step with {shape}")
+    b.run(args.batch_graph)  # warm-up + graph capture
+    t0 = time.perf_counter()
+    b.run(args.batch_steps)
+    wall = time.perf_counter() - t0
+    value = n * args.batch_steps / wall
+    b.prof_enable(True)
+    b.run(10)
+    prof = {name: b.prof_get(k) for k, name in ((0, "bin"), (1, "pair"), (2, "ring"))}
+    b.prof_enable(False)
+    del b
+
+    # the same replicas through the plain single-system path, one after another
+    seq_steps = args.batch_seq_steps
+    devs = []
+    for (kind, case, seed), (p, v, pol) in zip(ens, inits):
+        if kind == "plain":
+            pos, vel, space, dyn, ic = case
+            d, _ = systems.describe(mv.SecondLawState(pos=pos, vel=vel), space, dyn, ic)
+            dev = _lib.DeviceSystem(ctx, d)
+        else:
+            st, space, cfg, ic = case
+            d, _, tables = rings.describe(st, space, cfg, ic, seed)
+            dev = _lib.DeviceSystem(ctx, d, keep=tables)
+        dev.upload(p, v, pol)
+        dev.step(4)  # warm-up + graph capture
+        devs.append(dev)
+    t0 = time.perf_counter()
+    for dev in devs:
+        dev.step(seq_steps)
+    wall_seq = time.perf_counter() - t0
+    seq_value = n * seq_steps / wall_seq
+    del devs
+    return {
+        "workload": f"{info['replicas']} replicas, particle counts log-uniform in [1000, 64000] (seed 3), even "
+                    "replicas LJ gases (density 0.7, cutoff 2.5, periodic), odd replicas active-ring mixtures "
+                    "(20-particle rings, packing fraction 0.8, device noise); dt 1e-3",
+        "particles": n, "replicas": info["replicas"], "chunks": info["chunks"], "steps": args.batch_steps,
+        "value": value, "unit": "particle-steps/s", "ms_per_step": wall / args.batch_steps * 1e3,
+        "graph_steps": args.batch_graph, "shape": shape,
+        "kernel_avg_ms_per_step": {k: v[0] / 10 for k, v in prof.items()},
+        "sequential_plain": {"value": seq_value, "unit": "particle-steps/s", "steps": seq_steps,
+                             "how": "mavi_step(replica, steps) for each replica in turn"},
+        "speedup_vs_sequential_plain": value / seq_value,
+        "ratio_vs_plain_1M": value / plain_value if plain_value else None,
+        "error": "none: every replica is bit-identical to its single-system run (tests/test_gpu_batch.py), so "
+                 "g(r) and segregation index are unchanged",
+    }
 
 
 def host_cores() -> int:
@@ -117,11 +200,16 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--nside", type=int, default=1024)
-    ap.add_argument("--cpu-steps", type=int, default=4, help="steps of the CPU sample (all host cores)")
+    ap.add_argument("--cpu-steps", type=int, default=48, help="steps of the CPU sample (all host cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--prof-steps", type=int, default=100)
     ap.add_argument("--rings", type=int, default=40000, help="rings in the rings form (0 = skip)")
     ap.add_argument("--rings-steps", type=int, default=500)
+    ap.add_argument("--rings-cpu-steps", type=int, default=40)
+    ap.add_argument("--batch", type=int, default=512, help="replicas in the batched form (0 = skip)")
+    ap.add_argument("--batch-steps", type=int, default=1000)
+    ap.add_argument("--batch-graph", type=int, default=100, help="steps per captured graph of the batch")
+    ap.add_argument("--batch-seq-steps", type=int, default=100, help="steps per replica of the sequential run")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="MiB copied device-to-device 4 times before the run (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -211,9 +299,12 @@ def main():
     if cpu:
         out["speedup_vs_cpu"] = value / cpu["value"]
     forms = {}
+    del system, dev
     if args.rings > 0:
-        del system, dev
         forms["rings"] = bench_rings(args)
+    if args.batch > 0:
+        log(f"[bench] batched form, {args.batch} replicas")
+        forms["batched"] = bench_batched(args, value)
     if forms:
         out["forms"] = forms
     print(json.dumps(out), flush=True)

@@ -168,6 +168,57 @@ int mavi_sys_set_option(mavi_sys *sys, int32_t option, int32_t value);
  * receives {workgroup, lds_tiles, two_phase}. */
 int mavi_autotune(mavi_sys *sys, int32_t trial_steps, double *best_us, int32_t *shape);
 
+/* Replica batches ---------------------------------------------------------
+ * Many independent systems of uneven size stepped together: the device side
+ * of run_experiment_batch (src/experiments.jl:246-300), where every replica
+ * is its own System stepped by its own step function.  Every phase of a step
+ * is one launch over all replicas of a kind, and multi-step runs replay a
+ * captured graph of `graph_steps` steps.  Each replica's results are
+ * bit-identical to the same descriptor stepped alone with mavi_step.
+ * Replicas need chunks (cols, rows > 0); rings replicas need nmax <= 32. */
+typedef struct mavi_batch mavi_batch;
+int mavi_batch_create(mavi_ctx *ctx, mavi_batch **out);
+int mavi_batch_destroy(mavi_batch *batch);
+/* Append a replica (descriptor as for mavi_sys_create; host tables are read
+ * during the call only).  *index receives its 0-based index. */
+int mavi_batch_add_replica(mavi_batch *batch, const mavi_sys_desc *desc, int32_t *index);
+/* Allocate the device arrays; no replicas may be added afterwards. */
+int mavi_batch_finalize(mavi_batch *batch);
+/* State of one replica, as mavi_sys_upload / mavi_sys_download. */
+int mavi_batch_upload(mavi_batch *batch, int32_t index, const double *pos, const double *vel, const double *pol);
+int mavi_batch_download(mavi_batch *batch, int32_t index, double *pos, double *vel, double *pol);
+int mavi_batch_get_forces(mavi_batch *batch, int32_t index, double *forces);
+/* nsteps of every replica's own dynamics (newton_step! or Rings step! with
+ * the device rotational noise).  A failing replica is named in
+ * mavi_last_error. */
+int mavi_batch_run(mavi_batch *batch, int64_t nsteps);
+/* Batch execution options:
+ *   MAVI_BOPT_WG_PLAIN / MAVI_BOPT_WG_RINGS  pair-kernel workgroup (64..1024)
+ *       for LJ/HarmTrunc replicas / rings replicas (default 256)
+ *   MAVI_BOPT_LIST_PLAIN / MAVI_BOPT_LIST_RINGS  two-phase candidate list
+ *       (as MAVI_OPT_TWO_PHASE, default 0)
+ *   MAVI_BOPT_GRAPH_STEPS  steps per captured graph (even, >= 2; default 100)
+ *   MAVI_BOPT_GRAPHS       1 (default): replay graphs; 0: direct launches */
+#define MAVI_BOPT_WG_PLAIN 0
+#define MAVI_BOPT_WG_RINGS 1
+#define MAVI_BOPT_LIST_PLAIN 2
+#define MAVI_BOPT_LIST_RINGS 3
+#define MAVI_BOPT_GRAPH_STEPS 4
+#define MAVI_BOPT_GRAPHS 5
+int mavi_batch_set_option(mavi_batch *batch, int32_t option, int32_t value);
+/* Sizes: info[0] replicas, [1] particle slots (sum of n), [2] active
+ * particles, [3] padded sorted slots, [4] chunks incl. one pad chunk per
+ * replica, [5] rings. */
+int mavi_batch_info(mavi_batch *batch, int64_t *info);
+/* Kernel timing as mavi_prof_enable / mavi_prof_get (classes MAVI_PROF_*). */
+int mavi_batch_prof_enable(mavi_batch *batch, int on);
+int mavi_batch_prof_get(mavi_batch *batch, int which, double *ms_total, int64_t *launches);
+/* Autotuner: for LJ/HarmTrunc replicas and then rings replicas, times
+ * trial_steps steps for every workgroup 64..512 x list on/off, keeps the
+ * fastest and restores the state.  shape (nullable, 4 ints) receives
+ * {wg_plain, list_plain, wg_rings, list_rings}. */
+int mavi_batch_autotune(mavi_batch *batch, int32_t trial_steps, double *best_us, int32_t *shape);
+
 #ifdef __cplusplus
 }
 #endif

@@ -6,39 +6,9 @@
 #include <string>
 #include <vector>
 
-#include "mavi_internal.h"
+#include "mavi_host.h"
 
 namespace {
-
-int set_err(mavi_ctx *ctx, int code, const std::string &msg) {
-    if (ctx) ctx->err = msg;
-    return code;
-}
-
-#define HIPCHK(ctx, expr)                                                                                  \
-    do {                                                                                                   \
-        hipError_t e_ = (expr);                                                                            \
-        if (e_ != hipSuccess)                                                                              \
-            return set_err((ctx), MAVI_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));           \
-    } while (0)
-
-// Allocate and zero on the context's stream: the library's stream is
-// non-blocking, so a zero fill on the null stream would not be ordered
-// before the stream's own work on the buffer.
-template <typename T>
-hipError_t dalloc(T **p, int64_t count, hipStream_t st) {
-    *p = nullptr;
-    if (count <= 0) count = 1;
-    hipError_t e = hipMalloc((void **)p, sizeof(T) * (size_t)count);
-    if (e == hipSuccess) e = hipMemsetAsync(*p, 0, sizeof(T) * (size_t)count, st);
-    return e;
-}
-
-template <typename T>
-void dfree(T *&p) {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-}
 
 // Turn device flags raised since the last check into a status.
 int check_flags(mavi_sys *s) {
@@ -56,8 +26,6 @@ int check_flags(mavi_sys *s) {
     return set_err(ctx, MAVI_E_CELL_OVERFLOW,
                    "a chunk holds more particles than its capacity nc (BoundsError in update_chunks!)");
 }
-
-bool is_pow2_wg(int32_t w) { return w == 64 || w == 128 || w == 256 || w == 512 || w == 1024; }
 
 }  // namespace
 
@@ -130,32 +98,35 @@ int mavi_sys_destroy(mavi_sys *s) {
     return MAVI_OK;
 }
 
-int mavi_sys_create(mavi_ctx *ctx, const mavi_sys_desc *d, mavi_sys **out) {
-    if (!ctx || !d || !out) return MAVI_E_BADARG;
-    *out = nullptr;
-    HIPCHK(ctx, hipSetDevice(ctx->device));
-    if (d->n <= 0 || d->n > (int64_t)0x7fffffff / 2)
-        return set_err(ctx, MAVI_E_BADARG, "n must be in [1, 2^30)");
-    if (!(d->size[0] > 0.0) || !(d->size[1] > 0.0)) return set_err(ctx, MAVI_E_BADARG, "box size must be > 0");
+}  // extern "C"
+
+// Validate a descriptor and derive the kernel parameters and ring tables
+// (System(...) / RingsSystem(...) construction, src/systems.jl:73-114,
+// src/rings/rings.jl:231-291).  Shared by single systems and batches.
+int build_params(const mavi_sys_desc *d, HostSys &h, std::string &err) {
+    auto fail = [&](int code, const char *msg) {
+        err = msg;
+        return code;
+    };
+    if (d->n <= 0 || d->n > (int64_t)0x7fffffff / 2) return fail(MAVI_E_BADARG, "n must be in [1, 2^30)");
+    if (!(d->size[0] > 0.0) || !(d->size[1] > 0.0)) return fail(MAVI_E_BADARG, "box size must be > 0");
     if (d->wall != MAVI_WALL_RIGID && d->wall != MAVI_WALL_PERIODIC)
-        return set_err(ctx, MAVI_E_UNSUPPORTED, "only RigidWalls and PeriodicWalls rectangles run on the GPU");
+        return fail(MAVI_E_UNSUPPORTED, "only RigidWalls and PeriodicWalls rectangles run on the GPU");
     if (d->potential < MAVI_POT_LJ || d->potential > MAVI_POT_RINGS)
-        return set_err(ctx, MAVI_E_UNSUPPORTED, "unknown potential kind");
+        return fail(MAVI_E_UNSUPPORTED, "unknown potential kind");
     const bool chunks = d->cols > 0 || d->rows > 0;
     if (chunks) {
-        if (d->cols <= 0 || d->rows <= 0) return set_err(ctx, MAVI_E_BADARG, "cols and rows must both be > 0");
+        if (d->cols <= 0 || d->rows <= 0) return fail(MAVI_E_BADARG, "cols and rows must both be > 0");
         if (d->wall == MAVI_WALL_PERIODIC && (d->cols < 3 || d->rows < 3))
-            return set_err(ctx, MAVI_E_UNSUPPORTED,
-                           "periodic chunk grids need >= 3 chunks per side (the reference stencil repeats "
-                           "neighbours below that)");
-        if ((int64_t)d->cols * d->rows > 0x3fffffff) return set_err(ctx, MAVI_E_BADARG, "chunk grid too large");
-        if (d->nc <= 0) return set_err(ctx, MAVI_E_BADARG, "chunk capacity nc must be > 0");
+            return fail(MAVI_E_UNSUPPORTED,
+                        "periodic chunk grids need >= 3 chunks per side (the reference stencil repeats "
+                        "neighbours below that)");
+        if ((int64_t)d->cols * d->rows > 0x3fffffff) return fail(MAVI_E_BADARG, "chunk grid too large");
+        if (d->nc <= 0) return fail(MAVI_E_BADARG, "chunk capacity nc must be > 0");
     }
-    if (!(d->dt > 0.0) && d->dt != 0.0) return set_err(ctx, MAVI_E_BADARG, "dt must be >= 0");
-
-    mavi_sys *s = new mavi_sys();
-    s->ctx = ctx;
-    SysParams &P = s->P;
+    if (!(d->dt > 0.0) && d->dt != 0.0) return fail(MAVI_E_BADARG, "dt must be >= 0");
+    SysParams &P = h.P;
+    P = SysParams{};
     P.n = (int32_t)d->n;
     P.cols = chunks ? d->cols : 0;
     P.rows = chunks ? d->rows : 0;
@@ -181,76 +152,87 @@ int mavi_sys_create(mavi_ctx *ctx, const mavi_sys_desc *d, mavi_sys **out) {
     P.p3 = d->pot[3];
     P.cut2 = INFINITY;
     P.eq2 = 0.0;
+    h.list2 = INFINITY;
     if (d->potential == MAVI_POT_LJ) {
         P.cut2 = exact_gt_threshold(d->pot[2]);
         const double s2 = d->pot[0] * d->pot[0], s6 = s2 * s2 * s2;
         P.lj_c12 = 48.0 * d->pot[1] * s6 * s6;
         P.lj_c6 = 24.0 * d->pot[1] * s6;
+        h.list2 = P.cut2;
     } else if (d->potential == MAVI_POT_HARMTRUNC) {
         P.cut2 = exact_gt_threshold(d->pot[3]);
         P.eq2 = exact_lt_threshold(d->pot[2]);
+        h.list2 = P.cut2;
     }
-
-    std::vector<int32_t> active;
-    std::vector<int32_t> host_rtype;
-    std::vector<RingType> host_types;
-    std::vector<RingInter> host_inter;
+    h.active.clear();
+    h.rtype.clear();
+    h.types.clear();
+    h.inter.clear();
+    h.seed = d->seed;
     if (d->potential == MAVI_POT_RINGS) {
-        if (d->wall != MAVI_WALL_PERIODIC) {
-            delete s;
-            return set_err(ctx, MAVI_E_UNSUPPORTED, "rings run on the GPU with PeriodicWalls");
-        }
+        if (d->wall != MAVI_WALL_PERIODIC) return fail(MAVI_E_UNSUPPORTED, "rings run on the GPU with PeriodicWalls");
         if (d->nmax <= 0 || d->nrings <= 0 || (int64_t)d->nmax * d->nrings != d->n || d->ntypes <= 0 ||
-            !d->type_np || !d->type_par || !d->interaction) {
-            delete s;
-            return set_err(ctx, MAVI_E_BADARG, "rings descriptor: need n == nmax*nrings and type tables");
-        }
+            !d->type_np || !d->type_par || !d->interaction)
+            return fail(MAVI_E_BADARG, "rings descriptor: need n == nmax*nrings and type tables");
         P.nmax = d->nmax;
         P.nrings = d->nrings;
         P.ntypes = d->ntypes;
         P.typed = d->ring_types != nullptr;
-        if (!P.typed && d->ntypes != 1) {
-            delete s;
-            return set_err(ctx, MAVI_E_BADARG, "untyped rings need ntypes == 1");
-        }
+        if (!P.typed && d->ntypes != 1) return fail(MAVI_E_BADARG, "untyped rings need ntypes == 1");
         for (int t = 0; t < d->ntypes; t++) {
             RingType T{};
             const double *q = d->type_par + 8 * t;
             T.p0 = q[0]; T.relax = q[1]; T.vo = q[2]; T.mu = q[3];
             T.rot = q[4]; T.karea = q[5]; T.kspring = q[6]; T.lspring = q[7];
             T.np = d->type_np[t];
-            if (T.np < 1 || T.np > d->nmax) {
-                delete s;
-                return set_err(ctx, MAVI_E_BADARG, "ring particle count outside [1, nmax]");
-            }
-            host_types.push_back(T);
+            if (T.np < 1 || T.np > d->nmax) return fail(MAVI_E_BADARG, "ring particle count outside [1, nmax]");
+            h.types.push_back(T);
         }
-        s->list2 = 0.0;
+        h.list2 = 0.0;
         for (int k = 0; k < d->ntypes * d->ntypes; k++) {
             RingInter I{};
             const double *q = d->interaction + 4 * k;
             I.k_rep = q[0]; I.k_atr = q[1]; I.dist_eq = q[2]; I.dist_max = q[3];
             I.max2 = exact_gt_threshold(q[3]);
             I.eq2 = exact_lt_threshold(q[2]);
-            if (I.max2 > s->list2) s->list2 = I.max2;
-            host_inter.push_back(I);
+            if (I.max2 > h.list2) h.list2 = I.max2;
+            h.inter.push_back(I);
         }
-        host_rtype.resize(d->nrings);
+        h.rtype.resize(d->nrings);
         for (int r = 0; r < d->nrings; r++) {
             int t = P.typed ? d->ring_types[r] - 1 : 0;
-            if (t < 0 || t >= d->ntypes) {
-                delete s;
-                return set_err(ctx, MAVI_E_BADARG, "ring type outside [1, ntypes]");
-            }
-            host_rtype[r] = t;
+            if (t < 0 || t >= d->ntypes) return fail(MAVI_E_BADARG, "ring type outside [1, ntypes]");
+            h.rtype[r] = t;
             // FixRingsIds.p_ids (src/rings/states.jl:46-62): the first
             // num_particles[type] slots of every ring, ascending.
-            for (int i = 0; i < host_types[t].np; i++) active.push_back(r * d->nmax + i);
+            for (int i = 0; i < h.types[t].np; i++) h.active.push_back(r * d->nmax + i);
         }
-        P.na = (int32_t)active.size();
+        P.na = (int32_t)h.active.size();
     } else {
         P.na = P.n;
     }
+    return MAVI_OK;
+}
+
+extern "C" {
+
+int mavi_sys_create(mavi_ctx *ctx, const mavi_sys_desc *d, mavi_sys **out) {
+    if (!ctx || !d || !out) return MAVI_E_BADARG;
+    *out = nullptr;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HostSys h;
+    std::string err;
+    int st = build_params(d, h, err);
+    if (st != MAVI_OK) return set_err(ctx, st, err);
+    mavi_sys *s = new mavi_sys();
+    s->ctx = ctx;
+    s->P = h.P;
+    s->list2 = h.list2;
+    SysParams &P = s->P;
+    std::vector<int32_t> &active = h.active;
+    std::vector<int32_t> &host_rtype = h.rtype;
+    std::vector<RingType> &host_types = h.types;
+    std::vector<RingInter> &host_inter = h.inter;
 
     hipError_t e = hipSuccess;
 #define ALLOC(ptr, cnt) \
@@ -302,7 +284,7 @@ int mavi_sys_create(mavi_ctx *ctx, const mavi_sys_desc *d, mavi_sys **out) {
         if (e == hipSuccess) e = up(s->inter, host_inter.data(), sizeof(RingInter) * host_inter.size());
     }
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    s->seed = d->seed;
+    s->seed = h.seed;
     if (e != hipSuccess) {
         std::string m = std::string("allocation/upload failed: ") + hipGetErrorString(e);
         mavi_sys_destroy(s);

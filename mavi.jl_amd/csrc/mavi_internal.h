@@ -151,7 +151,9 @@ __device__ __forceinline__ double min_image(double dr, double size, double half)
 namespace mavi {
 hipError_t launch_iota_state(mavi_sys *s, hipStream_t st);          // id-order staging -> state copy cur
 hipError_t launch_unpermute(mavi_sys *s, bool vel, bool force, hipStream_t st);  // state -> staging
-hipError_t launch_bin(mavi_sys *s, hipStream_t st);                 // binning + permutation (cur flips)
+// binning + permutation (cur flips); count_step advances the step counter
+// of the device rotational noise (a step's binning, not a lone update_chunks!)
+hipError_t launch_bin(mavi_sys *s, hipStream_t st, bool count_step = false);
 hipError_t launch_export_chunks(mavi_sys *s, int64_t *d_cp, int64_t *d_counts, hipStream_t st);
 hipError_t launch_forces_only(mavi_sys *s, hipStream_t st);
 hipError_t launch_verlet_step(mavi_sys *s, hipStream_t st);
@@ -161,6 +163,19 @@ hipError_t launch_rings_init(mavi_sys *s, hipStream_t st);
 hipError_t launch_observables(mavi_sys *s, double *d_out, hipStream_t st);
 hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t st);
 }  // namespace mavi
+
+// Host-side result of validating a descriptor (mavi_capi.hip build_params).
+#include <vector>
+struct HostSys {
+    SysParams P{};
+    double list2 = INFINITY;
+    uint64_t seed = 0;
+    std::vector<int32_t> active;   // active ids (rings)
+    std::vector<int32_t> rtype;    // per ring, 0-based type
+    std::vector<RingType> types;
+    std::vector<RingInter> inter;
+};
+int build_params(const mavi_sys_desc *d, HostSys &h, std::string &err);
 
 // Host helpers (mavi_capi.hip)
 double exact_gt_threshold(double rc);   // largest r2 with sqrt(r2) <= rc

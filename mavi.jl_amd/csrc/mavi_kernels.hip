@@ -22,98 +22,16 @@
 // The library is compiled with -ffp-contract=off: a*b+c is only fused where
 // the code calls __builtin_fma, so the arithmetic that follows Julia's
 // rounding (positions, velocities, HarmTrunc forces, ring updates) does.
-#include "mavi_internal.h"
+#include "mavi_device.h"
+#include "mavi_host.h"
 
 namespace {
-
-constexpr int kXCD = 8;  // MI355X: workgroups go round-robin over 8 XCDs
-
-// Map the hardware block id so that every XCD (and its L2) works on one
-// contiguous stretch of the sorted particles: neighbouring chunk rows are
-// then read from the same L2.
-__device__ __forceinline__ int xcd_block() {
-    const int b = blockIdx.x, nb = gridDim.x;
-    const int full = (nb / kXCD) * kXCD;
-    if (b >= full) return b;
-    return (b % kXCD) * (nb / kXCD) + b / kXCD;
-}
-
-__device__ __forceinline__ double rcp_nr(double x) {
-    // v_rcp_f64 followed by two Newton steps: ~1 ulp, far inside the 1e-12
-    // force tolerance of the LJ path.
-    double y = __builtin_amdgcn_rcp(x);
-    double e = __builtin_fma(-x, y, 1.0);
-    y = __builtin_fma(y, e, y);
-    e = __builtin_fma(-x, y, 1.0);
-    y = __builtin_fma(y, e, y);
-    return y;
-}
-
-__device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
-
-__global__ void k_counter_inc(unsigned long long *counter) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] += 1ull;
-}
-
-__global__ void k_fill_i32(int32_t *p, int32_t v, int64_t n) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
-// ---------------------------------------------------------------------------
-// State staging (host id order <-> sorted state)
-// ---------------------------------------------------------------------------
-
-__global__ void k_iota_state(int32_t na, const int32_t *__restrict__ active, const double2 *__restrict__ pos_id,
-                             const double2 *__restrict__ vel_id, double2 *__restrict__ P, double2 *__restrict__ V,
-                             int32_t *__restrict__ ID, int32_t *__restrict__ slot_of) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= na) return;
-    int id = active ? active[i] : i;
-    P[i] = pos_id[id];
-    if (V) V[i] = vel_id[id];
-    ID[i] = id;
-    if (slot_of) slot_of[id] = i;
-}
-
-__global__ void k_unpermute(int32_t na, const int32_t *__restrict__ ID, const double2 *__restrict__ P,
-                            const double2 *__restrict__ V, const double2 *__restrict__ F, double2 *__restrict__ pos_id,
-                            double2 *__restrict__ vel_id, double2 *__restrict__ force_id) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= na) return;
-    int id = ID[t];
-    if (pos_id) pos_id[id] = P[t];
-    if (vel_id) vel_id[id] = V[t];
-    if (force_id) force_id[id] = F[t];
-}
 
 // ---------------------------------------------------------------------------
 // K1 binning
 // ---------------------------------------------------------------------------
 
-// Cell of one position, update_particle_chunk!, src/chunks.jl:120-142.
-__device__ __forceinline__ int32_t cell_of(const SysParams &P, double x, double y, uint32_t *flag) {
-    double qr = exact_trunc_div((-y + P.bly) + P.H, P.ch);
-    double qc = exact_trunc_div(x - P.blx, P.cl);
-    if (!(fabs(qr) < 1e15) || !(fabs(qc) < 1e15)) {  // NaN / Inf / absurd
-        *flag |= MAVI_FLAG_NONFINITE;
-        return 0;
-    }
-    int64_t r = (int64_t)qr;  // 0-based (reference row - 1)
-    int64_t c = (int64_t)qc;
-    r -= (r == P.rows) ? 1 : 0;  // the N+1 -> N clamp
-    c -= (c == P.cols) ? 1 : 0;
-    if (r < 0 || r >= P.rows || c < 0 || c >= P.cols) {
-        *flag |= MAVI_FLAG_OUT_OF_BOX;
-        r = r < 0 ? 0 : (r >= P.rows ? P.rows - 1 : r);  // keep memory access in bounds
-        c = c < 0 ? 0 : (c >= P.cols ? P.cols - 1 : c);
-    }
-    return (int32_t)(r * P.cols + c);
-}
-
-// Chunk of every particle and its arrival rank in that chunk.  Lanes of a
-// wave that land in the same chunk (most of them: the state is sorted by the
-// previous step's chunks) are counted by one atomic of their leader lane.
+// Chunk of every particle and its arrival rank in that chunk.
 __global__ void k_bin_count(SysParams P, const double2 *__restrict__ pos, int32_t *__restrict__ key,
                             int32_t *__restrict__ rank, int32_t *__restrict__ cnt, uint32_t *__restrict__ flags) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -125,125 +43,11 @@ __global__ void k_bin_count(SysParams P, const double2 *__restrict__ pos, int32_
         k = cell_of(P, p.x, p.y, &flag);
     }
     if (flag) atomicOr(flags, flag);
-    const unsigned lane = lane_id();
-    const unsigned long long lt = (1ull << lane) - 1ull;
-    unsigned long long remaining = __ballot(valid);
-    int my_leader = (int)lane, my_pos = 0, grp_n = 0;
-    while (remaining) {
-        const int leader = __ffsll((unsigned long long)remaining) - 1;
-        const int lkey = __shfl(k, leader, 64);
-        const unsigned long long grp = __ballot(valid && k == lkey);
-        if (valid && k == lkey) {
-            my_leader = leader;
-            my_pos = __popcll(grp & lt);
-        }
-        if ((int)lane == leader) grp_n = __popcll(grp);
-        remaining &= ~grp;
-    }
-    int base = 0;
-    if (valid && (int)lane == my_leader) base = atomicAdd(&cnt[k], grp_n);
-    base = __shfl(base, my_leader, 64);
+    const int32_t rk = wave_arrival_rank(valid, k, cnt);
     if (valid) {
         key[s] = k;
-        rank[s] = base + my_pos;
+        rank[s] = rk;
     }
-}
-
-constexpr int kScanThreads = 256;
-constexpr int kScanItems = 4;
-constexpr int kScanTile = kScanThreads * kScanItems;
-
-// Block-wide exclusive scan of one int per thread (256 threads, wave64).
-__device__ __forceinline__ int32_t block_excl_scan(int32_t v, int32_t *total) {
-    __shared__ int32_t wsum[kScanThreads / 64];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int32_t incl = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        int32_t t = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += t;
-    }
-    if (lane == 63) wsum[w] = incl;
-    __syncthreads();
-    int32_t woff = 0, tot = 0;
-#pragma unroll
-    for (int k = 0; k < kScanThreads / 64; k++) {
-        if (k < w) woff += wsum[k];
-        tot += wsum[k];
-    }
-    *total = tot;
-    return woff + incl - v;
-}
-
-// Exclusive scan of cnt[0..n) in three passes with a fixed cost: tile sums,
-// a one-workgroup scan of the tile sums, then each tile's local scan plus
-// its offset.  Tiles hold kScanTile2 = 4096 counts.
-constexpr int kScanItems2 = 16;
-constexpr int kScanTile2 = kScanThreads * kScanItems2;
-
-__global__ void k_tile_reduce(const int32_t *__restrict__ in, int32_t n, int32_t *__restrict__ tile_sum) {
-    const int base = blockIdx.x * kScanTile2;
-    int32_t sum = 0;
-#pragma unroll
-    for (int k = 0; k < kScanItems2; k++) {
-        const int i = base + k * kScanThreads + threadIdx.x;  // coalesced
-        sum += (i < n) ? in[i] : 0;
-    }
-    int32_t total;
-    (void)block_excl_scan(sum, &total);
-    if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
-}
-
-__global__ void k_tile_scan(int32_t *__restrict__ tile_sum, int32_t ntiles) {
-    __shared__ int32_t carry;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    for (int t0 = 0; t0 < ntiles; t0 += kScanThreads) {
-        const int i = t0 + threadIdx.x;
-        const int32_t v = (i < ntiles) ? tile_sum[i] : 0;
-        int32_t total;
-        const int32_t ex = block_excl_scan(v, &total);
-        const int32_t c = carry;
-        if (i < ntiles) tile_sum[i] = ex + c;
-        __syncthreads();
-        if (threadIdx.x == 0) carry = c + total;
-        __syncthreads();
-    }
-}
-
-__global__ void k_tile_apply(const int32_t *__restrict__ in, int32_t n, const int32_t *__restrict__ tile_sum,
-                             int32_t *__restrict__ out) {
-    const int base = blockIdx.x * kScanTile2 + threadIdx.x * kScanItems2;
-    int32_t v[kScanItems2];
-    int32_t sum = 0;
-#pragma unroll
-    for (int k = 0; k < kScanItems2; k++) {
-        v[k] = (base + k < n) ? in[base + k] : 0;
-        sum += v[k];
-    }
-    int32_t total;
-    int32_t off = block_excl_scan(sum, &total) + tile_sum[blockIdx.x];
-#pragma unroll
-    for (int k = 0; k < kScanItems2; k++) {
-        if (base + k < n) out[base + k] = off;
-        off += v[k];
-    }
-}
-
-// Place every particle at start[chunk] + arrival rank; reset the counts for
-// the next binning; advance the step counter of the device rotational noise.
-__global__ void k_scatter(SysParams P, const int32_t *__restrict__ key, const int32_t *__restrict__ rank,
-                          const int32_t *__restrict__ start, int32_t *__restrict__ src, int32_t *__restrict__ scell,
-                          int32_t *__restrict__ cnt, unsigned long long *__restrict__ counter) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < P.na) {
-        const int k = key[i];
-        const int t = start[k] + rank[i];
-        src[t] = i;
-        scell[t] = k;
-    }
-    if (i <= P.ncell) cnt[i] = 0;
-    if (i == 0) counter[0] += 1ull;
 }
 
 // One thread per new slot: the final slot inside the chunk is the number of
@@ -288,168 +92,6 @@ __global__ void k_export_chunks(SysParams P, const int32_t *__restrict__ start, 
 // K2/K3 pair forces with fused integrator epilogues
 // ---------------------------------------------------------------------------
 
-enum { MODE_FORCES = 0, MODE_VERLET1 = 1, MODE_VERLET2 = 2 };
-enum { POT_LJ = 0, POT_HT = 1, POT_RINGS = 2 };
-
-struct RingCtx {
-    const int32_t *ids;  // sorted slot -> id
-    const int32_t *rtype;
-    const RingType *types;
-    const RingInter *inter;
-    uint32_t nmax;
-    float inv_nmax;
-};
-
-__device__ __forceinline__ int32_t ring_of(uint32_t id, const RingCtx &R) {
-    // id / nmax via a float estimate and an exact integer correction
-    int32_t q = (int32_t)((float)id * R.inv_nmax);
-    int32_t r = (int32_t)id - q * (int32_t)R.nmax;
-    if (r < 0) q -= 1;
-    else if (r >= (int32_t)R.nmax) q += 1;
-    r = (int32_t)id - q * (int32_t)R.nmax;
-    if (r < 0) q -= 1;
-    else if (r >= (int32_t)R.nmax) q += 1;
-    return q;
-}
-
-template <int POT>
-struct PairAcc {
-    double fx = 0.0, fy = 0.0;
-    int32_t id_i = 0, ring_i = 0, type_i = 0;  // rings only
-
-    // id_j: particle id of the candidate (read by the rings interaction only)
-    __device__ __forceinline__ void add(const SysParams &P, const RingCtx &R, double dx, double dy, bool self_ok,
-                                        int id_j) {
-        const double r2 = dx * dx + dy * dy;  // sum(dr.^2): two roundings, no FMA
-        if constexpr (POT == POT_LJ) {
-            // fmod/dist = 4 eps (12 s^12/d^14 - 6 s^6/d^8) = inv^4 (c12 inv^3 - c6), inv = 1/d^2
-            const bool ok = self_ok && (r2 <= P.cut2);
-            const double inv = rcp_nr(r2);
-            const double inv3 = inv * inv * inv;
-            double fr = inv * inv3 * __builtin_fma(P.lj_c12, inv3, -P.lj_c6);
-            fr = ok ? fr : 0.0;
-            fx = __builtin_fma(fr, dx, fx);
-            fy = __builtin_fma(fr, dy, fy);
-        } else if constexpr (POT == POT_HT) {
-            // potential_force(::HarmTruncCfg), src/configs.jl:354-368, term by term
-            if (self_ok && r2 <= P.cut2) {
-                const double dist = sqrt(r2);
-                const double fmod = (r2 < P.eq2) ? -P.p0 * (dist / P.p2 - 1.0) : -P.p1 * (dist / P.p2 - 1.0);
-                const double q = fmod / dist;
-                fx += q * dx;
-                fy += q * dy;
-            }
-        } else {
-            // rings calc_interaction_force, src/rings/integration.jl:47-77
-            if (!self_ok) return;
-            const int32_t ring_j = ring_of((uint32_t)id_j, R);
-            const RingInter *ic = R.inter;
-            if (P.typed) ic += type_i + P.ntypes * R.rtype[ring_j];
-            if (r2 > ic->max2) return;
-            const bool same = ring_j == ring_i;
-            if (same) {
-                const int32_t diff = abs(id_i - id_j);
-                const int32_t np = R.types[type_i].np;
-                if (diff == 1 || diff == np - 1) return;
-            }
-            const double dist = sqrt(r2);
-            double fmod;
-            if (r2 < ic->eq2) fmod = -ic->k_rep * (dist / ic->dist_eq - 1.0);
-            else fmod = same ? 0.0 : -ic->k_atr * (dist / ic->dist_eq - 1.0);
-            const double q = fmod / dist;
-            fx += q * dx;
-            fy += q * dy;
-        }
-    }
-};
-
-// Candidates of one particle as up to three contiguous slot ranges, walked
-// as one loop (fewer idle lanes than one loop per range) with the next
-// position loaded before the current one is used.
-template <int POT, bool WRAP>
-__device__ __forceinline__ void scan3(const SysParams &P, const RingCtx &R, PairAcc<POT> &acc,
-                                      const double2 *__restrict__ pin, int s, double xi, double yi, int a0, int l0,
-                                      int a1, int l1, int a2, int l2) {
-    const int tot = l0 + l1 + l2;
-    if (tot <= 0) return;
-    const int b1 = l0, b2 = l0 + l1;
-    const int o0 = a0, o1 = a1 - l0, o2 = a2 - l0 - l1;
-    int j = (0 < b1) ? o0 : ((0 < b2) ? o1 : o2);
-    double2 pj = pin[j];
-    for (int k = 0; k < tot; k++) {
-        const int kn = k + 1;
-        const int jn = (kn < tot) ? kn + ((kn < b1) ? o0 : ((kn < b2) ? o1 : o2)) : j;
-        const double2 pn = pin[jn];
-        double dx = xi - pj.x, dy = yi - pj.y;
-        if constexpr (WRAP) {
-            dx = min_image(dx, P.L, P.halfL);
-            dy = min_image(dy, P.H, P.halfH);
-        }
-        acc.add(P, R, dx, dy, j != s, POT == POT_RINGS ? R.ids[j] : 0);
-        j = jn;
-        pj = pn;
-    }
-}
-
-// Two-phase form for finite-range interactions: a first pass over the
-// candidates only computes r^2 and keeps the in-range candidate numbers
-// (uint16, in LDS, column per lane); the FP64 force math then runs on the
-// kept candidates only, so lanes of a wave stay busy on pairs that matter.
-constexpr int kListCap = 32;
-
-template <int POT, bool WRAP>
-__device__ __forceinline__ bool scan3_listed(const SysParams &P, const RingCtx &R, PairAcc<POT> &acc,
-                                             const double2 *__restrict__ pin, int s, double xi, double yi, int a0,
-                                             int l0, int a1, int l1, int a2, int l2, double list2,
-                                             uint16_t *__restrict__ lst, int stride) {
-    const int tot = l0 + l1 + l2;
-    const int b1 = l0, b2 = l0 + l1;
-    const int o0 = a0, o1 = a1 - l0, o2 = a2 - l0 - l1;
-    int cnt = 0;
-    for (int k = 0; k < tot; k++) {
-        const int j = k + ((k < b1) ? o0 : ((k < b2) ? o1 : o2));
-        const double2 pj = pin[j];
-        double dx = xi - pj.x, dy = yi - pj.y;
-        if constexpr (WRAP) {
-            dx = min_image(dx, P.L, P.halfL);
-            dy = min_image(dy, P.H, P.halfH);
-        }
-        const double r2 = dx * dx + dy * dy;
-        if (j != s && r2 <= list2) {
-            if (cnt < kListCap) lst[cnt * stride] = (uint16_t)k;
-            cnt++;
-        }
-    }
-    const int m = cnt < kListCap ? cnt : kListCap;
-    for (int q = 0; q < m; q++) {
-        const int k = lst[q * stride];
-        const int j = k + ((k < b1) ? o0 : ((k < b2) ? o1 : o2));
-        const double2 pj = pin[j];
-        double dx = xi - pj.x, dy = yi - pj.y;
-        if constexpr (WRAP) {
-            dx = min_image(dx, P.L, P.halfL);
-            dy = min_image(dy, P.H, P.halfH);
-        }
-        acc.add(P, R, dx, dy, true, POT == POT_RINGS ? R.ids[j] : 0);
-    }
-    return cnt > kListCap;
-}
-
-template <int POT>
-__device__ __forceinline__ void scan_range(const SysParams &P, const RingCtx &R, PairAcc<POT> &acc,
-                                           const double2 *__restrict__ pin, int s, double xi, double yi, bool wrap,
-                                           int a, int b) {
-    for (int j = a; j < b; j++) {
-        const double2 pj = pin[j];
-        double dx = xi - pj.x, dy = yi - pj.y;
-        if (wrap) {
-            dx = min_image(dx, P.L, P.halfL);
-            dy = min_image(dy, P.H, P.halfH);
-        }
-        acc.add(P, R, dx, dy, j != s, POT == POT_RINGS ? R.ids[j] : 0);
-    }
-}
-
 template <int MODE, int POT, bool ALLPAIRS>
 __global__ void k_pair(SysParams P, RingCtx R, const double2 *__restrict__ pin, const int32_t *__restrict__ scell,
                        const int32_t *__restrict__ start, double2 *__restrict__ Pst, double2 *__restrict__ Vst,
@@ -462,114 +104,17 @@ __global__ void k_pair(SysParams P, RingCtx R, const double2 *__restrict__ pin, 
     const double2 pi = pin[sv];
     PairAcc<POT> acc;
     if constexpr (POT == POT_RINGS) {
-        acc.id_i = R.ids[sv];
+        acc.id_i = R.ids[sv] - R.id_base;
         acc.ring_i = ring_of((uint32_t)acc.id_i, R);
         acc.type_i = P.typed ? R.rtype[acc.ring_i] : 0;
     }
     if constexpr (ALLPAIRS) {
         if (valid) scan_range<POT>(P, R, acc, pin, s, pi.x, pi.y, P.periodic != 0, 0, P.na);
     } else {
-        const int c = scell[sv];
-        const int row = c / P.cols, col = c - row * P.cols;
-        const bool edge_col = col == 0 || col == P.cols - 1;
-        const bool wrap = valid && P.periodic && (row == 0 || row == P.rows - 1 || edge_col);
-        const bool split = valid && P.periodic && edge_col;
-        if (!__any(split)) {
-            // every lane: one slot range per chunk row
-            const int c0 = col > 0 ? col - 1 : 0, c1 = col < P.cols - 1 ? col + 1 : P.cols - 1;
-            auto row_range = [&](int d, int &a, int &l) {
-                int r = row + d;
-                bool ok = valid;
-                if (P.periodic) r = (r < 0) ? r + P.rows : (r >= P.rows ? r - P.rows : r);
-                else ok = ok && r >= 0 && r < P.rows;
-                const int base = (ok ? r : 0) * P.cols;
-                a = ok ? start[base + c0] : 0;
-                l = ok ? start[base + c1 + 1] - a : 0;
-            };
-            int a0, l0, a1, l1, a2, l2;
-            row_range(-1, a0, l0);
-            row_range(0, a1, l1);
-            row_range(1, a2, l2);
-            const bool wrapw = __any(wrap);
-            // list entries are 16-bit candidate numbers
-            bool direct = !(list2 < INFINITY) || __any(l0 + l1 + l2 > 65535);
-            if (!direct) {
-                uint16_t *lst = s_list + threadIdx.x;
-                const bool over =
-                    wrapw ? scan3_listed<POT, true>(P, R, acc, pin, s, pi.x, pi.y, a0, l0, a1, l1, a2, l2, list2, lst,
-                                                    blockDim.x)
-                          : scan3_listed<POT, false>(P, R, acc, pin, s, pi.x, pi.y, a0, l0, a1, l1, a2, l2, list2, lst,
-                                                     blockDim.x);
-                if (__any(over)) {  // a lane had more in-range candidates than the list holds
-                    acc.fx = 0.0;
-                    acc.fy = 0.0;
-                    direct = true;
-                }
-            }
-            if (direct) {
-                if (wrapw) scan3<POT, true>(P, R, acc, pin, s, pi.x, pi.y, a0, l0, a1, l1, a2, l2);
-                else scan3<POT, false>(P, R, acc, pin, s, pi.x, pi.y, a0, l0, a1, l1, a2, l2);
-            }
-        } else if (valid) {
-            // waves touching the periodic left/right edge: split row windows
-            for (int d = -1; d <= 1; d++) {
-                int r = row + d;
-                r = (r < 0) ? r + P.rows : (r >= P.rows ? r - P.rows : r);
-                const int base = r * P.cols;
-                int a0, a1, b0 = 0, b1 = 0;
-                if (col == 0) {
-                    a0 = start[base];
-                    a1 = start[base + 2];
-                    b0 = start[base + P.cols - 1];
-                    b1 = start[base + P.cols];
-                } else if (col == P.cols - 1) {
-                    a0 = start[base + P.cols - 2];
-                    a1 = start[base + P.cols];
-                    b0 = start[base];
-                    b1 = start[base + 1];
-                } else {
-                    a0 = start[base + col - 1];
-                    a1 = start[base + col + 2];
-                }
-                scan_range<POT>(P, R, acc, pin, s, pi.x, pi.y, wrap, a0, a1);
-                if (b1 > b0) scan_range<POT>(P, R, acc, pin, s, pi.x, pi.y, wrap, b0, b1);
-            }
-        }
+        pair_gather_chunks<POT>(P, R, acc, pin, scell[sv], start, s, valid, pi, list2, s_list);
     }
     if (!valid) return;
-    const double fx = acc.fx, fy = acc.fy;
-    if constexpr (MODE == MODE_FORCES) {
-        Fout[s] = make_double2(fx, fy);
-    } else if constexpr (MODE == MODE_VERLET1) {
-        // update_verlet!: pos += vel*dt + forces*term (src/integration.jl:424)
-        const double2 v = Vst[s];
-        P2[s] = make_double2(pi.x + (v.x * P.dt + fx * P.term), pi.y + (v.y * P.dt + fy * P.term));
-        F1[s] = make_double2(fx, fy);
-    } else {
-        // vel += dt/2 * (forces + old_forces) (src/integration.jl:430), then walls!
-        const double2 v = Vst[s];
-        const double2 fo = F1[s];
-        double vx = v.x + P.hdt * (fx + fo.x);
-        double vy = v.y + P.hdt * (fy + fo.y);
-        double x = pi.x, y = pi.y;
-        if (P.periodic) {
-            // src/integration.jl:309-324
-            const double dxc = x - (P.blx + P.halfL), dyc = y - (P.bly + P.halfH);
-            const bool ox = fabs(dxc) > P.halfL, oy = fabs(dyc) > P.halfH;
-            if (ox) x = x - copysign(P.halfL * 2.0, dxc);
-            if (oy) y = y - copysign(P.halfH * 2.0, dyc);
-        } else {
-            // src/integration.jl:271-285
-            const double rx = x - P.blx, ry = y - P.bly;
-            const bool ox = ((rx + P.radius) > P.L) || ((rx - P.radius) < 0.0);
-            const bool oy = ((ry + P.radius) > P.H) || ((ry - P.radius) < 0.0);
-            if (ox) vx = -vx;
-            if (oy) vy = -vy;
-        }
-        Pst[s] = make_double2(x, y);
-        Vst[s] = make_double2(vx, vy);
-        Fout[s] = make_double2(fx, fy);
-    }
+    pair_epilogue<MODE>(P, s, pi, acc.fx, acc.fy, Pst, Vst, Fout, P2, F1);
 }
 
 // ---------------------------------------------------------------------------
@@ -702,7 +247,7 @@ __global__ __launch_bounds__(64 * kSegWaves) void k_pair_seg(
     if (staged) {
         for (int q = 0; q < gcount; q++) {
             stg[excl + q] = pin[gstart + q];
-            if constexpr (POT == POT_RINGS) s_sid[w][excl + q] = R.ids[gstart + q];
+            if constexpr (POT == POT_RINGS) s_sid[w][excl + q] = R.ids[gstart + q] - R.id_base;
         }
     }
     wave_sync();
@@ -721,7 +266,7 @@ __global__ __launch_bounds__(64 * kSegWaves) void k_pair_seg(
         const double2 pi = pin[sv];
         PairAcc<POT> acc;
         if constexpr (POT == POT_RINGS) {
-            acc.id_i = R.ids[sv];
+            acc.id_i = R.ids[sv] - R.id_base;
             acc.ring_i = ring_of((uint32_t)acc.id_i, R);
             acc.type_i = P.typed ? R.rtype[acc.ring_i] : 0;
         }
@@ -789,67 +334,6 @@ __global__ __launch_bounds__(64 * kSegWaves) void k_pair_seg(
 // ---------------------------------------------------------------------------
 // K4 rings: one thread per ring
 // ---------------------------------------------------------------------------
-
-__device__ __forceinline__ double2 cdiff(const SysParams &P, double2 a, double2 b) {
-    double dx = a.x - b.x, dy = a.y - b.y;
-    if (P.periodic) {
-        dx = min_image(dx, P.L, P.halfL);
-        dy = min_image(dy, P.H, P.halfH);
-    }
-    return make_double2(dx, dy);
-}
-
-// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) -> one standard normal
-// through Box-Muller on two 53-bit uniforms.
-__device__ __forceinline__ void philox_round(uint32_t &c0, uint32_t &c1, uint32_t &c2, uint32_t &c3, uint32_t k0,
-                                             uint32_t k1) {
-    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
-    uint32_t hi0 = __umulhi(M0, c0), lo0 = M0 * c0;
-    uint32_t hi1 = __umulhi(M1, c2), lo1 = M1 * c2;
-    uint32_t n0 = hi1 ^ c1 ^ k0, n1 = lo1, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
-    c0 = n0;
-    c1 = n1;
-    c2 = n2;
-    c3 = n3;
-}
-
-__device__ double philox_normal(uint64_t seed, uint64_t step, uint32_t ring) {
-    uint32_t c0 = (uint32_t)step, c1 = (uint32_t)(step >> 32), c2 = ring, c3 = 0x6d617669u;
-    uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
-#pragma unroll
-    for (int r = 0; r < 10; r++) {
-        philox_round(c0, c1, c2, c3, k0, k1);
-        k0 += 0x9E3779B9u;
-        k1 += 0xBB67AE85u;
-    }
-    const double u1 = ((double)(((uint64_t)c0 << 21) ^ ((uint64_t)c1 >> 11)) + 1.0) * 0x1p-53;  // (0,1]
-    const double u2 = (double)((((uint64_t)c2 << 21) ^ ((uint64_t)c3 >> 11)) & ((1ull << 53) - 1)) * 0x1p-53;
-    return sqrt(-2.0 * log(u1)) * cospi(2.0 * u2);
-}
-
-// Unwrap (update_continuos_pos!, :118-136), centre of mass and shoelace area
-// of ring r from positions x(i) (calc_area, :103-116).
-template <typename PosFn>
-__device__ __forceinline__ double ring_geometry(const SysParams &P, int np, PosFn x, double2 *__restrict__ c,
-                                                double2 *cm) {
-    for (int i = 0; i < P.nmax; i++) c[i] = x(i);
-    if (P.periodic) {
-        for (int i = 1; i < np; i++) {
-            const double2 d = cdiff(P, x(i), x(i - 1));
-            c[i] = make_double2(c[i - 1].x + d.x, c[i - 1].y + d.y);
-        }
-    }
-    double sx = 0.0, sy = 0.0;
-    for (int i = 0; i < np; i++) {
-        sx += c[i].x;
-        sy += c[i].y;
-    }
-    *cm = make_double2(sx / (double)np, sy / (double)np);
-    double area = 0.0;
-    for (int i = 0; i < np - 1; i++) area += c[i].x * c[i + 1].y - c[i].y * c[i + 1].x;
-    area += c[np - 1].x * c[0].y - c[np - 1].y * c[0].x;
-    return area / 2.0;
-}
 
 template <bool UPDATE>
 __global__ void k_rings(SysParams P, const int32_t *__restrict__ rtype, const RingType *__restrict__ types,
@@ -952,146 +436,6 @@ __global__ void k_rings(SysParams P, const int32_t *__restrict__ rtype, const Ri
     pol[r] = theta + d_theta;
 }
 
-// One ring per 32 lanes (lane = particle index inside the ring).  Every
-// reduction the reference does as a sequential loop (unwrap, centre of mass,
-// shoelace area, mean velocity) is done here as the same left-to-right loop
-// over values broadcast from the owning lanes, so the result is bit-identical
-// to the one-thread-per-ring form.
-template <bool UPDATE>
-__global__ __launch_bounds__(256) void k_rings_lanes(
-    SysParams P, const int32_t *__restrict__ rtype, const RingType *__restrict__ types,
-    const int32_t *__restrict__ slot_of, double2 *__restrict__ Pst, double2 *__restrict__ F, double *__restrict__ pol,
-    double *__restrict__ areas, double2 *__restrict__ cms, double2 *__restrict__ cms_next, double2 *__restrict__ cpos,
-    const double *__restrict__ noise, uint64_t seed, const unsigned long long *__restrict__ counter) {
-    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-    const int r = gt >> 5;
-    const int i = gt & 31;
-    const bool ring_ok = r < P.nrings;
-    const int rr = ring_ok ? r : 0;
-    const RingType T = types[P.typed ? rtype[rr] : 0];
-    const int np = T.np;
-    const bool act = ring_ok && i < np;
-    const int64_t base = (int64_t)rr * P.nmax;
-    const int slot = act ? slot_of[base + i] : 0;
-    const double2 x = act ? Pst[slot] : make_double2(0.0, 0.0);
-    auto bc = [](double v, int src) { return __shfl(v, src, 32); };
-
-    if constexpr (UPDATE) {
-        if (ring_ok && i == 0) cms[r] = cms_next[r];
-    }
-    // neighbours along the ring
-    const int inext = (i == np - 1) ? 0 : i + 1;
-    const int iprev = (i == 0) ? np - 1 : i - 1;
-    const double2 xn = make_double2(bc(x.x, inext), bc(x.y, inext));
-    const double2 xp = make_double2(bc(x.x, iprev), bc(x.y, iprev));
-
-    // update_continuos_pos!: c_0 = x_0, c_i = c_{i-1} + calc_diff(x_i, x_{i-1})
-    const double2 d = (i == 0) ? x : cdiff(P, x, xp);
-    double cx = 0.0, cy = 0.0;
-    for (int k = 0; k < np; k++) {
-        const double dx = bc(d.x, k), dy = bc(d.y, k);
-        if (k == 0) {
-            cx = dx;
-            cy = dy;
-        } else if (k <= i) {
-            cx = cx + dx;
-            cy = cy + dy;
-        }
-    }
-    if (!P.periodic) {  // update_continuos_pos! is a no-op: continuous = stored positions
-        cx = x.x;
-        cy = x.y;
-    }
-    if (act) cpos[base + i] = make_double2(cx, cy);
-    // centre of mass and shoelace area, sums in ring order
-    const double cnx = bc(cx, inext), cny = bc(cy, inext);
-    const double term = cx * cny - cy * cnx;
-    double sx = 0.0, sy = 0.0, area = 0.0;
-    for (int k = 0; k < np; k++) {
-        sx += bc(cx, k);
-        sy += bc(cy, k);
-        area += bc(term, k);
-    }
-    area = area / 2.0;
-    if (ring_ok && i == 0) {
-        cms_next[r] = make_double2(sx / (double)np, sy / (double)np);
-        areas[r] = area;
-    }
-
-    // springs: lane i holds f_i = spring(i, i+1); particle i gets +f_1 then
-    // -f_n (i = 0) or -f_{i-1} then +f_i, then the area term
-    const double2 ds = cdiff(P, x, xn);
-    const double dist = sqrt(ds.x * ds.x + ds.y * ds.y);
-    const double q = -T.kspring * (dist - T.lspring) / dist;
-    const double2 fi = make_double2(q * ds.x, q * ds.y);
-    const double2 fprev = make_double2(bc(fi.x, iprev), bc(fi.y, iprev));
-    double area0 = (double)np * T.lspring / T.p0;
-    area0 = area0 * area0;
-    const double fa = T.karea * (area - area0);
-    const double2 da = cdiff(P, xn, xp);
-    double2 Fi = act ? F[slot] : make_double2(0.0, 0.0);
-    if (i == 0) {
-        Fi.x += fi.x;
-        Fi.y += fi.y;
-        Fi.x -= fprev.x;
-        Fi.y -= fprev.y;
-    } else {
-        Fi.x -= fprev.x;
-        Fi.y -= fprev.y;
-        Fi.x += fi.x;
-        Fi.y += fi.y;
-    }
-    Fi.x -= fa * (da.y / 2.0);
-    Fi.y -= fa * (-da.x / 2.0);
-    if (act) F[slot] = Fi;
-    if constexpr (!UPDATE) return;
-
-    // update!: v = vo pol + mu F, x += v dt (then walls!), polarisation relaxation
-    const double theta = ring_ok ? pol[rr] : 0.0;
-    double sn, cs;
-    sincos(theta, &sn, &cs);
-    const double vx = T.vo * cs + T.mu * Fi.x;
-    const double vy = T.vo * sn + T.mu * Fi.y;
-    double vcx = 0.0, vcy = 0.0;
-    for (int k = 0; k < np; k++) {
-        vcx += bc(vx, k);
-        vcy += bc(vy, k);
-    }
-    double px = x.x + vx * P.dt, py = x.y + vy * P.dt;
-    if (P.periodic) {
-        const double dxc = px - (P.blx + P.halfL), dyc = py - (P.bly + P.halfH);
-        const bool ox = fabs(dxc) > P.halfL, oy = fabs(dyc) > P.halfH;
-        if (ox) px = px - copysign(P.halfL * 2.0, dxc);
-        if (oy) py = py - copysign(P.halfH * 2.0, dyc);
-    }
-    if (act) Pst[slot] = make_double2(px, py);
-    if (ring_ok && i == 0) {
-        vcx = vcx / (double)np;
-        vcy = vcy / (double)np;
-        const double speed = sqrt(vcx * vcx + vcy * vcy);
-        double cr = 0.0;
-        if (speed != 0.0) {
-            cr = (cs * vcy - sn * vcx) / speed;
-            if (fabs(cr) > 1.0) cr = cr > 0.0 ? 1.0 : -1.0;
-        }
-        const double z = noise ? noise[r] : (T.rot != 0.0 ? philox_normal(seed, counter[0], (uint32_t)r) : 0.0);
-        const double d_theta = 1.0 / T.relax * asin(cr) * P.dt + sqrt(2.0 * T.rot * P.dt) * z;
-        pol[r] = theta + d_theta;
-    }
-}
-
-__global__ void k_rings_init(SysParams P, const int32_t *__restrict__ rtype, const RingType *__restrict__ types,
-                             const double2 *__restrict__ pos_id, double2 *__restrict__ cms,
-                             double2 *__restrict__ cms_next, double2 *__restrict__ cpos, double *__restrict__ areas) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= P.nrings) return;
-    const int np = types[P.typed ? rtype[r] : 0].np;
-    const double2 *x = pos_id + (int64_t)r * P.nmax;
-    double2 cm;
-    areas[r] = ring_geometry(P, np, [&](int i) { return x[i]; }, cpos + (int64_t)r * P.nmax, &cm);
-    cms[r] = cms_next[r] = cm;
-}
-
 // ---------------------------------------------------------------------------
 // K7 observables: per-block partial sums (fixed reduction tree), summed on
 // the host in block order, so the result is reproducible run to run.
@@ -1145,11 +489,11 @@ __global__ void k_observables(SysParams P, const double2 *__restrict__ pos, cons
     }
 }
 
-inline int nblk(int64_t n, int t) { return (int)((n + t - 1) / t); }
 
 RingCtx ring_ctx(mavi_sys *s) {
     RingCtx R{};
     R.ids = s->ID_[s->cur];
+    R.id_base = 0;
     R.rtype = s->rtype;
     R.types = s->types;
     R.inter = s->inter;
@@ -1241,7 +585,7 @@ hipError_t launch_iota_state(mavi_sys *s, hipStream_t st) {
     const int c = s->cur;
     const bool has_vel = s->P.potential != MAVI_POT_RINGS;
     hipLaunchKernelGGL(k_iota_state, dim3(nblk(s->P.na, 256)), dim3(256), 0, st, s->P.na, s->active_ids, s->pos_id,
-                       has_vel ? s->vel_id : nullptr, s->P_[c], has_vel ? s->V_[c] : nullptr, s->ID_[c], s->slot_of);
+                       has_vel ? s->vel_id : nullptr, s->P_[c], has_vel ? s->V_[c] : nullptr, s->ID_[c], s->slot_of, 0, 0);
     return hipGetLastError();
 }
 
@@ -1254,7 +598,7 @@ hipError_t launch_unpermute(mavi_sys *s, bool vel, bool force, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_bin(mavi_sys *s, hipStream_t st) {
+hipError_t launch_bin(mavi_sys *s, hipStream_t st, bool count_step) {
     const SysParams &P = s->P;
     if (P.cols == 0) return hipSuccess;  // chunks = nothing: the state order stays the active id order
     ProfScope ps(s, MAVI_PROF_BIN, st);
@@ -1268,7 +612,7 @@ hipError_t launch_bin(mavi_sys *s, hipStream_t st) {
     hipLaunchKernelGGL(k_tile_apply, dim3(ntiles), dim3(kScanThreads), 0, st, s->cnt, n, s->tile_sum, s->start);
     const int grid = nblk(P.na > n ? P.na : n, 256);
     hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(256), 0, st, P, s->key, s->rank, s->start, s->src, s->scell,
-                       s->cnt, s->counter);
+                       s->cnt, count_step ? s->counter : nullptr);
     const bool has_vel = P.potential != MAVI_POT_RINGS;
     hipLaunchKernelGGL(k_finalize, dim3(nblk(P.na, 256)), dim3(256), 0, st, P, s->start, s->src, s->scell, s->P_[c],
                        has_vel ? s->V_[c] : nullptr, s->ID_[c], s->P_[nc], has_vel ? s->V_[nc] : nullptr, s->ID_[nc],
@@ -1290,7 +634,7 @@ hipError_t launch_forces_only(mavi_sys *s, hipStream_t st) {
 }
 
 hipError_t launch_verlet_step(mavi_sys *s, hipStream_t st) {
-    hipError_t e = launch_bin(s, st);
+    hipError_t e = launch_bin(s, st, true);
     if (e != hipSuccess) return e;
     {
         ProfScope ps(s, MAVI_PROF_FORCE, st);
@@ -1338,7 +682,7 @@ hipError_t launch_rings_forces(mavi_sys *s, hipStream_t st) {
 }
 
 hipError_t launch_rings_step(mavi_sys *s, const double *noise_dev, hipStream_t st) {
-    hipError_t e = launch_bin(s, st);
+    hipError_t e = launch_bin(s, st, true);
     if (e != hipSuccess) return e;
     if (s->P.cols == 0) {  // no binning kernel to advance the noise step counter
         hipLaunchKernelGGL(k_counter_inc, dim3(1), dim3(64), 0, st, s->counter);

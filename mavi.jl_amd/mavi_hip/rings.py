@@ -110,43 +110,52 @@ class RingsState:  # src/rings/states.jl:74-124
         return np.asarray(ids)
 
 
+def describe(state: RingsState, space_cfg: SpaceCfg, dynamic_cfg: RingsCfg, int_cfg, seed: int = 0):
+    """Descriptor of a rings system for the C ABI, with the checks of
+    RingsSystem(...) (src/rings/rings.jl:231-291).  Returns (SysDesc, Chunks |
+    None, tables): the descriptor points into ``tables``, which must outlive
+    the call that reads it."""
+    if dynamic_cfg.typed != (state.types is not None):
+        raise ValueError("DynamicCfg types and state.types disagree")
+    if not isinstance(space_cfg.wall_type, PeriodicWalls) or not isinstance(int_cfg.device, GPU):
+        raise _lib.UnsupportedOnGPU(_lib.MAVI_E_UNSUPPORTED,
+                                    "rings run on the GPU with PeriodicWalls rectangles")
+    ids = state.particle_ids()
+    ok, out_ids = check_inside(state.pos, space_cfg.geometry_cfg, ids)
+    if not ok:
+        raise ValueError(f"Particles with ids={list(out_ids + 1)} outside space.")
+    r = min(dynamic_cfg.particle_radii())  # minimum(particle_radius(dynamic_cfg)), src/systems.jl:26
+    chunks = None
+    if int_cfg.chunks_cfg is not None:
+        chunks = Chunks(int_cfg.chunks_cfg.num_cols, int_cfg.chunks_cfg.num_rows, space_cfg.geometry_cfg, r)
+    d = base_desc(len(state.pos), space_cfg, dynamic_cfg, int_cfg, chunks, r)
+    d.potential = _lib.MAVI_POT_RINGS
+    d.nmax, d.nrings, d.ntypes = state.nmax, state.num_rings, dynamic_cfg.num_types
+    np_types = np.atleast_1d(np.asarray(state.num_particles, dtype=np.int32))
+    tables = (
+        np.ascontiguousarray(np_types),
+        np.ascontiguousarray(dynamic_cfg.type_table()),
+        np.ascontiguousarray(dynamic_cfg.interaction_table()),
+        None if state.types is None else np.ascontiguousarray(state.types, dtype=np.int32),
+    )
+    import ctypes as C
+    d.type_np = tables[0].ctypes.data_as(C.POINTER(C.c_int32))
+    d.type_par = tables[1].ctypes.data_as(C.POINTER(C.c_double))
+    d.interaction = tables[2].ctypes.data_as(C.POINTER(C.c_double))
+    if tables[3] is not None:
+        d.ring_types = tables[3].ctypes.data_as(C.POINTER(C.c_int32))
+    d.seed = int(seed)
+    return d, chunks, tables
+
+
 class RingsSystem:
     """GPU-backed RingsSystem (src/rings/rings.jl:231-291)."""
 
     def __init__(self, *, state: RingsState, space_cfg: SpaceCfg, dynamic_cfg: RingsCfg, int_cfg,
                  time_info: TimeInfo | None = None, seed: int = 0, ctx: _lib.Context | None = None):
-        if dynamic_cfg.typed != (state.types is not None):
-            raise ValueError("DynamicCfg types and state.types disagree")
-        if not isinstance(space_cfg.wall_type, PeriodicWalls) or not isinstance(int_cfg.device, GPU):
-            raise _lib.UnsupportedOnGPU(_lib.MAVI_E_UNSUPPORTED,
-                                        "rings run on the GPU with PeriodicWalls rectangles")
-        ids = state.particle_ids()
-        ok, out_ids = check_inside(state.pos, space_cfg.geometry_cfg, ids)
-        if not ok:
-            raise ValueError(f"Particles with ids={list(out_ids + 1)} outside space.")
+        d, self.chunks, self._tables = describe(state, space_cfg, dynamic_cfg, int_cfg, seed)
         self.state, self.space_cfg, self.dynamic_cfg, self.int_cfg = state, space_cfg, dynamic_cfg, int_cfg
         self.time_info = time_info or TimeInfo()
-        r = min(dynamic_cfg.particle_radii())  # minimum(particle_radius(dynamic_cfg)), src/systems.jl:26
-        self.chunks = None
-        if int_cfg.chunks_cfg is not None:
-            self.chunks = Chunks(int_cfg.chunks_cfg.num_cols, int_cfg.chunks_cfg.num_rows, space_cfg.geometry_cfg, r)
-        d = base_desc(len(state.pos), space_cfg, dynamic_cfg, int_cfg, self.chunks, r)
-        d.potential = _lib.MAVI_POT_RINGS
-        d.nmax, d.nrings, d.ntypes = state.nmax, state.num_rings, dynamic_cfg.num_types
-        np_types = np.atleast_1d(np.asarray(state.num_particles, dtype=np.int32))
-        self._tables = (
-            np.ascontiguousarray(np_types),
-            np.ascontiguousarray(dynamic_cfg.type_table()),
-            np.ascontiguousarray(dynamic_cfg.interaction_table()),
-            None if state.types is None else np.ascontiguousarray(state.types, dtype=np.int32),
-        )
-        import ctypes as C
-        d.type_np = self._tables[0].ctypes.data_as(C.POINTER(C.c_int32))
-        d.type_par = self._tables[1].ctypes.data_as(C.POINTER(C.c_double))
-        d.interaction = self._tables[2].ctypes.data_as(C.POINTER(C.c_double))
-        if self._tables[3] is not None:
-            d.ring_types = self._tables[3].ctypes.data_as(C.POINTER(C.c_int32))
-        d.seed = int(seed)
         self.ctx = ctx or _lib.Context.for_device(int_cfg.device.device)
         self.dev = _lib.DeviceSystem(self.ctx, d, keep=self._tables)
         self.forces = np.zeros_like(state.pos)

@@ -104,41 +104,48 @@ def base_desc(n, space_cfg, dynamic_cfg, int_cfg, chunks: Chunks | None, radius)
     return d
 
 
+def describe(state: SecondLawState, space_cfg: SpaceCfg, dynamic_cfg, int_cfg):
+    """Descriptor of a plain system for the C ABI, with the checks of
+    System(...) (src/systems.jl:73-114): particles inside the geometry, a
+    GPU-capable potential and walls.  Returns (SysDesc, Chunks | None)."""
+    ok, out_ids = check_inside(state.pos, space_cfg.geometry_cfg)
+    if not ok:
+        raise ValueError(f"Particles with ids={list(out_ids + 1)} outside space.")
+    kind = gpu_kind(dynamic_cfg)
+    wall = wall_kind(space_cfg)
+    if kind is None or wall is None or not isinstance(int_cfg.device, GPU):
+        raise _lib.UnsupportedOnGPU(
+            _lib.MAVI_E_UNSUPPORTED,
+            f"{type(dynamic_cfg).__name__} / {type(space_cfg.wall_type).__name__} / "
+            f"{type(int_cfg.device).__name__} is not on the GPU path; Mavi's own CPU "
+            "methods run it (the Julia integration dispatches there)",
+        )
+    r = particle_radius(dynamic_cfg)
+    chunks = None
+    if int_cfg.chunks_cfg is not None:
+        chunks = Chunks(int_cfg.chunks_cfg.num_cols, int_cfg.chunks_cfg.num_rows, space_cfg.geometry_cfg, r)
+    d = base_desc(len(state.pos), space_cfg, dynamic_cfg, int_cfg, chunks, r)
+    d.potential = kind
+    if kind == _lib.MAVI_POT_LJ:
+        d.pot[0], d.pot[1] = dynamic_cfg.sigma, dynamic_cfg.epsilon
+        d.pot[2] = dynamic_cfg.cutoff if dynamic_cfg.cutoff else 0.0
+    else:
+        d.pot[0], d.pot[1] = dynamic_cfg.k_rep, dynamic_cfg.k_atr
+        d.pot[2], d.pot[3] = dynamic_cfg.dist_eq, dynamic_cfg.dist_max
+    return d, chunks
+
+
 class System:
     """GPU-backed System (src/systems.jl:45-114)."""
 
     def __init__(self, *, state: SecondLawState, space_cfg: SpaceCfg, dynamic_cfg, int_cfg,
                  time_info: TimeInfo | None = None, ctx: _lib.Context | None = None):
-        ok, out_ids = check_inside(state.pos, space_cfg.geometry_cfg)
-        if not ok:
-            raise ValueError(f"Particles with ids={list(out_ids + 1)} outside space.")
-        kind = gpu_kind(dynamic_cfg)
-        wall = wall_kind(space_cfg)
-        if kind is None or wall is None or not isinstance(int_cfg.device, GPU):
-            raise _lib.UnsupportedOnGPU(
-                _lib.MAVI_E_UNSUPPORTED,
-                f"{type(dynamic_cfg).__name__} / {type(space_cfg.wall_type).__name__} / "
-                f"{type(int_cfg.device).__name__} is not on the GPU path; Mavi's own CPU "
-                "methods run it (the Julia integration dispatches there)",
-            )
+        d, self.chunks = describe(state, space_cfg, dynamic_cfg, int_cfg)
         self.state = state
         self.space_cfg = space_cfg
         self.dynamic_cfg = dynamic_cfg
         self.int_cfg = int_cfg
         self.time_info = time_info or TimeInfo()
-        r = particle_radius(dynamic_cfg)
-        self.chunks = None
-        if int_cfg.chunks_cfg is not None:
-            self.chunks = Chunks(int_cfg.chunks_cfg.num_cols, int_cfg.chunks_cfg.num_rows,
-                                 space_cfg.geometry_cfg, r)
-        d = base_desc(len(state.pos), space_cfg, dynamic_cfg, int_cfg, self.chunks, r)
-        d.potential = kind
-        if kind == _lib.MAVI_POT_LJ:
-            d.pot[0], d.pot[1] = dynamic_cfg.sigma, dynamic_cfg.epsilon
-            d.pot[2] = dynamic_cfg.cutoff if dynamic_cfg.cutoff else 0.0
-        else:
-            d.pot[0], d.pot[1] = dynamic_cfg.k_rep, dynamic_cfg.k_atr
-            d.pot[2], d.pot[3] = dynamic_cfg.dist_eq, dynamic_cfg.dist_max
         self.ctx = ctx or _lib.Context.for_device(int_cfg.device.device)
         self.dev = _lib.DeviceSystem(self.ctx, d)
         self.forces = np.zeros_like(state.pos)

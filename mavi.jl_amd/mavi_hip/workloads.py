@@ -19,6 +19,11 @@ and the CPU oracle.
                      interaction radius 1 (dist_eq = dist_max = 1), two types
                      50/50 (seed 2) whose target areas are 1.0 and 0.9 of the
                      regular n-gon area, v0 = 1, D_r = 0.1, dt = 1e-3
+  batch_ensemble     512 independent replicas, particle counts log-uniform
+                     in [1000, 64000] (seed 3); even replicas are lj_periodic
+                     gases (nside = round(sqrt(N)), seed 1000 + i), odd ones
+                     rings_mixture systems (round(N / 20) rings of 20,
+                     seed 2000 + i, device noise seed 3000 + i)
 """
 from __future__ import annotations
 
@@ -94,3 +99,19 @@ def rings_mixture(nrings=40000, nper=20, phi=0.8, seed=2, rot_diff=0.1, k_area=1
     space = SpaceCfg(wall_type=PeriodicWalls(), geometry_cfg=RectangleCfg(length=L, height=L))
     int_cfg = IntCfg(dt=1e-3, chunks_cfg=ChunksCfg(num_cols=cols, num_rows=cols), device=GPU())
     return state, space, cfg, int_cfg
+
+
+def batch_ensemble(nrep=512, nmin=1000, nmax=64000, seed=3):
+    """Replica list [(kind, case, noise_seed)], kind "plain" with case =
+    lj_periodic(...) or "rings" with case = rings_mixture(...)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(math.log(nmin), math.log(nmax), size=nrep))
+    out = []
+    for i, n in enumerate(sizes):
+        if i % 2 == 0:
+            nside = max(8, int(round(math.sqrt(n))))
+            out.append(("plain", lj_periodic(nside=nside, density=0.7, seed=1000 + i, cutoff=True), 0))
+        else:
+            nr = max(2, int(round(n / 20)))
+            out.append(("rings", rings_mixture(nrings=nr, seed=2000 + i), 3000 + i))
+    return out
