@@ -219,6 +219,46 @@ int mavi_batch_prof_get(mavi_batch *batch, int which, double *ms_total, int64_t 
  * {wg_plain, list_plain, wg_rings, list_rings}. */
 int mavi_batch_autotune(mavi_batch *batch, int32_t trial_steps, double *best_us, int32_t *shape);
 
+/* Learned propagator: network --------------------------------------------
+ * No counterpart in the reference (an addition beside the plain path).  An
+ * MLP MAVI_MLP_IN -> 128 -> 128 -> 128 -> MAVI_MLP_OUT with GELU; BF16
+ * weights and activations, FP32 accumulation on the matrix cores.  Features
+ * and weights cross the boundary as BF16 bit patterns (uint16_t); parameters
+ * as the FP32 master copy, layout W0^T[128][160], b0[128], W1^T[128][128],
+ * b1, W2^T[128][128], b2, W3^T[2][128], b3[2] (W^T: out x in, row-major). */
+#define MAVI_MLP_IN 160
+#define MAVI_MLP_HIDDEN 128
+#define MAVI_MLP_OUT 2
+#define MAVI_MLP_NPARAMS 53890
+typedef struct mavi_mlp mavi_mlp;
+int mavi_mlp_create(mavi_ctx *ctx, uint64_t seed, mavi_mlp **out);
+int mavi_mlp_destroy(mavi_mlp *mlp);
+int mavi_mlp_num_params(void);
+int mavi_mlp_get_params(mavi_mlp *mlp, float *params);
+int mavi_mlp_set_params(mavi_mlp *mlp, const float *params);
+/* y[m][2] = MLP(x[m][160]); fused = 1: the inference kernel (weights in
+ * LDS), 0: the training forward (GEMM chain). */
+int mavi_mlp_forward(mavi_mlp *mlp, int64_t m, const uint16_t *x, float *y, int32_t fused);
+/* Loss L = 1/(2m) sum (y - t)^2 and its gradient w.r.t. every parameter
+ * (MAVI_MLP_NPARAMS floats, parameter layout); either output may be NULL. */
+int mavi_mlp_loss_grad(mavi_mlp *mlp, int64_t m, const uint16_t *x, const float *t, double *loss, float *grad);
+/* Device-resident training set: rows of features and targets. */
+int mavi_mlp_dataset_clear(mavi_mlp *mlp);
+int mavi_mlp_dataset_append(mavi_mlp *mlp, int64_t n, const uint16_t *x, const float *t);
+int mavi_mlp_dataset_size(mavi_mlp *mlp, int64_t *n);
+int mavi_mlp_dataset_get(mavi_mlp *mlp, int64_t first, int64_t n, uint16_t *x, float *t);
+/* steps x (minibatch of `batch` rows drawn with replacement from the data
+ * set by (seed, step), forward + backward, Adam with learning rate lr);
+ * losses (nullable, steps floats) receives each step's minibatch loss. */
+int mavi_mlp_train(mavi_mlp *mlp, int64_t steps, int32_t batch, float lr, uint64_t seed, float *losses);
+/* Test hook for the matrix-core tiles: c[m][n] = sum_k A(m,k) B(k,n) in FP32
+ * with A = a[m][k] (ta = 0) or a[k][m] (ta = 1), B = b[k][n] (tb = 0) or
+ * b[n][k] (tb = 1), all BF16 bit patterns. */
+int mavi_mlp_gemm_check(mavi_ctx *ctx, int32_t m, int32_t n, int32_t k, int32_t ta, int32_t tb, const uint16_t *a,
+                        const uint16_t *b, float *c);
+/* Average time of the inference kernel over m device-resident rows. */
+int mavi_mlp_time_forward(mavi_mlp *mlp, int64_t m, int32_t reps, double *us_per_call);
+
 #ifdef __cplusplus
 }
 #endif
