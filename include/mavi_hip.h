@@ -259,6 +259,42 @@ int mavi_mlp_gemm_check(mavi_ctx *ctx, int32_t m, int32_t n, int32_t k, int32_t 
 /* Average time of the inference kernel over m device-resident rows. */
 int mavi_mlp_time_forward(mavi_mlp *mlp, int64_t m, int32_t reps, double *us_per_call);
 
+/* Learned propagator: stepping ----------------------------------------------
+ * No counterpart in the reference.  Binds a rings system (periodic, chunks
+ * >= r_feat wide, full rings with nmax <= 32) and a network.  One learned
+ * step stands for k Rings step! calls: features of every particle in its
+ * ring's polarisation frame (32 self features, 32 nearest neighbours within
+ * r_feat), displacement = k dt (v0 p + mu F) + target_scale * R^T MLP(x),
+ * gated by displacement, overlap and ring-energy checks; a rejected step is
+ * replaced by k plain FP64 steps.  Feature rows are in particle-id order. */
+typedef struct {
+    int32_t k;             /* plain steps one learned step stands for */
+    int32_t reserved;
+    double r_feat;         /* neighbour radius of the features */
+    double target_scale;   /* s_t: network output unit of displacement */
+    double d_min;          /* overlap gate distance (<= 0: half the smallest dist_eq) */
+    double max_disp;       /* displacement gate (<= 0: (r_feat - d_min)/2, the largest allowed) */
+    double e_tol;          /* per-ring spring + area energy rise gate (<= 0: 1.0) */
+    uint64_t seed;         /* rotational-noise stream of accepted learned steps */
+} mavi_learned_desc;
+typedef struct mavi_learned mavi_learned;
+/* mlp may be NULL: only the network-free mode of mavi_learned_run is then available. */
+int mavi_learned_create(mavi_sys *sys, mavi_mlp *mlp, const mavi_learned_desc *desc, mavi_learned **out);
+int mavi_learned_destroy(mavi_learned *learned);
+/* Bin, forces!, features at the current state: x (n x 160 BF16, nullable) and
+ * the explicit k dt displacement (n x 2, nullable); the staging copies read
+ * by mavi_sys_download / mavi_get_forces / mavi_rings_info match them. */
+int mavi_learned_features(mavi_learned *learned, uint16_t *x, double *euler);
+/* Training data from the plain method: nsnap x (features, k plain steps with
+ * the device noise, targets), rows_per_snap rows per snapshot (>= n: every
+ * particle in id order, else drawn by (seed, snapshot, row)) appended to the
+ * network's data set, then `stride` further plain steps. */
+int mavi_learned_collect(mavi_learned *learned, int32_t nsnap, int32_t stride, int32_t rows_per_snap, uint64_t seed);
+/* nsteps learned steps (each k plain-step equivalents); use_net = 0 runs the
+ * same gated step without the network correction.  *accepted (nullable)
+ * receives how many passed the gate. */
+int mavi_learned_run(mavi_learned *learned, int64_t nsteps, int32_t use_net, int64_t *accepted);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,11 +1,12 @@
 // mavi_mlp.hip -- the learned propagator's network: an MLP 160 -> 128 -> 128
-// -> 128 -> 2 with GELU, BF16 weights and activations, FP32 accumulation on
+// -> 128 -> 2 with GELU (sigmoid form), BF16 weights and activations, FP32 accumulation on
 // the gfx950 matrix cores (v_mfma_f32_16x16x32_bf16).
 //
 //   inference  k_mlp_fused: one persistent kernel, all weights resident in LDS
-//              (150 KB), each wavefront runs 32-row tiles through the four
-//              layers; activations go through a per-wave LDS tile between
-//              layers (C layout of one layer -> A layout of the next)
+//              (117 KB), 8 wavefronts per CU (two per SIMD, so one wave's
+//              activation epilogue overlaps another's MFMAs), each running
+//              16-row tiles through the four layers; activations go through
+//              a per-wave LDS tile between layers
 //   training   k_gemm<TA, TB, EPI>: LDS-staged 64x64x32 tiles with optional
 //              transposed operands and fused epilogues (bias + GELU keeping
 //              the pre-activation, GELU backward, split-K partials), plus
@@ -29,20 +30,18 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// Round to nearest even (NaN stays NaN).
-__device__ __forceinline__ uint16_t f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
+// Round to nearest even (v_cvt_pk_bf16_f32).
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 
-// GELU(z) = z Phi(z) and its derivative Phi(z) + z phi(z).
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// GELU in its sigmoid form, g(z) = z sigma(1.702 z) (Hendrycks & Gimpel):
+// one hardware exp and one reciprocal per activation (a few ulp in FP32, far
+// below the BF16 rounding that follows).  g'(z) = s + 1.702 z s (1 - s).
+constexpr float kGs = 1.702f;
+__device__ __forceinline__ float sig(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-kGs * z)); }
+__device__ __forceinline__ float gelu(float z) { return z * sig(z); }
 __device__ __forceinline__ float gelu_grad(float z) {
-    const float cdf = 0.5f * (1.0f + erff(z * 0.70710678118654752f));
-    const float pdf = 0.39894228040143268f * expf(-0.5f * z * z);
-    return cdf + z * pdf;
+    const float s = sig(z);
+    return s + kGs * z * s * (1.0f - s);
 }
 
 __device__ __forceinline__ bf16x8 ld8(const uint16_t *p) {
@@ -250,55 +249,83 @@ __global__ void k_sync_weights(const float *__restrict__ p, uint16_t *__restrict
 // ---------------------------------------------------------------------------
 constexpr int kP0 = kMlpIn + 8, kPH = kMlpHid + 8;
 constexpr int kLdsW0 = kMlpHid * kP0, kLdsW1 = kMlpHid * kPH, kLdsW3 = kMlpOutPad * kPH;
-constexpr int kFusedWaves = 4, kTileRows = 32;
+constexpr int kFusedWaves = 8, kTileRows = 16;
 constexpr size_t kFusedLds = sizeof(uint16_t) * (size_t)(kLdsW0 + 2 * kLdsW1 + kLdsW3) +
                              sizeof(float) * (size_t)(3 * kMlpHid + kMlpOutPad) +
                              sizeof(uint16_t) * (size_t)kFusedWaves * kTileRows * kPH;
 
-// One hidden layer for a 32-row tile: acc = A . W^T over K_in, then bias +
-// GELU into the wave's activation tile.  a_lds is the previous activation
-// tile (K_in = 128) or nullptr for the first layer reading X from global.
-template <int KIN, int PIN>
-__device__ __forceinline__ void fused_hidden(const uint16_t *__restrict__ X, int64_t row0, int M,
-                                             const uint16_t *a_lds, const uint16_t *W, const float *bias,
-                                             uint16_t *act, int lane) {
+// One hidden layer for a 16-row tile, computed transposed: out^T[n][m] =
+// sum_k W^T[n][k] act[m][k], weights as the first MFMA operand (16 output
+// features per block), activations as the second (the 16 rows).  A lane's
+// accumulator then holds 4 consecutive features of one row, written to the
+// wave's activation tile as one 8-byte store after bias + GELU.
+// FIRST: the input rows come from X in global memory.
+template <int KIN, int PIN, bool FIRST>
+__device__ __forceinline__ void fused_hidden(const uint16_t *__restrict__ X, int64_t row0, int M, uint16_t *act,
+                                             const uint16_t *W, const float *bias, int lane) {
     const int fr = lane & 15, fk = (lane >> 4) * 8;
-    f32x4 acc[2][8];
+    constexpr int KS = KIN / 32;
+    f32x4 acc[8];
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int f = 0; f < 8; f++) acc[f] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // the tile's input fragments and the first K step of weights in flight
+    // before the first MFMA; later weight fragments one K step ahead
+    bf16x8 h[KS];
 #pragma unroll
-        for (int j = 0; j < 8; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int ks = 0; ks < KIN / 32; ks++) {
-        bf16x8 a[2];
-#pragma unroll
-        for (int i = 0; i < 2; i++) {
-            if (a_lds) {
-                a[i] = ld8(a_lds + (16 * i + fr) * kPH + ks * 32 + fk);
-            } else {
-                const int64_t r = row0 + 16 * i + fr;
-                if (r < M) a[i] = ld8(X + r * kMlpIn + ks * 32 + fk);
-                else a[i] = __builtin_bit_cast(bf16x8, uint4{0u, 0u, 0u, 0u});
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const bf16x8 b = ld8(W + (16 * j + fr) * PIN + ks * 32 + fk);
-            acc[0][j] = mfma16(a[0], b, acc[0][j]);
-            acc[1][j] = mfma16(a[1], b, acc[1][j]);
+    for (int ks = 0; ks < KS; ks++) {
+        if constexpr (FIRST) {
+            const int64_t r = row0 + fr;
+            h[ks] = r < M ? ld8(X + r * kMlpIn + ks * 32 + fk) : __builtin_bit_cast(bf16x8, uint4{0u, 0u, 0u, 0u});
+        } else {
+            h[ks] = ld8(act + fr * kPH + ks * 32 + fk);
         }
     }
-    wave_sync();  // every lane has read the previous tile before it is overwritten
+    bf16x8 wf[2][8];
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int f = 0; f < 8; f++) wf[0][f] = ld8(W + (16 * f + fr) * PIN + fk);
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const int n = 16 * j + fr;
-            const float bn = bias[n];
+    for (int ks = 0; ks < KS; ks++) {
+        const int cb = ks & 1;
+        if (ks + 1 < KS) {
 #pragma unroll
-            for (int r = 0; r < 4; r++) act[(16 * i + (lane >> 4) * 4 + r) * kPH + n] = f2bf(gelu(acc[i][j][r] + bn));
+            for (int f = 0; f < 8; f++) wf[cb ^ 1][f] = ld8(W + (16 * f + fr) * PIN + (ks + 1) * 32 + fk);
         }
+#pragma unroll
+        for (int f = 0; f < 8; f++) acc[f] = mfma16(wf[cb][f], h[ks], acc[f]);
+        if (ks + 1 < KS) __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);
+    }
+    wave_sync();  // every lane has read the previous tile before it is overwritten
+    const int n4 = (lane >> 4) * 4;
+#pragma unroll
+    for (int f = 0; f < 8; f++) {
+        const float4 bb = *reinterpret_cast<const float4 *>(bias + 16 * f + n4);
+        const uint32_t lo = (uint32_t)f2bf(gelu(acc[f][0] + bb.x)) | ((uint32_t)f2bf(gelu(acc[f][1] + bb.y)) << 16);
+        const uint32_t hi = (uint32_t)f2bf(gelu(acc[f][2] + bb.z)) | ((uint32_t)f2bf(gelu(acc[f][3] + bb.w)) << 16);
+        *reinterpret_cast<uint2 *>(act + fr * kPH + 16 * f + n4) = make_uint2(lo, hi);
+    }
     wave_sync();
+}
+
+// Copy a [ROWS][COLS] BF16 matrix into LDS rows of PITCH elements, 16 bytes
+// per thread and 8 loads in flight (the chunk index is clamped, so every load
+// is unconditional; the clamped copies store the same value twice).
+template <int ROWS, int COLS, int PITCH>
+__device__ __forceinline__ void stage_rows(uint16_t *dst, const uint16_t *__restrict__ src, int t) {
+    constexpr int kQ = ROWS * COLS / 8, kStride = 64 * kFusedWaves, kU = 8;
+    for (int q0 = t; q0 < kQ; q0 += kU * kStride) {
+        uint4 v[kU];
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const int q = min(q0 + u * kStride, kQ - 1);
+            v[u] = *reinterpret_cast<const uint4 *>(src + (q / (COLS / 8)) * COLS + (q % (COLS / 8)) * 8);
+        }
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const int q = min(q0 + u * kStride, kQ - 1);
+            *reinterpret_cast<uint4 *>(dst + (q / (COLS / 8)) * PITCH + (q % (COLS / 8)) * 8) = v[u];
+        }
+    }
 }
 
 __global__ __launch_bounds__(64 * kFusedWaves) void k_mlp_fused(
@@ -313,20 +340,11 @@ __global__ __launch_bounds__(64 * kFusedWaves) void k_mlp_fused(
     float *B = reinterpret_cast<float *>(W3 + kLdsW3);  // b0 | b1 | b2 | b3 (16)
     uint16_t *ACT = reinterpret_cast<uint16_t *>(B + 3 * kMlpHid + kMlpOutPad);
     const int t = threadIdx.x;
-    // stage the weights (16-byte chunks) and biases
-    for (int q = t; q < kMlpHid * kMlpIn / 8; q += blockDim.x) {
-        const int row = q / (kMlpIn / 8), c = (q % (kMlpIn / 8)) * 8;
-        *reinterpret_cast<uint4 *>(W0 + row * kP0 + c) = *reinterpret_cast<const uint4 *>(w0 + row * kMlpIn + c);
-    }
-    for (int q = t; q < kMlpHid * kMlpHid / 8; q += blockDim.x) {
-        const int row = q / (kMlpHid / 8), c = (q % (kMlpHid / 8)) * 8;
-        *reinterpret_cast<uint4 *>(W1 + row * kPH + c) = *reinterpret_cast<const uint4 *>(w1 + row * kMlpHid + c);
-        *reinterpret_cast<uint4 *>(W2 + row * kPH + c) = *reinterpret_cast<const uint4 *>(w2 + row * kMlpHid + c);
-    }
-    for (int q = t; q < kMlpOutPad * kMlpHid / 8; q += blockDim.x) {
-        const int row = q / (kMlpHid / 8), c = (q % (kMlpHid / 8)) * 8;
-        *reinterpret_cast<uint4 *>(W3 + row * kPH + c) = *reinterpret_cast<const uint4 *>(w3 + row * kMlpHid + c);
-    }
+    // stage the weights (16-byte chunks, 8 loads in flight per thread) and biases
+    stage_rows<kMlpHid, kMlpIn, kP0>(W0, w0, t);
+    stage_rows<kMlpHid, kMlpHid, kPH>(W1, w1, t);
+    stage_rows<kMlpHid, kMlpHid, kPH>(W2, w2, t);
+    stage_rows<kMlpOutPad, kMlpHid, kPH>(W3, w3, t);
     for (int q = t; q < kMlpHid; q += blockDim.x) {
         B[q] = params[kOffB0 + q];
         B[kMlpHid + q] = params[kOffB1 + q];
@@ -341,26 +359,20 @@ __global__ __launch_bounds__(64 * kFusedWaves) void k_mlp_fused(
     const int ntiles = (M + kTileRows - 1) / kTileRows;
     for (int tile = blockIdx.x * kFusedWaves + w; tile < ntiles; tile += gridDim.x * kFusedWaves) {
         const int64_t row0 = (int64_t)tile * kTileRows;
-        fused_hidden<kMlpIn, kP0>(X, row0, M, nullptr, W0, B, act, lane);
-        fused_hidden<kMlpHid, kPH>(X, row0, M, act, W1, B + kMlpHid, act, lane);
-        fused_hidden<kMlpHid, kPH>(X, row0, M, act, W2, B + 2 * kMlpHid, act, lane);
-        // output layer (16 padded columns, 2 used)
-        f32x4 acc[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
+        fused_hidden<kMlpIn, kP0, true>(X, row0, M, act, W0, B, lane);
+        fused_hidden<kMlpHid, kPH, false>(X, row0, M, act, W1, B + kMlpHid, lane);
+        fused_hidden<kMlpHid, kPH, false>(X, row0, M, act, W2, B + 2 * kMlpHid, lane);
+        // output layer, transposed: 16 padded outputs (2 used) x 16 rows
+        f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int ks = 0; ks < kMlpHid / 32; ks++) {
-            const bf16x8 b = ld8(W3 + fr * kPH + ks * 32 + fk);
-#pragma unroll
-            for (int i = 0; i < 2; i++) acc[i] = mfma16(ld8(act + (16 * i + fr) * kPH + ks * 32 + fk), b, acc[i]);
-        }
+        for (int ks = 0; ks < kMlpHid / 32; ks++)
+            acc = mfma16(ld8(W3 + fr * kPH + ks * 32 + fk), ld8(act + fr * kPH + ks * 32 + fk), acc);
         wave_sync();  // the next tile's first layer overwrites act
-        if (fr < kMlpOut) {
-#pragma unroll
-            for (int i = 0; i < 2; i++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int64_t m = row0 + 16 * i + (lane >> 4) * 4 + r;
-                    if (m < M) Y[m * kMlpOut + fr] = acc[i][r] + B[3 * kMlpHid + fr];
-                }
+        if ((lane >> 4) == 0) {  // outputs 0 and 1 of row fr
+            const int64_t m = row0 + fr;
+            if (m < M)
+                *reinterpret_cast<float2 *>(Y + m * kMlpOut) =
+                    make_float2(acc[0] + B[3 * kMlpHid], acc[1] + B[3 * kMlpHid + 1]);
         }
     }
 }
@@ -417,14 +429,13 @@ hipError_t gemm_t(hipStream_t st, int M, int N, int K, const uint16_t *A, int ld
     return hipGetLastError();
 }
 
+// The fused kernel's dynamic LDS (150 KB) is above the default limit: raised
+// once per device, at network creation (not inside a graph capture).
+hipError_t mlp_fused_attr() {
+    return hipFuncSetAttribute((const void *)k_mlp_fused, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFusedLds);
+}
+
 hipError_t mlp_forward_fused(mavi_mlp *p, const uint16_t *X, int M, float *Y, hipStream_t st) {
-    static bool attr = false;
-    if (!attr) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_mlp_fused, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)kFusedLds);
-        if (e != hipSuccess) return e;
-        attr = true;
-    }
     if (M <= 0) return hipSuccess;
     const int ntiles = (M + kTileRows - 1) / kTileRows;
     int grid = (ntiles + kFusedWaves - 1) / kFusedWaves;
@@ -641,6 +652,7 @@ int mavi_mlp_create(mavi_ctx *ctx, uint64_t seed, mavi_mlp **out) {
     if (e == hipSuccess) e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     p->fused_grid = ncu > 0 ? ncu : 256;
+    if (e == hipSuccess) e = mavi::mlp_fused_attr();
     std::vector<float> h;
     init_params(h, seed);
     if (e == hipSuccess)

@@ -6,6 +6,8 @@ which stays its fallback and the judge of its error.
   MLP           the network (BF16 weights, FP32 accumulation on the matrix
                 cores): inference, loss + gradients, a device-resident data
                 set and Adam training
+  Learned       the gated learned step on a rings system, feature export and
+                training-data collection from plain trajectories
 """
 from __future__ import annotations
 
@@ -151,6 +153,79 @@ class MLP:
         try:
             if getattr(self, "handle", None):
                 _lib_bound().mavi_mlp_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
+class LearnedDesc(C.Structure):
+    _fields_ = [
+        ("k", C.c_int32),
+        ("reserved", C.c_int32),
+        ("r_feat", C.c_double),
+        ("target_scale", C.c_double),
+        ("d_min", C.c_double),
+        ("max_disp", C.c_double),
+        ("e_tol", C.c_double),
+        ("seed", C.c_uint64),
+    ]
+
+
+SIGNATURES.update({
+    "mavi_learned_create": (C.c_int, [_P, _P, C.POINTER(LearnedDesc), C.POINTER(_P)]),
+    "mavi_learned_destroy": (C.c_int, [_P]),
+    "mavi_learned_features": (C.c_int, [_P, _U16P, _DP]),
+    "mavi_learned_collect": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_uint64]),
+    "mavi_learned_run": (C.c_int, [_P, C.c_int64, C.c_int32, _I64P]),
+})
+
+
+class Learned:
+    """Learned propagator bound to a rings system (a RingsSystem's device
+    system) and optionally a network.  ``run(n)`` advances n learned steps,
+    each standing for k Rings step! calls; rejected steps are replaced by k
+    plain FP64 steps."""
+
+    def __init__(self, dev: _lib.DeviceSystem, mlp: MLP | None = None, k: int = 10, r_feat: float = 1.25,
+                 target_scale: float = 1e-3, d_min: float = 0.0, max_disp: float = 0.0, e_tol: float = 0.0,
+                 seed: int = 0):
+        lib = _lib_bound()
+        self.dev, self.mlp, self.k = dev, mlp, int(k)
+        self.desc = LearnedDesc(k=int(k), reserved=0, r_feat=r_feat, target_scale=target_scale, d_min=d_min,
+                                max_disp=max_disp, e_tol=e_tol, seed=int(seed))
+        h = _P()
+        check(lib.mavi_learned_create(dev.handle, mlp.handle if mlp else None, C.byref(self.desc), C.byref(h)),
+              dev.ctx.handle)
+        self.handle = h
+        self.accepted = 0
+        self.rejected = 0
+
+    def _call(self, name, *args):
+        check(getattr(_lib_bound(), name)(self.handle, *args), self.dev.ctx.handle)
+
+    def features(self):
+        """(x bits [n][160], explicit k dt displacement [n][2]) at the current state."""
+        n = self.dev.n
+        x = np.zeros((n, N_IN), dtype=np.uint16)
+        e = np.zeros((n, 2))
+        self._call("mavi_learned_features", x.ctypes.data_as(_U16P), e.ctypes.data_as(_DP))
+        return x, e
+
+    def collect(self, nsnap: int, stride: int = 0, rows_per_snap: int = 1 << 30, seed: int = 0):
+        self._call("mavi_learned_collect", int(nsnap), int(stride), int(min(rows_per_snap, 2**31 - 1)),
+                   C.c_uint64(seed))
+
+    def run(self, nsteps: int, use_net: bool = True) -> int:
+        acc = C.c_int64()
+        self._call("mavi_learned_run", C.c_int64(int(nsteps)), 1 if use_net else 0, C.byref(acc))
+        self.accepted += acc.value
+        self.rejected += int(nsteps) - acc.value
+        return acc.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                _lib_bound().mavi_learned_destroy(self.handle)
                 self.handle = None
         except Exception:
             pass

@@ -66,10 +66,12 @@ def lj_periodic(nside=1024, density=0.7, rc=2.5, seed=1, cutoff=True):
     return pos, vel, space, dyn, int_cfg
 
 
-def rings_mixture(nrings=40000, nper=20, phi=0.8, seed=2, rot_diff=0.1, k_area=1.0):
+def rings_mixture(nrings=40000, nper=20, phi=0.8, seed=2, rot_diff=0.1, k_area=1.0, chunk=1.0):
     """Binary active-ring mixture (see the module docstring).  The target area
     of a ring is A0 = (n l / p0)^2 (src/rings/integration.jl:170); the
-    contraction c of a type maps to p0 = n l / sqrt(c * A_ngon(n, l))."""
+    contraction c of a type maps to p0 = n l / sqrt(c * A_ngon(n, l)).
+    Chunks are the largest whose side is >= ``chunk`` (>= the interaction
+    range 1; the learned propagator's features need >= their radius)."""
     from .rings import HarmTruncCfg, RingsCfg, RingsState
 
     rng = np.random.default_rng(seed)
@@ -95,7 +97,7 @@ def rings_mixture(nrings=40000, nper=20, phi=0.8, seed=2, rot_diff=0.1, k_area=1
     rings_pos = np.mod(centers[:, None, :] + circle[None, :, :], L)
     pol = rng.random(nrings) * 2 * math.pi
     state = RingsState(rings_pos=rings_pos, pol=pol, num_particles=[nper, nper], types=types)
-    cols = int(L // (2 * p_r))
+    cols = int(L // max(chunk, 2 * p_r))
     space = SpaceCfg(wall_type=PeriodicWalls(), geometry_cfg=RectangleCfg(length=L, height=L))
     int_cfg = IntCfg(dt=1e-3, chunks_cfg=ChunksCfg(num_cols=cols, num_rows=cols), device=GPU())
     return state, space, cfg, int_cfg

@@ -12,7 +12,8 @@ forward/backward:
     dY = (Y - T)/M,  dW_l = dZ_l^T H_{l-1},  db_l = sum_m dZ_l,
     dZ_{l-1} = (dZ_l W_l) * GELU'(Z_{l-1})
 
-with GELU(z) = z Phi(z).  With ``bf16=True`` it rounds to BF16 exactly where
+with GELU in its sigmoid form, g(z) = z sigma(1.702 z).
+With ``bf16=True`` it rounds to BF16 exactly where
 the device kernels store BF16 (weights, hidden activations, dY and dZ), so
 the device gradients are compared against the same rounded quantities.
 
@@ -76,17 +77,22 @@ def rbf16(x) -> np.ndarray:
     return bf16_bits_to_f64(f32_to_bf16_bits(x))
 
 
-_erf = np.vectorize(math.erf, otypes=[np.float64])
+GS = 1.702
+
+
+def _sigmoid(x):
+    from scipy.special import expit
+
+    return expit(x)
 
 
 def gelu(z):
-    return 0.5 * z * (1.0 + _erf(z / math.sqrt(2.0)))
+    return z * _sigmoid(GS * z)
 
 
 def gelu_grad(z):
-    cdf = 0.5 * (1.0 + _erf(z / math.sqrt(2.0)))
-    pdf = np.exp(-0.5 * z * z) / math.sqrt(2.0 * math.pi)
-    return cdf + z * pdf
+    s = _sigmoid(GS * z)
+    return s + GS * z * s * (1.0 - s)
 
 
 def forward(params, x, bf16=True):

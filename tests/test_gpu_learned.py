@@ -1,0 +1,174 @@
+"""The learned propagator on the GPU: feature builder and training targets
+against the CPU oracle (oracle/learned_oracle.py), the gate's fallback to
+plain FP64 steps, and what a trained network gains over the same step
+without it.
+
+Tolerances: features are computed in FP64 and rounded to BF16 like the
+oracle; the only differences come from sin/cos of the polarisation (device
+vs numpy, ~1 ulp in FP64), so at most 1 BF16 ulp on at most 0.1% of the
+entries.  Targets (FP32) agree to 1e-6 of their scale for the same reason.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import learned_oracle as lo
+import mlp_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+K = 10
+RF = 1.25
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def small_rings(nrings=150, seed=4, rot_diff=0.1):
+    from mavi_hip import rings
+    from mavi_hip import workloads as wl
+
+    st, sp, cfg, ic = wl.rings_mixture(nrings=nrings, seed=seed, chunk=RF, rot_diff=rot_diff)
+    sys_ = rings.RingsSystem(state=st, space_cfg=sp, dynamic_cfg=cfg, int_cfg=ic, seed=seed)
+    return sys_, cfg
+
+
+def host_state(system):
+    system.download()
+    areas, _, _ = system.dev.rings_info()
+    return system.state.pos.copy(), system.state.pol.copy(), system.forces.copy(), areas
+
+
+def oracle_features(system, cfg):
+    pos, pol, forces, areas = host_state(system)
+    st = system.state
+    g = system.space_cfg.geometry_cfg
+    tt = cfg.type_table()  # per type: p0, relax, vo, mu, rot, k_area, k_spring, l_spring
+    ring_type = (np.asarray(st.types) - 1) if st.types is not None else np.zeros(st.num_rings, dtype=int)
+    np_t = np.atleast_1d(np.asarray(st.num_particles))
+    return lo.features(pos, pol, forces, areas, g.length, g.height, st.nmax, ring_type, np_t, tt[:, 0], tt[:, 2],
+                       tt[:, 3], tt[:, 7], st.types is not None, RF, K * system.int_cfg.dt), (pos, pol)
+
+
+def bits_close(a, b):
+    a = a.astype(np.int32)
+    b = b.astype(np.int32)
+    diff = np.abs(a - b)
+    return int(diff.max()), float(np.mean(diff != 0))
+
+
+def test_features_vs_oracle(gpu_ctx):
+    from mavi_hip import learned
+
+    system, cfg = small_rings()
+    system.advance(50)  # leave the initial lattice
+    lp = learned.Learned(system.dev, None, k=K, r_feat=RF)
+    x, euler = lp.features()
+    (xr, er, over), _ = oracle_features(system, cfg)
+    assert not over
+    say(f"[features] rows {len(x)}, non-empty neighbour slots {(x[:, 32::4] != 0).sum()}")
+    dmax, frac = bits_close(x, xr)
+    assert dmax <= 1 and frac <= 1e-3, (dmax, frac)
+    assert np.abs(euler - er).max() <= 1e-12 * max(np.abs(er).max(), 1e-300)
+    # structure: every particle lists its two ring neighbours among its neighbours (spacing ~1 < r_feat)
+    same = x[:, 34::4] == 0x3F80
+    assert np.all(same.sum(axis=1) >= 2)
+    assert np.any(x[:, 34::4] == 0xBF80)  # some contacts with other rings
+
+
+def test_targets_vs_oracle(gpu_ctx):
+    from mavi_hip import learned
+
+    system, cfg = small_rings(nrings=120, seed=6)
+    system.advance(30)
+    net = learned.MLP(ctx=gpu_ctx, seed=1)
+    lp = learned.Learned(system.dev, net, k=K, r_feat=RF, target_scale=1e-3)
+    x0, e0 = lp.features()
+    pos0, pol0, _, _ = host_state(system)
+    lp.collect(nsnap=1, stride=0)
+    system.download()
+    pos1 = system.state.pos.copy()
+    n = len(pos0)
+    assert net.dataset_size() == n
+    xd, td = net.dataset_get(0, n)
+    assert np.array_equal(xd, x0)
+    g = system.space_cfg.geometry_cfg
+    tr = lo.targets(pos0, pos1, pol0, e0, system.state.nmax, g.length, g.height, 1e-3)
+    scale = np.abs(tr).max()
+    say(f"[targets] scale {scale:.3g} (units of 1e-3), rms {np.sqrt(np.mean(tr ** 2)):.3g}")
+    assert np.abs(td - tr).max() <= 1e-6 * scale + 1e-6
+
+
+def test_rejected_steps_equal_plain_steps(gpu_ctx):
+    """A network whose output is absurd fails the gate every time; the
+    propagator then runs exactly the plain steps (bit for bit)."""
+    from mavi_hip import learned
+
+    a, _ = small_rings(nrings=100, seed=8)
+    b, _ = small_rings(nrings=100, seed=8)
+    net = learned.MLP(ctx=gpu_ctx, seed=3)
+    p = net.params
+    p[mo.OFFS[3][0]:mo.OFFS[3][1]] *= 1e4  # output weights
+    net.params = p
+    lp = learned.Learned(a.dev, net, k=K, r_feat=RF)
+    acc = lp.run(5)
+    assert acc == 0
+    b.dev.step(5 * K)
+    a.download()
+    b.download()
+    assert np.array_equal(a.state.pos, b.state.pos)
+    assert np.array_equal(a.state.pol, b.state.pol)
+
+
+def test_learned_steps_stay_close_to_plain(gpu_ctx):
+    """Without a network the gated step is the explicit k dt step: accepted,
+    and a short trajectory stays within a small distance of the plain one."""
+    from mavi_hip import learned
+
+    a, _ = small_rings(nrings=100, seed=9)
+    b, _ = small_rings(nrings=100, seed=9)
+    a.advance(20)
+    b.advance(20)
+    lp = learned.Learned(a.dev, None, k=K, r_feat=RF, seed=77)
+    acc = lp.run(10, use_net=False)
+    say(f"[learned no-net] accepted {acc}/10")
+    assert acc >= 9
+    b.dev.step(10 * K)
+    a.download()
+    b.download()
+    L = a.space_cfg.geometry_cfg.length
+    d = a.state.pos - b.state.pos
+    d -= L * np.round(d / L)
+    # noise streams differ; ring drift over 100 steps is ~0.1
+    assert np.abs(d).max() < 0.5
+
+
+def test_trained_network_beats_the_step_without_it(gpu_ctx):
+    """Trained on plain trajectories, the network's correction brings the
+    predicted k-step displacement closer to the plain result than the
+    explicit k dt step alone, on snapshots it has not seen.  The rotational
+    noise is off here: with it, most of the k-step residual of the explicit
+    step is the noise-driven turn of the polarisation during the k steps,
+    which no function of the state at t can predict."""
+    from mavi_hip import learned
+
+    system, _ = small_rings(nrings=400, seed=12, rot_diff=0.0)
+    system.advance(200)
+    net = learned.MLP(ctx=gpu_ctx, seed=2)
+    lp = learned.Learned(system.dev, net, k=K, r_feat=RF, target_scale=1e-3)
+    lp.collect(nsnap=40, stride=20)
+    n_train = net.dataset_size()
+    losses = net.train(steps=1500, batch=4096, lr=1e-3, seed=5)
+    say(f"[gain] {n_train} rows, loss {losses[:10].mean():.4f} -> {losses[-50:].mean():.4f}")
+    # held-out snapshots
+    lp.collect(nsnap=4, stride=50)
+    xh, th = net.dataset_get(n_train, net.dataset_size() - n_train)
+    yh = net.forward(xh)
+    err_without = float(np.mean(np.sum(th.astype(np.float64) ** 2, axis=1)))
+    err_with = float(np.mean(np.sum((th - yh).astype(np.float64) ** 2, axis=1)))
+    say(f"[gain] mean squared displacement error (units 1e-3): without network {err_without:.4f}, "
+        f"with network {err_with:.4f}")
+    assert err_with < 0.5 * err_without

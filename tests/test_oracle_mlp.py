@@ -63,7 +63,7 @@ def test_gradient_matches_torch_autograd():
         W = pt[ow:ow + nout * nin].view(nout, nin)
         b = pt[ob:ob + nout]
         z = h @ W.T + b
-        h = torch.nn.functional.gelu(z) if l < 3 else z
+        h = z * torch.sigmoid(1.702 * z) if l < 3 else z
     L = ((h - torch.tensor(t)) ** 2).sum() / (2 * len(x))
     L.backward()
     assert abs(L.item() - loss) <= 1e-12 * abs(loss)
