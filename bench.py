@@ -24,6 +24,8 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 ROOT = Path(__file__).resolve().parent
 for _p in (ROOT / "mavi.jl_amd", ROOT / "oracle"):
     if str(_p) not in sys.path:
@@ -186,6 +188,104 @@ def bench_batched(args, plain_value):
     }
 
 
+def bench_learned(args):
+    """Learned propagator on the active-ring mixture (nrings x 20 particles,
+    chunks >= the 1.25 feature radius).  Training: snapshots of plain GPU
+    trajectories from seeds 100.. (equilibrated first), network trained on
+    the GPU by the project's kernels.  Evaluation on a held-out seed: plain
+    FP64 steps vs learned steps (k = 10) from the same equilibrated state;
+    g(r) and the segregation index averaged over snapshots, with a second
+    plain run (other noise seed) as the run-to-run reference."""
+    import mavi_hip as mv
+    from mavi_hip import learned, rings, observables as obs, workloads as wl
+
+    k = 10
+    ctx = mv._lib.Context.for_device(0)
+    net = learned.MLP(ctx=ctx, seed=7)
+    t_collect = 0.0
+    for q in range(args.learned_seeds):
+        seed = 100 + q
+        st, sp, cfg, ic = wl.rings_mixture(nrings=args.learned_rings, seed=seed, chunk=1.25)
+        s = rings.RingsSystem(state=st, space_cfg=sp, dynamic_cfg=cfg, int_cfg=ic, seed=seed, ctx=ctx)
+        s.dev.step(args.learned_equil)
+        lp = learned.Learned(s.dev, net, k=k, r_feat=1.25, target_scale=1e-3, seed=seed)
+        t0 = time.perf_counter()
+        lp.collect(nsnap=args.learned_snaps, stride=20, rows_per_snap=args.learned_rows, seed=seed)
+        t_collect += time.perf_counter() - t0
+        del lp, s
+    rows = net.dataset_size()
+    log(f"[bench] learned: {rows} training rows from {args.learned_seeds} seeds")
+    t0 = time.perf_counter()
+    losses = net.train(steps=args.learned_train, batch=8192, lr=1e-3, seed=3)
+    t_train = time.perf_counter() - t0
+
+    # held-out seed
+    seed = 199
+    st, sp, cfg, ic = wl.rings_mixture(nrings=args.learned_rings, seed=seed, chunk=1.25)
+    ref = rings.RingsSystem(state=st, space_cfg=sp, dynamic_cfg=cfg, int_cfg=ic, seed=seed, ctx=ctx)
+    ref.dev.step(args.learned_equil)
+    ref.download()
+    pos0, pol0 = ref.state.pos.copy(), ref.state.pol.copy()
+    g = sp.geometry_cfg
+    L, H = g.length, g.height
+    types = np.asarray(st.types)
+    n = len(pos0)
+    nmac = args.learned_eval // k
+    nsnap = 4
+
+    def run(kind, noise_seed):
+        st2, sp2, cfg2, ic2 = wl.rings_mixture(nrings=args.learned_rings, seed=seed, chunk=1.25)
+        s = rings.RingsSystem(state=st2, space_cfg=sp2, dynamic_cfg=cfg2, int_cfg=ic2, seed=noise_seed, ctx=ctx)
+        s.dev.upload(pos0, None, pol0)
+        lp = learned.Learned(s.dev, net, k=k, r_feat=1.25, target_scale=1e-3, seed=noise_seed) if kind else None
+        if lp:
+            lp.run(2)  # graph capture
+            s.dev.upload(pos0, None, pol0)
+        else:
+            s.dev.step(2 * k)
+            s.dev.upload(pos0, None, pol0)
+        gs, segs, wall, acc = [], [], 0.0, 0
+        per = nmac // nsnap
+        for _ in range(nsnap):
+            t0 = time.perf_counter()
+            if lp:
+                acc += lp.run(per)
+            else:
+                s.dev.step(per * k)
+            wall += time.perf_counter() - t0
+            pos = np.zeros((n, 2))
+            s.dev.download(pos, None, None)
+            r, gr = obs.radial_distribution(pos, L, H, r_max=4.0, dr=0.05)
+            gs.append(gr)
+            segs.append(obs.segregation_index(pos, types, 20, L, H))
+        return r, np.mean(gs, axis=0), float(np.mean(segs)), n * per * nsnap * k / wall, acc
+
+    r, g_plain, seg_plain, v_plain, _ = run(False, 1001)
+    _, g_plain2, seg_plain2, _, _ = run(False, 1002)
+    _, g_learn, seg_learn, v_learn, acc = run(True, 1003)
+    us_mlp = net.time_forward(n, 20)
+    flop = 2.0 * n * (160 * 128 + 128 * 128 * 2 + 128 * 2)
+    return {
+        "workload": f"active rings binary mixture, {args.learned_rings} rings x 20 = {n} particles (ring physics of "
+                    "the 800k mixture, chunks >= 1.25), k = 10 plain steps per learned step, held-out seed 199",
+        "particles": n, "k": k, "value": v_learn, "unit": "particle-steps/s",
+        "plain_value": v_plain, "speedup_vs_plain": v_learn / v_plain,
+        "accepted": acc, "learned_steps": nmac,
+        "error": {"g_r_rel_l2": obs.relative_l2(g_learn, g_plain),
+                  "g_r_rel_l2_plain_vs_plain": obs.relative_l2(g_plain2, g_plain),
+                  "segregation_index": {"plain": seg_plain, "plain_other_noise": seg_plain2, "learned": seg_learn,
+                                        "abs_error": abs(seg_learn - seg_plain)},
+                  "how": f"{nsnap} snapshots over {args.learned_eval} steps from one equilibrated state; g(r) to r = 4 "
+                         "in bins of 0.05"},
+        "training": {"seeds": f"100..{99 + args.learned_seeds}", "snapshots": args.learned_seeds * args.learned_snaps,
+                     "rows": rows, "rows_per_snapshot": args.learned_rows, "steps": args.learned_train,
+                     "batch": 8192, "loss_first": float(losses[:20].mean()), "loss_last": float(losses[-50:].mean()),
+                     "collect_s": t_collect, "train_s": t_train},
+        "mlp_kernel": {"rows": n, "avg_us": us_mlp, "bf16_TFLOPs": flop / (us_mlp * 1e-6) / 1e12,
+                       "frac_of_bf16_peak": flop / (us_mlp * 1e-6) / 2.5e15},
+    }
+
+
 def host_cores() -> int:
     try:
         n = len(os.sched_getaffinity(0))
@@ -210,6 +310,13 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--batch-graph", type=int, default=100, help="steps per captured graph of the batch")
     ap.add_argument("--batch-seq-steps", type=int, default=100, help="steps per replica of the sequential run")
+    ap.add_argument("--learned-rings", type=int, default=10000, help="rings of the learned form (0 = skip)")
+    ap.add_argument("--learned-seeds", type=int, default=20)
+    ap.add_argument("--learned-snaps", type=int, default=30, help="snapshots per training seed")
+    ap.add_argument("--learned-rows", type=int, default=40000, help="sampled particles per snapshot")
+    ap.add_argument("--learned-equil", type=int, default=1000)
+    ap.add_argument("--learned-train", type=int, default=4000)
+    ap.add_argument("--learned-eval", type=int, default=10000, help="plain-step equivalents of the evaluation")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="MiB copied device-to-device 4 times before the run (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -305,6 +412,9 @@ def main():
     if args.batch > 0:
         log(f"[bench] batched form, {args.batch} replicas")
         forms["batched"] = bench_batched(args, value)
+    if args.learned_rings > 0:
+        log(f"[bench] learned form, {args.learned_rings} rings")
+        forms["learned"] = bench_learned(args)
     if forms:
         out["forms"] = forms
     print(json.dumps(out), flush=True)
