@@ -258,14 +258,19 @@ def bench_learned(args):
             r, gr = obs.radial_distribution(pos, L, H, r_max=4.0, dr=0.05)
             gs.append(gr)
             segs.append(obs.segregation_index(pos, types, 20, L, H))
+        if lp:
+            stats.update(lp.stats())
         return r, np.mean(gs, axis=0), float(np.mean(segs)), n * per * nsnap * k / wall, acc
 
+    stats = {}
     r, g_plain, seg_plain, v_plain, _ = run(False, 1001)
     _, g_plain2, seg_plain2, _, _ = run(False, 1002)
     _, g_learn, seg_learn, v_learn, acc = run(True, 1003)
+    gate_stats = dict(stats)
     us_mlp = net.time_forward(n, 20)
     flop = 2.0 * n * (160 * 128 + 128 * 128 * 2 + 128 * 2)
     return {
+        "gate": gate_stats,
         "workload": f"active rings binary mixture, {args.learned_rings} rings x 20 = {n} particles (ring physics of "
                     "the 800k mixture, chunks >= 1.25), k = 10 plain steps per learned step, held-out seed 199",
         "particles": n, "k": k, "value": v_learn, "unit": "particle-steps/s",

@@ -272,7 +272,7 @@ typedef struct {
     int32_t reserved;
     double r_feat;         /* neighbour radius of the features */
     double target_scale;   /* s_t: network output unit of displacement */
-    double d_min;          /* overlap gate distance (<= 0: half the smallest dist_eq) */
+    double d_min;          /* overlap gate distance (<= 0: 0.3 x the smallest dist_eq) */
     double max_disp;       /* displacement gate (<= 0: (r_feat - d_min)/2, the largest allowed) */
     double e_tol;          /* per-ring spring + area energy rise gate (<= 0: 1.0) */
     uint64_t seed;         /* rotational-noise stream of accepted learned steps */
@@ -294,6 +294,10 @@ int mavi_learned_collect(mavi_learned *learned, int32_t nsnap, int32_t stride, i
  * same gated step without the network correction.  *accepted (nullable)
  * receives how many passed the gate. */
 int mavi_learned_run(mavi_learned *learned, int64_t nsteps, int32_t use_net, int64_t *accepted);
+/* Totals since creation (7 values): accepted, rejected, then rejections by
+ * gate check: displacement, overlap, ring energy, more than 32 neighbours,
+ * non-finite network output (a step may fail several). */
+int mavi_learned_stats(mavi_learned *learned, int64_t *stats);
 
 #ifdef __cplusplus
 }

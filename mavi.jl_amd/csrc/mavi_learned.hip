@@ -16,8 +16,9 @@
 //   4. k_apply: displacement = k dt (v0 p + mu F) + s_t R^T y  (the explicit
 //      k dt step plus the learned correction), candidate positions x'
 //   5. gate: |displacement| <= max_disp, every listed pair (all pairs within
-//      r_feat; more than 32 fails) stays >= d_min apart, and no ring's spring
-//      + area energy rises by more than e_tol.  Pairs that were farther than
+//      r_feat; more than 32 fails) stays >= d_min apart (default 0.3 dist_eq,
+//      below what the plain dynamics reaches), and no ring's spring + area
+//      energy rises by more than e_tol.  Pairs that were farther than
 //      r_feat cannot come closer than d_min when max_disp <= (r_feat - d_min)/2
 //   6. accepted: x <- x', theta += k dt/tau asin(p x v_cm/|v_cm|) +
 //      sqrt(2 D_r k dt) z (one normal per ring from the device noise stream);
@@ -591,6 +592,7 @@ struct mavi_learned {
     uint32_t *gate_host = nullptr;  // pinned
     hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [start copy][use_net]
     int64_t accepted = 0, rejected = 0;
+    int64_t reasons[5] = {0, 0, 0, 0, 0};  // rejections by gate bit
 };
 
 namespace {
@@ -682,7 +684,7 @@ int mavi_learned_create(mavi_sys *s, mavi_mlp *mlp, const mavi_learned_desc *d, 
         HIPCHK(ctx, hipMemcpy(hi.data(), s->inter, sizeof(RingInter) * hi.size(), hipMemcpyDeviceToHost));
         double m = INFINITY;
         for (const RingInter &I : hi) m = fmin(m, I.dist_eq);
-        dmin = 0.5 * m;
+        dmin = 0.3 * m;  // the plain dynamics of dense active rings reaches ~0.4 dist_eq in jams
     }
     double maxd = d->max_disp;
     if (!(maxd > 0.0)) maxd = 0.5 * (d->r_feat - dmin);
@@ -818,9 +820,12 @@ int mavi_learned_run(mavi_learned *l, int64_t nmacro, int32_t use_net, int64_t *
         }
         HIPCHK(ctx, hipMemcpyAsync(l->gate_host, l->gate, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         HIPCHK(ctx, hipStreamSynchronize(st));
-        if (*l->gate_host == 0) {
+        const uint32_t gw = *l->gate_host;
+        if (gw == 0) {
             acc++;
         } else {
+            for (int b = 0; b < 5; b++)
+                if (gw & (1u << b)) l->reasons[b]++;
             for (int k = 0; k < l->L.k; k++) HIPCHK(ctx, mavi::launch_rings_step(s, nullptr, st));
         }
     }
@@ -829,6 +834,14 @@ int mavi_learned_run(mavi_learned *l, int64_t nmacro, int32_t use_net, int64_t *
     if (accepted) *accepted = acc;
     HIPCHK(ctx, mavi::launch_unpermute(s, true, true, st));
     HIPCHK(ctx, hipStreamSynchronize(st));
+    return MAVI_OK;
+}
+
+int mavi_learned_stats(mavi_learned *l, int64_t *stats) {
+    if (!l || !stats) return MAVI_E_BADARG;
+    stats[0] = l->accepted;
+    stats[1] = l->rejected;
+    for (int b = 0; b < 5; b++) stats[2 + b] = l->reasons[b];
     return MAVI_OK;
 }
 

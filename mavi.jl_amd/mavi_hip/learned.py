@@ -177,6 +177,7 @@ SIGNATURES.update({
     "mavi_learned_features": (C.c_int, [_P, _U16P, _DP]),
     "mavi_learned_collect": (C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_uint64]),
     "mavi_learned_run": (C.c_int, [_P, C.c_int64, C.c_int32, _I64P]),
+    "mavi_learned_stats": (C.c_int, [_P, _I64P]),
 })
 
 
@@ -221,6 +222,12 @@ class Learned:
         self.accepted += acc.value
         self.rejected += int(nsteps) - acc.value
         return acc.value
+
+    def stats(self) -> dict:
+        a = np.zeros(7, dtype=np.int64)
+        self._call("mavi_learned_stats", a.ctypes.data_as(_I64P))
+        keys = ("accepted", "rejected", "displacement", "overlap", "ring_energy", "neighbours", "nonfinite")
+        return {k: int(v) for k, v in zip(keys, a)}
 
     def __del__(self):
         try:
