@@ -152,6 +152,10 @@ def bench_batched(args, plain_value):
 
     # the same replicas through the plain single-system path, one after another
     seq_steps = args.batch_seq_steps
+    if seq_steps <= 0:
+        return {"particles": n, "replicas": info["replicas"], "value": value, "unit": "particle-steps/s",
+                "ms_per_step": wall / args.batch_steps * 1e3, "shape": shape,
+                "kernel_avg_ms_per_step": {k: v[0] / 10 for k, v in prof.items()}}
     devs = []
     for (kind, case, seed), (p, v, pol) in zip(ens, inits):
         if kind == "plain":
@@ -314,7 +318,8 @@ def main():
     ap.add_argument("--batch", type=int, default=512, help="replicas in the batched form (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--batch-graph", type=int, default=100, help="steps per captured graph of the batch")
-    ap.add_argument("--batch-seq-steps", type=int, default=100, help="steps per replica of the sequential run")
+    ap.add_argument("--batch-seq-steps", type=int, default=100,
+                    help="steps per replica of the sequential run (0 = skip it)")
     ap.add_argument("--learned-rings", type=int, default=10000, help="rings of the learned form (0 = skip)")
     ap.add_argument("--learned-seeds", type=int, default=20)
     ap.add_argument("--learned-snaps", type=int, default=30, help="snapshots per training seed")
@@ -322,6 +327,8 @@ def main():
     ap.add_argument("--learned-equil", type=int, default=1000)
     ap.add_argument("--learned-train", type=int, default=4000)
     ap.add_argument("--learned-eval", type=int, default=10000, help="plain-step equivalents of the evaluation")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="direct kernel launches instead of graph replay (for kernel traces)")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="MiB copied device-to-device 4 times before the run (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -345,6 +352,8 @@ def main():
     import mavi_hip as mv
     from mavi_hip import workloads as wl
 
+    if args.no_graphs:
+        mv._lib.USE_GRAPHS = False
     pos, vel, space, dyn, ic = wl.lj_periodic(nside=args.nside)
     n = len(pos)
     log(f"[bench] building system: {n} particles, chunks {ic.chunks_cfg.num_cols}x{ic.chunks_cfg.num_rows}")

@@ -10,7 +10,7 @@
 //   training   k_gemm<TA, TB, EPI>: LDS-staged 64x64x32 tiles with optional
 //              transposed operands and fused epilogues (bias + GELU keeping
 //              the pre-activation, GELU backward, split-K partials), plus
-//              loss/dY, column-sum (bias gradients), split-K reduction and
+//              loss/dY, two-stage column sums (bias gradients), split-K reduction and
 //              Adam kernels
 //
 // Parameters (FP32 master copy, flat): for layer l = 0..3, W_l^T (out x in,
@@ -176,18 +176,21 @@ __global__ void k_split_reduce(const float *__restrict__ ws, int nsplit, int64_t
     out[i] = s;
 }
 
-// out[n] = sum_m H[m][n] (BF16 input, FP32 sum in a fixed order), n < N.
-// One block of 256 threads per 64 columns: 4 row groups, then a fixed tree.
-__global__ void k_colsum(const uint16_t *__restrict__ H, int M, int N, int ldh, float *__restrict__ out) {
-    __shared__ float part[4][64];
+// Column sums of H (BF16 [M][N], FP32 sums in a fixed order), first stage:
+// block (x, y) sums rows [256 y, 256 y + 256) of columns [64 x, 64 x + 64)
+// into part[y][n]; k_split_reduce then adds the row chunks in order.
+constexpr int kColRows = 256;
+__global__ void k_colsum_part(const uint16_t *__restrict__ H, int M, int N, int ldh, float *__restrict__ part) {
+    __shared__ float red[4][64];
     const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
     const int n = blockIdx.x * 64 + c;
+    const int m0 = blockIdx.y * kColRows;
     float s = 0.f;
     if (n < N)
-        for (int m = g; m < M; m += 4) s += bf2f(H[(int64_t)m * ldh + n]);
-    part[g][c] = s;
+        for (int m = m0 + g; m < min(M, m0 + kColRows); m += 4) s += bf2f(H[(int64_t)m * ldh + n]);
+    red[g][c] = s;
     __syncthreads();
-    if (g == 0 && n < N) out[n] = (part[0][c] + part[1][c]) + (part[2][c] + part[3][c]);
+    if (g == 0 && n < N) part[(int64_t)blockIdx.y * N + n] = (red[0][c] + red[1][c]) + (red[2][c] + red[3][c]);
 }
 
 // Mean-squared error L = 1/(2M) sum_m sum_o (Y - T)^2 over the 2 outputs;
@@ -525,7 +528,11 @@ hipError_t mlp_loss_grad_dev(mavi_mlp *p, const uint16_t *X, const float *T, int
         return hipGetLastError();
     };
     auto bgrad = [&](const uint16_t *dZ, int ld, int n, int64_t off) -> hipError_t {
-        hipLaunchKernelGGL(k_colsum, dim3((n + 63) / 64), dim3(256), 0, st, dZ, M, n, ld, p->grad + off);
+        const int nchunk = (M + kColRows - 1) / kColRows;
+        if ((int64_t)nchunk * n > p->ws_cap) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_colsum_part, dim3((n + 63) / 64, nchunk), dim3(256), 0, st, dZ, M, n, ld, p->ws);
+        hipLaunchKernelGGL(k_split_reduce, dim3((n + 255) / 256), dim3(256), 0, st, p->ws, nchunk, (int64_t)n,
+                           (int64_t)n, p->grad + off);
         return hipGetLastError();
     };
     // output layer: dW3t = dY^T H2 (rows 0..1 of the 16-row padded product), db3 = colsum dY

@@ -121,6 +121,10 @@ MAVI_OPT_TWO_PHASE = 0
 MAVI_OPT_GRAPHS = 1
 MAVI_OPT_LDS_TILES = 2
 
+# Multi-step calls replay captured hipGraphs unless this is set to False
+# before systems are created (kernel traces are then plain launches).
+USE_GRAPHS = True
+
 _lib = None
 
 
@@ -222,6 +226,8 @@ class DeviceSystem:
         self.cols = int(desc.cols)
         self.rows = int(desc.rows)
         self.nc = int(desc.nc)
+        if not USE_GRAPHS:
+            self.set_option(MAVI_OPT_GRAPHS, 0)
 
     def _call(self, name, *args):
         check(getattr(load(), name)(self.handle, *args), self.ctx.handle)

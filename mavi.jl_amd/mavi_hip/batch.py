@@ -70,6 +70,8 @@ class Batch:
         self.handle = h
         self.replicas: list[dict] = []
         self.finalized = False
+        if not _lib.USE_GRAPHS:
+            self.set_option(MAVI_BOPT_GRAPHS, 0)
 
     def _call(self, name, *args):
         check(getattr(_lib_bound(), name)(self.handle, *args), self.ctx.handle)
