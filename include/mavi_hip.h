@@ -137,9 +137,11 @@ int mavi_run_record(mavi_sys *sys, int64_t nsteps, double *ke, double *pe);
  * kernel classes is bracketed by events.  mavi_prof_get returns, for kernel
  * class `which`, the summed duration in ms and the number of launches. */
 #define MAVI_PROF_BIN 0
-#define MAVI_PROF_FORCE 1    /* pair-force kernels (both evaluations) */
+#define MAVI_PROF_FORCE 1    /* pair-force kernels (full evaluations) */
 #define MAVI_PROF_RINGS 2    /* per-ring springs/area/overdamped kernel */
-#define MAVI_PROF_NCLASS 3
+#define MAVI_PROF_FORCE_REUSE 3  /* first evaluations of Verlet steps that may reuse
+                                    the previous step's second one (MAVI_OPT_REUSE_FORCES) */
+#define MAVI_PROF_NCLASS 4
 int mavi_prof_enable(mavi_sys *sys, int on);
 int mavi_prof_get(mavi_sys *sys, int which, double *ms_total, int64_t *launches);
 
@@ -155,10 +157,17 @@ int mavi_sys_set_tile(mavi_sys *sys, int32_t workgroup);
  *   MAVI_OPT_GRAPHS     1 (default): multi-step calls replay a captured
  *                       two-step hipGraph
  *   MAVI_OPT_LDS_TILES  1: pair kernels stage chunk-row segments in LDS;
- *                       0 (default): per-particle gathers from global memory */
+ *                       0 (default): per-particle gathers from global memory
+ *   MAVI_OPT_REUSE_FORCES  1 (default): the first force evaluation of a
+ *                       newton_step! takes the previous step's second
+ *                       evaluation for every wavefront whose particles' 3x3
+ *                       chunk blocks kept their members (same positions,
+ *                       same candidates in the same order: bit-identical);
+ *                       0: always evaluate */
 #define MAVI_OPT_TWO_PHASE 0
 #define MAVI_OPT_GRAPHS 1
 #define MAVI_OPT_LDS_TILES 2
+#define MAVI_OPT_REUSE_FORCES 3
 int mavi_sys_set_option(mavi_sys *sys, int32_t option, int32_t value);
 
 /* Autotuner: times `trial_steps` steps of the system's own dynamics for every

@@ -259,7 +259,7 @@ hipError_t batch_bin(mavi_batch *b, hipStream_t st) {
     hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(kScanThreads), 0, st, b->tile_sum, ntiles);
     hipLaunchKernelGGL(k_tile_apply, dim3(ntiles), dim3(kScanThreads), 0, st, b->cnt, n, b->tile_sum, b->start);
     hipLaunchKernelGGL(k_scatter, dim3(nblk(std::max(ns, n), 256)), dim3(256), 0, st, b->G, b->key, b->rank,
-                       b->start, b->src, b->scell, b->cnt, b->counter);
+                       b->start, b->src, b->scell, b->cnt, b->counter, nullptr);
     hipLaunchKernelGGL(kb_finalize, dim3(nblk(ns, 256)), dim3(256), 0, st, b->d_reps, b->wave_rep, ns, b->start,
                        b->src, b->scell, b->P_[c], b->V_[c], b->ID_[c], b->P_[nc], b->V_[nc], b->ID_[nc],
                        b->slot_of, b->flags);

@@ -89,6 +89,7 @@ int mavi_sys_destroy(mavi_sys *s) {
     dfree(s->F); dfree(s->P2); dfree(s->F1);
     dfree(s->key); dfree(s->rank); dfree(s->src); dfree(s->scell); dfree(s->slot_of);
     dfree(s->cnt); dfree(s->start); dfree(s->tile_sum); dfree(s->flags); dfree(s->counter);
+    dfree(s->stamp); dfree(s->epoch);
     dfree(s->active_ids); dfree(s->pos_id); dfree(s->vel_id); dfree(s->force_id);
     dfree(s->pol); dfree(s->areas); dfree(s->cms); dfree(s->cms_next); dfree(s->cpos);
     dfree(s->rtype); dfree(s->types); dfree(s->inter); dfree(s->noise); dfree(s->red);
@@ -252,6 +253,8 @@ int mavi_sys_create(mavi_ctx *ctx, const mavi_sys_desc *d, mavi_sys **out) {
     ALLOC(s->scell, P.na);
     ALLOC(s->slot_of, P.n);
     ALLOC(s->flags, 1);
+    ALLOC(s->stamp, (int64_t)P.ncell + 1);
+    ALLOC(s->epoch, 1);
     ALLOC(s->counter, 1);
     ALLOC(s->cnt, (int64_t)P.ncell + 1);
     ALLOC(s->start, (int64_t)P.ncell + 1);
@@ -308,6 +311,7 @@ int mavi_sys_upload(mavi_sys *s, const double *pos, const double *vel, const dou
     if (s->P.potential == MAVI_POT_RINGS) HIPCHK(ctx, mavi::launch_rings_init(s, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     s->has_state = true;
+    s->f_valid = false;
     s->sorted = false;
     return MAVI_OK;
 }
@@ -333,6 +337,7 @@ int mavi_sys_download(mavi_sys *s, double *pos, double *vel, double *pol) {
 }
 
 int mavi_bin(mavi_sys *s, int64_t *chunk_particles, int64_t *counts) {
+    if (s) s->f_valid = false;  // a binning the next step's stamps would not see
     if (!s) return MAVI_E_BADARG;
     mavi_ctx *ctx = s->ctx;
     if (s->P.cols == 0) return set_err(ctx, MAVI_E_BADARG, "system has no chunks");
@@ -362,6 +367,7 @@ int mavi_bin(mavi_sys *s, int64_t *chunk_particles, int64_t *counts) {
 }
 
 int mavi_forces(mavi_sys *s, double *forces) {
+    if (s) s->f_valid = false;
     if (!s) return MAVI_E_BADARG;
     mavi_ctx *ctx = s->ctx;
     if (!s->has_state) return set_err(ctx, MAVI_E_BADARG, "no state uploaded");
@@ -390,22 +396,24 @@ int mavi_get_forces(mavi_sys *s, double *forces) {
 }
 
 // One step of the system's dynamics (newton_step! or Rings step!) with the
-// device rotational noise.
-static hipError_t launch_step(mavi_sys *s, hipStream_t st) {
+// device rotational noise.  reuse: the first force evaluation may take the
+// previous step's second one where it is provably identical.
+static hipError_t launch_step(mavi_sys *s, hipStream_t st, bool reuse) {
     if (s->P.potential == MAVI_POT_RINGS) return mavi::launch_rings_step(s, nullptr, st);
-    return mavi::launch_verlet_step(s, st);
+    return mavi::launch_verlet_step(s, st, reuse);
 }
 
 // Two steps captured once into a graph (state starting in copy 0, so the
-// ping-pong returns to copy 0) and replayed: one launch per two steps.
+// ping-pong returns to copy 0) and replayed: one launch per two steps.  The
+// graph's steps reuse forces, so it is replayed only after a Verlet step.
 static hipError_t ensure_graph(mavi_sys *s, hipStream_t st) {
     if (s->graph2) return hipSuccess;
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return e;
     const int cur0 = s->cur;
-    hipError_t e1 = launch_step(s, st);
-    hipError_t e2 = (e1 == hipSuccess) ? launch_step(s, st) : e1;
+    hipError_t e1 = launch_step(s, st, true);
+    hipError_t e2 = (e1 == hipSuccess) ? launch_step(s, st, true) : e1;
     e = hipStreamEndCapture(st, &g);
     s->cur = cur0;
     if (e1 != hipSuccess) e = e1;
@@ -434,16 +442,17 @@ static int steps_impl(mavi_sys *s, int64_t nsteps, const double *noise_host) {
         for (int64_t k = 0; k < nsteps; k++) HIPCHK(ctx, mavi::launch_rings_step(s, s->noise + k * nr, st));
     } else {
         int64_t k = 0;
+        const bool rings = s->P.potential == MAVI_POT_RINGS;
         const bool graphs = s->use_graphs && !s->prof && nsteps >= 4;
-        if (graphs && s->cur != 0) {
-            HIPCHK(ctx, launch_step(s, st));
+        while (graphs && k < nsteps && (s->cur != 0 || (!rings && !s->f_valid))) {
+            HIPCHK(ctx, launch_step(s, st, s->f_valid));
             k++;
         }
-        if (graphs) {
+        if (graphs && (rings || s->f_valid)) {
             HIPCHK(ctx, ensure_graph(s, st));
             for (; k + 2 <= nsteps; k += 2) HIPCHK(ctx, hipGraphLaunch(s->graph2, st));
         }
-        for (; k < nsteps; k++) HIPCHK(ctx, launch_step(s, st));
+        for (; k < nsteps; k++) HIPCHK(ctx, launch_step(s, st, s->f_valid));
     }
     int rc = sync_staging(s, true);
     if (rc != MAVI_OK) return rc;
@@ -534,6 +543,7 @@ int mavi_prof_get(mavi_sys *s, int which, double *ms_total, int64_t *launches) {
 }
 
 int mavi_sys_set_tile(mavi_sys *s, int32_t workgroup) {
+    if (s) s->f_valid = false;
     if (!s) return MAVI_E_BADARG;
     if (workgroup == 0) workgroup = 256;
     if (!is_pow2_wg(workgroup)) return set_err(s->ctx, MAVI_E_BADARG, "workgroup must be 64..1024, power of two");
@@ -561,9 +571,14 @@ int mavi_sys_set_option(mavi_sys *s, int32_t option, int32_t value) {
             changed = s->lds_tiles != (value != 0);
             s->lds_tiles = value != 0;
             break;
+        case MAVI_OPT_REUSE_FORCES:
+            changed = s->reuse != (value != 0);
+            s->reuse = value != 0;
+            break;
         default:
             return set_err(s->ctx, MAVI_E_BADARG, "unknown option");
     }
+    if (changed) s->f_valid = false;
     if (changed && s->graph2) {  // the captured graph holds the old kernels
         (void)hipStreamSynchronize(s->ctx->stream);
         (void)hipGraphExecDestroy(s->graph2);
@@ -603,6 +618,7 @@ int mavi_autotune(mavi_sys *s, int32_t trial_steps, double *best_us, int32_t *sh
         if (r == hipSuccess && pol0)
             r = hipMemcpyAsync(s->pol, pol0, sizeof(double) * s->P.nrings, hipMemcpyDeviceToDevice, st);
         s->cur = 0;
+        s->f_valid = false;
         if (r == hipSuccess) r = mavi::launch_iota_state(s, st);
         if (r == hipSuccess && s->P.potential == MAVI_POT_RINGS) r = mavi::launch_rings_init(s, st);
         if (r == hipSuccess) r = hipStreamSynchronize(st);

@@ -518,10 +518,11 @@ __global__ void k_tile_apply(const int32_t *__restrict__ in, int32_t n, const in
 
 // Place every particle at start[chunk] + arrival rank; reset the counts for
 // the next binning; advance the step counter of the device rotational noise
-// when one is given.
+// and the binning epoch when they are given.
 __global__ void k_scatter(SysParams P, const int32_t *__restrict__ key, const int32_t *__restrict__ rank,
                           const int32_t *__restrict__ start, int32_t *__restrict__ src, int32_t *__restrict__ scell,
-                          int32_t *__restrict__ cnt, unsigned long long *__restrict__ counter) {
+                          int32_t *__restrict__ cnt, unsigned long long *__restrict__ counter,
+                          uint32_t *__restrict__ epoch) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < P.na) {
         const int k = key[i];
@@ -531,6 +532,7 @@ __global__ void k_scatter(SysParams P, const int32_t *__restrict__ key, const in
     }
     if (i <= P.ncell) cnt[i] = 0;
     if (i == 0 && counter) counter[0] += 1ull;
+    if (i == 0 && epoch) epoch[0] += 1u;
 }
 
 // Unwrap (update_continuos_pos!, :118-136), centre of mass and shoelace area

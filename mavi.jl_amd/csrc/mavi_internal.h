@@ -112,6 +112,12 @@ struct mavi_sys {
     int32_t red_cap = 0;
     bool has_state = false;
     bool sorted = false;        // a binning has put the state in chunk order
+    // reuse of the second force evaluation of a Verlet step as the first one
+    // of the next step where a particle's 3x3 chunk block kept its members
+    int32_t *stamp = nullptr;   // per chunk: binning epoch in which its membership last changed
+    uint32_t *epoch = nullptr;  // binnings so far (device)
+    bool reuse = true;          // option
+    bool f_valid = false;       // F holds the last step's second evaluation
     // graphs of two steps (state starting in copy 0)
     hipGraphExec_t graph2 = nullptr;
     bool use_graphs = true;
@@ -153,10 +159,12 @@ hipError_t launch_iota_state(mavi_sys *s, hipStream_t st);          // id-order 
 hipError_t launch_unpermute(mavi_sys *s, bool vel, bool force, hipStream_t st);  // state -> staging
 // binning + permutation (cur flips); count_step advances the step counter
 // of the device rotational noise (a step's binning, not a lone update_chunks!)
-hipError_t launch_bin(mavi_sys *s, hipStream_t st, bool count_step = false);
+hipError_t launch_bin(mavi_sys *s, hipStream_t st, bool count_step = false, bool carry_forces = false);
 hipError_t launch_export_chunks(mavi_sys *s, int64_t *d_cp, int64_t *d_counts, hipStream_t st);
 hipError_t launch_forces_only(mavi_sys *s, hipStream_t st);
-hipError_t launch_verlet_step(mavi_sys *s, hipStream_t st);
+// One newton_step!; reuse = the previous call was a Verlet step of this
+// system (F holds its second evaluation), so unchanged blocks reuse it.
+hipError_t launch_verlet_step(mavi_sys *s, hipStream_t st, bool reuse = false);
 hipError_t launch_rings_step(mavi_sys *s, const double *noise_dev, hipStream_t st);
 hipError_t launch_rings_forces(mavi_sys *s, hipStream_t st);
 hipError_t launch_rings_init(mavi_sys *s, hipStream_t st);

@@ -31,9 +31,13 @@ namespace {
 // K1 binning
 // ---------------------------------------------------------------------------
 
-// Chunk of every particle and its arrival rank in that chunk.
+// Chunk of every particle and its arrival rank in that chunk.  With stamp:
+// a particle whose chunk differs from the previous binning's (prev_cell,
+// indexed by the current slot) stamps both chunks with this binning's epoch.
 __global__ void k_bin_count(SysParams P, const double2 *__restrict__ pos, int32_t *__restrict__ key,
-                            int32_t *__restrict__ rank, int32_t *__restrict__ cnt, uint32_t *__restrict__ flags) {
+                            int32_t *__restrict__ rank, int32_t *__restrict__ cnt, uint32_t *__restrict__ flags,
+                            const int32_t *__restrict__ prev_cell, int32_t *__restrict__ stamp,
+                            const uint32_t *__restrict__ epoch) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = s < P.na;
     uint32_t flag = 0;
@@ -43,6 +47,14 @@ __global__ void k_bin_count(SysParams P, const double2 *__restrict__ pos, int32_
         k = cell_of(P, p.x, p.y, &flag);
     }
     if (flag) atomicOr(flags, flag);
+    if (stamp && valid) {
+        const int32_t pc = prev_cell[s];
+        if (pc != k) {
+            const int32_t e = (int32_t)(epoch[0] + 1u);
+            stamp[k] = e;
+            stamp[pc] = e;
+        }
+    }
     const int32_t rk = wave_arrival_rank(valid, k, cnt);
     if (valid) {
         key[s] = k;
@@ -52,12 +64,13 @@ __global__ void k_bin_count(SysParams P, const double2 *__restrict__ pos, int32_
 
 // One thread per new slot: the final slot inside the chunk is the number of
 // chunk members with a smaller id (update_chunks! appends ids ascending);
-// move the particle's state there.
+// move the particle's state there (and, with Fnew, its last forces).
 __global__ void k_finalize(SysParams P, const int32_t *__restrict__ start, const int32_t *__restrict__ src,
                            const int32_t *__restrict__ scell, const double2 *__restrict__ Pold,
                            const double2 *__restrict__ Vold, const int32_t *__restrict__ IDold,
                            double2 *__restrict__ Pnew, double2 *__restrict__ Vnew, int32_t *__restrict__ IDnew,
-                           int32_t *__restrict__ slot_of, uint32_t *__restrict__ flags) {
+                           int32_t *__restrict__ slot_of, uint32_t *__restrict__ flags,
+                           const double2 *__restrict__ Fold, double2 *__restrict__ Fnew) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= P.na) return;
     const int c = scell[t];
@@ -69,6 +82,7 @@ __global__ void k_finalize(SysParams P, const int32_t *__restrict__ start, const
     const int t2 = b + r;
     Pnew[t2] = Pold[so];
     if (Vnew) Vnew[t2] = Vold[so];
+    if (Fnew) Fnew[t2] = Fold[so];
     IDnew[t2] = myid;
     if (slot_of) slot_of[myid] = t2;
     if (t == b && e - b > P.nc) atomicOr(flags, MAVI_FLAG_OVERFLOW);
@@ -92,16 +106,50 @@ __global__ void k_export_chunks(SysParams P, const int32_t *__restrict__ start, 
 // K2/K3 pair forces with fused integrator epilogues
 // ---------------------------------------------------------------------------
 
+// With stamp (first evaluation of a Verlet step only): a wave none of whose
+// particles has a chunk of its 3x3 block stamped in this binning's epoch
+// takes its forces from F1, where the binning moved the previous step's
+// second evaluation.  Those forces were computed at the same positions over
+// the same candidates in the same order, so they are bit-identical to a
+// recomputation.  (A compacted worklist of the particles to recompute was
+// slower: it loses the sorted order's neighbour locality.)
 template <int MODE, int POT, bool ALLPAIRS>
 __global__ void k_pair(SysParams P, RingCtx R, const double2 *__restrict__ pin, const int32_t *__restrict__ scell,
                        const int32_t *__restrict__ start, double2 *__restrict__ Pst, double2 *__restrict__ Vst,
                        double2 *__restrict__ Fout, double2 *__restrict__ P2, double2 *__restrict__ F1,
-                       double list2) {
+                       double list2, const int32_t *__restrict__ stamp, const uint32_t *__restrict__ epoch) {
     extern __shared__ uint16_t s_list[];
     const int s = xcd_block() * blockDim.x + threadIdx.x;
     const bool valid = s < P.na;
     const int sv = valid ? s : 0;
     const double2 pi = pin[sv];
+    if constexpr (MODE == MODE_VERLET1 && !ALLPAIRS) {
+        if (stamp) {
+            bool need = false;
+            if (valid) {
+                const int32_t e = (int32_t)epoch[0];
+                const int c = scell[sv];
+                const int row = c / P.cols, col = c - row * P.cols;
+                for (int dr = -1; dr <= 1; dr++) {
+                    int r = row + dr;
+                    if (P.periodic) r = r < 0 ? r + P.rows : (r >= P.rows ? r - P.rows : r);
+                    else if (r < 0 || r >= P.rows) continue;
+                    for (int dc = -1; dc <= 1; dc++) {
+                        int cc = col + dc;
+                        if (P.periodic) cc = cc < 0 ? cc + P.cols : (cc >= P.cols ? cc - P.cols : cc);
+                        else if (cc < 0 || cc >= P.cols) continue;
+                        need = need || stamp[r * P.cols + cc] == e;
+                    }
+                }
+            }
+            if (!__any(need)) {
+                if (!valid) return;
+                const double2 f = F1[s];
+                pair_epilogue<MODE>(P, s, pi, f.x, f.y, Pst, Vst, Fout, P2, F1);
+                return;
+            }
+        }
+    }
     PairAcc<POT> acc;
     if constexpr (POT == POT_RINGS) {
         acc.id_i = R.ids[sv] - R.id_base;
@@ -511,7 +559,7 @@ double list_range2(const mavi_sys *s) {
 }
 
 template <int MODE, int POT, bool ALL>
-hipError_t launch_pair_t(mavi_sys *s, const double2 *pin, hipStream_t st) {
+hipError_t launch_pair_t(mavi_sys *s, const double2 *pin, hipStream_t st, bool reuse) {
     const int wg = s->wg;
     const int c = s->cur;
     const double l2 = s->two_phase ? list_range2(s) : INFINITY;
@@ -533,21 +581,24 @@ hipError_t launch_pair_t(mavi_sys *s, const double2 *pin, hipStream_t st) {
     }
     const size_t lds = (l2 < INFINITY) ? sizeof(uint16_t) * kListCap * (size_t)wg : 0;
     hipLaunchKernelGGL((k_pair<MODE, POT, ALL>), dim3(nblk(s->P.na, wg)), dim3(wg), lds, st, s->P, ring_ctx(s), pin,
-                       s->scell, s->start, s->P_[c], s->V_[c], s->F, s->P2, s->F1, l2);
+                       s->scell, s->start, s->P_[c], s->V_[c], s->F, s->P2, s->F1, l2, reuse ? s->stamp : nullptr,
+                       s->epoch);
     return hipGetLastError();
 }
 
 template <int MODE>
-hipError_t launch_pair(mavi_sys *s, const double2 *pin, hipStream_t st) {
+hipError_t launch_pair(mavi_sys *s, const double2 *pin, hipStream_t st, bool reuse = false) {
     const bool all = s->P.cols == 0;
     switch (s->P.potential) {
         case MAVI_POT_LJ:
-            return all ? launch_pair_t<MODE, POT_LJ, true>(s, pin, st) : launch_pair_t<MODE, POT_LJ, false>(s, pin, st);
+            return all ? launch_pair_t<MODE, POT_LJ, true>(s, pin, st, false)
+                       : launch_pair_t<MODE, POT_LJ, false>(s, pin, st, reuse);
         case MAVI_POT_HARMTRUNC:
-            return all ? launch_pair_t<MODE, POT_HT, true>(s, pin, st) : launch_pair_t<MODE, POT_HT, false>(s, pin, st);
+            return all ? launch_pair_t<MODE, POT_HT, true>(s, pin, st, false)
+                       : launch_pair_t<MODE, POT_HT, false>(s, pin, st, reuse);
         default:
-            return all ? launch_pair_t<MODE, POT_RINGS, true>(s, pin, st)
-                       : launch_pair_t<MODE, POT_RINGS, false>(s, pin, st);
+            return all ? launch_pair_t<MODE, POT_RINGS, true>(s, pin, st, false)
+                       : launch_pair_t<MODE, POT_RINGS, false>(s, pin, st, false);
     }
 }
 
@@ -598,13 +649,15 @@ hipError_t launch_unpermute(mavi_sys *s, bool vel, bool force, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_bin(mavi_sys *s, hipStream_t st, bool count_step) {
+hipError_t launch_bin(mavi_sys *s, hipStream_t st, bool count_step, bool carry_forces) {
     const SysParams &P = s->P;
     if (P.cols == 0) return hipSuccess;  // chunks = nothing: the state order stays the active id order
     ProfScope ps(s, MAVI_PROF_BIN, st);
     const int c = s->cur, nc = 1 - c;
+    // stamps need the previous binning's chunk of every slot (scell)
+    const bool stamping = carry_forces && s->sorted;
     hipLaunchKernelGGL(k_bin_count, dim3(nblk(P.na, 256)), dim3(256), 0, st, P, s->P_[c], s->key, s->rank, s->cnt,
-                       s->flags);
+                       s->flags, s->scell, stamping ? s->stamp : nullptr, s->epoch);
     const int n = P.ncell + 1;
     const int ntiles = nblk(n, kScanTile2);
     hipLaunchKernelGGL(k_tile_reduce, dim3(ntiles), dim3(kScanThreads), 0, st, s->cnt, n, s->tile_sum);
@@ -612,11 +665,11 @@ hipError_t launch_bin(mavi_sys *s, hipStream_t st, bool count_step) {
     hipLaunchKernelGGL(k_tile_apply, dim3(ntiles), dim3(kScanThreads), 0, st, s->cnt, n, s->tile_sum, s->start);
     const int grid = nblk(P.na > n ? P.na : n, 256);
     hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(256), 0, st, P, s->key, s->rank, s->start, s->src, s->scell,
-                       s->cnt, count_step ? s->counter : nullptr);
+                       s->cnt, count_step ? s->counter : nullptr, s->epoch);
     const bool has_vel = P.potential != MAVI_POT_RINGS;
     hipLaunchKernelGGL(k_finalize, dim3(nblk(P.na, 256)), dim3(256), 0, st, P, s->start, s->src, s->scell, s->P_[c],
                        has_vel ? s->V_[c] : nullptr, s->ID_[c], s->P_[nc], has_vel ? s->V_[nc] : nullptr, s->ID_[nc],
-                       s->slot_of, s->flags);
+                       s->slot_of, s->flags, s->F, carry_forces ? s->F1 : nullptr);
     s->cur = nc;
     s->sorted = true;
     return hipGetLastError();
@@ -633,15 +686,20 @@ hipError_t launch_forces_only(mavi_sys *s, hipStream_t st) {
     return launch_pair<MODE_FORCES>(s, s->P_[s->cur], st);
 }
 
-hipError_t launch_verlet_step(mavi_sys *s, hipStream_t st) {
-    hipError_t e = launch_bin(s, st, true);
+hipError_t launch_verlet_step(mavi_sys *s, hipStream_t st, bool reuse) {
+    // forces reuse needs the gather kernel for both evaluations
+    const bool can = s->reuse && !s->lds_tiles && s->P.cols > 0;
+    reuse = reuse && can && s->sorted;
+    hipError_t e = launch_bin(s, st, true, can);
     if (e != hipSuccess) return e;
     {
-        ProfScope ps(s, MAVI_PROF_FORCE, st);
-        if ((e = launch_pair<MODE_VERLET1>(s, s->P_[s->cur], st)) != hipSuccess) return e;
+        ProfScope ps(s, reuse ? MAVI_PROF_FORCE_REUSE : MAVI_PROF_FORCE, st);
+        if ((e = launch_pair<MODE_VERLET1>(s, s->P_[s->cur], st, reuse)) != hipSuccess) return e;
     }
     ProfScope ps(s, MAVI_PROF_FORCE, st);
-    return launch_pair<MODE_VERLET2>(s, s->P2, st);
+    e = launch_pair<MODE_VERLET2>(s, s->P2, st);
+    s->f_valid = can && e == hipSuccess;
+    return e;
 }
 
 static hipError_t launch_rings_kernel(mavi_sys *s, bool update, const double *noise_dev, hipStream_t st) {
@@ -672,7 +730,7 @@ static hipError_t launch_rings_kernel(mavi_sys *s, bool update, const double *no
 }
 
 hipError_t launch_rings_forces(mavi_sys *s, hipStream_t st) {
-    hipError_t e = launch_bin(s, st);
+    hipError_t e = launch_bin(s, st, false, false);
     if (e != hipSuccess) return e;
     {
         ProfScope ps(s, MAVI_PROF_FORCE, st);
@@ -682,7 +740,7 @@ hipError_t launch_rings_forces(mavi_sys *s, hipStream_t st) {
 }
 
 hipError_t launch_rings_step(mavi_sys *s, const double *noise_dev, hipStream_t st) {
-    hipError_t e = launch_bin(s, st, true);
+    hipError_t e = launch_bin(s, st, true, false);
     if (e != hipSuccess) return e;
     if (s->P.cols == 0) {  // no binning kernel to advance the noise step counter
         hipLaunchKernelGGL(k_counter_inc, dim3(1), dim3(64), 0, st, s->counter);

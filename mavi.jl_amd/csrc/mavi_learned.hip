@@ -218,7 +218,8 @@ __global__ __launch_bounds__(kFeatWG) void k_features(
     }
     {  // all window loads in flight, then the LDS stores
         constexpr int kPer = kWin / kFeatWG;
-        double2 vp[3][kPer];
+        // plain doubles: an array of double2 stays in scratch memory here
+        double vx[3][kPer], vy[3][kPer];
         int32_t vi[3][kPer];
 #pragma unroll
         for (int d = 0; d < 3; d++)
@@ -226,18 +227,20 @@ __global__ __launch_bounds__(kFeatWG) void k_features(
             for (int u = 0; u < kPer; u++) {
                 const int q = lane + u * kFeatWG;
                 const int j = q < wb[d] - wa[d] ? wa[d] + q : 0;
-                vp[d][u] = Pst[j];
+                const double2 p = Pst[j];
+                vx[d][u] = p.x;
+                vy[d][u] = p.y;
                 vi[d][u] = ID[j];
             }
+        // q < kWin always; slots past a window's length get slot 0's values
+        // and are never read
 #pragma unroll
         for (int d = 0; d < 3; d++)
 #pragma unroll
             for (int u = 0; u < kPer; u++) {
                 const int q = lane + u * kFeatWG;
-                if (q < wb[d] - wa[d]) {
-                    spos[d][q] = vp[d][u];
-                    sid[d][q] = vi[d][u];
-                }
+                spos[d][q] = make_double2(vx[d][u], vy[d][u]);
+                sid[d][q] = vi[d][u];
             }
     }
     __syncthreads();

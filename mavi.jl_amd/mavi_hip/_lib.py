@@ -30,6 +30,7 @@ MAVI_POT_RINGS = 2
 MAVI_PROF_BIN = 0
 MAVI_PROF_FORCE = 1
 MAVI_PROF_RINGS = 2
+MAVI_PROF_FORCE_REUSE = 3
 
 _STATUS_NAMES = {
     MAVI_E_CELL_OVERFLOW: "E_CELL_OVERFLOW",
@@ -120,6 +121,7 @@ SIGNATURES = {
 MAVI_OPT_TWO_PHASE = 0
 MAVI_OPT_GRAPHS = 1
 MAVI_OPT_LDS_TILES = 2
+MAVI_OPT_REUSE_FORCES = 3
 
 # Multi-step calls replay captured hipGraphs unless this is set to False
 # before systems are created (kernel traces are then plain launches).

@@ -339,3 +339,30 @@ def test_autotune_restores_state(gpu_ctx, oracle_mod):
     p_ref, v_ref, _, _, _ = o.newton_steps(pos, vel, 50)
     ep, ev = traj_err(sys_.state.pos, sys_.state.vel, p_ref, v_ref, space.geometry_cfg.length)
     assert ep <= TRAJ_RTOL and ev <= TRAJ_RTOL
+
+
+@pytest.mark.parametrize("name", [n for n in PLAIN if n != "lj_allpairs"])
+def test_force_reuse_bitexact(gpu_ctx, name):
+    """Reusing the previous step's second force evaluation (MAVI_OPT_REUSE_FORCES,
+    the default) gives bit-identical trajectories to evaluating every time,
+    also across calls that invalidate it (forces, binning, graph and direct
+    paths, odd step counts)."""
+    from mavi_hip import _lib
+
+    say(f"[reuse] {name}")
+    pos, vel, space, dyn, ic = plain_case(name)
+    vel = vel * 8.0  # many chunk crossings and periodic wraps
+    a = make_plain(pos.copy(), vel.copy(), space, dyn, ic)
+    b = make_plain(pos.copy(), vel.copy(), space, dyn, ic)
+    b.dev.set_option(_lib.MAVI_OPT_REUSE_FORCES, 0)
+    for s_ in (a, b):
+        s_.dev.step(37)
+        s_.dev.forces()
+        s_.dev.step(5)
+        s_.dev.bin(export=False)
+        s_.dev.step(3)
+        s_.dev.step(12)
+        s_.download()
+    assert np.array_equal(a.state.pos, b.state.pos)
+    assert np.array_equal(a.state.vel, b.state.vel)
+    assert np.array_equal(a.forces, b.forces)
