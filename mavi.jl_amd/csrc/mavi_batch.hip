@@ -201,7 +201,8 @@ struct mavi_batch {
     int32_t *ID_[2] = {nullptr, nullptr};
     double2 *F = nullptr, *P2 = nullptr, *F1 = nullptr;
     int32_t *key = nullptr, *rank = nullptr, *src = nullptr, *scell = nullptr, *slot_of = nullptr;
-    int32_t *cnt = nullptr, *start = nullptr, *tile_sum = nullptr;
+    int32_t *cnt = nullptr, *start = nullptr;
+    unsigned long long *scan_state = nullptr, *scan_ticket = nullptr;
     uint32_t *flags = nullptr;
     unsigned long long *counter = nullptr;
     int32_t *active_ids = nullptr;
@@ -255,11 +256,11 @@ hipError_t batch_bin(mavi_batch *b, hipStream_t st) {
                        b->key, b->rank, b->cnt, b->flags);
     const int n = (int)b->ncells + 1;
     const int ntiles = nblk(n, kScanTile2);
-    hipLaunchKernelGGL(k_tile_reduce, dim3(ntiles), dim3(kScanThreads), 0, st, b->cnt, n, b->tile_sum);
-    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(kScanThreads), 0, st, b->tile_sum, ntiles);
-    hipLaunchKernelGGL(k_tile_apply, dim3(ntiles), dim3(kScanThreads), 0, st, b->cnt, n, b->tile_sum, b->start);
+    // a stalled scan is reported through replica 0's flags
+    hipLaunchKernelGGL(k_scan_lookback, dim3(ntiles), dim3(kScanThreads), 0, st, b->cnt, n, b->start,
+                       b->scan_ticket, b->scan_state, ntiles, b->flags);
     hipLaunchKernelGGL(k_scatter, dim3(nblk(std::max(ns, n), 256)), dim3(256), 0, st, b->G, b->key, b->rank,
-                       b->start, b->src, b->scell, b->cnt, b->counter, nullptr);
+                       b->start, b->src, b->scell, b->cnt, b->counter, nullptr, nullptr, nullptr);
     hipLaunchKernelGGL(kb_finalize, dim3(nblk(ns, 256)), dim3(256), 0, st, b->d_reps, b->wave_rep, ns, b->start,
                        b->src, b->scell, b->P_[c], b->V_[c], b->ID_[c], b->P_[nc], b->V_[nc], b->ID_[nc],
                        b->slot_of, b->flags);
@@ -336,6 +337,7 @@ int batch_check_flags(mavi_batch *b) {
         HIPCHK(ctx, hipMemsetAsync(b->flags, 0, sizeof(uint32_t) * nr, ctx->stream));
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
         const std::string who = "replica " + std::to_string(r) + ": ";
+        if (h[r] & MAVI_FLAG_SCAN_STALL) return set_err(ctx, MAVI_E_HIP, "chunk-count scan: look-back did not complete");
         if (h[r] & MAVI_FLAG_NONFINITE)
             return set_err(ctx, MAVI_E_NONFINITE, who + "non-finite particle position met while binning");
         if (h[r] & MAVI_FLAG_OUT_OF_BOX)
@@ -409,7 +411,7 @@ int mavi_batch_destroy(mavi_batch *b) {
     dfree(b->ring_wave_rep); dfree(b->d_reps); dfree(b->wave_rep);
     dfree(b->F); dfree(b->P2); dfree(b->F1);
     dfree(b->key); dfree(b->rank); dfree(b->src); dfree(b->scell); dfree(b->slot_of);
-    dfree(b->cnt); dfree(b->start); dfree(b->tile_sum); dfree(b->flags); dfree(b->counter);
+    dfree(b->cnt); dfree(b->start); dfree(b->scan_state); dfree(b->scan_ticket); dfree(b->flags); dfree(b->counter);
     dfree(b->active_ids); dfree(b->pos_id); dfree(b->vel_id); dfree(b->force_id);
     dfree(b->pol); dfree(b->areas); dfree(b->cms); dfree(b->cms_next); dfree(b->cpos);
     dfree(b->rtype); dfree(b->types); dfree(b->inter);
@@ -521,7 +523,8 @@ int mavi_batch_finalize(mavi_batch *b) {
     ALLOC(b->slot_of, nids);
     ALLOC(b->cnt, ncells + 1);
     ALLOC(b->start, ncells + 1);
-    ALLOC(b->tile_sum, nblk(ncells + 1, kScanTile2));
+    ALLOC(b->scan_state, nblk(ncells + 1, kScanTile2));
+    ALLOC(b->scan_ticket, 1);
     ALLOC(b->flags, nr);
     ALLOC(b->counter, 1);
     ALLOC(b->pos_id, nids);

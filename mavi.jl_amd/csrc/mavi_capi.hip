@@ -19,6 +19,7 @@ int check_flags(mavi_sys *s) {
     if (h == 0) return MAVI_OK;
     HIPCHK(ctx, hipMemsetAsync(s->flags, 0, sizeof(uint32_t), ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (h & MAVI_FLAG_SCAN_STALL) return set_err(ctx, MAVI_E_HIP, "chunk-count scan: look-back did not complete");
     if (h & MAVI_FLAG_NONFINITE)
         return set_err(ctx, MAVI_E_NONFINITE, "non-finite particle position met while binning");
     if (h & MAVI_FLAG_OUT_OF_BOX)
@@ -88,8 +89,8 @@ int mavi_sys_destroy(mavi_sys *s) {
     }
     dfree(s->F); dfree(s->P2); dfree(s->F1);
     dfree(s->key); dfree(s->rank); dfree(s->src); dfree(s->scell); dfree(s->slot_of);
-    dfree(s->cnt); dfree(s->start); dfree(s->tile_sum); dfree(s->flags); dfree(s->counter);
-    dfree(s->stamp); dfree(s->epoch);
+    dfree(s->cnt); dfree(s->start); dfree(s->scan_state); dfree(s->scan_ticket); dfree(s->flags); dfree(s->counter);
+    dfree(s->stamp); dfree(s->epoch); dfree(s->perm); dfree(s->needc);
     dfree(s->active_ids); dfree(s->pos_id); dfree(s->vel_id); dfree(s->force_id);
     dfree(s->pol); dfree(s->areas); dfree(s->cms); dfree(s->cms_next); dfree(s->cpos);
     dfree(s->rtype); dfree(s->types); dfree(s->inter); dfree(s->noise); dfree(s->red);
@@ -254,12 +255,15 @@ int mavi_sys_create(mavi_ctx *ctx, const mavi_sys_desc *d, mavi_sys **out) {
     ALLOC(s->slot_of, P.n);
     ALLOC(s->flags, 1);
     ALLOC(s->stamp, (int64_t)P.ncell + 1);
+    ALLOC(s->perm, P.na);
+    ALLOC(s->needc, (int64_t)P.ncell + 1);
     ALLOC(s->epoch, 1);
     ALLOC(s->counter, 1);
     ALLOC(s->cnt, (int64_t)P.ncell + 1);
     ALLOC(s->start, (int64_t)P.ncell + 1);
     s->nscan_blocks = (P.ncell + 1 + 4095) / 4096;
-    ALLOC(s->tile_sum, s->nscan_blocks);
+    ALLOC(s->scan_state, s->nscan_blocks);
+    ALLOC(s->scan_ticket, 1);
     ALLOC(s->pos_id, P.n);
     if (!rings) ALLOC(s->vel_id, P.n);
     ALLOC(s->force_id, P.n);

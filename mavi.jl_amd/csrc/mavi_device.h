@@ -461,54 +461,114 @@ __global__ void k_unpermute(int32_t na, const int32_t *__restrict__ ID, const do
     if (force_id) force_id[id] = F[t];
 }
 
-// Exclusive scan of cnt[0..n) in three passes with a fixed cost: tile sums,
-// a one-workgroup scan of the tile sums, then each tile's local scan plus
-// its offset.  Tiles hold kScanTile2 = 4096 counts.
+// Exclusive scan of cnt[0..n) in one pass (decoupled look-back: Merrill and
+// Garland, "Single-pass Parallel Prefix Scan with Decoupled Look-back", 2016).
+// A block takes its tile from a 64-bit ticket counter that is never reset, so
+// tiles are started in order and a block only waits on tiles already
+// running.  The ticket's quotient by the tile count is the launch's epoch;
+// every published tile state carries it, so no state is cleared between
+// launches (and a captured graph needs no memset node).  State word:
+// bits 63..34 epoch, 33..32 kind (1 = tile total, 2 = inclusive prefix),
+// 31..0 value.  Each tile holds kScanTile2 = 4096 counts.  A look-back that
+// does not see its predecessors within a bounded number of polls raises
+// MAVI_FLAG_SCAN_STALL and takes 0 for the missing part instead of waiting
+// for ever.
 constexpr int kScanItems2 = 16;
 constexpr int kScanTile2 = kScanThreads * kScanItems2;
+constexpr uint32_t kScanMaxPolls = 1u << 20;
 
-__global__ void k_tile_reduce(const int32_t *__restrict__ in, int32_t n, int32_t *__restrict__ tile_sum) {
-    const int base = blockIdx.x * kScanTile2;
-    int32_t sum = 0;
-#pragma unroll
-    for (int k = 0; k < kScanItems2; k++) {
-        const int i = base + k * kScanThreads + threadIdx.x;  // coalesced
-        sum += (i < n) ? in[i] : 0;
-    }
-    int32_t total;
-    (void)block_excl_scan(sum, &total);
-    if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
+__device__ __forceinline__ unsigned long long scan_word(uint32_t epoch, uint32_t kind, int32_t v) {
+    return ((unsigned long long)epoch << 34) | ((unsigned long long)kind << 32) | (uint32_t)v;
 }
 
-__global__ void k_tile_scan(int32_t *__restrict__ tile_sum, int32_t ntiles) {
-    __shared__ int32_t carry;
-    if (threadIdx.x == 0) carry = 0;
+__global__ __launch_bounds__(kScanThreads) void k_scan_lookback(const int32_t *__restrict__ in, int32_t n,
+                                                                int32_t *__restrict__ out,
+                                                                unsigned long long *__restrict__ ticket,
+                                                                unsigned long long *__restrict__ state,
+                                                                int32_t ntiles, uint32_t *__restrict__ flags) {
+    __shared__ int32_t s_tile, s_prefix;
+    __shared__ uint32_t s_epoch;
+    if (threadIdx.x == 0) {
+        const unsigned long long t = atomicAdd(ticket, 1ull);
+        s_tile = (int32_t)(t % (unsigned long long)ntiles);
+        s_epoch = (uint32_t)((t / (unsigned long long)ntiles) & 0x3fffffffull);
+    }
     __syncthreads();
-    for (int t0 = 0; t0 < ntiles; t0 += kScanThreads) {
-        const int i = t0 + threadIdx.x;
-        const int32_t v = (i < ntiles) ? tile_sum[i] : 0;
-        int32_t total;
-        const int32_t ex = block_excl_scan(v, &total);
-        const int32_t c = carry;
-        if (i < ntiles) tile_sum[i] = ex + c;
-        __syncthreads();
-        if (threadIdx.x == 0) carry = c + total;
-        __syncthreads();
-    }
-}
-
-__global__ void k_tile_apply(const int32_t *__restrict__ in, int32_t n, const int32_t *__restrict__ tile_sum,
-                             int32_t *__restrict__ out) {
-    const int base = blockIdx.x * kScanTile2 + threadIdx.x * kScanItems2;
+    const int tile = s_tile;
+    const uint32_t epoch = s_epoch;
+    const int base = tile * kScanTile2 + threadIdx.x * kScanItems2;
     int32_t v[kScanItems2];
+    if (base + kScanItems2 <= n) {
+        const int4 *q = reinterpret_cast<const int4 *>(in + base);
+#pragma unroll
+        for (int k = 0; k < kScanItems2 / 4; k++) {
+            const int4 w = q[k];
+            v[4 * k] = w.x;
+            v[4 * k + 1] = w.y;
+            v[4 * k + 2] = w.z;
+            v[4 * k + 3] = w.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanItems2; k++) v[k] = (base + k < n) ? in[base + k] : 0;
+    }
     int32_t sum = 0;
 #pragma unroll
-    for (int k = 0; k < kScanItems2; k++) {
-        v[k] = (base + k < n) ? in[base + k] : 0;
-        sum += v[k];
-    }
+    for (int k = 0; k < kScanItems2; k++) sum += v[k];
     int32_t total;
-    int32_t off = block_excl_scan(sum, &total) + tile_sum[blockIdx.x];
+    const int32_t local = block_excl_scan(sum, &total);
+    if (threadIdx.x == 0)
+        __hip_atomic_store(&state[tile], scan_word(epoch, tile == 0 ? 2u : 1u, total), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        int32_t excl = 0;
+        int j = tile - 1;  // nearest tile not yet accounted for
+        uint32_t polls = 0;
+        bool stalled = false;
+        while (j >= 0) {
+            const int t = j - lane;
+            uint32_t kind = 0;
+            int32_t val = 0;
+            if (t >= 0) {
+                const unsigned long long w =
+                    __hip_atomic_load(&state[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((uint32_t)(w >> 34) == epoch) {
+                    kind = (uint32_t)(w >> 32) & 3u;
+                    val = (int32_t)(uint32_t)w;
+                }
+            }
+            // tiles in [j-63, j]: stop at the nearest inclusive prefix, and
+            // never step over a tile that has published nothing yet
+            const unsigned long long pre = __ballot(t >= 0 && kind == 2u);
+            const unsigned long long none = __ballot(t >= 0 && kind == 0u);
+            const int stop = pre ? __ffsll(pre) - 1 : 64;
+            const unsigned long long below = stop >= 63 ? ~0ull : ((2ull << stop) - 1ull);
+            if (none & below) {
+                if (++polls >= kScanMaxPolls) {
+                    stalled = true;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            int32_t part = (lane <= stop && t >= 0) ? val : 0;
+#pragma unroll
+            for (int d = 32; d > 0; d >>= 1) part += __shfl_xor(part, d, 64);
+            excl += part;
+            if (pre) break;
+            j -= 64;
+        }
+        if (lane == 0) {
+            if (tile > 0)
+                __hip_atomic_store(&state[tile], scan_word(epoch, 2u, excl + total), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            if (stalled) atomicOr(flags, MAVI_FLAG_SCAN_STALL);
+            s_prefix = excl;
+        }
+    }
+    __syncthreads();
+    int32_t off = s_prefix + local;
 #pragma unroll
     for (int k = 0; k < kScanItems2; k++) {
         if (base + k < n) out[base + k] = off;
@@ -518,11 +578,14 @@ __global__ void k_tile_apply(const int32_t *__restrict__ in, int32_t n, const in
 
 // Place every particle at start[chunk] + arrival rank; reset the counts for
 // the next binning; advance the step counter of the device rotational noise
-// and the binning epoch when they are given.
+// when it is given.  With stamp: mark every chunk whose 3x3 block holds a
+// chunk stamped in this binning (stamp == epoch + 1) with needc = epoch + 1;
+// k_finalize advances the epoch afterwards.
 __global__ void k_scatter(SysParams P, const int32_t *__restrict__ key, const int32_t *__restrict__ rank,
                           const int32_t *__restrict__ start, int32_t *__restrict__ src, int32_t *__restrict__ scell,
                           int32_t *__restrict__ cnt, unsigned long long *__restrict__ counter,
-                          uint32_t *__restrict__ epoch) {
+                          const int32_t *__restrict__ stamp, const uint32_t *__restrict__ epoch,
+                          int32_t *__restrict__ needc) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < P.na) {
         const int k = key[i];
@@ -532,7 +595,23 @@ __global__ void k_scatter(SysParams P, const int32_t *__restrict__ key, const in
     }
     if (i <= P.ncell) cnt[i] = 0;
     if (i == 0 && counter) counter[0] += 1ull;
-    if (i == 0 && epoch) epoch[0] += 1u;
+    if (stamp && i < P.ncell) {
+        const int32_t e = (int32_t)(epoch[0] + 1u);
+        const int row = i / P.cols, col = i - row * P.cols;
+        bool need = false;
+        for (int dr = -1; dr <= 1; dr++) {
+            int r = row + dr;
+            if (P.periodic) r = r < 0 ? r + P.rows : (r >= P.rows ? r - P.rows : r);
+            else if (r < 0 || r >= P.rows) continue;
+            for (int dc = -1; dc <= 1; dc++) {
+                int cc = col + dc;
+                if (P.periodic) cc = cc < 0 ? cc + P.cols : (cc >= P.cols ? cc - P.cols : cc);
+                else if (cc < 0 || cc >= P.cols) continue;
+                need = need || stamp[r * P.cols + cc] == e;
+            }
+        }
+        if (need) needc[i] = e;
+    }
 }
 
 // Unwrap (update_continuos_pos!, :118-136), centre of mass and shoelace area

@@ -13,7 +13,8 @@
 //   src, scell   int32[na]     new slot -> old slot / chunk of new slot
 //   slot_of      int32[n]      particle id -> sorted slot (rings)
 //   cnt, start   int32[ncell+1] particles per chunk / exclusive scan
-//   tile_sum     int32[ntiles]  scan tile sums (4096 chunks per tile)
+//   scan_state   u64[ntiles]   published tile states of the one-pass scan
+//                              (4096 chunks per tile), scan_ticket its counter
 //   pos_id, vel_id, force_id   double2[n] id-order staging for the host
 // na = number of active ids (get_particles_ids); the chunk index is
 // row0*cols + col0 with row0 = reference row - 1 (row 1 = top of the box,
@@ -30,6 +31,7 @@
 #define MAVI_FLAG_OVERFLOW 1u
 #define MAVI_FLAG_OUT_OF_BOX 2u
 #define MAVI_FLAG_NONFINITE 4u
+#define MAVI_FLAG_SCAN_STALL 8u
 
 struct mavi_ctx {
     int device = 0;
@@ -92,7 +94,7 @@ struct mavi_sys {
     double2 *F = nullptr, *P2 = nullptr, *F1 = nullptr;
     int32_t *key = nullptr, *rank = nullptr, *src = nullptr, *scell = nullptr, *slot_of = nullptr;
     int32_t *cnt = nullptr, *start = nullptr;
-    int32_t *tile_sum = nullptr;            // scan tile sums
+    unsigned long long *scan_state = nullptr, *scan_ticket = nullptr;  // one-pass scan
     uint32_t *flags = nullptr;
     unsigned long long *counter = nullptr;  // device step counter (rotational noise)
     int32_t *active_ids = nullptr;          // na ids ascending
@@ -115,6 +117,8 @@ struct mavi_sys {
     // reuse of the second force evaluation of a Verlet step as the first one
     // of the next step where a particle's 3x3 chunk block kept its members
     int32_t *stamp = nullptr;   // per chunk: binning epoch in which its membership last changed
+    int32_t *needc = nullptr;   // per chunk: binning epoch in which a chunk of its 3x3 block changed
+    int32_t *perm = nullptr;    // sorted slot -> slot before the last binning
     uint32_t *epoch = nullptr;  // binnings so far (device)
     bool reuse = true;          // option
     bool f_valid = false;       // F holds the last step's second evaluation
@@ -159,7 +163,11 @@ hipError_t launch_iota_state(mavi_sys *s, hipStream_t st);          // id-order 
 hipError_t launch_unpermute(mavi_sys *s, bool vel, bool force, hipStream_t st);  // state -> staging
 // binning + permutation (cur flips); count_step advances the step counter
 // of the device rotational noise (a step's binning, not a lone update_chunks!)
-hipError_t launch_bin(mavi_sys *s, hipStream_t st, bool count_step = false, bool carry_forces = false);
+// stamp: mark chunks whose membership changed (force reuse); defer: leave
+// velocities in the old order and write perm (new slot -> old slot) for the
+// next pair launch to move them
+hipError_t launch_bin(mavi_sys *s, hipStream_t st, bool count_step = false, bool stamp = false,
+                      bool defer = false);
 hipError_t launch_export_chunks(mavi_sys *s, int64_t *d_cp, int64_t *d_counts, hipStream_t st);
 hipError_t launch_forces_only(mavi_sys *s, hipStream_t st);
 // One newton_step!; reuse = the previous call was a Verlet step of this

@@ -1,7 +1,7 @@
 // mavi_kernels.hip -- gfx950 kernels of the step hot path.
 //
-//   K1 binning      k_bin_count (wave-aggregated chunk counts) -> k_tile_*
-//                   (reduce-then-scan) -> k_scatter -> k_finalize
+//   K1 binning      k_bin_count (wave-aggregated chunk counts) ->
+//                   k_scan_lookback (one-pass scan) -> k_scatter -> k_finalize
 //                   (ids ascending inside a chunk, as update_chunks! appends
 //                   them, src/chunks.jl:150-161; the state is permuted into
 //                   the new chunk order)
@@ -64,14 +64,16 @@ __global__ void k_bin_count(SysParams P, const double2 *__restrict__ pos, int32_
 
 // One thread per new slot: the final slot inside the chunk is the number of
 // chunk members with a smaller id (update_chunks! appends ids ascending);
-// move the particle's state there (and, with Fnew, its last forces).
+// move the particle's position and id there, and its velocity (Vnew) or
+// its old slot (perm: the next pair launch moves velocity and forces).
 __global__ void k_finalize(SysParams P, const int32_t *__restrict__ start, const int32_t *__restrict__ src,
                            const int32_t *__restrict__ scell, const double2 *__restrict__ Pold,
                            const double2 *__restrict__ Vold, const int32_t *__restrict__ IDold,
                            double2 *__restrict__ Pnew, double2 *__restrict__ Vnew, int32_t *__restrict__ IDnew,
                            int32_t *__restrict__ slot_of, uint32_t *__restrict__ flags,
-                           const double2 *__restrict__ Fold, double2 *__restrict__ Fnew) {
+                           int32_t *__restrict__ perm, uint32_t *__restrict__ epoch) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) epoch[0] += 1u;  // binnings so far
     if (t >= P.na) return;
     const int c = scell[t];
     const int b = start[c], e = start[c + 1];
@@ -82,7 +84,7 @@ __global__ void k_finalize(SysParams P, const int32_t *__restrict__ start, const
     const int t2 = b + r;
     Pnew[t2] = Pold[so];
     if (Vnew) Vnew[t2] = Vold[so];
-    if (Fnew) Fnew[t2] = Fold[so];
+    if (perm) perm[t2] = so;
     IDnew[t2] = myid;
     if (slot_of) slot_of[myid] = t2;
     if (t == b && e - b > P.nc) atomicOr(flags, MAVI_FLAG_OVERFLOW);
@@ -106,48 +108,59 @@ __global__ void k_export_chunks(SysParams P, const int32_t *__restrict__ start, 
 // K2/K3 pair forces with fused integrator epilogues
 // ---------------------------------------------------------------------------
 
-// With stamp (first evaluation of a Verlet step only): a wave none of whose
-// particles has a chunk of its 3x3 block stamped in this binning's epoch
-// takes its forces from F1, where the binning moved the previous step's
-// second evaluation.  Those forces were computed at the same positions over
-// the same candidates in the same order, so they are bit-identical to a
+// With perm (first evaluation of a Verlet step after a binning that left
+// velocities in the old order): velocities move to the new order here,
+// Vst[s] = Vold[perm[s]].  With needc (needs perm): a particle whose chunk
+// is not marked in this binning's epoch (no chunk of its 3x3 block changed
+// members) takes its forces from Fold[perm[s]], the previous step's second
+// evaluation.  Those forces were computed at the same positions over the
+// same candidates in the same order, so they are bit-identical to a
 // recomputation.  (A compacted worklist of the particles to recompute was
 // slower: it loses the sorted order's neighbour locality.)
 template <int MODE, int POT, bool ALLPAIRS>
 __global__ void k_pair(SysParams P, RingCtx R, const double2 *__restrict__ pin, const int32_t *__restrict__ scell,
                        const int32_t *__restrict__ start, double2 *__restrict__ Pst, double2 *__restrict__ Vst,
                        double2 *__restrict__ Fout, double2 *__restrict__ P2, double2 *__restrict__ F1,
-                       double list2, const int32_t *__restrict__ stamp, const uint32_t *__restrict__ epoch) {
+                       double list2, const int32_t *__restrict__ needc, const uint32_t *__restrict__ epoch,
+                       const int32_t *__restrict__ perm, const double2 *__restrict__ Vold,
+                       const double2 *__restrict__ Fold) {
     extern __shared__ uint16_t s_list[];
-    const int s = xcd_block() * blockDim.x + threadIdx.x;
-    const bool valid = s < P.na;
-    const int sv = valid ? s : 0;
-    const double2 pi = pin[sv];
+    int s = xcd_block() * blockDim.x + threadIdx.x;
+    bool valid = s < P.na;
+    int sv = valid ? s : 0;
+    double2 pi = pin[sv];
     if constexpr (MODE == MODE_VERLET1 && !ALLPAIRS) {
-        if (stamp) {
-            bool need = false;
-            if (valid) {
-                const int32_t e = (int32_t)epoch[0];
-                const int c = scell[sv];
-                const int row = c / P.cols, col = c - row * P.cols;
-                for (int dr = -1; dr <= 1; dr++) {
-                    int r = row + dr;
-                    if (P.periodic) r = r < 0 ? r + P.rows : (r >= P.rows ? r - P.rows : r);
-                    else if (r < 0 || r >= P.rows) continue;
-                    for (int dc = -1; dc <= 1; dc++) {
-                        int cc = col + dc;
-                        if (P.periodic) cc = cc < 0 ? cc + P.cols : (cc >= P.cols ? cc - P.cols : cc);
-                        else if (cc < 0 || cc >= P.cols) continue;
-                        need = need || stamp[r * P.cols + cc] == e;
-                    }
-                }
-            }
-            if (!__any(need)) {
-                if (!valid) return;
-                const double2 f = F1[s];
+        const int so = (perm && valid) ? perm[s] : 0;
+        if (perm && valid) Vst[s] = Vold[so];
+        if (needc) {
+            // chunks whose 3x3 block changed members in this binning
+            const bool need = valid && needc[scell[sv]] == (int32_t)epoch[0];
+            if (valid && !need) {
+                const double2 f = Fold[so];
                 pair_epilogue<MODE>(P, s, pi, f.x, f.y, Pst, Vst, Fout, P2, F1);
-                return;
             }
+            // the block's particles to recompute go to its first lanes (a
+            // particle's forces do not depend on the lane or the wave it is
+            // computed in); waves left without one exit
+            __shared__ int32_t s_redo[1024];
+            __shared__ int32_t s_nredo;
+            if (threadIdx.x == 0) s_nredo = 0;
+            __syncthreads();
+            const unsigned long long m = __ballot(need);
+            if (m) {
+                const int leader = __ffsll(m) - 1;
+                int base = 0;
+                if ((int)lane_id() == leader) base = atomicAdd(&s_nredo, __popcll(m));
+                base = __shfl(base, leader, 64);
+                if (need) s_redo[base + __popcll(m & ((1ull << lane_id()) - 1ull))] = s;
+            }
+            __syncthreads();
+            const int nredo = s_nredo;
+            if ((int)(threadIdx.x & ~63u) >= nredo) return;  // whole wave
+            valid = (int)threadIdx.x < nredo;
+            s = s_redo[valid ? threadIdx.x : 0];
+            sv = s;
+            pi = pin[sv];
         }
     }
     PairAcc<POT> acc;
@@ -559,7 +572,7 @@ double list_range2(const mavi_sys *s) {
 }
 
 template <int MODE, int POT, bool ALL>
-hipError_t launch_pair_t(mavi_sys *s, const double2 *pin, hipStream_t st, bool reuse) {
+hipError_t launch_pair_t(mavi_sys *s, const double2 *pin, hipStream_t st, bool reuse, bool defer) {
     const int wg = s->wg;
     const int c = s->cur;
     const double l2 = s->two_phase ? list_range2(s) : INFINITY;
@@ -581,24 +594,24 @@ hipError_t launch_pair_t(mavi_sys *s, const double2 *pin, hipStream_t st, bool r
     }
     const size_t lds = (l2 < INFINITY) ? sizeof(uint16_t) * kListCap * (size_t)wg : 0;
     hipLaunchKernelGGL((k_pair<MODE, POT, ALL>), dim3(nblk(s->P.na, wg)), dim3(wg), lds, st, s->P, ring_ctx(s), pin,
-                       s->scell, s->start, s->P_[c], s->V_[c], s->F, s->P2, s->F1, l2, reuse ? s->stamp : nullptr,
-                       s->epoch);
+                       s->scell, s->start, s->P_[c], s->V_[c], s->F, s->P2, s->F1, l2, reuse ? s->needc : nullptr,
+                       s->epoch, defer ? s->perm : nullptr, s->V_[1 - c], s->F);
     return hipGetLastError();
 }
 
 template <int MODE>
-hipError_t launch_pair(mavi_sys *s, const double2 *pin, hipStream_t st, bool reuse = false) {
+hipError_t launch_pair(mavi_sys *s, const double2 *pin, hipStream_t st, bool reuse = false, bool defer = false) {
     const bool all = s->P.cols == 0;
     switch (s->P.potential) {
         case MAVI_POT_LJ:
-            return all ? launch_pair_t<MODE, POT_LJ, true>(s, pin, st, false)
-                       : launch_pair_t<MODE, POT_LJ, false>(s, pin, st, reuse);
+            return all ? launch_pair_t<MODE, POT_LJ, true>(s, pin, st, false, false)
+                       : launch_pair_t<MODE, POT_LJ, false>(s, pin, st, reuse, defer);
         case MAVI_POT_HARMTRUNC:
-            return all ? launch_pair_t<MODE, POT_HT, true>(s, pin, st, false)
-                       : launch_pair_t<MODE, POT_HT, false>(s, pin, st, reuse);
+            return all ? launch_pair_t<MODE, POT_HT, true>(s, pin, st, false, false)
+                       : launch_pair_t<MODE, POT_HT, false>(s, pin, st, reuse, defer);
         default:
-            return all ? launch_pair_t<MODE, POT_RINGS, true>(s, pin, st, false)
-                       : launch_pair_t<MODE, POT_RINGS, false>(s, pin, st, false);
+            return all ? launch_pair_t<MODE, POT_RINGS, true>(s, pin, st, false, false)
+                       : launch_pair_t<MODE, POT_RINGS, false>(s, pin, st, false, false);
     }
 }
 
@@ -649,27 +662,27 @@ hipError_t launch_unpermute(mavi_sys *s, bool vel, bool force, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_bin(mavi_sys *s, hipStream_t st, bool count_step, bool carry_forces) {
+hipError_t launch_bin(mavi_sys *s, hipStream_t st, bool count_step, bool stamp, bool defer) {
     const SysParams &P = s->P;
     if (P.cols == 0) return hipSuccess;  // chunks = nothing: the state order stays the active id order
     ProfScope ps(s, MAVI_PROF_BIN, st);
     const int c = s->cur, nc = 1 - c;
     // stamps need the previous binning's chunk of every slot (scell)
-    const bool stamping = carry_forces && s->sorted;
+    const bool stamping = stamp && s->sorted;
     hipLaunchKernelGGL(k_bin_count, dim3(nblk(P.na, 256)), dim3(256), 0, st, P, s->P_[c], s->key, s->rank, s->cnt,
                        s->flags, s->scell, stamping ? s->stamp : nullptr, s->epoch);
     const int n = P.ncell + 1;
     const int ntiles = nblk(n, kScanTile2);
-    hipLaunchKernelGGL(k_tile_reduce, dim3(ntiles), dim3(kScanThreads), 0, st, s->cnt, n, s->tile_sum);
-    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(kScanThreads), 0, st, s->tile_sum, ntiles);
-    hipLaunchKernelGGL(k_tile_apply, dim3(ntiles), dim3(kScanThreads), 0, st, s->cnt, n, s->tile_sum, s->start);
+    hipLaunchKernelGGL(k_scan_lookback, dim3(ntiles), dim3(kScanThreads), 0, st, s->cnt, n, s->start,
+                       s->scan_ticket, s->scan_state, ntiles, s->flags);
     const int grid = nblk(P.na > n ? P.na : n, 256);
     hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(256), 0, st, P, s->key, s->rank, s->start, s->src, s->scell,
-                       s->cnt, count_step ? s->counter : nullptr, s->epoch);
-    const bool has_vel = P.potential != MAVI_POT_RINGS;
+                       s->cnt, count_step ? s->counter : nullptr, stamping ? s->stamp : nullptr, s->epoch, s->needc);
+    const bool has_vel = P.potential != MAVI_POT_RINGS && !defer;
     hipLaunchKernelGGL(k_finalize, dim3(nblk(P.na, 256)), dim3(256), 0, st, P, s->start, s->src, s->scell, s->P_[c],
                        has_vel ? s->V_[c] : nullptr, s->ID_[c], s->P_[nc], has_vel ? s->V_[nc] : nullptr, s->ID_[nc],
-                       s->slot_of, s->flags, s->F, carry_forces ? s->F1 : nullptr);
+                       P.potential == MAVI_POT_RINGS ? s->slot_of : nullptr, s->flags, defer ? s->perm : nullptr,
+                       s->epoch);
     s->cur = nc;
     s->sorted = true;
     return hipGetLastError();
@@ -687,14 +700,16 @@ hipError_t launch_forces_only(mavi_sys *s, hipStream_t st) {
 }
 
 hipError_t launch_verlet_step(mavi_sys *s, hipStream_t st, bool reuse) {
-    // forces reuse needs the gather kernel for both evaluations
-    const bool can = s->reuse && !s->lds_tiles && s->P.cols > 0;
+    // the gather kernel's first evaluation moves velocities (and reused
+    // forces) into the new order, so the binning leaves them where they are
+    const bool defer = !s->lds_tiles && s->P.cols > 0;
+    const bool can = s->reuse && defer;
     reuse = reuse && can && s->sorted;
-    hipError_t e = launch_bin(s, st, true, can);
+    hipError_t e = launch_bin(s, st, true, can, defer);
     if (e != hipSuccess) return e;
     {
         ProfScope ps(s, reuse ? MAVI_PROF_FORCE_REUSE : MAVI_PROF_FORCE, st);
-        if ((e = launch_pair<MODE_VERLET1>(s, s->P_[s->cur], st, reuse)) != hipSuccess) return e;
+        if ((e = launch_pair<MODE_VERLET1>(s, s->P_[s->cur], st, reuse, defer)) != hipSuccess) return e;
     }
     ProfScope ps(s, MAVI_PROF_FORCE, st);
     e = launch_pair<MODE_VERLET2>(s, s->P2, st);
@@ -730,7 +745,7 @@ static hipError_t launch_rings_kernel(mavi_sys *s, bool update, const double *no
 }
 
 hipError_t launch_rings_forces(mavi_sys *s, hipStream_t st) {
-    hipError_t e = launch_bin(s, st, false, false);
+    hipError_t e = launch_bin(s, st, false, false, false);
     if (e != hipSuccess) return e;
     {
         ProfScope ps(s, MAVI_PROF_FORCE, st);
@@ -740,7 +755,7 @@ hipError_t launch_rings_forces(mavi_sys *s, hipStream_t st) {
 }
 
 hipError_t launch_rings_step(mavi_sys *s, const double *noise_dev, hipStream_t st) {
-    hipError_t e = launch_bin(s, st, true, false);
+    hipError_t e = launch_bin(s, st, true, false, false);
     if (e != hipSuccess) return e;
     if (s->P.cols == 0) {  // no binning kernel to advance the noise step counter
         hipLaunchKernelGGL(k_counter_inc, dim3(1), dim3(64), 0, st, s->counter);
