@@ -75,19 +75,20 @@ __global__ void kb_bin_count(const Replica *__restrict__ reps, const int32_t *__
 // keep id -1 in both state copies.
 __global__ void kb_finalize(const Replica *__restrict__ reps, const int32_t *__restrict__ wave_rep, int32_t nslots,
                             const int32_t *__restrict__ start, const int32_t *__restrict__ src,
-                            const int32_t *__restrict__ scell, const double2 *__restrict__ Pold,
+                            const int32_t *__restrict__ scell, const int32_t *__restrict__ sid,
+                            const double2 *__restrict__ Pold,
                             const double2 *__restrict__ Vold, const int32_t *__restrict__ IDold,
                             double2 *__restrict__ Pnew, double2 *__restrict__ Vnew, int32_t *__restrict__ IDnew,
                             int32_t *__restrict__ slot_of, uint32_t *__restrict__ flags) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nslots) return;
     const int so = src[t];
-    const int myid = IDold[so];
+    const int myid = sid[t];
     if (myid < 0) return;
     const int c = scell[t];
     const int b = start[c], e = start[c + 1];
     int rk = 0;
-    for (int q = b; q < e; q++) rk += (IDold[src[q]] < myid) ? 1 : 0;
+    for (int q = b; q < e; q++) rk += (sid[q] < myid) ? 1 : 0;
     const int t2 = b + rk;
     Pnew[t2] = Pold[so];
     Vnew[t2] = Vold[so];
@@ -200,7 +201,7 @@ struct mavi_batch {
     double2 *P_[2] = {nullptr, nullptr}, *V_[2] = {nullptr, nullptr};
     int32_t *ID_[2] = {nullptr, nullptr};
     double2 *F = nullptr, *P2 = nullptr, *F1 = nullptr;
-    int32_t *key = nullptr, *rank = nullptr, *src = nullptr, *scell = nullptr, *slot_of = nullptr;
+    int32_t *key = nullptr, *rank = nullptr, *src = nullptr, *scell = nullptr, *sid = nullptr, *slot_of = nullptr;
     int32_t *cnt = nullptr, *start = nullptr;
     unsigned long long *scan_state = nullptr, *scan_ticket = nullptr;
     uint32_t *flags = nullptr;
@@ -260,9 +261,9 @@ hipError_t batch_bin(mavi_batch *b, hipStream_t st) {
     hipLaunchKernelGGL(k_scan_lookback, dim3(ntiles), dim3(kScanThreads), 0, st, b->cnt, n, b->start,
                        b->scan_ticket, b->scan_state, ntiles, b->flags);
     hipLaunchKernelGGL(k_scatter, dim3(nblk(std::max(ns, n), 256)), dim3(256), 0, st, b->G, b->key, b->rank,
-                       b->start, b->src, b->scell, b->cnt, b->counter, nullptr, nullptr, nullptr);
+                       b->start, b->src, b->scell, b->ID_[c], b->sid, b->cnt, b->counter, nullptr, nullptr, nullptr);
     hipLaunchKernelGGL(kb_finalize, dim3(nblk(ns, 256)), dim3(256), 0, st, b->d_reps, b->wave_rep, ns, b->start,
-                       b->src, b->scell, b->P_[c], b->V_[c], b->ID_[c], b->P_[nc], b->V_[nc], b->ID_[nc],
+                       b->src, b->scell, b->sid, b->P_[c], b->V_[c], b->ID_[c], b->P_[nc], b->V_[nc], b->ID_[nc],
                        b->slot_of, b->flags);
     b->cur = nc;
     return hipGetLastError();
@@ -410,7 +411,7 @@ int mavi_batch_destroy(mavi_batch *b) {
     for (int k = 0; k < NKIND; k++) dfree(b->items[k]);
     dfree(b->ring_wave_rep); dfree(b->d_reps); dfree(b->wave_rep);
     dfree(b->F); dfree(b->P2); dfree(b->F1);
-    dfree(b->key); dfree(b->rank); dfree(b->src); dfree(b->scell); dfree(b->slot_of);
+    dfree(b->key); dfree(b->rank); dfree(b->src); dfree(b->scell); dfree(b->sid); dfree(b->slot_of);
     dfree(b->cnt); dfree(b->start); dfree(b->scan_state); dfree(b->scan_ticket); dfree(b->flags); dfree(b->counter);
     dfree(b->active_ids); dfree(b->pos_id); dfree(b->vel_id); dfree(b->force_id);
     dfree(b->pol); dfree(b->areas); dfree(b->cms); dfree(b->cms_next); dfree(b->cpos);
@@ -520,6 +521,7 @@ int mavi_batch_finalize(mavi_batch *b) {
     ALLOC(b->rank, nslots);
     ALLOC(b->src, nslots);
     ALLOC(b->scell, nslots);
+    ALLOC(b->sid, nslots);
     ALLOC(b->slot_of, nids);
     ALLOC(b->cnt, ncells + 1);
     ALLOC(b->start, ncells + 1);

@@ -88,7 +88,7 @@ int mavi_sys_destroy(mavi_sys *s) {
         dfree(s->ID_[c]);
     }
     dfree(s->F); dfree(s->P2); dfree(s->F1);
-    dfree(s->key); dfree(s->rank); dfree(s->src); dfree(s->scell); dfree(s->slot_of);
+    dfree(s->key); dfree(s->rank); dfree(s->src); dfree(s->scell); dfree(s->sid); dfree(s->slot_of);
     dfree(s->cnt); dfree(s->start); dfree(s->scan_state); dfree(s->scan_ticket); dfree(s->flags); dfree(s->counter);
     dfree(s->stamp); dfree(s->epoch); dfree(s->perm); dfree(s->needc);
     dfree(s->active_ids); dfree(s->pos_id); dfree(s->vel_id); dfree(s->force_id);
@@ -252,6 +252,7 @@ int mavi_sys_create(mavi_ctx *ctx, const mavi_sys_desc *d, mavi_sys **out) {
     ALLOC(s->rank, P.na);
     ALLOC(s->src, P.na);
     ALLOC(s->scell, P.na);
+    ALLOC(s->sid, P.na);
     ALLOC(s->slot_of, P.n);
     ALLOC(s->flags, 1);
     ALLOC(s->stamp, (int64_t)P.ncell + 1);

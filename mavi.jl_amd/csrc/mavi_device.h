@@ -52,29 +52,25 @@ __device__ __forceinline__ int32_t cell_of(const SysParams &P, double x, double 
     return (int32_t)(r * P.cols + c);
 }
 
-// Arrival rank of each lane's particle in its chunk k.  Lanes of a wave that
-// land in the same chunk (most of them: the state is sorted by the previous
-// step's chunks) are counted by one atomic of their leader lane.
+// Arrival rank of each lane's particle in its chunk k.  Consecutive lanes
+// with the same chunk form a run (the state is sorted by the previous step's
+// chunks, so a wave holds ~15 runs); the first lane of a run reserves the
+// run's ranks with one atomic.  Ranks are unique for any input: a chunk
+// met in several runs gets one atomic per run.
 __device__ __forceinline__ int32_t wave_arrival_rank(const bool valid, const int32_t k, int32_t *__restrict__ cnt) {
     const unsigned lane = lane_id();
-    const unsigned long long lt = (1ull << lane) - 1ull;
-    unsigned long long remaining = __ballot(valid);
-    int my_leader = (int)lane, my_pos = 0, grp_n = 0;
-    while (remaining) {
-        const int leader = __ffsll((unsigned long long)remaining) - 1;
-        const int lkey = __shfl(k, leader, 64);
-        const unsigned long long grp = __ballot(valid && k == lkey);
-        if (valid && k == lkey) {
-            my_leader = leader;
-            my_pos = __popcll(grp & lt);
-        }
-        if ((int)lane == leader) grp_n = __popcll(grp);
-        remaining &= ~grp;
-    }
+    const int32_t kprev = __shfl_up(k, 1, 64);
+    const int vprev = __shfl_up(valid ? 1 : 0, 1, 64);
+    const bool head = lane == 0 || k != kprev || (valid ? 1 : 0) != vprev;
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long upto = (2ull << lane) - 1ull;  // lanes 0..lane (all 64 for lane 63)
+    const int first = 63 - __clzll((long long)(heads & upto));
+    const unsigned long long after = heads & ~upto;
+    const int end = after ? __ffsll((long long)after) - 1 : 64;
     int base = 0;
-    if (valid && (int)lane == my_leader) base = atomicAdd(&cnt[k], grp_n);
-    base = __shfl(base, my_leader, 64);
-    return base + my_pos;
+    if (valid && (int)lane == first) base = atomicAdd(&cnt[k], end - first);
+    base = __shfl(base, first, 64);
+    return base + (int)lane - first;
 }
 
 constexpr int kScanThreads = 256;
@@ -576,13 +572,15 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_lookback(const int32_t *_
     }
 }
 
-// Place every particle at start[chunk] + arrival rank; reset the counts for
+// Place every particle at start[chunk] + arrival rank (old slot src, chunk
+// scell and id sid of each arrival-order slot); reset the counts for
 // the next binning; advance the step counter of the device rotational noise
 // when it is given.  With stamp: mark every chunk whose 3x3 block holds a
 // chunk stamped in this binning (stamp == epoch + 1) with needc = epoch + 1;
 // k_finalize advances the epoch afterwards.
 __global__ void k_scatter(SysParams P, const int32_t *__restrict__ key, const int32_t *__restrict__ rank,
                           const int32_t *__restrict__ start, int32_t *__restrict__ src, int32_t *__restrict__ scell,
+                          const int32_t *__restrict__ ID, int32_t *__restrict__ sid,
                           int32_t *__restrict__ cnt, unsigned long long *__restrict__ counter,
                           const int32_t *__restrict__ stamp, const uint32_t *__restrict__ epoch,
                           int32_t *__restrict__ needc) {
@@ -592,6 +590,7 @@ __global__ void k_scatter(SysParams P, const int32_t *__restrict__ key, const in
         const int t = start[k] + rank[i];
         src[t] = i;
         scell[t] = k;
+        sid[t] = ID[i];
     }
     if (i <= P.ncell) cnt[i] = 0;
     if (i == 0 && counter) counter[0] += 1ull;

@@ -11,6 +11,7 @@
 //                              forces inside a Verlet step
 //   key, rank    int32[na]     new chunk of each old slot / rank in it
 //   src, scell   int32[na]     new slot -> old slot / chunk of new slot
+//   sid          int32[na]     id of each new slot in arrival order
 //   slot_of      int32[n]      particle id -> sorted slot (rings)
 //   cnt, start   int32[ncell+1] particles per chunk / exclusive scan
 //   scan_state   u64[ntiles]   published tile states of the one-pass scan
@@ -92,7 +93,7 @@ struct mavi_sys {
     double2 *P_[2] = {nullptr, nullptr}, *V_[2] = {nullptr, nullptr};
     int32_t *ID_[2] = {nullptr, nullptr};
     double2 *F = nullptr, *P2 = nullptr, *F1 = nullptr;
-    int32_t *key = nullptr, *rank = nullptr, *src = nullptr, *scell = nullptr, *slot_of = nullptr;
+    int32_t *key = nullptr, *rank = nullptr, *src = nullptr, *scell = nullptr, *sid = nullptr, *slot_of = nullptr;
     int32_t *cnt = nullptr, *start = nullptr;
     unsigned long long *scan_state = nullptr, *scan_ticket = nullptr;  // one-pass scan
     uint32_t *flags = nullptr;

@@ -67,7 +67,8 @@ __global__ void k_bin_count(SysParams P, const double2 *__restrict__ pos, int32_
 // move the particle's position and id there, and its velocity (Vnew) or
 // its old slot (perm: the next pair launch moves velocity and forces).
 __global__ void k_finalize(SysParams P, const int32_t *__restrict__ start, const int32_t *__restrict__ src,
-                           const int32_t *__restrict__ scell, const double2 *__restrict__ Pold,
+                           const int32_t *__restrict__ scell, const int32_t *__restrict__ sid,
+                           const double2 *__restrict__ Pold,
                            const double2 *__restrict__ Vold, const int32_t *__restrict__ IDold,
                            double2 *__restrict__ Pnew, double2 *__restrict__ Vnew, int32_t *__restrict__ IDnew,
                            int32_t *__restrict__ slot_of, uint32_t *__restrict__ flags,
@@ -78,9 +79,9 @@ __global__ void k_finalize(SysParams P, const int32_t *__restrict__ start, const
     const int c = scell[t];
     const int b = start[c], e = start[c + 1];
     const int so = src[t];
-    const int myid = IDold[so];
+    const int myid = sid[t];
     int r = 0;
-    for (int q = b; q < e; q++) r += (IDold[src[q]] < myid) ? 1 : 0;
+    for (int q = b; q < e; q++) r += (sid[q] < myid) ? 1 : 0;
     const int t2 = b + r;
     Pnew[t2] = Pold[so];
     if (Vnew) Vnew[t2] = Vold[so];
@@ -592,8 +593,11 @@ hipError_t launch_pair_t(mavi_sys *s, const double2 *pin, hipStream_t st, bool r
             return hipGetLastError();
         }
     }
-    const size_t lds = (l2 < INFINITY) ? sizeof(uint16_t) * kListCap * (size_t)wg : 0;
-    hipLaunchKernelGGL((k_pair<MODE, POT, ALL>), dim3(nblk(s->P.na, wg)), dim3(wg), lds, st, s->P, ring_ctx(s), pin,
+    // a first evaluation that reuses forces compacts its recomputations per
+    // block: large blocks leave fewer partly filled waves
+    const int wgk = (MODE == MODE_VERLET1 && reuse) ? 1024 : wg;
+    const size_t lds = (l2 < INFINITY) ? sizeof(uint16_t) * kListCap * (size_t)wgk : 0;
+    hipLaunchKernelGGL((k_pair<MODE, POT, ALL>), dim3(nblk(s->P.na, wgk)), dim3(wgk), lds, st, s->P, ring_ctx(s), pin,
                        s->scell, s->start, s->P_[c], s->V_[c], s->F, s->P2, s->F1, l2, reuse ? s->needc : nullptr,
                        s->epoch, defer ? s->perm : nullptr, s->V_[1 - c], s->F);
     return hipGetLastError();
@@ -677,9 +681,10 @@ hipError_t launch_bin(mavi_sys *s, hipStream_t st, bool count_step, bool stamp, 
                        s->scan_ticket, s->scan_state, ntiles, s->flags);
     const int grid = nblk(P.na > n ? P.na : n, 256);
     hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(256), 0, st, P, s->key, s->rank, s->start, s->src, s->scell,
-                       s->cnt, count_step ? s->counter : nullptr, stamping ? s->stamp : nullptr, s->epoch, s->needc);
+                       s->ID_[c], s->sid, s->cnt, count_step ? s->counter : nullptr, stamping ? s->stamp : nullptr, s->epoch, s->needc);
     const bool has_vel = P.potential != MAVI_POT_RINGS && !defer;
-    hipLaunchKernelGGL(k_finalize, dim3(nblk(P.na, 256)), dim3(256), 0, st, P, s->start, s->src, s->scell, s->P_[c],
+    hipLaunchKernelGGL(k_finalize, dim3(nblk(P.na, 256)), dim3(256), 0, st, P, s->start, s->src, s->scell, s->sid,
+                       s->P_[c],
                        has_vel ? s->V_[c] : nullptr, s->ID_[c], s->P_[nc], has_vel ? s->V_[nc] : nullptr, s->ID_[nc],
                        P.potential == MAVI_POT_RINGS ? s->slot_of : nullptr, s->flags, defer ? s->perm : nullptr,
                        s->epoch);
