@@ -124,6 +124,16 @@ __device__ __forceinline__ int32_t ring_of(uint32_t id, const RingCtx &R) {
     return q;
 }
 
+// LJ force over distance: fmod/dist = 4 eps (12 s^12/d^14 - 6 s^6/d^8)
+// = inv^4 (c12 inv^3 - c6), inv = 1/d^2; 0 outside the cutoff or for self.
+__device__ __forceinline__ double lj_scale(const SysParams &P, double r2, bool self_ok) {
+    const bool ok = self_ok && (r2 <= P.cut2);
+    const double inv = rcp_nr(r2);
+    const double inv3 = inv * inv * inv;
+    const double fr = inv * inv3 * __builtin_fma(P.lj_c12, inv3, -P.lj_c6);
+    return ok ? fr : 0.0;
+}
+
 template <int POT>
 struct PairAcc {
     double fx = 0.0, fy = 0.0;
@@ -134,12 +144,7 @@ struct PairAcc {
                                         int id_j) {
         const double r2 = dx * dx + dy * dy;  // sum(dr.^2): two roundings, no FMA
         if constexpr (POT == POT_LJ) {
-            // fmod/dist = 4 eps (12 s^12/d^14 - 6 s^6/d^8) = inv^4 (c12 inv^3 - c6), inv = 1/d^2
-            const bool ok = self_ok && (r2 <= P.cut2);
-            const double inv = rcp_nr(r2);
-            const double inv3 = inv * inv * inv;
-            double fr = inv * inv3 * __builtin_fma(P.lj_c12, inv3, -P.lj_c6);
-            fr = ok ? fr : 0.0;
+            const double fr = lj_scale(P, r2, self_ok);
             fx = __builtin_fma(fr, dx, fx);
             fy = __builtin_fma(fr, dy, fy);
         } else if constexpr (POT == POT_HT) {
@@ -238,6 +243,56 @@ __device__ __forceinline__ void scan3(const SysParams &P, const RingCtx &R, Pair
         acc.add(P, R, dx, dy, j != s, POT == POT_RINGS ? R.ids[j] - R.id_base : 0);
         j = jn;
         pj = pn;
+    }
+}
+
+constexpr int kLjUnroll = 2;  // candidates per iteration of the LJ gather loop
+
+// LJ form of scan3 with U candidates per iteration: their loads are issued
+// together and their force chains are independent, so one wave keeps U
+// chains in flight; the sums still run in candidate order.  A lane past its
+// last candidate repeats a valid one with a zero factor.
+template <int U, bool WRAP>
+__device__ __forceinline__ void scan3_lj(const SysParams &P, PairAcc<POT_LJ> &acc, const double2 *__restrict__ pin,
+                                         int s, double xi, double yi, int a0, int l0, int a1, int l1, int a2,
+                                         int l2) {
+    const int tot = l0 + l1 + l2;
+    const int b1 = l0, b2 = l0 + l1;
+    const int o0 = a0, o1 = a1 - l0, o2 = a2 - l0 - l1;
+    for (int k = 0; k < tot; k += U) {
+        int j[U];
+        bool in[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int kk = k + u;
+            in[u] = kk < tot;
+            const int kc = in[u] ? kk : k;
+            j[u] = kc + ((kc < b1) ? o0 : ((kc < b2) ? o1 : o2));
+        }
+        double px[U], py[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const double2 p = pin[j[u]];
+            px[u] = p.x;
+            py[u] = p.y;
+        }
+        double dx[U], dy[U], fr[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            dx[u] = xi - px[u];
+            dy[u] = yi - py[u];
+            if constexpr (WRAP) {
+                dx[u] = min_image(dx[u], P.L, P.halfL);
+                dy[u] = min_image(dy[u], P.H, P.halfH);
+            }
+            const double r2 = dx[u] * dx[u] + dy[u] * dy[u];
+            fr[u] = lj_scale(P, r2, in[u] && j[u] != s);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            acc.fx = __builtin_fma(fr[u], dx[u], acc.fx);
+            acc.fy = __builtin_fma(fr[u], dy[u], acc.fy);
+        }
     }
 }
 
@@ -346,8 +401,13 @@ __device__ __forceinline__ void pair_gather_chunks(const SysParams &P, const Rin
             }
         }
         if (direct) {
-            if (wrapw) scan3<POT, true>(P, R, acc, pin, s, pi.x, pi.y, a0, l0, a1, l1, a2, l2);
-            else scan3<POT, false>(P, R, acc, pin, s, pi.x, pi.y, a0, l0, a1, l1, a2, l2);
+            if constexpr (POT == POT_LJ) {
+                if (wrapw) scan3_lj<kLjUnroll, true>(P, acc, pin, s, pi.x, pi.y, a0, l0, a1, l1, a2, l2);
+                else scan3_lj<kLjUnroll, false>(P, acc, pin, s, pi.x, pi.y, a0, l0, a1, l1, a2, l2);
+            } else {
+                if (wrapw) scan3<POT, true>(P, R, acc, pin, s, pi.x, pi.y, a0, l0, a1, l1, a2, l2);
+                else scan3<POT, false>(P, R, acc, pin, s, pi.x, pi.y, a0, l0, a1, l1, a2, l2);
+            }
         }
     } else if (valid) {
         // waves touching the periodic left/right edge: split row windows
