@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define MAVI_ABI_VERSION 1
+#define MAVI_ABI_VERSION 2
 
 /* Status codes. */
 #define MAVI_OK 0
@@ -45,6 +45,10 @@ extern "C" {
 #define MAVI_POT_HARMTRUNC 1 /* HarmTruncCfg, src/configs.jl:341-368 */
 #define MAVI_POT_RINGS 2     /* RingsCfg + HarmTruncCfg/InteractionMatrix,
                                 src/rings/configs.jl:26-164 */
+#define MAVI_POT_SZABO 3     /* SzaboCfg, src/configs.jl:399-408, with a
+                                SelfPropelledState (src/states.jl:104-125) */
+#define MAVI_POT_RTP 4       /* RunTumbleCfg, src/configs.jl:410-415, with a
+                                SelfPropelledState */
 
 typedef struct mavi_ctx mavi_ctx;
 typedef struct mavi_sys mavi_sys;
@@ -58,7 +62,8 @@ typedef struct {
     int32_t potential;      /* MAVI_POT_* */
     double pot[4];          /* LJ: sigma, epsilon, rcut (<= 0: none, the
                                reference's LenJonesCfg); HT: k_rep, k_atr,
-                               dist_eq, dist_max */
+                               dist_eq, dist_max; Szabo: k_rep, k_adh, r_eq,
+                               r_max; RTP: sigma, epsilon */
     double dt;              /* IntCfg.dt */
     int32_t cols, rows;     /* ChunksCfg(num_cols, num_rows); 0,0 = chunks nothing */
     int32_t nc;             /* chunk capacity (src/chunks.jl:32-34), checked */
@@ -76,7 +81,9 @@ typedef struct {
     const double *interaction; /* ntypes x ntypes HarmTrunc (k_rep, k_atr,
                                   dist_eq, dist_max), column-major like
                                   InteractionMatrix.matrix */
-    uint64_t seed;          /* device rotational-noise stream (Philox4x32-10) */
+    uint64_t seed;          /* device noise stream (Philox4x32-10) */
+    double active[4];       /* Szabo: vo, mobility, relax_time, rot_diff;
+                               RTP: vo, tumble_rate (self-propelled only) */
 } mavi_sys_desc;
 
 /* Library / context ------------------------------------------------------ */
@@ -92,7 +99,8 @@ int mavi_sys_create(mavi_ctx *ctx, const mavi_sys_desc *desc, mavi_sys **out);
 int mavi_sys_destroy(mavi_sys *sys);
 
 /* State transfer (AoS host <-> device).  vel is SecondLawState.vel, pol is
- * RingsState.pol (nrings angles); either may be NULL. */
+ * RingsState.pol (nrings angles) or SelfPropelledState.pol_angle (n angles);
+ * either may be NULL. */
 int mavi_sys_upload(mavi_sys *sys, const double *pos, const double *vel, const double *pol);
 int mavi_sys_download(mavi_sys *sys, double *pos, double *vel, double *pol);
 
@@ -111,10 +119,20 @@ int mavi_forces(mavi_sys *sys, double *forces);
 /* get_forces(system) as left by the last step (2n doubles). */
 int mavi_get_forces(mavi_sys *sys, double *forces);
 
-/* nsteps x newton_step! (src/integration.jl:507-515) for LJ/HT systems, or
+/* nsteps x newton_step! (src/integration.jl:507-515) for LJ/HT systems,
  * nsteps x Rings.Integration.step! (src/rings/integration.jl:522-543) for
- * rings with device rotational noise.  State stays resident. */
+ * rings, or nsteps x szabo_step! / rtp_step! (src/integration.jl:517-535)
+ * for self-propelled systems, with device noise.  State stays resident. */
 int mavi_step(mavi_sys *sys, int64_t nsteps);
+
+/* nsteps x szabo_step! / rtp_step! (src/integration.jl:517-535) for
+ * self-propelled systems (PeriodicWalls).  noise replaces the global RNG,
+ * per step and particle id (0-based):
+ *   Szabo: noise[step*n + i]          standard normal, the randn() of :459
+ *   RTP:   noise[(step*n + i)*2 + 0]  uniform [0,1), the rand() of :484
+ *          noise[(step*n + i)*2 + 1]  uniform [0,1), the rand() of :486
+ * NULL draws from the device Philox4x32-10 stream (the same as mavi_step). */
+int mavi_sp_step(mavi_sys *sys, int64_t nsteps, const double *noise);
 
 /* Rings step with host-supplied standard-normal draws: noise[step*nrings + r]
  * replaces randn(system.rng) of src/rings/integration.jl:348.  NULL = zero. */

@@ -10,6 +10,11 @@
 
 namespace {
 
+// Self-propelled system (SelfPropelledState with SzaboCfg / RunTumbleCfg).
+bool is_active(const mavi_sys *s) {
+    return s->P.potential == MAVI_POT_SZABO || s->P.potential == MAVI_POT_RTP;
+}
+
 // Turn device flags raised since the last check into a status.
 int check_flags(mavi_sys *s) {
     mavi_ctx *ctx = s->ctx;
@@ -114,8 +119,13 @@ int build_params(const mavi_sys_desc *d, HostSys &h, std::string &err) {
     if (!(d->size[0] > 0.0) || !(d->size[1] > 0.0)) return fail(MAVI_E_BADARG, "box size must be > 0");
     if (d->wall != MAVI_WALL_RIGID && d->wall != MAVI_WALL_PERIODIC)
         return fail(MAVI_E_UNSUPPORTED, "only RigidWalls and PeriodicWalls rectangles run on the GPU");
-    if (d->potential < MAVI_POT_LJ || d->potential > MAVI_POT_RINGS)
+    if (d->potential < MAVI_POT_LJ || d->potential > MAVI_POT_RTP)
         return fail(MAVI_E_UNSUPPORTED, "unknown potential kind");
+    const bool active = d->potential == MAVI_POT_SZABO || d->potential == MAVI_POT_RTP;
+    if (active && d->wall != MAVI_WALL_PERIODIC)
+        return fail(MAVI_E_UNSUPPORTED,
+                    "self-propelled states run with PeriodicWalls (walls! with RigidWalls reads state.vel, which "
+                    "SelfPropelledState does not have)");
     const bool chunks = d->cols > 0 || d->rows > 0;
     if (chunks) {
         if (d->cols <= 0 || d->rows <= 0) return fail(MAVI_E_BADARG, "cols and rows must both be > 0");
@@ -164,6 +174,30 @@ int build_params(const mavi_sys_desc *d, HostSys &h, std::string &err) {
     } else if (d->potential == MAVI_POT_HARMTRUNC) {
         P.cut2 = exact_gt_threshold(d->pot[3]);
         P.eq2 = exact_lt_threshold(d->pot[2]);
+        h.list2 = P.cut2;
+    } else if (d->potential == MAVI_POT_SZABO) {
+        // pot: k_rep, k_adh, r_eq, r_max; active: vo, mobility, relax_time, rot_diff
+        if (!(d->pot[3] > d->pot[2]) || !(d->pot[2] > 0.0) || !(d->active[2] > 0.0))
+            return fail(MAVI_E_BADARG, "SzaboCfg needs 0 < r_eq < r_max and relax_time > 0");
+        P.cut2 = exact_gt_threshold(d->pot[3]);
+        P.sz_rep = d->pot[0] / (d->pot[3] - d->pot[2]);
+        P.sz_adh = d->pot[1] / d->pot[2];
+        P.ap_vo = d->active[0];
+        P.ap_mu = d->active[1];
+        P.ap_inv_tau = 1.0 / d->active[2];
+        P.ap_noise = sqrt(2.0 * d->active[3] * d->dt);
+        h.list2 = P.cut2;
+    } else if (d->potential == MAVI_POT_RTP) {
+        // pot: sigma, epsilon; the force is the LJ one cut at 2^(1/6) sigma
+        // (src/integration.jl:88-109); active: vo, tumble_rate
+        const double cutoff = pow(2.0, 1.0 / 6.0) * d->pot[0];
+        P.p2 = cutoff;
+        P.cut2 = exact_gt_threshold(cutoff);
+        const double s2 = d->pot[0] * d->pot[0], s6 = s2 * s2 * s2;
+        P.lj_c12 = 48.0 * d->pot[1] * s6 * s6;
+        P.lj_c6 = 24.0 * d->pot[1] * s6;
+        P.ap_vo = d->active[0];
+        P.ap_tumble = d->active[1] * d->dt;
         h.list2 = P.cut2;
     }
     h.active.clear();
@@ -308,7 +342,15 @@ int mavi_sys_upload(mavi_sys *s, const double *pos, const double *vel, const dou
     HIPCHK(ctx, hipSetDevice(ctx->device));
     const size_t bytes = sizeof(double2) * (size_t)s->P.n;
     HIPCHK(ctx, hipMemcpyAsync(s->pos_id, pos, bytes, hipMemcpyHostToDevice, ctx->stream));
-    if (vel && s->vel_id) HIPCHK(ctx, hipMemcpyAsync(s->vel_id, vel, bytes, hipMemcpyHostToDevice, ctx->stream));
+    if (is_active(s)) {
+        // pol_angle (n) -> the x components of the velocity staging
+        HIPCHK(ctx, hipMemsetAsync(s->vel_id, 0, bytes, ctx->stream));
+        if (pol)
+            HIPCHK(ctx, hipMemcpy2DAsync(s->vel_id, sizeof(double2), pol, sizeof(double), sizeof(double),
+                                         (size_t)s->P.n, hipMemcpyHostToDevice, ctx->stream));
+    } else if (vel && s->vel_id) {
+        HIPCHK(ctx, hipMemcpyAsync(s->vel_id, vel, bytes, hipMemcpyHostToDevice, ctx->stream));
+    }
     if (pol && s->P.potential == MAVI_POT_RINGS)
         HIPCHK(ctx, hipMemcpyAsync(s->pol, pol, sizeof(double) * s->P.nrings, hipMemcpyHostToDevice, ctx->stream));
     s->cur = 0;
@@ -334,7 +376,13 @@ int mavi_sys_download(mavi_sys *s, double *pos, double *vel, double *pol) {
     HIPCHK(ctx, hipSetDevice(ctx->device));
     const size_t bytes = sizeof(double2) * (size_t)s->P.n;
     if (pos) HIPCHK(ctx, hipMemcpyAsync(pos, s->pos_id, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    if (vel && s->vel_id) HIPCHK(ctx, hipMemcpyAsync(vel, s->vel_id, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    if (is_active(s)) {
+        if (pol)
+            HIPCHK(ctx, hipMemcpy2DAsync(pol, sizeof(double), s->vel_id, sizeof(double2), sizeof(double),
+                                         (size_t)s->P.n, hipMemcpyDeviceToHost, ctx->stream));
+    } else if (vel && s->vel_id) {
+        HIPCHK(ctx, hipMemcpyAsync(vel, s->vel_id, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    }
     if (pol && s->P.potential == MAVI_POT_RINGS)
         HIPCHK(ctx, hipMemcpyAsync(pol, s->pol, sizeof(double) * s->P.nrings, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -405,6 +453,7 @@ int mavi_get_forces(mavi_sys *s, double *forces) {
 // previous step's second one where it is provably identical.
 static hipError_t launch_step(mavi_sys *s, hipStream_t st, bool reuse) {
     if (s->P.potential == MAVI_POT_RINGS) return mavi::launch_rings_step(s, nullptr, st);
+    if (is_active(s)) return mavi::launch_active_step(s, nullptr, st);
     return mavi::launch_verlet_step(s, st, reuse);
 }
 
@@ -435,7 +484,10 @@ static int steps_impl(mavi_sys *s, int64_t nsteps, const double *noise_host) {
     HIPCHK(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     if (noise_host) {
-        const int64_t nr = s->P.nrings;
+        // draws per step: one per ring (rings), one (Szabo) or two (RTP) per particle
+        const int64_t nr = s->P.potential == MAVI_POT_RINGS ? s->P.nrings
+                           : s->P.potential == MAVI_POT_RTP ? 2 * (int64_t)s->P.n
+                                                             : s->P.n;
         const int64_t need = nsteps * nr;
         if (need > s->noise_cap) {
             dfree(s->noise);
@@ -444,10 +496,14 @@ static int steps_impl(mavi_sys *s, int64_t nsteps, const double *noise_host) {
             s->noise_cap = need;
         }
         HIPCHK(ctx, hipMemcpyAsync(s->noise, noise_host, sizeof(double) * need, hipMemcpyHostToDevice, st));
-        for (int64_t k = 0; k < nsteps; k++) HIPCHK(ctx, mavi::launch_rings_step(s, s->noise + k * nr, st));
+        for (int64_t k = 0; k < nsteps; k++) {
+            if (s->P.potential == MAVI_POT_RINGS) HIPCHK(ctx, mavi::launch_rings_step(s, s->noise + k * nr, st));
+            else HIPCHK(ctx, mavi::launch_active_step(s, s->noise + k * nr, st));
+        }
     } else {
         int64_t k = 0;
-        const bool rings = s->P.potential == MAVI_POT_RINGS;
+        // only Verlet steps carry forces from one step to the next
+        const bool rings = s->P.potential == MAVI_POT_RINGS || is_active(s);
         const bool graphs = s->use_graphs && !s->prof && nsteps >= 4;
         while (graphs && k < nsteps && (s->cur != 0 || (!rings && !s->f_valid))) {
             HIPCHK(ctx, launch_step(s, st, s->f_valid));
@@ -482,6 +538,12 @@ int mavi_rings_step(mavi_sys *s, int64_t nsteps, const double *noise) {
     return steps_impl(s, nsteps, noise);
 }
 
+int mavi_sp_step(mavi_sys *s, int64_t nsteps, const double *noise) {
+    if (!s) return MAVI_E_BADARG;
+    if (!is_active(s)) return set_err(s->ctx, MAVI_E_BADARG, "not a self-propelled (Szabo / RTP) system");
+    return steps_impl(s, nsteps, noise);
+}
+
 int mavi_rings_info(mavi_sys *s, double *areas, double *cms, double *continuous_pos) {
     if (!s) return MAVI_E_BADARG;
     mavi_ctx *ctx = s->ctx;
@@ -498,6 +560,7 @@ int mavi_rings_info(mavi_sys *s, double *areas, double *cms, double *continuous_
 int mavi_observables(mavi_sys *s, double *ke, double *pe) {
     if (!s) return MAVI_E_BADARG;
     mavi_ctx *ctx = s->ctx;
+    if (is_active(s)) return set_err(ctx, MAVI_E_BADARG, "a self-propelled state has no velocities");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     const int nb = (s->P.n + 255) / 256;
     if (s->red_cap < 2 * nb) {
@@ -521,7 +584,7 @@ int mavi_observables(mavi_sys *s, double *ke, double *pe) {
 }
 
 int mavi_run_record(mavi_sys *s, int64_t nsteps, double *ke, double *pe) {
-    if (!s || !ke || s->P.potential == MAVI_POT_RINGS) return MAVI_E_BADARG;
+    if (!s || !ke || s->P.potential == MAVI_POT_RINGS || is_active(s)) return MAVI_E_BADARG;
     for (int64_t k = 0; k < nsteps; k++) {
         int st = mavi_step(s, 1);
         if (st != MAVI_OK) return st;

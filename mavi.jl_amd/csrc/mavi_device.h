@@ -99,8 +99,8 @@ __device__ __forceinline__ int32_t block_excl_scan(int32_t v, int32_t *total) {
     return woff + incl - v;
 }
 
-enum { MODE_FORCES = 0, MODE_VERLET1 = 1, MODE_VERLET2 = 2 };
-enum { POT_LJ = 0, POT_HT = 1, POT_RINGS = 2 };
+enum { MODE_FORCES = 0, MODE_VERLET1 = 1, MODE_VERLET2 = 2, MODE_SZABO = 3, MODE_RTP = 4 };
+enum { POT_LJ = 0, POT_HT = 1, POT_RINGS = 2, POT_SZABO = 3 };
 
 struct RingCtx {
     const int32_t *ids;  // sorted slot -> id
@@ -147,6 +147,16 @@ struct PairAcc {
             const double fr = lj_scale(P, r2, self_ok);
             fx = __builtin_fma(fr, dx, fx);
             fy = __builtin_fma(fr, dy, fy);
+        } else if constexpr (POT == POT_SZABO) {
+            // calc_interaction(i, j, ::SzaboCfg), src/integration.jl:68-86:
+            // ((-f_mod) * (dist - r_eq)) * dr, zero beyond r_max
+            if (self_ok && r2 <= P.cut2) {
+                const double dist = sqrt(r2);
+                const double f_mod = (dist > P.p2) ? P.sz_adh : P.sz_rep;
+                const double q = -f_mod * (dist - P.p2);
+                fx += q * dx;
+                fy += q * dy;
+            }
         } else if constexpr (POT == POT_HT) {
             // potential_force(::HarmTruncCfg), src/configs.jl:354-368, term by term
             if (self_ok && r2 <= P.cut2) {
@@ -201,6 +211,22 @@ __device__ __forceinline__ void philox_round(uint32_t &c0, uint32_t &c1, uint32_
     c1 = n1;
     c2 = n2;
     c3 = n3;
+}
+
+// Two uniforms in [0, 1) with 53 random bits each from Philox4x32-10 keyed
+// by (seed, step, id); a counter word different from philox_normal's keeps
+// the streams apart.
+__device__ void philox_uniform2(uint64_t seed, uint64_t step, uint32_t id, double *u, double *w) {
+    uint32_t c0 = (uint32_t)step, c1 = (uint32_t)(step >> 32), c2 = id, c3 = 0x72746d62u;
+    uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        philox_round(c0, c1, c2, c3, k0, k1);
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    *u = (double)((((uint64_t)c0 << 21) ^ ((uint64_t)c1 >> 11)) & ((1ull << 53) - 1)) * 0x1p-53;
+    *w = (double)((((uint64_t)c2 << 21) ^ ((uint64_t)c3 >> 11)) & ((1ull << 53) - 1)) * 0x1p-53;
 }
 
 __device__ double philox_normal(uint64_t seed, uint64_t step, uint32_t ring) {
@@ -437,6 +463,14 @@ __device__ __forceinline__ void pair_gather_chunks(const SysParams &P, const Rin
 
 }
 
+// walls! for a periodic rectangle, src/integration.jl:309-324.
+__device__ __forceinline__ void periodic_wrap(const SysParams &P, double &x, double &y) {
+    const double dxc = x - (P.blx + P.halfL), dyc = y - (P.bly + P.halfH);
+    const bool ox = fabs(dxc) > P.halfL, oy = fabs(dyc) > P.halfH;
+    if (ox) x = x - copysign(P.halfL * 2.0, dxc);
+    if (oy) y = y - copysign(P.halfH * 2.0, dyc);
+}
+
 // Integrator epilogue of a pair-force launch for sorted slot s.
 template <int MODE>
 __device__ __forceinline__ void pair_epilogue(const SysParams &P, const int s, const double2 pi, const double fx,
@@ -458,11 +492,7 @@ __device__ __forceinline__ void pair_epilogue(const SysParams &P, const int s, c
         double vy = v.y + P.hdt * (fy + fo.y);
         double x = pi.x, y = pi.y;
         if (P.periodic) {
-            // src/integration.jl:309-324
-            const double dxc = x - (P.blx + P.halfL), dyc = y - (P.bly + P.halfH);
-            const bool ox = fabs(dxc) > P.halfL, oy = fabs(dyc) > P.halfH;
-            if (ox) x = x - copysign(P.halfL * 2.0, dxc);
-            if (oy) y = y - copysign(P.halfH * 2.0, dyc);
+            periodic_wrap(P, x, y);
         } else {
             // src/integration.jl:271-285
             const double rx = x - P.blx, ry = y - P.bly;

@@ -60,6 +60,12 @@ struct SysParams {
     // rings
     int32_t nmax, nrings, ntypes;
     int32_t typed;
+    // self-propelled particles (SzaboCfg / RunTumbleCfg)
+    double sz_rep, sz_adh;    // k_rep / (r_max - r_eq), k_adh / r_eq
+    double ap_vo, ap_mu;      // vo, mobility
+    double ap_inv_tau;        // 1 / relax_time
+    double ap_noise;          // sqrt(2 rot_diff dt)
+    double ap_tumble;         // tumble_rate * dt
 };
 
 // Per ring type parameters.
@@ -175,6 +181,9 @@ hipError_t launch_forces_only(mavi_sys *s, hipStream_t st);
 // system (F holds its second evaluation), so unchanged blocks reuse it.
 hipError_t launch_verlet_step(mavi_sys *s, hipStream_t st, bool reuse = false);
 hipError_t launch_rings_step(mavi_sys *s, const double *noise_dev, hipStream_t st);
+// One szabo_step! / rtp_step!; noise_dev = this step's host draws, or null
+// for the device Philox stream.
+hipError_t launch_active_step(mavi_sys *s, const double *noise_dev, hipStream_t st);
 hipError_t launch_rings_forces(mavi_sys *s, hipStream_t st);
 hipError_t launch_rings_init(mavi_sys *s, hipStream_t st);
 hipError_t launch_observables(mavi_sys *s, double *d_out, hipStream_t st);

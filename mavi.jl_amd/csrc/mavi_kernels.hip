@@ -180,6 +180,76 @@ __global__ void k_pair(SysParams P, RingCtx R, const double2 *__restrict__ pin, 
 }
 
 // ---------------------------------------------------------------------------
+// Self-propelled particles: szabo_step! / rtp_step! (src/integration.jl:
+// 517-535).  One launch after the binning: pair forces at the binned
+// positions (Szabo, or the truncated LJ of run-and-tumble), then, fused,
+// update_szabo! / update_rtp! (:433-489) and the periodic walls!.  The
+// polarisation angle travels in the velocity slot (x component).  The new
+// state goes to the other state copy in the same slot order, because
+// neighbouring lanes still read pin.  Noise: this step's host draws by
+// particle id, or the Philox stream keyed by (seed, step, id).
+// ---------------------------------------------------------------------------
+constexpr double kTwoPi = 2.0 * 3.141592653589793;  // 2*pi in Julia
+
+template <int MODE, int POT, bool ALLPAIRS>
+__global__ void k_active(SysParams P, const double2 *__restrict__ pin, const double2 *__restrict__ ain,
+                         const int32_t *__restrict__ idin, const int32_t *__restrict__ scell,
+                         const int32_t *__restrict__ start, double2 *__restrict__ pout, double2 *__restrict__ aout,
+                         int32_t *__restrict__ idout, double2 *__restrict__ Fout, const double *__restrict__ noise,
+                         uint64_t seed, const unsigned long long *__restrict__ counter) {
+    const int s = xcd_block() * blockDim.x + threadIdx.x;
+    const bool valid = s < P.na;
+    const int sv = valid ? s : 0;
+    const double2 pi = pin[sv];
+    const RingCtx R{};
+    PairAcc<POT> acc;
+    if constexpr (ALLPAIRS) {
+        if (valid) scan_range<POT>(P, R, acc, pin, s, pi.x, pi.y, P.periodic != 0, 0, P.na);
+    } else {
+        pair_gather_chunks<POT>(P, R, acc, pin, scell[sv], start, s, valid, pi, INFINITY, nullptr);
+    }
+    if (!valid) return;
+    const double fx = acc.fx, fy = acc.fy;
+    Fout[s] = make_double2(fx, fy);
+    const int id = idin[s];
+    double th = ain[s].x;
+    double sn, cs;
+    sincos(th, &sn, &cs);
+    double x = pi.x, y = pi.y;
+    if constexpr (MODE == MODE_SZABO) {
+        // vel = vo * pol + mu * F; speed = sqrt(sum(abs, vel)) (:448-452)
+        const double vx = P.ap_vo * cs + P.ap_mu * fx;
+        const double vy = P.ap_vo * sn + P.ap_mu * fy;
+        const double speed = sqrt(fabs(vx) + fabs(vy));
+        double cr = (speed > 0.0) ? (cs * vy - sn * vx) / speed : 0.0;
+        if (fabs(cr) > 1.0) cr = (cr > 0.0) ? 1.0 : -1.0;
+        const double xi = noise ? noise[id] : philox_normal(seed, counter[0], (uint32_t)id);
+        const double dth = P.ap_inv_tau * asin(cr) * P.dt + P.ap_noise * xi;
+        x = x + vx * P.dt;
+        y = y + vy * P.dt;
+        th = th + dth;
+    } else {
+        // vel = vo * pol + F; tumble to 2 pi rand() when rand() < rate dt (:476-488)
+        const double vx = P.ap_vo * cs + fx;
+        const double vy = P.ap_vo * sn + fy;
+        x = x + vx * P.dt;
+        y = y + vy * P.dt;
+        double u, w;
+        if (noise) {
+            u = noise[2 * (int64_t)id];
+            w = noise[2 * (int64_t)id + 1];
+        } else {
+            philox_uniform2(seed, counter[0], (uint32_t)id, &u, &w);
+        }
+        if (u < P.ap_tumble) th = kTwoPi * w;
+    }
+    periodic_wrap(P, x, y);  // self-propelled systems run with PeriodicWalls only
+    pout[s] = make_double2(x, y);
+    aout[s] = make_double2(th, 0.0);
+    idout[s] = id;
+}
+
+// ---------------------------------------------------------------------------
 // K2/K3, LDS-tiled form.  One wave owns a segment of kSegCells consecutive
 // chunks of one chunk row.  It stages the particles of the three neighbouring
 // chunk rows over the segment's columns +-1 (about 3 * 14 * 4.4 = 185 at
@@ -613,10 +683,35 @@ hipError_t launch_pair(mavi_sys *s, const double2 *pin, hipStream_t st, bool reu
         case MAVI_POT_HARMTRUNC:
             return all ? launch_pair_t<MODE, POT_HT, true>(s, pin, st, false, false)
                        : launch_pair_t<MODE, POT_HT, false>(s, pin, st, reuse, defer);
+        case MAVI_POT_RTP:  // truncated LJ (cut2 = the 2^(1/6) sigma threshold)
+            if constexpr (MODE != MODE_FORCES) return hipErrorInvalidValue;
+            return all ? launch_pair_t<MODE, POT_LJ, true>(s, pin, st, false, false)
+                       : launch_pair_t<MODE, POT_LJ, false>(s, pin, st, false, false);
+        case MAVI_POT_SZABO:
+            if constexpr (MODE != MODE_FORCES) return hipErrorInvalidValue;
+            else
+                return all ? launch_pair_t<MODE, POT_SZABO, true>(s, pin, st, false, false)
+                           : launch_pair_t<MODE, POT_SZABO, false>(s, pin, st, false, false);
         default:
             return all ? launch_pair_t<MODE, POT_RINGS, true>(s, pin, st, false, false)
                        : launch_pair_t<MODE, POT_RINGS, false>(s, pin, st, false, false);
     }
+}
+
+template <int MODE, int POT>
+hipError_t launch_active_t(mavi_sys *s, const double *noise_dev, hipStream_t st) {
+    const int wg = s->wg;
+    const int c = s->cur, o = 1 - c;
+    if (s->P.cols == 0)
+        hipLaunchKernelGGL((k_active<MODE, POT, true>), dim3(nblk(s->P.na, wg)), dim3(wg), 0, st, s->P, s->P_[c],
+                           s->V_[c], s->ID_[c], s->scell, s->start, s->P_[o], s->V_[o], s->ID_[o], s->F, noise_dev,
+                           s->seed, s->counter);
+    else
+        hipLaunchKernelGGL((k_active<MODE, POT, false>), dim3(nblk(s->P.na, wg)), dim3(wg), 0, st, s->P, s->P_[c],
+                           s->V_[c], s->ID_[c], s->scell, s->start, s->P_[o], s->V_[o], s->ID_[o], s->F, noise_dev,
+                           s->seed, s->counter);
+    s->cur = o;
+    return hipGetLastError();
 }
 
 struct ProfScope {
@@ -770,6 +865,15 @@ hipError_t launch_rings_step(mavi_sys *s, const double *noise_dev, hipStream_t s
         if ((e = launch_pair<MODE_FORCES>(s, s->P_[s->cur], st)) != hipSuccess) return e;
     }
     return launch_rings_kernel(s, true, noise_dev, st);
+}
+
+hipError_t launch_active_step(mavi_sys *s, const double *noise_dev, hipStream_t st) {
+    hipError_t e = launch_bin(s, st, true, false, false);  // angles move with the velocity slot
+    if (e != hipSuccess) return e;
+    if (s->P.cols == 0) hipLaunchKernelGGL(k_counter_inc, dim3(1), dim3(64), 0, st, s->counter);
+    ProfScope ps(s, MAVI_PROF_FORCE, st);
+    if (s->P.potential == MAVI_POT_SZABO) return launch_active_t<MODE_SZABO, POT_SZABO>(s, noise_dev, st);
+    return launch_active_t<MODE_RTP, POT_LJ>(s, noise_dev, st);
 }
 
 hipError_t launch_rings_init(mavi_sys *s, hipStream_t st) {

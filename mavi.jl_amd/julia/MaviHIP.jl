@@ -43,6 +43,8 @@ const MAVI_WALL_PERIODIC = Cint(1)
 const MAVI_POT_LJ = Cint(0)
 const MAVI_POT_HARMTRUNC = Cint(1)
 const MAVI_POT_RINGS = Cint(2)
+const MAVI_POT_SZABO = Cint(3)
+const MAVI_POT_RTP = Cint(4)
 
 struct SysDesc                      # mavi_sys_desc
     n::Int64
@@ -66,6 +68,7 @@ struct SysDesc                      # mavi_sys_desc
     type_par::Ptr{Float64}
     interaction::Ptr{Float64}
     seed::UInt64
+    active::NTuple{4,Float64}    # Szabo: vo, mobility, relax_time, rot_diff; RTP: vo, tumble_rate
 end
 
 mutable struct DeviceSys
@@ -99,6 +102,8 @@ end
 gpu_kind(::Any) = nothing
 gpu_kind(::LenJonesCfg) = MAVI_POT_LJ
 gpu_kind(::HarmTruncCfg) = MAVI_POT_HARMTRUNC
+gpu_kind(::SzaboCfg) = MAVI_POT_SZABO
+gpu_kind(::RunTumbleCfg) = MAVI_POT_RTP
 gpu_kind(::Mavi.Rings.Configs.RingsCfg{U,T,F}) where {U,T<:Mavi.Rings.Configs.HarmTruncCfg,F} = MAVI_POT_RINGS
 
 wall_kind(::SpaceCfg{RigidWalls,<:RectangleCfg}) = MAVI_WALL_RIGID
@@ -119,11 +124,15 @@ function device_system(system)
     rows = isnothing(chunks) ? 0 : chunks.num_rows
     nc = isnothing(chunks) ? 0 : size(chunks.chunk_particles, 1)
     pot = dyn isa LenJonesCfg ? (dyn.sigma, dyn.epsilon, 0.0, 0.0) :
-          dyn isa HarmTruncCfg ? (dyn.k_rep, dyn.k_atr, dyn.dist_eq, dyn.dist_max) : (0.0, 0.0, 0.0, 0.0)
+          dyn isa HarmTruncCfg ? (dyn.k_rep, dyn.k_atr, dyn.dist_eq, dyn.dist_max) :
+          dyn isa SzaboCfg ? (dyn.k_rep, dyn.k_adh, dyn.r_eq, dyn.r_max) :
+          dyn isa RunTumbleCfg ? (dyn.sigma, dyn.epsilon, 0.0, 0.0) : (0.0, 0.0, 0.0, 0.0)
+    active = dyn isa SzaboCfg ? (dyn.vo, dyn.mobility, dyn.relax_time, dyn.rot_diff) :
+             dyn isa RunTumbleCfg ? (dyn.vo, dyn.tumble_rate, 0.0, 0.0) : (0.0, 0.0, 0.0, 0.0)
     desc = SysDesc(length(system.state.pos), Tuple(g.bottom_left), (g.length, g.height),
                    wall_kind(system.space_cfg), gpu_kind(dyn), pot, system.int_cfg.dt,
                    cols, rows, nc, 0, minimum(particle_radius(dyn)),
-                   0, 0, 0, 0, C_NULL, C_NULL, C_NULL, C_NULL, UInt64(0))
+                   0, 0, 0, 0, C_NULL, C_NULL, C_NULL, C_NULL, UInt64(0), active)
     # (RingsSystem fills nmax/nrings/ntypes and the type tables the same way,
     #  from RingsCfg fields p0, relax_time, vo, mobility, rot_diff, k_area,
     #  k_spring, l_spring, num_particles and interaction_finder.)
@@ -138,12 +147,15 @@ function device_system(system)
     return d
 end
 
+# SelfPropelledState.pol_angle travels through the `pol` argument
 function upload!(system, d::DeviceSys)
     pos = system.state.pos
     vel = hasproperty(system.state, :vel) ? system.state.vel : nothing
-    GC.@preserve pos vel begin
+    pol = hasproperty(system.state, :pol_angle) ? system.state.pol_angle : nothing
+    GC.@preserve pos vel pol begin
         check(d.ctx, ccall((:mavi_sys_upload, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
-                           d.sys, pointer(pos), isnothing(vel) ? C_NULL : pointer(vel), C_NULL))
+                           d.sys, pointer(pos), isnothing(vel) ? C_NULL : pointer(vel),
+                           isnothing(pol) ? C_NULL : pointer(pol)))
     end
     d.host_dirty = false
 end
@@ -151,10 +163,12 @@ end
 function download!(system, d::DeviceSys)
     pos = system.state.pos
     vel = hasproperty(system.state, :vel) ? system.state.vel : nothing
+    pol = hasproperty(system.state, :pol_angle) ? system.state.pol_angle : nothing
     forces = get_forces(system)
-    GC.@preserve pos vel forces begin
+    GC.@preserve pos vel pol forces begin
         check(d.ctx, ccall((:mavi_sys_download, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
-                           d.sys, pointer(pos), isnothing(vel) ? C_NULL : pointer(vel), C_NULL))
+                           d.sys, pointer(pos), isnothing(vel) ? C_NULL : pointer(vel),
+                           isnothing(pol) ? C_NULL : pointer(pol)))
         check(d.ctx, ccall((:mavi_get_forces, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), d.sys, pointer(forces)))
     end
 end
@@ -171,7 +185,9 @@ function Integration.calc_forces!(system::System, chunks, device::GPU)
     return nothing
 end
 
-"newton_step! on the GPU: upload, one step, download (per-step sync: user code may read the state)."
+"""newton_step! (or szabo_step! / rtp_step!, by the system's DynamicCfg) on
+the GPU: upload, one step, download (per-step sync: user code may read the
+state).  Self-propelled steps draw their noise from the device stream."""
 function gpu_newton_step!(system)
     d = device_system(system)
     upload!(system, d)
@@ -182,9 +198,7 @@ function gpu_newton_step!(system)
 end
 
 function Integration.get_step_function(t::StandardSys, system::System)
-    if on_gpu(system) && !(system.dynamic_cfg isa SzaboCfg || system.dynamic_cfg isa RunTumbleCfg)
-        return gpu_newton_step!
-    end
+    on_gpu(system) && return gpu_newton_step!
     return Integration.get_step_function(t, system, Val(:cpu))
 end
 # the reference's selection, src/integration.jl:538-548

@@ -21,8 +21,10 @@ from .configs import (
     PeriodicWalls,
     RectangleCfg,
     RigidWalls,
+    RunTumbleCfg,
     Sequencial,
     SpaceCfg,
+    SzaboCfg,
     Threaded,
     chunk_capacity,
     particle_radius,
@@ -33,15 +35,18 @@ from .integration import (
     kinetic_energy,
     newton_step,
     potential_energy,
+    rtp_step,
     run_system,
+    szabo_step,
     update_chunks,
 )
-from .systems import SecondLawState, System, TimeInfo
+from .systems import SecondLawState, SelfPropelledState, System, TimeInfo
 
 __all__ = [
     "GPU", "ChunksCfg", "HarmTruncCfg", "IntCfg", "LenJonesCfg", "PeriodicWalls", "RectangleCfg",
     "RigidWalls", "Sequencial", "SpaceCfg", "Threaded", "chunk_capacity", "particle_radius",
     "calc_forces", "get_step_function", "kinetic_energy", "newton_step", "potential_energy",
     "run_system", "update_chunks", "SecondLawState", "System", "TimeInfo", "Context",
+    "SzaboCfg", "RunTumbleCfg", "SelfPropelledState", "szabo_step", "rtp_step",
     "MaviError", "CellOverflowError", "OutOfBoxError", "UnsupportedOnGPU",
 ]

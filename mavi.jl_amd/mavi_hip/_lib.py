@@ -26,6 +26,8 @@ MAVI_WALL_PERIODIC = 1
 MAVI_POT_LJ = 0
 MAVI_POT_HARMTRUNC = 1
 MAVI_POT_RINGS = 2
+MAVI_POT_SZABO = 3
+MAVI_POT_RTP = 4
 
 MAVI_PROF_BIN = 0
 MAVI_PROF_FORCE = 1
@@ -86,6 +88,7 @@ class SysDesc(C.Structure):
         ("type_par", C.POINTER(C.c_double)),
         ("interaction", C.POINTER(C.c_double)),
         ("seed", C.c_uint64),
+        ("active", C.c_double * 4),
     ]
 
 
@@ -108,6 +111,7 @@ SIGNATURES = {
     "mavi_get_forces": (C.c_int, [_P, _DP]),
     "mavi_step": (C.c_int, [_P, C.c_int64]),
     "mavi_rings_step": (C.c_int, [_P, C.c_int64, _DP]),
+    "mavi_sp_step": (C.c_int, [_P, C.c_int64, _DP]),
     "mavi_rings_info": (C.c_int, [_P, _DP, _DP, _DP]),
     "mavi_observables": (C.c_int, [_P, _DP, _DP]),
     "mavi_run_record": (C.c_int, [_P, C.c_int64, _DP, _DP]),
@@ -262,6 +266,13 @@ class DeviceSystem:
 
     def step(self, nsteps: int):
         self._call("mavi_step", C.c_int64(int(nsteps)))
+
+    def sp_step(self, nsteps: int, noise=None):
+        """Self-propelled steps; noise: (nsteps, n) normals (Szabo) or
+        (nsteps, n, 2) uniforms (RTP), or None for the device stream."""
+        if noise is not None:
+            noise = np.ascontiguousarray(noise, dtype=np.float64)
+        self._call("mavi_sp_step", C.c_int64(int(nsteps)), dptr(noise))
 
     def rings_step(self, nsteps: int, noise=None):
         if noise is not None:

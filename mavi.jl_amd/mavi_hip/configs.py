@@ -2,7 +2,8 @@
 
 Names, fields and argument meanings follow the reference; only the types the
 GPU path runs are provided: rectangles with RigidWalls or PeriodicWalls,
-LenJonesCfg and HarmTruncCfg, IntCfg/ChunksCfg and the device modes.
+LenJonesCfg, HarmTruncCfg, SzaboCfg and RunTumbleCfg, IntCfg/ChunksCfg and
+the device modes.
 """
 from __future__ import annotations
 
@@ -79,12 +80,47 @@ class LenJonesCfg(PotentialCfg):  # src/configs.jl:380-397
     cutoff: float | None = None
 
 
+@dataclass
+class SzaboCfg(DynamicCfg):  # src/configs.jl:399-408
+    """Self-propelled particles of Szabo et al.: linear repulsion/adhesion
+    between r_eq and r_max (calc_interaction, src/integration.jl:68-86) and
+    polarisations relaxing towards the velocity (update_szabo!, :433-465)."""
+
+    vo: float
+    mobility: float
+    relax_time: float
+    k_rep: float
+    k_adh: float
+    r_eq: float
+    r_max: float
+    rot_diff: float
+
+
+@dataclass
+class RunTumbleCfg(DynamicCfg):  # src/configs.jl:410-415
+    """Run-and-tumble particles: LJ repulsion truncated at 2^(1/6) sigma
+    (calc_interaction, src/integration.jl:88-109) and tumbles at rate
+    tumble_rate (update_rtp!, :467-489)."""
+
+    vo: float
+    sigma: float
+    epsilon: float
+    tumble_rate: float
+
+    def cutoff(self) -> float:
+        return 2 ** (1 / 6) * self.sigma  # src/integration.jl:96
+
+
 def particle_radius(dynamic_cfg) -> float:
     """src/configs.jl:418-421."""
     if isinstance(dynamic_cfg, LenJonesCfg):
         return dynamic_cfg.sigma * 2 ** (1 / 6) / 2
     if isinstance(dynamic_cfg, HarmTruncCfg):
         return dynamic_cfg.dist_eq / 2.0
+    if isinstance(dynamic_cfg, SzaboCfg):
+        return dynamic_cfg.r_eq / 2
+    if isinstance(dynamic_cfg, RunTumbleCfg):
+        return dynamic_cfg.sigma * 2 ** (1 / 6) / 2
     if hasattr(dynamic_cfg, "particle_radius"):
         return dynamic_cfg.particle_radius()
     raise TypeError(f"no particle_radius for {type(dynamic_cfg).__name__}")

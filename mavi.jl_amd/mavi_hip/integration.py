@@ -1,6 +1,7 @@
 """Mirror of Mavi.Integration / Mavi.RunSystem / Mavi.Quantities on the GPU.
 
   newton_step(system)       newton_step!, src/integration.jl:507-515
+  szabo_step / rtp_step     szabo_step! / rtp_step!, src/integration.jl:517-535
   get_step_function(system) src/integration.jl:537-548, src/rings/integration.jl:545
   calc_forces(system)       calc_forces!, src/integration.jl:226
   update_chunks(system)     update_chunks!, src/chunks.jl:150-161
@@ -16,10 +17,26 @@ downloads at the end.
 from __future__ import annotations
 
 
-def newton_step(system):
+def _synced_step(system):
+    # the library runs the system's own dynamics for its kind
     system.upload()
     system.advance(1)
     system.download()
+
+
+def newton_step(system):
+    """newton_step!, src/integration.jl:507-515."""
+    _synced_step(system)
+
+
+def szabo_step(system):
+    """szabo_step!, src/integration.jl:517-525 (device noise stream)."""
+    _synced_step(system)
+
+
+def rtp_step(system):
+    """rtp_step!, src/integration.jl:527-535 (device noise stream)."""
+    _synced_step(system)
 
 
 def calc_forces(system):
@@ -35,6 +52,12 @@ def get_step_function(system):
 
     if isinstance(system, RingsSystem):
         return step
+    from .configs import RunTumbleCfg, SzaboCfg
+
+    if isinstance(system.dynamic_cfg, SzaboCfg):
+        return szabo_step
+    if isinstance(system.dynamic_cfg, RunTumbleCfg):
+        return rtp_step
     return newton_step
 
 

@@ -19,7 +19,9 @@ from .configs import (
     PeriodicWalls,
     RectangleCfg,
     RigidWalls,
+    RunTumbleCfg,
     SpaceCfg,
+    SzaboCfg,
     as_points,
     chunk_capacity,
     particle_radius,
@@ -36,6 +38,18 @@ class SecondLawState:  # src/states.jl:233-259
         self.vel = as_points(self.vel).copy()
         if self.pos.shape != self.vel.shape:
             raise ValueError("pos and vel must have the same number of particles")
+
+
+@dataclass
+class SelfPropelledState:  # src/states.jl:104-125
+    pos: np.ndarray
+    pol_angle: np.ndarray
+
+    def __post_init__(self):
+        self.pos = as_points(self.pos).copy()
+        self.pol_angle = np.array(self.pol_angle, dtype=np.float64).reshape(-1).copy()
+        if len(self.pol_angle) != len(self.pos):
+            raise ValueError("pos and pol_angle must have the same number of particles")
 
 
 @dataclass
@@ -77,6 +91,10 @@ def gpu_kind(dynamic_cfg):
         return _lib.MAVI_POT_LJ
     if isinstance(dynamic_cfg, HarmTruncCfg):
         return _lib.MAVI_POT_HARMTRUNC
+    if isinstance(dynamic_cfg, SzaboCfg):
+        return _lib.MAVI_POT_SZABO
+    if isinstance(dynamic_cfg, RunTumbleCfg):
+        return _lib.MAVI_POT_RTP
     return None
 
 
@@ -104,7 +122,7 @@ def base_desc(n, space_cfg, dynamic_cfg, int_cfg, chunks: Chunks | None, radius)
     return d
 
 
-def describe(state: SecondLawState, space_cfg: SpaceCfg, dynamic_cfg, int_cfg):
+def describe(state, space_cfg: SpaceCfg, dynamic_cfg, int_cfg):
     """Descriptor of a plain system for the C ABI, with the checks of
     System(...) (src/systems.jl:73-114): particles inside the geometry, a
     GPU-capable potential and walls.  Returns (SysDesc, Chunks | None)."""
@@ -113,6 +131,10 @@ def describe(state: SecondLawState, space_cfg: SpaceCfg, dynamic_cfg, int_cfg):
         raise ValueError(f"Particles with ids={list(out_ids + 1)} outside space.")
     kind = gpu_kind(dynamic_cfg)
     wall = wall_kind(space_cfg)
+    active = kind in (_lib.MAVI_POT_SZABO, _lib.MAVI_POT_RTP)
+    if active != isinstance(state, SelfPropelledState):
+        raise TypeError(f"{type(dynamic_cfg).__name__} runs with a "
+                        f"{'SelfPropelledState' if active else 'SecondLawState'}")
     if kind is None or wall is None or not isinstance(int_cfg.device, GPU):
         raise _lib.UnsupportedOnGPU(
             _lib.MAVI_E_UNSUPPORTED,
@@ -126,21 +148,29 @@ def describe(state: SecondLawState, space_cfg: SpaceCfg, dynamic_cfg, int_cfg):
         chunks = Chunks(int_cfg.chunks_cfg.num_cols, int_cfg.chunks_cfg.num_rows, space_cfg.geometry_cfg, r)
     d = base_desc(len(state.pos), space_cfg, dynamic_cfg, int_cfg, chunks, r)
     d.potential = kind
+    c = dynamic_cfg
     if kind == _lib.MAVI_POT_LJ:
-        d.pot[0], d.pot[1] = dynamic_cfg.sigma, dynamic_cfg.epsilon
-        d.pot[2] = dynamic_cfg.cutoff if dynamic_cfg.cutoff else 0.0
+        d.pot[0], d.pot[1] = c.sigma, c.epsilon
+        d.pot[2] = c.cutoff if c.cutoff else 0.0
+    elif kind == _lib.MAVI_POT_HARMTRUNC:
+        d.pot[0], d.pot[1] = c.k_rep, c.k_atr
+        d.pot[2], d.pot[3] = c.dist_eq, c.dist_max
+    elif kind == _lib.MAVI_POT_SZABO:
+        d.pot[0], d.pot[1], d.pot[2], d.pot[3] = c.k_rep, c.k_adh, c.r_eq, c.r_max
+        d.active[0], d.active[1], d.active[2], d.active[3] = c.vo, c.mobility, c.relax_time, c.rot_diff
     else:
-        d.pot[0], d.pot[1] = dynamic_cfg.k_rep, dynamic_cfg.k_atr
-        d.pot[2], d.pot[3] = dynamic_cfg.dist_eq, dynamic_cfg.dist_max
+        d.pot[0], d.pot[1] = c.sigma, c.epsilon
+        d.active[0], d.active[1] = c.vo, c.tumble_rate
     return d, chunks
 
 
 class System:
     """GPU-backed System (src/systems.jl:45-114)."""
 
-    def __init__(self, *, state: SecondLawState, space_cfg: SpaceCfg, dynamic_cfg, int_cfg,
-                 time_info: TimeInfo | None = None, ctx: _lib.Context | None = None):
+    def __init__(self, *, state, space_cfg: SpaceCfg, dynamic_cfg, int_cfg,
+                 time_info: TimeInfo | None = None, ctx: _lib.Context | None = None, seed: int = 0):
         d, self.chunks = describe(state, space_cfg, dynamic_cfg, int_cfg)
+        d.seed = int(seed)  # device noise stream of self-propelled systems
         self.state = state
         self.space_cfg = space_cfg
         self.dynamic_cfg = dynamic_cfg
@@ -155,11 +185,21 @@ class System:
             self.update_chunks()
 
     # --- host <-> device ---------------------------------------------------
+    @property
+    def self_propelled(self) -> bool:
+        return isinstance(self.state, SelfPropelledState)
+
     def upload(self):
-        self.dev.upload(self.state.pos, self.state.vel)
+        if self.self_propelled:
+            self.dev.upload(self.state.pos, None, self.state.pol_angle)
+        else:
+            self.dev.upload(self.state.pos, self.state.vel)
 
     def download(self):
-        self.dev.download(self.state.pos, self.state.vel)
+        if self.self_propelled:
+            self.dev.download(self.state.pos, None, self.state.pol_angle)
+        else:
+            self.dev.download(self.state.pos, self.state.vel)
         self.forces[:] = self.dev.get_forces()
 
     # --- reference operations ----------------------------------------------
@@ -181,9 +221,18 @@ class System:
     def get_num_total_particles(self):
         return len(self.state.pos)
 
-    def advance(self, nsteps: int):
-        """nsteps newton_step! with the state resident on the device."""
-        self.dev.step(nsteps)
+    def advance(self, nsteps: int, noise=None):
+        """nsteps steps of the system's dynamics (newton_step!, or
+        szabo_step! / rtp_step! for a SelfPropelledState) with the state
+        resident on the device.  noise (self-propelled only) replaces the
+        device stream: (nsteps, n) normals for Szabo, (nsteps, n, 2)
+        uniforms for run-and-tumble."""
+        if noise is not None:
+            if not self.self_propelled:
+                raise ValueError("host noise is for self-propelled systems")
+            self.dev.sp_step(nsteps, noise)
+        else:
+            self.dev.step(nsteps)
         self.time_info.num_steps += int(nsteps)
         for _ in range(int(nsteps)):
             self.time_info.time += self.int_cfg.dt

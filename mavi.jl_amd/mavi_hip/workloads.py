@@ -24,6 +24,16 @@ and the CPU oracle.
                      gases (nside = round(sqrt(N)), seed 1000 + i), odd ones
                      rings_mixture systems (round(N / 20) rings of 20,
                      seed 2000 + i, device noise seed 3000 + i)
+  szabo_lattice      nside x nside Szabo particles (examples/szabo.jl
+                     parameters: vo 1, mobility 1, relax_time 1, k_rep 10,
+                     k_adh 0.75, r_eq 1, r_max 1.1, rot_diff 0.01), square
+                     lattice of spacing r_eq in a periodic box, polarisations
+                     uniform (seed), chunks of side >= r_max, dt = 0.01
+  rtp_gas            nside x nside run-and-tumble particles (examples/rtp.jl
+                     parameters: vo 1, sigma 1, epsilon 1, tumble_rate 1) on a
+                     lattice at number density 0.8 (spacing 1.118, inside the
+                     2^(1/6) sigma range) jittered by +-0.02 (seed), periodic,
+                     chunks of side >= 2^(1/6) sigma, dt = 1e-3
 """
 from __future__ import annotations
 
@@ -31,7 +41,8 @@ import math
 
 import numpy as np
 
-from .configs import ChunksCfg, GPU, IntCfg, LenJonesCfg, PeriodicWalls, RectangleCfg, RigidWalls, SpaceCfg, particle_radius
+from .configs import (ChunksCfg, GPU, IntCfg, LenJonesCfg, PeriodicWalls, RectangleCfg, RigidWalls, RunTumbleCfg,
+                      SpaceCfg, SzaboCfg, particle_radius)
 from .init_states import rectangular_grid
 
 
@@ -117,3 +128,38 @@ def batch_ensemble(nrep=512, nmin=1000, nmax=64000, seed=3):
             nr = max(2, int(round(n / 20)))
             out.append(("rings", rings_mixture(nrings=nr, seed=2000 + i), 3000 + i))
     return out
+
+
+def szabo_lattice(nside=1024, seed=4, rot_diff=0.01, dt=0.01):
+    """Szabo particles on a square lattice (see the module docstring).
+    Returns (pos, pol_angle, space, dyn, int_cfg)."""
+    dyn = SzaboCfg(vo=1.0, mobility=1.0, relax_time=1.0, k_rep=10.0, k_adh=0.75, r_eq=1.0, r_max=1.1,
+                   rot_diff=rot_diff)
+    a = dyn.r_eq
+    L = nside * a
+    ix, iy = np.meshgrid(np.arange(nside), np.arange(nside), indexing="xy")
+    pos = np.stack([(ix.ravel() + 0.5) * a, (iy.ravel() + 0.5) * a], axis=1)
+    rng = np.random.default_rng(seed)
+    ang = rng.random(len(pos)) * 2 * math.pi
+    cols = int(L // dyn.r_max)
+    space = SpaceCfg(wall_type=PeriodicWalls(), geometry_cfg=RectangleCfg(length=L, height=L))
+    int_cfg = IntCfg(dt=dt, chunks_cfg=ChunksCfg(num_cols=cols, num_rows=cols), device=GPU())
+    return pos, ang, space, dyn, int_cfg
+
+
+def rtp_gas(nside=1024, density=0.8, seed=5, tumble_rate=1.0, dt=1e-3, jitter=0.02):
+    """Run-and-tumble particles on a jittered lattice (see the module
+    docstring).  Returns (pos, pol_angle, space, dyn, int_cfg)."""
+    dyn = RunTumbleCfg(vo=1.0, sigma=1.0, epsilon=1.0, tumble_rate=tumble_rate)
+    n = nside * nside
+    L = math.sqrt(n / density)
+    a = L / nside
+    rng = np.random.default_rng(seed)
+    ix, iy = np.meshgrid(np.arange(nside), np.arange(nside), indexing="xy")
+    pos = np.stack([(ix.ravel() + 0.5) * a, (iy.ravel() + 0.5) * a], axis=1)
+    pos += rng.uniform(-jitter, jitter, size=pos.shape)
+    ang = rng.random(n) * 2 * math.pi
+    cols = int(L // dyn.cutoff())
+    space = SpaceCfg(wall_type=PeriodicWalls(), geometry_cfg=RectangleCfg(length=L, height=L))
+    int_cfg = IntCfg(dt=dt, chunks_cfg=ChunksCfg(num_cols=cols, num_rows=cols), device=GPU())
+    return pos, ang, space, dyn, int_cfg

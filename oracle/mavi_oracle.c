@@ -32,6 +32,13 @@
  *                              energy conservation O(dt^2), time reversal,
  *                              wall reflection / wrap closed forms.
  *   or_kinetic/or_potential    brute force on tiny inputs in Python floats.
+ *   Szabo force                minus the gradient of U(d) = f_mod (d^3/3 - r_eq
+ *                              d^2/2) by finite differences on each branch, zero
+ *                              at r_eq and beyond r_max, Newton's third law.
+ *   or_sp_steps                free self-propulsion x + k vo pol dt, the
+ *                              polarisation turning towards the force,
+ *                              var(dtheta) = 2 Dr dt, tumbles at u < rate dt
+ *                              to 2 pi v exactly, periodic wrap.
  *   rings                      regular n-gon area n l^2/(4 tan(pi/n)), spring
  *                              force zero at rest length, area force zero at
  *                              A0 and equal to -dE/dx by finite differences,
@@ -225,6 +232,20 @@ static void ht_force(const double *p, const double *dr, double *f) {
     f[1] = q * dr[1];
 }
 
+/* calc_interaction(i, j, ::SzaboCfg), src/integration.jl:68-86:
+ *   dist > r_max -> 0; f_mod = k_adh/r_eq (dist > r_eq) or k_rep/(r_max - r_eq);
+ *   -f_mod * (dist - r_eq) * dr, evaluated ((-f_mod) * r) * dr. */
+static void szabo_force(const double *p, const double *dr, double *f) {
+    double k_rep = p[0], k_adh = p[1], r_eq = p[2], r_max = p[3];
+    double dist = sqrt(dr[0] * dr[0] + dr[1] * dr[1]);
+    if (dist > r_max) { f[0] = 0.0; f[1] = 0.0; return; }
+    double f_mod = (dist > r_eq) ? k_adh / r_eq : k_rep / (r_max - r_eq);
+    double r = dist - r_eq;
+    double q = -f_mod * r;
+    f[0] = q * dr[0];
+    f[1] = q * dr[1];
+}
+
 static inline int32_t ring_of(const or_sys *s, int64_t idx) { /* get_ring_id, src/rings/states.jl:142 */
     return (int32_t)((idx - 1) / s->nmax + 1);
 }
@@ -268,7 +289,8 @@ void or_pair_force(const or_sys *s, const double *pos, int64_t i, int64_t j, dou
     if (s->kind == OR_KIND_RINGS) { rings_force(s, pos, i, j, f); return; }
     double dr[2];
     or_calc_diff(s, pos + 2 * (i - 1), pos + 2 * (j - 1), dr);
-    if (s->kind == OR_KIND_LJ) lj_force(s->p, dr, f);
+    if (s->kind == OR_KIND_LJ || s->kind == OR_KIND_RTP) lj_force(s->p, dr, f);
+    else if (s->kind == OR_KIND_SZABO) szabo_force(s->p, dr, f);
     else ht_force(s->p, dr, f);
 }
 
@@ -472,6 +494,69 @@ int or_newton_steps(const or_sys *s, double *pos, double *vel, double *forces, i
         if (pe) pe[step] = (s->kind == OR_KIND_LJ) ? or_potential_energy_lj(s, pos) : 0.0;
     }
     free(old);
+    free(cp);
+    free(counts);
+    return err;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Self-propelled particles (SelfPropelledState, src/states.jl:104-125)     */
+/* ------------------------------------------------------------------------ */
+
+int or_sp_steps(const or_sys *s, double *pos, double *ang, int64_t nsteps, const double *noise, double *forces,
+                int nthreads) {
+    if (s->kind != OR_KIND_SZABO && s->kind != OR_KIND_RTP) return OR_E_BADARG;
+    if (!s->periodic) return OR_E_BADARG; /* walls! rigid reads state.vel, absent here */
+    const int64_t n = s->n;
+    int64_t ncell = (int64_t)s->rows * s->cols;
+    int64_t *cp = NULL, *counts = NULL;
+    if (s->cols > 0) {
+        cp = (int64_t *)malloc(sizeof(int64_t) * (size_t)s->nc * (ncell > 0 ? ncell : 1));
+        counts = (int64_t *)malloc(sizeof(int64_t) * (ncell > 0 ? ncell : 1));
+    }
+    double *f = forces ? forces : (double *)malloc(sizeof(double) * 2 * n);
+    const double dt = s->dt;
+    const double vo = s->ap[0];
+    const double two_pi = 2.0 * 3.141592653589793; /* 2*pi in Julia: Float64(2) * Float64(pi) */
+    int err = OR_OK;
+    for (int64_t step = 0; step < nsteps; step++) {
+        if (s->cols > 0) {
+            err = or_bin(s, pos, cp, counts); /* update_chunks! */
+            if (err) break;
+        }
+        or_calc_forces_threaded(s, pos, cp, counts, f, nthreads); /* clean_forces! + calc_forces! */
+        for (int64_t i = 0; i < n; i++) {
+            const double theta = ang[i];
+            const double pol[2] = {cos(theta), sin(theta)};
+            double vel[2];
+            if (s->kind == OR_KIND_SZABO) {
+                /* update_szabo!, src/integration.jl:433-465 */
+                const double mu = s->ap[1], relax_time = s->ap[2], dr = s->ap[3];
+                vel[0] = vo * pol[0] + mu * f[2 * i];
+                vel[1] = vo * pol[1] + mu * f[2 * i + 1];
+                const double speed = sqrt(fabs(vel[0]) + fabs(vel[1])); /* sqrt(sum(abs, vel)) */
+                double cross_prod = 0.0;
+                if (speed > 0.0) cross_prod = (pol[0] * vel[1] - pol[1] * vel[0]) / speed;
+                if (fabs(cross_prod) > 1.0) cross_prod = jl_sign(cross_prod);
+                const double xi = noise ? noise[step * n + i] : 0.0;
+                const double d_theta = 1.0 / relax_time * asin(cross_prod) * dt + sqrt(2.0 * dr * dt) * xi;
+                pos[2 * i] = pos[2 * i] + vel[0] * dt;
+                pos[2 * i + 1] = pos[2 * i + 1] + vel[1] * dt;
+                ang[i] = ang[i] + d_theta;
+            } else {
+                /* update_rtp!, src/integration.jl:467-489 */
+                const double tumble_rate = s->ap[1];
+                vel[0] = vo * pol[0] + f[2 * i];
+                vel[1] = vo * pol[1] + f[2 * i + 1];
+                pos[2 * i] = pos[2 * i] + vel[0] * dt;
+                pos[2 * i + 1] = pos[2 * i + 1] + vel[1] * dt;
+                if (noise && noise[(step * n + i) * 2] < tumble_rate * dt)
+                    ang[i] = two_pi * noise[(step * n + i) * 2 + 1];
+            }
+        }
+        for (int64_t i = 0; i < n; i++) walls_periodic(s, pos, i); /* walls! */
+    }
+    if (!forces) free(f);
     free(cp);
     free(counts);
     return err;

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-enum { OR_KIND_LJ = 0, OR_KIND_HARMTRUNC = 1, OR_KIND_RINGS = 2 };
+enum { OR_KIND_LJ = 0, OR_KIND_HARMTRUNC = 1, OR_KIND_RINGS = 2, OR_KIND_SZABO = 3, OR_KIND_RTP = 4 };
 enum { OR_OK = 0, OR_E_CELL_OVERFLOW = 1, OR_E_OUT_OF_BOX = 2, OR_E_BADARG = 5 };
 
 /* One system.  Mirrors what Mavi's System holds for the hot path:
@@ -39,7 +39,9 @@ typedef struct {
     int32_t periodic;   /* 1: PeriodicWalls, 0: RigidWalls */
     int32_t kind;       /* OR_KIND_* */
     double p[4];        /* LJ: sigma, epsilon, rcut (<=0: no cutoff, reference semantics)
-                           HT: k_rep, k_atr, dist_eq, dist_max */
+                           HT: k_rep, k_atr, dist_eq, dist_max
+                           Szabo: k_rep, k_adh, r_eq, r_max
+                           RTP: sigma, epsilon, cutoff 2^(1/6) sigma */
     double dt;
     int32_t cols, rows; /* ChunksCfg; cols == 0 -> chunks = nothing (all pairs) */
     int32_t nc;         /* chunk capacity (src/chunks.jl:32-34) */
@@ -54,6 +56,9 @@ typedef struct {
                                 rot_diff, k_area, k_spring, l_spring */
     const double *inter;     /* ntypes x ntypes HarmTrunc params, 4 doubles each,
                                 index (t1-1) + ntypes*(t2-1) (Julia column-major) */
+    /* Self-propelled update (kind SZABO: vo, mobility, relax_time, rot_diff;
+       kind RTP: vo, tumble_rate, -, -) */
+    double ap[4];
 } or_sys;
 
 /* Julia Base `div(a::Float64, b::Float64)` (RoundToZero): round((a - rem(a,b))/b). */
@@ -127,6 +132,20 @@ int or_rings_forces(const or_sys *s, const double *pos, const double *cpos,
  * forces/areas/cms (nullable) receive the last step's values. */
 int or_rings_steps(const or_sys *s, double *pos, double *pol, int64_t nsteps,
                    const double *noise, double *forces, double *areas, double *cms);
+
+/* szabo_step! / rtp_step! x nsteps, src/integration.jl:433-489, 517-535:
+ * update_chunks!, calc_forces! (Szabo calc_interaction :68-86, run-and-tumble
+ * :88-109), update_szabo! / update_rtp!, periodic walls!.  pos and the
+ * polarisation angles ang (n) are updated in place; forces (nullable)
+ * receives the last step's forces.  noise (nullable = zero noise / never
+ * tumble) replaces the global RNG draws, per step and particle id:
+ *   Szabo: noise[step*n + i]           the randn() of :459
+ *   RTP:   noise[(step*n + i)*2 + 0]   the rand() of :484 (tumble test)
+ *          noise[(step*n + i)*2 + 1]   the rand() of :486 (new angle)
+ * (the reference draws the second uniform only when tumbling; here both are
+ * given, so a stream of draws cannot be reproduced, only the values). */
+int or_sp_steps(const or_sys *s, double *pos, double *ang, int64_t nsteps, const double *noise, double *forces,
+                int nthreads);
 
 #ifdef __cplusplus
 }

@@ -15,7 +15,7 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIB = HERE / "liboracle.so"
 
-KIND_LJ, KIND_HT, KIND_RINGS = 0, 1, 2
+KIND_LJ, KIND_HT, KIND_RINGS, KIND_SZABO, KIND_RTP = 0, 1, 2, 3, 4
 
 
 def build() -> Path:
@@ -49,6 +49,7 @@ class OrSys(C.Structure):
         ("np_type", C.POINTER(C.c_int32)),
         ("ring_par", C.POINTER(C.c_double)),
         ("inter", C.POINTER(C.c_double)),
+        ("ap", C.c_double * 4),
     ]
 
 
@@ -75,6 +76,7 @@ _SIG = {
     "or_rings_area": (C.c_double, [_S, _D, C.c_int32]),
     "or_rings_forces": (C.c_int, [_S, _D, _D, _D, _D]),
     "or_rings_steps": (C.c_int, [_S, _D, _D, C.c_int64, _D, _D, _D, _D]),
+    "or_sp_steps": (C.c_int, [_S, _D, _D, C.c_int64, _D, _D, C.c_int]),
 }
 
 _lib = None
@@ -114,7 +116,7 @@ class Sys:
     """One oracle system.  Positions are (N, 2) float64 arrays; ids 1-based."""
 
     def __init__(self, *, n, bl, size, periodic, kind, p, dt=0.0, cols=0, rows=0, nc=0, radius=0.0,
-                 nmax=0, nrings=0, types=None, np_type=None, ring_par=None, inter=None):
+                 nmax=0, nrings=0, types=None, np_type=None, ring_par=None, inter=None, ap=None):
         s = OrSys()
         s.n = n
         s.bl[0], s.bl[1] = bl
@@ -130,6 +132,8 @@ class Sys:
             s.ch = size[1] / rows
         s.nc = nc
         s.radius = radius
+        for i, v in enumerate(ap or ()):
+            s.ap[i] = v
         self._keep = []
         if kind == KIND_RINGS:
             s.nmax, s.nrings = nmax, nrings
@@ -264,6 +268,19 @@ class Sys:
         return pos, pol, f, areas, cms
 
 
+    # self-propelled (Szabo / run-and-tumble)
+    def sp_steps(self, pos, ang, nsteps, noise=None, nthreads=1):
+        pos = np.array(pos, dtype=np.float64, order="C")
+        ang = np.array(ang, dtype=np.float64, order="C")
+        if noise is not None:
+            noise = np.ascontiguousarray(noise, dtype=np.float64)
+        f = np.zeros((self.n, 2))
+        err = lib().or_sp_steps(self.ptr, _d(pos), _d(ang), nsteps, _d(noise), _d(f), nthreads)
+        if err:
+            raise RuntimeError(f"oracle sp_steps error {err}")
+        return pos, ang, f
+
+
 def neighbors(rows, cols, periodic):
     nb = np.zeros(rows * cols * 8, dtype=np.int64)
     nnb = np.zeros(rows * cols, dtype=np.int32)
@@ -279,13 +296,23 @@ def neighbors(rows, cols, periodic):
 # --- build an oracle system from the mavi_hip host objects -----------------
 def from_system(pos_n, space_cfg, dynamic_cfg, int_cfg, chunks=None, radius=None):
     """Oracle Sys mirroring a plain GPU System configuration."""
-    from mavi_hip.configs import HarmTruncCfg, LenJonesCfg, PeriodicWalls, particle_radius
+    from mavi_hip.configs import (HarmTruncCfg, LenJonesCfg, PeriodicWalls, RunTumbleCfg, SzaboCfg,
+                                  particle_radius)
 
     g = space_cfg.geometry_cfg
+    ap = None
     if isinstance(dynamic_cfg, LenJonesCfg):
         kind, p = KIND_LJ, [dynamic_cfg.sigma, dynamic_cfg.epsilon, dynamic_cfg.cutoff or 0.0, 0.0]
     elif isinstance(dynamic_cfg, HarmTruncCfg):
         kind, p = KIND_HT, [dynamic_cfg.k_rep, dynamic_cfg.k_atr, dynamic_cfg.dist_eq, dynamic_cfg.dist_max]
+    elif isinstance(dynamic_cfg, SzaboCfg):
+        c = dynamic_cfg
+        kind, p = KIND_SZABO, [c.k_rep, c.k_adh, c.r_eq, c.r_max]
+        ap = [c.vo, c.mobility, c.relax_time, c.rot_diff]
+    elif isinstance(dynamic_cfg, RunTumbleCfg):
+        c = dynamic_cfg
+        kind, p = KIND_RTP, [c.sigma, c.epsilon, c.cutoff(), 0.0]
+        ap = [c.vo, c.tumble_rate, 0.0, 0.0]
     else:
         raise TypeError(type(dynamic_cfg))
     cols = rows = nc = 0
@@ -295,7 +322,8 @@ def from_system(pos_n, space_cfg, dynamic_cfg, int_cfg, chunks=None, radius=None
         nc = chunk_capacity(g.length / cols, g.height / rows, r)
     return Sys(n=pos_n, bl=g.bottom_left, size=(g.length, g.height),
                periodic=isinstance(space_cfg.wall_type, PeriodicWalls), kind=kind, p=p, dt=int_cfg.dt,
-               cols=cols, rows=rows, nc=nc, radius=radius if radius is not None else particle_radius(dynamic_cfg))
+               cols=cols, rows=rows, nc=nc, radius=radius if radius is not None else particle_radius(dynamic_cfg),
+               ap=ap)
 
 
 def from_rings(state, space_cfg, cfg, int_cfg):

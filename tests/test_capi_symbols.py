@@ -48,7 +48,27 @@ def test_abi_version_callable():
     from mavi_hip.build import library_path
 
     lib = ctypes.CDLL(str(library_path()))
-    assert lib.mavi_abi_version() == 1
+    v = re.search(r"#define MAVI_ABI_VERSION (\d+)", (ROOT / "include" / "mavi_hip.h").read_text())
+    assert lib.mavi_abi_version() == int(v.group(1))
+
+
+def test_descriptor_layout_matches_header(tmp_path):
+    """The ctypes mirror of mavi_sys_desc has the C layout (gcc on the header)."""
+    import subprocess
+
+    from mavi_hip._lib import SysDesc
+
+    fields = [f for f, _ in SysDesc._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "mavi_hip.h"\nint main(void) {\n'
+                   '  printf("%zu\\n", sizeof(mavi_sys_desc));\n'
+                   + "".join(f'  printf("%zu\\n", offsetof(mavi_sys_desc, {f}));\n' for f in fields)
+                   + "  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)], check=True)
+    out = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert out[0] == ctypes.sizeof(SysDesc)
+    assert out[1:] == [getattr(SysDesc, f).offset for f in fields]
 
 
 def test_host_layout_helpers():
