@@ -108,6 +108,54 @@ def bench_rings(args):
     return out
 
 
+def bench_active(args, kind):
+    """Self-propelled particles, nside^2 = 1M: szabo_step! (examples/szabo.jl
+    parameters, dt 0.01) or rtp_step! (examples/rtp.jl parameters, density
+    0.8, dt 1e-3), device Philox noise.  One step = binning + one fused
+    pair-force/update launch."""
+    import mavi_hip as mv
+    from mavi_hip import workloads as wl
+
+    if kind == "szabo":
+        pos, ang, sp, dyn, ic = wl.szabo_lattice(nside=args.active_nside)
+        what = ("Szabo self-propelled particles, lattice spacing r_eq in a periodic box, vo 1, mobility 1, "
+                "relax_time 1, k_rep 10, k_adh 0.75, r_eq 1, r_max 1.1, rot_diff 0.01, dt 0.01")
+    else:
+        pos, ang, sp, dyn, ic = wl.rtp_gas(nside=args.active_nside)
+        what = ("run-and-tumble particles, density 0.8 in a periodic box, vo 1, LJ sigma 1 epsilon 1 cut at "
+                "2^(1/6) sigma, tumble_rate 1, dt 1e-3")
+    n = len(pos)
+    log(f"[bench] {kind}: {n} particles, chunks {ic.chunks_cfg.num_cols}^2")
+    system = mv.System(state=mv.SelfPropelledState(pos=pos, pol_angle=ang), space_cfg=sp, dynamic_cfg=dyn,
+                       int_cfg=ic, seed=6)
+    dev = system.dev
+    dev.step(args.warmup)
+    t0 = time.perf_counter()
+    dev.step(args.active_steps)
+    wall = time.perf_counter() - t0
+    dev.prof_enable(True)
+    dev.step(20)
+    prof = {name: dev.prof_get(k) for k, name in ((0, "bin"), (1, "pair_update"))}
+    dev.prof_enable(False)
+    out = {"workload": f"{what}, {n} particles, {ic.chunks_cfg.num_cols}^2 chunks, device Philox noise",
+           "particles": n, "steps": args.active_steps, "value": n * args.active_steps / wall,
+           "unit": "particle-steps/s", "ms_per_step": wall / args.active_steps * 1e3,
+           "kernel_avg_us": {k: v[0] / max(v[1], 1) * 1e3 for k, v in prof.items()}}
+    if not args.no_cpu:
+        import oracle
+
+        o = oracle.from_system(n, sp, dyn, ic)
+        t0 = time.perf_counter()
+        o.sp_steps(pos, ang, args.active_cpu_steps)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": n * args.active_cpu_steps / dt, "unit": "particle-steps/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"{n} particles x {args.active_cpu_steps} steps on one core, zero noise "
+                                         f"({dt:.1f} s)"}
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    return out
+
+
 def bench_batched(args, plain_value):
     """Replica batch: 512 replicas (log-uniform 1k-64k particles, half LJ
     gases, half active-ring mixtures), every phase one launch over all
@@ -317,6 +365,10 @@ def main():
     ap.add_argument("--rings", type=int, default=40000, help="rings in the rings form (0 = skip)")
     ap.add_argument("--rings-steps", type=int, default=500)
     ap.add_argument("--rings-cpu-steps", type=int, default=40)
+    ap.add_argument("--active-nside", type=int, default=1024,
+                    help="side of the Szabo and run-and-tumble lattices (0 = skip those forms)")
+    ap.add_argument("--active-steps", type=int, default=500)
+    ap.add_argument("--active-cpu-steps", type=int, default=20)
     ap.add_argument("--batch", type=int, default=512, help="replicas in the batched form (0 = skip)")
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--batch-graph", type=int, default=100, help="steps per captured graph of the batch")
@@ -428,6 +480,9 @@ def main():
     del system, dev
     if args.rings > 0:
         forms["rings"] = bench_rings(args)
+    if args.active_nside > 0:
+        forms["szabo"] = bench_active(args, "szabo")
+        forms["run_and_tumble"] = bench_active(args, "rtp")
     if args.batch > 0:
         log(f"[bench] batched form, {args.batch} replicas")
         forms["batched"] = bench_batched(args, value)
