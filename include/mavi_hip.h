@@ -53,6 +53,32 @@ extern "C" {
 typedef struct mavi_ctx mavi_ctx;
 typedef struct mavi_sys mavi_sys;
 
+/* One (wall, geometry) pair of a SpaceCfg.  A list of them is ManyWalls /
+ * ManyGeometries (src/configs.jl:30-32, 242-244, 289-301), applied in list
+ * order; its first entry is the main wall (calc_diff and the chunk stencil,
+ * src/integration.jl:50-52, src/chunks.jl:27). */
+#define MAVI_WK_RIGID_RECT 0        /* RigidWalls + RectangleCfg: walls! src/integration.jl:271-285 */
+#define MAVI_WK_PERIODIC_RECT 1     /* PeriodicWalls + RectangleCfg: walls! :309-324 */
+#define MAVI_WK_RIGID_CIRCLE 2      /* RigidWalls + CircleCfg: walls! :287-307 */
+#define MAVI_WK_SLIPPERY_LINES 3    /* SlipperyWalls + LinesCfg: walls! :327-375 */
+#define MAVI_WK_SLIPPERY_CIRCLE 4   /* SlipperyWalls + CircleCfg: walls! :377-396 */
+#define MAVI_WK_POTENTIAL_CIRCLE 5  /* PotentialWalls + CircleCfg: calc_walls_forces! :230-240 */
+#define MAVI_WK_POTENTIAL_LINES 6   /* PotentialWalls + LinesCfg: calc_walls_forces! :242-255 */
+#define MAVI_WMODE_OUTSIDE 0        /* PotentialWallMode, src/configs.jl:255-261 */
+#define MAVI_WMODE_INSIDE 1
+#define MAVI_WMODE_REPULSION 2
+typedef struct {
+    int32_t kind;           /* MAVI_WK_* */
+    int32_t mode;           /* potential walls: MAVI_WMODE_* */
+    int32_t pot_kind;       /* potential walls: MAVI_POT_LJ or MAVI_POT_HARMTRUNC */
+    int32_t nlines;         /* LinesCfg: segments in `lines` */
+    double geom[4];         /* rectangle: bottom_left x, y, length, height;
+                               circle: center x, y, radius */
+    double pot[4];          /* potential walls: LJ sigma, epsilon; HarmTrunc
+                               k_rep, k_atr, dist_eq, dist_max */
+    const double *lines;    /* LinesCfg: nlines x (p1x, p1y, p2x, p2y) */
+} mavi_wall;
+
 /* Everything the step needs from System/SpaceCfg/DynamicCfg/IntCfg/Chunks. */
 typedef struct {
     int64_t n;              /* length(state.pos): particle slots */
@@ -84,6 +110,12 @@ typedef struct {
     uint64_t seed;          /* device noise stream (Philox4x32-10) */
     double active[4];       /* Szabo: vo, mobility, relax_time, rot_diff;
                                RTP: vo, tumble_rate (self-propelled only) */
+    int32_t nwalls;         /* > 0: the walls list replaces `wall`; the
+                               rectangle above is then the chunk space, the
+                               bounding box of every geometry (get_chunks,
+                               src/systems.jl:14-28) */
+    int32_t reserved2;
+    const mavi_wall *walls; /* read during mavi_sys_create only */
 } mavi_sys_desc;
 
 /* Library / context ------------------------------------------------------ */

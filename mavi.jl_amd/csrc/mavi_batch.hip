@@ -433,6 +433,7 @@ int mavi_batch_add_replica(mavi_batch *b, const mavi_sys_desc *d, int32_t *index
     if (h.P.cols == 0) return set_err(ctx, MAVI_E_UNSUPPORTED, "batch replicas need chunks (cols, rows > 0)");
     if (h.P.potential == MAVI_POT_SZABO || h.P.potential == MAVI_POT_RTP)
         return set_err(ctx, MAVI_E_UNSUPPORTED, "batch replicas are LJ / HarmTrunc gases or rings");
+    if (h.P.nwalls > 0) return set_err(ctx, MAVI_E_UNSUPPORTED, "batch replicas take a single rectangle wall");
     if (h.P.potential == MAVI_POT_RINGS && h.P.nmax > 32)
         return set_err(ctx, MAVI_E_UNSUPPORTED, "batch rings replicas need nmax <= 32");
     b->hs.push_back(std::move(h));

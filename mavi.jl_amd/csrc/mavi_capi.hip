@@ -99,6 +99,7 @@ int mavi_sys_destroy(mavi_sys *s) {
     dfree(s->active_ids); dfree(s->pos_id); dfree(s->vel_id); dfree(s->force_id);
     dfree(s->pol); dfree(s->areas); dfree(s->cms); dfree(s->cms_next); dfree(s->cpos);
     dfree(s->rtype); dfree(s->types); dfree(s->inter); dfree(s->noise); dfree(s->red);
+    dfree(s->walls); dfree(s->lines);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
     delete s;
@@ -117,15 +118,74 @@ int build_params(const mavi_sys_desc *d, HostSys &h, std::string &err) {
     };
     if (d->n <= 0 || d->n > (int64_t)0x7fffffff / 2) return fail(MAVI_E_BADARG, "n must be in [1, 2^30)");
     if (!(d->size[0] > 0.0) || !(d->size[1] > 0.0)) return fail(MAVI_E_BADARG, "box size must be > 0");
-    if (d->wall != MAVI_WALL_RIGID && d->wall != MAVI_WALL_PERIODIC)
-        return fail(MAVI_E_UNSUPPORTED, "only RigidWalls and PeriodicWalls rectangles run on the GPU");
+    if (d->nwalls < 0 || (d->nwalls > 0 && !d->walls)) return fail(MAVI_E_BADARG, "walls list: nwalls < 0 or null");
+    if (d->nwalls == 0 && d->wall != MAVI_WALL_RIGID && d->wall != MAVI_WALL_PERIODIC)
+        return fail(MAVI_E_UNSUPPORTED, "unknown wall type");
     if (d->potential < MAVI_POT_LJ || d->potential > MAVI_POT_RTP)
         return fail(MAVI_E_UNSUPPORTED, "unknown potential kind");
     const bool active = d->potential == MAVI_POT_SZABO || d->potential == MAVI_POT_RTP;
-    if (active && d->wall != MAVI_WALL_PERIODIC)
+    // walls list: the main (first) wall decides periodicity
+    bool periodic = d->wall == MAVI_WALL_PERIODIC;
+    bool rigid_listed = false;
+    h.walls.clear();
+    h.lines.clear();
+    if (d->nwalls > 0) {
+        if (d->nwalls > 64) return fail(MAVI_E_BADARG, "walls list longer than 64");
+        periodic = d->walls[0].kind == MAVI_WK_PERIODIC_RECT;
+        for (int k = 0; k < d->nwalls; k++) {
+            const mavi_wall &w = d->walls[k];
+            if (w.kind < MAVI_WK_RIGID_RECT || w.kind > MAVI_WK_POTENTIAL_LINES)
+                return fail(MAVI_E_UNSUPPORTED, "unknown wall kind");
+            if (w.kind == MAVI_WK_PERIODIC_RECT && k > 0)
+                return fail(MAVI_E_UNSUPPORTED, "PeriodicWalls must be the main (first) wall");
+            if ((w.kind == MAVI_WK_POTENTIAL_CIRCLE || w.kind == MAVI_WK_POTENTIAL_LINES) &&
+                w.pot_kind != MAVI_POT_LJ && w.pot_kind != MAVI_POT_HARMTRUNC)
+                return fail(MAVI_E_UNSUPPORTED, "potential walls take a LenJonesCfg or HarmTruncCfg");
+            if ((w.kind == MAVI_WK_SLIPPERY_LINES || w.kind == MAVI_WK_POTENTIAL_LINES) && (w.nlines <= 0 || !w.lines))
+                return fail(MAVI_E_BADARG, "LinesCfg wall without lines");
+            rigid_listed = rigid_listed || w.kind == MAVI_WK_RIGID_RECT || w.kind == MAVI_WK_RIGID_CIRCLE;
+            DevWall dw{};
+            dw.kind = w.kind;
+            dw.mode = w.mode;
+            dw.pot_kind = w.pot_kind;
+            for (int j = 0; j < 4; j++) {
+                dw.g[j] = w.geom[j];
+                dw.pot[j] = w.pot[j];
+            }
+            if (w.kind == MAVI_WK_SLIPPERY_LINES || w.kind == MAVI_WK_POTENTIAL_LINES) {
+                dw.first = (int32_t)h.lines.size();
+                dw.nlines = w.nlines;
+                for (int l = 0; l < w.nlines; l++) {
+                    // Line2D(p1, p2), src/configs.jl:99-117
+                    const double *q = w.lines + 4 * l;
+                    DevLine L{};
+                    L.p1x = q[0]; L.p1y = q[1]; L.p2x = q[2]; L.p2y = q[3];
+                    const double drx = q[2] - q[0], dry = q[3] - q[1];
+                    const double norm = sqrt(drx * drx + dry * dry);
+                    if (!(norm > 0.0)) return fail(MAVI_E_BADARG, "a line of zero length");
+                    L.nx = -dry / norm; L.ny = drx / norm;
+                    L.tx = drx / norm; L.ty = dry / norm;
+                    L.len = norm;
+                    h.lines.push_back(L);
+                }
+            }
+            h.walls.push_back(dw);
+        }
+        if (periodic) {
+            const double *g = d->walls[0].geom;
+            if (g[0] != d->bottom_left[0] || g[1] != d->bottom_left[1] || g[2] != d->size[0] || g[3] != d->size[1])
+                return fail(MAVI_E_UNSUPPORTED,
+                            "a periodic main rectangle must be the bounding box of every geometry");
+        }
+    }
+    if (active && (rigid_listed || (d->nwalls == 0 && !periodic)))
         return fail(MAVI_E_UNSUPPORTED,
-                    "self-propelled states run with PeriodicWalls (walls! with RigidWalls reads state.vel, which "
+                    "self-propelled states take no RigidWalls (walls! with RigidWalls reads state.vel, which "
                     "SelfPropelledState does not have)");
+    if (d->potential == MAVI_POT_RINGS && rigid_listed)
+        return fail(MAVI_E_UNSUPPORTED, "rings take no RigidWalls (walls! reads state.vel, which RingsState lacks)");
+    if (d->potential == MAVI_POT_RINGS && d->nwalls > 0 && d->nmax > 32)
+        return fail(MAVI_E_UNSUPPORTED, "rings with a walls list need nmax <= 32");
     const bool chunks = d->cols > 0 || d->rows > 0;
     if (chunks) {
         if (d->cols <= 0 || d->rows <= 0) return fail(MAVI_E_BADARG, "cols and rows must both be > 0");
@@ -144,7 +204,8 @@ int build_params(const mavi_sys_desc *d, HostSys &h, std::string &err) {
     P.rows = chunks ? d->rows : 0;
     P.ncell = P.cols * P.rows;
     P.nc = chunks ? d->nc : 0;
-    P.periodic = d->wall == MAVI_WALL_PERIODIC;
+    P.periodic = periodic;
+    P.nwalls = (int32_t)h.walls.size();
     P.potential = d->potential;
     P.blx = d->bottom_left[0];
     P.bly = d->bottom_left[1];
@@ -206,7 +267,7 @@ int build_params(const mavi_sys_desc *d, HostSys &h, std::string &err) {
     h.inter.clear();
     h.seed = d->seed;
     if (d->potential == MAVI_POT_RINGS) {
-        if (d->wall != MAVI_WALL_PERIODIC) return fail(MAVI_E_UNSUPPORTED, "rings run on the GPU with PeriodicWalls");
+        if (!periodic) return fail(MAVI_E_UNSUPPORTED, "rings run on the GPU with a periodic main wall");
         if (d->nmax <= 0 || d->nrings <= 0 || (int64_t)d->nmax * d->nrings != d->n || d->ntypes <= 0 ||
             !d->type_np || !d->type_par || !d->interaction)
             return fail(MAVI_E_BADARG, "rings descriptor: need n == nmax*nrings and type tables");
@@ -222,6 +283,8 @@ int build_params(const mavi_sys_desc *d, HostSys &h, std::string &err) {
             T.rot = q[4]; T.karea = q[5]; T.kspring = q[6]; T.lspring = q[7];
             T.np = d->type_np[t];
             if (T.np < 1 || T.np > d->nmax) return fail(MAVI_E_BADARG, "ring particle count outside [1, nmax]");
+            // get_particle_radius: the type's own interaction, dist_eq / 2
+            T.wradius = d->interaction[4 * (t + d->ntypes * t) + 2] / 2.0;
             h.types.push_back(T);
         }
         h.list2 = 0.0;
@@ -313,7 +376,20 @@ int mavi_sys_create(mavi_ctx *ctx, const mavi_sys_desc *d, mavi_sys **out) {
         ALLOC(s->types, P.ntypes);
         ALLOC(s->inter, (int64_t)P.ntypes * P.ntypes);
     }
+    if (!h.walls.empty()) {
+        ALLOC(s->walls, (int64_t)h.walls.size());
+        if (!h.lines.empty()) ALLOC(s->lines, (int64_t)h.lines.size());
+    }
 #undef ALLOC
+    if (e == hipSuccess && !h.walls.empty()) {
+        e = hipMemcpyAsync(s->walls, h.walls.data(), sizeof(DevWall) * h.walls.size(), hipMemcpyHostToDevice,
+                           ctx->stream);
+        if (e == hipSuccess && !h.lines.empty())
+            e = hipMemcpyAsync(s->lines, h.lines.data(), sizeof(DevLine) * h.lines.size(), hipMemcpyHostToDevice,
+                               ctx->stream);
+        P.walls = s->walls;
+        P.lines = s->lines;
+    }
     if (e == hipSuccess) e = hipEventCreate(&s->ev0);
     if (e == hipSuccess) e = hipEventCreate(&s->ev1);
     if (e == hipSuccess && P.potential == MAVI_POT_RINGS) {

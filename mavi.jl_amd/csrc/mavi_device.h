@@ -471,27 +471,180 @@ __device__ __forceinline__ void periodic_wrap(const SysParams &P, double &x, dou
     if (oy) y = y - copysign(P.halfH * 2.0, dyc);
 }
 
-// Integrator epilogue of a pair-force launch for sorted slot s.
-template <int MODE>
-__device__ __forceinline__ void pair_epilogue(const SysParams &P, const int s, const double2 pi, const double fx,
-                                              const double fy, double2 *__restrict__ Pst, double2 *__restrict__ Vst,
+// ---------------------------------------------------------------------------
+// Walls list (ManyWalls and the non-rectangle geometries), in list order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double jl_sign(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : x); }
+
+// potential_force(dr, dist, potential) of a potential wall at the processed
+// distance (which may be negative), src/configs.jl:354-368, 389-397.
+__device__ __forceinline__ void wall_potential_force(const DevWall &w, double drx, double dry, double dist,
+                                                     double &fx, double &fy) {
+    double q;
+    if (w.pot_kind == MAVI_POT_LJ) {
+        const double sigma = w.pot[0], eps = w.pot[1];
+        const double fmod =
+            4.0 * eps * (12.0 * pow(sigma, 12.0) / pow(dist, 13.0) - 6.0 * pow(sigma, 6.0) / pow(dist, 7.0));
+        q = fmod / dist;
+    } else {
+        if (dist > w.pot[3]) return;
+        const double fmod = (dist < w.pot[2]) ? -w.pot[0] * (dist / w.pot[2] - 1.0) : -w.pot[1] * (dist / w.pot[2] - 1.0);
+        q = fmod / dist;
+    }
+    fx += q * drx;
+    fy += q * dry;
+}
+
+__device__ __forceinline__ double process_dist(int mode, double dist, double flag) {
+    // src/configs.jl:259-261
+    if (mode == MAVI_WMODE_OUTSIDE) return flag * dist;
+    if (mode == MAVI_WMODE_INSIDE) return -flag * dist;
+    return dist;
+}
+
+// calc_walls_forces!, src/integration.jl:230-255: potential walls only.
+__device__ __forceinline__ void wall_forces(const SysParams &P, double x, double y, double &fx, double &fy) {
+    for (int k = 0; k < P.nwalls; k++) {
+        const DevWall w = P.walls[k];
+        if (w.kind == MAVI_WK_POTENTIAL_CIRCLE) {
+            // signed_pos(point, ::CircleCfg), src/configs.jl:154-162
+            const double d0 = x - w.g[0], d1 = y - w.g[1];
+            const double dist = sqrt(d0 * d0 + d1 * d1);
+            const double h0 = d0 / dist, h1 = d1 / dist;
+            const double sd = dist - w.g[2];
+            wall_potential_force(w, d0 - h0 * w.g[2], d1 - h1 * w.g[2], process_dist(w.mode, fabs(sd), jl_sign(sd)),
+                                 fx, fy);
+        } else if (w.kind == MAVI_WK_POTENTIAL_LINES) {
+            for (int l = 0; l < w.nlines; l++) {
+                // signed_pos(point, ::Line2D), src/configs.jl:119-134
+                const DevLine L = P.lines[w.first + l];
+                double drx = x - L.p1x, dry = y - L.p1y;
+                const double delta_t = drx * L.tx + dry * L.ty;
+                if (delta_t > 0.0) {
+                    if (delta_t < L.len) {
+                        drx = x - (L.p1x + L.tx * delta_t);
+                        dry = y - (L.p1y + L.ty * delta_t);
+                    } else {
+                        drx = x - L.p2x;
+                        dry = y - L.p2y;
+                    }
+                }
+                const double dist = sqrt(drx * drx + dry * dry);
+                wall_potential_force(w, drx, dry, process_dist(w.mode, dist, 1.0), fx, fy);
+            }
+        }
+    }
+}
+
+// walls! over the list, src/integration.jl:271-403.  vx/vy may be null when
+// no rigid wall is listed (rings, self-propelled states).
+__device__ __forceinline__ void walls_apply(const SysParams &P, double &x, double &y, double *vx, double *vy,
+                                            double radius) {
+    for (int k = 0; k < P.nwalls; k++) {
+        const DevWall w = P.walls[k];
+        if (w.kind == MAVI_WK_RIGID_RECT) {
+            const double rx = x - w.g[0], ry = y - w.g[1];
+            const bool ox = ((rx + radius) > w.g[2]) || ((rx - radius) < 0.0);
+            const bool oy = ((ry + radius) > w.g[3]) || ((ry - radius) < 0.0);
+            if (ox) *vx = -*vx;
+            if (oy) *vy = -*vy;
+        } else if (w.kind == MAVI_WK_PERIODIC_RECT) {
+            const double hx = w.g[2] / 2.0, hy = w.g[3] / 2.0;
+            const double dxc = x - (w.g[0] + hx), dyc = y - (w.g[1] + hy);
+            if (fabs(dxc) > hx) x = x - copysign(hx * 2.0, dxc);
+            if (fabs(dyc) > hy) y = y - copysign(hy * 2.0, dyc);
+        } else if (w.kind == MAVI_WK_RIGID_CIRCLE) {
+            // the reflection uses pos.x * pos.y as the reference writes it
+            const double m = w.g[2] - radius;
+            const double dx = x - w.g[0], dy = y - w.g[1];
+            const double dx2 = dx * dx, dy2 = dy * dy;
+            const double r2 = dx2 + dy2;
+            if (r2 > m * m) {
+                const double ux = *vx, uy = *vy;
+                *vx = (ux * (dy2 - dx2) - 2.0 * uy * x * y) / r2;
+                *vy = (-uy * (dy2 - dx2) - 2.0 * ux * x * y) / r2;
+            }
+        } else if (w.kind == MAVI_WK_SLIPPERY_LINES) {
+            // corrections from the position at the start of this wall, added
+            const double x0 = x, y0 = y;
+            for (int l = 0; l < w.nlines; l++) {
+                const DevLine L = P.lines[w.first + l];
+                const double drx = x0 - L.p1x, dry = y0 - L.p1y;
+                const double delta_s = drx * L.nx + dry * L.ny;
+                if (fabs(delta_s) > radius) continue;
+                const double delta_t = drx * L.tx + dry * L.ty;
+                bool corner = false;
+                double cpx = 0.0, cpy = 0.0;
+                if (delta_t > 0.0) {
+                    if (delta_t > L.len) {
+                        if (delta_t > (L.len + radius)) continue;
+                        corner = true;
+                        cpx = L.p2x;
+                        cpy = L.p2y;
+                    }
+                } else if (delta_t > -radius) {
+                    corner = true;
+                    cpx = L.p1x;
+                    cpy = L.p1y;
+                } else {
+                    continue;
+                }
+                if (corner) {
+                    const double c0 = x0 - cpx, c1 = y0 - cpy;
+                    const double norm = sqrt(c0 * c0 + c1 * c1);
+                    if (norm > radius) continue;
+                    const double alpha = radius / norm - 1.0;
+                    x = x + alpha * c0;
+                    y = y + alpha * c1;
+                } else {
+                    const double sgn = jl_sign(delta_s);
+                    const double alpha = sgn * (radius - sgn * delta_s);
+                    x = x + alpha * L.nx;
+                    y = y + alpha * L.ny;
+                }
+            }
+        } else if (w.kind == MAVI_WK_SLIPPERY_CIRCLE) {
+            // dr through the space's calc_diff (the main wall's)
+            const double m = w.g[2] + radius;
+            const double2 dr = cdiff(P, make_double2(x, y), make_double2(w.g[0], w.g[1]));
+            const double dr_2 = dr.x * dr.x + dr.y * dr.y;
+            if (dr_2 > m * m) continue;
+            const double dr_norm = sqrt(dr_2);
+            const double kk = w.g[2] + radius - dr_norm;
+            x = x + kk * dr.x / dr_norm;
+            y = y + kk * dr.y / dr_norm;
+        }
+    }
+}
+
+// Integrator epilogue of a pair-force launch for sorted slot s (WALLS: the
+// system has a walls list; only kernels instantiated for it carry its code).
+template <int MODE, bool WALLS = false>
+__device__ __forceinline__ void pair_epilogue(const SysParams &P, const int s, const double2 pi, const double fx_,
+                                              const double fy_, double2 *__restrict__ Pst, double2 *__restrict__ Vst,
                                               double2 *__restrict__ Fout, double2 *__restrict__ P2,
                                               double2 *__restrict__ F1) {
     if constexpr (MODE == MODE_FORCES) {
-        Fout[s] = make_double2(fx, fy);
+        Fout[s] = make_double2(fx_, fy_);
     } else if constexpr (MODE == MODE_VERLET1) {
+        // calc_walls_forces! joins the first evaluation only (src/integration.jl:512)
+        double fx = fx_, fy = fy_;
+        if constexpr (WALLS) wall_forces(P, pi.x, pi.y, fx, fy);
         // update_verlet!: pos += vel*dt + forces*term (src/integration.jl:424)
         const double2 v = Vst[s];
         P2[s] = make_double2(pi.x + (v.x * P.dt + fx * P.term), pi.y + (v.y * P.dt + fy * P.term));
         F1[s] = make_double2(fx, fy);
     } else {
         // vel += dt/2 * (forces + old_forces) (src/integration.jl:430), then walls!
+        const double fx = fx_, fy = fy_;
         const double2 v = Vst[s];
         const double2 fo = F1[s];
         double vx = v.x + P.hdt * (fx + fo.x);
         double vy = v.y + P.hdt * (fy + fo.y);
         double x = pi.x, y = pi.y;
-        if (P.periodic) {
+        if constexpr (WALLS) {
+            walls_apply(P, x, y, &vx, &vy, P.radius);
+        } else if (P.periodic) {
             periodic_wrap(P, x, y);
         } else {
             // src/integration.jl:271-285
@@ -746,7 +899,7 @@ __global__ void k_rings_init(SysParams P, const int32_t *__restrict__ rtype, con
 // to the one-thread-per-ring form.  slot_of and cpos start at the system's
 // first particle id, pol/areas/cms/cms_next at its first ring; noise (host
 // draws) or the Philox stream (seed, step) give the rotational noise of ring rr.
-template <bool UPDATE>
+template <bool UPDATE, bool WALLS = false>
 __device__ __forceinline__ void ring_lanes(const SysParams &P, const RingType &T, const bool ring_ok, const int rr,
                                            const int i, const int32_t *__restrict__ slot_of,
                                            double2 *__restrict__ Pst, double2 *__restrict__ F,
@@ -829,6 +982,9 @@ __device__ __forceinline__ void ring_lanes(const SysParams &P, const RingType &T
     }
     Fi.x -= fa * (da.y / 2.0);
     Fi.y -= fa * (-da.x / 2.0);
+    if constexpr (UPDATE && WALLS) {
+        if (act) wall_forces(P, x.x, x.y, Fi.x, Fi.y);  // calc_walls_forces!
+    }
     if (act) F[slot] = Fi;
     if constexpr (!UPDATE) return;
 
@@ -844,11 +1000,10 @@ __device__ __forceinline__ void ring_lanes(const SysParams &P, const RingType &T
         vcy += bc(vy, k);
     }
     double px = x.x + vx * P.dt, py = x.y + vy * P.dt;
-    if (P.periodic) {
-        const double dxc = px - (P.blx + P.halfL), dyc = py - (P.bly + P.halfH);
-        const bool ox = fabs(dxc) > P.halfL, oy = fabs(dyc) > P.halfH;
-        if (ox) px = px - copysign(P.halfL * 2.0, dxc);
-        if (oy) py = py - copysign(P.halfH * 2.0, dyc);
+    if constexpr (WALLS) {
+        if (act) walls_apply(P, px, py, nullptr, nullptr, T.wradius);
+    } else if (P.periodic) {
+        periodic_wrap(P, px, py);
     }
     if (act) Pst[slot] = make_double2(px, py);
     if (ring_ok && i == 0) {
@@ -866,7 +1021,7 @@ __device__ __forceinline__ void ring_lanes(const SysParams &P, const RingType &T
     }
 }
 
-template <bool UPDATE>
+template <bool UPDATE, bool WALLS = false>
 __global__ __launch_bounds__(256) void k_rings_lanes(
     SysParams P, const int32_t *__restrict__ rtype, const RingType *__restrict__ types,
     const int32_t *__restrict__ slot_of, double2 *__restrict__ Pst, double2 *__restrict__ F, double *__restrict__ pol,
@@ -877,7 +1032,7 @@ __global__ __launch_bounds__(256) void k_rings_lanes(
     const bool ring_ok = r < P.nrings;
     const int rr = ring_ok ? r : 0;
     const RingType T = types[P.typed ? rtype[rr] : 0];
-    ring_lanes<UPDATE>(P, T, ring_ok, rr, gt & 31, slot_of, Pst, F, pol, areas, cms, cms_next, cpos, noise, seed,
+    ring_lanes<UPDATE, WALLS>(P, T, ring_ok, rr, gt & 31, slot_of, Pst, F, pol, areas, cms, cms_next, cpos, noise, seed,
                        counter[0]);
 }
 

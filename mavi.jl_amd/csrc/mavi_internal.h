@@ -40,6 +40,9 @@ struct mavi_ctx {
     std::string err;
 };
 
+struct DevWall;
+struct DevLine;
+
 // Parameters every kernel of a system reads (passed by value).
 struct SysParams {
     int32_t n;         // slots
@@ -66,13 +69,30 @@ struct SysParams {
     double ap_inv_tau;        // 1 / relax_time
     double ap_noise;          // sqrt(2 rot_diff dt)
     double ap_tumble;         // tumble_rate * dt
+    // walls list (nwalls > 0 replaces the periodic / rigid rectangle)
+    int32_t nwalls, pad_w;
+    const DevWall *walls;
+    const DevLine *lines;
 };
 
-// Per ring type parameters.
+// Per ring type parameters (wradius: get_particle_radius of the type, the
+// slippery walls' particle radius, src/rings/rings.jl:44-53).
 struct RingType {
     double p0, relax, vo, mu, rot, karea, kspring, lspring;
+    double wradius;
     int32_t np;
     int32_t pad;
+};
+
+// Device copy of a mavi_wall (lines in a separate table, with the unit
+// normal, tangent and length of Line2D, src/configs.jl:99-117).
+struct DevWall {
+    int32_t kind, mode, pot_kind, first, nlines, pad;
+    double g[4];
+    double pot[4];
+};
+struct DevLine {
+    double p1x, p1y, p2x, p2y, nx, ny, tx, ty, len, pad;
 };
 
 // Interaction (HarmTrunc) for one pair of ring types, with exact thresholds.
@@ -124,6 +144,8 @@ struct mavi_sys {
     // reuse of the second force evaluation of a Verlet step as the first one
     // of the next step where a particle's 3x3 chunk block kept its members
     int32_t *stamp = nullptr;   // per chunk: binning epoch in which its membership last changed
+    DevWall *walls = nullptr;   // walls list (P.walls / P.lines point here)
+    DevLine *lines = nullptr;
     int32_t *needc = nullptr;   // per chunk: binning epoch in which a chunk of its 3x3 block changed
     int32_t *perm = nullptr;    // sorted slot -> slot before the last binning
     uint32_t *epoch = nullptr;  // binnings so far (device)
@@ -200,6 +222,8 @@ struct HostSys {
     std::vector<int32_t> rtype;    // per ring, 0-based type
     std::vector<RingType> types;
     std::vector<RingInter> inter;
+    std::vector<DevWall> walls;    // walls list (device copies are set into P)
+    std::vector<DevLine> lines;
 };
 int build_params(const mavi_sys_desc *d, HostSys &h, std::string &err);
 

@@ -118,7 +118,7 @@ __global__ void k_export_chunks(SysParams P, const int32_t *__restrict__ start, 
 // same candidates in the same order, so they are bit-identical to a
 // recomputation.  (A compacted worklist of the particles to recompute was
 // slower: it loses the sorted order's neighbour locality.)
-template <int MODE, int POT, bool ALLPAIRS>
+template <int MODE, int POT, bool ALLPAIRS, bool WALLS = false>
 __global__ void k_pair(SysParams P, RingCtx R, const double2 *__restrict__ pin, const int32_t *__restrict__ scell,
                        const int32_t *__restrict__ start, double2 *__restrict__ Pst, double2 *__restrict__ Vst,
                        double2 *__restrict__ Fout, double2 *__restrict__ P2, double2 *__restrict__ F1,
@@ -138,7 +138,7 @@ __global__ void k_pair(SysParams P, RingCtx R, const double2 *__restrict__ pin, 
             const bool need = valid && needc[scell[sv]] == (int32_t)epoch[0];
             if (valid && !need) {
                 const double2 f = Fold[so];
-                pair_epilogue<MODE>(P, s, pi, f.x, f.y, Pst, Vst, Fout, P2, F1);
+                pair_epilogue<MODE, WALLS>(P, s, pi, f.x, f.y, Pst, Vst, Fout, P2, F1);
             }
             // the block's particles to recompute go to its first lanes (a
             // particle's forces do not depend on the lane or the wave it is
@@ -176,7 +176,7 @@ __global__ void k_pair(SysParams P, RingCtx R, const double2 *__restrict__ pin, 
         pair_gather_chunks<POT>(P, R, acc, pin, scell[sv], start, s, valid, pi, list2, s_list);
     }
     if (!valid) return;
-    pair_epilogue<MODE>(P, s, pi, acc.fx, acc.fy, Pst, Vst, Fout, P2, F1);
+    pair_epilogue<MODE, WALLS>(P, s, pi, acc.fx, acc.fy, Pst, Vst, Fout, P2, F1);
 }
 
 // ---------------------------------------------------------------------------
@@ -191,7 +191,7 @@ __global__ void k_pair(SysParams P, RingCtx R, const double2 *__restrict__ pin, 
 // ---------------------------------------------------------------------------
 constexpr double kTwoPi = 2.0 * 3.141592653589793;  // 2*pi in Julia
 
-template <int MODE, int POT, bool ALLPAIRS>
+template <int MODE, int POT, bool ALLPAIRS, bool WALLS>
 __global__ void k_active(SysParams P, const double2 *__restrict__ pin, const double2 *__restrict__ ain,
                          const int32_t *__restrict__ idin, const int32_t *__restrict__ scell,
                          const int32_t *__restrict__ start, double2 *__restrict__ pout, double2 *__restrict__ aout,
@@ -209,7 +209,8 @@ __global__ void k_active(SysParams P, const double2 *__restrict__ pin, const dou
         pair_gather_chunks<POT>(P, R, acc, pin, scell[sv], start, s, valid, pi, INFINITY, nullptr);
     }
     if (!valid) return;
-    const double fx = acc.fx, fy = acc.fy;
+    double fx = acc.fx, fy = acc.fy;
+    if constexpr (WALLS) wall_forces(P, pi.x, pi.y, fx, fy);  // calc_walls_forces!
     Fout[s] = make_double2(fx, fy);
     const int id = idin[s];
     double th = ain[s].x;
@@ -243,7 +244,9 @@ __global__ void k_active(SysParams P, const double2 *__restrict__ pin, const dou
         }
         if (u < P.ap_tumble) th = kTwoPi * w;
     }
-    periodic_wrap(P, x, y);  // self-propelled systems run with PeriodicWalls only
+    // walls! (self-propelled states have no velocity: no rigid walls)
+    if constexpr (WALLS) walls_apply(P, x, y, nullptr, nullptr, P.radius);
+    else periodic_wrap(P, x, y);
     pout[s] = make_double2(x, y);
     aout[s] = make_double2(th, 0.0);
     idout[s] = id;
@@ -647,6 +650,15 @@ hipError_t launch_pair_t(mavi_sys *s, const double2 *pin, hipStream_t st, bool r
     const int wg = s->wg;
     const int c = s->cur;
     const double l2 = s->two_phase ? list_range2(s) : INFINITY;
+    if constexpr (MODE == MODE_VERLET1 || MODE == MODE_VERLET2) {
+        if (s->P.nwalls > 0) {  // gather kernel with the walls list
+            const size_t lds = (l2 < INFINITY) ? sizeof(uint16_t) * kListCap * (size_t)wg : 0;
+            hipLaunchKernelGGL((k_pair<MODE, POT, ALL, true>), dim3(nblk(s->P.na, wg)), dim3(wg), lds, st, s->P,
+                               ring_ctx(s), pin, s->scell, s->start, s->P_[c], s->V_[c], s->F, s->P2, s->F1, l2,
+                               nullptr, s->epoch, defer ? s->perm : nullptr, s->V_[1 - c], s->F);
+            return hipGetLastError();
+        }
+    }
     if constexpr (!ALL) {
         if (s->lds_tiles) {
             const int nsegc = (s->P.cols + kSegCells - 1) / kSegCells;
@@ -698,19 +710,26 @@ hipError_t launch_pair(mavi_sys *s, const double2 *pin, hipStream_t st, bool reu
     }
 }
 
-template <int MODE, int POT>
-hipError_t launch_active_t(mavi_sys *s, const double *noise_dev, hipStream_t st) {
+template <int MODE, int POT, bool ALL, bool WALLS>
+void launch_active_k(mavi_sys *s, const double *noise_dev, hipStream_t st) {
     const int wg = s->wg;
     const int c = s->cur, o = 1 - c;
-    if (s->P.cols == 0)
-        hipLaunchKernelGGL((k_active<MODE, POT, true>), dim3(nblk(s->P.na, wg)), dim3(wg), 0, st, s->P, s->P_[c],
-                           s->V_[c], s->ID_[c], s->scell, s->start, s->P_[o], s->V_[o], s->ID_[o], s->F, noise_dev,
-                           s->seed, s->counter);
-    else
-        hipLaunchKernelGGL((k_active<MODE, POT, false>), dim3(nblk(s->P.na, wg)), dim3(wg), 0, st, s->P, s->P_[c],
-                           s->V_[c], s->ID_[c], s->scell, s->start, s->P_[o], s->V_[o], s->ID_[o], s->F, noise_dev,
-                           s->seed, s->counter);
-    s->cur = o;
+    hipLaunchKernelGGL((k_active<MODE, POT, ALL, WALLS>), dim3(nblk(s->P.na, wg)), dim3(wg), 0, st, s->P, s->P_[c],
+                       s->V_[c], s->ID_[c], s->scell, s->start, s->P_[o], s->V_[o], s->ID_[o], s->F, noise_dev,
+                       s->seed, s->counter);
+}
+
+template <int MODE, int POT>
+hipError_t launch_active_t(mavi_sys *s, const double *noise_dev, hipStream_t st) {
+    const bool all = s->P.cols == 0, walls = s->P.nwalls > 0;
+    if (all) {
+        if (walls) launch_active_k<MODE, POT, true, true>(s, noise_dev, st);
+        else launch_active_k<MODE, POT, true, false>(s, noise_dev, st);
+    } else {
+        if (walls) launch_active_k<MODE, POT, false, true>(s, noise_dev, st);
+        else launch_active_k<MODE, POT, false, false>(s, noise_dev, st);
+    }
+    s->cur = 1 - s->cur;
     return hipGetLastError();
 }
 
@@ -823,7 +842,11 @@ static hipError_t launch_rings_kernel(mavi_sys *s, bool update, const double *no
     const double *nz = update ? noise_dev : nullptr;
     if (s->P.nmax <= 32) {
         const int nb = nblk((int64_t)s->P.nrings * 32, 256);
-        if (update)
+        if (update && s->P.nwalls > 0)
+            hipLaunchKernelGGL((k_rings_lanes<true, true>), dim3(nb), dim3(256), 0, st, s->P, s->rtype, s->types,
+                               s->slot_of, s->P_[c], s->F, s->pol, s->areas, s->cms, s->cms_next, s->cpos, nz,
+                               s->seed, s->counter);
+        else if (update)
             hipLaunchKernelGGL(k_rings_lanes<true>, dim3(nb), dim3(256), 0, st, s->P, s->rtype, s->types, s->slot_of,
                                s->P_[c], s->F, s->pol, s->areas, s->cms, s->cms_next, s->cpos, nz, s->seed,
                                s->counter);

@@ -674,6 +674,7 @@ int mavi_learned_create(mavi_sys *s, mavi_mlp *mlp, const mavi_learned_desc *d, 
         return set_err(ctx, MAVI_E_UNSUPPORTED,
                        "the learned propagator runs periodic rings systems with chunks (>= 3x3) and nmax <= 32");
     if (P.na != P.n) return set_err(ctx, MAVI_E_UNSUPPORTED, "the learned propagator needs full rings (np == nmax)");
+    if (P.nwalls > 0) return set_err(ctx, MAVI_E_UNSUPPORTED, "the learned propagator runs without a walls list");
     if (d->k < 1 || !(d->r_feat > 0.0) || !(d->target_scale > 0.0))
         return set_err(ctx, MAVI_E_BADARG, "learned descriptor: k >= 1, r_feat > 0, target_scale > 0");
     if (P.cl < d->r_feat || P.ch < d->r_feat)

@@ -15,6 +15,14 @@ from ._lib import (
 from .configs import (
     GPU,
     ChunksCfg,
+    CircleCfg,
+    Line2D,
+    LinesCfg,
+    ManyGeometries,
+    ManyWalls,
+    PotentialWalls,
+    SlipperyWalls,
+    get_bounding_box,
     HarmTruncCfg,
     IntCfg,
     LenJonesCfg,
@@ -48,5 +56,7 @@ __all__ = [
     "calc_forces", "get_step_function", "kinetic_energy", "newton_step", "potential_energy",
     "run_system", "update_chunks", "SecondLawState", "System", "TimeInfo", "Context",
     "SzaboCfg", "RunTumbleCfg", "SelfPropelledState", "szabo_step", "rtp_step",
+    "CircleCfg", "Line2D", "LinesCfg", "ManyGeometries", "ManyWalls", "PotentialWalls", "SlipperyWalls",
+    "get_bounding_box",
     "MaviError", "CellOverflowError", "OutOfBoxError", "UnsupportedOnGPU",
 ]

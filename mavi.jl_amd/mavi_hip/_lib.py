@@ -28,6 +28,9 @@ MAVI_POT_HARMTRUNC = 1
 MAVI_POT_RINGS = 2
 MAVI_POT_SZABO = 3
 MAVI_POT_RTP = 4
+(MAVI_WK_RIGID_RECT, MAVI_WK_PERIODIC_RECT, MAVI_WK_RIGID_CIRCLE, MAVI_WK_SLIPPERY_LINES, MAVI_WK_SLIPPERY_CIRCLE,
+ MAVI_WK_POTENTIAL_CIRCLE, MAVI_WK_POTENTIAL_LINES) = range(7)
+MAVI_WMODE = {"outside": 0, "inside": 1, "repulsion": 2}
 
 MAVI_PROF_BIN = 0
 MAVI_PROF_FORCE = 1
@@ -65,6 +68,18 @@ class UnsupportedOnGPU(MaviError):
     own CPU methods for it (custom DynamicCfg, Szabo/RTP, other geometries)."""
 
 
+class MaviWall(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("mode", C.c_int32),
+        ("pot_kind", C.c_int32),
+        ("nlines", C.c_int32),
+        ("geom", C.c_double * 4),
+        ("pot", C.c_double * 4),
+        ("lines", C.POINTER(C.c_double)),
+    ]
+
+
 class SysDesc(C.Structure):
     _fields_ = [
         ("n", C.c_int64),
@@ -89,6 +104,9 @@ class SysDesc(C.Structure):
         ("interaction", C.POINTER(C.c_double)),
         ("seed", C.c_uint64),
         ("active", C.c_double * 4),
+        ("nwalls", C.c_int32),
+        ("reserved2", C.c_int32),
+        ("walls", C.POINTER(MaviWall)),
     ]
 
 

@@ -1,9 +1,10 @@
 """Mirror of Mavi.Configs for the GPU path (src/configs.jl).
 
 Names, fields and argument meanings follow the reference; only the types the
-GPU path runs are provided: rectangles with RigidWalls or PeriodicWalls,
-LenJonesCfg, HarmTruncCfg, SzaboCfg and RunTumbleCfg, IntCfg/ChunksCfg and
-the device modes.
+GPU path runs are provided: rectangles, circles, lines and lists of them
+(ManyGeometries) with RigidWalls, PeriodicWalls, SlipperyWalls,
+PotentialWalls and ManyWalls; LenJonesCfg, HarmTruncCfg, SzaboCfg and
+RunTumbleCfg; IntCfg/ChunksCfg and the device modes.
 """
 from __future__ import annotations
 
@@ -30,7 +31,67 @@ class RectangleCfg:
         return (self.length, self.height)
 
 
-# --- walls (src/configs.jl:238-239) ---------------------------------------
+@dataclass
+class CircleCfg:  # src/configs.jl:145-152
+    radius: float
+    center: tuple = (0.0, 0.0)
+
+    def __post_init__(self):
+        self.radius = float(self.radius)
+        self.center = (float(self.center[0]), float(self.center[1]))
+
+
+@dataclass
+class Line2D:  # src/configs.jl:92-117
+    p1: tuple
+    p2: tuple
+
+    def __post_init__(self):
+        self.p1 = (float(self.p1[0]), float(self.p1[1]))
+        self.p2 = (float(self.p2[0]), float(self.p2[1]))
+
+
+@dataclass
+class LinesCfg:  # src/configs.jl:137-143
+    lines: list
+    bbox: RectangleCfg | None = None
+
+    def __post_init__(self):
+        self.lines = [ln if isinstance(ln, Line2D) else Line2D(*ln) for ln in self.lines]
+
+
+@dataclass
+class ManyGeometries:  # src/configs.jl:30-32
+    list: tuple
+
+
+def get_bounding_box(g) -> RectangleCfg:
+    """src/configs.jl:186-225."""
+    if isinstance(g, RectangleCfg):
+        return g
+    if isinstance(g, CircleCfg):
+        r = g.radius
+        return RectangleCfg(length=2 * r, height=2 * r, bottom_left=(g.center[0] - r, g.center[1] - r))
+    if isinstance(g, LinesCfg):
+        if g.bbox is not None:
+            return g.bbox
+        xs = [v for ln in g.lines for v in (ln.p1[0], ln.p2[0])]
+        ys = [v for ln in g.lines for v in (ln.p1[1], ln.p2[1])]
+        return RectangleCfg(length=max(xs) - min(xs), height=max(ys) - min(ys), bottom_left=(min(xs), min(ys)))
+    if isinstance(g, ManyGeometries):
+        box = get_bounding_box(g.list[0])
+        for h in g.list[1:]:
+            b = get_bounding_box(h)  # RectangleCfg + RectangleCfg, src/configs.jl:70-79
+            max_x = max(box.bottom_left[0] + box.length, b.bottom_left[0] + b.length)
+            max_y = max(box.bottom_left[1] + box.height, b.bottom_left[1] + b.height)
+            min_x = min(box.bottom_left[0], b.bottom_left[0])
+            min_y = min(box.bottom_left[1], b.bottom_left[1])
+            box = RectangleCfg(length=max_x - min_x, height=max_y - min_y, bottom_left=(min_x, min_y))
+        return box
+    raise TypeError(type(g))
+
+
+# --- walls (src/configs.jl:235-283) ---------------------------------------
 class WallType:
     pass
 
@@ -45,10 +106,49 @@ class PeriodicWalls(WallType):
         return "PeriodicWalls()"
 
 
+class SlipperyWalls(WallType):
+    def __repr__(self):
+        return "SlipperyWalls()"
+
+
+class PotentialWalls(WallType):
+    """A wall pushing with a PotentialCfg (LenJonesCfg or HarmTruncCfg);
+    mode "outside", "inside" or "repulsion" (src/configs.jl:252-283)."""
+
+    def __init__(self, potential, mode: str = "repulsion"):
+        if mode not in ("outside", "inside", "repulsion"):
+            raise ValueError(mode)
+        self.potential = potential
+        self.mode = mode
+
+    def __repr__(self):
+        return f"PotentialWalls({self.potential!r}, {self.mode!r})"
+
+
 @dataclass
-class SpaceCfg:  # src/configs.jl:285-288
+class ManyWalls(WallType):  # src/configs.jl:242-244
+    list: tuple
+
+
+@dataclass
+class SpaceCfg:  # src/configs.jl:285-301
     wall_type: WallType
-    geometry_cfg: RectangleCfg
+    geometry_cfg: object
+
+    @classmethod
+    def from_list(cls, pairs):
+        """SpaceCfg(spaces_cfg_list): ManyWalls / ManyGeometries of (wall, geometry) pairs."""
+        walls, geoms = zip(*pairs)
+        return cls(wall_type=ManyWalls(tuple(walls)), geometry_cfg=ManyGeometries(tuple(geoms)))
+
+    def pairs(self):
+        if isinstance(self.wall_type, ManyWalls):
+            return list(zip(self.wall_type.list, self.geometry_cfg.list))
+        return [(self.wall_type, self.geometry_cfg)]
+
+    def main(self):
+        """get_main_wall / get_main_geometry, src/configs.jl:303-311."""
+        return self.pairs()[0]
 
 
 # --- dynamics -------------------------------------------------------------

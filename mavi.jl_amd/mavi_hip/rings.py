@@ -19,8 +19,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from .configs import GPU, PeriodicWalls, RectangleCfg, SpaceCfg, chunk_capacity
-from .systems import Chunks, TimeInfo, base_desc, check_inside, wall_kind
+from .configs import GPU, PeriodicWalls, RectangleCfg, SpaceCfg, chunk_capacity, get_bounding_box
+from .systems import Chunks, TimeInfo, attach_walls, base_desc, check_inside, wall_kind, wall_records
 
 
 @dataclass
@@ -117,9 +117,9 @@ def describe(state: RingsState, space_cfg: SpaceCfg, dynamic_cfg: RingsCfg, int_
     the call that reads it."""
     if dynamic_cfg.typed != (state.types is not None):
         raise ValueError("DynamicCfg types and state.types disagree")
-    if not isinstance(space_cfg.wall_type, PeriodicWalls) or not isinstance(int_cfg.device, GPU):
+    if not isinstance(space_cfg.main()[0], PeriodicWalls) or not isinstance(int_cfg.device, GPU):
         raise _lib.UnsupportedOnGPU(_lib.MAVI_E_UNSUPPORTED,
-                                    "rings run on the GPU with PeriodicWalls rectangles")
+                                    "rings run on the GPU with a PeriodicWalls rectangle as the main wall")
     ids = state.particle_ids()
     ok, out_ids = check_inside(state.pos, space_cfg.geometry_cfg, ids)
     if not ok:
@@ -127,8 +127,10 @@ def describe(state: RingsState, space_cfg: SpaceCfg, dynamic_cfg: RingsCfg, int_
     r = min(dynamic_cfg.particle_radii())  # minimum(particle_radius(dynamic_cfg)), src/systems.jl:26
     chunks = None
     if int_cfg.chunks_cfg is not None:
-        chunks = Chunks(int_cfg.chunks_cfg.num_cols, int_cfg.chunks_cfg.num_rows, space_cfg.geometry_cfg, r)
+        chunks = Chunks(int_cfg.chunks_cfg.num_cols, int_cfg.chunks_cfg.num_rows,
+                        get_bounding_box(space_cfg.geometry_cfg), r)
     d = base_desc(len(state.pos), space_cfg, dynamic_cfg, int_cfg, chunks, r)
+    walls_keep = attach_walls(d, wall_records(space_cfg))
     d.potential = _lib.MAVI_POT_RINGS
     d.nmax, d.nrings, d.ntypes = state.nmax, state.num_rings, dynamic_cfg.num_types
     np_types = np.atleast_1d(np.asarray(state.num_particles, dtype=np.int32))
@@ -137,6 +139,7 @@ def describe(state: RingsState, space_cfg: SpaceCfg, dynamic_cfg: RingsCfg, int_
         np.ascontiguousarray(dynamic_cfg.type_table()),
         np.ascontiguousarray(dynamic_cfg.interaction_table()),
         None if state.types is None else np.ascontiguousarray(state.types, dtype=np.int32),
+        walls_keep,
     )
     import ctypes as C
     d.type_np = tables[0].ctypes.data_as(C.POINTER(C.c_int32))

@@ -14,16 +14,22 @@ import numpy as np
 from . import _lib
 from .configs import (
     GPU,
+    CircleCfg,
     HarmTruncCfg,
     LenJonesCfg,
+    LinesCfg,
+    ManyWalls,
     PeriodicWalls,
+    PotentialWalls,
     RectangleCfg,
     RigidWalls,
     RunTumbleCfg,
+    SlipperyWalls,
     SpaceCfg,
     SzaboCfg,
     as_points,
     chunk_capacity,
+    get_bounding_box,
     particle_radius,
 )
 
@@ -74,8 +80,18 @@ class Chunks:
         self.num_particles_in_chunk = None
 
 
-def check_inside(pos: np.ndarray, geometry_cfg: RectangleCfg, ids=None):
-    """src/space_checks.jl:9-20, 53-60."""
+def check_inside(pos: np.ndarray, geometry_cfg, ids=None):
+    """src/space_checks.jl:9-60: rectangles by their box, circles by the
+    distance from the origin (as the reference writes it), other geometries
+    unchecked."""
+    if isinstance(geometry_cfg, CircleCfg):
+        p = pos if ids is None else pos[ids]
+        out = np.nonzero(np.sum(p * p, axis=1) > geometry_cfg.radius ** 2)[0]
+        if ids is not None:
+            out = np.asarray(ids)[out]
+        return len(out) == 0, out
+    if not isinstance(geometry_cfg, RectangleCfg):
+        return True, np.zeros(0, dtype=int)
     bl = np.asarray(geometry_cfg.bottom_left)
     tr = bl + np.asarray((geometry_cfg.length, geometry_cfg.height))
     p = pos if ids is None else pos[ids]
@@ -99,17 +115,79 @@ def gpu_kind(dynamic_cfg):
 
 
 def wall_kind(space_cfg: SpaceCfg):
-    if not isinstance(space_cfg.geometry_cfg, RectangleCfg):
-        return None
-    if isinstance(space_cfg.wall_type, RigidWalls):
-        return _lib.MAVI_WALL_RIGID
-    if isinstance(space_cfg.wall_type, PeriodicWalls):
+    """Main wall as the descriptor's `wall` (stencil and calc_diff)."""
+    w, g = space_cfg.main()
+    if isinstance(w, PeriodicWalls) and isinstance(g, RectangleCfg):
         return _lib.MAVI_WALL_PERIODIC
+    if isinstance(w, (RigidWalls, SlipperyWalls, PotentialWalls)):
+        return _lib.MAVI_WALL_RIGID
     return None
 
 
+def _pot_record(potential):
+    if isinstance(potential, LenJonesCfg):
+        return _lib.MAVI_POT_LJ, (potential.sigma, potential.epsilon, 0.0, 0.0)
+    if isinstance(potential, HarmTruncCfg):
+        return _lib.MAVI_POT_HARMTRUNC, (potential.k_rep, potential.k_atr, potential.dist_eq, potential.dist_max)
+    raise _lib.UnsupportedOnGPU(_lib.MAVI_E_UNSUPPORTED, "potential walls take a LenJonesCfg or HarmTruncCfg")
+
+
+def wall_records(space_cfg: SpaceCfg):
+    """The walls list of a SpaceCfg as plain records {kind, mode, pot_kind,
+    geom, pot, lines} (the mavi_wall fields), or None for a single rigid or
+    periodic rectangle (the descriptor's `wall`)."""
+    if not isinstance(space_cfg.wall_type, ManyWalls) and isinstance(space_cfg.geometry_cfg, RectangleCfg) and \
+            isinstance(space_cfg.wall_type, (RigidWalls, PeriodicWalls)):
+        return None
+    out = []
+    for w, g in space_cfg.pairs():
+        rec = dict(mode=0, pot_kind=0, geom=(0.0, 0.0, 0.0, 0.0), pot=(0.0, 0.0, 0.0, 0.0), lines=None)
+        if isinstance(g, RectangleCfg) and isinstance(w, (RigidWalls, PeriodicWalls)):
+            rec["kind"] = _lib.MAVI_WK_RIGID_RECT if isinstance(w, RigidWalls) else _lib.MAVI_WK_PERIODIC_RECT
+            rec["geom"] = (g.bottom_left[0], g.bottom_left[1], g.length, g.height)
+        elif isinstance(g, CircleCfg) and isinstance(w, (RigidWalls, SlipperyWalls, PotentialWalls)):
+            rec["kind"] = {RigidWalls: _lib.MAVI_WK_RIGID_CIRCLE, SlipperyWalls: _lib.MAVI_WK_SLIPPERY_CIRCLE,
+                           PotentialWalls: _lib.MAVI_WK_POTENTIAL_CIRCLE}[type(w)]
+            rec["geom"] = (g.center[0], g.center[1], g.radius, 0.0)
+        elif isinstance(g, LinesCfg) and isinstance(w, (SlipperyWalls, PotentialWalls)):
+            rec["kind"] = _lib.MAVI_WK_SLIPPERY_LINES if isinstance(w, SlipperyWalls) else _lib.MAVI_WK_POTENTIAL_LINES
+            rec["lines"] = np.array([[*ln.p1, *ln.p2] for ln in g.lines], dtype=np.float64)
+        else:
+            raise _lib.UnsupportedOnGPU(_lib.MAVI_E_UNSUPPORTED,
+                                        f"{type(w).__name__} with {type(g).__name__} is not on the GPU path")
+        if isinstance(w, PotentialWalls):
+            rec["mode"] = _lib.MAVI_WMODE[w.mode]
+            rec["pot_kind"], rec["pot"] = _pot_record(w.potential)
+        out.append(rec)
+    return out
+
+
+def attach_walls(d: _lib.SysDesc, records):
+    """Point the descriptor at a ctypes copy of the records; returns what
+    must stay alive until mavi_sys_create returns."""
+    import ctypes as C
+
+    if not records:
+        return ()
+    arr = (_lib.MaviWall * len(records))()
+    keep = [arr]
+    for k, r in enumerate(records):
+        arr[k].kind, arr[k].mode, arr[k].pot_kind = r["kind"], r["mode"], r["pot_kind"]
+        for j in range(4):
+            arr[k].geom[j] = r["geom"][j]
+            arr[k].pot[j] = r["pot"][j]
+        if r["lines"] is not None:
+            ln = np.ascontiguousarray(r["lines"])
+            arr[k].nlines = len(ln)
+            arr[k].lines = ln.ctypes.data_as(C.POINTER(C.c_double))
+            keep.append(ln)
+    d.nwalls = len(records)
+    d.walls = C.cast(arr, C.POINTER(_lib.MaviWall))
+    return tuple(keep)
+
+
 def base_desc(n, space_cfg, dynamic_cfg, int_cfg, chunks: Chunks | None, radius) -> _lib.SysDesc:
-    g = space_cfg.geometry_cfg
+    g = get_bounding_box(space_cfg.geometry_cfg)  # the chunk space (get_chunks, src/systems.jl:19)
     d = _lib.SysDesc()
     d.n = int(n)
     d.bottom_left[0], d.bottom_left[1] = g.bottom_left
@@ -145,8 +223,10 @@ def describe(state, space_cfg: SpaceCfg, dynamic_cfg, int_cfg):
     r = particle_radius(dynamic_cfg)
     chunks = None
     if int_cfg.chunks_cfg is not None:
-        chunks = Chunks(int_cfg.chunks_cfg.num_cols, int_cfg.chunks_cfg.num_rows, space_cfg.geometry_cfg, r)
+        chunks = Chunks(int_cfg.chunks_cfg.num_cols, int_cfg.chunks_cfg.num_rows,
+                        get_bounding_box(space_cfg.geometry_cfg), r)
     d = base_desc(len(state.pos), space_cfg, dynamic_cfg, int_cfg, chunks, r)
+    d._walls_keep = attach_walls(d, wall_records(space_cfg))
     d.potential = kind
     c = dynamic_cfg
     if kind == _lib.MAVI_POT_LJ:

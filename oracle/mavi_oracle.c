@@ -39,6 +39,11 @@
  *                              polarisation turning towards the force,
  *                              var(dtheta) = 2 Dr dt, tumbles at u < rate dt
  *                              to 2 pi v exactly, periodic wrap.
+ *   walls list                 rigid circle: speed kept, normal component
+ *                              reversed (centred at the origin); slippery
+ *                              circle / line / corner: ends at distance r;
+ *                              potential walls: minus the finite-difference
+ *                              gradient of the wall energy; list order.
  *   rings                      regular n-gon area n l^2/(4 tan(pi/n)), spring
  *                              force zero at rest length, area force zero at
  *                              A0 and equal to -dE/dx by finite differences,
@@ -458,6 +463,189 @@ static void walls_periodic(const or_sys *s, double *pos, int64_t i) {
     }
 }
 
+/* ------------------------------------------------------------------------ */
+/* Walls list (ManyWalls / other geometries)                                */
+/* ------------------------------------------------------------------------ */
+
+/* Line2D(p1, p2), src/configs.jl:99-117: unit normal (-dr.y, dr.x)/norm,
+ * tangent dr/norm, length norm. */
+typedef struct {
+    double p1[2], p2[2], n[2], t[2], len;
+} or_line;
+
+static or_line make_line(const double *q) {
+    or_line L;
+    L.p1[0] = q[0]; L.p1[1] = q[1];
+    L.p2[0] = q[2]; L.p2[1] = q[3];
+    double dr0 = q[2] - q[0], dr1 = q[3] - q[1];
+    double norm = sqrt(dr0 * dr0 + dr1 * dr1); /* sqrt(sum(abs2, dr)) */
+    L.n[0] = -dr1 / norm; L.n[1] = dr0 / norm;
+    L.t[0] = dr0 / norm; L.t[1] = dr1 / norm;
+    L.len = norm;
+    return L;
+}
+
+/* signed_pos(point, ::Line2D), src/configs.jl:119-134 (inside flag 1). */
+static double signed_pos_line(const or_line *L, const double *p, double *dr) {
+    dr[0] = p[0] - L->p1[0];
+    dr[1] = p[1] - L->p1[1];
+    double delta_t = dr[0] * L->t[0] + dr[1] * L->t[1];
+    if (delta_t > 0.0) {
+        if (delta_t < L->len) {
+            double b0 = L->p1[0] + L->t[0] * delta_t, b1 = L->p1[1] + L->t[1] * delta_t;
+            dr[0] = p[0] - b0;
+            dr[1] = p[1] - b1;
+        } else {
+            dr[0] = p[0] - L->p2[0];
+            dr[1] = p[1] - L->p2[1];
+        }
+    }
+    return sqrt(dr[0] * dr[0] + dr[1] * dr[1]);
+}
+
+/* signed_pos(point, ::CircleCfg), src/configs.jl:154-162: returns |dist - R|,
+ * flag = sign(dist - R). */
+static double signed_pos_circle(const double *g, const double *p, double *dr, double *flag) {
+    double d0 = p[0] - g[0], d1 = p[1] - g[1];
+    double dist = pow(d0 * d0 + d1 * d1, 0.5); /* sum(dr.^2)^.5 */
+    double h0 = d0 / dist, h1 = d1 / dist;
+    dr[0] = d0 - h0 * g[2];
+    dr[1] = d1 - h1 * g[2];
+    double sd = dist - g[2];
+    *flag = jl_sign(sd);
+    return fabs(sd);
+}
+
+/* process_dist, src/configs.jl:259-261. */
+static double process_dist(int mode, double dist, double flag) {
+    if (mode == OR_WMODE_OUTSIDE) return flag * dist;
+    if (mode == OR_WMODE_INSIDE) return -flag * dist;
+    return dist;
+}
+
+/* potential_force(dr, dist, potential) of a wall, src/configs.jl:354-368, 389-397. */
+static void wall_potential_force(const or_wall *w, const double *dr, double dist, double *f) {
+    double q;
+    if (w->pot_kind == OR_KIND_LJ) {
+        double sigma = w->pot[0], eps = w->pot[1];
+        double fmod = 4.0 * eps * (12.0 * pow(sigma, 12.0) / pow(dist, 13.0) - 6.0 * pow(sigma, 6.0) / pow(dist, 7.0));
+        q = fmod / dist;
+    } else {
+        double k_rep = w->pot[0], k_atr = w->pot[1], dist_eq = w->pot[2], dist_max = w->pot[3];
+        if (dist > dist_max) return;
+        double fmod = (dist < dist_eq) ? -k_rep * (dist / dist_eq - 1.0) : -k_atr * (dist / dist_eq - 1.0);
+        q = fmod / dist;
+    }
+    f[0] += q * dr[0];
+    f[1] += q * dr[1];
+}
+
+void or_wall_forces(const or_sys *s, const double *pos, int64_t i, double *f) {
+    const double *p = pos + 2 * i;
+    for (int32_t k = 0; k < s->nwalls; k++) {
+        const or_wall *w = &s->walls[k];
+        double dr[2], flag = 1.0;
+        if (w->kind == OR_WK_POTENTIAL_CIRCLE) {
+            double dist = signed_pos_circle(w->geom, p, dr, &flag);
+            wall_potential_force(w, dr, process_dist(w->mode, dist, flag), f);
+        } else if (w->kind == OR_WK_POTENTIAL_LINES) {
+            for (int32_t l = 0; l < w->nlines; l++) {
+                or_line L = make_line(w->lines + 4 * l);
+                double dist = signed_pos_line(&L, p, dr);
+                wall_potential_force(w, dr, process_dist(w->mode, dist, 1.0), f);
+            }
+        }
+    }
+}
+
+void or_walls_apply(const or_sys *s, double *pos, double *vel, int64_t i, double radius) {
+    double *p = pos + 2 * i;
+    for (int32_t k = 0; k < s->nwalls; k++) {
+        const or_wall *w = &s->walls[k];
+        const double *g = w->geom;
+        if (w->kind == OR_WK_RIGID_RECT) {
+            int out[2];
+            for (int d = 0; d < 2; d++) {
+                double rel = p[d] - g[d];
+                out[d] = ((rel + radius) > g[2 + d]) || ((rel - radius) < 0.0);
+            }
+            if (out[0] || out[1])
+                for (int d = 0; d < 2; d++) vel[2 * i + d] = vel[2 * i + d] * (double)(-2 * out[d] + 1);
+        } else if (w->kind == OR_WK_PERIODIC_RECT) {
+            double diff[2], half[2];
+            int out[2];
+            for (int d = 0; d < 2; d++) {
+                half[d] = g[2 + d] / 2.0;
+                diff[d] = p[d] - (g[d] + g[2 + d] / 2.0);
+                out[d] = fabs(diff[d]) > half[d];
+            }
+            if (out[0] || out[1])
+                for (int d = 0; d < 2; d++) p[d] = p[d] - jl_sign(diff[d]) * (half[d] * 2.0) * (double)out[d];
+        } else if (w->kind == OR_WK_RIGID_CIRCLE) {
+            /* src/integration.jl:287-307 (the reflection uses pos.x*pos.y as written) */
+            double m = g[2] - radius;
+            double max_r2 = m * m;
+            double dx = p[0] - g[0], dy = p[1] - g[1];
+            double dx2 = dx * dx, dy2 = dy * dy;
+            double r2 = dx2 + dy2;
+            if (r2 <= max_r2) continue;
+            double vx = vel[2 * i], vy = vel[2 * i + 1];
+            vel[2 * i] = (vx * (dy2 - dx2) - 2.0 * vy * p[0] * p[1]) / r2;
+            vel[2 * i + 1] = (-vy * (dy2 - dx2) - 2.0 * vx * p[0] * p[1]) / r2;
+        } else if (w->kind == OR_WK_SLIPPERY_LINES) {
+            /* src/integration.jl:327-375: every line's correction is computed
+             * from the position at the start of this wall and added */
+            const double pi0[2] = {p[0], p[1]};
+            for (int32_t l = 0; l < w->nlines; l++) {
+                or_line L = make_line(w->lines + 4 * l);
+                double dr0 = pi0[0] - L.p1[0], dr1 = pi0[1] - L.p1[1];
+                double delta_s = dr0 * L.n[0] + dr1 * L.n[1];
+                if (fabs(delta_s) > radius) continue;
+                double delta_t = dr0 * L.t[0] + dr1 * L.t[1];
+                int is_corner = 0;
+                const double *corner_p = NULL;
+                if (delta_t > 0.0) {
+                    if (delta_t > L.len) {
+                        if (delta_t > (L.len + radius)) continue;
+                        is_corner = 1;
+                        corner_p = L.p2;
+                    }
+                } else if (delta_t > -radius) {
+                    is_corner = 1;
+                    corner_p = L.p1;
+                } else {
+                    continue;
+                }
+                if (is_corner) {
+                    double c0 = pi0[0] - corner_p[0], c1 = pi0[1] - corner_p[1];
+                    double norm = sqrt(c0 * c0 + c1 * c1);
+                    if (norm > radius) continue;
+                    double alpha = radius / norm - 1.0;
+                    p[0] = p[0] + alpha * c0;
+                    p[1] = p[1] + alpha * c1;
+                } else {
+                    double sgn = jl_sign(delta_s);
+                    double alpha = sgn * (radius - sgn * delta_s);
+                    p[0] = p[0] + alpha * L.n[0];
+                    p[1] = p[1] + alpha * L.n[1];
+                }
+            }
+        } else if (w->kind == OR_WK_SLIPPERY_CIRCLE) {
+            /* src/integration.jl:377-396; dr through the space's calc_diff */
+            double c[2] = {g[0], g[1]}, dr[2];
+            double m = g[2] + radius;
+            double max_r_2 = m * m;
+            or_calc_diff(s, p, c, dr);
+            double dr_2 = dr[0] * dr[0] + dr[1] * dr[1];
+            if (dr_2 > max_r_2) continue;
+            double dr_norm = sqrt(dr_2);
+            double kk = g[2] + radius - dr_norm;
+            p[0] = p[0] + kk * dr[0] / dr_norm;
+            p[1] = p[1] + kk * dr[1] / dr_norm;
+        }
+    }
+}
+
 int or_newton_steps(const or_sys *s, double *pos, double *vel, double *forces, int64_t nsteps,
                     double *ke, double *pe, int nthreads) {
     if (s->kind == OR_KIND_RINGS) return OR_E_BADARG;
@@ -479,7 +667,8 @@ int or_newton_steps(const or_sys *s, double *pos, double *vel, double *forces, i
             if (err) break;
         }
         or_calc_forces_threaded(s, pos, cp, counts, forces, nthreads); /* clean + calc_forces! */
-        /* calc_walls_forces!: no-op for Rigid/Periodic walls (src/integration.jl:228) */
+        /* calc_walls_forces! (src/integration.jl:228-263): potential walls only */
+        for (int64_t i = 0; i < n && s->nwalls > 0; i++) or_wall_forces(s, pos, i, forces + 2 * i);
         /* update_verlet!, src/integration.jl:415-431 */
         memcpy(old, forces, sizeof(double) * 2 * n);
         for (int64_t k = 0; k < 2 * n; k++) pos[k] = pos[k] + (vel[k] * dt + forces[k] * term);
@@ -487,7 +676,8 @@ int or_newton_steps(const or_sys *s, double *pos, double *vel, double *forces, i
         for (int64_t k = 0; k < 2 * n; k++) vel[k] = vel[k] + (dt / 2.0) * (forces[k] + old[k]);
         /* walls! */
         for (int64_t i = 0; i < n; i++) {
-            if (s->periodic) walls_periodic(s, pos, i);
+            if (s->nwalls > 0) or_walls_apply(s, pos, vel, i, s->radius);
+            else if (s->periodic) walls_periodic(s, pos, i);
             else walls_rigid(s, pos, vel, i);
         }
         if (ke) ke[step] = or_kinetic_energy(n, vel);
@@ -506,7 +696,10 @@ int or_newton_steps(const or_sys *s, double *pos, double *vel, double *forces, i
 int or_sp_steps(const or_sys *s, double *pos, double *ang, int64_t nsteps, const double *noise, double *forces,
                 int nthreads) {
     if (s->kind != OR_KIND_SZABO && s->kind != OR_KIND_RTP) return OR_E_BADARG;
-    if (!s->periodic) return OR_E_BADARG; /* walls! rigid reads state.vel, absent here */
+    /* walls! with rigid walls reads state.vel, which a SelfPropelledState lacks */
+    if (s->nwalls == 0 && !s->periodic) return OR_E_BADARG;
+    for (int32_t k = 0; k < s->nwalls; k++)
+        if (s->walls[k].kind == OR_WK_RIGID_RECT || s->walls[k].kind == OR_WK_RIGID_CIRCLE) return OR_E_BADARG;
     const int64_t n = s->n;
     int64_t ncell = (int64_t)s->rows * s->cols;
     int64_t *cp = NULL, *counts = NULL;
@@ -525,6 +718,7 @@ int or_sp_steps(const or_sys *s, double *pos, double *ang, int64_t nsteps, const
             if (err) break;
         }
         or_calc_forces_threaded(s, pos, cp, counts, f, nthreads); /* clean_forces! + calc_forces! */
+        for (int64_t i = 0; i < n && s->nwalls > 0; i++) or_wall_forces(s, pos, i, f + 2 * i);
         for (int64_t i = 0; i < n; i++) {
             const double theta = ang[i];
             const double pol[2] = {cos(theta), sin(theta)};
@@ -554,7 +748,10 @@ int or_sp_steps(const or_sys *s, double *pos, double *ang, int64_t nsteps, const
                     ang[i] = two_pi * noise[(step * n + i) * 2 + 1];
             }
         }
-        for (int64_t i = 0; i < n; i++) walls_periodic(s, pos, i); /* walls! */
+        for (int64_t i = 0; i < n; i++) { /* walls! */
+            if (s->nwalls > 0) or_walls_apply(s, pos, NULL, i, s->radius);
+            else walls_periodic(s, pos, i);
+        }
     }
     if (!forces) free(f);
     free(cp);
@@ -690,6 +887,13 @@ int or_rings_steps(const or_sys *s, double *pos, double *pol, int64_t nsteps, co
         or_rings_continuous_pos(s, pos, cpos);   /* update_chunks_all! then update_continuos_pos! */
         err = or_rings_forces(s, pos, cpos, forces, areas); /* cleaning! + forces! (bins inside) */
         if (err) break;
+        if (s->nwalls > 0) { /* calc_walls_forces! over get_particles_ids */
+            int64_t nids = or_particle_ids(s, NULL);
+            int64_t *ids = (int64_t *)malloc(sizeof(int64_t) * nids);
+            or_particle_ids(s, ids);
+            for (int64_t k = 0; k < nids; k++) or_wall_forces(s, pos, ids[k] - 1, forces + 2 * (ids[k] - 1));
+            free(ids);
+        }
         /* update!, src/rings/integration.jl:300-351 */
         for (int32_t r = 1; r <= s->nrings; r++) {
             int32_t np = ring_np(s, r);
@@ -721,12 +925,20 @@ int or_rings_steps(const or_sys *s, double *pos, double *pol, int64_t nsteps, co
             double d_theta = 1.0 / relax * asin(cr) * dt + sqrt(2.0 * drot * dt) * z;
             pol[r - 1] += d_theta;
         }
-        /* walls!: periodic wrap over get_particles_ids */
-        if (s->periodic) {
+        /* walls! over get_particles_ids: the periodic wrap, or the walls list
+         * with each particle's radius (get_particle_radius, src/rings/rings.jl:44-53) */
+        if (s->periodic || s->nwalls > 0) {
             int64_t nids = or_particle_ids(s, NULL);
             int64_t *ids = (int64_t *)malloc(sizeof(int64_t) * nids);
             or_particle_ids(s, ids);
-            for (int64_t k = 0; k < nids; k++) walls_periodic(s, pos, ids[k] - 1);
+            for (int64_t k = 0; k < nids; k++) {
+                if (s->nwalls > 0) {
+                    const int32_t r = ring_of(s, ids[k]);
+                    or_walls_apply(s, pos, NULL, ids[k] - 1, inter_of(s, r, r)[2] / 2.0);
+                } else {
+                    walls_periodic(s, pos, ids[k] - 1);
+                }
+            }
             free(ids);
         }
     }

@@ -26,6 +26,27 @@ extern "C" {
 enum { OR_KIND_LJ = 0, OR_KIND_HARMTRUNC = 1, OR_KIND_RINGS = 2, OR_KIND_SZABO = 3, OR_KIND_RTP = 4 };
 enum { OR_OK = 0, OR_E_CELL_OVERFLOW = 1, OR_E_OUT_OF_BOX = 2, OR_E_BADARG = 5 };
 
+/* One (wall, geometry) pair of a SpaceCfg, applied in list order (ManyWalls,
+ * src/configs.jl:242-244, 289-301; src/integration.jl:257-263, 398-403). */
+enum {
+    OR_WK_RIGID_RECT = 0,        /* walls! src/integration.jl:271-285 */
+    OR_WK_PERIODIC_RECT = 1,     /* walls! :309-324 */
+    OR_WK_RIGID_CIRCLE = 2,      /* walls! :287-307 */
+    OR_WK_SLIPPERY_LINES = 3,    /* walls! :327-375 */
+    OR_WK_SLIPPERY_CIRCLE = 4,   /* walls! :377-396 */
+    OR_WK_POTENTIAL_CIRCLE = 5,  /* calc_walls_forces! :230-240 */
+    OR_WK_POTENTIAL_LINES = 6    /* calc_walls_forces! :242-255 */
+};
+enum { OR_WMODE_OUTSIDE = 0, OR_WMODE_INSIDE = 1, OR_WMODE_REPULSION = 2 }; /* src/configs.jl:255-261 */
+typedef struct {
+    int32_t kind, mode;   /* OR_WK_*, OR_WMODE_* (potential walls) */
+    int32_t pot_kind;     /* potential walls: OR_KIND_LJ or OR_KIND_HARMTRUNC */
+    int32_t nlines;       /* LinesCfg: number of segments */
+    double geom[4];       /* rectangle: bl_x, bl_y, length, height; circle: center_x, center_y, radius */
+    double pot[4];        /* potential walls: LJ sigma, epsilon; HT k_rep, k_atr, dist_eq, dist_max */
+    const double *lines;  /* LinesCfg: nlines x (p1x, p1y, p2x, p2y) */
+} or_wall;
+
 /* One system.  Mirrors what Mavi's System holds for the hot path:
  *   space_cfg  (src/configs.jl:285-288) -> bl, size, periodic
  *   dynamic_cfg (src/configs.jl:341-397, src/rings/configs.jl:95-107) -> kind, p, ring_*
@@ -59,6 +80,10 @@ typedef struct {
     /* Self-propelled update (kind SZABO: vo, mobility, relax_time, rot_diff;
        kind RTP: vo, tumble_rate, -, -) */
     double ap[4];
+    /* Walls list (nwalls > 0): replaces `periodic`-selected rigid/periodic
+       rectangle walls; the first entry is the main wall (calc_diff, stencil) */
+    int32_t nwalls, pad2;
+    const or_wall *walls;
 } or_sys;
 
 /* Julia Base `div(a::Float64, b::Float64)` (RoundToZero): round((a - rem(a,b))/b). */
@@ -112,6 +137,13 @@ int or_calc_forces_threaded(const or_sys *s, const double *pos, const int64_t *c
  * nthreads > 1 uses the Threaded force loop. */
 int or_newton_steps(const or_sys *s, double *pos, double *vel, double *forces,
                     int64_t nsteps, double *ke, double *pe, int nthreads);
+
+/* calc_walls_forces! for particle i (0-based): adds the potential walls'
+ * forces to f[0..1], src/integration.jl:230-255. */
+void or_wall_forces(const or_sys *s, const double *pos, int64_t i, double *f);
+/* walls! for particle i (0-based) over the walls list, in list order; vel may
+ * be NULL when no rigid wall is listed; radius is get_particle_radius. */
+void or_walls_apply(const or_sys *s, double *pos, double *vel, int64_t i, double radius);
 
 /* kinetic_energy, src/quantities.jl:12-18. */
 double or_kinetic_energy(int64_t n, const double *vel);

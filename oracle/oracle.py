@@ -16,6 +16,9 @@ HERE = Path(__file__).resolve().parent
 LIB = HERE / "liboracle.so"
 
 KIND_LJ, KIND_HT, KIND_RINGS, KIND_SZABO, KIND_RTP = 0, 1, 2, 3, 4
+(WK_RIGID_RECT, WK_PERIODIC_RECT, WK_RIGID_CIRCLE, WK_SLIPPERY_LINES, WK_SLIPPERY_CIRCLE, WK_POTENTIAL_CIRCLE,
+ WK_POTENTIAL_LINES) = range(7)
+WMODE_OUTSIDE, WMODE_INSIDE, WMODE_REPULSION = 0, 1, 2
 
 
 def build() -> Path:
@@ -23,6 +26,18 @@ def build() -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"oracle build failed:\n{r.stdout}\n{r.stderr}")
     return LIB
+
+
+class OrWall(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("mode", C.c_int32),
+        ("pot_kind", C.c_int32),
+        ("nlines", C.c_int32),
+        ("geom", C.c_double * 4),
+        ("pot", C.c_double * 4),
+        ("lines", C.POINTER(C.c_double)),
+    ]
 
 
 class OrSys(C.Structure):
@@ -50,6 +65,9 @@ class OrSys(C.Structure):
         ("ring_par", C.POINTER(C.c_double)),
         ("inter", C.POINTER(C.c_double)),
         ("ap", C.c_double * 4),
+        ("nwalls", C.c_int32),
+        ("pad2", C.c_int32),
+        ("walls", C.POINTER(OrWall)),
     ]
 
 
@@ -77,6 +95,8 @@ _SIG = {
     "or_rings_forces": (C.c_int, [_S, _D, _D, _D, _D]),
     "or_rings_steps": (C.c_int, [_S, _D, _D, C.c_int64, _D, _D, _D, _D]),
     "or_sp_steps": (C.c_int, [_S, _D, _D, C.c_int64, _D, _D, C.c_int]),
+    "or_wall_forces": (None, [_S, _D, C.c_int64, _D]),
+    "or_walls_apply": (None, [_S, _D, _D, C.c_int64, C.c_double]),
 }
 
 _lib = None
@@ -116,7 +136,8 @@ class Sys:
     """One oracle system.  Positions are (N, 2) float64 arrays; ids 1-based."""
 
     def __init__(self, *, n, bl, size, periodic, kind, p, dt=0.0, cols=0, rows=0, nc=0, radius=0.0,
-                 nmax=0, nrings=0, types=None, np_type=None, ring_par=None, inter=None, ap=None):
+                 nmax=0, nrings=0, types=None, np_type=None, ring_par=None, inter=None, ap=None, walls=None):
+        """walls: list of dicts {kind, mode=0, pot_kind=0, geom=(4), pot=(4), lines=(k,4)} (WK_*)."""
         s = OrSys()
         s.n = n
         s.bl[0], s.bl[1] = bl
@@ -135,6 +156,24 @@ class Sys:
         for i, v in enumerate(ap or ()):
             s.ap[i] = v
         self._keep = []
+        if walls:
+            arr = (OrWall * len(walls))()
+            for k, w in enumerate(walls):
+                arr[k].kind = w["kind"]
+                arr[k].mode = w.get("mode", 0)
+                arr[k].pot_kind = w.get("pot_kind", 0)
+                for j, v in enumerate(w.get("geom", ())):
+                    arr[k].geom[j] = v
+                for j, v in enumerate(w.get("pot", ())):
+                    arr[k].pot[j] = v
+                if w.get("lines") is not None:
+                    ln = np.ascontiguousarray(w["lines"], dtype=np.float64).reshape(-1, 4)
+                    arr[k].nlines = len(ln)
+                    arr[k].lines = ln.ctypes.data_as(_D)
+                    self._keep.append(ln)
+            s.nwalls = len(walls)
+            s.walls = C.cast(arr, C.POINTER(OrWall))
+            self._keep.append(arr)
         if kind == KIND_RINGS:
             s.nmax, s.nrings = nmax, nrings
             np_type = np.ascontiguousarray(np.atleast_1d(np_type), dtype=np.int32)
@@ -268,6 +307,20 @@ class Sys:
         return pos, pol, f, areas, cms
 
 
+    # walls
+    def wall_forces(self, pos, i):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        f = np.zeros(2)
+        lib().or_wall_forces(self.ptr, _d(pos), i, _d(f))
+        return f
+
+    def walls_apply(self, pos, vel, i, radius):
+        """walls! on particle i (0-based); returns the new (pos, vel) arrays."""
+        pos = np.array(pos, dtype=np.float64, order="C")
+        vel = None if vel is None else np.array(vel, dtype=np.float64, order="C")
+        lib().or_walls_apply(self.ptr, _d(pos), _d(vel), i, radius)
+        return pos, vel
+
     # self-propelled (Szabo / run-and-tumble)
     def sp_steps(self, pos, ang, nsteps, noise=None, nthreads=1):
         pos = np.array(pos, dtype=np.float64, order="C")
@@ -297,9 +350,11 @@ def neighbors(rows, cols, periodic):
 def from_system(pos_n, space_cfg, dynamic_cfg, int_cfg, chunks=None, radius=None):
     """Oracle Sys mirroring a plain GPU System configuration."""
     from mavi_hip.configs import (HarmTruncCfg, LenJonesCfg, PeriodicWalls, RunTumbleCfg, SzaboCfg,
-                                  particle_radius)
+                                  get_bounding_box, particle_radius)
+    from mavi_hip.systems import wall_records
 
-    g = space_cfg.geometry_cfg
+    g = get_bounding_box(space_cfg.geometry_cfg)
+    walls = wall_records(space_cfg)  # mavi_wall kinds and modes are the oracle's
     ap = None
     if isinstance(dynamic_cfg, LenJonesCfg):
         kind, p = KIND_LJ, [dynamic_cfg.sigma, dynamic_cfg.epsilon, dynamic_cfg.cutoff or 0.0, 0.0]
@@ -321,16 +376,17 @@ def from_system(pos_n, space_cfg, dynamic_cfg, int_cfg, chunks=None, radius=None
         r = particle_radius(dynamic_cfg)
         nc = chunk_capacity(g.length / cols, g.height / rows, r)
     return Sys(n=pos_n, bl=g.bottom_left, size=(g.length, g.height),
-               periodic=isinstance(space_cfg.wall_type, PeriodicWalls), kind=kind, p=p, dt=int_cfg.dt,
+               periodic=isinstance(space_cfg.main()[0], PeriodicWalls), kind=kind, p=p, dt=int_cfg.dt,
                cols=cols, rows=rows, nc=nc, radius=radius if radius is not None else particle_radius(dynamic_cfg),
-               ap=ap)
+               ap=ap, walls=walls)
 
 
 def from_rings(state, space_cfg, cfg, int_cfg):
     """Oracle Sys mirroring a rings GPU configuration."""
-    from mavi_hip.configs import PeriodicWalls
+    from mavi_hip.configs import PeriodicWalls, get_bounding_box
+    from mavi_hip.systems import wall_records
 
-    g = space_cfg.geometry_cfg
+    g = get_bounding_box(space_cfg.geometry_cfg)
     cols = rows = nc = 0
     r = min(cfg.particle_radii())
     if int_cfg.chunks_cfg is not None:
@@ -338,7 +394,7 @@ def from_rings(state, space_cfg, cfg, int_cfg):
         nc = chunk_capacity(g.length / cols, g.height / rows, r)
     inter = cfg.interaction_table()
     return Sys(n=len(state.pos), bl=g.bottom_left, size=(g.length, g.height),
-               periodic=isinstance(space_cfg.wall_type, PeriodicWalls), kind=KIND_RINGS, p=[0, 0, 0, 0],
+               periodic=isinstance(space_cfg.main()[0], PeriodicWalls), kind=KIND_RINGS, p=[0, 0, 0, 0],
                dt=int_cfg.dt, cols=cols, rows=rows, nc=nc, radius=r, nmax=state.nmax, nrings=state.num_rings,
                types=state.types, np_type=np.atleast_1d(state.num_particles), ring_par=cfg.type_table(),
-               inter=inter.reshape(-1))
+               inter=inter.reshape(-1), walls=wall_records(space_cfg))
