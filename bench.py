@@ -108,6 +108,31 @@ def bench_rings(args):
     return out
 
 
+def bench_walls(args, plain_value):
+    """The plain 1M LJ workload inside a walls list (ManyWalls: periodic box,
+    16 slippery circles, 8 slippery lines, one potential circle): the cost
+    of the walls on the step (the walls-carrying instantiations of the pair
+    kernels, with force reuse)."""
+    import mavi_hip as mv
+    from mavi_hip import workloads as wl
+
+    pos, vel, space, dyn, ic = wl.lj_obstacles(nside=args.nside)
+    n = len(pos)
+    log(f"[bench] walls: {n} particles, {len(space.pairs())} walls")
+    system = mv.System(state=mv.SecondLawState(pos=pos, vel=vel), space_cfg=space, dynamic_cfg=dyn, int_cfg=ic)
+    dev = system.dev
+    dev.step(args.warmup)
+    t0 = time.perf_counter()
+    dev.step(args.walls_steps)
+    wall = time.perf_counter() - t0
+    value = n * args.walls_steps / wall
+    return {"workload": f"plain LJ as the headline, {n} particles (those within 1.5 of an obstacle left out), "
+                        "walls list: PeriodicWalls box, 16 SlipperyWalls circles (radius 8), SlipperyWalls "
+                        "LinesCfg of 8 segments (length 40), PotentialWalls circle (HarmTrunc k_rep 20, range 1)",
+            "particles": n, "steps": args.walls_steps, "value": value, "unit": "particle-steps/s",
+            "ms_per_step": wall / args.walls_steps * 1e3, "ratio_vs_plain": value / plain_value}
+
+
 def bench_active(args, kind):
     """Self-propelled particles, nside^2 = 1M: szabo_step! (examples/szabo.jl
     parameters, dt 0.01) or rtp_step! (examples/rtp.jl parameters, density
@@ -365,6 +390,7 @@ def main():
     ap.add_argument("--rings", type=int, default=40000, help="rings in the rings form (0 = skip)")
     ap.add_argument("--rings-steps", type=int, default=500)
     ap.add_argument("--rings-cpu-steps", type=int, default=40)
+    ap.add_argument("--walls-steps", type=int, default=500, help="timed steps of the walls form (0 = skip)")
     ap.add_argument("--active-nside", type=int, default=1024,
                     help="side of the Szabo and run-and-tumble lattices (0 = skip those forms)")
     ap.add_argument("--active-steps", type=int, default=500)
@@ -480,6 +506,8 @@ def main():
     del system, dev
     if args.rings > 0:
         forms["rings"] = bench_rings(args)
+    if args.walls_steps > 0:
+        forms["walls"] = bench_walls(args, value)
     if args.active_nside > 0:
         forms["szabo"] = bench_active(args, "szabo")
         forms["run_and_tumble"] = bench_active(args, "rtp")

@@ -619,11 +619,13 @@ __device__ __forceinline__ void walls_apply(const SysParams &P, double &x, doubl
 
 // Integrator epilogue of a pair-force launch for sorted slot s (WALLS: the
 // system has a walls list; only kernels instantiated for it carry its code).
+// Returns whether walls! moved the particle (second evaluation only).
 template <int MODE, bool WALLS = false>
-__device__ __forceinline__ void pair_epilogue(const SysParams &P, const int s, const double2 pi, const double fx_,
+__device__ __forceinline__ bool pair_epilogue(const SysParams &P, const int s, const double2 pi, const double fx_,
                                               const double fy_, double2 *__restrict__ Pst, double2 *__restrict__ Vst,
                                               double2 *__restrict__ Fout, double2 *__restrict__ P2,
                                               double2 *__restrict__ F1) {
+    bool moved = false;
     if constexpr (MODE == MODE_FORCES) {
         Fout[s] = make_double2(fx_, fy_);
     } else if constexpr (MODE == MODE_VERLET1) {
@@ -644,6 +646,7 @@ __device__ __forceinline__ void pair_epilogue(const SysParams &P, const int s, c
         double x = pi.x, y = pi.y;
         if constexpr (WALLS) {
             walls_apply(P, x, y, &vx, &vy, P.radius);
+            moved = x != pi.x || y != pi.y;
         } else if (P.periodic) {
             periodic_wrap(P, x, y);
         } else {
@@ -658,6 +661,7 @@ __device__ __forceinline__ void pair_epilogue(const SysParams &P, const int s, c
         Vst[s] = make_double2(vx, vy);
         Fout[s] = make_double2(fx, fy);
     }
+    return moved;
 }
 
 __global__ void k_counter_inc(unsigned long long *counter) {

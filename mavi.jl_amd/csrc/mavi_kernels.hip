@@ -124,7 +124,7 @@ __global__ void k_pair(SysParams P, RingCtx R, const double2 *__restrict__ pin, 
                        double2 *__restrict__ Fout, double2 *__restrict__ P2, double2 *__restrict__ F1,
                        double list2, const int32_t *__restrict__ needc, const uint32_t *__restrict__ epoch,
                        const int32_t *__restrict__ perm, const double2 *__restrict__ Vold,
-                       const double2 *__restrict__ Fold) {
+                       const double2 *__restrict__ Fold, int32_t *__restrict__ wstamp) {
     extern __shared__ uint16_t s_list[];
     int s = xcd_block() * blockDim.x + threadIdx.x;
     bool valid = s < P.na;
@@ -176,7 +176,12 @@ __global__ void k_pair(SysParams P, RingCtx R, const double2 *__restrict__ pin, 
         pair_gather_chunks<POT>(P, R, acc, pin, scell[sv], start, s, valid, pi, list2, s_list);
     }
     if (!valid) return;
-    pair_epilogue<MODE, WALLS>(P, s, pi, acc.fx, acc.fy, Pst, Vst, Fout, P2, F1);
+    const bool moved = pair_epilogue<MODE, WALLS>(P, s, pi, acc.fx, acc.fy, Pst, Vst, Fout, P2, F1);
+    if constexpr (MODE == MODE_VERLET2 && WALLS) {
+        // forces of a particle walls! moved cannot be reused: mark its chunk
+        // for the next binning (epoch + 1), as a chunk change would
+        if (moved && wstamp) wstamp[scell[s]] = (int32_t)(epoch[0] + 1u);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -650,17 +655,9 @@ hipError_t launch_pair_t(mavi_sys *s, const double2 *pin, hipStream_t st, bool r
     const int wg = s->wg;
     const int c = s->cur;
     const double l2 = s->two_phase ? list_range2(s) : INFINITY;
-    if constexpr (MODE == MODE_VERLET1 || MODE == MODE_VERLET2) {
-        if (s->P.nwalls > 0) {  // gather kernel with the walls list
-            const size_t lds = (l2 < INFINITY) ? sizeof(uint16_t) * kListCap * (size_t)wg : 0;
-            hipLaunchKernelGGL((k_pair<MODE, POT, ALL, true>), dim3(nblk(s->P.na, wg)), dim3(wg), lds, st, s->P,
-                               ring_ctx(s), pin, s->scell, s->start, s->P_[c], s->V_[c], s->F, s->P2, s->F1, l2,
-                               nullptr, s->epoch, defer ? s->perm : nullptr, s->V_[1 - c], s->F);
-            return hipGetLastError();
-        }
-    }
+    const bool walls = (MODE == MODE_VERLET1 || MODE == MODE_VERLET2) && s->P.nwalls > 0;
     if constexpr (!ALL) {
-        if (s->lds_tiles) {
+        if (s->lds_tiles && !walls) {  // the LDS-tiled form carries no walls list
             const int nsegc = (s->P.cols + kSegCells - 1) / kSegCells;
             const int nseg = nsegc * s->P.rows;
             const int nb = (nseg + kSegWaves - 1) / kSegWaves;
@@ -679,9 +676,20 @@ hipError_t launch_pair_t(mavi_sys *s, const double2 *pin, hipStream_t st, bool r
     // block: large blocks leave fewer partly filled waves
     const int wgk = (MODE == MODE_VERLET1 && reuse) ? 1024 : wg;
     const size_t lds = (l2 < INFINITY) ? sizeof(uint16_t) * kListCap * (size_t)wgk : 0;
+    const int32_t *needc = reuse ? s->needc : nullptr;
+    const int32_t *perm = defer ? s->perm : nullptr;
+    if constexpr (MODE == MODE_VERLET1 || MODE == MODE_VERLET2) {
+        if (walls) {  // the walls list's forces and walls! in the epilogue
+            hipLaunchKernelGGL((k_pair<MODE, POT, ALL, true>), dim3(nblk(s->P.na, wgk)), dim3(wgk), lds, st, s->P,
+                               ring_ctx(s), pin, s->scell, s->start, s->P_[c], s->V_[c], s->F, s->P2, s->F1, l2,
+                               needc, s->epoch, perm, s->V_[1 - c], s->F,
+                               (MODE == MODE_VERLET2 && !ALL) ? s->stamp : nullptr);
+            return hipGetLastError();
+        }
+    }
     hipLaunchKernelGGL((k_pair<MODE, POT, ALL>), dim3(nblk(s->P.na, wgk)), dim3(wgk), lds, st, s->P, ring_ctx(s), pin,
-                       s->scell, s->start, s->P_[c], s->V_[c], s->F, s->P2, s->F1, l2, reuse ? s->needc : nullptr,
-                       s->epoch, defer ? s->perm : nullptr, s->V_[1 - c], s->F);
+                       s->scell, s->start, s->P_[c], s->V_[c], s->F, s->P2, s->F1, l2, needc, s->epoch, perm,
+                       s->V_[1 - c], s->F, nullptr);
     return hipGetLastError();
 }
 
@@ -821,7 +829,7 @@ hipError_t launch_forces_only(mavi_sys *s, hipStream_t st) {
 hipError_t launch_verlet_step(mavi_sys *s, hipStream_t st, bool reuse) {
     // the gather kernel's first evaluation moves velocities (and reused
     // forces) into the new order, so the binning leaves them where they are
-    const bool defer = !s->lds_tiles && s->P.cols > 0;
+    const bool defer = (!s->lds_tiles || s->P.nwalls > 0) && s->P.cols > 0;
     const bool can = s->reuse && defer;
     reuse = reuse && can && s->sorted;
     hipError_t e = launch_bin(s, st, true, can, defer);

@@ -24,6 +24,11 @@ and the CPU oracle.
                      gases (nside = round(sqrt(N)), seed 1000 + i), odd ones
                      rings_mixture systems (round(N / 20) rings of 20,
                      seed 2000 + i, device noise seed 3000 + i)
+  lj_obstacles       lj_periodic with a walls list: the periodic box first,
+                     then 16 slippery circles (radius 8, on a 4x4 grid),
+                     8 slippery line segments (length 40) and one potential
+                     circle (HarmTrunc k_rep 20, range 1, repulsion);
+                     particles within 1.5 of an obstacle are left out
   szabo_lattice      nside x nside Szabo particles (examples/szabo.jl
                      parameters: vo 1, mobility 1, relax_time 1, k_rep 10,
                      k_adh 0.75, r_eq 1, r_max 1.1, rot_diff 0.01), square
@@ -128,6 +133,36 @@ def batch_ensemble(nrep=512, nmin=1000, nmax=64000, seed=3):
             nr = max(2, int(round(n / 20)))
             out.append(("rings", rings_mixture(nrings=nr, seed=2000 + i), 3000 + i))
     return out
+
+
+def lj_obstacles(nside=1024, seed=1):
+    """(pos, vel, space, dyn, int_cfg) of lj_periodic inside a walls list (see
+    the module docstring)."""
+    from .configs import CircleCfg, LinesCfg, ManyGeometries, PotentialWalls, SlipperyWalls
+    from .configs import HarmTruncCfg as HT
+
+    pos, vel, space, dyn, ic = lj_periodic(nside=nside, seed=seed)
+    box = space.geometry_cfg
+    L = box.length
+    circles = [CircleCfg(radius=8.0, center=((i + 0.5) * L / 4, (j + 0.5) * L / 4)) for i in range(4) for j in range(4)]
+    segs = []
+    for k in range(8):
+        x0, y0 = (k + 0.25) * L / 8, (0.1 + 0.1 * k) * L
+        segs.append(((x0, y0), (x0 + 40.0, y0)))
+    lines = LinesCfg(segs)
+    pc = CircleCfg(radius=5.0, center=(0.13 * L, 0.87 * L))
+    keep = np.ones(len(pos), dtype=bool)
+    for c in circles + [pc]:
+        keep &= np.linalg.norm(pos - np.array(c.center), axis=1) > c.radius + 1.5
+    for (a, b) in segs:
+        inside = (pos[:, 0] > a[0] - 1.5) & (pos[:, 0] < b[0] + 1.5)
+        keep &= ~(inside & (np.abs(pos[:, 1] - a[1]) < 1.5))
+    pos, vel = pos[keep], vel[keep]
+    vel = vel - vel.mean(axis=0)
+    pairs = [(PeriodicWalls(), box)] + [(SlipperyWalls(), c) for c in circles] + [(SlipperyWalls(), lines),
+             (PotentialWalls(HT(k_rep=20.0, k_atr=0.0, dist_eq=1.0, dist_max=1.0)), pc)]
+    space = SpaceCfg.from_list(pairs)
+    return pos, vel, space, dyn, ic
 
 
 def szabo_lattice(nside=1024, seed=4, rot_diff=0.01, dt=0.01):
