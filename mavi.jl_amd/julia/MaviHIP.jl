@@ -69,7 +69,51 @@ struct SysDesc                      # mavi_sys_desc
     interaction::Ptr{Float64}
     seed::UInt64
     active::NTuple{4,Float64}    # Szabo: vo, mobility, relax_time, rot_diff; RTP: vo, tumble_rate
+    nwalls::Int32
+    reserved2::Int32
+    walls::Ptr{Cvoid}            # Ptr{MaviWall}
 end
+
+struct MaviWall                     # mavi_wall
+    kind::Int32
+    mode::Int32
+    pot_kind::Int32
+    nlines::Int32
+    geom::NTuple{4,Float64}
+    pot::NTuple{4,Float64}
+    lines::Ptr{Float64}
+end
+
+const WMODE = Dict(Configs.Outside => Int32(0), Configs.Inside => Int32(1), Configs.Repulsion => Int32(2))
+wall_pot(p::LenJonesCfg) = (MAVI_POT_LJ, (p.sigma, p.epsilon, 0.0, 0.0))
+wall_pot(p::HarmTruncCfg) = (MAVI_POT_HARMTRUNC, (p.k_rep, p.k_atr, p.dist_eq, p.dist_max))
+
+# One mavi_wall per (wall, geometry) pair; `keep` holds the line tables alive.
+function wall_record(w, g::RectangleCfg, keep)
+    kind = w isa PeriodicWalls ? Int32(1) : Int32(0)
+    MaviWall(kind, 0, 0, 0, (g.bottom_left[1], g.bottom_left[2], g.length, g.height), (0.0, 0.0, 0.0, 0.0), C_NULL)
+end
+function wall_record(w, g::Configs.CircleCfg, keep)
+    kind = w isa RigidWalls ? Int32(2) : w isa Configs.SlipperyWalls ? Int32(4) : Int32(5)
+    mode, pk, pot = w isa Configs.PotentialWalls ? (WMODE[typeof(w.mode)], wall_pot(w.potential)...) :
+                    (Int32(0), Cint(0), (0.0, 0.0, 0.0, 0.0))
+    MaviWall(kind, mode, pk, 0, (g.center[1], g.center[2], g.radius, 0.0), pot, C_NULL)
+end
+function wall_record(w, g::Configs.LinesCfg, keep)
+    kind = w isa Configs.SlipperyWalls ? Int32(3) : Int32(6)
+    tab = Float64[]
+    for l in g.lines
+        append!(tab, (l.p1[1], l.p1[2], l.p2[1], l.p2[2]))
+    end
+    push!(keep, tab)
+    mode, pk, pot = w isa Configs.PotentialWalls ? (WMODE[typeof(w.mode)], wall_pot(w.potential)...) :
+                    (Int32(0), Cint(0), (0.0, 0.0, 0.0, 0.0))
+    MaviWall(kind, mode, pk, length(g.lines), (0.0, 0.0, 0.0, 0.0), pot, pointer(tab))
+end
+wall_records(s::SpaceCfg{<:Configs.ManyWalls}, keep) =
+    [wall_record(w, g, keep) for (w, g) in zip(s.wall_type.list, s.geometry_cfg.list)]
+wall_records(s::SpaceCfg{<:Union{RigidWalls,PeriodicWalls},<:RectangleCfg}, keep) = MaviWall[]
+wall_records(s::SpaceCfg, keep) = [wall_record(s.wall_type, s.geometry_cfg, keep)]
 
 mutable struct DeviceSys
     ctx::Ptr{Cvoid}
@@ -108,6 +152,9 @@ gpu_kind(::Mavi.Rings.Configs.RingsCfg{U,T,F}) where {U,T<:Mavi.Rings.Configs.Ha
 
 wall_kind(::SpaceCfg{RigidWalls,<:RectangleCfg}) = MAVI_WALL_RIGID
 wall_kind(::SpaceCfg{PeriodicWalls,<:RectangleCfg}) = MAVI_WALL_PERIODIC
+wall_kind(s::SpaceCfg{<:Configs.ManyWalls}) = Configs.get_main_wall(s) isa PeriodicWalls ? MAVI_WALL_PERIODIC :
+                                               MAVI_WALL_RIGID
+wall_kind(::SpaceCfg{<:Union{RigidWalls,Configs.SlipperyWalls,Configs.PotentialWalls}}) = MAVI_WALL_RIGID
 wall_kind(::Any) = nothing
 
 on_gpu(system) = system.int_cfg.device isa GPU &&
@@ -117,7 +164,7 @@ function device_system(system)
     key = objectid(system)
     haskey(SYSTEMS, key) && return SYSTEMS[key]
     ctx = context(system.int_cfg.device.device)
-    g = system.space_cfg.geometry_cfg
+    g = Configs.get_bounding_box(system.space_cfg.geometry_cfg)  # the chunk space
     dyn = system.dynamic_cfg
     chunks = system.chunks
     cols = isnothing(chunks) ? 0 : chunks.num_cols
@@ -132,12 +179,19 @@ function device_system(system)
     desc = SysDesc(length(system.state.pos), Tuple(g.bottom_left), (g.length, g.height),
                    wall_kind(system.space_cfg), gpu_kind(dyn), pot, system.int_cfg.dt,
                    cols, rows, nc, 0, minimum(particle_radius(dyn)),
-                   0, 0, 0, 0, C_NULL, C_NULL, C_NULL, C_NULL, UInt64(0), active)
+                   0, 0, 0, 0, C_NULL, C_NULL, C_NULL, C_NULL, UInt64(0), active, 0, 0, C_NULL)
+    keep = Any[]
+    walls = wall_records(system.space_cfg, keep)
+    if !isempty(walls)
+        desc = SysDesc(ntuple(i -> getfield(desc, i), fieldcount(SysDesc) - 3)..., Int32(length(walls)), 0,
+                       Ptr{Cvoid}(pointer(walls)))
+    end
     # (RingsSystem fills nmax/nrings/ntypes and the type tables the same way,
     #  from RingsCfg fields p0, relax_time, vo, mobility, rot_diff, k_area,
     #  k_spring, l_spring, num_particles and interaction_finder.)
     out = Ref{Ptr{Cvoid}}(C_NULL)
-    check(ctx, ccall((:mavi_sys_create, LIB), Cint, (Ptr{Cvoid}, Ref{SysDesc}, Ref{Ptr{Cvoid}}), ctx, desc, out))
+    GC.@preserve walls keep check(ctx, ccall((:mavi_sys_create, LIB), Cint,
+                                             (Ptr{Cvoid}, Ref{SysDesc}, Ref{Ptr{Cvoid}}), ctx, desc, out))
     d = DeviceSys(ctx, out[], true)
     SYSTEMS[key] = d
     finalizer(system.state) do _
