@@ -170,6 +170,20 @@ int mavi_sp_step(mavi_sys *sys, int64_t nsteps, const double *noise);
  * replaces randn(system.rng) of src/rings/integration.jl:348.  NULL = zero. */
 int mavi_rings_step(mavi_sys *sys, int64_t nsteps, const double *noise);
 
+/* Contact neighbours of a rings system: ParticleNeighbors
+ * (src/rings/neighbors.jl:47-137) as calc_interaction fills them during
+ * forces! (src/rings/integration.jl:32-45).  With enable, every forces!
+ * (each step, and mavi_forces) records for each particle the others closer
+ * than tol * 2 * particle_radius of the pair's interaction; only_rings
+ * keeps pairs of different rings only (type = :rings). */
+int mavi_rings_neighbors_cfg(mavi_sys *sys, int enable, double tol, int only_rings);
+/* The last forces!' neighbours: counts (nslots) and, when given, lists
+ * (nslots x 15, per particle 15 entries of 1-based ids in ascending order,
+ * -1 padded; the reference's list holds the same set in its traversal
+ * order).  More than 15 neighbours fails the step with
+ * MAVI_E_CELL_OVERFLOW, as the reference's BoundsError. */
+int mavi_rings_neighbors_get(mavi_sys *sys, int64_t *counts, int64_t *lists);
+
 /* RingsInfo: areas (nrings), cms (2*nrings), continuos_pos (2*nmax*nrings),
  * src/rings/rings.jl:118-128; any may be NULL. */
 int mavi_rings_info(mavi_sys *sys, double *areas, double *cms, double *continuous_pos);

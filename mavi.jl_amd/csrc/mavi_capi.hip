@@ -25,6 +25,9 @@ int check_flags(mavi_sys *s) {
     HIPCHK(ctx, hipMemsetAsync(s->flags, 0, sizeof(uint32_t), ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     if (h & MAVI_FLAG_SCAN_STALL) return set_err(ctx, MAVI_E_HIP, "chunk-count scan: look-back did not complete");
+    if (h & MAVI_FLAG_NEIGH_OVERFLOW)
+        return set_err(ctx, MAVI_E_CELL_OVERFLOW,
+                       "a particle has more than 15 contact neighbours (BoundsError in neigh_update_data!)");
     if (h & MAVI_FLAG_NONFINITE)
         return set_err(ctx, MAVI_E_NONFINITE, "non-finite particle position met while binning");
     if (h & MAVI_FLAG_OUT_OF_BOX)
@@ -99,7 +102,7 @@ int mavi_sys_destroy(mavi_sys *s) {
     dfree(s->active_ids); dfree(s->pos_id); dfree(s->vel_id); dfree(s->force_id);
     dfree(s->pol); dfree(s->areas); dfree(s->cms); dfree(s->cms_next); dfree(s->cpos);
     dfree(s->rtype); dfree(s->types); dfree(s->inter); dfree(s->noise); dfree(s->red);
-    dfree(s->walls); dfree(s->lines);
+    dfree(s->walls); dfree(s->lines); dfree(s->neigh_cnt); dfree(s->neigh_list);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
     delete s;
@@ -400,6 +403,7 @@ int mavi_sys_create(mavi_ctx *ctx, const mavi_sys_desc *d, mavi_sys **out) {
         if (e == hipSuccess) e = up(s->rtype, host_rtype.data(), sizeof(int32_t) * host_rtype.size());
         if (e == hipSuccess) e = up(s->types, host_types.data(), sizeof(RingType) * host_types.size());
         if (e == hipSuccess) e = up(s->inter, host_inter.data(), sizeof(RingInter) * host_inter.size());
+        s->host_inter = host_inter;
     }
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     s->seed = h.seed;
@@ -618,6 +622,55 @@ int mavi_sp_step(mavi_sys *s, int64_t nsteps, const double *noise) {
     if (!s) return MAVI_E_BADARG;
     if (!is_active(s)) return set_err(s->ctx, MAVI_E_BADARG, "not a self-propelled (Szabo / RTP) system");
     return steps_impl(s, nsteps, noise);
+}
+
+int mavi_rings_neighbors_cfg(mavi_sys *s, int enable, double tol, int only_rings) {
+    if (!s) return MAVI_E_BADARG;
+    mavi_ctx *ctx = s->ctx;
+    if (s->P.potential != MAVI_POT_RINGS) return set_err(ctx, MAVI_E_BADARG, "not a rings system");
+    if (enable && !(tol > 0.0)) return set_err(ctx, MAVI_E_BADARG, "tol must be > 0");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    if (enable && !s->neigh_cnt) {
+        HIPCHK(ctx, dalloc(&s->neigh_cnt, s->P.n, st));
+        HIPCHK(ctx, dalloc(&s->neigh_list, (int64_t)s->P.n * 15, st));
+    }
+    if (enable) {
+        // max_dist * tol with max_dist = 2 particle_radius(interaction) = dist_eq
+        for (auto &I : s->host_inter) I.nb2 = exact_lt_threshold(2.0 * (I.dist_eq / 2.0) * tol);
+        HIPCHK(ctx, hipMemcpyAsync(s->inter, s->host_inter.data(), sizeof(RingInter) * s->host_inter.size(),
+                                   hipMemcpyHostToDevice, st));
+        HIPCHK(ctx, mavi::launch_fill_i32(s->neigh_cnt, 0, s->P.n, st));
+        HIPCHK(ctx, mavi::launch_fill_i32(s->neigh_list, -1, (int64_t)s->P.n * 15, st));
+    }
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    const bool changed = s->neigh_on != (enable != 0) || s->neigh_only_rings != (only_rings != 0);
+    s->neigh_on = enable != 0;
+    s->neigh_only_rings = only_rings != 0;
+    s->neigh_tol = tol;
+    if (changed && s->graph2) {  // the captured graph holds the old launch sequence
+        (void)hipGraphExecDestroy(s->graph2);
+        s->graph2 = nullptr;
+    }
+    return MAVI_OK;
+}
+
+int mavi_rings_neighbors_get(mavi_sys *s, int64_t *counts, int64_t *lists) {
+    if (!s) return MAVI_E_BADARG;
+    mavi_ctx *ctx = s->ctx;
+    if (!s->neigh_on) return set_err(ctx, MAVI_E_BADARG, "contact neighbours are not enabled");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    const int64_t n = s->P.n;
+    std::vector<int32_t> c(n), l(counts && lists ? n * 15 : 0);
+    HIPCHK(ctx, hipMemcpy(c.data(), s->neigh_cnt, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    if (lists) HIPCHK(ctx, hipMemcpy(l.data(), s->neigh_list, sizeof(int32_t) * n * 15, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; i++) {
+        if (counts) counts[i] = c[i];
+        if (lists)
+            for (int k = 0; k < 15; k++) lists[i * 15 + k] = l[i * 15 + k];
+    }
+    return MAVI_OK;
 }
 
 int mavi_rings_info(mavi_sys *s, double *areas, double *cms, double *continuous_pos) {

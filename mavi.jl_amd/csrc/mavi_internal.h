@@ -33,6 +33,7 @@
 #define MAVI_FLAG_OUT_OF_BOX 2u
 #define MAVI_FLAG_NONFINITE 4u
 #define MAVI_FLAG_SCAN_STALL 8u
+#define MAVI_FLAG_NEIGH_OVERFLOW 16u
 
 struct mavi_ctx {
     int device = 0;
@@ -100,6 +101,7 @@ struct RingInter {
     double k_rep, k_atr, dist_eq, dist_max;
     double max2;   // dist > dist_max <=> r2 > max2
     double eq2;    // dist < dist_eq  <=> r2 < eq2
+    double nb2;    // contact neighbours: dist < 2 particle_radius * tol <=> r2 < nb2
 };
 
 struct ProfSlot {
@@ -144,6 +146,12 @@ struct mavi_sys {
     // reuse of the second force evaluation of a Verlet step as the first one
     // of the next step where a particle's 3x3 chunk block kept its members
     int32_t *stamp = nullptr;   // per chunk: binning epoch in which its membership last changed
+    // contact neighbours of rings (ParticleNeighbors), 15 per particle
+    bool neigh_on = false;
+    int neigh_only_rings = 0;
+    double neigh_tol = 1.1;
+    int32_t *neigh_cnt = nullptr, *neigh_list = nullptr;  // by particle id
+    std::vector<RingInter> host_inter;
     DevWall *walls = nullptr;   // walls list (P.walls / P.lines point here)
     DevLine *lines = nullptr;
     int32_t *needc = nullptr;   // per chunk: binning epoch in which a chunk of its 3x3 block changed
@@ -208,6 +216,7 @@ hipError_t launch_rings_step(mavi_sys *s, const double *noise_dev, hipStream_t s
 hipError_t launch_active_step(mavi_sys *s, const double *noise_dev, hipStream_t st);
 hipError_t launch_rings_forces(mavi_sys *s, hipStream_t st);
 hipError_t launch_rings_init(mavi_sys *s, hipStream_t st);
+hipError_t launch_ring_neigh(mavi_sys *s, hipStream_t st);
 hipError_t launch_observables(mavi_sys *s, double *d_out, hipStream_t st);
 hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t st);
 }  // namespace mavi

@@ -185,6 +185,78 @@ __global__ void k_pair(SysParams P, RingCtx R, const double2 *__restrict__ pin, 
 }
 
 // ---------------------------------------------------------------------------
+// Contact neighbours of rings: ParticleNeighbors (src/rings/neighbors.jl:
+// 117-137) as calc_interaction fills them (src/rings/integration.jl:32-45)
+// in forces!.  One thread per slot gathers over its 3x3 chunk block (all
+// slots without chunks): the particles closer than tol * dist_eq of the
+// pair's interaction (of another ring only, with only_rings), as 1-based ids
+// in ascending order, -1 padded.  More than 15 raises a flag (the reference's
+// BoundsError).  Runs between the pair kernel and the update, on the
+// positions forces! sees.
+// ---------------------------------------------------------------------------
+constexpr int kNeighMax = 15;
+
+__global__ void k_ring_neigh(SysParams P, RingCtx R, const double2 *__restrict__ pin,
+                             const int32_t *__restrict__ start, const int32_t *__restrict__ scell, int only_rings,
+                             int32_t *__restrict__ cnt_out, int32_t *__restrict__ list_out,
+                             uint32_t *__restrict__ flags) {
+    const int s = xcd_block() * blockDim.x + threadIdx.x;
+    if (s >= P.na) return;
+    const int id = R.ids[s] - R.id_base;
+    const int ri = ring_of((uint32_t)id, R);
+    const int ti = P.typed ? R.rtype[ri] : 0;
+    const double2 pi = pin[s];
+    int list[kNeighMax];
+    int cnt = 0;
+    auto visit = [&](int a, int b) {
+        for (int j = a; j < b; j++) {
+            if (j == s) continue;
+            const int idj = R.ids[j] - R.id_base;
+            const int rj = ring_of((uint32_t)idj, R);
+            if (only_rings && rj == ri) continue;
+            const RingInter *ic = R.inter + (P.typed ? ti + P.ntypes * R.rtype[rj] : 0);
+            const double2 d = cdiff(P, pi, pin[j]);
+            if (d.x * d.x + d.y * d.y < ic->nb2) {
+                if (cnt < kNeighMax) list[cnt] = idj + 1;
+                cnt++;
+            }
+        }
+    };
+    if (P.cols == 0) {
+        visit(0, P.na);
+    } else {
+        const int c = scell[s];
+        const int row = c / P.cols, col = c - row * P.cols;
+        for (int dr = -1; dr <= 1; dr++) {
+            int r = row + dr;
+            if (P.periodic) r = r < 0 ? r + P.rows : (r >= P.rows ? r - P.rows : r);
+            else if (r < 0 || r >= P.rows) continue;
+            for (int dc = -1; dc <= 1; dc++) {
+                int cc = col + dc;
+                if (P.periodic) cc = cc < 0 ? cc + P.cols : (cc >= P.cols ? cc - P.cols : cc);
+                else if (cc < 0 || cc >= P.cols) continue;
+                visit(start[r * P.cols + cc], start[r * P.cols + cc + 1]);
+            }
+        }
+    }
+    if (cnt > kNeighMax) {
+        atomicOr(flags, MAVI_FLAG_NEIGH_OVERFLOW);
+        cnt = kNeighMax;
+    }
+    for (int a = 1; a < cnt; a++) {  // ascending ids
+        const int v = list[a];
+        int b = a - 1;
+        while (b >= 0 && list[b] > v) {
+            list[b + 1] = list[b];
+            b--;
+        }
+        list[b + 1] = v;
+    }
+    cnt_out[id] = cnt;
+    for (int k = 0; k < kNeighMax; k++) list_out[(int64_t)id * kNeighMax + k] = k < cnt ? list[k] : -1;
+}
+
+// ---------------------------------------------------------------------------
 // Self-propelled particles: szabo_step! / rtp_step! (src/integration.jl:
 // 517-535).  One launch after the binning: pair forces at the binned
 // positions (Szabo, or the truncated LJ of run-and-tumble), then, fused,
@@ -875,6 +947,12 @@ static hipError_t launch_rings_kernel(mavi_sys *s, bool update, const double *no
     return hipGetLastError();
 }
 
+hipError_t launch_ring_neigh(mavi_sys *s, hipStream_t st) {
+    hipLaunchKernelGGL(k_ring_neigh, dim3(nblk(s->P.na, 256)), dim3(256), 0, st, s->P, ring_ctx(s), s->P_[s->cur],
+                       s->start, s->scell, s->neigh_only_rings, s->neigh_cnt, s->neigh_list, s->flags);
+    return hipGetLastError();
+}
+
 hipError_t launch_rings_forces(mavi_sys *s, hipStream_t st) {
     hipError_t e = launch_bin(s, st, false, false, false);
     if (e != hipSuccess) return e;
@@ -882,6 +960,7 @@ hipError_t launch_rings_forces(mavi_sys *s, hipStream_t st) {
         ProfScope ps(s, MAVI_PROF_FORCE, st);
         if ((e = launch_pair<MODE_FORCES>(s, s->P_[s->cur], st)) != hipSuccess) return e;
     }
+    if (s->neigh_on && (e = launch_ring_neigh(s, st)) != hipSuccess) return e;
     return launch_rings_kernel(s, false, nullptr, st);
 }
 
@@ -895,6 +974,7 @@ hipError_t launch_rings_step(mavi_sys *s, const double *noise_dev, hipStream_t s
         ProfScope ps(s, MAVI_PROF_FORCE, st);
         if ((e = launch_pair<MODE_FORCES>(s, s->P_[s->cur], st)) != hipSuccess) return e;
     }
+    if (s->neigh_on && (e = launch_ring_neigh(s, st)) != hipSuccess) return e;
     return launch_rings_kernel(s, true, noise_dev, st);
 }
 

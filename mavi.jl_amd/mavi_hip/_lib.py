@@ -131,6 +131,8 @@ SIGNATURES = {
     "mavi_rings_step": (C.c_int, [_P, C.c_int64, _DP]),
     "mavi_sp_step": (C.c_int, [_P, C.c_int64, _DP]),
     "mavi_rings_info": (C.c_int, [_P, _DP, _DP, _DP]),
+    "mavi_rings_neighbors_cfg": (C.c_int, [_P, C.c_int, C.c_double, C.c_int]),
+    "mavi_rings_neighbors_get": (C.c_int, [_P, _I64P, _I64P]),
     "mavi_observables": (C.c_int, [_P, _DP, _DP]),
     "mavi_run_record": (C.c_int, [_P, C.c_int64, _DP, _DP]),
     "mavi_prof_enable": (C.c_int, [_P, C.c_int]),
@@ -297,6 +299,16 @@ class DeviceSystem:
             noise = np.ascontiguousarray(noise, dtype=np.float64)
             assert noise.size >= nsteps * self.nrings
         self._call("mavi_rings_step", C.c_int64(int(nsteps)), dptr(noise))
+
+    def rings_neighbors_cfg(self, enable: bool, tol: float = 1.1, only_rings: bool = False):
+        self._call("mavi_rings_neighbors_cfg", 1 if enable else 0, C.c_double(tol), 1 if only_rings else 0)
+
+    def rings_neighbors(self):
+        """(counts (n,), lists (n, 15)): 1-based ids ascending, -1 padded."""
+        counts = np.zeros(self.n, dtype=np.int64)
+        lists = np.zeros((self.n, 15), dtype=np.int64)
+        self._call("mavi_rings_neighbors_get", i64ptr(counts), i64ptr(lists))
+        return counts, lists
 
     def rings_info(self):
         areas = np.zeros(self.nrings)

@@ -151,16 +151,34 @@ def describe(state: RingsState, space_cfg: SpaceCfg, dynamic_cfg: RingsCfg, int_
     return d, chunks, tables
 
 
+@dataclass
+class NeighborsCfg:  # src/rings/neighbors.jl:11-15
+    """Contact neighbours of the particles: type "all" or "rings" (pairs of
+    different rings only), recorded below tol * 2 * particle_radius.
+    only_count is accepted for the reference's signature; counts and lists
+    are both kept (at most 15 per particle)."""
+
+    only_count: bool = False
+    type: str = "all"
+    tol: float = 1.1
+
+
 class RingsSystem:
     """GPU-backed RingsSystem (src/rings/rings.jl:231-291)."""
 
     def __init__(self, *, state: RingsState, space_cfg: SpaceCfg, dynamic_cfg: RingsCfg, int_cfg,
-                 time_info: TimeInfo | None = None, seed: int = 0, ctx: _lib.Context | None = None):
+                 time_info: TimeInfo | None = None, seed: int = 0, ctx: _lib.Context | None = None,
+                 p_neighbors_cfg: NeighborsCfg | None = None):
         d, self.chunks, self._tables = describe(state, space_cfg, dynamic_cfg, int_cfg, seed)
         self.state, self.space_cfg, self.dynamic_cfg, self.int_cfg = state, space_cfg, dynamic_cfg, int_cfg
         self.time_info = time_info or TimeInfo()
         self.ctx = ctx or _lib.Context.for_device(int_cfg.device.device)
         self.dev = _lib.DeviceSystem(self.ctx, d, keep=self._tables)
+        self.p_neighbors_cfg = p_neighbors_cfg
+        if p_neighbors_cfg is not None:
+            if p_neighbors_cfg.type not in ("all", "rings"):
+                raise ValueError(p_neighbors_cfg.type)
+            self.dev.rings_neighbors_cfg(True, p_neighbors_cfg.tol, p_neighbors_cfg.type == "rings")
         self.forces = np.zeros_like(state.pos)
         self.upload()
         # the constructor's init sequence (src/rings/rings.jl:280-288) ends with forces!
@@ -193,6 +211,17 @@ class RingsSystem:
         self.chunks.chunk_particles = cp
         self.chunks.num_particles_in_chunk = counts
         return cp, counts
+
+    # --- contact neighbours (get_neigh_count / get_neigh_list, src/rings/neighbors.jl:67-71)
+    def get_neigh_count(self) -> np.ndarray:
+        """Neighbour count per particle slot (0-based index) of the last forces!."""
+        counts, _ = self.dev.rings_neighbors()
+        return counts
+
+    def get_neigh_list(self, pid: int) -> np.ndarray:
+        """1-based ids of the neighbours of particle ``pid`` (1-based), ascending."""
+        counts, lists = self.dev.rings_neighbors()
+        return lists[pid - 1, :counts[pid - 1]]
 
 
 def step(system: RingsSystem, noise=None):

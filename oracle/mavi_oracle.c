@@ -39,6 +39,9 @@
  *                              polarisation turning towards the force,
  *                              var(dtheta) = 2 Dr dt, tumbles at u < rate dt
  *                              to 2 pi v exactly, periodic wrap.
+ *   or_rings_neighbors         brute force over all pairs on random ring
+ *                              gases (same sets), counts = list lengths,
+ *                              symmetry (j in list(i) <=> i in list(j)).
  *   walls list                 rigid circle: speed kept, normal component
  *                              reversed (centred at the origin); slippery
  *                              circle / line / corner: ends at distance r;
@@ -369,6 +372,70 @@ int or_calc_forces(const or_sys *s, const double *pos, const int64_t *cp, const 
     free(nb);
     free(nnb);
     return OR_OK;
+}
+
+/* neigh_update!(::ParticleNeighbors, i, j, ri, rj, dist, max_dist) and
+ * neigh_update_data!, src/rings/neighbors.jl:102-137. */
+static int neigh_pair(const or_sys *s, const double *pos, int64_t i, int64_t j, double tol, int only_rings,
+                      int64_t *counts, int64_t *lists) {
+    const int32_t ri = ring_of(s, i), rj = ring_of(s, j);
+    const double *ic = inter_of(s, ri, rj);
+    double dr[2];
+    or_calc_diff(s, pos + 2 * (i - 1), pos + 2 * (j - 1), dr);
+    const double dist = sqrt(dr[0] * dr[0] + dr[1] * dr[1]);
+    const double max_dist = 2.0 * (ic[2] / 2.0); /* 2 * particle_radius(HarmTruncCfg) */
+    if (dist < max_dist * tol && (!only_rings || ri != rj)) {
+        if (counts[i - 1] >= 15 || counts[j - 1] >= 15) return OR_E_CELL_OVERFLOW;
+        lists[15 * (i - 1) + counts[i - 1]++] = j;
+        lists[15 * (j - 1) + counts[j - 1]++] = i;
+    }
+    return OR_OK;
+}
+
+int or_rings_neighbors(const or_sys *s, const double *pos, double tol, int only_rings, int64_t *counts,
+                       int64_t *lists) {
+    if (s->kind != OR_KIND_RINGS) return OR_E_BADARG;
+    const int64_t nslots = (int64_t)s->nmax * s->nrings;
+    for (int64_t k = 0; k < nslots; k++) counts[k] = 0;
+    for (int64_t k = 0; k < 15 * nslots; k++) lists[k] = -1;
+    int err = OR_OK;
+    if (s->cols <= 0) {
+        int64_t nids = or_particle_ids(s, NULL);
+        int64_t *ids = (int64_t *)malloc(sizeof(int64_t) * (nids > 0 ? nids : 1));
+        or_particle_ids(s, ids);
+        for (int64_t a = 0; a < nids && !err; a++)
+            for (int64_t b = a + 1; b < nids && !err; b++)
+                err = neigh_pair(s, pos, ids[a], ids[b], tol, only_rings, counts, lists);
+        free(ids);
+        return err;
+    }
+    int64_t ncell = (int64_t)s->rows * s->cols;
+    int64_t *cp = (int64_t *)malloc(sizeof(int64_t) * (size_t)s->nc * ncell);
+    int64_t *cnt = (int64_t *)malloc(sizeof(int64_t) * ncell);
+    err = or_bin(s, pos, cp, cnt);
+    int64_t *nb;
+    int32_t *nnb;
+    alloc_neighbors(s, &nb, &nnb);
+    const int64_t rows = s->rows, nc = s->nc;
+    for (int64_t col = 1; col <= s->cols && !err; col++) /* the chunk loop of force_column */
+        for (int64_t row = 1; row <= rows && !err; row++) {
+            int64_t k = (col - 1) * rows + (row - 1);
+            const int64_t *chunk = cp + nc * k;
+            for (int64_t i = 0; i < cnt[k] && !err; i++) {
+                for (int64_t j = i + 1; j < cnt[k] && !err; j++)
+                    err = neigh_pair(s, pos, chunk[i], chunk[j], tol, only_rings, counts, lists);
+                for (int e = 0; e < nnb[k] && !err; e++) {
+                    int64_t nk = (nb[(k * 4 + e) * 2 + 1] - 1) * rows + (nb[(k * 4 + e) * 2] - 1);
+                    for (int64_t j = 0; j < cnt[nk] && !err; j++)
+                        err = neigh_pair(s, pos, chunk[i], cp[nc * nk + j], tol, only_rings, counts, lists);
+                }
+            }
+        }
+    free(nb);
+    free(nnb);
+    free(cp);
+    free(cnt);
+    return err;
 }
 
 int or_calc_forces_threaded(const or_sys *s, const double *pos, const int64_t *cp, const int64_t *counts,

@@ -138,6 +138,17 @@ int or_calc_forces_threaded(const or_sys *s, const double *pos, const int64_t *c
 int or_newton_steps(const or_sys *s, double *pos, double *vel, double *forces,
                     int64_t nsteps, double *ke, double *pe, int nthreads);
 
+/* ParticleNeighbors of a rings system (src/rings/neighbors.jl:117-137) as
+ * filled by calc_interaction (src/rings/integration.jl:32-45) during one
+ * calc_forces! traversal (chunk loop or all pairs) at positions pos: a
+ * visited pair closer than tol * 2 * particle_radius(interaction) (only
+ * pairs of different rings when only_rings) is appended to both lists.
+ * counts (nslots), lists (15 x nslots, column per particle, 1-based ids in
+ * visiting order, -1 padded).  Returns OR_E_CELL_OVERFLOW past 15 (the
+ * BoundsError of neigh_update_data!). */
+int or_rings_neighbors(const or_sys *s, const double *pos, double tol, int only_rings, int64_t *counts,
+                       int64_t *lists);
+
 /* calc_walls_forces! for particle i (0-based): adds the potential walls'
  * forces to f[0..1], src/integration.jl:230-255. */
 void or_wall_forces(const or_sys *s, const double *pos, int64_t i, double *f);

@@ -96,6 +96,7 @@ _SIG = {
     "or_rings_steps": (C.c_int, [_S, _D, _D, C.c_int64, _D, _D, _D, _D]),
     "or_sp_steps": (C.c_int, [_S, _D, _D, C.c_int64, _D, _D, C.c_int]),
     "or_wall_forces": (None, [_S, _D, C.c_int64, _D]),
+    "or_rings_neighbors": (C.c_int, [_S, _D, C.c_double, C.c_int, _I64, _I64]),
     "or_walls_apply": (None, [_S, _D, _D, C.c_int64, C.c_double]),
 }
 
@@ -306,6 +307,18 @@ class Sys:
             raise RuntimeError(f"oracle rings_steps error {err}")
         return pos, pol, f, areas, cms
 
+
+    def rings_neighbors(self, pos, tol=1.1, only_rings=False):
+        """(counts (nslots,), lists (nslots, 15)) of ParticleNeighbors: 1-based
+        ids in the reference's visiting order, -1 padded."""
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        ns = self.nmax * self.nrings
+        counts = np.zeros(ns, dtype=np.int64)
+        lists = np.zeros((ns, 15), dtype=np.int64)
+        err = lib().or_rings_neighbors(self.ptr, _d(pos), tol, 1 if only_rings else 0, _i64(counts), _i64(lists))
+        if err:
+            raise RuntimeError(f"oracle rings_neighbors error {err}")
+        return counts, lists
 
     # walls
     def wall_forces(self, pos, i):

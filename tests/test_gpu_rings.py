@@ -204,3 +204,32 @@ def test_rings_kernel_shapes(gpu_ctx, oracle_mod, name, shape):
     ids = state.particle_ids()
     ep, _ = traj_err(sys_.state.pos[ids], None, p_ref[ids], None, space.geometry_cfg.length)
     assert ep <= TRAJ_RTOL and float(np.abs(sys_.state.pol - pol_ref).max()) <= 1e-9
+
+
+@pytest.mark.parametrize("ptype", ["all", "rings"])
+@pytest.mark.parametrize("chunks", [True, False])
+def test_contact_neighbors_vs_oracle(gpu_ctx, oracle_mod, ptype, chunks):
+    """ParticleNeighbors (src/rings/neighbors.jl:117-137): after k steps the
+    lists hold the pairs of the last forces! (positions after k-1 steps),
+    equal as sets to the oracle's (the device sorts them by id)."""
+    from mavi_hip import rings as R
+    from mavi_hip import workloads as wl
+
+    st, sp, cfg, ic = wl.rings_mixture(nrings=80, seed=7)
+    if not chunks:
+        ic.chunks_cfg = None
+    k = 30
+    noise = np.random.default_rng(3).standard_normal((k, st.num_rings)) * 0.3
+    o = oracle_mod.from_rings(st, sp, cfg, ic)
+    pos_km1, _, _, _, _ = o.rings_steps(st.pos.copy(), st.pol.copy(), k - 1, noise=noise[:k - 1])
+    s = R.RingsSystem(state=st, space_cfg=sp, dynamic_cfg=cfg, int_cfg=ic,
+                      p_neighbors_cfg=R.NeighborsCfg(type=ptype, tol=1.1))
+    s.advance(k, noise)
+    counts, lists = s.dev.rings_neighbors()
+    rc, rl = o.rings_neighbors(pos_km1, tol=1.1, only_rings=(ptype == "rings"))
+    assert counts.sum() > 0
+    assert np.array_equal(counts, rc)
+    for i in range(len(counts)):
+        assert np.array_equal(lists[i, :counts[i]], np.sort(rl[i, :rc[i]]))
+        assert np.all(lists[i, counts[i]:] == -1)
+    assert np.array_equal(s.get_neigh_list(1), lists[0, :counts[0]])

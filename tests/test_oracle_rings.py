@@ -152,3 +152,38 @@ def test_polarisation_relaxes_toward_velocity(oracle_mod):
     _, pol0, _, _, _ = s.rings_steps(pos, [0.0], 1)
     _, pol1, _, _, _ = s2.rings_steps(pos, [0.0], 1, noise=noise)
     assert pol1[0] - pol0[0] == pytest.approx(math.sqrt(2 * 0.2 * 0.01) * 0.5, rel=1e-12)
+
+
+@pytest.mark.parametrize("only_rings", [False, True])
+@pytest.mark.parametrize("chunks", [True, False])
+def test_contact_neighbors_match_brute_force(oracle_mod, only_rings, chunks):
+    """ParticleNeighbors (src/rings/neighbors.jl:117-137): the oracle's lists,
+    filled along its chunk traversal, hold exactly the pairs closer than
+    tol * dist_eq found by brute force, symmetrically."""
+    import sys
+
+    sys.path.insert(0, "mavi.jl_amd")
+    from mavi_hip import workloads as wl
+
+    st, sp, cfg, ic = wl.rings_mixture(nrings=36, seed=3)
+    rng = np.random.default_rng(1)
+    pos = np.mod(st.pos + rng.normal(scale=0.08, size=st.pos.shape), sp.geometry_cfg.length)
+    if not chunks:
+        ic.chunks_cfg = None
+    o = oracle_mod.from_rings(st, sp, cfg, ic)
+    tol = 1.1
+    counts, lists = o.rings_neighbors(pos, tol=tol, only_rings=only_rings)
+    L = sp.geometry_cfg.length
+    n, nmax = len(pos), st.nmax
+    d = pos[:, None, :] - pos[None, :, :]
+    d -= L * np.round(d / L)
+    dist = np.sqrt((d ** 2).sum(-1))
+    ring = np.arange(n) // nmax
+    close = (dist < 1.0 * tol) & ~np.eye(n, dtype=bool)
+    if only_rings:
+        close &= ring[:, None] != ring[None, :]
+    assert counts.sum() > (20 if only_rings else n)  # contacts exist
+    for i in range(n):
+        got = set(int(v) - 1 for v in lists[i, :counts[i]])
+        assert got == set(np.nonzero(close[i])[0].tolist())
+        assert np.all(lists[i, counts[i]:] == -1)
