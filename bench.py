@@ -94,10 +94,31 @@ def bench_rings(args):
            "particles": n, "steps": args.rings_steps, "value": n * args.rings_steps / wall,
            "unit": "particle-steps/s", "ms_per_step": wall / args.rings_steps * 1e3,
            "kernel_avg_us": {k: v[0] / max(v[1], 1) * 1e3 for k, v in prof.items()}}
+    # check_invasions! at the last step's ring points, ring chunks of about
+    # one ring diameter (r_chunks, capacity from the equilibrium ring radius)
+    g = sp.geometry_cfg
+    r_out = max(rings.equilibrium_ring_radius(cfg, 20, t) for t in (1, 2))
+    rcols = int(g.length // (2 * r_out))
+    grid = rings.ring_chunks_grid(st, sp, cfg, mv.ChunksCfg(rcols, rcols))
+    dev.rings_invasions_cfg(False, 0, *grid)
+    dev.rings_invasions_check()
+    t0 = time.perf_counter()
+    for _ in range(args.inv_checks):
+        dev.rings_invasions_check()
+    inv_ms = (time.perf_counter() - t0) / args.inv_checks * 1e3
+    nrec = len(dev.rings_invasions())
+    out["invasions"] = {"grid": list(grid), "checks": args.inv_checks, "check_ms": inv_ms, "records": nrec,
+                        "note": "host wall clock per mavi_rings_invasions_check (ring boxes + ring grid + "
+                                "crossing tests + one record-count read back)"}
     if not args.no_cpu:
         import oracle
 
         o = oracle.from_rings(st, sp, cfg, ic)
+        _, cms_d, cpos_d = dev.rings_info()
+        t0 = time.perf_counter()
+        o.rings_invasions(cpos_d, cms_d, *grid)
+        out["invasions"]["cpu_baseline_ms"] = (time.perf_counter() - t0) * 1e3
+        out["invasions"]["cpu_baseline_kind"] = "port, one core, the same check"
         t0 = time.perf_counter()
         o.rings_steps(pos0, pol0, args.rings_cpu_steps)
         dt = time.perf_counter() - t0
@@ -390,6 +411,7 @@ def main():
     ap.add_argument("--rings", type=int, default=40000, help="rings in the rings form (0 = skip)")
     ap.add_argument("--rings-steps", type=int, default=500)
     ap.add_argument("--rings-cpu-steps", type=int, default=40)
+    ap.add_argument("--inv-checks", type=int, default=20, help="timed invasion checks in the rings form")
     ap.add_argument("--walls-steps", type=int, default=500, help="timed steps of the walls form (0 = skip)")
     ap.add_argument("--active-nside", type=int, default=1024,
                     help="side of the Szabo and run-and-tumble lattices (0 = skip those forms)")

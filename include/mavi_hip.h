@@ -184,6 +184,30 @@ int mavi_rings_neighbors_cfg(mavi_sys *sys, int enable, double tol, int only_rin
  * MAVI_E_CELL_OVERFLOW, as the reference's BoundsError. */
 int mavi_rings_neighbors_get(mavi_sys *sys, int64_t *counts, int64_t *lists);
 
+/* Invasions of rings: check_invasions! (src/rings/integration.jl:379-509)
+ * with the ring chunks r_chunks (src/rings/rings.jl:188-202: r_rows x r_cols
+ * chunks of capacity r_nc over the chunk space, binning the rings' centres
+ * of mass; r_cols = r_rows = 0: r_chunks = nothing, every pair of rings) and,
+ * when scheduled, InvasionsCfg(steps_to_update) (src/rings/configs.jl:334-
+ * 350): a step checks when num_steps - last_check >= steps_to_update
+ * (update_invasions!, :511-520), last_check being the step count of this
+ * call.  The check uses the ring points and centres of mass of the step's
+ * forces! (RingsInfo.continuos_pos and cms).  Clears the list; with
+ * steps_to_update <= 0 checks at once, as the constructor does.  A centre of
+ * mass off the ring grid fails with MAVI_E_OUT_OF_BOX, a full ring chunk
+ * with MAVI_E_CELL_OVERFLOW (the reference's BoundsError). */
+int mavi_rings_invasions_cfg(mavi_sys *sys, int scheduled, int64_t steps_to_update, int32_t r_cols,
+                             int32_t r_rows, int32_t r_nc);
+/* check_invasions!(system) now, at the last step's ring points and centres
+ * of mass (after creation: the initial ones). */
+int mavi_rings_invasions_check(mavi_sys *sys);
+/* InvasionsInfo.list of the last check (src/rings/rings.jl:86-100): *count
+ * records and up to cap of them as (invasor, invaded, p_id) triples of
+ * 1-based ring ids and the invading particle's scalar index, ordered by
+ * (invasor, invaded, p_id); the reference's list holds the same set in the
+ * order of its chunk traversal. */
+int mavi_rings_invasions_get(mavi_sys *sys, int64_t *count, int64_t *records, int64_t cap);
+
 /* RingsInfo: areas (nrings), cms (2*nrings), continuos_pos (2*nmax*nrings),
  * src/rings/rings.jl:118-128; any may be NULL. */
 int mavi_rings_info(mavi_sys *sys, double *areas, double *cms, double *continuous_pos);

@@ -4,6 +4,7 @@
 
 #include <cstdio>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "mavi_host.h"
@@ -28,6 +29,13 @@ int check_flags(mavi_sys *s) {
     if (h & MAVI_FLAG_NEIGH_OVERFLOW)
         return set_err(ctx, MAVI_E_CELL_OVERFLOW,
                        "a particle has more than 15 contact neighbours (BoundsError in neigh_update_data!)");
+    if (h & MAVI_FLAG_RING_OUT_OF_BOX)
+        return set_err(ctx, MAVI_E_OUT_OF_BOX,
+                       "a ring's centre of mass is outside the ring chunk grid (BoundsError in update_chunks! of "
+                       "r_chunks)");
+    if (h & MAVI_FLAG_RING_CHUNK_OVERFLOW)
+        return set_err(ctx, MAVI_E_CELL_OVERFLOW,
+                       "a ring chunk holds more rings than its capacity (BoundsError in update_chunks! of r_chunks)");
     if (h & MAVI_FLAG_NONFINITE)
         return set_err(ctx, MAVI_E_NONFINITE, "non-finite particle position met while binning");
     if (h & MAVI_FLAG_OUT_OF_BOX)
@@ -103,6 +111,8 @@ int mavi_sys_destroy(mavi_sys *s) {
     dfree(s->pol); dfree(s->areas); dfree(s->cms); dfree(s->cms_next); dfree(s->cpos);
     dfree(s->rtype); dfree(s->types); dfree(s->inter); dfree(s->noise); dfree(s->red);
     dfree(s->walls); dfree(s->lines); dfree(s->neigh_cnt); dfree(s->neigh_list);
+    dfree(s->inv_cnt); dfree(s->inv_cell); dfree(s->inv_chunk); dfree(s->inv_box); dfree(s->inv_rec);
+    dfree(s->inv_n);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
     delete s;
@@ -557,12 +567,54 @@ static hipError_t ensure_graph(mavi_sys *s, hipStream_t st) {
     return e;
 }
 
+// check_invasions! on the device, then the record count read back: the
+// record buffer grows and the tests rerun when a check finds more records
+// than it holds.  Ring-grid errors (BoundsError in the reference) end the
+// call here.
+static int run_invasions(mavi_sys *s) {
+    mavi_ctx *ctx = s->ctx;
+    hipStream_t st = ctx->stream;
+    HIPCHK(ctx, mavi::launch_invasions(s, true, st));
+    for (;;) {
+        uint32_t n = 0;
+        HIPCHK(ctx, hipMemcpyAsync(&n, s->inv_n, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(ctx, hipStreamSynchronize(st));
+        const int rc = check_flags(s);
+        if (rc != MAVI_OK) {
+            s->inv_count = 0;
+            return rc;
+        }
+        if ((int64_t)n <= s->inv_cap) {
+            s->inv_count = n;
+            return MAVI_OK;
+        }
+        dfree(s->inv_rec);
+        s->inv_cap = 0;
+        HIPCHK(ctx, dalloc(&s->inv_rec, 3 * (int64_t)n, st));
+        s->inv_cap = n;
+        HIPCHK(ctx, mavi::launch_invasions(s, false, st));
+    }
+}
+
+// update_invasions! (src/rings/integration.jl:511-520) at the start of a
+// step: num_steps - last_check >= steps_to_update.
+static bool invasions_due(const mavi_sys *s, int64_t ahead) {
+    return s->inv_sched && s->inv_since + ahead >= s->inv_every;
+}
+
 static int steps_impl(mavi_sys *s, int64_t nsteps, const double *noise_host) {
     mavi_ctx *ctx = s->ctx;
     if (nsteps < 0) return set_err(ctx, MAVI_E_BADARG, "nsteps < 0");
     if (!s->has_state) return set_err(ctx, MAVI_E_BADARG, "no state uploaded");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
+    // a due invasion check runs after the step's rings kernel, which leaves
+    // cms and cpos as the reference's step sees them before its update
+    auto after_step = [&](bool due) -> int {
+        if (!s->inv_sched) return MAVI_OK;
+        s->inv_since = due ? 1 : s->inv_since + 1;
+        return due ? run_invasions(s) : MAVI_OK;
+    };
     if (noise_host) {
         // draws per step: one per ring (rings), one (Szabo) or two (RTP) per particle
         const int64_t nr = s->P.potential == MAVI_POT_RINGS ? s->P.nrings
@@ -577,23 +629,32 @@ static int steps_impl(mavi_sys *s, int64_t nsteps, const double *noise_host) {
         }
         HIPCHK(ctx, hipMemcpyAsync(s->noise, noise_host, sizeof(double) * need, hipMemcpyHostToDevice, st));
         for (int64_t k = 0; k < nsteps; k++) {
+            const bool due = invasions_due(s, 0);
             if (s->P.potential == MAVI_POT_RINGS) HIPCHK(ctx, mavi::launch_rings_step(s, s->noise + k * nr, st));
             else HIPCHK(ctx, mavi::launch_active_step(s, s->noise + k * nr, st));
+            const int rc = after_step(due);
+            if (rc != MAVI_OK) return rc;
         }
     } else {
-        int64_t k = 0;
-        // only Verlet steps carry forces from one step to the next
+        // only Verlet steps carry forces from one step to the next; the
+        // two-step graph replays from state copy 0 and holds no invasion check
         const bool rings = s->P.potential == MAVI_POT_RINGS || is_active(s);
         const bool graphs = s->use_graphs && !s->prof && nsteps >= 4;
-        while (graphs && k < nsteps && (s->cur != 0 || (!rings && !s->f_valid))) {
+        int64_t k = 0;
+        while (k < nsteps) {
+            if (graphs && k + 2 <= nsteps && s->cur == 0 && (rings || s->f_valid) && !invasions_due(s, 1)) {
+                HIPCHK(ctx, ensure_graph(s, st));
+                HIPCHK(ctx, hipGraphLaunch(s->graph2, st));
+                if (s->inv_sched) s->inv_since += 2;
+                k += 2;
+                continue;
+            }
+            const bool due = invasions_due(s, 0);
             HIPCHK(ctx, launch_step(s, st, s->f_valid));
+            const int rc = after_step(due);
+            if (rc != MAVI_OK) return rc;
             k++;
         }
-        if (graphs && (rings || s->f_valid)) {
-            HIPCHK(ctx, ensure_graph(s, st));
-            for (; k + 2 <= nsteps; k += 2) HIPCHK(ctx, hipGraphLaunch(s->graph2, st));
-        }
-        for (; k < nsteps; k++) HIPCHK(ctx, launch_step(s, st, s->f_valid));
     }
     int rc = sync_staging(s, true);
     if (rc != MAVI_OK) return rc;
@@ -670,6 +731,96 @@ int mavi_rings_neighbors_get(mavi_sys *s, int64_t *counts, int64_t *lists) {
         if (lists)
             for (int k = 0; k < 15; k++) lists[i * 15 + k] = l[i * 15 + k];
     }
+    return MAVI_OK;
+}
+
+int mavi_rings_invasions_cfg(mavi_sys *s, int scheduled, int64_t steps_to_update, int32_t r_cols, int32_t r_rows,
+                             int32_t r_nc) {
+    if (!s) return MAVI_E_BADARG;
+    mavi_ctx *ctx = s->ctx;
+    if (s->P.potential != MAVI_POT_RINGS) return set_err(ctx, MAVI_E_BADARG, "not a rings system");
+    if (r_cols < 0 || r_rows < 0 || (r_cols == 0) != (r_rows == 0))
+        return set_err(ctx, MAVI_E_BADARG, "ring chunk grid: give both r_cols and r_rows (0, 0: no grid)");
+    if (r_cols > 0 && r_nc < 1) return set_err(ctx, MAVI_E_BADARG, "ring chunk capacity must be >= 1");
+    if (r_cols > 0 && s->P.periodic && (r_cols < 3 || r_rows < 3))
+        return set_err(ctx, MAVI_E_UNSUPPORTED,
+                       "periodic ring chunk grids need >= 3 chunks per side (the reference stencil repeats "
+                       "neighbours below that)");
+    if (r_cols > 0 && (int64_t)r_cols * r_rows * r_nc > INT32_MAX)
+        return set_err(ctx, MAVI_E_BADARG, "ring chunk grid too large");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int32_t nr = s->P.nrings;
+    if (!s->inv_box) {
+        HIPCHK(ctx, dalloc(&s->inv_box, nr, st));
+        HIPCHK(ctx, dalloc(&s->inv_chunk, nr, st));
+        HIPCHK(ctx, dalloc(&s->inv_n, 1, st));
+        const int64_t cap = s->P.n > 1024 ? s->P.n : 1024;
+        HIPCHK(ctx, dalloc(&s->inv_rec, 3 * cap, st));
+        s->inv_cap = cap;
+    }
+    const int64_t ncell = (int64_t)r_cols * r_rows;
+    if (r_cols > 0 && ncell > (int64_t)s->inv_cnt_cap) {
+        dfree(s->inv_cnt);
+        s->inv_cnt_cap = 0;
+        HIPCHK(ctx, dalloc(&s->inv_cnt, ncell, st));
+        s->inv_cnt_cap = (int32_t)ncell;
+    }
+    if (r_cols > 0 && ncell * r_nc > (int64_t)s->inv_cell_cap) {
+        dfree(s->inv_cell);
+        s->inv_cell_cap = 0;
+        HIPCHK(ctx, dalloc(&s->inv_cell, ncell * r_nc, st));
+        s->inv_cell_cap = (int32_t)(ncell * r_nc);
+    }
+    InvGrid G{};
+    G.rows = r_rows;
+    G.cols = r_cols;
+    G.nc = r_nc;
+    if (r_cols > 0) {  // chunk_length / chunk_height of the ring Chunks, src/chunks.jl:29-30
+        G.cl = s->P.L / (double)r_cols;
+        G.ch = s->P.H / (double)r_rows;
+    }
+    s->inv_grid = G;
+    s->inv_sched = scheduled != 0;
+    s->inv_every = steps_to_update;
+    s->inv_since = 0;  // last_check = num_steps (src/rings/rings.jl:276-278)
+    s->inv_count = 0;  // InvasionsInfo(): an empty list
+    // the constructor's update_invasions! checks at once only when
+    // steps_to_update <= 0
+    if (s->inv_sched && s->has_state && steps_to_update <= 0) return run_invasions(s);
+    return MAVI_OK;
+}
+
+int mavi_rings_invasions_check(mavi_sys *s) {
+    if (!s) return MAVI_E_BADARG;
+    mavi_ctx *ctx = s->ctx;
+    if (!s->inv_box) return set_err(ctx, MAVI_E_BADARG, "invasions are not configured (mavi_rings_invasions_cfg)");
+    if (!s->has_state) return set_err(ctx, MAVI_E_BADARG, "no state uploaded");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    return run_invasions(s);
+}
+
+int mavi_rings_invasions_get(mavi_sys *s, int64_t *count, int64_t *records, int64_t cap) {
+    if (!s || !count) return MAVI_E_BADARG;
+    mavi_ctx *ctx = s->ctx;
+    if (!s->inv_box) return set_err(ctx, MAVI_E_BADARG, "invasions are not configured (mavi_rings_invasions_cfg)");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    const int64_t n = s->inv_count;
+    *count = n;
+    if (!records || n == 0) return MAVI_OK;
+    std::vector<int32_t> r((size_t)(3 * n));
+    HIPCHK(ctx, hipMemcpy(r.data(), s->inv_rec, sizeof(int32_t) * 3 * n, hipMemcpyDeviceToHost));
+    std::vector<int64_t> order((size_t)n);
+    for (int64_t i = 0; i < n; i++) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+        for (int k = 0; k < 3; k++)
+            if (r[3 * a + k] != r[3 * b + k]) return r[3 * a + k] < r[3 * b + k];
+        return false;
+    });
+    const int64_t m = n < cap ? n : cap;
+    for (int64_t i = 0; i < m; i++)
+        for (int k = 0; k < 3; k++) records[3 * i + k] = r[3 * order[i] + k];
     return MAVI_OK;
 }
 

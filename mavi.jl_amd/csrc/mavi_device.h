@@ -32,24 +32,32 @@ __device__ __forceinline__ double rcp_nr(double x) {
 
 __device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
 
-// Cell of one position, update_particle_chunk!, src/chunks.jl:120-142.
-__device__ __forceinline__ int32_t cell_of(const SysParams &P, double x, double y, uint32_t *flag) {
-    double qr = exact_trunc_div((-y + P.bly) + P.H, P.ch);
-    double qc = exact_trunc_div(x - P.blx, P.cl);
+// Cell of one position, update_particle_chunk!, src/chunks.jl:120-142, on a
+// rows x cols grid of cl x ch chunks over the chunk space.  A position off
+// the grid raises out_flag and is clamped so that memory accesses stay in
+// bounds.
+__device__ __forceinline__ int32_t cell_in_grid(const SysParams &P, int32_t rows, int32_t cols, double cl,
+                                                double ch, double x, double y, uint32_t *flag, uint32_t out_flag) {
+    double qr = exact_trunc_div((-y + P.bly) + P.H, ch);
+    double qc = exact_trunc_div(x - P.blx, cl);
     if (!(fabs(qr) < 1e15) || !(fabs(qc) < 1e15)) {  // NaN / Inf / absurd
         *flag |= MAVI_FLAG_NONFINITE;
         return 0;
     }
     int64_t r = (int64_t)qr;  // 0-based (reference row - 1)
     int64_t c = (int64_t)qc;
-    r -= (r == P.rows) ? 1 : 0;  // the N+1 -> N clamp
-    c -= (c == P.cols) ? 1 : 0;
-    if (r < 0 || r >= P.rows || c < 0 || c >= P.cols) {
-        *flag |= MAVI_FLAG_OUT_OF_BOX;
-        r = r < 0 ? 0 : (r >= P.rows ? P.rows - 1 : r);  // keep memory access in bounds
-        c = c < 0 ? 0 : (c >= P.cols ? P.cols - 1 : c);
+    r -= (r == rows) ? 1 : 0;  // the N+1 -> N clamp
+    c -= (c == cols) ? 1 : 0;
+    if (r < 0 || r >= rows || c < 0 || c >= cols) {
+        *flag |= out_flag;
+        r = r < 0 ? 0 : (r >= rows ? rows - 1 : r);
+        c = c < 0 ? 0 : (c >= cols ? cols - 1 : c);
     }
-    return (int32_t)(r * P.cols + c);
+    return (int32_t)(r * cols + c);
+}
+
+__device__ __forceinline__ int32_t cell_of(const SysParams &P, double x, double y, uint32_t *flag) {
+    return cell_in_grid(P, P.rows, P.cols, P.cl, P.ch, x, y, flag, MAVI_FLAG_OUT_OF_BOX);
 }
 
 // Arrival rank of each lane's particle in its chunk k.  Consecutive lanes

@@ -34,6 +34,8 @@
 #define MAVI_FLAG_NONFINITE 4u
 #define MAVI_FLAG_SCAN_STALL 8u
 #define MAVI_FLAG_NEIGH_OVERFLOW 16u
+#define MAVI_FLAG_RING_OUT_OF_BOX 32u
+#define MAVI_FLAG_RING_CHUNK_OVERFLOW 64u
 
 struct mavi_ctx {
     int device = 0;
@@ -104,6 +106,13 @@ struct RingInter {
     double nb2;    // contact neighbours: dist < 2 particle_radius * tol <=> r2 < nb2
 };
 
+// Chunk grid of the rings' centres of mass (r_chunks, src/rings/rings.jl:
+// 188-202) over the system's chunk space; cols == 0: no grid (all pairs).
+struct InvGrid {
+    int32_t rows, cols, nc, pad;
+    double cl, ch;
+};
+
 struct ProfSlot {
     double ms = 0.0;
     int64_t launches = 0;
@@ -152,6 +161,16 @@ struct mavi_sys {
     double neigh_tol = 1.1;
     int32_t *neigh_cnt = nullptr, *neigh_list = nullptr;  // by particle id
     std::vector<RingInter> host_inter;
+    // invasions of rings (InvasionsCfg, r_chunks)
+    bool inv_sched = false;     // InvasionsCfg given: check every inv_every steps
+    int64_t inv_every = 0, inv_since = 0;  // steps_to_update; num_steps - last_check
+    InvGrid inv_grid{};
+    int32_t *inv_cnt = nullptr, *inv_cell = nullptr, *inv_chunk = nullptr;  // ring grid
+    double4 *inv_box = nullptr; // per ring: bounding box of its ring points
+    int32_t *inv_rec = nullptr; // records (invasor, invaded, p_id), 1-based
+    uint32_t *inv_n = nullptr;  // records found by the last check
+    int64_t inv_cap = 0, inv_count = 0;
+    int32_t inv_cnt_cap = 0, inv_cell_cap = 0;  // allocated ring chunks / chunk slots
     DevWall *walls = nullptr;   // walls list (P.walls / P.lines point here)
     DevLine *lines = nullptr;
     int32_t *needc = nullptr;   // per chunk: binning epoch in which a chunk of its 3x3 block changed
@@ -217,6 +236,9 @@ hipError_t launch_active_step(mavi_sys *s, const double *noise_dev, hipStream_t 
 hipError_t launch_rings_forces(mavi_sys *s, hipStream_t st);
 hipError_t launch_rings_init(mavi_sys *s, hipStream_t st);
 hipError_t launch_ring_neigh(mavi_sys *s, hipStream_t st);
+// check_invasions!: ring boxes and the ring grid, then the crossing tests;
+// prep = false reruns only the tests (after the record buffer grew)
+hipError_t launch_invasions(mavi_sys *s, bool prep, hipStream_t st);
 hipError_t launch_observables(mavi_sys *s, double *d_out, hipStream_t st);
 hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t st);
 }  // namespace mavi

@@ -257,6 +257,139 @@ __global__ void k_ring_neigh(SysParams P, RingCtx R, const double2 *__restrict__
 }
 
 // ---------------------------------------------------------------------------
+// Invasions of rings: check_invasions! (src/rings/integration.jl:379-509) at
+// the ring points (continuous positions cpos) and centres of mass cms the
+// last forces! step saw.
+// k_inv_prep: one thread per ring: the bounding box of its points and, with
+// a ring grid, its chunk (update_particle_chunk! of the centre of mass)
+// appended to that chunk's list.  The order inside a chunk does not matter:
+// the host sorts the records.
+// k_invasions: one wave per ring r1, lanes over r1's points.  Every other
+// ring r2 of the 3x3 block of r1's chunk (every ring without a grid) is
+// tested: the reference visits each unordered pair of rings in the same or
+// adjacent chunks once and tests both directions, the gather visits each
+// ordered pair once.  A point can count a crossing of r2 only when its
+// height is strictly inside r2's box and it lies left of the box's right
+// side (the height tests of point_line_intersect are strict, and an
+// intersection lies between the segment's end abscissas), so boxes skip
+// pairs and points without changing any count.  An odd count is an
+// invasion record (invasor r1, invaded r2, p_id), 1-based.
+// ---------------------------------------------------------------------------
+__global__ void k_inv_prep(SysParams P, InvGrid G, const int32_t *__restrict__ rtype,
+                           const RingType *__restrict__ types, const double2 *__restrict__ cpos,
+                           const double2 *__restrict__ cms, double4 *__restrict__ box,
+                           int32_t *__restrict__ gcnt, int32_t *__restrict__ gcell, int32_t *__restrict__ gchunk,
+                           uint32_t *__restrict__ flags) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= P.nrings) return;
+    const int np = types[P.typed ? rtype[r] : 0].np;
+    const double2 *c = cpos + (int64_t)r * P.nmax;
+    double x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
+    for (int i = 0; i < np; i++) {
+        const double2 q = c[i];
+        x0 = fmin(x0, q.x);
+        x1 = fmax(x1, q.x);
+        y0 = fmin(y0, q.y);
+        y1 = fmax(y1, q.y);
+    }
+    box[r] = make_double4(x0, x1, y0, y1);
+    if (G.cols == 0) return;
+    uint32_t f = 0;
+    const double2 m = cms[r];
+    const int k = cell_in_grid(P, G.rows, G.cols, G.cl, G.ch, m.x, m.y, &f, MAVI_FLAG_RING_OUT_OF_BOX);
+    gchunk[r] = k;
+    const int slot = atomicAdd(gcnt + k, 1);
+    if (slot < G.nc) gcell[(int64_t)k * G.nc + slot] = r;
+    else f |= MAVI_FLAG_RING_CHUNK_OVERFLOW;  // BoundsError on chunk_particles
+    if (f) atomicOr(flags, f);
+}
+
+// point_line_intersect(p, (a, b)), src/rings/integration.jl:379-419, with
+// the reference's operations in its order (the library is built without
+// FP contraction; the division is correctly rounded).
+__device__ __forceinline__ int ray_crosses(const double2 p, const double2 a, const double2 b) {
+    const double dx = b.x - a.x;
+    const double dy = b.y - a.y;
+    const double ylo = dy < 0 ? b.y : a.y, yhi = dy < 0 ? a.y : b.y;
+    if (!(ylo < p.y && p.y < yhi)) return 0;
+    if (dx == 0) return a.x > p.x ? 1 : 0;
+    if (dy == 0) return 0;
+    const double c = dy * a.x - dx * a.y;
+    const double x_inter = (c + dx * p.y) / dy;
+    const double xlo = dx < 0 ? b.x : a.x, xhi = dx < 0 ? a.x : b.x;
+    return (x_inter > p.x && xlo < x_inter && x_inter < xhi) ? 1 : 0;
+}
+
+constexpr int kInvWaves = 4;
+
+__global__ __launch_bounds__(64 * kInvWaves) void k_invasions(
+    SysParams P, InvGrid G, const int32_t *__restrict__ rtype, const RingType *__restrict__ types,
+    const double2 *__restrict__ cpos, const double4 *__restrict__ box, const int32_t *__restrict__ gcnt,
+    const int32_t *__restrict__ gcell, const int32_t *__restrict__ gchunk, int32_t *__restrict__ rec,
+    uint32_t *__restrict__ nrec, const int64_t cap) {
+    const int r1 = blockIdx.x * kInvWaves + (int)(threadIdx.x >> 6);
+    if (r1 >= P.nrings) return;  // whole waves leave together
+    const unsigned lane = threadIdx.x & 63;
+    const int np1 = types[P.typed ? rtype[r1] : 0].np;
+    const double4 b1 = box[r1];
+    const double2 *c1 = cpos + (int64_t)r1 * P.nmax;
+    auto test = [&](const int r2, const double2 p, const bool act, const int i) {
+        if (r2 == r1) return;
+        const double4 b2 = box[r2];
+        if (b1.w <= b2.z || b1.z >= b2.w || b1.x >= b2.y) return;  // no point of r1 can count
+        const bool cand = act && p.y > b2.z && p.y < b2.w && p.x < b2.y;
+        if (__ballot(cand) == 0) return;
+        const int np2 = types[P.typed ? rtype[r2] : 0].np;
+        const double2 *c2 = cpos + (int64_t)r2 * P.nmax;
+        int cross = 0;
+        double2 a = c2[0];
+        for (int e = 0; e < np2; e++) {
+            const double2 b = c2[e + 1 < np2 ? e + 1 : 0];
+            cross += ray_crosses(p, a, b);
+            a = b;
+        }
+        const bool odd = cand && (cross & 1);
+        const unsigned long long m = __ballot(odd);
+        if (m == 0) return;
+        uint32_t base = 0;
+        if (lane == (unsigned)(__ffsll((long long)m) - 1)) base = atomicAdd(nrec, (uint32_t)__popcll(m));
+        base = __shfl(base, __ffsll((long long)m) - 1, 64);
+        if (odd) {
+            const int64_t k = (int64_t)base + __popcll(m & ((1ull << lane) - 1));
+            if (k < cap) {
+                rec[3 * k + 0] = r1 + 1;
+                rec[3 * k + 1] = r2 + 1;
+                rec[3 * k + 2] = r1 * P.nmax + i + 1;  // to_scalar_idx(state, r1, i)
+            }
+        }
+    };
+    for (int i0 = 0; i0 < np1; i0 += 64) {
+        const int i = i0 + (int)lane;
+        const bool act = i < np1;
+        const double2 p = act ? c1[i] : make_double2(0.0, 0.0);
+        if (G.cols == 0) {
+            for (int r2 = 0; r2 < P.nrings; r2++) test(r2, p, act, i);
+            continue;
+        }
+        const int k = gchunk[r1];
+        const int row = k / G.cols, col = k - row * G.cols;
+        for (int dr = -1; dr <= 1; dr++) {
+            int rr = row + dr;
+            if (P.periodic) rr = rr < 0 ? rr + G.rows : (rr >= G.rows ? rr - G.rows : rr);
+            else if (rr < 0 || rr >= G.rows) continue;
+            for (int dc = -1; dc <= 1; dc++) {
+                int cc = col + dc;
+                if (P.periodic) cc = cc < 0 ? cc + G.cols : (cc >= G.cols ? cc - G.cols : cc);
+                else if (cc < 0 || cc >= G.cols) continue;
+                const int kk = rr * G.cols + cc;
+                const int n = min(gcnt[kk], G.nc);
+                for (int j = 0; j < n; j++) test(gcell[(int64_t)kk * G.nc + j], p, act, i);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Self-propelled particles: szabo_step! / rtp_step! (src/integration.jl:
 // 517-535).  One launch after the binning: pair forces at the binned
 // positions (Szabo, or the truncated LJ of run-and-tumble), then, fused,
@@ -950,6 +1083,23 @@ static hipError_t launch_rings_kernel(mavi_sys *s, bool update, const double *no
 hipError_t launch_ring_neigh(mavi_sys *s, hipStream_t st) {
     hipLaunchKernelGGL(k_ring_neigh, dim3(nblk(s->P.na, 256)), dim3(256), 0, st, s->P, ring_ctx(s), s->P_[s->cur],
                        s->start, s->scell, s->neigh_only_rings, s->neigh_cnt, s->neigh_list, s->flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_invasions(mavi_sys *s, bool prep, hipStream_t st) {
+    const InvGrid G = s->inv_grid;
+    if (prep) {
+        if (G.cols > 0) {
+            hipError_t e = launch_fill_i32(s->inv_cnt, 0, (int64_t)G.rows * G.cols, st);
+            if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(k_inv_prep, dim3(nblk(s->P.nrings, 256)), dim3(256), 0, st, s->P, G, s->rtype, s->types,
+                           s->cpos, s->cms, s->inv_box, s->inv_cnt, s->inv_cell, s->inv_chunk, s->flags);
+    }
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, (int32_t *)s->inv_n, 0, (int64_t)1);
+    hipLaunchKernelGGL(k_invasions, dim3(nblk(s->P.nrings, kInvWaves)), dim3(64 * kInvWaves), 0, st, s->P, G,
+                       s->rtype, s->types, s->cpos, s->inv_box, s->inv_cnt, s->inv_cell, s->inv_chunk, s->inv_rec,
+                       s->inv_n, s->inv_cap);
     return hipGetLastError();
 }
 

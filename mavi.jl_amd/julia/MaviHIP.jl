@@ -263,6 +263,42 @@ function Integration.get_step_function(::StandardSys, system, ::Val{:cpu})
     return Integration.newton_step!
 end
 
+# --- rings: invasions (InvasionsInfo, src/rings/rings.jl:86-100) -------------
+
+"""Hand a RingsSystem's r_chunks and InvasionsCfg to the device
+(src/rings/rings.jl:188-202, 276-284): later device steps check for
+invasions on the reference's schedule."""
+function invasions_cfg!(system, d::DeviceSys)
+    extra = system.int_cfg.extra
+    rc = system.info.r_chunks
+    sched = !isnothing(extra.invasions_cfg)
+    (sched || !isnothing(rc)) || return nothing
+    cols = isnothing(rc) ? 0 : rc.num_cols
+    rows = isnothing(rc) ? 0 : rc.num_rows
+    nc = isnothing(rc) ? 0 : size(rc.chunk_particles, 1)
+    every = sched ? extra.invasions_cfg.steps_to_update : 0
+    check(d.ctx, ccall((:mavi_rings_invasions_cfg, LIB), Cint, (Ptr{Cvoid}, Cint, Int64, Int32, Int32, Int32),
+                       d.sys, sched ? 1 : 0, every, cols, rows, nc))
+end
+
+"""system.info.invasions.list <- the device's last check (the same set as
+check_invasions!, ordered by invasor, invaded, p_id)."""
+function invasions_download!(system, d::DeviceSys)
+    count = Ref{Int64}(0)
+    check(d.ctx, ccall((:mavi_rings_invasions_get, LIB), Cint, (Ptr{Cvoid}, Ref{Int64}, Ptr{Int64}, Int64),
+                       d.sys, count, C_NULL, 0))
+    rec = Vector{Int64}(undef, 3 * count[])
+    GC.@preserve rec check(d.ctx, ccall((:mavi_rings_invasions_get, LIB), Cint,
+                                        (Ptr{Cvoid}, Ref{Int64}, Ptr{Int64}, Int64), d.sys, count, pointer(rec),
+                                        count[]))
+    list = system.info.invasions.list
+    empty!(list)
+    for k in 0:count[]-1
+        push!(list, Mavi.Rings.Invasion(invasor=rec[3k+1], invaded=rec[3k+2], p_id=rec[3k+3]))
+    end
+    return list
+end
+
 "run_system with a resident fast path: upload once, all steps on the device, download once."
 function run_gpu!(system; tf=nothing, num_steps=nothing)
     n = if isnothing(tf)

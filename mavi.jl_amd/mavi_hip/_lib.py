@@ -133,6 +133,9 @@ SIGNATURES = {
     "mavi_rings_info": (C.c_int, [_P, _DP, _DP, _DP]),
     "mavi_rings_neighbors_cfg": (C.c_int, [_P, C.c_int, C.c_double, C.c_int]),
     "mavi_rings_neighbors_get": (C.c_int, [_P, _I64P, _I64P]),
+    "mavi_rings_invasions_cfg": (C.c_int, [_P, C.c_int, C.c_int64, C.c_int32, C.c_int32, C.c_int32]),
+    "mavi_rings_invasions_check": (C.c_int, [_P]),
+    "mavi_rings_invasions_get": (C.c_int, [_P, _I64P, _I64P, C.c_int64]),
     "mavi_observables": (C.c_int, [_P, _DP, _DP]),
     "mavi_run_record": (C.c_int, [_P, C.c_int64, _DP, _DP]),
     "mavi_prof_enable": (C.c_int, [_P, C.c_int]),
@@ -309,6 +312,24 @@ class DeviceSystem:
         lists = np.zeros((self.n, 15), dtype=np.int64)
         self._call("mavi_rings_neighbors_get", i64ptr(counts), i64ptr(lists))
         return counts, lists
+
+    def rings_invasions_cfg(self, scheduled: bool, steps_to_update: int = 0, r_cols: int = 0, r_rows: int = 0,
+                            r_nc: int = 0):
+        self._call("mavi_rings_invasions_cfg", 1 if scheduled else 0, C.c_int64(int(steps_to_update)),
+                   int(r_cols), int(r_rows), int(r_nc))
+
+    def rings_invasions_check(self):
+        self._call("mavi_rings_invasions_check")
+
+    def rings_invasions(self) -> np.ndarray:
+        """(k, 3) int64 records (invasor, invaded, p_id) of the last check,
+        1-based, sorted."""
+        count = np.zeros(1, dtype=np.int64)
+        self._call("mavi_rings_invasions_get", i64ptr(count), None, C.c_int64(0))
+        rec = np.zeros((int(count[0]), 3), dtype=np.int64)
+        if len(rec):
+            self._call("mavi_rings_invasions_get", i64ptr(count), i64ptr(rec), C.c_int64(len(rec)))
+        return rec
 
     def rings_info(self):
         areas = np.zeros(self.nrings)

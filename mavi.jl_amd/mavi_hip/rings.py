@@ -6,6 +6,9 @@
   step(system)                                Rings.Integration.step!, src/rings/integration.jl:522-543
   rectangular_grid, create_circle, random_pol src/rings/init_states.jl
   get_ring_radius                             src/rings/utils.jl:5-7
+  InvasionsCfg, IntCfgExtra, RingsIntCfg      src/rings/configs.jl:334-350
+  Invasion, RingsSystem.invasions             src/rings/rings.jl:86-100
+  equilibrium_area, equilibrium_ring_radius   src/rings/configs.jl:221-328
 
 Host layout: ``rings_pos`` is an (nrings, nmax, 2) array, i.e. Julia's
 Matrix{SVector{2,T}}(nmax, nrings) in memory order, and ``pos`` is its
@@ -19,7 +22,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from .configs import GPU, PeriodicWalls, RectangleCfg, SpaceCfg, chunk_capacity, get_bounding_box
+from .configs import GPU, ChunksCfg, IntCfg, PeriodicWalls, RectangleCfg, SpaceCfg, chunk_capacity, get_bounding_box
 from .systems import Chunks, TimeInfo, attach_walls, base_desc, check_inside, wall_kind, wall_records
 
 
@@ -102,6 +105,9 @@ class RingsState:  # src/rings/states.jl:74-124
             return int(self.num_particles)
         return int(np.asarray(self.num_particles)[self.types[ring_id - 1] - 1])
 
+    def ring_num_particles_of_type(self, type_):  # 1-based type
+        return int(np.atleast_1d(np.asarray(self.num_particles))[type_ - 1 if self.types is not None else 0])
+
     def particle_ids(self) -> np.ndarray:
         """get_particles_ids: FixRingsIds.p_ids (0-based slots)."""
         if self.types is None:
@@ -151,6 +157,99 @@ def describe(state: RingsState, space_cfg: SpaceCfg, dynamic_cfg: RingsCfg, int_
     return d, chunks, tables
 
 
+# --- integration configs and invasions (src/rings/configs.jl:334-350,
+# src/rings/rings.jl:86-100) ---------------------------------------------
+@dataclass
+class InvasionsCfg:
+    """Check for invasions every steps_to_update steps."""
+
+    steps_to_update: int
+
+
+@dataclass
+class IntCfgExtra:
+    r_chunks_cfg: ChunksCfg | None = None
+    invasions_cfg: InvasionsCfg | None = None
+
+
+def RingsIntCfg(*, dt, p_chunks_cfg=None, r_chunks_cfg=None, invasions_cfg=None, device=None) -> IntCfg:
+    """IntCfg of a rings system: particle chunks, ring chunks (binning the
+    rings' centres of mass) and the invasion check schedule."""
+    return IntCfg(dt=dt, chunks_cfg=p_chunks_cfg, device=device or GPU(),
+                  extra=IntCfgExtra(r_chunks_cfg, invasions_cfg))
+
+
+@dataclass(frozen=True)
+class Invasion:
+    """Particle p_id (scalar index, 1-based) of ring `invasor` lies inside
+    ring `invaded`."""
+
+    invasor: int
+    invaded: int
+    p_id: int
+
+
+def _type_prop(cfg: RingsCfg, name: str, type_: int) -> float:
+    return float(getattr(cfg, name)[type_ - 1 if cfg.num_types > 1 else 0])
+
+
+def equilibrium_area(cfg: RingsCfg, num_particles: int, type_: int = 1) -> float:
+    """get_equilibrium_area (src/rings/configs.jl:246-293): area0 below the
+    springs-only limit, otherwise the root of the area/spring force balance.
+    The reference solves that balance with NLsolve.jl 4.5.1 (nlsolve from
+    f = sqrt(0.5)); this takes Newton steps on the same scalar equation from
+    the same start, which reaches the same root to the solver's tolerance."""
+    n = num_particles
+    l_spring, p0 = _type_prop(cfg, "l_spring", type_), _type_prop(cfg, "p0", type_)
+    a0 = (n * l_spring / p0) ** 2  # get_area0, :221-226
+    theta = 2 * math.pi / n
+    p0_lim = 2 * (n * (1 - math.cos(theta)) / math.sin(theta)) ** 0.5  # get_equilibrium_p0, :233-237
+    if a0 < (n * l_spring / p0_lim) ** 2:
+        return a0
+    k_a, k_m = _type_prop(cfg, "k_area", type_), _type_prop(cfg, "k_spring", type_)
+
+    def get_r(f):
+        return math.sqrt(f * 2 * a0 / (n * math.sin(theta)))
+
+    def balance(x):
+        f = x * x
+        r = get_r(f)
+        fa = k_a * (a0 - 10 / 2 * r ** 2 * math.sin(theta)) * r * math.sin(theta)
+        fm = k_m * (r * math.sqrt(2 * (1 - math.cos(theta))) - l_spring)
+        return fa - 2 * fm * math.sin(theta / 2)
+
+    x = math.sqrt(0.5)
+    for _ in range(100):
+        h = 1e-7 * max(1.0, abs(x))
+        d = (balance(x + h) - balance(x - h)) / (2 * h)
+        if d == 0.0:
+            break
+        step = balance(x) / d
+        x -= step
+        if abs(step) <= 1e-14 * max(1.0, abs(x)):
+            break
+    a0_sol = x * x * a0
+    return a0 if a0_sol < a0 else a0_sol
+
+
+def equilibrium_ring_radius(cfg: RingsCfg, num_particles: int, type_: int = 1) -> float:
+    """get_ring_radius(dynamic_cfg, type), src/rings/configs.jl:320-328: from
+    the centre to the outer edge of the particles at the equilibrium area."""
+    area_eq = equilibrium_area(cfg, num_particles, type_)
+    n = num_particles
+    return (2 * area_eq / (n * math.sin(2 * math.pi / n))) ** 0.5 + cfg.particle_radius(type_)
+
+
+def ring_chunks_grid(state: RingsState, space_cfg: SpaceCfg, cfg: RingsCfg, r_chunks_cfg: ChunksCfg):
+    """(cols, rows, nc) of r_chunks, src/rings/rings.jl:188-202: chunks over
+    the bounding box, capacity from the smallest equilibrium ring radius."""
+    g = get_bounding_box(space_cfg.geometry_cfg)
+    cols, rows = r_chunks_cfg.num_cols, r_chunks_cfg.num_rows
+    rings_r = min(equilibrium_ring_radius(cfg, state.ring_num_particles_of_type(t + 1), t + 1)
+                  for t in range(cfg.num_types))
+    return cols, rows, chunk_capacity(g.length / cols, g.height / rows, rings_r)
+
+
 @dataclass
 class NeighborsCfg:  # src/rings/neighbors.jl:11-15
     """Contact neighbours of the particles: type "all" or "rings" (pairs of
@@ -180,10 +279,40 @@ class RingsSystem:
                 raise ValueError(p_neighbors_cfg.type)
             self.dev.rings_neighbors_cfg(True, p_neighbors_cfg.tol, p_neighbors_cfg.type == "rings")
         self.forces = np.zeros_like(state.pos)
+        extra = getattr(int_cfg, "extra", None)
+        self.r_chunks_grid = None
+        self.invasions_cfg = getattr(extra, "invasions_cfg", None)
+        if getattr(extra, "r_chunks_cfg", None) is not None:
+            self.r_chunks_grid = ring_chunks_grid(state, space_cfg, dynamic_cfg, extra.r_chunks_cfg)
         self.upload()
-        # the constructor's init sequence (src/rings/rings.jl:280-288) ends with forces!
+        # the constructor's init sequence (src/rings/rings.jl:270-288): the
+        # invasion schedule starts at this step count, then forces!
+        self._invasions_cfg_call()
         self.forces[:] = self.dev.forces()
         self.areas, self.cms, self.continuos_pos = self.dev.rings_info()
+
+    def _invasions_cfg_call(self):
+        if self.invasions_cfg is None and self.r_chunks_grid is None:
+            return
+        cols, rows, nc = self.r_chunks_grid or (0, 0, 0)
+        sched = self.invasions_cfg is not None
+        self.dev.rings_invasions_cfg(sched, self.invasions_cfg.steps_to_update if sched else 0, cols, rows, nc)
+
+    # --- invasions (InvasionsInfo.list, check_invasions!, src/rings/integration.jl:470-509)
+    @property
+    def invasions(self) -> list[Invasion]:
+        """The last check's invasions, ordered by (invasor, invaded, p_id)."""
+        if self.invasions_cfg is None and self.r_chunks_grid is None:
+            return []
+        return [Invasion(*map(int, r)) for r in self.dev.rings_invasions()]
+
+    def check_invasions(self) -> list[Invasion]:
+        """check_invasions!(system) at the last step's ring points."""
+        if self.invasions_cfg is None and self.r_chunks_grid is None:
+            self.r_chunks_grid = (0, 0, 0)
+            self._invasions_cfg_call()
+        self.dev.rings_invasions_check()
+        return self.invasions
 
     def upload(self):
         self.dev.upload(self.state.pos, None, self.state.pol)

@@ -39,6 +39,14 @@
  *                              polarisation turning towards the force,
  *                              var(dtheta) = 2 Dr dt, tumbles at u < rate dt
  *                              to 2 pi v exactly, periodic wrap.
+ *   or_rings_invasions         point_line_intersect: the parametric crossing
+ *                              of the +x ray with the open segment on random
+ *                              segments; hand polygons (square, concave L,
+ *                              a point level with a vertex, horizontal and
+ *                              vertical edges); ring pairs: a winding-number
+ *                              point-in-polygon over every chunk-adjacent
+ *                              pair (random overlapping rings), chunks =
+ *                              all pairs where the grid is coarse.
  *   or_rings_neighbors         brute force over all pairs on random ring
  *                              gases (same sets), counts = list lengths,
  *                              symmetry (j in list(i) <=> i in list(j)).
@@ -79,20 +87,26 @@ int64_t or_chunk_capacity(double cl, double ch, double r) {
     return (int64_t)trunc(ceil(nc * 2.0));
 }
 
-/* src/chunks.jl:120-142 */
-int or_cell(const or_sys *s, double x, double y, int64_t *row, int64_t *col) {
+/* update_particle_chunk!, src/chunks.jl:120-142, on a grid of rows x cols
+ * chunks of size cl x ch over the system's chunk space. */
+static int cell_in(const or_sys *s, int64_t rows, int64_t cols, double cl, double ch, double x, double y,
+                   int64_t *row, int64_t *col) {
     double H = s->size[1];
-    double qr = or_jl_div((-y + s->bl[1]) + H, s->ch);
-    double qc = or_jl_div(x - s->bl[0], s->cl);
+    double qr = or_jl_div((-y + s->bl[1]) + H, ch);
+    double qc = or_jl_div(x - s->bl[0], cl);
     if (!isfinite(qr) || !isfinite(qc)) return OR_E_OUT_OF_BOX; /* trunc(Int, NaN) throws */
     int64_t r = (int64_t)qr + 1;
     int64_t c = (int64_t)qc + 1;
-    r -= (r == (int64_t)s->rows + 1) ? 1 : 0;
-    c -= (c == (int64_t)s->cols + 1) ? 1 : 0;
+    r -= (r == rows + 1) ? 1 : 0;
+    c -= (c == cols + 1) ? 1 : 0;
     *row = r;
     *col = c;
-    if (r < 1 || r > s->rows || c < 1 || c > s->cols) return OR_E_OUT_OF_BOX; /* BoundsError */
+    if (r < 1 || r > rows || c < 1 || c > cols) return OR_E_OUT_OF_BOX; /* BoundsError */
     return OR_OK;
+}
+
+int or_cell(const or_sys *s, double x, double y, int64_t *row, int64_t *col) {
+    return cell_in(s, s->rows, s->cols, s->cl, s->ch, x, y, row, col);
 }
 
 /* get_particles_ids: all slots for plain states (src/states.jl:213-228 with
@@ -428,6 +442,106 @@ int or_rings_neighbors(const or_sys *s, const double *pos, double tol, int only_
                     int64_t nk = (nb[(k * 4 + e) * 2 + 1] - 1) * rows + (nb[(k * 4 + e) * 2] - 1);
                     for (int64_t j = 0; j < cnt[nk] && !err; j++)
                         err = neigh_pair(s, pos, chunk[i], cp[nc * nk + j], tol, only_rings, counts, lists);
+                }
+            }
+        }
+    free(nb);
+    free(nnb);
+    free(cp);
+    free(cnt);
+    return err;
+}
+
+/* point_line_intersect(p, line), src/rings/integration.jl:379-419: does the
+ * ray from p towards +x cross the segment (a, b)?  Strict comparisons as in
+ * the reference; a horizontal segment never counts. */
+int or_point_line_intersect(const double *p, const double *a, const double *b) {
+    const double dx = b[0] - a[0];
+    const double dy = b[1] - a[1];
+    const double *ylo = a, *yhi = b;
+    if (dy < 0) { ylo = b; yhi = a; }
+    const int in_y = (ylo[1] < p[1]) && (p[1] < yhi[1]);
+    if (dx == 0) return (a[0] > p[0]) && in_y;
+    if (dy == 0) return 0;
+    const double c = dy * a[0] - dx * a[1];
+    const double x_inter = (c + dx * p[1]) / dy;
+    const double *xlo = a, *xhi = b;
+    if (dx < 0) { xlo = b; xhi = a; }
+    return (x_inter > p[0]) && ((xlo[0] < x_inter) && (x_inter < xhi[0])) && in_y;
+}
+
+/* polygons_intersect(poly1, poly2) + the pushes of find_invasions!,
+ * src/rings/integration.jl:421-468: every point of ring `invasor` whose ray
+ * crosses ring `invaded`'s closed polygon an odd number of times. */
+static void ring_in_ring(const or_sys *s, const double *rpts, int32_t invasor, int32_t invaded, int64_t *rec,
+                         int64_t cap, int64_t *count) {
+    const int32_t n1 = ring_np(s, invasor), n2 = ring_np(s, invaded);
+    const double *p1 = rpts + 2 * ((int64_t)(invasor - 1) * s->nmax);
+    const double *p2 = rpts + 2 * ((int64_t)(invaded - 1) * s->nmax);
+    for (int32_t id = 1; id <= n1; id++) {
+        int64_t crossings = 0;
+        for (int32_t e1 = 1; e1 <= n2; e1++) {
+            const int32_t e2 = (e1 == n2) ? 1 : e1 + 1;
+            crossings += or_point_line_intersect(p1 + 2 * (id - 1), p2 + 2 * (e1 - 1), p2 + 2 * (e2 - 1));
+        }
+        if (crossings % 2 != 0) {
+            if (*count < cap) {
+                rec[3 * *count + 0] = invasor;
+                rec[3 * *count + 1] = invaded;
+                rec[3 * *count + 2] = (int64_t)(invasor - 1) * s->nmax + id; /* to_scalar_idx */
+            }
+            (*count)++;
+        }
+    }
+}
+
+static void find_invasions(const or_sys *s, const double *rpts, int32_t r1, int32_t r2, int64_t *rec,
+                           int64_t cap, int64_t *count) {
+    ring_in_ring(s, rpts, r1, r2, rec, cap, count);
+    ring_in_ring(s, rpts, r2, r1, rec, cap, count);
+}
+
+int or_rings_invasions(const or_sys *s, const double *rpts, const double *cms, int32_t rcols, int32_t rrows,
+                       int64_t rnc, int64_t *rec, int64_t cap, int64_t *count) {
+    if (s->kind != OR_KIND_RINGS) return OR_E_BADARG;
+    *count = 0;
+    if (rcols <= 0) { /* check_invasions!(system, ::Nothing), :498-507 */
+        for (int32_t r1 = 1; r1 <= s->nrings; r1++)
+            for (int32_t r2 = r1 + 1; r2 <= s->nrings; r2++) find_invasions(s, rpts, r1, r2, rec, cap, count);
+        return OR_OK;
+    }
+    /* r_chunks: Chunks over the centres of mass of get_rings_ids (every ring
+     * here), src/rings/rings.jl:188-202, binned by update_chunks!
+     * (src/rings/integration.jl:18-23) */
+    const int64_t rows = rrows, cols = rcols, ncell = rows * cols;
+    const double cl = s->size[0] / (double)rcols, ch = s->size[1] / (double)rrows;
+    int64_t *cp = (int64_t *)malloc(sizeof(int64_t) * (size_t)(rnc > 0 ? rnc : 1) * ncell);
+    int64_t *cnt = (int64_t *)calloc((size_t)ncell, sizeof(int64_t));
+    int err = OR_OK;
+    for (int32_t r = 1; r <= s->nrings && !err; r++) {
+        int64_t row, col;
+        err = cell_in(s, rows, cols, cl, ch, cms[2 * (r - 1)], cms[2 * (r - 1) + 1], &row, &col);
+        if (err) break;
+        const int64_t k = (row - 1) + rows * (col - 1);
+        if (cnt[k] + 1 > rnc) { err = OR_E_CELL_OVERFLOW; break; }
+        cp[rnc * k + cnt[k]] = r;
+        cnt[k]++;
+    }
+    int64_t *nb = (int64_t *)malloc(sizeof(int64_t) * (size_t)ncell * 8);
+    int32_t *nnb = (int32_t *)malloc(sizeof(int32_t) * (size_t)ncell);
+    or_neighbors(rrows, rcols, s->periodic, nb, nnb);
+    /* check_invasions!(system, chunks::Chunks), :470-496 */
+    for (int64_t col = 1; col <= cols && !err; col++)
+        for (int64_t row = 1; row <= rows; row++) {
+            const int64_t k = (col - 1) * rows + (row - 1);
+            const int64_t *chunk = cp + rnc * k;
+            for (int64_t i = 0; i < cnt[k]; i++) {
+                const int32_t r1 = (int32_t)chunk[i];
+                for (int64_t j = i + 1; j < cnt[k]; j++) find_invasions(s, rpts, r1, (int32_t)chunk[j], rec, cap, count);
+                for (int e = 0; e < nnb[k]; e++) {
+                    const int64_t nk = (nb[(k * 4 + e) * 2 + 1] - 1) * rows + (nb[(k * 4 + e) * 2] - 1);
+                    for (int64_t j = 0; j < cnt[nk]; j++)
+                        find_invasions(s, rpts, r1, (int32_t)cp[rnc * nk + j], rec, cap, count);
                 }
             }
         }
@@ -950,7 +1064,9 @@ int or_rings_steps(const or_sys *s, double *pos, double *pol, int64_t nsteps, co
      * step! uses it in update_cms! before recomputing it. */
     or_rings_continuous_pos(s, pos, cpos);
     for (int64_t step = 0; step < nsteps; step++) {
-        rings_cms(s, cpos, cms);                 /* update_cms! */
+        /* update_cms!: get_continuos_pos is info.continuos_pos (last computed)
+         * for a periodic main wall, the live rings_pos otherwise */
+        rings_cms(s, s->periodic ? cpos : pos, cms);
         or_rings_continuous_pos(s, pos, cpos);   /* update_chunks_all! then update_continuos_pos! */
         err = or_rings_forces(s, pos, cpos, forces, areas); /* cleaning! + forces! (bins inside) */
         if (err) break;

@@ -149,6 +149,20 @@ int or_newton_steps(const or_sys *s, double *pos, double *vel, double *forces,
 int or_rings_neighbors(const or_sys *s, const double *pos, double tol, int only_rings, int64_t *counts,
                        int64_t *lists);
 
+/* point_line_intersect, src/rings/integration.jl:379-419: 1 if the ray from
+ * p towards +x crosses the segment (a, b) under the reference's strict tests. */
+int or_point_line_intersect(const double *p, const double *a, const double *b);
+/* check_invasions!(system), src/rings/integration.jl:379-509, at the ring
+ * points rpts (ring_points: info.continuos_pos for a periodic main wall,
+ * state.rings_pos otherwise, 2*nmax*nrings) and centres of mass cms
+ * (2*nrings).  rcols == 0: all ring pairs (r_chunks = nothing); otherwise
+ * r_chunks of rrows x rcols chunks of capacity rnc over the chunk space.
+ * rec receives up to cap records (invasor, invaded, p_id), 1-based, in the
+ * reference's push order; *count the total.  Returns OR_E_OUT_OF_BOX /
+ * OR_E_CELL_OVERFLOW where binning a centre of mass throws BoundsError. */
+int or_rings_invasions(const or_sys *s, const double *rpts, const double *cms, int32_t rcols, int32_t rrows,
+                       int64_t rnc, int64_t *rec, int64_t cap, int64_t *count);
+
 /* calc_walls_forces! for particle i (0-based): adds the potential walls'
  * forces to f[0..1], src/integration.jl:230-255. */
 void or_wall_forces(const or_sys *s, const double *pos, int64_t i, double *f);

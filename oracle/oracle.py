@@ -97,6 +97,8 @@ _SIG = {
     "or_sp_steps": (C.c_int, [_S, _D, _D, C.c_int64, _D, _D, C.c_int]),
     "or_wall_forces": (None, [_S, _D, C.c_int64, _D]),
     "or_rings_neighbors": (C.c_int, [_S, _D, C.c_double, C.c_int, _I64, _I64]),
+    "or_point_line_intersect": (C.c_int, [_D, _D, _D]),
+    "or_rings_invasions": (C.c_int, [_S, _D, _D, C.c_int32, C.c_int32, C.c_int64, _I64, C.c_int64, _I64]),
     "or_walls_apply": (None, [_S, _D, _D, C.c_int64, C.c_double]),
 }
 
@@ -127,6 +129,11 @@ def _i64(a):
 
 def jl_div(a: float, b: float) -> float:
     return lib().or_jl_div(a, b)
+
+
+def point_line_intersect(p, a, b) -> bool:
+    arrs = [np.ascontiguousarray(v, dtype=np.float64) for v in (p, a, b)]
+    return bool(lib().or_point_line_intersect(*[_d(v) for v in arrs]))
 
 
 def chunk_capacity(cl, ch, r) -> int:
@@ -319,6 +326,24 @@ class Sys:
         if err:
             raise RuntimeError(f"oracle rings_neighbors error {err}")
         return counts, lists
+
+    def rings_invasions(self, rpts, cms, rcols=0, rrows=0, rnc=0):
+        """check_invasions!: list of (invasor, invaded, p_id), 1-based, in the
+        reference's push order.  rpts = ring_points (continuous positions for
+        a periodic main wall), cms = centres of mass (nrings, 2)."""
+        rpts = np.ascontiguousarray(rpts, dtype=np.float64)
+        cms = np.ascontiguousarray(cms, dtype=np.float64)
+        cap = max(16, self.nmax * self.nrings)
+        while True:
+            rec = np.zeros((cap, 3), dtype=np.int64)
+            count = np.zeros(1, dtype=np.int64)
+            err = lib().or_rings_invasions(self.ptr, _d(rpts), _d(cms), rcols, rrows, rnc, _i64(rec), cap,
+                                           _i64(count))
+            if err:
+                raise RuntimeError(f"oracle rings_invasions error {err}")
+            if count[0] <= cap:
+                return [tuple(int(v) for v in r) for r in rec[:count[0]]]
+            cap = int(count[0])
 
     # walls
     def wall_forces(self, pos, i):
