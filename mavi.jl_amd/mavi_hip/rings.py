@@ -9,6 +9,8 @@
   InvasionsCfg, IntCfgExtra, RingsIntCfg      src/rings/configs.jl:334-350
   Invasion, RingsSystem.invasions             src/rings/rings.jl:86-100
   equilibrium_area, equilibrium_ring_radius   src/rings/configs.jl:221-328
+  ActiveState, VarRingsIds                    src/states.jl:14-20, src/rings/states.jl:20-39, 200-223
+  SourceCfg, SinkCfg, Source                  src/rings/sources.jl:31-185, 244-251
 
 Host layout: ``rings_pos`` is an (nrings, nmax, 2) array, i.e. Julia's
 Matrix{SVector{2,T}}(nmax, nrings) in memory order, and ``pos`` is its
@@ -22,7 +24,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from .configs import GPU, ChunksCfg, IntCfg, PeriodicWalls, RectangleCfg, SpaceCfg, chunk_capacity, get_bounding_box
+from .configs import (GPU, ChunksCfg, CircleCfg, IntCfg, PeriodicWalls, RectangleCfg, SpaceCfg, chunk_capacity,
+                      get_bounding_box)
 from .systems import Chunks, TimeInfo, attach_walls, base_desc, check_inside, wall_kind, wall_records
 
 
@@ -86,8 +89,42 @@ class RingsCfg:  # src/rings/configs.jl:95-150
         return [self.particle_radius(t + 1) for t in range(self.num_types)]
 
 
+@dataclass
+class ActiveState:  # src/states.jl:14-20
+    """Initial active mask of a variable ring population (True: all)."""
+
+    mask: object = True
+
+
+class VarRingsIds:  # src/rings/states.jl:20-39
+    """Active rings of a variable population: mask, ids (1-based; the
+    entries past num_active keep what an earlier update left), uids,
+    p_ids (0-based slots) and the counts, as calc_active_ids! leaves them."""
+
+    def __init__(self, mask, nmax, ring_np):
+        n = len(mask)
+        self.mask = np.asarray(mask, dtype=bool).copy()
+        self.ids = np.arange(1, n + 1, dtype=np.int64)
+        self.uids = np.arange(1, n + 1, dtype=np.int64)
+        self.num_active = 0
+        self._nmax, self._ring_np = nmax, ring_np
+        self.update()
+
+    def update(self):
+        """calc_active_ids!, src/rings/states.jl:200-223."""
+        act = np.nonzero(self.mask)[0]
+        self.ids[:len(act)] = act + 1
+        self.num_active = len(act)
+        self.p_ids = np.concatenate([r * self._nmax + np.arange(self._ring_np(r + 1)) for r in act]
+                                    + [np.zeros(0, dtype=np.int64)]).astype(np.int64)
+        self.num_p_active = len(self.p_ids)
+
+    def get_rings_ids(self):
+        return self.ids[:self.num_active]
+
+
 class RingsState:  # src/rings/states.jl:74-124
-    def __init__(self, *, rings_pos, pol, num_particles=None, types=None):
+    def __init__(self, *, rings_pos, pol, num_particles=None, types=None, active_state=None):
         rp = np.asarray(rings_pos, dtype=np.float64)
         if rp.ndim == 3 and rp.shape[0] == 2 and rp.shape[2] != 2:
             rp = np.transpose(rp, (2, 1, 0))  # Julia (2, nmax, nrings) -> (nrings, nmax, 2)
@@ -99,6 +136,12 @@ class RingsState:  # src/rings/states.jl:74-124
             num_particles = self.nmax
         self.num_particles = num_particles
         self.types = None if types is None else np.asarray(types, dtype=np.int32)
+        self.rings_ids = None  # FixRingsIds
+        if active_state is not None:
+            mask = active_state.mask
+            if np.ndim(mask) == 0:
+                mask = np.full(self.num_rings, bool(mask))
+            self.rings_ids = VarRingsIds(mask, self.nmax, self.ring_num_particles)
 
     def ring_num_particles(self, ring_id):  # 1-based ring id
         if self.types is None:
@@ -109,7 +152,9 @@ class RingsState:  # src/rings/states.jl:74-124
         return int(np.atleast_1d(np.asarray(self.num_particles))[type_ - 1 if self.types is not None else 0])
 
     def particle_ids(self) -> np.ndarray:
-        """get_particles_ids: FixRingsIds.p_ids (0-based slots)."""
+        """get_particles_ids: FixRingsIds.p_ids or VarRingsIds.p_ids (0-based slots)."""
+        if self.rings_ids is not None:
+            return self.rings_ids.p_ids
         if self.types is None:
             return np.arange(len(self.pos))
         ids = [r * self.nmax + i for r in range(self.num_rings) for i in range(self.ring_num_particles(r + 1))]
@@ -248,6 +293,112 @@ def ring_chunks_grid(state: RingsState, space_cfg: SpaceCfg, cfg: RingsCfg, r_ch
     rings_r = min(equilibrium_ring_radius(cfg, state.ring_num_particles_of_type(t + 1), t + 1)
                   for t in range(cfg.num_types))
     return cols, rows, chunk_capacity(g.length / cols, g.height / rows, rings_r)
+
+
+# --- sources and sinks (src/rings/sources.jl) ------------------------------
+@dataclass
+class SourceCfg:  # src/rings/sources.jl:31-49
+    """Spawn boxes of rings: size[0] x size[1] boxes around the spawn
+    template, each refilled with a ring when no particle lies inside it
+    (grown by pad).  spawn_pol: a number, or "random" (2 pi u per empty box)."""
+
+    bottom_left: tuple
+    spawn_pos: object
+    spawn_pol: object = "random"
+    pad: float = 0.0
+    offset: tuple = (0.0, 0.0)
+    size: tuple = (1, 1)
+
+
+@dataclass
+class SinkCfg:  # src/rings/sources.jl:244-251
+    """Rings whose centre of mass lies inside geometry_cfg are removed."""
+
+    geometry_cfg: object
+
+
+def _rect_add(a, b):
+    """Base.:+(::RectangleCfg, ::RectangleCfg), src/configs.jl:68-77, on
+    (blx, bly, length, height) tuples."""
+    max_x = max(a[0] + a[2], b[0] + b[2])
+    max_y = max(a[1] + a[3], b[1] + b[3])
+    min_x = min(a[0], b[0])
+    min_y = min(a[1], b[1])
+    return (min_x, min_y, max_x - min_x, max_y - min_y)
+
+
+def _check_intersection(r1, r2):
+    """check_intersection(::RectangleCfg, ::RectangleCfg), src/configs.jl:79-87."""
+    ix1 = r2[0] <= r1[0] <= r2[0] + r2[2]
+    ix2 = r2[0] <= r1[0] + r1[2] <= r2[0] + r2[2]
+    iy1 = r2[1] <= r1[1] <= r2[1] + r2[3]
+    iy2 = r2[1] <= r1[1] + r1[3] <= r2[1] + r2[3]
+    return (ix1 or ix2) and (iy1 or iy2)
+
+
+class Source:  # src/rings/sources.jl:127-185
+    """Spawn boxes (bbox: (blx, bly, length, height)), their spawn positions
+    and, with particle chunks, ChunksChecker's chunk lists (0-based chunk
+    indices row0 * num_cols + col0); without chunks the PosChecker (every
+    active particle) decides emptiness."""
+
+    def __init__(self, cfg: SourceCfg, chunks: Chunks | None = None):
+        self.cfg = cfg
+        sp = np.asarray(cfg.spawn_pos, dtype=np.float64).reshape(-1, 2)
+        pad = float(cfg.pad)
+        min_x, max_x = float(sp[:, 0].min()), float(sp[:, 0].max())
+        min_y, max_y = float(sp[:, 1].min()), float(sp[:, 1].max())
+        bbox_length = max_x - min_x + 2 * pad
+        bbox_height = max_y - min_y + 2 * pad
+        blx, bly = float(cfg.bottom_left[0]), float(cfg.bottom_left[1])
+        offx, offy = float(cfg.offset[0]), float(cfg.offset[1])
+        self.bbox, bbox_pad = [], []
+        for i in range(1, cfg.size[0] + 1):
+            for j in range(1, cfg.size[1] + 1):
+                bx = blx + ((i - 1) * bbox_length + i * offx)
+                by = bly + ((j - 1) * bbox_height + j * offy)
+                self.bbox.append((bx, by, bbox_length, bbox_height))
+                bbox_pad.append((bx - pad, by - pad, bbox_length + 2 * pad, bbox_height + 2 * pad))
+        # bbox_spawn_pos: spawn_pos .+ (box bottom_left - template bottom_left + (pad, pad))
+        self.bbox_spawn_pos = [sp + np.array([(b[0] - min_x) + pad, (b[1] - min_y) + pad]) for b in self.bbox]
+        self.chunk_ids = None if chunks is None else self._chunks_checker(chunks, bbox_pad)
+
+    @staticmethod
+    def _chunks_checker(chunks: Chunks, bbox_vec):
+        """ChunksChecker(pos, chunks, bbox_vec), src/rings/sources.jl:64-119."""
+        g = bbox_vec[0]
+        for b in bbox_vec[1:]:  # sum(bbox_vec): a left fold of +
+            g = _rect_add(g, b)
+        geo = chunks.geometry_cfg
+        cl, ch = chunks.chunk_length, chunks.chunk_height
+        tlx, tly = geo.bottom_left[0], geo.bottom_left[1] + geo.height
+        found, sx, ex = False, 0, 0
+        for i in range(1, chunks.num_cols + 1):
+            x1 = tlx + (i - 1) * cl
+            x2 = x1 + cl
+            if not found and x1 <= g[0] <= x2:
+                found, sx = True, i
+            if found and x1 >= g[0] + g[2]:
+                ex = i - 1
+                break
+        found, sy, ey = False, 0, 0
+        for i in range(1, chunks.num_rows + 1):
+            y2 = tly - (i - 1) * ch
+            y1 = y2 - ch
+            if not found and y1 <= g[1] + g[3] <= y2:
+                found, sy = True, i
+            if found and y2 <= g[1]:
+                ey = i - 1
+                break
+        ids = [[] for _ in bbox_vec]
+        for col in range(sx, ex + 1):
+            for row in range(sy, ey + 1):
+                # get_chunk_rect, src/chunks.jl:50-58
+                rect = (geo.bottom_left[0] + (col - 1) * cl, tly - row * ch, cl, ch)
+                for b, lst in zip(bbox_vec, ids):
+                    if _check_intersection(rect, b):
+                        lst.append((row - 1) * chunks.num_cols + (col - 1))
+        return ids
 
 
 @dataclass

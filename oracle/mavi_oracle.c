@@ -118,14 +118,21 @@ static int32_t ring_np(const or_sys *s, int32_t ring /*1-based*/) {
     return s->np_type[s->types[ring - 1] - 1];
 }
 
+static inline int ring_active(const or_sys *s, int32_t ring /*1-based*/) {
+    return s->var == NULL || s->var->mask[ring - 1] != 0;
+}
+
 int64_t or_particle_ids(const or_sys *s, int64_t *ids) {
-    if (s->kind != OR_KIND_RINGS || s->types == NULL) {
+    if (s->kind != OR_KIND_RINGS || (s->types == NULL && s->var == NULL)) {
         if (ids)
             for (int64_t i = 0; i < s->n; i++) ids[i] = i + 1;
         return s->n;
     }
+    /* typed rings: FixRingsIds.p_ids; a variable population: the p_ids of
+     * calc_active_ids! (src/rings/states.jl:200-223), active rings ascending */
     int64_t k = 0;
     for (int32_t r = 1; r <= s->nrings; r++) {
+        if (!ring_active(s, r)) continue;
         int64_t base = (int64_t)(r - 1) * s->nmax + 1; /* to_scalar_idx(r, 1), src/rings/states.jl:137-139 */
         for (int32_t i = 1; i <= ring_np(s, r); i++) {
             if (ids) ids[k] = base + i - 1;
@@ -505,9 +512,12 @@ int or_rings_invasions(const or_sys *s, const double *rpts, const double *cms, i
                        int64_t rnc, int64_t *rec, int64_t cap, int64_t *count) {
     if (s->kind != OR_KIND_RINGS) return OR_E_BADARG;
     *count = 0;
-    if (rcols <= 0) { /* check_invasions!(system, ::Nothing), :498-507 */
-        for (int32_t r1 = 1; r1 <= s->nrings; r1++)
-            for (int32_t r2 = r1 + 1; r2 <= s->nrings; r2++) find_invasions(s, rpts, r1, r2, rec, cap, count);
+    if (rcols <= 0) {
+        /* check_invasions!(system, ::Nothing), :498-507: ring ids 1 ..
+         * length(get_rings_ids), every id of a fixed population */
+        const int32_t nr = s->var ? (int32_t)s->var->num_active : s->nrings;
+        for (int32_t r1 = 1; r1 <= nr; r1++)
+            for (int32_t r2 = r1 + 1; r2 <= nr; r2++) find_invasions(s, rpts, r1, r2, rec, cap, count);
         return OR_OK;
     }
     /* r_chunks: Chunks over the centres of mass of get_rings_ids (every ring
@@ -519,6 +529,7 @@ int or_rings_invasions(const or_sys *s, const double *rpts, const double *cms, i
     int64_t *cnt = (int64_t *)calloc((size_t)ncell, sizeof(int64_t));
     int err = OR_OK;
     for (int32_t r = 1; r <= s->nrings && !err; r++) {
+        if (!ring_active(s, r)) continue;
         int64_t row, col;
         err = cell_in(s, rows, cols, cl, ch, cms[2 * (r - 1)], cms[2 * (r - 1) + 1], &row, &col);
         if (err) break;
@@ -956,10 +967,18 @@ static inline int64_t sidx(const or_sys *s, int32_t ring, int32_t i) { /* to_sca
 
 void or_rings_continuous_pos(const or_sys *s, const double *pos, double *cpos) {
     const int64_t nslots = (int64_t)s->nmax * s->nrings;
-    /* continuos_pos[:, ring_id] .= rings_pos[:, ring_id] (all Nmax slots) */
-    memcpy(cpos, pos, sizeof(double) * 2 * nslots);
+    /* continuos_pos[:, ring_id] .= rings_pos[:, ring_id] (all Nmax slots) for
+     * the rings of get_rings_ids; the others keep their last values */
+    if (s->var == NULL) {
+        memcpy(cpos, pos, sizeof(double) * 2 * nslots);
+    } else {
+        for (int32_t r = 1; r <= s->nrings; r++)
+            if (ring_active(s, r))
+                memcpy(cpos + 2 * (sidx(s, r, 1) - 1), pos + 2 * (sidx(s, r, 1) - 1), sizeof(double) * 2 * s->nmax);
+    }
     if (!s->periodic) return; /* update_continuos_pos!(system, wall_type) is a no-op */
     for (int32_t r = 1; r <= s->nrings; r++) {
+        if (!ring_active(s, r)) continue;
         int32_t np = ring_np(s, r);
         for (int32_t i = 2; i <= np; i++) {
             int64_t a = sidx(s, r, i) - 1, b = sidx(s, r, i - 1) - 1;
@@ -981,22 +1000,14 @@ double or_rings_area(const or_sys *s, const double *cpos, int32_t ring) {
     return area / 2.0;
 }
 
-int or_rings_forces(const or_sys *s, const double *pos, const double *cpos, double *forces, double *areas) {
-    /* forces!, src/rings/integration.jl:197-226 */
-    int64_t ncell = (int64_t)s->rows * s->cols;
-    int64_t *cp = NULL, *counts = NULL;
-    int err = OR_OK;
-    if (s->cols > 0) {
-        cp = (int64_t *)malloc(sizeof(int64_t) * (size_t)s->nc * ncell);
-        counts = (int64_t *)malloc(sizeof(int64_t) * ncell);
-        err = or_bin(s, pos, cp, counts);
-        if (err) { free(cp); free(counts); return err; }
-    }
+/* forces!, src/rings/integration.jl:197-226, on particle chunks already
+ * binned (cp/counts NULL without chunks). */
+static void rings_forces_binned(const or_sys *s, const double *pos, const double *cpos, const int64_t *cp,
+                                const int64_t *counts, double *forces, double *areas) {
     or_calc_forces(s, pos, cp, counts, forces, NULL);
-    free(cp);
-    free(counts);
     /* springs, :205-223 */
     for (int32_t r = 1; r <= s->nrings; r++) {
+        if (!ring_active(s, r)) continue;
         int32_t np = ring_np(s, r);
         const double *rp = rpar(s, r);
         double k = rp[RP_KSPRING], l = rp[RP_LSPRING];
@@ -1017,6 +1028,7 @@ int or_rings_forces(const or_sys *s, const double *pos, const double *cpos, doub
     }
     /* area_forces!, :140-195 */
     for (int32_t r = 1; r <= s->nrings; r++) {
+        if (!ring_active(s, r)) continue;
         int32_t np = ring_np(s, r);
         const double *rp = rpar(s, r);
         double area = or_rings_area(s, cpos, r);
@@ -1035,12 +1047,28 @@ int or_rings_forces(const or_sys *s, const double *pos, const double *cpos, doub
             forces[2 * c + 1] -= fmod * ad1;
         }
     }
+}
+
+int or_rings_forces(const or_sys *s, const double *pos, const double *cpos, double *forces, double *areas) {
+    int64_t ncell = (int64_t)s->rows * s->cols;
+    int64_t *cp = NULL, *counts = NULL;
+    int err = OR_OK;
+    if (s->cols > 0) {
+        cp = (int64_t *)malloc(sizeof(int64_t) * (size_t)s->nc * ncell);
+        counts = (int64_t *)malloc(sizeof(int64_t) * ncell);
+        err = or_bin(s, pos, cp, counts);
+        if (err) { free(cp); free(counts); return err; }
+    }
+    rings_forces_binned(s, pos, cpos, cp, counts, forces, areas);
+    free(cp);
+    free(counts);
     return OR_OK;
 }
 
 static void rings_cms(const or_sys *s, const double *cpos, double *cms) {
-    /* update_cms!, src/rings/integration.jl:366-372 */
+    /* update_cms!, src/rings/integration.jl:366-372, over get_rings_ids */
     for (int32_t r = 1; r <= s->nrings; r++) {
+        if (!ring_active(s, r)) continue;
         int32_t np = ring_np(s, r);
         const double *c = cpos + 2 * (sidx(s, r, 1) - 1);
         double sx = 0.0, sy = 0.0;
@@ -1050,35 +1078,164 @@ static void rings_cms(const or_sys *s, const double *cpos, double *cms) {
     }
 }
 
+/* calc_active_ids!, src/rings/states.jl:200-223: the active rings in
+ * ascending order at the front of ids (later entries untouched). */
+static void update_ids(const or_sys *s) {
+    or_var_rings *v = s->var;
+    int64_t k = 0;
+    for (int32_t r = 1; r <= s->nrings; r++)
+        if (v->mask[r - 1]) v->ids[k++] = r;
+    v->num_active = k;
+}
+
+static int var_bin(const or_sys *s, const double *pos) {
+    if (s->cols <= 0) return OR_OK;
+    return or_bin(s, pos, s->var->cp, s->var->counts);
+}
+
+int or_rings_var_init(const or_sys *s, const double *pos) {
+    or_var_rings *v = s->var;
+    if (s->kind != OR_KIND_RINGS || v == NULL) return OR_E_BADARG;
+    for (int32_t r = 1; r <= s->nrings; r++) { /* VarRingsIds(ids = 1:n, uids = 1:n) */
+        v->ids[r - 1] = r;
+        v->uids[r - 1] = r;
+    }
+    update_ids(s);
+    v->pol_used = 0;
+    or_rings_continuous_pos(s, pos, v->cpos);
+    for (int64_t k = 0; k < 2 * (int64_t)s->nrings; k++) v->cms[k] = 0.0; /* zero(state.pos[1]) */
+    rings_cms(s, s->periodic ? v->cpos : pos, v->cms);
+    return var_bin(s, pos);
+}
+
+/* is_inside(point, ::RectangleCfg; pad) and is_inside(point, ::CircleCfg;
+ * pad), src/configs.jl:89-93, 165-168. */
+static int inside_rect(const double *p, const double *g, double pad) {
+    return (g[0] - pad <= p[0] && p[0] <= g[0] + g[2] + pad) && (g[1] - pad <= p[1] && p[1] <= g[1] + g[3] + pad);
+}
+static int inside_geom(const double *p, int32_t geom, const double *g) {
+    if (geom == OR_GEOM_RECT) return inside_rect(p, g, 0.0);
+    const double dx = p[0] - g[0], dy = p[1] - g[1];
+    return dx * dx + dy * dy <= g[2] * g[2];
+}
+
+/* update_sources!, src/rings/integration.jl:353-358 with process_source!,
+ * update_area_empty!, add_ring!, process_sink!, remove_ring!
+ * (src/rings/sources.jl:187-260, src/rings/states.jl:173-191).  pids are the
+ * particle ids of the last update_ids!. */
+static int sources_step(const or_sys *s, double *pos, double *pol, const int64_t *pids, int64_t npids) {
+    or_var_rings *v = s->var;
+    for (int32_t q = 0; q < s->nsrc; q++) {
+        const or_source *src = s->src + q;
+        if (src->kind == OR_SRC_SINK) {
+            /* for ring_id in get_rings_ids(state): the view ids[1:num_active]
+             * taken when the loop starts */
+            const int64_t nview = v->num_active;
+            for (int64_t k = 0; k < nview; k++) {
+                const int64_t r = v->ids[k];
+                if (inside_geom(v->cms + 2 * (r - 1), src->geom, src->g)) {
+                    v->mask[r - 1] = 0;
+                    v->num_active -= 1;
+                }
+            }
+            continue;
+        }
+        for (int32_t b = 0; b < src->nbox; b++) {
+            const double *box = src->bbox + 4 * b;
+            int empty = 1;
+            if (src->chunk_off) { /* ChunksChecker: the particles of the listed chunks */
+                for (int32_t e = src->chunk_off[b]; e < src->chunk_off[b + 1] && empty; e++) {
+                    const int64_t k = src->chunk_ids[e];
+                    const int64_t row0 = k / s->cols, col0 = k % s->cols;
+                    const int64_t cell = row0 + (int64_t)s->rows * col0;
+                    for (int64_t j = 0; j < v->counts[cell] && empty; j++) {
+                        const int64_t p = v->cp[s->nc * cell + j];
+                        if (inside_rect(pos + 2 * (p - 1), box, src->pad)) empty = 0;
+                    }
+                }
+            } else { /* PosChecker: every id of the last update_ids! */
+                for (int64_t j = 0; j < npids && empty; j++)
+                    if (inside_rect(pos + 2 * (pids[j] - 1), box, src->pad)) empty = 0;
+            }
+            if (!empty) continue;
+            /* get_spawn_pol is evaluated before add_ring! looks for a slot */
+            double new_pol = src->pol;
+            if (src->random_pol) {
+                if (v->pol_used >= v->npol_draws) return OR_E_BADARG;
+                new_pol = v->pol_draws[v->pol_used++] * 2.0 * 3.141592653589793; /* rand(rng, T) * 2 * pi */
+            }
+            int32_t slot = 0;
+            for (int32_t r = 1; r <= s->nrings; r++)
+                if (!v->mask[r - 1]) { slot = r; break; }
+            if (slot == 0) continue; /* no room: add_ring! returns nothing */
+            memcpy(pos + 2 * (sidx(s, slot, 1) - 1), src->spawn + 2 * (int64_t)s->nmax * b,
+                   sizeof(double) * 2 * s->nmax);
+            pol[slot - 1] = new_pol;
+            v->mask[slot - 1] = 1;
+            int64_t umax = v->uids[0];
+            for (int32_t r = 2; r <= s->nrings; r++) umax = v->uids[r - 1] > umax ? v->uids[r - 1] : umax;
+            v->uids[slot - 1] = umax + 1;
+            v->num_active += 1;
+            /* info.cms[ring_id] = sum(rings_pos[1:num_p, ring_id]) / num_p */
+            const int32_t np = ring_np(s, slot);
+            const double *c = pos + 2 * (sidx(s, slot, 1) - 1);
+            double sx = 0.0, sy = 0.0;
+            for (int32_t i = 0; i < np; i++) { sx += c[2 * i]; sy += c[2 * i + 1]; }
+            v->cms[2 * (slot - 1)] = sx / (double)np;
+            v->cms[2 * (slot - 1) + 1] = sy / (double)np;
+        }
+    }
+    return OR_OK;
+}
+
 int or_rings_steps(const or_sys *s, double *pos, double *pol, int64_t nsteps, const double *noise,
                    double *forces_out, double *areas_out, double *cms_out) {
     if (s->kind != OR_KIND_RINGS) return OR_E_BADARG;
     const int64_t nslots = (int64_t)s->nmax * s->nrings;
-    double *cpos = (double *)malloc(sizeof(double) * 2 * nslots);
-    double *forces = (double *)malloc(sizeof(double) * 2 * nslots);
-    double *areas = (double *)malloc(sizeof(double) * s->nrings);
-    double *cms = (double *)malloc(sizeof(double) * 2 * s->nrings);
+    or_var_rings *v = s->var;
+    double *cpos = v ? v->cpos : (double *)malloc(sizeof(double) * 2 * nslots);
+    double *forces = (double *)calloc((size_t)(2 * nslots), sizeof(double));
+    double *areas = (double *)calloc((size_t)s->nrings, sizeof(double));
+    double *cms = v ? v->cms : (double *)malloc(sizeof(double) * 2 * s->nrings);
+    int64_t ncell = (int64_t)s->rows * s->cols;
+    int64_t *cp = NULL, *counts = NULL;
+    if (s->cols > 0 && !v) {
+        cp = (int64_t *)malloc(sizeof(int64_t) * (size_t)s->nc * ncell);
+        counts = (int64_t *)malloc(sizeof(int64_t) * ncell);
+    }
+    int64_t *pids = (int64_t *)malloc(sizeof(int64_t) * (nslots > 0 ? nslots : 1));
     const double dt = s->dt;
     int err = OR_OK;
     /* The constructor leaves continuos_pos computed (src/rings/rings.jl:280);
      * step! uses it in update_cms! before recomputing it. */
-    or_rings_continuous_pos(s, pos, cpos);
+    if (!v) or_rings_continuous_pos(s, pos, cpos);
     for (int64_t step = 0; step < nsteps; step++) {
         /* update_cms!: get_continuos_pos is info.continuos_pos (last computed)
          * for a periodic main wall, the live rings_pos otherwise */
         rings_cms(s, s->periodic ? cpos : pos, cms);
-        or_rings_continuous_pos(s, pos, cpos);   /* update_chunks_all! then update_continuos_pos! */
-        err = or_rings_forces(s, pos, cpos, forces, areas); /* cleaning! + forces! (bins inside) */
-        if (err) break;
+        if (v) {
+            const int64_t npids = or_particle_ids(s, pids); /* the ids of the last update_ids! */
+            err = sources_step(s, pos, pol, pids, npids);
+            if (err) break;
+            update_ids(s);
+            err = var_bin(s, pos); /* update_chunks_all! */
+            if (err) break;
+        } else if (s->cols > 0) {
+            err = or_bin(s, pos, cp, counts); /* update_chunks_all! */
+            if (err) break;
+        }
+        or_rings_continuous_pos(s, pos, cpos); /* update_continuos_pos! */
+        rings_forces_binned(s, pos, cpos, v ? v->cp : cp, v ? v->counts : counts, forces, areas);
         if (s->nwalls > 0) { /* calc_walls_forces! over get_particles_ids */
             int64_t nids = or_particle_ids(s, NULL);
-            int64_t *ids = (int64_t *)malloc(sizeof(int64_t) * nids);
+            int64_t *ids = (int64_t *)malloc(sizeof(int64_t) * (nids > 0 ? nids : 1));
             or_particle_ids(s, ids);
             for (int64_t k = 0; k < nids; k++) or_wall_forces(s, pos, ids[k] - 1, forces + 2 * (ids[k] - 1));
             free(ids);
         }
-        /* update!, src/rings/integration.jl:300-351 */
+        /* update!, src/rings/integration.jl:300-351, over get_rings_ids */
         for (int32_t r = 1; r <= s->nrings; r++) {
+            if (!ring_active(s, r)) continue;
             int32_t np = ring_np(s, r);
             const double *rp = rpar(s, r);
             double vo = rp[RP_VO], relax = rp[RP_RELAX], mu = rp[RP_MU], drot = rp[RP_DR];
@@ -1112,7 +1269,7 @@ int or_rings_steps(const or_sys *s, double *pos, double *pol, int64_t nsteps, co
          * with each particle's radius (get_particle_radius, src/rings/rings.jl:44-53) */
         if (s->periodic || s->nwalls > 0) {
             int64_t nids = or_particle_ids(s, NULL);
-            int64_t *ids = (int64_t *)malloc(sizeof(int64_t) * nids);
+            int64_t *ids = (int64_t *)malloc(sizeof(int64_t) * (nids > 0 ? nids : 1));
             or_particle_ids(s, ids);
             for (int64_t k = 0; k < nids; k++) {
                 if (s->nwalls > 0) {
@@ -1128,9 +1285,14 @@ int or_rings_steps(const or_sys *s, double *pos, double *pol, int64_t nsteps, co
     if (forces_out) memcpy(forces_out, forces, sizeof(double) * 2 * nslots);
     if (areas_out) memcpy(areas_out, areas, sizeof(double) * s->nrings);
     if (cms_out) memcpy(cms_out, cms, sizeof(double) * 2 * s->nrings);
-    free(cpos);
+    if (!v) {
+        free(cpos);
+        free(cms);
+    }
     free(forces);
     free(areas);
-    free(cms);
+    free(cp);
+    free(counts);
+    free(pids);
     return err;
 }

@@ -53,6 +53,41 @@ typedef struct {
  *   int_cfg    (src/configs.jl:482-487) -> dt, cols/rows (ChunksCfg)
  *   Chunks     (src/chunks.jl:10-40)    -> cols, rows, cl, ch, nc
  */
+/* Variable ring population: VarRingsIds (src/rings/states.jl:20-39,
+ * 173-223) and the RingsInfo fields that persist between steps.  Owned by
+ * the caller, updated in place by or_rings_var_init / or_rings_steps. */
+typedef struct {
+    int32_t *mask;        /* nrings, 1 = active */
+    int64_t *ids;         /* nrings, VarRingsIds.ids, 1-based; entries past
+                             num_active keep what an earlier update left */
+    int64_t *uids;        /* nrings */
+    int64_t num_active;
+    double *cms;          /* 2*nrings, RingsInfo.cms (stale for inactive rings) */
+    double *cpos;         /* 2*nmax*nrings, RingsInfo.continuos_pos */
+    int64_t *cp, *counts; /* the particle chunks of the last update_chunks!
+                             (nc*rows*cols, rows*cols), read by ChunksChecker */
+    const double *pol_draws;   /* uniform draws consumed by spawn_pol = :random */
+    int64_t npol_draws, pol_used;
+} or_var_rings;
+
+/* A source or a sink of rings (src/rings/sources.jl), processed in list
+ * order by update_sources! (src/rings/integration.jl:353-358). */
+enum { OR_SRC_SOURCE = 0, OR_SRC_SINK = 1 };
+enum { OR_GEOM_RECT = 0, OR_GEOM_CIRCLE = 1 };
+typedef struct {
+    int32_t kind;          /* OR_SRC_* */
+    int32_t geom;          /* sink geometry: OR_GEOM_RECT (blx, bly, length, height) or OR_GEOM_CIRCLE (cx, cy, r) */
+    double g[4];
+    int32_t nbox;          /* source: Source.bbox boxes */
+    int32_t random_pol;    /* 1: spawn_pol = :random (2 pi u per empty box), 0: the fixed pol below */
+    double pad;            /* SourceCfg.pad */
+    double pol;
+    const double *bbox;    /* nbox x 4: bottom_left x, y, length, height */
+    const double *spawn;   /* nbox x nmax x 2: Source.bbox_spawn_pos */
+    const int32_t *chunk_off;  /* nbox+1 offsets into chunk_ids; NULL: PosChecker */
+    const int32_t *chunk_ids;  /* ChunksChecker.ids as 0-based row0*cols + col0 */
+} or_source;
+
 typedef struct {
     int64_t n;          /* particle slots (length(state.pos)) */
     double bl[2];       /* geometry_cfg.bottom_left */
@@ -84,6 +119,11 @@ typedef struct {
        rectangle walls; the first entry is the main wall (calc_diff, stencil) */
     int32_t nwalls, pad2;
     const or_wall *walls;
+    /* Rings with a variable population (NULL: FixRingsIds) and their
+       sources and sinks */
+    or_var_rings *var;
+    int32_t nsrc, pad3;
+    const or_source *src;
 } or_sys;
 
 /* Julia Base `div(a::Float64, b::Float64)` (RoundToZero): round((a - rem(a,b))/b). */
@@ -183,10 +223,19 @@ double or_rings_area(const or_sys *s, const double *cpos, int32_t ring);
 /* forces! (pair + springs + area), src/rings/integration.jl:140-226. */
 int or_rings_forces(const or_sys *s, const double *pos, const double *cpos,
                     double *forces, double *areas);
+/* The RingsSystem constructor's sequence for a variable population
+ * (src/rings/rings.jl:280-284, src/rings/states.jl:106-123): update_ids!
+ * from var->mask (ids start as 1:nrings), continuos_pos and cms of the
+ * active rings (cms of the others 0), the first particle binning. */
+int or_rings_var_init(const or_sys *s, const double *pos);
 /* Rings.Integration.step! x nsteps, src/rings/integration.jl:522-543.
  * noise (nullable) holds nsteps*nrings standard normal draws consumed in
- * ring order per step (the randn(system.rng) of :348); NULL means zero noise.
- * forces/areas/cms (nullable) receive the last step's values. */
+ * ring order per step (the randn(system.rng) of :348), indexed by ring slot;
+ * NULL means zero noise.  forces/areas/cms (nullable) receive the last
+ * step's values.  With s->var, a step also runs update_cms! over the active
+ * rings, the sources and sinks in list order, update_ids! and keeps the
+ * persistent fields of s->var (continue from or_rings_var_init or from a
+ * previous call). */
 int or_rings_steps(const or_sys *s, double *pos, double *pol, int64_t nsteps,
                    const double *noise, double *forces, double *areas, double *cms);
 

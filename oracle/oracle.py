@@ -40,6 +40,42 @@ class OrWall(C.Structure):
     ]
 
 
+class OrVarRings(C.Structure):
+    _fields_ = [
+        ("mask", C.POINTER(C.c_int32)),
+        ("ids", C.POINTER(C.c_int64)),
+        ("uids", C.POINTER(C.c_int64)),
+        ("num_active", C.c_int64),
+        ("cms", C.POINTER(C.c_double)),
+        ("cpos", C.POINTER(C.c_double)),
+        ("cp", C.POINTER(C.c_int64)),
+        ("counts", C.POINTER(C.c_int64)),
+        ("pol_draws", C.POINTER(C.c_double)),
+        ("npol_draws", C.c_int64),
+        ("pol_used", C.c_int64),
+    ]
+
+
+class OrSource(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("geom", C.c_int32),
+        ("g", C.c_double * 4),
+        ("nbox", C.c_int32),
+        ("random_pol", C.c_int32),
+        ("pad", C.c_double),
+        ("pol", C.c_double),
+        ("bbox", C.POINTER(C.c_double)),
+        ("spawn", C.POINTER(C.c_double)),
+        ("chunk_off", C.POINTER(C.c_int32)),
+        ("chunk_ids", C.POINTER(C.c_int32)),
+    ]
+
+
+SRC_SOURCE, SRC_SINK = 0, 1
+GEOM_RECT, GEOM_CIRCLE = 0, 1
+
+
 class OrSys(C.Structure):
     _fields_ = [
         ("n", C.c_int64),
@@ -68,6 +104,10 @@ class OrSys(C.Structure):
         ("nwalls", C.c_int32),
         ("pad2", C.c_int32),
         ("walls", C.POINTER(OrWall)),
+        ("var", C.c_void_p),
+        ("nsrc", C.c_int32),
+        ("pad3", C.c_int32),
+        ("src", C.c_void_p),
     ]
 
 
@@ -94,6 +134,7 @@ _SIG = {
     "or_rings_area": (C.c_double, [_S, _D, C.c_int32]),
     "or_rings_forces": (C.c_int, [_S, _D, _D, _D, _D]),
     "or_rings_steps": (C.c_int, [_S, _D, _D, C.c_int64, _D, _D, _D, _D]),
+    "or_rings_var_init": (C.c_int, [_S, _D]),
     "or_sp_steps": (C.c_int, [_S, _D, _D, C.c_int64, _D, _D, C.c_int]),
     "or_wall_forces": (None, [_S, _D, C.c_int64, _D]),
     "or_rings_neighbors": (C.c_int, [_S, _D, C.c_double, C.c_int, _I64, _I64]),
@@ -301,6 +342,77 @@ class Sys:
             raise RuntimeError(f"oracle rings_forces error {err}")
         return f, areas
 
+    # variable ring population (VarRingsIds) with sources and sinks
+    def set_var_rings(self, mask, sources=(), pol_draws=None):
+        """Give the rings a variable population: mask (nrings) of active
+        rings and a list of source/sink dicts
+        {kind: SRC_SOURCE, bbox (nbox,4), spawn (nbox,nmax,2), pad, pol |
+         random_pol, chunk_lists: list of 0-based chunk ids per box or None}
+        {kind: SRC_SINK, geom: GEOM_RECT/GEOM_CIRCLE, g: (4,)}.
+        Call rings_var_init(pos) next (the constructor's sequence)."""
+        nr, nm = self.nrings, self.nmax
+        ncell = max(self.rows * self.cols, 1)
+        self.var = dict(mask=np.ascontiguousarray(mask, dtype=np.int32).copy(),
+                        ids=np.zeros(nr, dtype=np.int64), uids=np.zeros(nr, dtype=np.int64),
+                        cms=np.zeros((nr, 2)), cpos=np.zeros((nr * nm, 2)),
+                        cp=np.zeros(max(self.nc, 1) * ncell, dtype=np.int64), counts=np.zeros(ncell, dtype=np.int64),
+                        pol_draws=np.ascontiguousarray(pol_draws if pol_draws is not None else [0.0],
+                                                       dtype=np.float64))
+        v = OrVarRings()
+        d = self.var
+        v.mask = d["mask"].ctypes.data_as(_I32)
+        v.ids = d["ids"].ctypes.data_as(_I64)
+        v.uids = d["uids"].ctypes.data_as(_I64)
+        v.cms = _d(d["cms"])
+        v.cpos = _d(d["cpos"])
+        v.cp = _i64(d["cp"])
+        v.counts = _i64(d["counts"])
+        v.pol_draws = _d(d["pol_draws"])
+        v.npol_draws = 0 if pol_draws is None else len(d["pol_draws"])
+        self._var_struct = v
+        srcs = (OrSource * max(len(sources), 1))()
+        keep = []
+        for k, sc in enumerate(sources):
+            o = srcs[k]
+            o.kind = sc["kind"]
+            if sc["kind"] == SRC_SINK:
+                o.geom = sc["geom"]
+                for j, val in enumerate(sc["g"]):
+                    o.g[j] = val
+                continue
+            bbox = np.ascontiguousarray(sc["bbox"], dtype=np.float64).reshape(-1, 4)
+            spawn = np.ascontiguousarray(sc["spawn"], dtype=np.float64).reshape(len(bbox), nm, 2)
+            o.nbox = len(bbox)
+            o.pad = sc.get("pad", 0.0)
+            o.random_pol = 1 if sc.get("random_pol") else 0
+            o.pol = sc.get("pol", 0.0)
+            o.bbox, o.spawn = _d(bbox), _d(spawn)
+            keep += [bbox, spawn]
+            if sc.get("chunk_lists") is not None:
+                off = np.zeros(len(bbox) + 1, dtype=np.int32)
+                off[1:] = np.cumsum([len(c) for c in sc["chunk_lists"]])
+                ids = np.ascontiguousarray(np.concatenate([np.asarray(c, dtype=np.int32) for c in sc["chunk_lists"]]
+                                                          + [np.zeros(0, np.int32)]), dtype=np.int32)
+                o.chunk_off, o.chunk_ids = off.ctypes.data_as(_I32), ids.ctypes.data_as(_I32)
+                keep += [off, ids]
+        self._src_keep = (srcs, keep)
+        self.s.var = C.cast(C.pointer(v), C.c_void_p)
+        self.s.nsrc = len(sources)
+        self.s.src = C.cast(srcs, C.c_void_p)
+
+    def rings_var_init(self, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        err = lib().or_rings_var_init(self.ptr, _d(pos))
+        if err:
+            raise RuntimeError(f"oracle rings_var_init error {err}")
+
+    def var_state(self):
+        """(mask, ids, uids, num_active, cms, pol draws used) of the variable population."""
+        v = self._var_struct
+        d = self.var
+        return (d["mask"].copy(), d["ids"].copy(), d["uids"].copy(), int(v.num_active), d["cms"].copy(),
+                int(v.pol_used))
+
     def rings_steps(self, pos, pol, nsteps, noise=None):
         pos = np.array(pos, dtype=np.float64, order="C")
         pol = np.array(pol, dtype=np.float64, order="C")
@@ -436,3 +548,26 @@ def from_rings(state, space_cfg, cfg, int_cfg):
                dt=int_cfg.dt, cols=cols, rows=rows, nc=nc, radius=r, nmax=state.nmax, nrings=state.num_rings,
                types=state.types, np_type=np.atleast_1d(state.num_particles), ring_par=cfg.type_table(),
                inter=inter.reshape(-1), walls=wall_records(space_cfg))
+
+
+def source_dicts(sources, nmax):
+    """Oracle records of the mirror's Source / SinkCfg objects (list order kept)."""
+    from mavi_hip.configs import CircleCfg
+
+    out = []
+    for s in sources:
+        if hasattr(s, "geometry_cfg"):  # SinkCfg
+            g = s.geometry_cfg
+            if isinstance(g, CircleCfg):
+                out.append(dict(kind=SRC_SINK, geom=GEOM_CIRCLE, g=(g.center[0], g.center[1], g.radius, 0.0)))
+            else:
+                out.append(dict(kind=SRC_SINK, geom=GEOM_RECT, g=(g.bottom_left[0], g.bottom_left[1], g.length,
+                                                                    g.height)))
+            continue
+        spawn = np.stack(s.bbox_spawn_pos)
+        if spawn.shape[1] != nmax:
+            raise ValueError("spawn_pos must hold num_max_particles points (add_ring! writes every slot)")
+        rnd = isinstance(s.cfg.spawn_pol, str)
+        out.append(dict(kind=SRC_SOURCE, bbox=np.array(s.bbox), spawn=spawn, pad=float(s.cfg.pad), random_pol=rnd,
+                        pol=0.0 if rnd else float(s.cfg.spawn_pol), chunk_lists=s.chunk_ids))
+    return out
