@@ -208,6 +208,47 @@ int mavi_rings_invasions_check(mavi_sys *sys);
  * order of its chunk traversal. */
 int mavi_rings_invasions_get(mavi_sys *sys, int64_t *count, int64_t *records, int64_t cap);
 
+/* Variable ring population (VarRingsIds, src/rings/states.jl:20-39,
+ * 173-223) with sources and sinks (src/rings/sources.jl), processed in list
+ * order at the start of every step (update_sources!,
+ * src/rings/integration.jl:353-358). */
+#define MAVI_SRC_SOURCE 0
+#define MAVI_SRC_SINK 1
+#define MAVI_GEOM_RECT 0   /* g = bottom_left x, y, length, height */
+#define MAVI_GEOM_CIRCLE 1 /* g = centre x, y, radius */
+typedef struct {
+    int32_t kind;             /* MAVI_SRC_SOURCE / MAVI_SRC_SINK */
+    int32_t geom;             /* sink: SinkCfg.geometry_cfg, MAVI_GEOM_* */
+    double g[4];
+    int32_t nbox;             /* source: length(Source.bbox) (at most 64 over all sources) */
+    int32_t random_pol;       /* 1: spawn_pol = :random (2 pi u, u from mavi_rings_pol_draws) */
+    double pad;               /* SourceCfg.pad */
+    double pol;               /* a fixed spawn_pol */
+    const double *bbox;       /* nbox x 4: Source.bbox (bottom_left x, y, length, height) */
+    const double *spawn;      /* nbox x nmax x 2: Source.bbox_spawn_pos (add_ring! writes every slot) */
+    const int32_t *chunk_off; /* nbox + 1 offsets into chunk_ids (ChunksChecker.ids), NULL: PosChecker */
+    const int32_t *chunk_ids; /* chunk (row, col) as (row - 1) * cols + (col - 1) */
+} mavi_ring_source;
+
+/* Make a rings system's population variable: the ActiveState mask
+ * (nrings, NULL: every ring) and its sources and sinks (at most 32 sinks).
+ * Call once, after the first upload: the constructor's update_ids!,
+ * centres of mass (0 for inactive rings) and first update_chunks! follow.
+ * Steps then run update_cms!, the sources and sinks, update_ids! and the
+ * rest of step! over the active rings; every step reads back a few words
+ * to decide (no graph replays).  The host noise of mavi_rings_step stays
+ * indexed by ring slot. */
+int mavi_rings_var_cfg(mavi_sys *sys, const int32_t *mask, int32_t nsrc, const mavi_ring_source *src);
+/* Uniform draws u in [0, 1) for random spawn polarisations (2 pi u),
+ * consumed in spawn order (one per empty box, found slot or not, as
+ * get_spawn_pol runs before add_ring!); appended to the pool. */
+int mavi_rings_pol_draws(mavi_sys *sys, const double *u, int64_t n);
+/* VarRingsIds: mask (nrings), ids (nrings, 1-based; past num_active the
+ * entries an earlier update left), uids (nrings), num_active, and the
+ * polarisation draws used so far; any may be NULL. */
+int mavi_rings_var_get(mavi_sys *sys, int32_t *mask, int64_t *ids, int64_t *uids, int64_t *num_active,
+                       int64_t *pol_used);
+
 /* RingsInfo: areas (nrings), cms (2*nrings), continuos_pos (2*nmax*nrings),
  * src/rings/rings.jl:118-128; any may be NULL. */
 int mavi_rings_info(mavi_sys *sys, double *areas, double *cms, double *continuous_pos);

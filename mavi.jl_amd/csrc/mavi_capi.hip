@@ -113,6 +113,9 @@ int mavi_sys_destroy(mavi_sys *s) {
     dfree(s->walls); dfree(s->lines); dfree(s->neigh_cnt); dfree(s->neigh_list);
     dfree(s->inv_cnt); dfree(s->inv_cell); dfree(s->inv_chunk); dfree(s->inv_box); dfree(s->inv_rec);
     dfree(s->inv_n);
+    dfree(s->rmask); dfree(s->dsinks); dfree(s->dboxes); dfree(s->box_mask); dfree(s->sink_n); dfree(s->sink_list);
+    dfree(s->nonempty);
+    if (s->vstage) (void)hipHostFree(s->vstage);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
     delete s;
@@ -414,6 +417,8 @@ int mavi_sys_create(mavi_ctx *ctx, const mavi_sys_desc *d, mavi_sys **out) {
         if (e == hipSuccess) e = up(s->types, host_types.data(), sizeof(RingType) * host_types.size());
         if (e == hipSuccess) e = up(s->inter, host_inter.data(), sizeof(RingInter) * host_inter.size());
         s->host_inter = host_inter;
+        s->host_ring_np.resize(P.nrings);
+        for (int32_t r = 0; r < P.nrings; r++) s->host_ring_np[r] = host_types[P.typed ? host_rtype[r] : 0].np;
     }
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     s->seed = h.seed;
@@ -596,6 +601,167 @@ static int run_invasions(mavi_sys *s) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Variable ring populations: update_sources! (src/rings/integration.jl:
+// 353-358, src/rings/sources.jl:187-260) and update_ids! at the start of a
+// step.  The device computes update_cms!, the sinks holding each ring
+// slot's centre of mass and the occupied spawn boxes; the host replays the
+// reference's sequential logic on its copy of VarRingsIds (list order, the
+// lowest free slot, the stale tail of ids); a changed population is pushed
+// back and the sorted state rebuilt from the id-order staging.
+// ---------------------------------------------------------------------------
+struct VarReadback {
+    uint32_t sink_n, pad;
+    unsigned long long nonempty;
+    int32_t list[2 * kSinkFetch];
+};
+
+static size_t var_stage_bytes(const mavi_sys *s) {
+    const size_t spawn = (size_t)kMaxBoxes * (2 * (size_t)s->P.nmax + 3) * sizeof(double);
+    return sizeof(VarReadback) + sizeof(int32_t) * ((size_t)s->P.nrings + (size_t)s->P.n) + spawn + 256;
+}
+
+// calc_active_ids! (src/rings/states.jl:200-223) on the host copy; active
+// receives the p_ids (0-based slots)
+static void var_update_ids(mavi_sys *s, std::vector<int32_t> &active) {
+    int64_t k = 0;
+    active.clear();
+    for (int32_t r = 0; r < s->P.nrings; r++) {
+        if (!s->vmask[r]) continue;
+        s->vids[k++] = r + 1;
+        for (int32_t i = 0; i < s->host_ring_np[r]; i++) active.push_back(r * s->P.nmax + i);
+    }
+    s->vnum_active = k;
+}
+
+// Push the host mask and active ids, the spawned rings and rebuild the state.
+struct VarSpawn {
+    int32_t slot, src, box;
+    double pol, cmx, cmy;
+};
+static int var_apply(mavi_sys *s, const std::vector<VarSpawn> &spawns, const std::vector<int32_t> &active) {
+    mavi_ctx *ctx = s->ctx;
+    hipStream_t st = ctx->stream;
+    char *base = (char *)s->vstage + sizeof(VarReadback);
+    int32_t *hmask = (int32_t *)base;
+    int32_t *hact = hmask + s->P.nrings;
+    double *hsp = (double *)(((uintptr_t)(hact + s->P.n) + 15) & ~(uintptr_t)15);
+    // positions of the current active particles back to id order
+    HIPCHK(ctx, mavi::launch_unpermute(s, false, false, st));
+    const int64_t nm = s->P.nmax;
+    for (size_t k = 0; k < spawns.size(); k++) {
+        const VarSpawn &v = spawns[k];
+        double *d = hsp + k * (2 * nm + 3);
+        memcpy(d, s->vsrc[v.src].spawn.data() + (size_t)v.box * 2 * nm, sizeof(double) * 2 * nm);
+        d[2 * nm] = v.pol;
+        d[2 * nm + 1] = v.cmx;
+        d[2 * nm + 2] = v.cmy;
+        HIPCHK(ctx, hipMemcpyAsync(s->pos_id + (int64_t)v.slot * nm, d, sizeof(double) * 2 * nm,
+                                   hipMemcpyHostToDevice, st));
+        HIPCHK(ctx, hipMemcpyAsync(s->pol + v.slot, d + 2 * nm, sizeof(double), hipMemcpyHostToDevice, st));
+        HIPCHK(ctx, hipMemcpyAsync(s->cms + v.slot, d + 2 * nm + 1, sizeof(double2), hipMemcpyHostToDevice, st));
+        HIPCHK(ctx, hipMemcpyAsync(s->cms_next + v.slot, d + 2 * nm + 1, sizeof(double2), hipMemcpyHostToDevice, st));
+    }
+    memcpy(hmask, s->vmask.data(), sizeof(int32_t) * s->P.nrings);
+    HIPCHK(ctx, hipMemcpyAsync(s->rmask, hmask, sizeof(int32_t) * s->P.nrings, hipMemcpyHostToDevice, st));
+    if (!active.empty()) {
+        memcpy(hact, active.data(), sizeof(int32_t) * active.size());
+        HIPCHK(ctx, hipMemcpyAsync(s->active_ids, hact, sizeof(int32_t) * active.size(), hipMemcpyHostToDevice, st));
+    }
+    s->P.na = (int32_t)active.size();
+    HIPCHK(ctx, mavi::launch_iota_state(s, st));
+    s->sorted = false;
+    return MAVI_OK;
+}
+
+static bool host_inside_sink(const HostSource &q, double x, double y) {
+    if (q.geom == MAVI_GEOM_RECT)
+        return (q.g[0] - 0.0 <= x && x <= q.g[0] + q.g[2] + 0.0) && (q.g[1] - 0.0 <= y && y <= q.g[1] + q.g[3] + 0.0);
+    const double dx = x - q.g[0], dy = y - q.g[1];
+    return dx * dx + dy * dy <= q.g[2] * q.g[2];
+}
+
+static int var_step_start(mavi_sys *s) {
+    mavi_ctx *ctx = s->ctx;
+    hipStream_t st = ctx->stream;
+    VarReadback *rb = (VarReadback *)s->vstage;
+    HIPCHK(ctx, mavi::launch_var_pre(s, st));
+    HIPCHK(ctx, hipMemcpyAsync(&rb->sink_n, s->sink_n, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx, hipMemcpyAsync(&rb->nonempty, s->nonempty, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    const int64_t fetch = s->P.nrings < kSinkFetch ? s->P.nrings : kSinkFetch;  // sink_list holds 2 x nrings
+    HIPCHK(ctx, hipMemcpyAsync(rb->list, s->sink_list, sizeof(int32_t) * 2 * fetch, hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    const uint32_t nhit = rb->sink_n;
+    std::vector<uint32_t> bits(s->P.nrings, 0u);
+    std::vector<int32_t> extra;
+    if ((int64_t)nhit > fetch) {
+        extra.resize(2 * (size_t)nhit);
+        HIPCHK(ctx, hipMemcpy(extra.data(), s->sink_list, sizeof(int32_t) * 2 * nhit, hipMemcpyDeviceToHost));
+    }
+    const int32_t *lst = extra.empty() ? rb->list : extra.data();
+    for (uint32_t k = 0; k < nhit; k++) bits[lst[2 * k]] = (uint32_t)lst[2 * k + 1];
+    const unsigned long long nonempty = rb->nonempty;
+    // the reference's sequential logic, in list order
+    std::vector<VarSpawn> spawns;
+    std::vector<char> added(s->P.nrings, 0);
+    std::vector<double> acm(2 * (size_t)s->P.nrings, 0.0);
+    bool changed = false;
+    for (size_t q = 0; q < s->vsrc.size(); q++) {
+        const HostSource &src = s->vsrc[q];
+        if (src.kind == MAVI_SRC_SINK) {
+            const int64_t nview = s->vnum_active;  // get_rings_ids(state) when the loop starts
+            for (int64_t k = 0; k < nview; k++) {
+                const int64_t r = s->vids[k] - 1;
+                const bool in = added[r] ? host_inside_sink(src, acm[2 * r], acm[2 * r + 1])
+                                         : ((bits[r] >> src.sink_bit) & 1u) != 0;
+                if (in) {  // remove_ring!
+                    s->vmask[r] = 0;
+                    s->vnum_active -= 1;
+                    changed = true;
+                }
+            }
+            continue;
+        }
+        const int64_t nm = s->P.nmax;
+        for (int32_t b = 0; b < src.nbox; b++) {
+            if ((nonempty >> (src.box0 + b)) & 1ull) continue;
+            double pol = src.pol;  // get_spawn_pol runs before add_ring! looks for a slot
+            if (src.random_pol) {
+                if (s->pol_used >= (int64_t)s->pol_draws.size())
+                    return set_err(ctx, MAVI_E_BADARG,
+                                   "spawn polarisation draws exhausted (give more with mavi_rings_pol_draws)");
+                pol = s->pol_draws[s->pol_used++] * 2.0 * 3.141592653589793;
+            }
+            int32_t slot = -1;
+            for (int32_t r = 0; r < s->P.nrings; r++)
+                if (!s->vmask[r]) { slot = r; break; }
+            if (slot < 0) continue;  // add_ring! found no room
+            s->vmask[slot] = 1;
+            int64_t umax = s->vuids[0];
+            for (int32_t r = 1; r < s->P.nrings; r++) umax = s->vuids[r] > umax ? s->vuids[r] : umax;
+            s->vuids[slot] = umax + 1;
+            s->vnum_active += 1;
+            // info.cms[ring_id] = sum(rings_pos[1:num_p, ring_id]) / num_p
+            const double *p = src.spawn.data() + (size_t)b * 2 * nm;
+            const int32_t np = s->host_ring_np[slot];
+            double sx = 0.0, sy = 0.0;
+            for (int32_t i = 0; i < np; i++) {
+                sx += p[2 * i];
+                sy += p[2 * i + 1];
+            }
+            acm[2 * slot] = sx / (double)np;
+            acm[2 * slot + 1] = sy / (double)np;
+            added[slot] = 1;
+            spawns.push_back(VarSpawn{slot, (int32_t)q, b, pol, acm[2 * slot], acm[2 * slot + 1]});
+            changed = true;
+        }
+    }
+    std::vector<int32_t> active;
+    var_update_ids(s, active);
+    if (!changed) return MAVI_OK;
+    return var_apply(s, spawns, active);
+}
+
 // update_invasions! (src/rings/integration.jl:511-520) at the start of a
 // step: num_steps - last_check >= steps_to_update.
 static bool invasions_due(const mavi_sys *s, int64_t ahead) {
@@ -610,6 +776,7 @@ static int steps_impl(mavi_sys *s, int64_t nsteps, const double *noise_host) {
     hipStream_t st = ctx->stream;
     // a due invasion check runs after the step's rings kernel, which leaves
     // cms and cpos as the reference's step sees them before its update
+    auto before_step = [&]() -> int { return s->var ? var_step_start(s) : MAVI_OK; };
     auto after_step = [&](bool due) -> int {
         if (!s->inv_sched) return MAVI_OK;
         s->inv_since = due ? 1 : s->inv_since + 1;
@@ -630,6 +797,8 @@ static int steps_impl(mavi_sys *s, int64_t nsteps, const double *noise_host) {
         HIPCHK(ctx, hipMemcpyAsync(s->noise, noise_host, sizeof(double) * need, hipMemcpyHostToDevice, st));
         for (int64_t k = 0; k < nsteps; k++) {
             const bool due = invasions_due(s, 0);
+            const int rc0 = before_step();
+            if (rc0 != MAVI_OK) return rc0;
             if (s->P.potential == MAVI_POT_RINGS) HIPCHK(ctx, mavi::launch_rings_step(s, s->noise + k * nr, st));
             else HIPCHK(ctx, mavi::launch_active_step(s, s->noise + k * nr, st));
             const int rc = after_step(due);
@@ -639,7 +808,7 @@ static int steps_impl(mavi_sys *s, int64_t nsteps, const double *noise_host) {
         // only Verlet steps carry forces from one step to the next; the
         // two-step graph replays from state copy 0 and holds no invasion check
         const bool rings = s->P.potential == MAVI_POT_RINGS || is_active(s);
-        const bool graphs = s->use_graphs && !s->prof && nsteps >= 4;
+        const bool graphs = s->use_graphs && !s->prof && nsteps >= 4 && !s->var;
         int64_t k = 0;
         while (k < nsteps) {
             if (graphs && k + 2 <= nsteps && s->cur == 0 && (rings || s->f_valid) && !invasions_due(s, 1)) {
@@ -650,6 +819,8 @@ static int steps_impl(mavi_sys *s, int64_t nsteps, const double *noise_host) {
                 continue;
             }
             const bool due = invasions_due(s, 0);
+            const int rc0 = before_step();
+            if (rc0 != MAVI_OK) return rc0;
             HIPCHK(ctx, launch_step(s, st, s->f_valid));
             const int rc = after_step(due);
             if (rc != MAVI_OK) return rc;
@@ -727,9 +898,11 @@ int mavi_rings_neighbors_get(mavi_sys *s, int64_t *counts, int64_t *lists) {
     HIPCHK(ctx, hipMemcpy(c.data(), s->neigh_cnt, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
     if (lists) HIPCHK(ctx, hipMemcpy(l.data(), s->neigh_list, sizeof(int32_t) * n * 15, hipMemcpyDeviceToHost));
     for (int64_t i = 0; i < n; i++) {
-        if (counts) counts[i] = c[i];
+        // neigh_clean! empties every list; a variable population fills the active ones
+        const bool act = !s->var || s->vmask[i / s->P.nmax];
+        if (counts) counts[i] = act ? c[i] : 0;
         if (lists)
-            for (int k = 0; k < 15; k++) lists[i * 15 + k] = l[i * 15 + k];
+            for (int k = 0; k < 15; k++) lists[i * 15 + k] = act ? l[i * 15 + k] : -1;
     }
     return MAVI_OK;
 }
@@ -821,6 +994,148 @@ int mavi_rings_invasions_get(mavi_sys *s, int64_t *count, int64_t *records, int6
     const int64_t m = n < cap ? n : cap;
     for (int64_t i = 0; i < m; i++)
         for (int k = 0; k < 3; k++) records[3 * i + k] = r[3 * order[i] + k];
+    return MAVI_OK;
+}
+
+int mavi_rings_var_cfg(mavi_sys *s, const int32_t *mask, int32_t nsrc, const mavi_ring_source *src) {
+    if (!s) return MAVI_E_BADARG;
+    mavi_ctx *ctx = s->ctx;
+    if (s->P.potential != MAVI_POT_RINGS) return set_err(ctx, MAVI_E_BADARG, "not a rings system");
+    if (!s->has_state) return set_err(ctx, MAVI_E_BADARG, "upload the state before configuring its population");
+    if (s->var) return set_err(ctx, MAVI_E_BADARG, "the population is already variable");
+    if (nsrc < 0 || (nsrc > 0 && !src)) return set_err(ctx, MAVI_E_BADARG, "bad source list");
+    std::vector<HostSource> vs;
+    int32_t nsinks = 0, nboxes = 0;
+    bool chunked = false;
+    const int64_t nm = s->P.nmax, ncell = (int64_t)s->P.ncell;
+    for (int32_t q = 0; q < nsrc; q++) {
+        const mavi_ring_source &d = src[q];
+        HostSource h;
+        h.kind = d.kind;
+        if (d.kind == MAVI_SRC_SINK) {
+            if (d.geom != MAVI_GEOM_RECT && d.geom != MAVI_GEOM_CIRCLE)
+                return set_err(ctx, MAVI_E_BADARG, "sink geometry must be a rectangle or a circle");
+            if (nsinks >= kMaxSinks) return set_err(ctx, MAVI_E_UNSUPPORTED, "more than 32 sinks");
+            h.geom = d.geom;
+            for (int k = 0; k < 4; k++) h.g[k] = d.g[k];
+            h.sink_bit = nsinks++;
+        } else if (d.kind == MAVI_SRC_SOURCE) {
+            if (d.nbox < 1 || !d.bbox || !d.spawn) return set_err(ctx, MAVI_E_BADARG, "source without boxes");
+            if (nboxes + d.nbox > kMaxBoxes) return set_err(ctx, MAVI_E_UNSUPPORTED, "more than 64 spawn boxes");
+            if (d.chunk_off) {
+                if (s->P.cols == 0)
+                    return set_err(ctx, MAVI_E_BADARG, "ChunksChecker lists given for a system without chunks");
+                for (int32_t e = d.chunk_off[0]; e < d.chunk_off[d.nbox]; e++)
+                    if (d.chunk_ids[e] < 0 || d.chunk_ids[e] >= ncell)
+                        return set_err(ctx, MAVI_E_BADARG, "ChunksChecker chunk index out of range");
+                chunked = true;
+            }
+            h.nbox = d.nbox;
+            h.random_pol = d.random_pol;
+            h.pad = d.pad;
+            h.pol = d.pol;
+            h.box0 = nboxes;
+            h.spawn.assign(d.spawn, d.spawn + (size_t)d.nbox * 2 * nm);
+            nboxes += d.nbox;
+        } else {
+            return set_err(ctx, MAVI_E_BADARG, "unknown source kind");
+        }
+        vs.push_back(std::move(h));
+    }
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int32_t nr = s->P.nrings;
+    HIPCHK(ctx, dalloc(&s->rmask, nr, st));
+    HIPCHK(ctx, dalloc(&s->dsinks, nsinks > 0 ? nsinks : 1, st));
+    HIPCHK(ctx, dalloc(&s->dboxes, nboxes > 0 ? nboxes : 1, st));
+    HIPCHK(ctx, dalloc(&s->sink_n, 1, st));
+    HIPCHK(ctx, dalloc(&s->sink_list, 2 * (int64_t)nr, st));
+    HIPCHK(ctx, dalloc(&s->nonempty, 1, st));
+    if (chunked) HIPCHK(ctx, dalloc(&s->box_mask, ncell, st));
+    s->vstage_bytes = var_stage_bytes(s);
+    HIPCHK(ctx, hipHostMalloc(&s->vstage, s->vstage_bytes, hipHostMallocDefault));
+    // device tables (synchronous copies: the host vectors are temporaries)
+    std::vector<DevSink> ds;
+    std::vector<DevBox> db;
+    std::vector<unsigned long long> bm(chunked ? (size_t)ncell : 0, 0ull);
+    for (int32_t q = 0; q < nsrc; q++) {
+        const mavi_ring_source &d = src[q];
+        if (d.kind == MAVI_SRC_SINK) {
+            DevSink k{};
+            k.geom = d.geom;
+            for (int j = 0; j < 4; j++) k.g[j] = d.g[j];
+            ds.push_back(k);
+            continue;
+        }
+        for (int32_t b = 0; b < d.nbox; b++) {
+            DevBox k{};
+            k.x = d.bbox[4 * b];
+            k.y = d.bbox[4 * b + 1];
+            k.len = d.bbox[4 * b + 2];
+            k.h = d.bbox[4 * b + 3];
+            k.pad = d.pad;
+            k.chunked = d.chunk_off ? 1 : 0;
+            const int32_t gb = (int32_t)db.size();
+            db.push_back(k);
+            if (d.chunk_off)
+                for (int32_t e = d.chunk_off[b]; e < d.chunk_off[b + 1]; e++) bm[d.chunk_ids[e]] |= 1ull << gb;
+        }
+    }
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    if (!ds.empty()) HIPCHK(ctx, hipMemcpy(s->dsinks, ds.data(), sizeof(DevSink) * ds.size(), hipMemcpyHostToDevice));
+    if (!db.empty()) HIPCHK(ctx, hipMemcpy(s->dboxes, db.data(), sizeof(DevBox) * db.size(), hipMemcpyHostToDevice));
+    if (chunked)
+        HIPCHK(ctx, hipMemcpy(s->box_mask, bm.data(), sizeof(unsigned long long) * bm.size(), hipMemcpyHostToDevice));
+    // VarRingsIds(mask, ids = 1:n, uids = 1:n), then update_ids!
+    s->vmask.assign(nr, 1);
+    if (mask)
+        for (int32_t r = 0; r < nr; r++) s->vmask[r] = mask[r] ? 1 : 0;
+    s->vids.resize(nr);
+    s->vuids.resize(nr);
+    for (int32_t r = 0; r < nr; r++) s->vids[r] = s->vuids[r] = r + 1;
+    s->vsrc = std::move(vs);
+    s->nsinks = nsinks;
+    s->nboxes = nboxes;
+    s->pol_draws.clear();
+    s->pol_used = 0;
+    s->var = true;
+    s->P.rmask = s->rmask;
+    if (s->graph2) {
+        (void)hipGraphExecDestroy(s->graph2);
+        s->graph2 = nullptr;
+    }
+    std::vector<int32_t> active;
+    var_update_ids(s, active);
+    // rebuild the state from the id-order staging with the active set, zero
+    // the inactive rings' centres of mass, then the constructor's first
+    // update_chunks!
+    int rc = var_apply(s, {}, active);
+    if (rc != MAVI_OK) return rc;
+    HIPCHK(ctx, mavi::launch_var_clear(s, st));
+    HIPCHK(ctx, mavi::launch_bin(s, st, false, false, false));
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    return check_flags(s);
+}
+
+int mavi_rings_pol_draws(mavi_sys *s, const double *u, int64_t n) {
+    if (!s || n < 0 || (n > 0 && !u)) return MAVI_E_BADARG;
+    if (!s->var) return set_err(s->ctx, MAVI_E_BADARG, "not a variable ring population (mavi_rings_var_cfg)");
+    s->pol_draws.insert(s->pol_draws.end(), u, u + n);
+    return MAVI_OK;
+}
+
+int mavi_rings_var_get(mavi_sys *s, int32_t *mask, int64_t *ids, int64_t *uids, int64_t *num_active,
+                       int64_t *pol_used) {
+    if (!s) return MAVI_E_BADARG;
+    if (!s->var) return set_err(s->ctx, MAVI_E_BADARG, "not a variable ring population (mavi_rings_var_cfg)");
+    const int32_t nr = s->P.nrings;
+    for (int32_t r = 0; r < nr; r++) {
+        if (mask) mask[r] = s->vmask[r];
+        if (ids) ids[r] = s->vids[r];
+        if (uids) uids[r] = s->vuids[r];
+    }
+    if (num_active) *num_active = s->vnum_active;
+    if (pol_used) *pol_used = s->pol_used;
     return MAVI_OK;
 }
 

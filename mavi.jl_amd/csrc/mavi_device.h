@@ -897,6 +897,7 @@ __global__ void k_rings_init(SysParams P, const int32_t *__restrict__ rtype, con
                              double2 *__restrict__ cms_next, double2 *__restrict__ cpos, double *__restrict__ areas) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= P.nrings) return;
+    if (P.rmask && !P.rmask[r]) return;  // inactive rings keep their info
     const int np = types[P.typed ? rtype[r] : 0].np;
     const double2 *x = pos_id + (int64_t)r * P.nmax;
     double2 cm;
@@ -1041,7 +1042,7 @@ __global__ __launch_bounds__(256) void k_rings_lanes(
     const double *__restrict__ noise, uint64_t seed, const unsigned long long *__restrict__ counter) {
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
     const int r = gt >> 5;
-    const bool ring_ok = r < P.nrings;
+    const bool ring_ok = r < P.nrings && (!P.rmask || P.rmask[r]);  // get_rings_ids
     const int rr = ring_ok ? r : 0;
     const RingType T = types[P.typed ? rtype[rr] : 0];
     ring_lanes<UPDATE, WALLS>(P, T, ring_ok, rr, gt & 31, slot_of, Pst, F, pol, areas, cms, cms_next, cpos, noise, seed,

@@ -26,6 +26,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/mavi_hip.h"
 
@@ -76,6 +77,8 @@ struct SysParams {
     int32_t nwalls, pad_w;
     const DevWall *walls;
     const DevLine *lines;
+    // rings with a variable population: 1 per active ring slot (null: all)
+    const int32_t *rmask;
 };
 
 // Per ring type parameters (wradius: get_particle_radius of the type, the
@@ -111,6 +114,33 @@ struct RingInter {
 struct InvGrid {
     int32_t rows, cols, nc, pad;
     double cl, ch;
+};
+
+// Sources and sinks of rings (src/rings/sources.jl) on the device: the
+// sinks' geometries and the sources' spawn boxes (bbox grown by pad is the
+// emptiness test; chunked: only the particles of the box's ChunksChecker
+// chunks count, else every active particle).
+struct DevSink {
+    int32_t geom, pad;
+    double g[4];
+};
+struct DevBox {
+    double x, y, len, h, pad;
+    int32_t chunked, pad2;
+};
+constexpr int kMaxSinks = 32;     // sink bits per ring
+constexpr int kMaxBoxes = 64;     // spawn boxes over all sources (one 64-bit word)
+constexpr int kSinkFetch = 512;   // sink hits read back with the step's first copy
+
+// Host copy of one source or sink (mavi_ring_source).
+struct HostSource {
+    int32_t kind = 0, geom = 0;
+    double g[4] = {0, 0, 0, 0};
+    int32_t nbox = 0, random_pol = 0;
+    double pad = 0.0, pol = 0.0;
+    int32_t box0 = 0;              // first global box index
+    int32_t sink_bit = 0;          // sinks: bit in the per-ring sink word
+    std::vector<double> spawn;     // nbox x nmax x 2
 };
 
 struct ProfSlot {
@@ -161,6 +191,7 @@ struct mavi_sys {
     double neigh_tol = 1.1;
     int32_t *neigh_cnt = nullptr, *neigh_list = nullptr;  // by particle id
     std::vector<RingInter> host_inter;
+    std::vector<int32_t> host_ring_np;  // particles per ring slot (its type's num_particles)
     // invasions of rings (InvasionsCfg, r_chunks)
     bool inv_sched = false;     // InvasionsCfg given: check every inv_every steps
     int64_t inv_every = 0, inv_since = 0;  // steps_to_update; num_steps - last_check
@@ -171,6 +202,24 @@ struct mavi_sys {
     uint32_t *inv_n = nullptr;  // records found by the last check
     int64_t inv_cap = 0, inv_count = 0;
     int32_t inv_cnt_cap = 0, inv_cell_cap = 0;  // allocated ring chunks / chunk slots
+    // rings with a variable population (VarRingsIds) and their sources/sinks
+    bool var = false;
+    std::vector<int32_t> vmask;         // host copy, authoritative
+    std::vector<int64_t> vids, vuids;   // VarRingsIds.ids (1-based, stale tail kept), uids
+    int64_t vnum_active = 0;
+    std::vector<HostSource> vsrc;
+    int32_t nsinks = 0, nboxes = 0;
+    std::vector<double> pol_draws;      // uniform draws for random spawn polarisations
+    int64_t pol_used = 0;
+    int32_t *rmask = nullptr;           // device mask (P.rmask)
+    DevSink *dsinks = nullptr;
+    DevBox *dboxes = nullptr;
+    unsigned long long *box_mask = nullptr;  // per chunk: boxes whose ChunksChecker lists it
+    uint32_t *sink_n = nullptr;         // sink hits this step
+    int32_t *sink_list = nullptr;       // (ring, sink bits) pairs
+    unsigned long long *nonempty = nullptr;  // spawn boxes holding a particle
+    void *vstage = nullptr;             // pinned host staging (read back and uploads)
+    size_t vstage_bytes = 0;
     DevWall *walls = nullptr;   // walls list (P.walls / P.lines point here)
     DevLine *lines = nullptr;
     int32_t *needc = nullptr;   // per chunk: binning epoch in which a chunk of its 3x3 block changed
@@ -239,12 +288,16 @@ hipError_t launch_ring_neigh(mavi_sys *s, hipStream_t st);
 // check_invasions!: ring boxes and the ring grid, then the crossing tests;
 // prep = false reruns only the tests (after the record buffer grew)
 hipError_t launch_invasions(mavi_sys *s, bool prep, hipStream_t st);
+// update_cms! over the active rings and the sinks' membership of every ring
+// slot, then the spawn boxes' emptiness (start of a variable-population step)
+hipError_t launch_var_pre(mavi_sys *s, hipStream_t st);
+// zero cms / cms_next of the inactive ring slots (RingsInfo.cms starts at zero)
+hipError_t launch_var_clear(mavi_sys *s, hipStream_t st);
 hipError_t launch_observables(mavi_sys *s, double *d_out, hipStream_t st);
 hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t st);
 }  // namespace mavi
 
 // Host-side result of validating a descriptor (mavi_capi.hip build_params).
-#include <vector>
 struct HostSys {
     SysParams P{};
     double list2 = INFINITY;

@@ -257,6 +257,68 @@ __global__ void k_ring_neigh(SysParams P, RingCtx R, const double2 *__restrict__
 }
 
 // ---------------------------------------------------------------------------
+// Start of a step of a variable ring population (src/rings/integration.jl:
+// 366-372, src/rings/sources.jl:187-260).  k_var_pre: one thread per ring
+// slot: update_cms! for the active rings (cms_next holds the mean of the
+// last continuous positions), then the sinks that hold the slot's centre of
+// mass (stale for inactive slots, as the reference's cms vector is), listed
+// as (ring, sink bits).  k_src_empty: one thread per active particle: the
+// spawn boxes (grown by pad) it lies in, among the boxes whose ChunksChecker
+// lists its chunk of the last binning (every box without chunks).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool inside_rect(const double x, const double y, const double bx, const double by,
+                                            const double len, const double h, const double pad) {
+    // is_inside(point, ::RectangleCfg; pad), src/configs.jl:89-93
+    return (bx - pad <= x && x <= bx + len + pad) && (by - pad <= y && y <= by + h + pad);
+}
+
+__global__ void k_var_pre(SysParams P, double2 *__restrict__ cms, const double2 *__restrict__ cms_next,
+                          const DevSink *__restrict__ sinks, int nsinks, uint32_t *__restrict__ sink_n,
+                          int32_t *__restrict__ sink_list) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= P.nrings) return;
+    double2 c = cms[r];
+    if (P.rmask[r]) {
+        c = cms_next[r];
+        cms[r] = c;
+    }
+    uint32_t bits = 0;
+    for (int j = 0; j < nsinks; j++) {
+        const DevSink S = sinks[j];
+        bool in;
+        if (S.geom == MAVI_GEOM_RECT) {
+            in = inside_rect(c.x, c.y, S.g[0], S.g[1], S.g[2], S.g[3], 0.0);
+        } else {  // is_inside(point, ::CircleCfg), src/configs.jl:165-168
+            const double dx = c.x - S.g[0], dy = c.y - S.g[1];
+            in = dx * dx + dy * dy <= S.g[2] * S.g[2];
+        }
+        if (in) bits |= 1u << j;
+    }
+    if (bits) {
+        const uint32_t k = atomicAdd(sink_n, 1u);
+        sink_list[2 * k] = r;
+        sink_list[2 * k + 1] = (int32_t)bits;
+    }
+}
+
+__global__ void k_src_empty(SysParams P, const double2 *__restrict__ Pst, const int32_t *__restrict__ scell,
+                            const DevBox *__restrict__ boxes, int nbox,
+                            const unsigned long long *__restrict__ box_mask,
+                            unsigned long long *__restrict__ nonempty) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= P.na) return;
+    const double2 p = Pst[s];
+    const unsigned long long listed = box_mask ? box_mask[scell[s]] : 0ull;
+    unsigned long long hit = 0;
+    for (int b = 0; b < nbox; b++) {
+        const DevBox B = boxes[b];
+        if (B.chunked && !((listed >> b) & 1ull)) continue;
+        if (inside_rect(p.x, p.y, B.x, B.y, B.len, B.h, B.pad)) hit |= 1ull << b;
+    }
+    if (hit) atomicOr(nonempty, hit);
+}
+
+// ---------------------------------------------------------------------------
 // Invasions of rings: check_invasions! (src/rings/integration.jl:379-509) at
 // the ring points (continuous positions cpos) and centres of mass cms the
 // last forces! step saw.
@@ -293,7 +355,7 @@ __global__ void k_inv_prep(SysParams P, InvGrid G, const int32_t *__restrict__ r
         y1 = fmax(y1, q.y);
     }
     box[r] = make_double4(x0, x1, y0, y1);
-    if (G.cols == 0) return;
+    if (G.cols == 0 || (P.rmask && !P.rmask[r])) return;  // r_chunks bin get_rings_ids
     uint32_t f = 0;
     const double2 m = cms[r];
     const int k = cell_in_grid(P, G.rows, G.cols, G.cl, G.ch, m.x, m.y, &f, MAVI_FLAG_RING_OUT_OF_BOX);
@@ -368,9 +430,12 @@ __global__ __launch_bounds__(64 * kInvWaves) void k_invasions(
         const bool act = i < np1;
         const double2 p = act ? c1[i] : make_double2(0.0, 0.0);
         if (G.cols == 0) {
-            for (int r2 = 0; r2 < P.nrings; r2++) test(r2, p, act, i);
+            // ring ids 1 .. length(get_rings_ids) (every ring of a fixed population)
+            if (r1 >= G.pad) return;
+            for (int r2 = 0; r2 < G.pad; r2++) test(r2, p, act, i);
             continue;
         }
+        if (P.rmask && !P.rmask[r1]) return;
         const int k = gchunk[r1];
         const int row = k / G.cols, col = k - row * G.cols;
         for (int dr = -1; dr <= 1; dr++) {
@@ -687,7 +752,7 @@ __global__ void k_rings(SysParams P, const int32_t *__restrict__ rtype, const Ri
                         double2 *__restrict__ cms_next, double2 *__restrict__ cpos, const double *__restrict__ noise,
                         uint64_t seed, const unsigned long long *__restrict__ counter) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= P.nrings) return;
+    if (r >= P.nrings || (P.rmask && !P.rmask[r])) return;  // get_rings_ids
     const RingType T = types[P.typed ? rtype[r] : 0];
     const int np = T.np;
     const int32_t *slot = slot_of + (int64_t)r * P.nmax;
@@ -977,6 +1042,7 @@ hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t st) {
 }
 
 hipError_t launch_iota_state(mavi_sys *s, hipStream_t st) {
+    if (s->P.na == 0) return hipSuccess;
     const int c = s->cur;
     const bool has_vel = s->P.potential != MAVI_POT_RINGS;
     hipLaunchKernelGGL(k_iota_state, dim3(nblk(s->P.na, 256)), dim3(256), 0, st, s->P.na, s->active_ids, s->pos_id,
@@ -985,6 +1051,11 @@ hipError_t launch_iota_state(mavi_sys *s, hipStream_t st) {
 }
 
 hipError_t launch_unpermute(mavi_sys *s, bool vel, bool force, hipStream_t st) {
+    if (force && s->var) {  // clean_forces! zeroes the forces of inactive particles too
+        hipError_t e = launch_fill_i32((int32_t *)s->force_id, 0, 4 * (int64_t)s->P.n, st);
+        if (e != hipSuccess) return e;
+    }
+    if (s->P.na == 0) return hipSuccess;
     const int c = s->cur;
     const bool has_vel = vel && s->P.potential != MAVI_POT_RINGS;
     hipLaunchKernelGGL(k_unpermute, dim3(nblk(s->P.na, 256)), dim3(256), 0, st, s->P.na, s->ID_[c], s->P_[c],
@@ -1000,8 +1071,9 @@ hipError_t launch_bin(mavi_sys *s, hipStream_t st, bool count_step, bool stamp, 
     const int c = s->cur, nc = 1 - c;
     // stamps need the previous binning's chunk of every slot (scell)
     const bool stamping = stamp && s->sorted;
-    hipLaunchKernelGGL(k_bin_count, dim3(nblk(P.na, 256)), dim3(256), 0, st, P, s->P_[c], s->key, s->rank, s->cnt,
-                       s->flags, s->scell, stamping ? s->stamp : nullptr, s->epoch);
+    if (P.na > 0)  // (a variable ring population can be empty)
+        hipLaunchKernelGGL(k_bin_count, dim3(nblk(P.na, 256)), dim3(256), 0, st, P, s->P_[c], s->key, s->rank,
+                           s->cnt, s->flags, s->scell, stamping ? s->stamp : nullptr, s->epoch);
     const int n = P.ncell + 1;
     const int ntiles = nblk(n, kScanTile2);
     hipLaunchKernelGGL(k_scan_lookback, dim3(ntiles), dim3(kScanThreads), 0, st, s->cnt, n, s->start,
@@ -1010,11 +1082,13 @@ hipError_t launch_bin(mavi_sys *s, hipStream_t st, bool count_step, bool stamp, 
     hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(256), 0, st, P, s->key, s->rank, s->start, s->src, s->scell,
                        s->ID_[c], s->sid, s->cnt, count_step ? s->counter : nullptr, stamping ? s->stamp : nullptr, s->epoch, s->needc);
     const bool has_vel = P.potential != MAVI_POT_RINGS && !defer;
-    hipLaunchKernelGGL(k_finalize, dim3(nblk(P.na, 256)), dim3(256), 0, st, P, s->start, s->src, s->scell, s->sid,
-                       s->P_[c],
-                       has_vel ? s->V_[c] : nullptr, s->ID_[c], s->P_[nc], has_vel ? s->V_[nc] : nullptr, s->ID_[nc],
-                       P.potential == MAVI_POT_RINGS ? s->slot_of : nullptr, s->flags, defer ? s->perm : nullptr,
-                       s->epoch);
+    if (P.na > 0) {
+        hipLaunchKernelGGL(k_finalize, dim3(nblk(P.na, 256)), dim3(256), 0, st, P, s->start, s->src, s->scell,
+                           s->sid, s->P_[c], has_vel ? s->V_[c] : nullptr, s->ID_[c], s->P_[nc],
+                           has_vel ? s->V_[nc] : nullptr, s->ID_[nc],
+                           P.potential == MAVI_POT_RINGS ? s->slot_of : nullptr, s->flags, defer ? s->perm : nullptr,
+                           s->epoch);
+    }
     s->cur = nc;
     s->sorted = true;
     return hipGetLastError();
@@ -1087,7 +1161,8 @@ hipError_t launch_ring_neigh(mavi_sys *s, hipStream_t st) {
 }
 
 hipError_t launch_invasions(mavi_sys *s, bool prep, hipStream_t st) {
-    const InvGrid G = s->inv_grid;
+    InvGrid G = s->inv_grid;
+    G.pad = s->var ? (int32_t)s->vnum_active : s->P.nrings;  // length(get_rings_ids)
     if (prep) {
         if (G.cols > 0) {
             hipError_t e = launch_fill_i32(s->inv_cnt, 0, (int64_t)G.rows * G.cols, st);
@@ -1103,7 +1178,31 @@ hipError_t launch_invasions(mavi_sys *s, bool prep, hipStream_t st) {
     return hipGetLastError();
 }
 
+__global__ void k_var_clear(SysParams P, double2 *__restrict__ cms, double2 *__restrict__ cms_next) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= P.nrings || P.rmask[r]) return;
+    cms[r] = make_double2(0.0, 0.0);
+    cms_next[r] = make_double2(0.0, 0.0);
+}
+
+hipError_t launch_var_clear(mavi_sys *s, hipStream_t st) {
+    hipLaunchKernelGGL(k_var_clear, dim3(nblk(s->P.nrings, 256)), dim3(256), 0, st, s->P, s->cms, s->cms_next);
+    return hipGetLastError();
+}
+
+hipError_t launch_var_pre(mavi_sys *s, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, (int32_t *)s->sink_n, 0, (int64_t)1);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, (int32_t *)s->nonempty, 0, (int64_t)2);
+    hipLaunchKernelGGL(k_var_pre, dim3(nblk(s->P.nrings, 256)), dim3(256), 0, st, s->P, s->cms, s->cms_next,
+                       s->dsinks, s->nsinks, s->sink_n, s->sink_list);
+    if (s->nboxes > 0 && s->P.na > 0)
+        hipLaunchKernelGGL(k_src_empty, dim3(nblk(s->P.na, 256)), dim3(256), 0, st, s->P, s->P_[s->cur], s->scell,
+                           s->dboxes, s->nboxes, s->box_mask, s->nonempty);
+    return hipGetLastError();
+}
+
 hipError_t launch_rings_forces(mavi_sys *s, hipStream_t st) {
+    if (s->P.na == 0) return hipSuccess;
     hipError_t e = launch_bin(s, st, false, false, false);
     if (e != hipSuccess) return e;
     {
@@ -1115,6 +1214,10 @@ hipError_t launch_rings_forces(mavi_sys *s, hipStream_t st) {
 }
 
 hipError_t launch_rings_step(mavi_sys *s, const double *noise_dev, hipStream_t st) {
+    if (s->P.na == 0) {  // no active ring: the step only advances the noise counter
+        hipLaunchKernelGGL(k_counter_inc, dim3(1), dim3(64), 0, st, s->counter);
+        return hipGetLastError();
+    }
     hipError_t e = launch_bin(s, st, true, false, false);
     if (e != hipSuccess) return e;
     if (s->P.cols == 0) {  // no binning kernel to advance the noise step counter

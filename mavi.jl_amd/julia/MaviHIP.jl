@@ -299,6 +299,89 @@ function invasions_download!(system, d::DeviceSys)
     return list
 end
 
+# --- rings: variable population, sources and sinks (src/rings/sources.jl) ----
+
+struct MaviRingSource
+    kind::Int32
+    geom::Int32
+    g::NTuple{4,Float64}
+    nbox::Int32
+    random_pol::Int32
+    pad::Float64
+    pol::Float64
+    bbox::Ptr{Float64}
+    spawn::Ptr{Float64}
+    chunk_off::Ptr{Int32}
+    chunk_ids::Ptr{Int32}
+end
+
+const MAVI_SRC_SOURCE = Int32(0)
+const MAVI_SRC_SINK = Int32(1)
+const MAVI_GEOM_RECT = Int32(0)
+const MAVI_GEOM_CIRCLE = Int32(1)
+
+sink_geom(g::RectangleCfg) = (MAVI_GEOM_RECT, (g.bottom_left[1], g.bottom_left[2], g.length, g.height))
+sink_geom(g::Configs.CircleCfg) = (MAVI_GEOM_CIRCLE, (g.center[1], g.center[2], g.radius, 0.0))
+
+"""Records of system.info.sources (Source / Sink objects the reference built,
+src/rings/rings.jl:154-186); the arrays they point into go to keep."""
+function source_records(system, keep)
+    recs = MaviRingSource[]
+    isnothing(system.info.sources) && return recs
+    cols = isnothing(system.chunks) ? 0 : system.chunks.num_cols
+    for s in system.info.sources
+        if s isa Mavi.Rings.Sources.Sink
+            geom, g = sink_geom(s.cfg.geometry_cfg)
+            push!(recs, MaviRingSource(MAVI_SRC_SINK, geom, g, 0, 0, 0.0, 0.0, C_NULL, C_NULL, C_NULL, C_NULL))
+            continue
+        bbox = Float64[]
+        for b in s.bbox
+            append!(bbox, (b.bottom_left[1], b.bottom_left[2], b.length, b.height))
+        end
+        spawn = Float64[]
+        for pts in s.bbox_spawn_pos, p in pts
+            append!(spawn, (p[1], p[2]))
+        end
+        off, ids = C_NULL, C_NULL
+        if s.checker isa Mavi.Rings.Sources.ChunksChecker
+            o = Int32[0]
+            l = Int32[]
+            for lst in s.checker.ids
+                for c in lst
+                    push!(l, Int32((c[1] - 1) * cols + (c[2] - 1)))
+                end
+                push!(o, length(l))
+            end
+            push!(keep, o, l)
+            off, ids = pointer(o), pointer(l)
+        end
+        push!(keep, bbox, spawn)
+        rnd = !(s.cfg.spawn_pol isa Number)
+        push!(recs, MaviRingSource(MAVI_SRC_SOURCE, 0, (0.0, 0.0, 0.0, 0.0), length(s.bbox), rnd ? 1 : 0,
+                                   s.cfg.pad, rnd ? 0.0 : s.cfg.spawn_pol, pointer(bbox), pointer(spawn), off, ids))
+    end
+    return recs
+end
+
+"""Hand a RingsSystem's VarRingsIds mask and sources to the device; random
+spawn polarisations come from `pol_draws!` (drawn from system.rng)."""
+function rings_var_cfg!(system, d::DeviceSys)
+    ids = system.state.rings_ids
+    ids isa Mavi.Rings.States.VarRingsIds || return nothing
+    keep = Any[]
+    recs = source_records(system, keep)
+    mask = Int32.(ids.mask)
+    GC.@preserve keep mask recs check(d.ctx, ccall((:mavi_rings_var_cfg, LIB), Cint,
+                                                   (Ptr{Cvoid}, Ptr{Int32}, Int32, Ptr{MaviRingSource}), d.sys,
+                                                   mask, length(recs), recs))
+end
+
+"Append n uniform draws of system.rng to the device's spawn polarisation pool."
+function pol_draws!(system, d::DeviceSys, n)
+    u = rand(system.rng, Float64, n)
+    check(d.ctx, ccall((:mavi_rings_pol_draws, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Int64), d.sys, u, n))
+end
+
 "run_system with a resident fast path: upload once, all steps on the device, download once."
 function run_gpu!(system; tf=nothing, num_steps=nothing)
     n = if isnothing(tf)

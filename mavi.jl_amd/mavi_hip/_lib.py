@@ -80,6 +80,26 @@ class MaviWall(C.Structure):
     ]
 
 
+MAVI_SRC_SOURCE, MAVI_SRC_SINK = 0, 1
+MAVI_GEOM_RECT, MAVI_GEOM_CIRCLE = 0, 1
+
+
+class MaviRingSource(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("geom", C.c_int32),
+        ("g", C.c_double * 4),
+        ("nbox", C.c_int32),
+        ("random_pol", C.c_int32),
+        ("pad", C.c_double),
+        ("pol", C.c_double),
+        ("bbox", C.POINTER(C.c_double)),
+        ("spawn", C.POINTER(C.c_double)),
+        ("chunk_off", C.POINTER(C.c_int32)),
+        ("chunk_ids", C.POINTER(C.c_int32)),
+    ]
+
+
 class SysDesc(C.Structure):
     _fields_ = [
         ("n", C.c_int64),
@@ -133,6 +153,9 @@ SIGNATURES = {
     "mavi_rings_info": (C.c_int, [_P, _DP, _DP, _DP]),
     "mavi_rings_neighbors_cfg": (C.c_int, [_P, C.c_int, C.c_double, C.c_int]),
     "mavi_rings_neighbors_get": (C.c_int, [_P, _I64P, _I64P]),
+    "mavi_rings_var_cfg": (C.c_int, [_P, C.POINTER(C.c_int32), C.c_int32, C.POINTER(MaviRingSource)]),
+    "mavi_rings_pol_draws": (C.c_int, [_P, C.POINTER(C.c_double), C.c_int64]),
+    "mavi_rings_var_get": (C.c_int, [_P, C.POINTER(C.c_int32), _I64P, _I64P, _I64P, _I64P]),
     "mavi_rings_invasions_cfg": (C.c_int, [_P, C.c_int, C.c_int64, C.c_int32, C.c_int32, C.c_int32]),
     "mavi_rings_invasions_check": (C.c_int, [_P]),
     "mavi_rings_invasions_get": (C.c_int, [_P, _I64P, _I64P, C.c_int64]),
@@ -312,6 +335,29 @@ class DeviceSystem:
         lists = np.zeros((self.n, 15), dtype=np.int64)
         self._call("mavi_rings_neighbors_get", i64ptr(counts), i64ptr(lists))
         return counts, lists
+
+    def rings_var_cfg(self, mask, records):
+        """records: list of MaviRingSource (their arrays kept alive by the caller)."""
+        m = np.ascontiguousarray(mask, dtype=np.int32)
+        arr = (MaviRingSource * max(len(records), 1))(*records)
+        self._call("mavi_rings_var_cfg", m.ctypes.data_as(C.POINTER(C.c_int32)), len(records),
+                   C.cast(arr, C.POINTER(MaviRingSource)))
+
+    def rings_pol_draws(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        self._call("mavi_rings_pol_draws", dptr(u), C.c_int64(len(u)))
+
+    def rings_var_state(self):
+        """(mask, ids, uids, num_active, pol draws used) of a variable population."""
+        nr = self.nrings
+        mask = np.zeros(nr, dtype=np.int32)
+        ids = np.zeros(nr, dtype=np.int64)
+        uids = np.zeros(nr, dtype=np.int64)
+        na = np.zeros(1, dtype=np.int64)
+        used = np.zeros(1, dtype=np.int64)
+        self._call("mavi_rings_var_get", mask.ctypes.data_as(C.POINTER(C.c_int32)), i64ptr(ids), i64ptr(uids),
+                   i64ptr(na), i64ptr(used))
+        return mask, ids, uids, int(na[0]), int(used[0])
 
     def rings_invasions_cfg(self, scheduled: bool, steps_to_update: int = 0, r_cols: int = 0, r_rows: int = 0,
                             r_nc: int = 0):

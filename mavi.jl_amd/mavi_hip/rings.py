@@ -401,6 +401,52 @@ class Source:  # src/rings/sources.jl:127-185
         return ids
 
 
+def source_records(sources, nmax: int, keep: list):
+    """C ABI records (mavi_ring_source) of Source / SinkCfg objects, in list
+    order; the arrays they point into are appended to keep."""
+    import ctypes as C
+
+    recs = []
+    for src in sources:
+        r = _lib.MaviRingSource()
+        if isinstance(src, SinkCfg):
+            g = src.geometry_cfg
+            r.kind = _lib.MAVI_SRC_SINK
+            if isinstance(g, CircleCfg):
+                r.geom, vals = _lib.MAVI_GEOM_CIRCLE, (g.center[0], g.center[1], g.radius, 0.0)
+            else:
+                r.geom, vals = _lib.MAVI_GEOM_RECT, (g.bottom_left[0], g.bottom_left[1], g.length, g.height)
+            for k, v in enumerate(vals):
+                r.g[k] = float(v)
+            recs.append(r)
+            continue
+        spawn = np.ascontiguousarray(np.stack(src.bbox_spawn_pos), dtype=np.float64)
+        if spawn.shape[1] != nmax:
+            raise ValueError(f"spawn_pos holds {spawn.shape[1]} points; add_ring! fills all {nmax} slots of a ring")
+        bbox = np.ascontiguousarray(src.bbox, dtype=np.float64)
+        r.kind = _lib.MAVI_SRC_SOURCE
+        r.nbox = len(bbox)
+        rnd = isinstance(src.cfg.spawn_pol, str)
+        if rnd and src.cfg.spawn_pol != "random":
+            raise ValueError(src.cfg.spawn_pol)
+        r.random_pol = 1 if rnd else 0
+        r.pol = 0.0 if rnd else float(src.cfg.spawn_pol)
+        r.pad = float(src.cfg.pad)
+        r.bbox = bbox.ctypes.data_as(C.POINTER(C.c_double))
+        r.spawn = spawn.ctypes.data_as(C.POINTER(C.c_double))
+        keep += [bbox, spawn]
+        if src.chunk_ids is not None:
+            off = np.zeros(len(bbox) + 1, dtype=np.int32)
+            off[1:] = np.cumsum([len(c) for c in src.chunk_ids])
+            ids = np.ascontiguousarray(np.concatenate([np.asarray(c, dtype=np.int32) for c in src.chunk_ids]
+                                                      + [np.zeros(1, np.int32)]), dtype=np.int32)
+            r.chunk_off = off.ctypes.data_as(C.POINTER(C.c_int32))
+            r.chunk_ids = ids.ctypes.data_as(C.POINTER(C.c_int32))
+            keep += [off, ids]
+        recs.append(r)
+    return recs
+
+
 @dataclass
 class NeighborsCfg:  # src/rings/neighbors.jl:11-15
     """Contact neighbours of the particles: type "all" or "rings" (pairs of
@@ -418,7 +464,10 @@ class RingsSystem:
 
     def __init__(self, *, state: RingsState, space_cfg: SpaceCfg, dynamic_cfg: RingsCfg, int_cfg,
                  time_info: TimeInfo | None = None, seed: int = 0, ctx: _lib.Context | None = None,
-                 p_neighbors_cfg: NeighborsCfg | None = None):
+                 p_neighbors_cfg: NeighborsCfg | None = None, source_cfg=None, rng=None):
+        """source_cfg: SourceCfg / SinkCfg (or a list of them), which makes the
+        population variable like state.rings_ids (ActiveState); rng draws the
+        random spawn polarisations (numpy Generator, default seeded by seed)."""
         d, self.chunks, self._tables = describe(state, space_cfg, dynamic_cfg, int_cfg, seed)
         self.state, self.space_cfg, self.dynamic_cfg, self.int_cfg = state, space_cfg, dynamic_cfg, int_cfg
         self.time_info = time_info or TimeInfo()
@@ -436,11 +485,59 @@ class RingsSystem:
         if getattr(extra, "r_chunks_cfg", None) is not None:
             self.r_chunks_grid = ring_chunks_grid(state, space_cfg, dynamic_cfg, extra.r_chunks_cfg)
         self.upload()
+        self.sources = None
+        self._rng = rng if rng is not None else np.random.default_rng(seed)
+        self._pol_supplied = 0
+        if state.rings_ids is not None or source_cfg is not None:
+            self._var_cfg(source_cfg)
         # the constructor's init sequence (src/rings/rings.jl:270-288): the
         # invasion schedule starts at this step count, then forces!
         self._invasions_cfg_call()
         self.forces[:] = self.dev.forces()
         self.areas, self.cms, self.continuos_pos = self.dev.rings_info()
+
+    def _var_cfg(self, source_cfg):
+        """RingsInfo's sources (src/rings/rings.jl:154-186) and the variable
+        population on the device."""
+        cfgs = [] if source_cfg is None else (list(source_cfg) if isinstance(source_cfg, (list, tuple))
+                                              else [source_cfg])
+        self.sources = []
+        for c in cfgs:
+            if isinstance(c, SourceCfg):
+                self.sources.append(Source(c, self.chunks))
+            elif isinstance(c, SinkCfg):
+                self.sources.append(c)
+            else:
+                raise ValueError(f"Unknown source configuration type: {type(c).__name__}")
+        st = self.state
+        if st.rings_ids is None:
+            st.rings_ids = VarRingsIds(np.ones(st.num_rings, bool), st.nmax, st.ring_num_particles)
+        self._src_keep = []
+        recs = source_records(self.sources, st.nmax, self._src_keep)
+        self._random_boxes = sum(len(s.bbox) for s in self.sources
+                                 if isinstance(s, Source) and isinstance(s.cfg.spawn_pol, str))
+        self.dev.rings_var_cfg(st.rings_ids.mask.astype(np.int32), recs)
+
+    def _supply_pol_draws(self, nsteps):
+        """Keep the device's pool of uniform draws ahead of the next nsteps
+        (at most one draw per random spawn box and step)."""
+        if not self._random_boxes:
+            return
+        used = self.dev.rings_var_state()[4]
+        need = used + nsteps * self._random_boxes - self._pol_supplied
+        if need > 0:
+            self.dev.rings_pol_draws(self._rng.random(need))
+            self._pol_supplied += need
+
+    def _sync_var(self):
+        if self.sources is None:
+            return
+        mask, ids, uids, na, _ = self.dev.rings_var_state()
+        r = self.state.rings_ids
+        r.mask[:] = mask != 0
+        r.update()
+        r.ids[:] = ids
+        r.uids[:] = uids
 
     def _invasions_cfg_call(self):
         if self.invasions_cfg is None and self.r_chunks_grid is None:
@@ -472,8 +569,11 @@ class RingsSystem:
         self.dev.download(self.state.pos, None, self.state.pol)
         self.forces[:] = self.dev.get_forces()
         self.areas, self.cms, self.continuos_pos = self.dev.rings_info()
+        self._sync_var()
 
     def advance(self, nsteps: int, noise=None):
+        if self.sources is not None:
+            self._supply_pol_draws(int(nsteps))
         if noise is None:
             self.dev.step(nsteps)
         else:
