@@ -129,6 +129,57 @@ def bench_rings(args):
     return out
 
 
+def bench_rings_sources(args, fixed_ms):
+    """The rings mixture with a variable population, a source of 8 x 8 spawn
+    boxes and a sink: every step adds update_cms!, the sink and spawn-box
+    tests, a read back of a few words and the host's source/sink logic; a
+    step that changes the population also rebuilds the state."""
+    import mavi_hip as mv
+    from mavi_hip import rings, workloads as wl
+
+    st, sp, cfg, ic, sources = wl.rings_sources(nrings=args.rings)
+    pos0, pol0 = st.pos.copy(), st.pol.copy()
+    mask0 = st.rings_ids.mask.copy()
+    system = rings.RingsSystem(state=st, space_cfg=sp, dynamic_cfg=cfg, int_cfg=ic, seed=2, source_cfg=sources,
+                               rng=np.random.default_rng(11))
+    dev = system.dev
+    na0 = dev.rings_var_state()[3]
+    log(f"[bench] rings with sources: {args.rings} ring slots, {na0} active")
+    system.advance(args.warmup)
+    uid0 = dev.rings_var_state()[2].max()
+    nrec, t0 = [], time.perf_counter()
+    k = 0
+    while k < args.sources_steps:
+        blk = min(100, args.sources_steps - k)
+        system.advance(blk)
+        nrec.append(dev.rings_var_state()[3])
+        k += blk
+    wall = time.perf_counter() - t0
+    _, _, uids, na, used = dev.rings_var_state()
+    nact = float(np.mean(nrec)) * 20
+    out = {"workload": f"rings_mixture ({args.rings} ring slots x 20) with a variable population: {int(na0)} rings "
+                       "active at the start; a source of 8 x 8 spawn boxes (random polarisation) and a rectangular "
+                       "sink; device Philox rotational noise",
+           "steps": args.sources_steps, "ms_per_step": wall / args.sources_steps * 1e3,
+           "value": nact * args.sources_steps / wall, "unit": "active particle-steps/s",
+           "active_rings_end": int(na), "spawned": int(uids.max() - uid0),
+           "ratio_ms_vs_fixed_rings": (wall / args.sources_steps * 1e3) / fixed_ms if fixed_ms else None}
+    if not args.no_cpu:
+        import oracle
+
+        o = oracle.from_rings(st, sp, cfg, ic)
+        o.set_var_rings(mask0.astype(np.int32), oracle.source_dicts(system.sources, st.nmax),
+                        pol_draws=np.random.default_rng(11).random(4096))
+        o.rings_var_init(pos0)
+        t0 = time.perf_counter()
+        o.rings_steps(pos0, pol0, args.sources_cpu_steps)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": na0 * 20 * args.sources_cpu_steps / dt, "unit": "active particle-steps/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"{args.sources_cpu_steps} steps of the same system on one core ({dt:.1f} s)"}
+    return out
+
+
 def bench_walls(args, plain_value):
     """The plain 1M LJ workload inside a walls list (ManyWalls: periodic box,
     16 slippery circles, 8 slippery lines, one potential circle): the cost
@@ -412,6 +463,8 @@ def main():
     ap.add_argument("--rings-steps", type=int, default=500)
     ap.add_argument("--rings-cpu-steps", type=int, default=40)
     ap.add_argument("--inv-checks", type=int, default=20, help="timed invasion checks in the rings form")
+    ap.add_argument("--sources-steps", type=int, default=500, help="steps of the rings-with-sources form (0 = skip)")
+    ap.add_argument("--sources-cpu-steps", type=int, default=10)
     ap.add_argument("--walls-steps", type=int, default=500, help="timed steps of the walls form (0 = skip)")
     ap.add_argument("--active-nside", type=int, default=1024,
                     help="side of the Szabo and run-and-tumble lattices (0 = skip those forms)")
@@ -528,6 +581,8 @@ def main():
     del system, dev
     if args.rings > 0:
         forms["rings"] = bench_rings(args)
+        if args.sources_steps > 0:
+            forms["rings_sources"] = bench_rings_sources(args, forms["rings"]["ms_per_step"])
     if args.walls_steps > 0:
         forms["walls"] = bench_walls(args, value)
     if args.active_nside > 0:

@@ -2,6 +2,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <cstddef>
 #include <cstdio>
 #include <string>
 #include <algorithm>
@@ -113,8 +114,8 @@ int mavi_sys_destroy(mavi_sys *s) {
     dfree(s->walls); dfree(s->lines); dfree(s->neigh_cnt); dfree(s->neigh_list);
     dfree(s->inv_cnt); dfree(s->inv_cell); dfree(s->inv_chunk); dfree(s->inv_box); dfree(s->inv_rec);
     dfree(s->inv_n);
-    dfree(s->rmask); dfree(s->dsinks); dfree(s->dboxes); dfree(s->box_mask); dfree(s->sink_n); dfree(s->sink_list);
-    dfree(s->nonempty);
+    dfree(s->rmask); dfree(s->dsinks); dfree(s->dboxes); dfree(s->box_mask);
+    if (s->vrb) (void)hipFree(s->vrb);
     if (s->vstage) (void)hipHostFree(s->vstage);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
@@ -610,11 +611,14 @@ static int run_invasions(mavi_sys *s) {
 // lowest free slot, the stale tail of ids); a changed population is pushed
 // back and the sorted state rebuilt from the id-order staging.
 // ---------------------------------------------------------------------------
-struct VarReadback {
+struct VarReadback {  // the layout of the device read-back buffer's first bytes
     uint32_t sink_n, pad;
     unsigned long long nonempty;
     int32_t list[2 * kSinkFetch];
 };
+constexpr size_t kVarHeader = 16;
+static_assert(offsetof(VarReadback, nonempty) == 8 && offsetof(VarReadback, list) == kVarHeader,
+              "read-back header layout");
 
 static size_t var_stage_bytes(const mavi_sys *s) {
     const size_t spawn = (size_t)kMaxBoxes * (2 * (size_t)s->P.nmax + 3) * sizeof(double);
@@ -686,13 +690,11 @@ static int var_step_start(mavi_sys *s) {
     hipStream_t st = ctx->stream;
     VarReadback *rb = (VarReadback *)s->vstage;
     HIPCHK(ctx, mavi::launch_var_pre(s, st));
-    HIPCHK(ctx, hipMemcpyAsync(&rb->sink_n, s->sink_n, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    HIPCHK(ctx, hipMemcpyAsync(&rb->nonempty, s->nonempty, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-    const int64_t fetch = s->P.nrings < kSinkFetch ? s->P.nrings : kSinkFetch;  // sink_list holds 2 x nrings
-    HIPCHK(ctx, hipMemcpyAsync(rb->list, s->sink_list, sizeof(int32_t) * 2 * fetch, hipMemcpyDeviceToHost, st));
+    const int64_t fetch = s->P.nrings < kSinkFetch ? s->P.nrings : kSinkFetch;  // the list holds 2 x nrings
+    HIPCHK(ctx, hipMemcpyAsync(rb, s->vrb, kVarHeader + sizeof(int32_t) * 2 * fetch, hipMemcpyDeviceToHost, st));
     HIPCHK(ctx, hipStreamSynchronize(st));
     const uint32_t nhit = rb->sink_n;
-    std::vector<uint32_t> bits(s->P.nrings, 0u);
+    std::vector<uint32_t> &bits = s->vbits;
     std::vector<int32_t> extra;
     if ((int64_t)nhit > fetch) {
         extra.resize(2 * (size_t)nhit);
@@ -703,8 +705,8 @@ static int var_step_start(mavi_sys *s) {
     const unsigned long long nonempty = rb->nonempty;
     // the reference's sequential logic, in list order
     std::vector<VarSpawn> spawns;
-    std::vector<char> added(s->P.nrings, 0);
-    std::vector<double> acm(2 * (size_t)s->P.nrings, 0.0);
+    std::vector<char> &added = s->vadded;
+    std::vector<double> &acm = s->vacm;
     bool changed = false;
     for (size_t q = 0; q < s->vsrc.size(); q++) {
         const HostSource &src = s->vsrc[q];
@@ -756,9 +758,12 @@ static int var_step_start(mavi_sys *s) {
             changed = true;
         }
     }
+    for (uint32_t k = 0; k < nhit; k++) bits[lst[2 * k]] = 0u;  // scratch back to zero
+    for (const VarSpawn &v : spawns) added[v.slot] = 0;
+    // an unchanged mask leaves update_ids!'s result as it is
+    if (!changed) return MAVI_OK;
     std::vector<int32_t> active;
     var_update_ids(s, active);
-    if (!changed) return MAVI_OK;
     return var_apply(s, spawns, active);
 }
 
@@ -1048,9 +1053,17 @@ int mavi_rings_var_cfg(mavi_sys *s, const int32_t *mask, int32_t nsrc, const mav
     HIPCHK(ctx, dalloc(&s->rmask, nr, st));
     HIPCHK(ctx, dalloc(&s->dsinks, nsinks > 0 ? nsinks : 1, st));
     HIPCHK(ctx, dalloc(&s->dboxes, nboxes > 0 ? nboxes : 1, st));
-    HIPCHK(ctx, dalloc(&s->sink_n, 1, st));
-    HIPCHK(ctx, dalloc(&s->sink_list, 2 * (int64_t)nr, st));
-    HIPCHK(ctx, dalloc(&s->nonempty, 1, st));
+    {
+        char *vrb = nullptr;
+        HIPCHK(ctx, dalloc(&vrb, (int64_t)kVarHeader + sizeof(int32_t) * 2 * (int64_t)nr, st));
+        s->vrb = vrb;
+        s->sink_n = (uint32_t *)vrb;
+        s->nonempty = (unsigned long long *)(vrb + 8);
+        s->sink_list = (int32_t *)(vrb + kVarHeader);
+    }
+    s->vbits.assign(nr, 0u);
+    s->vadded.assign(nr, 0);
+    s->vacm.assign(2 * (size_t)nr, 0.0);
     if (chunked) HIPCHK(ctx, dalloc(&s->box_mask, ncell, st));
     s->vstage_bytes = var_stage_bytes(s);
     HIPCHK(ctx, hipHostMalloc(&s->vstage, s->vstage_bytes, hipHostMallocDefault));

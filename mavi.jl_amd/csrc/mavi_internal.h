@@ -215,9 +215,15 @@ struct mavi_sys {
     DevSink *dsinks = nullptr;
     DevBox *dboxes = nullptr;
     unsigned long long *box_mask = nullptr;  // per chunk: boxes whose ChunksChecker lists it
-    uint32_t *sink_n = nullptr;         // sink hits this step
-    int32_t *sink_list = nullptr;       // (ring, sink bits) pairs
-    unsigned long long *nonempty = nullptr;  // spawn boxes holding a particle
+    // per-step read back, one buffer: header (sink hits, occupied spawn
+    // boxes) then the (ring, sink bits) pairs
+    void *vrb = nullptr;
+    uint32_t *sink_n = nullptr;         // -> vrb header
+    unsigned long long *nonempty = nullptr;
+    int32_t *sink_list = nullptr;       // -> after the header, 2 x nrings
+    std::vector<uint32_t> vbits;        // host scratch: sink bits per ring slot (kept zero between steps)
+    std::vector<char> vadded;           // host scratch: spawned this step
+    std::vector<double> vacm;           // host scratch: centres of mass of this step's spawns
     void *vstage = nullptr;             // pinned host staging (read back and uploads)
     size_t vstage_bytes = 0;
     DevWall *walls = nullptr;   // walls list (P.walls / P.lines point here)

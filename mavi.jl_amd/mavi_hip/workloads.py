@@ -39,6 +39,13 @@ and the CPU oracle.
                      lattice at number density 0.8 (spacing 1.118, inside the
                      2^(1/6) sigma range) jittered by +-0.02 (seed), periodic,
                      chunks of side >= 2^(1/6) sigma, dt = 1e-3
+  rings_sources      rings_mixture with a variable population: the rings
+                     whose centres lie within one ring radius of a source
+                     region start inactive, and 20% of the others (seed 6);
+                     one source of 8 x 8 spawn boxes (a regular 20-gon of the
+                     mixture's placement radius each, pad = particle radius,
+                     random spawn polarisation) and one rectangular sink of
+                     width 4 ring radii and height L / 2 at the far side
 """
 from __future__ import annotations
 
@@ -198,3 +205,30 @@ def rtp_gas(nside=1024, density=0.8, seed=5, tumble_rate=1.0, dt=1e-3, jitter=0.
     space = SpaceCfg(wall_type=PeriodicWalls(), geometry_cfg=RectangleCfg(length=L, height=L))
     int_cfg = IntCfg(dt=dt, chunks_cfg=ChunksCfg(num_cols=cols, num_rows=cols), device=GPU())
     return pos, ang, space, dyn, int_cfg
+
+
+def rings_sources(nrings=40000, seed=6):
+    """Returns (state, space_cfg, dynamic_cfg, int_cfg, [SourceCfg, SinkCfg])."""
+    from .rings import ActiveState, RingsState, SinkCfg, SourceCfg
+
+    st, sp, cfg, ic = rings_mixture(nrings=nrings)
+    L = sp.geometry_cfg.length
+    nper, l0, p_r = 20, 1.0, 0.5
+    ring_r = l0 / (2 * math.sin(math.pi / nper))
+    r_out = ring_r + p_r
+    ang = np.arange(nper) * 2 * math.pi / nper
+    spawn = np.stack([np.cos(ang), np.sin(ang)], axis=1) * (ring_r * 0.97)
+    box = 2 * ring_r * 0.97 + 2 * p_r  # spawn box side (template extent + 2 pad)
+    x0, y0 = 0.1 * L, 0.5 * L - 4 * box
+    m = int(math.ceil(math.sqrt(nrings)))
+    k = np.arange(nrings)
+    centres = np.stack([(k % m + 0.5) * (L / m), (k // m + 0.5) * (L / m)], axis=1)
+    inside = ((centres[:, 0] > x0 - r_out) & (centres[:, 0] < x0 + 8 * box + r_out) &
+              (centres[:, 1] > y0 - r_out) & (centres[:, 1] < y0 + 8 * box + r_out))
+    rng = np.random.default_rng(seed)
+    mask = ~inside & (rng.random(nrings) >= 0.2)
+    state = RingsState(rings_pos=st.rings_pos, pol=st.pol, num_particles=st.num_particles, types=st.types,
+                       active_state=ActiveState(mask))
+    sources = [SourceCfg(bottom_left=(x0, y0), spawn_pos=spawn, spawn_pol="random", pad=p_r, size=(8, 8)),
+               SinkCfg(RectangleCfg(length=4 * r_out, height=0.5 * L, bottom_left=(0.8 * L, 0.25 * L)))]
+    return state, sp, cfg, ic, sources
