@@ -155,7 +155,7 @@ def bench_rings_sources(args, fixed_ms):
         nrec.append(dev.rings_var_state()[3])
         k += blk
     wall = time.perf_counter() - t0
-    _, _, uids, na, used = dev.rings_var_state()
+    _, _, uids, na, used, _ = dev.rings_var_state()
     nact = float(np.mean(nrec)) * 20
     out = {"workload": f"rings_mixture ({args.rings} ring slots x 20) with a variable population: {int(na0)} rings "
                        "active at the start; a source of 8 x 8 spawn boxes (random polarisation) and a rectangular "

@@ -241,13 +241,17 @@ typedef struct {
 int mavi_rings_var_cfg(mavi_sys *sys, const int32_t *mask, int32_t nsrc, const mavi_ring_source *src);
 /* Uniform draws u in [0, 1) for random spawn polarisations (2 pi u),
  * consumed in spawn order (one per empty box, found slot or not, as
- * get_spawn_pol runs before add_ring!); appended to the pool. */
+ * get_spawn_pol runs before add_ring!); appended to the pool.  Once the
+ * pool is empty the draws come from a built-in stream (Philox4x32-10 keyed
+ * by the system's seed and the stream position). */
 int mavi_rings_pol_draws(mavi_sys *sys, const double *u, int64_t n);
+/* Position of the built-in spawn polarisation stream (to resume a saved run). */
+int mavi_rings_pol_stream_set(mavi_sys *sys, int64_t position);
 /* VarRingsIds: mask (nrings), ids (nrings, 1-based; past num_active the
- * entries an earlier update left), uids (nrings), num_active, and the
- * polarisation draws used so far; any may be NULL. */
+ * entries an earlier update left), uids (nrings), num_active, the pool draws
+ * used so far and the built-in stream's position; any may be NULL. */
 int mavi_rings_var_get(mavi_sys *sys, int32_t *mask, int64_t *ids, int64_t *uids, int64_t *num_active,
-                       int64_t *pol_used);
+                       int64_t *pol_used, int64_t *pol_stream);
 
 /* RingsInfo: areas (nrings), cms (2*nrings), continuos_pos (2*nmax*nrings),
  * src/rings/rings.jl:118-128; any may be NULL. */
@@ -293,6 +297,14 @@ int mavi_sys_set_tile(mavi_sys *sys, int32_t workgroup);
  *                       chunk blocks kept their members (same positions,
  *                       same candidates in the same order: bit-identical);
  *                       0: always evaluate */
+/* Position of the device noise stream (Philox4x32-10 keyed by the seed and
+ * a step counter that every step advances): saving both with a state and
+ * setting them on the system rebuilt from it continues the stream exactly
+ * (the role the serialized system.rng plays in save_system/load_system,
+ * src/serder.jl:62-66, 140-146). */
+int mavi_sys_noise_get(mavi_sys *sys, uint64_t *seed, uint64_t *counter);
+int mavi_sys_noise_set(mavi_sys *sys, uint64_t seed, uint64_t counter);
+
 #define MAVI_OPT_TWO_PHASE 0
 #define MAVI_OPT_GRAPHS 1
 #define MAVI_OPT_LDS_TILES 2
@@ -330,6 +342,11 @@ int mavi_batch_get_forces(mavi_batch *batch, int32_t index, double *forces);
  * the device rotational noise).  A failing replica is named in
  * mavi_last_error. */
 int mavi_batch_run(mavi_batch *batch, int64_t nsteps);
+/* The batch's step counter (shared by its replicas' noise streams, each
+ * keyed by its own seed): replicas resumed together at step k continue with
+ * the counter set to k. */
+int mavi_batch_noise_get_counter(mavi_batch *b, uint64_t *counter);
+int mavi_batch_noise_set_counter(mavi_batch *b, uint64_t counter);
 /* Batch execution options:
  *   MAVI_BOPT_WG_PLAIN / MAVI_BOPT_WG_RINGS  pair-kernel workgroup (64..1024)
  *       for LJ/HarmTrunc replicas / rings replicas (default 256)

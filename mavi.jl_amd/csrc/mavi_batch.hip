@@ -649,6 +649,29 @@ int mavi_batch_run(mavi_batch *b, int64_t nsteps) {
     return run_impl(b, nsteps);
 }
 
+int mavi_batch_noise_get_counter(mavi_batch *b, uint64_t *counter) {
+    if (!b || !counter) return MAVI_E_BADARG;
+    mavi_ctx *ctx = b->ctx;
+    if (!b->finalized) return set_err(ctx, MAVI_E_BADARG, "batch not finalized");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    unsigned long long c = 0;
+    HIPCHK(ctx, hipMemcpyAsync(&c, b->counter, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    *counter = c;
+    return MAVI_OK;
+}
+
+int mavi_batch_noise_set_counter(mavi_batch *b, uint64_t counter) {
+    if (!b) return MAVI_E_BADARG;
+    mavi_ctx *ctx = b->ctx;
+    if (!b->finalized) return set_err(ctx, MAVI_E_BADARG, "batch not finalized");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    const unsigned long long c = counter;
+    HIPCHK(ctx, hipMemcpy(b->counter, &c, sizeof(c), hipMemcpyHostToDevice));
+    return MAVI_OK;
+}
+
 int mavi_batch_set_option(mavi_batch *b, int32_t option, int32_t value) {
     if (!b) return MAVI_E_BADARG;
     mavi_ctx *ctx = b->ctx;

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "mavi_host.h"
+#include "mavi_philox.h"
 
 namespace {
 
@@ -611,6 +612,8 @@ static int run_invasions(mavi_sys *s) {
 // lowest free slot, the stale tail of ids); a changed population is pushed
 // back and the sorted state rebuilt from the id-order staging.
 // ---------------------------------------------------------------------------
+constexpr uint32_t kPolStreamId = 0xFFFFFFFFu;  // the spawn polarisation stream's id word
+
 struct VarReadback {  // the layout of the device read-back buffer's first bytes
     uint32_t sink_n, pad;
     unsigned long long nonempty;
@@ -729,10 +732,13 @@ static int var_step_start(mavi_sys *s) {
             if ((nonempty >> (src.box0 + b)) & 1ull) continue;
             double pol = src.pol;  // get_spawn_pol runs before add_ring! looks for a slot
             if (src.random_pol) {
-                if (s->pol_used >= (int64_t)s->pol_draws.size())
-                    return set_err(ctx, MAVI_E_BADARG,
-                                   "spawn polarisation draws exhausted (give more with mavi_rings_pol_draws)");
-                pol = s->pol_draws[s->pol_used++] * 2.0 * 3.141592653589793;
+                double u, w;
+                if (s->pol_used < (int64_t)s->pol_draws.size()) {
+                    u = s->pol_draws[s->pol_used++];
+                } else {  // the built-in stream: Philox4x32-10 keyed by the system's seed
+                    philox_uniform2(s->seed, (uint64_t)s->pol_stream++, kPolStreamId, &u, &w);
+                }
+                pol = u * 2.0 * 3.141592653589793;  // rand(rng, T) * 2 * pi
             }
             int32_t slot = -1;
             for (int32_t r = 0; r < s->P.nrings; r++)
@@ -1111,6 +1117,7 @@ int mavi_rings_var_cfg(mavi_sys *s, const int32_t *mask, int32_t nsrc, const mav
     s->nboxes = nboxes;
     s->pol_draws.clear();
     s->pol_used = 0;
+    s->pol_stream = 0;
     s->var = true;
     s->P.rmask = s->rmask;
     if (s->graph2) {
@@ -1137,8 +1144,15 @@ int mavi_rings_pol_draws(mavi_sys *s, const double *u, int64_t n) {
     return MAVI_OK;
 }
 
+int mavi_rings_pol_stream_set(mavi_sys *s, int64_t position) {
+    if (!s || position < 0) return MAVI_E_BADARG;
+    if (!s->var) return set_err(s->ctx, MAVI_E_BADARG, "not a variable ring population (mavi_rings_var_cfg)");
+    s->pol_stream = position;
+    return MAVI_OK;
+}
+
 int mavi_rings_var_get(mavi_sys *s, int32_t *mask, int64_t *ids, int64_t *uids, int64_t *num_active,
-                       int64_t *pol_used) {
+                       int64_t *pol_used, int64_t *pol_stream) {
     if (!s) return MAVI_E_BADARG;
     if (!s->var) return set_err(s->ctx, MAVI_E_BADARG, "not a variable ring population (mavi_rings_var_cfg)");
     const int32_t nr = s->P.nrings;
@@ -1149,6 +1163,7 @@ int mavi_rings_var_get(mavi_sys *s, int32_t *mask, int64_t *ids, int64_t *uids, 
     }
     if (num_active) *num_active = s->vnum_active;
     if (pol_used) *pol_used = s->pol_used;
+    if (pol_stream) *pol_stream = s->pol_stream;
     return MAVI_OK;
 }
 
@@ -1229,6 +1244,33 @@ int mavi_sys_set_tile(mavi_sys *s, int32_t workgroup) {
         s->graph2 = nullptr;
     }
     s->wg = workgroup;
+    return MAVI_OK;
+}
+
+int mavi_sys_noise_get(mavi_sys *s, uint64_t *seed, uint64_t *counter) {
+    if (!s) return MAVI_E_BADARG;
+    mavi_ctx *ctx = s->ctx;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    unsigned long long c = 0;
+    HIPCHK(ctx, hipMemcpyAsync(&c, s->counter, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (seed) *seed = s->seed;
+    if (counter) *counter = c;
+    return MAVI_OK;
+}
+
+int mavi_sys_noise_set(mavi_sys *s, uint64_t seed, uint64_t counter) {
+    if (!s) return MAVI_E_BADARG;
+    mavi_ctx *ctx = s->ctx;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    const unsigned long long c = counter;
+    HIPCHK(ctx, hipMemcpy(s->counter, &c, sizeof(c), hipMemcpyHostToDevice));
+    if (seed != s->seed && s->graph2) {  // the captured graph holds the seed
+        (void)hipGraphExecDestroy(s->graph2);
+        s->graph2 = nullptr;
+    }
+    s->seed = seed;
     return MAVI_OK;
 }
 

@@ -5,6 +5,8 @@
 #pragma once
 #include "mavi_internal.h"
 
+#include "mavi_philox.h"
+
 namespace {
 
 constexpr int kXCD = 8;  // MI355X: workgroups go round-robin over 8 XCDs
@@ -205,36 +207,6 @@ __device__ __forceinline__ double2 cdiff(const SysParams &P, double2 a, double2 
         dy = min_image(dy, P.H, P.halfH);
     }
     return make_double2(dx, dy);
-}
-
-// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) -> one standard normal
-// through Box-Muller on two 53-bit uniforms.
-__device__ __forceinline__ void philox_round(uint32_t &c0, uint32_t &c1, uint32_t &c2, uint32_t &c3, uint32_t k0,
-                                             uint32_t k1) {
-    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
-    uint32_t hi0 = __umulhi(M0, c0), lo0 = M0 * c0;
-    uint32_t hi1 = __umulhi(M1, c2), lo1 = M1 * c2;
-    uint32_t n0 = hi1 ^ c1 ^ k0, n1 = lo1, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
-    c0 = n0;
-    c1 = n1;
-    c2 = n2;
-    c3 = n3;
-}
-
-// Two uniforms in [0, 1) with 53 random bits each from Philox4x32-10 keyed
-// by (seed, step, id); a counter word different from philox_normal's keeps
-// the streams apart.
-__device__ void philox_uniform2(uint64_t seed, uint64_t step, uint32_t id, double *u, double *w) {
-    uint32_t c0 = (uint32_t)step, c1 = (uint32_t)(step >> 32), c2 = id, c3 = 0x72746d62u;
-    uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
-#pragma unroll
-    for (int r = 0; r < 10; r++) {
-        philox_round(c0, c1, c2, c3, k0, k1);
-        k0 += 0x9E3779B9u;
-        k1 += 0xBB67AE85u;
-    }
-    *u = (double)((((uint64_t)c0 << 21) ^ ((uint64_t)c1 >> 11)) & ((1ull << 53) - 1)) * 0x1p-53;
-    *w = (double)((((uint64_t)c2 << 21) ^ ((uint64_t)c3 >> 11)) & ((1ull << 53) - 1)) * 0x1p-53;
 }
 
 __device__ double philox_normal(uint64_t seed, uint64_t step, uint32_t ring) {

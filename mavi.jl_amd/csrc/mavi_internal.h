@@ -211,6 +211,7 @@ struct mavi_sys {
     int32_t nsinks = 0, nboxes = 0;
     std::vector<double> pol_draws;      // uniform draws for random spawn polarisations
     int64_t pol_used = 0;
+    int64_t pol_stream = 0;             // draws taken from the built-in stream once the pool is empty
     int32_t *rmask = nullptr;           // device mask (P.rmask)
     DevSink *dsinks = nullptr;
     DevBox *dboxes = nullptr;

@@ -376,7 +376,7 @@ function rings_var_cfg!(system, d::DeviceSys)
                                                    mask, length(recs), recs))
 end
 
-"Append n uniform draws of system.rng to the device's spawn polarisation pool."
+"Append n uniform draws of system.rng to the device's spawn polarisation pool (the library draws from its own stream once the pool is empty)."
 function pol_draws!(system, d::DeviceSys, n)
     u = rand(system.rng, Float64, n)
     check(d.ctx, ccall((:mavi_rings_pol_draws, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Int64), d.sys, u, n))

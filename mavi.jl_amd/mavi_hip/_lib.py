@@ -153,9 +153,12 @@ SIGNATURES = {
     "mavi_rings_info": (C.c_int, [_P, _DP, _DP, _DP]),
     "mavi_rings_neighbors_cfg": (C.c_int, [_P, C.c_int, C.c_double, C.c_int]),
     "mavi_rings_neighbors_get": (C.c_int, [_P, _I64P, _I64P]),
+    "mavi_sys_noise_get": (C.c_int, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "mavi_sys_noise_set": (C.c_int, [_P, C.c_uint64, C.c_uint64]),
     "mavi_rings_var_cfg": (C.c_int, [_P, C.POINTER(C.c_int32), C.c_int32, C.POINTER(MaviRingSource)]),
     "mavi_rings_pol_draws": (C.c_int, [_P, C.POINTER(C.c_double), C.c_int64]),
-    "mavi_rings_var_get": (C.c_int, [_P, C.POINTER(C.c_int32), _I64P, _I64P, _I64P, _I64P]),
+    "mavi_rings_var_get": (C.c_int, [_P, C.POINTER(C.c_int32), _I64P, _I64P, _I64P, _I64P, _I64P]),
+    "mavi_rings_pol_stream_set": (C.c_int, [_P, C.c_int64]),
     "mavi_rings_invasions_cfg": (C.c_int, [_P, C.c_int, C.c_int64, C.c_int32, C.c_int32, C.c_int32]),
     "mavi_rings_invasions_check": (C.c_int, [_P]),
     "mavi_rings_invasions_get": (C.c_int, [_P, _I64P, _I64P, C.c_int64]),
@@ -336,6 +339,15 @@ class DeviceSystem:
         self._call("mavi_rings_neighbors_get", i64ptr(counts), i64ptr(lists))
         return counts, lists
 
+    def noise_state(self):
+        """(seed, step counter) of the device noise stream."""
+        seed, counter = C.c_uint64(), C.c_uint64()
+        self._call("mavi_sys_noise_get", C.byref(seed), C.byref(counter))
+        return int(seed.value), int(counter.value)
+
+    def set_noise_state(self, seed: int, counter: int):
+        self._call("mavi_sys_noise_set", C.c_uint64(int(seed)), C.c_uint64(int(counter)))
+
     def rings_var_cfg(self, mask, records):
         """records: list of MaviRingSource (their arrays kept alive by the caller)."""
         m = np.ascontiguousarray(mask, dtype=np.int32)
@@ -348,16 +360,21 @@ class DeviceSystem:
         self._call("mavi_rings_pol_draws", dptr(u), C.c_int64(len(u)))
 
     def rings_var_state(self):
-        """(mask, ids, uids, num_active, pol draws used) of a variable population."""
+        """(mask, ids, uids, num_active, pool draws used, built-in stream
+        position) of a variable population."""
         nr = self.nrings
         mask = np.zeros(nr, dtype=np.int32)
         ids = np.zeros(nr, dtype=np.int64)
         uids = np.zeros(nr, dtype=np.int64)
         na = np.zeros(1, dtype=np.int64)
         used = np.zeros(1, dtype=np.int64)
+        stream = np.zeros(1, dtype=np.int64)
         self._call("mavi_rings_var_get", mask.ctypes.data_as(C.POINTER(C.c_int32)), i64ptr(ids), i64ptr(uids),
-                   i64ptr(na), i64ptr(used))
-        return mask, ids, uids, int(na[0]), int(used[0])
+                   i64ptr(na), i64ptr(used), i64ptr(stream))
+        return mask, ids, uids, int(na[0]), int(used[0]), int(stream[0])
+
+    def rings_pol_stream_set(self, position: int):
+        self._call("mavi_rings_pol_stream_set", C.c_int64(int(position)))
 
     def rings_invasions_cfg(self, scheduled: bool, steps_to_update: int = 0, r_cols: int = 0, r_rows: int = 0,
                             r_nc: int = 0):

@@ -35,6 +35,8 @@ SIGNATURES = {
     "mavi_batch_prof_enable": (C.c_int, [_P, C.c_int]),
     "mavi_batch_prof_get": (C.c_int, [_P, C.c_int, _DP, _I64P]),
     "mavi_batch_autotune": (C.c_int, [_P, C.c_int32, _DP, C.POINTER(C.c_int32)]),
+    "mavi_batch_noise_get_counter": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    "mavi_batch_noise_set_counter": (C.c_int, [_P, C.c_uint64]),
 }
 
 MAVI_BOPT_WG_PLAIN = 0
@@ -127,6 +129,14 @@ class Batch:
 
     def run(self, nsteps: int):
         self._call("mavi_batch_run", C.c_int64(int(nsteps)))
+
+    def noise_counter(self) -> int:
+        c = C.c_uint64()
+        self._call("mavi_batch_noise_get_counter", C.byref(c))
+        return int(c.value)
+
+    def set_noise_counter(self, counter: int):
+        self._call("mavi_batch_noise_set_counter", C.c_uint64(int(counter)))
 
     def set_option(self, option: int, value: int):
         self._call("mavi_batch_set_option", C.c_int32(int(option)), C.c_int32(int(value)))

@@ -464,10 +464,14 @@ class RingsSystem:
 
     def __init__(self, *, state: RingsState, space_cfg: SpaceCfg, dynamic_cfg: RingsCfg, int_cfg,
                  time_info: TimeInfo | None = None, seed: int = 0, ctx: _lib.Context | None = None,
-                 p_neighbors_cfg: NeighborsCfg | None = None, source_cfg=None, rng=None):
+                 p_neighbors_cfg: NeighborsCfg | None = None, source_cfg=None, rng=None, pol_draws=None,
+                 pol_stream: int = 0):
         """source_cfg: SourceCfg / SinkCfg (or a list of them), which makes the
-        population variable like state.rings_ids (ActiveState); rng draws the
-        random spawn polarisations (numpy Generator, default seeded by seed)."""
+        population variable like state.rings_ids (ActiveState).  Random spawn
+        polarisations: rng (a numpy Generator) feeds the device's pool of
+        uniform draws, else the library's built-in stream keyed by seed;
+        pol_draws (pool draws handed out but not used yet) and pol_stream
+        (the built-in stream's position) resume a saved run."""
         d, self.chunks, self._tables = describe(state, space_cfg, dynamic_cfg, int_cfg, seed)
         self.state, self.space_cfg, self.dynamic_cfg, self.int_cfg = state, space_cfg, dynamic_cfg, int_cfg
         self.time_info = time_info or TimeInfo()
@@ -486,10 +490,16 @@ class RingsSystem:
             self.r_chunks_grid = ring_chunks_grid(state, space_cfg, dynamic_cfg, extra.r_chunks_cfg)
         self.upload()
         self.sources = None
-        self._rng = rng if rng is not None else np.random.default_rng(seed)
-        self._pol_supplied = 0
+        self._rng = rng
+        self._pool = np.zeros(0)  # draws handed to the device and not known to be used
+        self._pool_base = 0       # pool index of _pool[0]
         if state.rings_ids is not None or source_cfg is not None:
             self._var_cfg(source_cfg)
+            if pol_draws is not None and len(pol_draws):
+                self._pool = np.asarray(pol_draws, dtype=np.float64)
+                self.dev.rings_pol_draws(self._pool)
+            if pol_stream:
+                self.dev.rings_pol_stream_set(pol_stream)
         # the constructor's init sequence (src/rings/rings.jl:270-288): the
         # invasion schedule starts at this step count, then forces!
         self._invasions_cfg_call()
@@ -518,21 +528,34 @@ class RingsSystem:
                                  if isinstance(s, Source) and isinstance(s.cfg.spawn_pol, str))
         self.dev.rings_var_cfg(st.rings_ids.mask.astype(np.int32), recs)
 
-    def _supply_pol_draws(self, nsteps):
-        """Keep the device's pool of uniform draws ahead of the next nsteps
-        (at most one draw per random spawn box and step)."""
-        if not self._random_boxes:
-            return
+    def _pool_used(self):
         used = self.dev.rings_var_state()[4]
-        need = used + nsteps * self._random_boxes - self._pol_supplied
+        self._pool = self._pool[used - self._pool_base:]
+        self._pool_base = used
+
+    def _supply_pol_draws(self, nsteps):
+        """Keep the device's pool of uniform draws (from rng) ahead of the
+        next nsteps: at most one draw per random spawn box and step."""
+        if self._rng is None or not self._random_boxes:
+            return
+        self._pool_used()
+        need = nsteps * self._random_boxes - len(self._pool)
         if need > 0:
-            self.dev.rings_pol_draws(self._rng.random(need))
-            self._pol_supplied += need
+            u = self._rng.random(need)
+            self.dev.rings_pol_draws(u)
+            self._pool = np.concatenate([self._pool, u])
+
+    def pending_pol_draws(self) -> np.ndarray:
+        """Pool draws handed to the device and not used yet."""
+        if self.sources is None:
+            return np.zeros(0)
+        self._pool_used()
+        return self._pool.copy()
 
     def _sync_var(self):
         if self.sources is None:
             return
-        mask, ids, uids, na, _ = self.dev.rings_var_state()
+        mask, ids, uids, na, _, _ = self.dev.rings_var_state()
         r = self.state.rings_ids
         r.mask[:] = mask != 0
         r.update()
@@ -572,9 +595,18 @@ class RingsSystem:
         self._sync_var()
 
     def advance(self, nsteps: int, noise=None):
-        if self.sources is not None:
-            self._supply_pol_draws(int(nsteps))
-        if noise is None:
+        if self.sources is not None and self._rng is not None and self._random_boxes:
+            # the pool is topped up in segments, so that it stays small
+            done = 0
+            while done < nsteps:
+                seg = min(256, int(nsteps) - done)
+                self._supply_pol_draws(seg)
+                if noise is None:
+                    self.dev.step(seg)
+                else:
+                    self.dev.rings_step(seg, np.asarray(noise)[done:done + seg])
+                done += seg
+        elif noise is None:
             self.dev.step(nsteps)
         else:
             self.dev.rings_step(nsteps, noise)

@@ -47,7 +47,7 @@ def test_population_follows_the_oracle(gpu_ctx, oracle_mod, use_chunks, pol):
         sys_.download()
         pos, pol_, *_ = o.rings_steps(pos, pol_, 100)
         mask, ids, uids, na, cms, used = o.var_state()
-        dmask, dids, duids, dna, dused = sys_.dev.rings_var_state()
+        dmask, dids, duids, dna, dused, _ = sys_.dev.rings_var_state()
         assert np.array_equal(dmask, mask), f"block {block}"
         assert np.array_equal(dids, ids) and np.array_equal(duids, uids)
         assert dna == na and dused == used
@@ -92,9 +92,9 @@ def test_errors(gpu_ctx, oracle_mod):
     state, space, cfg, ic, sources, ring_d = source_case(oracle_mod, pol="random")
     src, sink = sources
     sys_ = rings.RingsSystem(state=state, space_cfg=space, dynamic_cfg=cfg, int_cfg=ic, source_cfg=[src.cfg, sink])
-    # the pool of spawn polarisation draws is empty: the first spawn fails
-    with pytest.raises(mv.MaviError):
-        sys_.dev.step(1)
+    # without a pool the spawns draw from the built-in stream
+    sys_.advance(1)
+    assert sys_.dev.rings_var_state()[5] == 4 and sys_.dev.rings_var_state()[3] == 4
     big = rings.SourceCfg(bottom_left=src.cfg.bottom_left, spawn_pos=src.cfg.spawn_pos, size=(9, 8), spawn_pol=0.0)
     state2, space2, cfg2, ic2, _, _ = source_case(oracle_mod)
     with pytest.raises(mv.UnsupportedOnGPU):
