@@ -180,6 +180,62 @@ def bench_rings_sources(args, fixed_ms):
     return out
 
 
+def bench_experiments(args):
+    """run_experiment_batch over LJ gases (nside^2 particles each, velocities
+    scaled by the value) with a DelayedCol collector and A/B checkpoints,
+    routed to one device batch; compared with the bare batch stepping the
+    same replicas the same number of steps without the runner's files."""
+    import shutil
+    import tempfile
+
+    import mavi_hip as mv
+    from mavi_hip import batch as mb
+    from mavi_hip import experiments as ex
+    from mavi_hip import workloads as wl
+
+    nside, nrep = args.exp_nside, args.exp_replicas
+    pos, vel, space, dyn, ic = wl.lj_periodic(nside=nside)
+    ic = mv.IntCfg(dt=ic.dt, chunks_cfg=ic.chunks_cfg, device=mv.GPU())
+    init = mv.System(state=mv.SecondLawState(pos=pos, vel=vel), space_cfg=space, dynamic_cfg=dyn, int_cfg=ic)
+
+    def get_system(init_system, value, idx):
+        return mv.System(state=mv.SecondLawState(pos=pos, vel=vel * value), space_cfg=space, dynamic_cfg=dyn,
+                         int_cfg=ic)
+
+    steps = args.exp_steps
+    tf = steps * ic.dt * (1 - 1e-9)
+    root = Path(tempfile.mkdtemp(prefix="mavi_exp_"))
+    try:
+        batch = ex.ExperimentBatch(exp_cfg=ex.ExperimentCfg(tf=tf, root=str(root),
+                                                            checkpoint_cfg=ex.CheckpointCfg(delta_time=tf / 4)),
+                                   col_cfg=ex.DelayedCfg(delay_time=tf / 8), init_system=init,
+                                   values=list(np.linspace(0.5, 1.5, nrep)))
+        log(f"[bench] experiments: {nrep} x {nside * nside} particles, {steps} steps")
+        t0 = time.perf_counter()
+        res = ex.run_experiment_batch(batch, get_system, verbose=False)
+        wall = time.perf_counter() - t0
+        done = sum((root / "data" / str(i + 1) / ".done").is_file() for i in range(nrep))
+    finally:
+        shutil.rmtree(root, ignore_errors=True)
+    b = mb.Batch()
+    for i in range(nrep):
+        b.add(mv.SecondLawState(pos=pos, vel=vel), space, dyn, ic)
+    b.finalize()
+    for i in range(nrep):
+        b.upload(i, pos, vel * (0.5 + i / max(nrep - 1, 1)))
+    b.run(steps)
+    t0 = time.perf_counter()
+    b.run(steps)
+    bare = time.perf_counter() - t0
+    n = nrep * nside * nside
+    return {"workload": f"run_experiment_batch: {nrep} LJ gases of {nside * nside} particles (lj_periodic), "
+                        f"{steps} steps each, DelayedCol every {steps // 8} steps, checkpoints every {steps // 4}, "
+                        "final states saved",
+            "value": n * steps / wall, "unit": "particle-steps/s (files included)", "wall_s": wall,
+            "experiments_done": done, "errors": sum(r is not None for r in res),
+            "bare_batch_value": n * steps / bare, "runner_overhead": wall / bare}
+
+
 def bench_walls(args, plain_value):
     """The plain 1M LJ workload inside a walls list (ManyWalls: periodic box,
     16 slippery circles, 8 slippery lines, one potential circle): the cost
@@ -465,6 +521,9 @@ def main():
     ap.add_argument("--inv-checks", type=int, default=20, help="timed invasion checks in the rings form")
     ap.add_argument("--sources-steps", type=int, default=500, help="steps of the rings-with-sources form (0 = skip)")
     ap.add_argument("--sources-cpu-steps", type=int, default=10)
+    ap.add_argument("--exp-replicas", type=int, default=32, help="experiments in the runner form (0 = skip)")
+    ap.add_argument("--exp-nside", type=int, default=100)
+    ap.add_argument("--exp-steps", type=int, default=400)
     ap.add_argument("--walls-steps", type=int, default=500, help="timed steps of the walls form (0 = skip)")
     ap.add_argument("--active-nside", type=int, default=1024,
                     help="side of the Szabo and run-and-tumble lattices (0 = skip those forms)")
@@ -583,6 +642,8 @@ def main():
         forms["rings"] = bench_rings(args)
         if args.sources_steps > 0:
             forms["rings_sources"] = bench_rings_sources(args, forms["rings"]["ms_per_step"])
+    if args.exp_replicas > 0:
+        forms["experiments"] = bench_experiments(args)
     if args.walls_steps > 0:
         forms["walls"] = bench_walls(args, value)
     if args.active_nside > 0:

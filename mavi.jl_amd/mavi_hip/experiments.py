@@ -238,8 +238,7 @@ def check_checkpoint(experiment):
     _write_cp_info(cp_dir, cp)
     save_state(experiment.system, path / "sys_state")
     save_component_json(experiment.system.time_info, path / "time_info", "time_info")
-    with open(path / "col_state.json", "w") as f:
-        json.dump(_col_state_json(experiment.col), f)
+    _save_col_state(experiment.col, path / "col_state")
     if cfg.save_data:
         experiment.col.save_data(Path(experiment.cfg.root) / COL_DIRNAME)
     cp.is_valid[name] = True
@@ -253,27 +252,51 @@ def _write_cp_info(cp_dir, cp):
         json.dump({"last_used": cp.last_used, "last_time": cp.last_time, "is_valid": cp.is_valid}, f)
 
 
-def _col_state_json(col):
-    """Collector state: snapshots as arrays in lists (JSON), per collector."""
-    if isinstance(col, ManyCols):
-        return {"type": "ManyCols", "data": {k: _col_state_json(c) for k, c in col.cols.items()}}
-    if isinstance(col, DelayedCol):
-        snaps = [{"time_info": get_obj_save_data_json(s[0]),
-                  "arrays": {k: get_obj_save_data_json(v) for k, v in s[1].items()},
-                  "noise": list(s[2]), "kind": s[3]} for s in col.state.sys_states]
-        return {"type": "DelayedCol", "data": {"last_save_time": col.state.last_save_time, "snaps": snaps}}
-    state = getattr(col, "state", None)
-    return {"type": "custom", "data": get_obj_save_data_json(state)}
+def _save_col_state(col, path):
+    """Collector state: JSON for the small fields, npz for snapshot arrays."""
+    path = Path(path)
+    path.mkdir(parents=True, exist_ok=True)
+    arrays = {}
+
+    def enc(c, prefix):
+        if isinstance(c, ManyCols):
+            return {"type": "ManyCols", "data": {k: enc(v, f"{prefix}{k}.") for k, v in c.cols.items()}}
+        if isinstance(c, DelayedCol):
+            snaps = []
+            for j, sn in enumerate(c.state.sys_states):
+                names = {}
+                for k, v in sn[1].items():
+                    key = f"{prefix}{j}.{k}"
+                    arrays[key] = v
+                    names[k] = key
+                snaps.append({"time_info": get_obj_save_data_json(sn[0]), "arrays": names, "noise": list(sn[2]),
+                              "kind": sn[3]})
+            return {"type": "DelayedCol", "data": {"last_save_time": c.state.last_save_time, "snaps": snaps}}
+        return {"type": "custom", "data": get_obj_save_data_json(getattr(c, "state", None))}
+
+    meta = enc(col, "")
+    np.savez(path / "arrays.npz", **arrays)
+    with open(path / "col_state.json", "w") as f:
+        json.dump(meta, f)
 
 
-def _col_state_load(d):
-    if d["type"] == "ManyCols":
-        return {k: _col_state_load(v) for k, v in d["data"].items()}
-    if d["type"] == "DelayedCol":
-        snaps = [(load_component(s["time_info"]), {k: load_component(v) for k, v in s["arrays"].items()},
-                  tuple(s["noise"]), s["kind"]) for s in d["data"]["snaps"]]
-        return DelayedState(d["data"]["last_save_time"], snaps)
-    return load_component(d["data"])
+def _load_col_state(path):
+    path = Path(path)
+    with open(path / "col_state.json") as f:
+        meta = json.load(f)
+    with np.load(path / "arrays.npz", allow_pickle=False) as z:
+        arrays = {k: z[k] for k in z.files}
+
+    def dec(d):
+        if d["type"] == "ManyCols":
+            return {k: dec(v) for k, v in d["data"].items()}
+        if d["type"] == "DelayedCol":
+            snaps = [(load_component(sn["time_info"]), {k: arrays[v] for k, v in sn["arrays"].items()},
+                      tuple(sn["noise"]), sn["kind"]) for sn in d["data"]["snaps"]]
+            return DelayedState(d["data"]["last_save_time"], snaps)
+        return load_component(d["data"])
+
+    return dec(meta)
 
 
 def _steps_to_event(system, experiment, stop_func):
@@ -337,7 +360,7 @@ def load_experiment(root, custom_step=None) -> Experiment:
     exp_cfg, col_cfg = configs["experiment"], configs["collector"]
     exp_cfg.root = str(root)
     system = load_system(root / "configs.json", path / "sys_state", path / "time_info")
-    col_state = _col_state_load(json.load(open(path / "col_state.json")))
+    col_state = _load_col_state(path / "col_state")
     col = get_collector(col_cfg, exp_cfg, system, col_state)
     return Experiment(exp_cfg, col_cfg, system, custom_step, col=col, checkpoint=cp)
 
