@@ -22,14 +22,13 @@ __device__ __forceinline__ int xcd_block() {
 }
 
 __device__ __forceinline__ double rcp_nr(double x) {
-    // v_rcp_f64 followed by two Newton steps: ~1 ulp, far inside the 1e-12
-    // force tolerance of the LJ path.
-    double y = __builtin_amdgcn_rcp(x);
-    double e = __builtin_fma(-x, y, 1.0);
-    y = __builtin_fma(y, e, y);
-    e = __builtin_fma(-x, y, 1.0);
-    y = __builtin_fma(y, e, y);
-    return y;
+    // v_rcp_f64 (relative error <= 4.6e-8, measured on gfx950 over 4M values
+    // of r^2 in [0.3, 6.3]) and one Newton step: <= 2.3e-15 relative, so
+    // the LJ force 4 eps (12 s^12 inv^7 - 6 s^6 inv^4) stays within ~2e-14,
+    // far inside the path's 1e-12 force tolerance.
+    const double y = __builtin_amdgcn_rcp(x);
+    const double e = __builtin_fma(-x, y, 1.0);
+    return __builtin_fma(y, e, y);
 }
 
 __device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
@@ -254,10 +253,17 @@ __device__ __forceinline__ void scan3(const SysParams &P, const RingCtx &R, Pair
 
 constexpr int kLjUnroll = 2;  // candidates per iteration of the LJ gather loop
 
+// Position of sorted slot j through a 32-bit byte offset from the array
+// base: the load keeps the base in scalar registers and costs one shift per
+// candidate (slots < 2^28).
+__device__ __forceinline__ double2 load_slot(const double2 *__restrict__ pin, int j) {
+    return *(const double2 *)((const char *)pin + ((uint32_t)j << 4));
+}
+
 // LJ form of scan3 with U candidates per iteration: their loads are issued
 // together and their force chains are independent, so one wave keeps U
-// chains in flight; the sums still run in candidate order.  A lane past its
-// last candidate repeats a valid one with a zero factor.
+// chains in flight; the sums still run in candidate order.  Full groups of
+// U run without a tail test; the last tot % U candidates run one by one.
 template <int U, bool WRAP>
 __device__ __forceinline__ void scan3_lj(const SysParams &P, PairAcc<POT_LJ> &acc, const double2 *__restrict__ pin,
                                          int s, double xi, double yi, int a0, int l0, int a1, int l1, int a2,
@@ -265,40 +271,49 @@ __device__ __forceinline__ void scan3_lj(const SysParams &P, PairAcc<POT_LJ> &ac
     const int tot = l0 + l1 + l2;
     const int b1 = l0, b2 = l0 + l1;
     const int o0 = a0, o1 = a1 - l0, o2 = a2 - l0 - l1;
-    for (int k = 0; k < tot; k += U) {
+    // Slot of candidate k as offsets added under masks (a select chain over
+    // o0, o1, o2 is turned into a table in scratch memory by the compiler).
+    const int d1 = o1 - o0, d2 = o2 - o1;
+    auto cand = [=](int k) { return k + o0 + ((k >= b1) ? d1 : 0) + ((k >= b2) ? d2 : 0); };
+    // the lane's own slot is candidate ks of its own row's range (the middle
+    // one); comparing the wave-uniform k with it costs one compare
+    const int ks = b1 + (s - a1);
+    auto diff = [&](const double2 p, double &dx, double &dy) {
+        dx = xi - p.x;
+        dy = yi - p.y;
+        if constexpr (WRAP) {
+            dx = min_image(dx, P.L, P.halfL);
+            dy = min_image(dy, P.H, P.halfH);
+        }
+    };
+    int k = 0;
+    for (; k + U <= tot; k += U) {
         int j[U];
-        bool in[U];
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int kk = k + u;
-            in[u] = kk < tot;
-            const int kc = in[u] ? kk : k;
-            j[u] = kc + ((kc < b1) ? o0 : ((kc < b2) ? o1 : o2));
-        }
-        double px[U], py[U];
+        for (int u = 0; u < U; u++) j[u] = cand(k + u);
+        double2 p[U];
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            const double2 p = pin[j[u]];
-            px[u] = p.x;
-            py[u] = p.y;
-        }
+        for (int u = 0; u < U; u++) p[u] = load_slot(pin, j[u]);
         double dx[U], dy[U], fr[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            dx[u] = xi - px[u];
-            dy[u] = yi - py[u];
-            if constexpr (WRAP) {
-                dx[u] = min_image(dx[u], P.L, P.halfL);
-                dy[u] = min_image(dy[u], P.H, P.halfH);
-            }
+            diff(p[u], dx[u], dy[u]);
             const double r2 = dx[u] * dx[u] + dy[u] * dy[u];
-            fr[u] = lj_scale(P, r2, in[u] && j[u] != s);
+            fr[u] = lj_scale(P, r2, k + u != ks);
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
             acc.fx = __builtin_fma(fr[u], dx[u], acc.fx);
             acc.fy = __builtin_fma(fr[u], dy[u], acc.fy);
         }
+    }
+    for (; k < tot; k++) {
+        const int j = cand(k);
+        double dx, dy;
+        diff(load_slot(pin, j), dx, dy);
+        const double fr = lj_scale(P, dx * dx + dy * dy, k != ks);
+        acc.fx = __builtin_fma(fr, dx, acc.fx);
+        acc.fy = __builtin_fma(fr, dy, acc.fy);
     }
 }
 

@@ -119,7 +119,7 @@ __global__ void k_export_chunks(SysParams P, const int32_t *__restrict__ start, 
 // recomputation.  (A compacted worklist of the particles to recompute was
 // slower: it loses the sorted order's neighbour locality.)
 template <int MODE, int POT, bool ALLPAIRS, bool WALLS = false>
-__global__ void k_pair(SysParams P, RingCtx R, const double2 *__restrict__ pin, const int32_t *__restrict__ scell,
+__global__ __attribute__((amdgpu_waves_per_eu(WALLS ? 1 : 8))) void k_pair(SysParams P, RingCtx R, const double2 *__restrict__ pin, const int32_t *__restrict__ scell,
                        const int32_t *__restrict__ start, double2 *__restrict__ Pst, double2 *__restrict__ Vst,
                        double2 *__restrict__ Fout, double2 *__restrict__ P2, double2 *__restrict__ F1,
                        double list2, const int32_t *__restrict__ needc, const uint32_t *__restrict__ epoch,
