@@ -18,6 +18,14 @@ bool is_active(const mavi_sys *s) {
     return s->P.potential == MAVI_POT_SZABO || s->P.potential == MAVI_POT_RTP;
 }
 
+// A state replaced on the device: the chunk keys and counts the last second
+// evaluation left for the next binning describe the old state.
+hipError_t drop_keys(mavi_sys *s, hipStream_t st) {
+    if (!s->keys_ready) return hipSuccess;
+    s->keys_ready = false;
+    return mavi::launch_fill_i32(s->cnt, 0, (int64_t)s->P.ncell + 1, st);
+}
+
 // Turn device flags raised since the last check into a status.
 int check_flags(mavi_sys *s) {
     mavi_ctx *ctx = s->ctx;
@@ -451,6 +459,7 @@ int mavi_sys_upload(mavi_sys *s, const double *pos, const double *vel, const dou
     if (pol && s->P.potential == MAVI_POT_RINGS)
         HIPCHK(ctx, hipMemcpyAsync(s->pol, pol, sizeof(double) * s->P.nrings, hipMemcpyHostToDevice, ctx->stream));
     s->cur = 0;
+    HIPCHK(ctx, drop_keys(s, ctx->stream));
     HIPCHK(ctx, mavi::launch_iota_state(s, ctx->stream));
     if (s->P.potential == MAVI_POT_RINGS) HIPCHK(ctx, mavi::launch_rings_init(s, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -1337,6 +1346,7 @@ int mavi_autotune(mavi_sys *s, int32_t trial_steps, double *best_us, int32_t *sh
             r = hipMemcpyAsync(s->pol, pol0, sizeof(double) * s->P.nrings, hipMemcpyDeviceToDevice, st);
         s->cur = 0;
         s->f_valid = false;
+        if (r == hipSuccess) r = drop_keys(s, st);
         if (r == hipSuccess) r = mavi::launch_iota_state(s, st);
         if (r == hipSuccess && s->P.potential == MAVI_POT_RINGS) r = mavi::launch_rings_init(s, st);
         if (r == hipSuccess) r = hipStreamSynchronize(st);

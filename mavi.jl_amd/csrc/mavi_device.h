@@ -619,7 +619,7 @@ template <int MODE, bool WALLS = false>
 __device__ __forceinline__ bool pair_epilogue(const SysParams &P, const int s, const double2 pi, const double fx_,
                                               const double fy_, double2 *__restrict__ Pst, double2 *__restrict__ Vst,
                                               double2 *__restrict__ Fout, double2 *__restrict__ P2,
-                                              double2 *__restrict__ F1) {
+                                              double2 *__restrict__ F1, double2 *xout = nullptr) {
     bool moved = false;
     if constexpr (MODE == MODE_FORCES) {
         Fout[s] = make_double2(fx_, fy_);
@@ -655,6 +655,7 @@ __device__ __forceinline__ bool pair_epilogue(const SysParams &P, const int s, c
         Pst[s] = make_double2(x, y);
         Vst[s] = make_double2(vx, vy);
         Fout[s] = make_double2(fx, fy);
+        if (xout) *xout = make_double2(x, y);
     }
     return moved;
 }

@@ -234,6 +234,7 @@ struct mavi_sys {
     uint32_t *epoch = nullptr;  // binnings so far (device)
     bool reuse = true;          // option
     bool f_valid = false;       // F holds the last step's second evaluation
+    bool keys_ready = false;    // the last second evaluation left key/rank/cnt for the next binning
     // graphs of two steps (state starting in copy 0)
     hipGraphExec_t graph2 = nullptr;
     bool use_graphs = true;
@@ -248,16 +249,20 @@ struct mavi_sys {
 // --------------------------------------------------------------------------
 
 // Exact truncated quotient of a/b, equal to Julia's Base div(a, b) for
-// Float64 (src/chunks.jl:129-130): q = trunc(a/b) can only be off by one when
-// the rounded quotient lands on an integer; the FMA residual a - q*b is exact
-// in sign and detects that case.
+// Float64 (src/chunks.jl:129-130).  q0 = trunc(a * rcp(b)) is within one of
+// trunc(a/b) while |a/b| < 2^20 (v_rcp_f64 is good to 5e-8); the FMA
+// residual r = a - q0 b is exact in sign and in its comparison with +-b, and
+// moves q0 to the quotient whose residual lies in [0, b) (a >= 0) or (-b, 0]
+// (a < 0).  Farther quotients (positions far off the grid) stay far off it.
 __device__ __forceinline__ double exact_trunc_div(double a, double b) {
-    double q = trunc(a / b);
-    double r = __builtin_fma(-q, b, a);
+    double q = trunc(a * __builtin_amdgcn_rcp(b));
+    const double r = __builtin_fma(-q, b, a);
     if (a >= 0.0) {
         if (r < 0.0) q -= 1.0;
+        else if (r >= b) q += 1.0;
     } else {
         if (r > 0.0) q += 1.0;
+        else if (r <= -b) q -= 1.0;
     }
     return q;
 }

@@ -118,13 +118,22 @@ __global__ void k_export_chunks(SysParams P, const int32_t *__restrict__ start, 
 // same candidates in the same order, so they are bit-identical to a
 // recomputation.  (A compacted worklist of the particles to recompute was
 // slower: it loses the sorted order's neighbour locality.)
+// The next binning's first pass (k_bin_count) done by the second
+// evaluation on the final positions it writes: chunk, arrival rank, flags,
+// and the force-reuse stamps of the particles that change chunk.
+struct NextBin {
+    int32_t *key, *rank, *cnt;
+    uint32_t *flags;
+    int32_t *stamp;
+};
+
 template <int MODE, int POT, bool ALLPAIRS, bool WALLS = false>
 __global__ __attribute__((amdgpu_waves_per_eu(WALLS ? 1 : 8))) void k_pair(SysParams P, RingCtx R, const double2 *__restrict__ pin, const int32_t *__restrict__ scell,
                        const int32_t *__restrict__ start, double2 *__restrict__ Pst, double2 *__restrict__ Vst,
                        double2 *__restrict__ Fout, double2 *__restrict__ P2, double2 *__restrict__ F1,
                        double list2, const int32_t *__restrict__ needc, const uint32_t *__restrict__ epoch,
                        const int32_t *__restrict__ perm, const double2 *__restrict__ Vold,
-                       const double2 *__restrict__ Fold, int32_t *__restrict__ wstamp) {
+                       const double2 *__restrict__ Fold, int32_t *__restrict__ wstamp, NextBin nb) {
     extern __shared__ uint16_t s_list[];
     int s = xcd_block() * blockDim.x + threadIdx.x;
     bool valid = s < P.na;
@@ -175,13 +184,37 @@ __global__ __attribute__((amdgpu_waves_per_eu(WALLS ? 1 : 8))) void k_pair(SysPa
     } else {
         pair_gather_chunks<POT>(P, R, acc, pin, scell[sv], start, s, valid, pi, list2, s_list);
     }
-    if (!valid) return;
-    const bool moved = pair_epilogue<MODE, WALLS>(P, s, pi, acc.fx, acc.fy, Pst, Vst, Fout, P2, F1);
-    if constexpr (MODE == MODE_VERLET2 && WALLS) {
-        // forces of a particle walls! moved cannot be reused: mark its chunk
-        // for the next binning (epoch + 1), as a chunk change would
-        if (moved && wstamp) wstamp[scell[s]] = (int32_t)(epoch[0] + 1u);
+    if constexpr (MODE == MODE_VERLET2 && !ALLPAIRS) {
+        double2 xf = make_double2(0.0, 0.0);
+        const bool moved = valid && pair_epilogue<MODE, WALLS>(P, s, pi, acc.fx, acc.fy, Pst, Vst, Fout, P2, F1, &xf);
+        if constexpr (WALLS) {
+            // forces of a particle walls! moved cannot be reused: mark its
+            // chunk for the next binning (epoch + 1), as a chunk change would
+            if (moved && wstamp) wstamp[scell[s]] = (int32_t)(epoch[0] + 1u);
+        }
+        if (nb.key) {  // every lane of the wave takes part in the arrival ranks
+            uint32_t flag = 0;
+            int32_t k = -1;
+            if (valid) k = cell_of(P, xf.x, xf.y, &flag);
+            if (flag) atomicOr(nb.flags, flag);
+            if (nb.stamp && valid) {
+                const int32_t pc = scell[s];
+                if (pc != k) {
+                    const int32_t e = (int32_t)(epoch[0] + 1u);
+                    nb.stamp[k] = e;
+                    nb.stamp[pc] = e;
+                }
+            }
+            const int32_t rk = wave_arrival_rank(valid, k, nb.cnt);
+            if (valid) {
+                nb.key[s] = k;
+                nb.rank[s] = rk;
+            }
+        }
+        return;
     }
+    if (!valid) return;
+    pair_epilogue<MODE, WALLS>(P, s, pi, acc.fx, acc.fy, Pst, Vst, Fout, P2, F1);
 }
 
 // ---------------------------------------------------------------------------
@@ -948,18 +981,26 @@ hipError_t launch_pair_t(mavi_sys *s, const double2 *pin, hipStream_t st, bool r
     const size_t lds = (l2 < INFINITY) ? sizeof(uint16_t) * kListCap * (size_t)wgk : 0;
     const int32_t *needc = reuse ? s->needc : nullptr;
     const int32_t *perm = defer ? s->perm : nullptr;
+    // a second evaluation on chunks does the next binning's first pass (with
+    // the stamps force reuse reads, as k_bin_count would make them)
+    NextBin nb{};
+    const bool fuse_stamp = s->reuse && (!s->lds_tiles || s->P.nwalls > 0);
+    if (MODE == MODE_VERLET2 && !ALL && s->P.cols > 0) {
+        nb = NextBin{s->key, s->rank, s->cnt, s->flags, fuse_stamp ? s->stamp : nullptr};
+        s->keys_ready = true;
+    }
     if constexpr (MODE == MODE_VERLET1 || MODE == MODE_VERLET2) {
         if (walls) {  // the walls list's forces and walls! in the epilogue
             hipLaunchKernelGGL((k_pair<MODE, POT, ALL, true>), dim3(nblk(s->P.na, wgk)), dim3(wgk), lds, st, s->P,
                                ring_ctx(s), pin, s->scell, s->start, s->P_[c], s->V_[c], s->F, s->P2, s->F1, l2,
                                needc, s->epoch, perm, s->V_[1 - c], s->F,
-                               (MODE == MODE_VERLET2 && !ALL) ? s->stamp : nullptr);
+                               (MODE == MODE_VERLET2 && !ALL) ? s->stamp : nullptr, nb);
             return hipGetLastError();
         }
     }
     hipLaunchKernelGGL((k_pair<MODE, POT, ALL>), dim3(nblk(s->P.na, wgk)), dim3(wgk), lds, st, s->P, ring_ctx(s), pin,
                        s->scell, s->start, s->P_[c], s->V_[c], s->F, s->P2, s->F1, l2, needc, s->epoch, perm,
-                       s->V_[1 - c], s->F, nullptr);
+                       s->V_[1 - c], s->F, nullptr, nb);
     return hipGetLastError();
 }
 
@@ -1071,9 +1112,12 @@ hipError_t launch_bin(mavi_sys *s, hipStream_t st, bool count_step, bool stamp, 
     const int c = s->cur, nc = 1 - c;
     // stamps need the previous binning's chunk of every slot (scell)
     const bool stamping = stamp && s->sorted;
-    if (P.na > 0)  // (a variable ring population can be empty)
+    // (a variable ring population can be empty; a second evaluation may have
+    // done this pass already)
+    if (P.na > 0 && !s->keys_ready)
         hipLaunchKernelGGL(k_bin_count, dim3(nblk(P.na, 256)), dim3(256), 0, st, P, s->P_[c], s->key, s->rank,
                            s->cnt, s->flags, s->scell, stamping ? s->stamp : nullptr, s->epoch);
+    s->keys_ready = false;
     const int n = P.ncell + 1;
     const int ntiles = nblk(n, kScanTile2);
     hipLaunchKernelGGL(k_scan_lookback, dim3(ntiles), dim3(kScanThreads), 0, st, s->cnt, n, s->start,
