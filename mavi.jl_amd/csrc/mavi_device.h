@@ -251,8 +251,6 @@ __device__ __forceinline__ void scan3(const SysParams &P, const RingCtx &R, Pair
     }
 }
 
-constexpr int kLjUnroll = 2;  // candidates per iteration of the LJ gather loop
-
 // Position of sorted slot j through a 32-bit byte offset from the array
 // base: the load keeps the base in scalar registers and costs one shift per
 // candidate (slots < 2^28).
@@ -260,61 +258,67 @@ __device__ __forceinline__ double2 load_slot(const double2 *__restrict__ pin, in
     return *(const double2 *)((const char *)pin + ((uint32_t)j << 4));
 }
 
-// LJ form of scan3 with U candidates per iteration: their loads are issued
-// together and their force chains are independent, so one wave keeps U
-// chains in flight; the sums still run in candidate order.  Full groups of
-// U run without a tail test; the last tot % U candidates run one by one.
-template <int U, bool WRAP>
-__device__ __forceinline__ void scan3_lj(const SysParams &P, PairAcc<POT_LJ> &acc, const double2 *__restrict__ pin,
-                                         int s, double xi, double yi, int a0, int l0, int a1, int l1, int a2,
-                                         int l2) {
-    const int tot = l0 + l1 + l2;
-    const int b1 = l0, b2 = l0 + l1;
-    const int o0 = a0, o1 = a1 - l0, o2 = a2 - l0 - l1;
-    // Slot of candidate k as offsets added under masks (a select chain over
-    // o0, o1, o2 is turned into a table in scratch memory by the compiler).
-    const int d1 = o1 - o0, d2 = o2 - o1;
-    auto cand = [=](int k) { return k + o0 + ((k >= b1) ? d1 : 0) + ((k >= b2) ? d2 : 0); };
-    // the lane's own slot is candidate ks of its own row's range (the middle
-    // one); comparing the wave-uniform k with it costs one compare
-    const int ks = b1 + (s - a1);
-    auto diff = [&](const double2 p, double &dx, double &dy) {
-        dx = xi - p.x;
-        dy = yi - p.y;
+// LJ factor of one candidate with the range test folded into the operand:
+// a candidate that is beyond the cutoff (or the lane itself) gets the high
+// word of 2^1023 in place of that of r^2, so its reciprocal is ~2^-1023
+// and inv^3 underflows to 0: the factor is +-0 and adds nothing (one 32-bit
+// select instead of two on the result; in-range factors are unchanged).
+__device__ __forceinline__ double lj_scale_masked(const SysParams &P, double r2, bool take) {
+    const bool ok = take && (r2 <= P.cut2);
+    const double rs = __hiloint2double(ok ? __double2hiint(r2) : 0x7FE00000, __double2loint(r2));
+    const double inv = rcp_nr(rs);
+    const double inv3 = inv * inv * inv;
+    return inv * inv3 * __builtin_fma(P.lj_c12, inv3, -P.lj_c6);
+}
+
+// LJ sum over one slot range [a, a + l) of a lane (one chunk row of its 3x3
+// block), two candidates per iteration: their loads are issued together and
+// their force chains are independent; the sums run in slot order.  The
+// iteration count k is the same for every lane still in the loop, so the
+// loads take a scalar base (pin + k) plus the lane's fixed byte offset of a
+// and need no per-candidate address arithmetic.  SELF: the range holds the
+// lane's own slot, as candidate ks.
+template <bool WRAP, bool SELF>
+__device__ __forceinline__ void lj_range(const SysParams &P, PairAcc<POT_LJ> &acc, const double2 *__restrict__ pin,
+                                         double xi, double yi, int a, int l, int ks) {
+    const uint32_t aoff = (uint32_t)a << 4;
+    auto force = [&](const double2 p, bool take) {
+        double dx = xi - p.x, dy = yi - p.y;
         if constexpr (WRAP) {
             dx = min_image(dx, P.L, P.halfL);
             dy = min_image(dy, P.H, P.halfH);
         }
+        struct FDr { double f, dx, dy; };
+        return FDr{lj_scale_masked(P, dx * dx + dy * dy, take), dx, dy};
     };
-    int k = 0;
-    for (; k + U <= tot; k += U) {
-        int j[U];
-#pragma unroll
-        for (int u = 0; u < U; u++) j[u] = cand(k + u);
-        double2 p[U];
-#pragma unroll
-        for (int u = 0; u < U; u++) p[u] = load_slot(pin, j[u]);
-        double dx[U], dy[U], fr[U];
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            diff(p[u], dx[u], dy[u]);
-            const double r2 = dx[u] * dx[u] + dy[u] * dy[u];
-            fr[u] = lj_scale(P, r2, k + u != ks);
-        }
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            acc.fx = __builtin_fma(fr[u], dx[u], acc.fx);
-            acc.fy = __builtin_fma(fr[u], dy[u], acc.fy);
-        }
+    for (int k = 0; k + 2 <= l; k += 2) {
+        const char *base = (const char *)pin + ((size_t)(uint32_t)k << 4);
+        const double2 p0 = *(const double2 *)(base + aoff);
+        const double2 p1 = *(const double2 *)(base + aoff + 16);
+        const auto f0 = force(p0, !SELF || k != ks);
+        const auto f1 = force(p1, !SELF || k + 1 != ks);
+        acc.fx = __builtin_fma(f0.f, f0.dx, acc.fx);
+        acc.fy = __builtin_fma(f0.f, f0.dy, acc.fy);
+        acc.fx = __builtin_fma(f1.f, f1.dx, acc.fx);
+        acc.fy = __builtin_fma(f1.f, f1.dy, acc.fy);
     }
-    for (; k < tot; k++) {
-        const int j = cand(k);
-        double dx, dy;
-        diff(load_slot(pin, j), dx, dy);
-        const double fr = lj_scale(P, dx * dx + dy * dy, k != ks);
-        acc.fx = __builtin_fma(fr, dx, acc.fx);
-        acc.fy = __builtin_fma(fr, dy, acc.fy);
+    if (l & 1) {  // the last candidate, l - 1, of an odd range
+        const int k = l - 1;
+        const auto f0 = force(load_slot(pin, a + k), !SELF || k != ks);
+        acc.fx = __builtin_fma(f0.f, f0.dx, acc.fx);
+        acc.fy = __builtin_fma(f0.f, f0.dy, acc.fy);
     }
+}
+
+// LJ form of scan3: the three chunk rows of the block one after another
+// (candidate order as scan3), the middle one holding the lane's own slot s.
+template <bool WRAP>
+__device__ __forceinline__ void scan3_lj(const SysParams &P, PairAcc<POT_LJ> &acc, const double2 *__restrict__ pin,
+                                         int s, double xi, double yi, int a0, int l0, int a1, int l1, int a2,
+                                         int l2) {
+    lj_range<WRAP, false>(P, acc, pin, xi, yi, a0, l0, 0);
+    lj_range<WRAP, true>(P, acc, pin, xi, yi, a1, l1, s - a1);
+    lj_range<WRAP, false>(P, acc, pin, xi, yi, a2, l2, 0);
 }
 
 // Two-phase form for finite-range interactions: a first pass over the
@@ -423,8 +427,8 @@ __device__ __forceinline__ void pair_gather_chunks(const SysParams &P, const Rin
         }
         if (direct) {
             if constexpr (POT == POT_LJ) {
-                if (wrapw) scan3_lj<kLjUnroll, true>(P, acc, pin, s, pi.x, pi.y, a0, l0, a1, l1, a2, l2);
-                else scan3_lj<kLjUnroll, false>(P, acc, pin, s, pi.x, pi.y, a0, l0, a1, l1, a2, l2);
+                if (wrapw) scan3_lj<true>(P, acc, pin, s, pi.x, pi.y, a0, l0, a1, l1, a2, l2);
+                else scan3_lj<false>(P, acc, pin, s, pi.x, pi.y, a0, l0, a1, l1, a2, l2);
             } else {
                 if (wrapw) scan3<POT, true>(P, R, acc, pin, s, pi.x, pi.y, a0, l0, a1, l1, a2, l2);
                 else scan3<POT, false>(P, R, acc, pin, s, pi.x, pi.y, a0, l0, a1, l1, a2, l2);
