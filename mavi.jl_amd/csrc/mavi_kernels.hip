@@ -594,7 +594,20 @@ __device__ __forceinline__ bool seg_accumulate(const SysParams &P, const RingCtx
     const int tot = l0 + l1 + l2;
     const int b1 = l0, b2 = l0 + l1;
     const int o0 = a0, o1 = a1 - l0, o2 = a2 - l0 - l1;
-    if constexpr (!LISTED) {
+    if constexpr (POT == POT_LJ && !LISTED) {
+        // the three staged rows one after another, as the gather form (wrap
+        // is the same for the whole wave: it owns one segment)
+        if (wrap) {
+            lj_range<true, false>(P, acc, stg, xi, yi, a0, l0, 0);
+            lj_range<true, true>(P, acc, stg, xi, yi, a1, l1, me - a1);
+            lj_range<true, false>(P, acc, stg, xi, yi, a2, l2, 0);
+        } else {
+            lj_range<false, false>(P, acc, stg, xi, yi, a0, l0, 0);
+            lj_range<false, true>(P, acc, stg, xi, yi, a1, l1, me - a1);
+            lj_range<false, false>(P, acc, stg, xi, yi, a2, l2, 0);
+        }
+        return false;
+    } else if constexpr (!LISTED) {
         for (int k = 0; k < tot; k++) {
             const int j = k + ((k < b1) ? o0 : ((k < b2) ? o1 : o2));
             const double2 pj = stg[j];
