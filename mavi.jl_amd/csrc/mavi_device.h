@@ -764,30 +764,50 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_lookback(const int32_t *_
         __hip_atomic_store(&state[tile], scan_word(epoch, tile == 0 ? 2u : 1u, total), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     if (threadIdx.x < 64) {
+        // one wave looks back over windows of 256 tiles, 4 per lane (lane l
+        // holds tiles j - 4l - q, q = 0..3, nearest first): a tile far from
+        // the start needs a quarter of the polling rounds one tile per lane
+        // would, and each round waits on memory (2.1M chunks are 523 tiles)
         const int lane = threadIdx.x;
         int32_t excl = 0;
         int j = tile - 1;  // nearest tile not yet accounted for
         uint32_t polls = 0;
         bool stalled = false;
         while (j >= 0) {
-            const int t = j - lane;
-            uint32_t kind = 0;
-            int32_t val = 0;
-            if (t >= 0) {
-                const unsigned long long w =
-                    __hip_atomic_load(&state[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if ((uint32_t)(w >> 34) == epoch) {
-                    kind = (uint32_t)(w >> 32) & 3u;
-                    val = (int32_t)(uint32_t)w;
+            uint32_t kind[4];
+            int32_t val[4];
+            int first_pre = 4;  // nearest tile of this lane holding an inclusive prefix
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int t = j - 4 * lane - q;
+                kind[q] = 3u;  // before tile 0: contributes nothing
+                val[q] = 0;
+                if (t >= 0) {
+                    const unsigned long long w =
+                        __hip_atomic_load(&state[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    kind[q] = 0u;
+                    if ((uint32_t)(w >> 34) == epoch) {
+                        kind[q] = (uint32_t)(w >> 32) & 3u;
+                        val[q] = (int32_t)(uint32_t)w;
+                    }
+                }
+                if (kind[q] == 2u && first_pre == 4) first_pre = q;
+            }
+            // stop at the nearest inclusive prefix, and never step over a
+            // tile that has published nothing yet
+            const unsigned long long pre = __ballot(first_pre < 4);
+            const int stop = pre ? __ffsll(pre) - 1 : 64;
+            const int last_q = lane < stop ? 3 : (lane == stop ? first_pre : -1);
+            bool missing = false;
+            int32_t part = 0;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                if (q <= last_q) {
+                    missing = missing || kind[q] == 0u;
+                    part += val[q];
                 }
             }
-            // tiles in [j-63, j]: stop at the nearest inclusive prefix, and
-            // never step over a tile that has published nothing yet
-            const unsigned long long pre = __ballot(t >= 0 && kind == 2u);
-            const unsigned long long none = __ballot(t >= 0 && kind == 0u);
-            const int stop = pre ? __ffsll(pre) - 1 : 64;
-            const unsigned long long below = stop >= 63 ? ~0ull : ((2ull << stop) - 1ull);
-            if (none & below) {
+            if (__ballot(missing)) {
                 if (++polls >= kScanMaxPolls) {
                     stalled = true;
                     break;
@@ -795,12 +815,11 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_lookback(const int32_t *_
                 __builtin_amdgcn_s_sleep(1);
                 continue;
             }
-            int32_t part = (lane <= stop && t >= 0) ? val : 0;
 #pragma unroll
             for (int d = 32; d > 0; d >>= 1) part += __shfl_xor(part, d, 64);
             excl += part;
             if (pre) break;
-            j -= 64;
+            j -= 256;
         }
         if (lane == 0) {
             if (tile > 0)
@@ -812,10 +831,24 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_lookback(const int32_t *_
     }
     __syncthreads();
     int32_t off = s_prefix + local;
+    if (base + kScanItems2 <= n) {
+        int4 *q = reinterpret_cast<int4 *>(out + base);
 #pragma unroll
-    for (int k = 0; k < kScanItems2; k++) {
-        if (base + k < n) out[base + k] = off;
-        off += v[k];
+        for (int k = 0; k < kScanItems2 / 4; k++) {
+            int4 w;
+            w.x = off;
+            w.y = w.x + v[4 * k];
+            w.z = w.y + v[4 * k + 1];
+            w.w = w.z + v[4 * k + 2];
+            off = w.w + v[4 * k + 3];
+            q[k] = w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanItems2; k++) {
+            if (base + k < n) out[base + k] = off;
+            off += v[k];
+        }
     }
 }
 

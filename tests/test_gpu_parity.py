@@ -298,6 +298,43 @@ def test_full_size_force_sample_vs_oracle(gpu_ctx, oracle_mod):
     assert e <= FORCE_RTOL
 
 
+def test_large_grid_binning_and_steps(gpu_ctx, oracle_mod):
+    """A grid of 3000 x 750 = 2.25M chunks: the one-pass scan runs 550 tiles
+    of 4096 chunks, so its look-back crosses three windows of 256 tiles.
+    Binning bit-exact against the oracle, then forces and 10 steps."""
+    import mavi_hip as mv
+
+    say("[large grid] 2.25M chunks, 120k particles")
+    rng = np.random.default_rng(11)
+    cols, rows = 3000, 750
+    L, H = float(cols), float(rows)
+    n = 120_000
+    pos = np.stack([rng.uniform(0.0, L, n), rng.uniform(0.0, H, n)], axis=1)
+    vel = rng.normal(size=(n, 2)) * 0.3
+    dyn = mv.HarmTruncCfg(k_rep=10.0, k_atr=1.0, dist_eq=1.0, dist_max=1.0)
+    space = mv.SpaceCfg(wall_type=mv.PeriodicWalls(), geometry_cfg=mv.RectangleCfg(length=L, height=H))
+    ic = mv.IntCfg(dt=1e-3, chunks_cfg=mv.ChunksCfg(num_cols=cols, num_rows=rows), device=mv.GPU())
+    assert cols * rows > 512 * 4096
+    sys_ = make_plain(pos, vel, space, dyn, ic)
+    cp, counts = sys_.update_chunks()
+    o = oracle_mod.from_system(n, space, dyn, ic)
+    err, cp_ref, counts_ref = o.bin(pos)
+    assert err == 0 and np.array_equal(counts, counts_ref)
+    mask = np.arange(cp.shape[0])[:, None, None] < counts[None, :, :]
+    assert np.array_equal(np.where(mask, cp, 0), np.where(mask, cp_ref, 0))
+    del cp, cp_ref, mask
+    say("[large grid] forces")
+    f = sys_.calc_forces().copy()
+    f_ref, ab = o.forces(pos, with_absum=True)
+    assert force_rel_err(f, f_ref, ab) <= FORCE_RTOL
+    say("[large grid] 10 steps")
+    mv.run_system(sys_, num_steps=10)
+    p_ref, v_ref, _, _, _ = o.newton_steps(pos, vel, 10)
+    ep, ev = traj_err(sys_.state.pos, sys_.state.vel, p_ref, v_ref, L)
+    say(f"  pos err {ep:.3e}  vel err {ev:.3e}")
+    assert ep <= TRAJ_RTOL and ev <= TRAJ_RTOL
+
+
 SHAPES = [(256, 0, 0), (128, 0, 1), (256, 1, 0), (256, 1, 1), (64, 0, 0), (512, 0, 1)]
 
 
