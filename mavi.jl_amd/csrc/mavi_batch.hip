@@ -256,10 +256,8 @@ hipError_t batch_bin(mavi_batch *b, hipStream_t st) {
     hipLaunchKernelGGL(kb_bin_count, dim3(nblk(ns, 256)), dim3(256), 0, st, b->d_reps, b->wave_rep, ns, b->P_[c],
                        b->key, b->rank, b->cnt, b->flags);
     const int n = (int)b->ncells + 1;
-    const int ntiles = nblk(n, kScanTile2);
     // a stalled scan is reported through replica 0's flags
-    hipLaunchKernelGGL(k_scan_lookback, dim3(ntiles), dim3(kScanThreads), 0, st, b->cnt, n, b->start,
-                       b->scan_ticket, b->scan_state, ntiles, b->flags);
+    launch_scan(b->cnt, n, b->start, b->scan_ticket, b->scan_state, b->flags, st);
     hipLaunchKernelGGL(k_scatter, dim3(nblk(std::max(ns, n), 256)), dim3(256), 0, st, b->G, b->key, b->rank,
                        b->start, b->src, b->scell, b->ID_[c], b->sid, b->cnt, b->counter, nullptr, nullptr, nullptr);
     hipLaunchKernelGGL(kb_finalize, dim3(nblk(ns, 256)), dim3(256), 0, st, b->d_reps, b->wave_rep, ns, b->start,
@@ -528,7 +526,7 @@ int mavi_batch_finalize(mavi_batch *b) {
     ALLOC(b->slot_of, nids);
     ALLOC(b->cnt, ncells + 1);
     ALLOC(b->start, ncells + 1);
-    ALLOC(b->scan_state, nblk(ncells + 1, kScanTile2));
+    ALLOC(b->scan_state, nblk(ncells + 1, kScanTileMin));
     ALLOC(b->scan_ticket, 1);
     ALLOC(b->flags, nr);
     ALLOC(b->counter, 1);

@@ -385,6 +385,7 @@ int mavi_sys_create(mavi_ctx *ctx, const mavi_sys_desc *d, mavi_sys **out) {
     ALLOC(s->counter, 1);
     ALLOC(s->cnt, (int64_t)P.ncell + 1);
     ALLOC(s->start, (int64_t)P.ncell + 1);
+    // one state word per scan tile (tiles hold at least 4096 counts)
     s->nscan_blocks = (P.ncell + 1 + 4095) / 4096;
     ALLOC(s->scan_state, s->nscan_blocks);
     ALLOC(s->scan_ticket, 1);

@@ -712,18 +712,22 @@ __global__ void k_unpermute(int32_t na, const int32_t *__restrict__ ID, const do
 // every published tile state carries it, so no state is cleared between
 // launches (and a captured graph needs no memset node).  State word:
 // bits 63..34 epoch, 33..32 kind (1 = tile total, 2 = inclusive prefix),
-// 31..0 value.  Each tile holds kScanTile2 = 4096 counts.  A look-back that
+// 31..0 value.  A tile holds kScanThreads * ITEMS counts: 4096 up to 1M
+// counts, 16384 above (fewer tiles: fewer block starts and polling rounds on
+// large grids, more tiles in flight on small ones).  A look-back that
 // does not see its predecessors within a bounded number of polls raises
 // MAVI_FLAG_SCAN_STALL and takes 0 for the missing part instead of waiting
 // for ever.
-constexpr int kScanItems2 = 16;
-constexpr int kScanTile2 = kScanThreads * kScanItems2;
+constexpr int kScanItemsSmall = 16, kScanItemsLarge = 64;
+constexpr int kScanTileMin = kScanThreads * kScanItemsSmall;  // bounds the tile count from above
+constexpr int kScanLargeFrom = 1 << 20;
 constexpr uint32_t kScanMaxPolls = 1u << 20;
 
 __device__ __forceinline__ unsigned long long scan_word(uint32_t epoch, uint32_t kind, int32_t v) {
     return ((unsigned long long)epoch << 34) | ((unsigned long long)kind << 32) | (uint32_t)v;
 }
 
+template <int ITEMS>
 __global__ __launch_bounds__(kScanThreads) void k_scan_lookback(const int32_t *__restrict__ in, int32_t n,
                                                                 int32_t *__restrict__ out,
                                                                 unsigned long long *__restrict__ ticket,
@@ -739,12 +743,12 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_lookback(const int32_t *_
     __syncthreads();
     const int tile = s_tile;
     const uint32_t epoch = s_epoch;
-    const int base = tile * kScanTile2 + threadIdx.x * kScanItems2;
-    int32_t v[kScanItems2];
-    if (base + kScanItems2 <= n) {
+    const int base = tile * (kScanThreads * ITEMS) + threadIdx.x * ITEMS;
+    int32_t v[ITEMS];
+    if (base + ITEMS <= n) {
         const int4 *q = reinterpret_cast<const int4 *>(in + base);
 #pragma unroll
-        for (int k = 0; k < kScanItems2 / 4; k++) {
+        for (int k = 0; k < ITEMS / 4; k++) {
             const int4 w = q[k];
             v[4 * k] = w.x;
             v[4 * k + 1] = w.y;
@@ -753,11 +757,11 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_lookback(const int32_t *_
         }
     } else {
 #pragma unroll
-        for (int k = 0; k < kScanItems2; k++) v[k] = (base + k < n) ? in[base + k] : 0;
+        for (int k = 0; k < ITEMS; k++) v[k] = (base + k < n) ? in[base + k] : 0;
     }
     int32_t sum = 0;
 #pragma unroll
-    for (int k = 0; k < kScanItems2; k++) sum += v[k];
+    for (int k = 0; k < ITEMS; k++) sum += v[k];
     int32_t total;
     const int32_t local = block_excl_scan(sum, &total);
     if (threadIdx.x == 0)
@@ -767,7 +771,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_lookback(const int32_t *_
         // one wave looks back over windows of 256 tiles, 4 per lane (lane l
         // holds tiles j - 4l - q, q = 0..3, nearest first): a tile far from
         // the start needs a quarter of the polling rounds one tile per lane
-        // would, and each round waits on memory (2.1M chunks are 523 tiles)
+        // would, and each round waits on memory
         const int lane = threadIdx.x;
         int32_t excl = 0;
         int j = tile - 1;  // nearest tile not yet accounted for
@@ -831,10 +835,10 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_lookback(const int32_t *_
     }
     __syncthreads();
     int32_t off = s_prefix + local;
-    if (base + kScanItems2 <= n) {
+    if (base + ITEMS <= n) {
         int4 *q = reinterpret_cast<int4 *>(out + base);
 #pragma unroll
-        for (int k = 0; k < kScanItems2 / 4; k++) {
+        for (int k = 0; k < ITEMS / 4; k++) {
             int4 w;
             w.x = off;
             w.y = w.x + v[4 * k];
@@ -845,10 +849,26 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_lookback(const int32_t *_
         }
     } else {
 #pragma unroll
-        for (int k = 0; k < kScanItems2; k++) {
+        for (int k = 0; k < ITEMS; k++) {
             if (base + k < n) out[base + k] = off;
             off += v[k];
         }
+    }
+}
+
+// Exclusive scan of cnt[0..n) into start on stream st (tile size by n; one
+// ticket counter and state array per scanned array, with at least
+// ceil(n / kScanTileMin) state words).
+inline void launch_scan(const int32_t *cnt, int32_t n, int32_t *start, unsigned long long *ticket,
+                        unsigned long long *state, uint32_t *flags, hipStream_t st) {
+    if (n > kScanLargeFrom) {
+        const int ntiles = (n + kScanThreads * kScanItemsLarge - 1) / (kScanThreads * kScanItemsLarge);
+        hipLaunchKernelGGL(k_scan_lookback<kScanItemsLarge>, dim3(ntiles), dim3(kScanThreads), 0, st, cnt, n, start,
+                           ticket, state, ntiles, flags);
+    } else {
+        const int ntiles = (n + kScanTileMin - 1) / kScanTileMin;
+        hipLaunchKernelGGL(k_scan_lookback<kScanItemsSmall>, dim3(ntiles), dim3(kScanThreads), 0, st, cnt, n, start,
+                           ticket, state, ntiles, flags);
     }
 }
 

@@ -15,7 +15,7 @@
 //   slot_of      int32[n]      particle id -> sorted slot (rings)
 //   cnt, start   int32[ncell+1] particles per chunk / exclusive scan
 //   scan_state   u64[ntiles]   published tile states of the one-pass scan
-//                              (4096 chunks per tile), scan_ticket its counter
+//                              (4096 or 16384 chunks per tile), scan_ticket its counter
 //   pos_id, vel_id, force_id   double2[n] id-order staging for the host
 // na = number of active ids (get_particles_ids); the chunk index is
 // row0*cols + col0 with row0 = reference row - 1 (row 1 = top of the box,

@@ -1132,9 +1132,7 @@ hipError_t launch_bin(mavi_sys *s, hipStream_t st, bool count_step, bool stamp, 
                            s->cnt, s->flags, s->scell, stamping ? s->stamp : nullptr, s->epoch);
     s->keys_ready = false;
     const int n = P.ncell + 1;
-    const int ntiles = nblk(n, kScanTile2);
-    hipLaunchKernelGGL(k_scan_lookback, dim3(ntiles), dim3(kScanThreads), 0, st, s->cnt, n, s->start,
-                       s->scan_ticket, s->scan_state, ntiles, s->flags);
+    launch_scan(s->cnt, n, s->start, s->scan_ticket, s->scan_state, s->flags, st);
     const int grid = nblk(P.na > n ? P.na : n, 256);
     hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(256), 0, st, P, s->key, s->rank, s->start, s->src, s->scell,
                        s->ID_[c], s->sid, s->cnt, count_step ? s->counter : nullptr, stamping ? s->stamp : nullptr, s->epoch, s->needc);

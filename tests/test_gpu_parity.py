@@ -299,14 +299,14 @@ def test_full_size_force_sample_vs_oracle(gpu_ctx, oracle_mod):
 
 
 def test_large_grid_binning_and_steps(gpu_ctx, oracle_mod):
-    """A grid of 3000 x 750 = 2.25M chunks: the one-pass scan runs 550 tiles
-    of 4096 chunks, so its look-back crosses three windows of 256 tiles.
+    """A grid of 4800 x 1000 = 4.8M chunks: the one-pass scan runs 293 tiles
+    of 16384 chunks, so its look-back crosses two windows of 256 tiles.
     Binning bit-exact against the oracle, then forces and 10 steps."""
     import mavi_hip as mv
 
-    say("[large grid] 2.25M chunks, 120k particles")
+    say("[large grid] 4.8M chunks, 120k particles")
     rng = np.random.default_rng(11)
-    cols, rows = 3000, 750
+    cols, rows = 4800, 1000
     L, H = float(cols), float(rows)
     n = 120_000
     pos = np.stack([rng.uniform(0.0, L, n), rng.uniform(0.0, H, n)], axis=1)
@@ -314,7 +314,7 @@ def test_large_grid_binning_and_steps(gpu_ctx, oracle_mod):
     dyn = mv.HarmTruncCfg(k_rep=10.0, k_atr=1.0, dist_eq=1.0, dist_max=1.0)
     space = mv.SpaceCfg(wall_type=mv.PeriodicWalls(), geometry_cfg=mv.RectangleCfg(length=L, height=H))
     ic = mv.IntCfg(dt=1e-3, chunks_cfg=mv.ChunksCfg(num_cols=cols, num_rows=rows), device=mv.GPU())
-    assert cols * rows > 512 * 4096
+    assert cols * rows > 256 * 16384
     sys_ = make_plain(pos, vel, space, dyn, ic)
     cp, counts = sys_.update_chunks()
     o = oracle_mod.from_system(n, space, dyn, ic)
