@@ -176,7 +176,8 @@ __global__ __launch_bounds__(256) void kb_rings(const Replica *__restrict__ reps
     const bool ring_ok = rr < P.nrings;
     const int rc = ring_ok ? rr : 0;
     const RingType T = types[Q.tbase + (P.typed ? rtype[Q.rbase + rc] : 0)];
-    ring_lanes<true>(P, T, ring_ok, rc, (int)(threadIdx.x & 31), slot_of + Q.nbase, Pst, F, pol + Q.rbase,
+    ring_lanes<true>(P, T, ring_ok, rc, (int)(threadIdx.x & 32), (int)(threadIdx.x & 31), slot_of + Q.nbase, Pst, F,
+                     pol + Q.rbase,
                      areas + Q.rbase, cms + Q.rbase, cms_next + Q.rbase, cpos + Q.cpbase, nullptr, Q.seed,
                      counter[0]);
 }

@@ -950,16 +950,17 @@ __global__ void k_rings_init(SysParams P, const int32_t *__restrict__ rtype, con
     cms[r] = cms_next[r] = cm;
 }
 
-// One ring per 32 lanes (lane = particle index inside the ring).  Every
-// reduction the reference does as a sequential loop (unwrap, centre of mass,
-// shoelace area, mean velocity) is done here as the same left-to-right loop
-// over values broadcast from the owning lanes, so the result is bit-identical
-// to the one-thread-per-ring form.  slot_of and cpos start at the system's
+// One ring per group of lanes of a wave (lane0 = the group's first lane in
+// the wave, i = particle index inside the ring).  Every reduction the
+// reference does as a sequential loop (unwrap, centre of mass, shoelace
+// area, mean velocity) is done here as the same left-to-right loop over
+// values broadcast from the owning lanes, so the result is bit-identical to
+// the one-thread-per-ring form.  slot_of and cpos start at the system's
 // first particle id, pol/areas/cms/cms_next at its first ring; noise (host
 // draws) or the Philox stream (seed, step) give the rotational noise of ring rr.
 template <bool UPDATE, bool WALLS = false>
 __device__ __forceinline__ void ring_lanes(const SysParams &P, const RingType &T, const bool ring_ok, const int rr,
-                                           const int i, const int32_t *__restrict__ slot_of,
+                                           const int lane0, const int i, const int32_t *__restrict__ slot_of,
                                            double2 *__restrict__ Pst, double2 *__restrict__ F,
                                            double *__restrict__ pol, double *__restrict__ areas,
                                            double2 *__restrict__ cms, double2 *__restrict__ cms_next,
@@ -971,7 +972,7 @@ __device__ __forceinline__ void ring_lanes(const SysParams &P, const RingType &T
     const int64_t base = (int64_t)rr * P.nmax;
     const int slot = act ? slot_of[base + i] : 0;
     const double2 x = act ? Pst[slot] : make_double2(0.0, 0.0);
-    auto bc = [](double v, int src) { return __shfl(v, src, 32); };
+    auto bc = [lane0](double v, int src) { return __shfl(v, lane0 + src, 64); };
 
     if constexpr (UPDATE) {
         if (ring_ok && i == 0) cms[r] = cms_next[r];
@@ -1085,13 +1086,18 @@ __global__ __launch_bounds__(256) void k_rings_lanes(
     const int32_t *__restrict__ slot_of, double2 *__restrict__ Pst, double2 *__restrict__ F, double *__restrict__ pol,
     double *__restrict__ areas, double2 *__restrict__ cms, double2 *__restrict__ cms_next, double2 *__restrict__ cpos,
     const double *__restrict__ noise, uint64_t seed, const unsigned long long *__restrict__ counter) {
-    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-    const int r = gt >> 5;
-    const bool ring_ok = r < P.nrings && (!P.rmask || P.rmask[r]);  // get_rings_ids
+    // floor(64 / nmax) rings per wave, nmax lanes each (3 rings of 20 per
+    // wave: 60 busy lanes instead of 40 with one ring per 32 lanes)
+    const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int w = (int)(threadIdx.x & 63);
+    const int per = 64 / P.nmax;
+    const int grp = w / P.nmax, i = w - grp * P.nmax;
+    const int r = wave * per + grp;
+    const bool ring_ok = grp < per && r < P.nrings && (!P.rmask || P.rmask[r]);  // get_rings_ids
     const int rr = ring_ok ? r : 0;
     const RingType T = types[P.typed ? rtype[rr] : 0];
-    ring_lanes<UPDATE, WALLS>(P, T, ring_ok, rr, gt & 31, slot_of, Pst, F, pol, areas, cms, cms_next, cpos, noise, seed,
-                       counter[0]);
+    ring_lanes<UPDATE, WALLS>(P, T, ring_ok, rr, grp * P.nmax, i, slot_of, Pst, F, pol, areas, cms, cms_next, cpos,
+                              noise, seed, counter[0]);
 }
 
 }  // namespace

@@ -13,8 +13,9 @@
 //                   >= 3x3 (src/integration.jl:112-157).
 //   K3 Verlet+walls MODE_VERLET1/2 epilogues of k_pair (src/integration.jl:
 //                   415-431 with walls! :271-285 / :309-324).
-//   K4 rings        k_rings_lanes<UPDATE> (one ring per 32 lanes, rings of
-//                   up to 32 particles) and k_rings<UPDATE> (one ring per
+//   K4 rings        k_rings_lanes<UPDATE> (floor(64 / nmax) rings of nmax
+//                   lanes per wave, rings of up to 32 particles) and
+//                   k_rings<UPDATE> (one ring per
 //                   thread, any size): unwrap, shoelace area, springs, area
 //                   force and overdamped update (src/rings/integration.jl:79-351).
 //   K7 observables  k_observables (src/quantities.jl:12-18, 46-66).
@@ -1183,7 +1184,8 @@ static hipError_t launch_rings_kernel(mavi_sys *s, bool update, const double *no
     const int c = s->cur;
     const double *nz = update ? noise_dev : nullptr;
     if (s->P.nmax <= 32) {
-        const int nb = nblk((int64_t)s->P.nrings * 32, 256);
+        const int per = 64 / s->P.nmax;  // rings per wave
+        const int nb = nblk((int64_t)((s->P.nrings + per - 1) / per) * 64, 256);
         if (update && s->P.nwalls > 0)
             hipLaunchKernelGGL((k_rings_lanes<true, true>), dim3(nb), dim3(256), 0, st, s->P, s->rtype, s->types,
                                s->slot_of, s->P_[c], s->F, s->pol, s->areas, s->cms, s->cms_next, s->cpos, nz,
