@@ -42,6 +42,7 @@ struct Replica {
     uint64_t seed;
     float inv_nmax;
     int32_t kind;         // 0 LJ, 1 HarmTrunc, 2 rings
+    int32_t rwbase;       // first wave of the ring kernel (rings only)
 };
 
 enum { KIND_LJ = 0, KIND_HT = 1, KIND_RINGS = 2, NKIND = 3 };
@@ -156,8 +157,8 @@ __global__ void kb_pair(const Replica *__restrict__ reps, const int2 *__restrict
 }
 
 // Springs, area force and overdamped update of every ring of the rings
-// replicas: wave wl holds the rings 2 wl and 2 wl + 1 of the padded ring
-// space, both of replica ring_wave_rep[wl].
+// replicas: wave wl holds rings of replica ring_wave_rep[wl] only, floor(64 /
+// nmax) of them, from ring (wl - rwbase) * floor(64 / nmax) of the replica.
 __global__ __launch_bounds__(256) void kb_rings(const Replica *__restrict__ reps,
                                                 const int32_t *__restrict__ ring_wave_rep, int32_t nwaves,
                                                 const int32_t *__restrict__ rtype, const RingType *__restrict__ types,
@@ -171,13 +172,15 @@ __global__ __launch_bounds__(256) void kb_rings(const Replica *__restrict__ reps
     const int r = uniform(ring_wave_rep[wl]);
     const Replica &Q = reps[r];
     const SysParams P = Q.P;
-    const int g = 2 * wl + (int)((threadIdx.x >> 5) & 1);
-    const int rr = g - Q.rbase;
-    const bool ring_ok = rr < P.nrings;
+    // floor(64 / nmax) rings of nmax lanes per wave, as the single-system kernel
+    const int w = (int)(threadIdx.x & 63);
+    const int per = 64 / P.nmax;
+    const int grp = w / P.nmax, i = w - grp * P.nmax;
+    const int rr = (wl - Q.rwbase) * per + grp;
+    const bool ring_ok = grp < per && rr < P.nrings;
     const int rc = ring_ok ? rr : 0;
     const RingType T = types[Q.tbase + (P.typed ? rtype[Q.rbase + rc] : 0)];
-    ring_lanes<true>(P, T, ring_ok, rc, (int)(threadIdx.x & 32), (int)(threadIdx.x & 31), slot_of + Q.nbase, Pst, F,
-                     pol + Q.rbase,
+    ring_lanes<true>(P, T, ring_ok, rc, grp * P.nmax, i, slot_of + Q.nbase, Pst, F, pol + Q.rbase,
                      areas + Q.rbase, cms + Q.rbase, cms_next + Q.rbase, cpos + Q.cpbase, nullptr, Q.seed,
                      counter[0]);
 }
@@ -478,7 +481,9 @@ int mavi_batch_finalize(mavi_batch *b) {
             Q.abase = (int32_t)nactive;
             Q.cpbase = ncpos;
             const int64_t nrpad = ((int64_t)h.P.nrings + 1) / 2 * 2;
-            for (int64_t w = 0; w < nrpad / 2; w++) ring_wave_rep.push_back(r);
+            const int per = 64 / std::max(h.P.nmax, 1);  // rings per wave of the ring kernel
+            Q.rwbase = (int32_t)ring_wave_rep.size();
+            for (int64_t w = 0; w < ((int64_t)h.P.nrings + per - 1) / per; w++) ring_wave_rep.push_back(r);
             for (int q = 0; q < h.P.nrings; q++) rtype.push_back(h.rtype[q]);
             for (int64_t q = h.P.nrings; q < nrpad; q++) rtype.push_back(0);
             types.insert(types.end(), h.types.begin(), h.types.end());
