@@ -1,7 +1,7 @@
 """Learned propagator (mavi_mlp_* and mavi_learned_* in include/mavi_hip.h).
 
 No counterpart in the reference: an addition beside the plain FP64 path,
-which stays its fallback and the judge of its error.
+which stays its fallback and the yardstick of its error.
 
   MLP           the network (BF16 weights, FP32 accumulation on the matrix
                 cores): inference, loss + gradients, a device-resident data

@@ -1,4 +1,4 @@
-"""Pins the MLP oracle (oracle/mlp_oracle.py) before it judges the device:
+"""Pins the MLP oracle (oracle/mlp_oracle.py) before it checks the device:
 BF16 rounding against torch's float32 -> bfloat16 conversion, the analytic
 gradient against central finite differences and against torch.autograd in
 float64."""
