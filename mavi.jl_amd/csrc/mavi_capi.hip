@@ -18,14 +18,6 @@ bool is_active(const mavi_sys *s) {
     return s->P.potential == MAVI_POT_SZABO || s->P.potential == MAVI_POT_RTP;
 }
 
-// A state replaced on the device: the chunk keys and counts the last second
-// evaluation left for the next binning describe the old state.
-hipError_t drop_keys(mavi_sys *s, hipStream_t st) {
-    if (!s->keys_ready) return hipSuccess;
-    s->keys_ready = false;
-    return mavi::launch_fill_i32(s->cnt, 0, (int64_t)s->P.ncell + 1, st);
-}
-
 // Turn device flags raised since the last check into a status.
 int check_flags(mavi_sys *s) {
     mavi_ctx *ctx = s->ctx;
@@ -460,7 +452,7 @@ int mavi_sys_upload(mavi_sys *s, const double *pos, const double *vel, const dou
     if (pol && s->P.potential == MAVI_POT_RINGS)
         HIPCHK(ctx, hipMemcpyAsync(s->pol, pol, sizeof(double) * s->P.nrings, hipMemcpyHostToDevice, ctx->stream));
     s->cur = 0;
-    HIPCHK(ctx, drop_keys(s, ctx->stream));
+    HIPCHK(ctx, mavi::drop_keys(s, ctx->stream));
     HIPCHK(ctx, mavi::launch_iota_state(s, ctx->stream));
     if (s->P.potential == MAVI_POT_RINGS) HIPCHK(ctx, mavi::launch_rings_init(s, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -701,6 +693,7 @@ static bool host_inside_sink(const HostSource &q, double x, double y) {
 static int var_step_start(mavi_sys *s) {
     mavi_ctx *ctx = s->ctx;
     hipStream_t st = ctx->stream;
+    HIPCHK(ctx, mavi::drop_keys(s, st));  // keys of steps before the population became variable
     VarReadback *rb = (VarReadback *)s->vstage;
     HIPCHK(ctx, mavi::launch_var_pre(s, st));
     const int64_t fetch = s->P.nrings < kSinkFetch ? s->P.nrings : kSinkFetch;  // the list holds 2 x nrings
@@ -832,7 +825,10 @@ static int steps_impl(mavi_sys *s, int64_t nsteps, const double *noise_host) {
         const bool graphs = s->use_graphs && !s->prof && nsteps >= 4 && !s->var;
         int64_t k = 0;
         while (k < nsteps) {
-            if (graphs && k + 2 <= nsteps && s->cur == 0 && (rings || s->f_valid) && !invasions_due(s, 1)) {
+            // (a rings graph starts from the keys its update passes leave)
+            const bool keys_ok = !mavi::rings_fuse_bin(s) || s->keys_ready;
+            if (graphs && k + 2 <= nsteps && s->cur == 0 && (rings || s->f_valid) && keys_ok &&
+                !invasions_due(s, 1)) {
                 HIPCHK(ctx, ensure_graph(s, st));
                 HIPCHK(ctx, hipGraphLaunch(s->graph2, st));
                 if (s->inv_sched) s->inv_since += 2;
@@ -1347,7 +1343,7 @@ int mavi_autotune(mavi_sys *s, int32_t trial_steps, double *best_us, int32_t *sh
             r = hipMemcpyAsync(s->pol, pol0, sizeof(double) * s->P.nrings, hipMemcpyDeviceToDevice, st);
         s->cur = 0;
         s->f_valid = false;
-        if (r == hipSuccess) r = drop_keys(s, st);
+        if (r == hipSuccess) r = mavi::drop_keys(s, st);
         if (r == hipSuccess) r = mavi::launch_iota_state(s, st);
         if (r == hipSuccess && s->P.potential == MAVI_POT_RINGS) r = mavi::launch_rings_init(s, st);
         if (r == hipSuccess) r = hipStreamSynchronize(st);

@@ -965,7 +965,9 @@ __device__ __forceinline__ void ring_lanes(const SysParams &P, const RingType &T
                                            double *__restrict__ pol, double *__restrict__ areas,
                                            double2 *__restrict__ cms, double2 *__restrict__ cms_next,
                                            double2 *__restrict__ cpos, const double *__restrict__ noise,
-                                           const uint64_t seed, const uint64_t step) {
+                                           const uint64_t seed, const uint64_t step,
+                                           int32_t *__restrict__ nkey = nullptr, int32_t *__restrict__ nrank = nullptr,
+                                           int32_t *__restrict__ ncnt = nullptr, uint32_t *__restrict__ nflags = nullptr) {
     const int r = rr;
     const int np = T.np;
     const bool act = ring_ok && i < np;
@@ -1065,6 +1067,20 @@ __device__ __forceinline__ void ring_lanes(const SysParams &P, const RingType &T
         periodic_wrap(P, px, py);
     }
     if (act) Pst[slot] = make_double2(px, py);
+    if (ncnt) {
+        // the next binning's first pass (chunk and arrival rank of the final
+        // position, as k_bin_count makes them); every lane of the wave takes
+        // part in the arrival ranks
+        uint32_t flag = 0;
+        int32_t k = -1;
+        if (act) k = cell_of(P, px, py, &flag);
+        if (flag) atomicOr(nflags, flag);
+        const int32_t rk = wave_arrival_rank(act, k, ncnt);
+        if (act) {
+            nkey[slot] = k;
+            nrank[slot] = rk;
+        }
+    }
     if (ring_ok && i == 0) {
         vcx = vcx / (double)np;
         vcy = vcy / (double)np;
@@ -1085,7 +1101,8 @@ __global__ __launch_bounds__(256) void k_rings_lanes(
     SysParams P, const int32_t *__restrict__ rtype, const RingType *__restrict__ types,
     const int32_t *__restrict__ slot_of, double2 *__restrict__ Pst, double2 *__restrict__ F, double *__restrict__ pol,
     double *__restrict__ areas, double2 *__restrict__ cms, double2 *__restrict__ cms_next, double2 *__restrict__ cpos,
-    const double *__restrict__ noise, uint64_t seed, const unsigned long long *__restrict__ counter) {
+    const double *__restrict__ noise, uint64_t seed, const unsigned long long *__restrict__ counter,
+    int32_t *__restrict__ nkey, int32_t *__restrict__ nrank, int32_t *__restrict__ ncnt, uint32_t *__restrict__ nflags) {
     // floor(64 / nmax) rings per wave, nmax lanes each (3 rings of 20 per
     // wave: 60 busy lanes instead of 40 with one ring per 32 lanes)
     const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
@@ -1097,7 +1114,7 @@ __global__ __launch_bounds__(256) void k_rings_lanes(
     const int rr = ring_ok ? r : 0;
     const RingType T = types[P.typed ? rtype[rr] : 0];
     ring_lanes<UPDATE, WALLS>(P, T, ring_ok, rr, grp * P.nmax, i, slot_of, Pst, F, pol, areas, cms, cms_next, cpos,
-                              noise, seed, counter[0]);
+                              noise, seed, counter[0], nkey, nrank, ncnt, nflags);
 }
 
 }  // namespace

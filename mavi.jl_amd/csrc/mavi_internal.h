@@ -305,6 +305,17 @@ hipError_t launch_invasions(mavi_sys *s, bool prep, hipStream_t st);
 hipError_t launch_var_pre(mavi_sys *s, hipStream_t st);
 // zero cms / cms_next of the inactive ring slots (RingsInfo.cms starts at zero)
 hipError_t launch_var_clear(mavi_sys *s, hipStream_t st);
+
+// A rings update does the next binning's first pass itself (fixed ring
+// populations on a chunk grid, rings of up to 32 particles): the binning
+// then starts from its keys (keys_ready).
+inline bool rings_fuse_bin(const mavi_sys *s) {
+    return s->P.potential == MAVI_POT_RINGS && !s->var && s->P.cols > 0 && s->P.nmax <= 32 && s->P.na > 0;
+}
+
+// The state was replaced or moved outside the kernels that keep the chunk
+// keys: drop the keys and counts a previous pass left for the next binning.
+hipError_t drop_keys(mavi_sys *s, hipStream_t st);
 hipError_t launch_observables(mavi_sys *s, double *d_out, hipStream_t st);
 hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t st);
 }  // namespace mavi

@@ -1179,6 +1179,12 @@ hipError_t launch_verlet_step(mavi_sys *s, hipStream_t st, bool reuse) {
     return e;
 }
 
+hipError_t drop_keys(mavi_sys *s, hipStream_t st) {
+    if (!s->keys_ready) return hipSuccess;
+    s->keys_ready = false;
+    return launch_fill_i32(s->cnt, 0, (int64_t)s->P.ncell + 1, st);
+}
+
 static hipError_t launch_rings_kernel(mavi_sys *s, bool update, const double *noise_dev, hipStream_t st) {
     ProfScope ps(s, MAVI_PROF_RINGS, st);
     const int c = s->cur;
@@ -1186,18 +1192,24 @@ static hipError_t launch_rings_kernel(mavi_sys *s, bool update, const double *no
     if (s->P.nmax <= 32) {
         const int per = 64 / s->P.nmax;  // rings per wave
         const int nb = nblk((int64_t)((s->P.nrings + per - 1) / per) * 64, 256);
+        // an update also does the next binning's first pass (chunk keys,
+        // arrival ranks, counts) on the positions it writes
+        const bool fuse = update && rings_fuse_bin(s);
+        int32_t *nk = fuse ? s->key : nullptr, *nr = fuse ? s->rank : nullptr, *nc = fuse ? s->cnt : nullptr;
+        uint32_t *nf = fuse ? s->flags : nullptr;
+        if (fuse) s->keys_ready = true;
         if (update && s->P.nwalls > 0)
             hipLaunchKernelGGL((k_rings_lanes<true, true>), dim3(nb), dim3(256), 0, st, s->P, s->rtype, s->types,
                                s->slot_of, s->P_[c], s->F, s->pol, s->areas, s->cms, s->cms_next, s->cpos, nz,
-                               s->seed, s->counter);
+                               s->seed, s->counter, nk, nr, nc, nf);
         else if (update)
             hipLaunchKernelGGL(k_rings_lanes<true>, dim3(nb), dim3(256), 0, st, s->P, s->rtype, s->types, s->slot_of,
                                s->P_[c], s->F, s->pol, s->areas, s->cms, s->cms_next, s->cpos, nz, s->seed,
-                               s->counter);
+                               s->counter, nk, nr, nc, nf);
         else
             hipLaunchKernelGGL(k_rings_lanes<false>, dim3(nb), dim3(256), 0, st, s->P, s->rtype, s->types,
                                s->slot_of, s->P_[c], s->F, s->pol, s->areas, s->cms, s->cms_next, s->cpos, nz,
-                               s->seed, s->counter);
+                               s->seed, s->counter, nk, nr, nc, nf);
         return hipGetLastError();
     }
     if (update)

@@ -814,6 +814,9 @@ int mavi_learned_run(mavi_learned *l, int64_t nmacro, int32_t use_net, int64_t *
     const int un = use_net ? 1 : 0;
     int64_t acc = 0;
     for (int64_t q = 0; q < nmacro; q++) {
+        // a learned step moves the particles outside the kernels that keep
+        // the chunk keys: its binning starts from scratch (also when captured)
+        HIPCHK(ctx, mavi::drop_keys(s, st));
         if (s->use_graphs && !s->prof) {
             hipGraphExec_t &g = l->graph[s->cur][un];
             if (!g) HIPCHK(ctx, learned_graph(l, st, &g));
