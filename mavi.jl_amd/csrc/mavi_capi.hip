@@ -825,8 +825,9 @@ static int steps_impl(mavi_sys *s, int64_t nsteps, const double *noise_host) {
         const bool graphs = s->use_graphs && !s->prof && nsteps >= 4 && !s->var;
         int64_t k = 0;
         while (k < nsteps) {
-            // (a rings graph starts from the keys its update passes leave)
-            const bool keys_ok = !mavi::rings_fuse_bin(s) || s->keys_ready;
+            // (a rings or self-propelled graph starts from the keys its
+            // update passes leave)
+            const bool keys_ok = !mavi::update_fuses_bin(s) || s->keys_ready;
             if (graphs && k + 2 <= nsteps && s->cur == 0 && (rings || s->f_valid) && keys_ok &&
                 !invasions_due(s, 1)) {
                 HIPCHK(ctx, ensure_graph(s, st));

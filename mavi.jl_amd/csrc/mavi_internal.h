@@ -306,11 +306,14 @@ hipError_t launch_var_pre(mavi_sys *s, hipStream_t st);
 // zero cms / cms_next of the inactive ring slots (RingsInfo.cms starts at zero)
 hipError_t launch_var_clear(mavi_sys *s, hipStream_t st);
 
-// A rings update does the next binning's first pass itself (fixed ring
-// populations on a chunk grid, rings of up to 32 particles): the binning
-// then starts from its keys (keys_ready).
-inline bool rings_fuse_bin(const mavi_sys *s) {
-    return s->P.potential == MAVI_POT_RINGS && !s->var && s->P.cols > 0 && s->P.nmax <= 32 && s->P.na > 0;
+// The update kernel of a rings step (fixed ring population, rings of up to
+// 32 particles) or of a self-propelled step does the next binning's first
+// pass itself on a chunk grid: the binning then starts from its keys
+// (keys_ready).
+inline bool update_fuses_bin(const mavi_sys *s) {
+    if (s->P.cols <= 0 || s->P.na <= 0) return false;
+    if (s->P.potential == MAVI_POT_RINGS) return !s->var && s->P.nmax <= 32;
+    return s->P.potential == MAVI_POT_SZABO || s->P.potential == MAVI_POT_RTP;
 }
 
 // The state was replaced or moved outside the kernels that keep the chunk

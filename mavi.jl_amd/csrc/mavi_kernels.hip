@@ -505,7 +505,7 @@ __global__ void k_active(SysParams P, const double2 *__restrict__ pin, const dou
                          const int32_t *__restrict__ idin, const int32_t *__restrict__ scell,
                          const int32_t *__restrict__ start, double2 *__restrict__ pout, double2 *__restrict__ aout,
                          int32_t *__restrict__ idout, double2 *__restrict__ Fout, const double *__restrict__ noise,
-                         uint64_t seed, const unsigned long long *__restrict__ counter) {
+                         uint64_t seed, const unsigned long long *__restrict__ counter, NextBin nb) {
     const int s = xcd_block() * blockDim.x + threadIdx.x;
     const bool valid = s < P.na;
     const int sv = valid ? s : 0;
@@ -517,7 +517,8 @@ __global__ void k_active(SysParams P, const double2 *__restrict__ pin, const dou
     } else {
         pair_gather_chunks<POT>(P, R, acc, pin, scell[sv], start, s, valid, pi, INFINITY, nullptr);
     }
-    if (!valid) return;
+    double xo = 0.0, yo = 0.0;  // final position
+    if (valid) {
     double fx = acc.fx, fy = acc.fy;
     if constexpr (WALLS) wall_forces(P, pi.x, pi.y, fx, fy);  // calc_walls_forces!
     Fout[s] = make_double2(fx, fy);
@@ -559,6 +560,20 @@ __global__ void k_active(SysParams P, const double2 *__restrict__ pin, const dou
     pout[s] = make_double2(x, y);
     aout[s] = make_double2(th, 0.0);
     idout[s] = id;
+    xo = x;
+    yo = y;
+    }
+    if (nb.key) {  // the next binning's first pass; every lane takes part in the arrival ranks
+        uint32_t flag = 0;
+        int32_t k = -1;
+        if (valid) k = cell_of(P, xo, yo, &flag);
+        if (flag) atomicOr(nb.flags, flag);
+        const int32_t rk = wave_arrival_rank(valid, k, nb.cnt);
+        if (valid) {
+            nb.key[s] = k;
+            nb.rank[s] = rk;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1047,9 +1062,15 @@ template <int MODE, int POT, bool ALL, bool WALLS>
 void launch_active_k(mavi_sys *s, const double *noise_dev, hipStream_t st) {
     const int wg = s->wg;
     const int c = s->cur, o = 1 - c;
+    // the next binning's first pass on the positions the step writes
+    NextBin nb{};
+    if (mavi::update_fuses_bin(s)) {
+        nb = NextBin{s->key, s->rank, s->cnt, s->flags, nullptr};
+        s->keys_ready = true;
+    }
     hipLaunchKernelGGL((k_active<MODE, POT, ALL, WALLS>), dim3(nblk(s->P.na, wg)), dim3(wg), 0, st, s->P, s->P_[c],
                        s->V_[c], s->ID_[c], s->scell, s->start, s->P_[o], s->V_[o], s->ID_[o], s->F, noise_dev,
-                       s->seed, s->counter);
+                       s->seed, s->counter, nb);
 }
 
 template <int MODE, int POT>
@@ -1194,7 +1215,7 @@ static hipError_t launch_rings_kernel(mavi_sys *s, bool update, const double *no
         const int nb = nblk((int64_t)((s->P.nrings + per - 1) / per) * 64, 256);
         // an update also does the next binning's first pass (chunk keys,
         // arrival ranks, counts) on the positions it writes
-        const bool fuse = update && rings_fuse_bin(s);
+        const bool fuse = update && update_fuses_bin(s);
         int32_t *nk = fuse ? s->key : nullptr, *nr = fuse ? s->rank : nullptr, *nc = fuse ? s->cnt : nullptr;
         uint32_t *nf = fuse ? s->flags : nullptr;
         if (fuse) s->keys_ready = true;
