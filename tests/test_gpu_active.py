@@ -192,6 +192,16 @@ def test_graph_equals_direct_and_replays(gpu_ctx, name):
     assert np.array_equal(a.state.pol_angle, b.state.pol_angle)
     assert np.array_equal(a.forces, b.forces)
     assert not np.array_equal(a.state.pol_angle, c.state.pol_angle)  # another noise stream
+    # a state uploaded over the chunk keys the last update left: the keys
+    # are dropped and the binning starts from scratch, graphs or not
+    for s_ in (a, b):
+        s_.state.pos[...] = pos
+        s_.state.pol_angle[...] = ang
+        s_.upload()
+        s_.advance(9)
+        s_.download()
+    assert np.array_equal(a.state.pos, b.state.pos)
+    assert np.array_equal(a.state.pol_angle, b.state.pol_angle)
 
 
 def test_unsupported_configurations(gpu_ctx):

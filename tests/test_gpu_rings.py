@@ -233,3 +233,34 @@ def test_contact_neighbors_vs_oracle(gpu_ctx, oracle_mod, ptype, chunks):
         assert np.array_equal(lists[i, :counts[i]], np.sort(rl[i, :rc[i]]))
         assert np.all(lists[i, counts[i]:] == -1)
     assert np.array_equal(s.get_neigh_list(1), lists[0, :counts[0]])
+
+
+def test_rings_fused_keys_graphs_equal_single_steps(gpu_ctx):
+    """The ring update does the next binning's first pass.  16 steps in one
+    call (two-step graphs that start from those keys) equal 16 one-step calls
+    (direct launches) bit for bit, also after the state is uploaded again in
+    the middle of the run (the keys are dropped and the binning starts from
+    scratch)."""
+    say("[rings fused keys] graphs vs single steps")
+    a, pos0, pol0, _ = build("normal")
+    b, _, _, _ = build("normal")
+    for s_ in (a, b):
+        s_.upload()
+    a.advance(16)
+    for _ in range(16):
+        b.advance(1)
+    for s_ in (a, b):
+        s_.download()
+    assert np.array_equal(a.state.pos, b.state.pos) and np.array_equal(a.state.pol, b.state.pol)
+    mid_pos, mid_pol = a.state.pos.copy(), a.state.pol.copy()
+    for s_ in (a, b):  # a new state over the kept keys of the last step
+        s_.state.pos[...] = pos0
+        s_.state.pol[...] = pol0
+        s_.upload()
+    a.advance(8)
+    for _ in range(8):
+        b.advance(1)
+    for s_ in (a, b):
+        s_.download()
+    assert np.array_equal(a.state.pos, b.state.pos) and np.array_equal(a.state.pol, b.state.pol)
+    assert not np.array_equal(a.state.pos, mid_pos) or not np.array_equal(a.state.pol, mid_pol)
