@@ -110,3 +110,73 @@ def test_observables_brute_force(oracle_mod):
             r = math.hypot(*d)
             pe += 4 * ((1 / r) ** 12 - (1 / r) ** 6)
     assert s.potential_energy_lj(pos) == pytest.approx(pe, rel=1e-12)
+
+
+def _lj_pair(ri, rj):
+    """Force on i from j for LJ(sigma = 1, eps = 1) with no cutoff, written from
+    U(d) = 4 (d^-12 - d^-6): F_i = -U'(d) (ri - rj) / d."""
+    dr = np.asarray(ri, dtype=np.float64) - np.asarray(rj, dtype=np.float64)
+    d = math.hypot(*dr)
+    return (48.0 / d ** 13 - 24.0 / d ** 7) / d * dr
+
+
+def _stale_setup(oracle_mod):
+    """Rigid 8 x 6 box, 4 x 3 chunks of 2 x 2, LJ with no cutoff (the
+    reference's LenJonesCfg): the pair set is "same or adjacent chunk".
+    A (chunk column 1) and C (column 4) are at rest; B moves right from column
+    2 (x = 3.95) into column 3 (x' = 4.05) during the drift."""
+    s = oracle_mod.Sys(n=3, bl=(0.0, 0.0), size=(8.0, 6.0), periodic=False, kind=oracle_mod.KIND_LJ,
+                       p=[1.0, 1.0, 0.0, 0.0], dt=0.01, cols=4, rows=3, nc=8, radius=0.2)
+    pos = np.array([[1.9, 3.0], [3.95, 3.0], [6.1, 3.0]])
+    vel = np.array([[0.0, 0.0], [10.0, 0.0], [0.0, 0.0]])
+    return s, pos, vel
+
+
+def test_second_evaluation_keeps_stale_chunks(oracle_mod):
+    """update_verlet! (src/integration.jl:415-431) evaluates the second force
+    on the chunks binned before the drift.  B leaves A's neighbourhood and
+    enters C's during the drift, so the stale pair set {A-B} and the rebinned
+    one {B-C} give different F' and different velocities."""
+    s, pos, vel = _stale_setup(oracle_mod)
+    dt = 0.01
+    # the chunk columns before and after the drift (1-based, as the reference)
+    assert [s.cell(*p)[2] for p in pos] == [1, 2, 4]
+    f0 = np.zeros((3, 2))
+    f0[0] = _lj_pair(pos[0], pos[1])  # A-B: chunks 1 and 2 are adjacent
+    f0[1] = -f0[0]  # B-C: chunks 2 and 4 are not; C has no partner
+    x1 = pos + vel * dt + f0 * (dt * dt / 2)
+    assert s.cell(*x1[1])[2] == 3
+    # stale chunks: the second evaluation keeps A-B at the new positions
+    f1 = np.zeros((3, 2))
+    f1[0] = _lj_pair(x1[0], x1[1])
+    f1[1] = -f1[0]
+    v_stale = vel + dt / 2 * (f0 + f1)
+    # rebinned before the second evaluation: B-C instead of A-B
+    g1 = np.zeros((3, 2))
+    g1[1] = _lj_pair(x1[1], x1[2])
+    g1[2] = -g1[1]
+    v_rebin = vel + dt / 2 * (f0 + g1)
+    p, v, f, _, _ = s.newton_steps(pos, vel, 1)
+    assert np.allclose(p, x1, rtol=0, atol=1e-14)
+    assert np.allclose(v, v_stale, rtol=1e-12, atol=1e-15)
+    # the two semantics differ by far more than the tolerance, on all three
+    # particles (the motion is along x)
+    assert np.abs(v_stale - v_rebin)[:, 0].min() > 1e-4
+    assert np.abs(v - v_rebin)[:, 0].min() > 1e-4
+    # the forces left in the system are the second (stale) evaluation
+    assert np.allclose(f, f1, rtol=1e-12, atol=1e-15)
+
+
+def test_stale_pair_set_across_two_steps(oracle_mod):
+    """The next step bins afresh: its first evaluation uses B-C (B now in
+    chunk 3), not A-B."""
+    s, pos, vel = _stale_setup(oracle_mod)
+    p1, v1, _, _, _ = s.newton_steps(pos, vel, 1)
+    p2, v2, f2, _, _ = s.newton_steps(pos, vel, 2)
+    dt = 0.01
+    g0 = np.zeros((3, 2))
+    g0[1] = _lj_pair(p1[1], p1[2])
+    g0[2] = -g0[1]
+    x2 = p1 + v1 * dt + g0 * (dt * dt / 2)
+    assert np.allclose(p2, x2, rtol=0, atol=1e-13)
+    assert abs(g0[0]).max() == 0.0 and abs(g0[1]).max() > 1e-3
