@@ -801,7 +801,19 @@ int mavi_batch_autotune(mavi_batch *b, int32_t trial_steps, double *best_us, int
         }
         b->wg[slot] = bw;
         b->listed[slot] = bl;
-        best_total = best;
+    }
+    // the chosen shapes of both slots together: a mixed batch steps both
+    // kinds, so the time of one slot's search is not the step's time
+    if (e == hipSuccess) e = restore();
+    if (e == hipSuccess) e = batch_step(b, st);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(b->ev0, st);
+    for (int k = 0; k < trial_steps && e == hipSuccess; k++) e = batch_step(b, st);
+    if (e == hipSuccess) e = hipEventRecord(b->ev1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(b->ev1);
+    if (e == hipSuccess) {
+        float ms = 0.f;
+        e = hipEventElapsedTime(&ms, b->ev0, b->ev1);
+        best_total = 1e3 * ms / trial_steps;
     }
     b->prof = saved_prof;
     drop_graph(b);

@@ -151,6 +151,18 @@ def test_batch_autotune_restores_state(gpu_ctx):
     best, shape = b.autotune(trial_steps=2)
     say(f"[batch autotune] {best:.1f} us/step, {shape}")
     assert best > 0 and shape["wg_plain"] in (64, 128, 256, 512) and shape["wg_rings"] in (64, 128, 256, 512)
+    # best_us is a whole step of the mixed batch with both chosen shapes: at
+    # least the pair and ring kernels of both kinds that one step launches
+    from mavi_hip import _lib
+
+    b.prof_enable(True)
+    b.run(4)
+    kern_ms = sum(b.prof_get(w)[0] for w in (_lib.MAVI_PROF_BIN, _lib.MAVI_PROF_FORCE, _lib.MAVI_PROF_RINGS))
+    b.prof_enable(False)
+    say(f"[batch autotune] profiled kernels {1e3 * kern_ms / 4:.1f} us/step")
+    assert best >= 0.5 * 1e3 * kern_ms / 4
+    b = build_batch([(spec, spec[2](), 3 + i) for i, spec in enumerate(specs)], gpu_ctx)
+    b.autotune(trial_steps=2)
     b.run(5)
     for i, (name, _, _) in enumerate(specs):
         pos, vel, pol = b.download(i)
