@@ -397,6 +397,7 @@ int mavi_batch_create(mavi_ctx *ctx, mavi_batch **out) {
     if (!ctx || !out) return MAVI_E_BADARG;
     *out = new mavi_batch();
     (*out)->ctx = ctx;
+    ctx_retain(ctx);
     return MAVI_OK;
 }
 
@@ -420,7 +421,9 @@ int mavi_batch_destroy(mavi_batch *b) {
     dfree(b->rtype); dfree(b->types); dfree(b->inter);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
+    mavi_ctx *ctx = b->ctx;
     delete b;
+    ctx_release(ctx);
     return MAVI_OK;
 }
 

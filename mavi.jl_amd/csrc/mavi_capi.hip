@@ -85,18 +85,49 @@ int mavi_ctx_create(int device, mavi_ctx **out) {
     return MAVI_OK;
 }
 
-int mavi_ctx_destroy(mavi_ctx *ctx) {
-    if (!ctx) return MAVI_OK;
+static void ctx_free(mavi_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+}
+
+void ctx_retain(mavi_ctx *ctx) { ctx->refs++; }
+
+void ctx_release(mavi_ctx *ctx) {
+    if (--ctx->refs == 0 && ctx->closing) ctx_free(ctx);
+}
+
+int mavi_ctx_destroy(mavi_ctx *ctx) {
+    if (!ctx) return MAVI_OK;
+    if (ctx->refs > 0) {  // systems, batches or networks still use its stream
+        ctx->closing = true;
+        return MAVI_OK;
+    }
+    ctx_free(ctx);
     return MAVI_OK;
 }
 
 const char *mavi_last_error(const mavi_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+static void sys_free(mavi_sys *s);
+
+void sys_retain(mavi_sys *s) { s->refs++; }
+
+void sys_release(mavi_sys *s) {
+    if (--s->refs == 0 && s->closing) sys_free(s);
+}
+
 int mavi_sys_destroy(mavi_sys *s) {
     if (!s) return MAVI_OK;
+    if (s->refs > 0) {  // a learned propagator still steps it
+        s->closing = true;
+        return MAVI_OK;
+    }
+    sys_free(s);
+    return MAVI_OK;
+}
+
+static void sys_free(mavi_sys *s) {
     (void)hipSetDevice(s->ctx->device);
     (void)hipStreamSynchronize(s->ctx->stream);
     if (s->graph2) (void)hipGraphExecDestroy(s->graph2);
@@ -120,8 +151,10 @@ int mavi_sys_destroy(mavi_sys *s) {
     if (s->vstage) (void)hipHostFree(s->vstage);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
+    mavi_ctx *ctx = s->ctx;
+    const bool held = s->ctx_held;
     delete s;
-    return MAVI_OK;
+    if (held) ctx_release(ctx);
 }
 
 }  // extern "C"
@@ -430,6 +463,8 @@ int mavi_sys_create(mavi_ctx *ctx, const mavi_sys_desc *d, mavi_sys **out) {
         mavi_sys_destroy(s);
         return set_err(ctx, MAVI_E_HIP, m);
     }
+    ctx_retain(ctx);
+    s->ctx_held = true;
     *out = s;
     return MAVI_OK;
 }

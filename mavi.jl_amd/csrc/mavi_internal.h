@@ -38,11 +38,23 @@
 #define MAVI_FLAG_RING_OUT_OF_BOX 32u
 #define MAVI_FLAG_RING_CHUNK_OVERFLOW 64u
 
+// Lifetimes.  Systems, batches and networks hold a reference to their
+// context, and a learned propagator to its system and network.  Destroying
+// an object that something still references only marks it; the last release
+// frees it.  Callers (and garbage collectors, which finalize in any order)
+// may therefore destroy objects in any order.
 struct mavi_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
+    int refs = 0;
+    bool closing = false;
 };
+#define MAVI_HIDDEN __attribute__((visibility("hidden")))
+extern "C" {
+MAVI_HIDDEN void ctx_retain(mavi_ctx *ctx);
+MAVI_HIDDEN void ctx_release(mavi_ctx *ctx);  // frees a destroyed context at its last release
+}
 
 struct DevWall;
 struct DevLine;
@@ -150,6 +162,9 @@ struct ProfSlot {
 
 struct mavi_sys {
     mavi_ctx *ctx = nullptr;
+    int refs = 0;          // learned propagators bound to this system
+    bool closing = false;  // destroyed while referenced: freed at the last release
+    bool ctx_held = false; // holds a reference to ctx (set once creation succeeded)
     SysParams P{};
     int32_t wg = 256;  // pair-force workgroup size
     bool two_phase = false; // candidate list for finite-range interactions
@@ -271,6 +286,11 @@ __device__ __forceinline__ double exact_trunc_div(double a, double b) {
 __device__ __forceinline__ double min_image(double dr, double size, double half) {
     double m = (fabs(dr) > half) ? 1.0 : 0.0;
     return dr - m * copysign(size, dr);
+}
+
+extern "C" {
+MAVI_HIDDEN void sys_retain(mavi_sys *s);
+MAVI_HIDDEN void sys_release(mavi_sys *s);  // frees a destroyed system at its last release
 }
 
 // Host-side launchers (mavi_kernels.hip).  Each works on the state copy

@@ -583,6 +583,7 @@ __global__ void k_append_rows(const uint16_t *__restrict__ X, const float *__res
 struct mavi_learned {
     mavi_sys *s = nullptr;
     mavi_mlp *mlp = nullptr;
+    bool held = false;  // holds references to s and mlp (set once creation succeeded)
     LearnParams L{};
     uint16_t *X = nullptr;
     float *Y = nullptr, *T = nullptr;
@@ -728,6 +729,9 @@ int mavi_learned_create(mavi_sys *s, mavi_mlp *mlp, const mavi_learned_desc *d, 
         mavi_learned_destroy(l);
         return set_err(ctx, MAVI_E_HIP, m);
     }
+    sys_retain(s);
+    if (mlp) mlp_retain(mlp);
+    l->held = true;
     *out = l;
     return MAVI_OK;
 }
@@ -742,7 +746,14 @@ int mavi_learned_destroy(mavi_learned *l) {
     dfree(l->X); dfree(l->Y); dfree(l->T); dfree(l->nbr); dfree(l->euler); dfree(l->disp); dfree(l->pos0);
     dfree(l->pol0); dfree(l->rate); dfree(l->omega); dfree(l->U); dfree(l->gate);
     if (l->gate_host) (void)hipHostFree(l->gate_host);
+    mavi_sys *s = l->s;
+    mavi_mlp *mlp = l->mlp;
+    const bool held = l->held;
     delete l;
+    if (held) {
+        if (mlp) mlp_release(mlp);
+        sys_release(s);
+    }
     return MAVI_OK;
 }
 

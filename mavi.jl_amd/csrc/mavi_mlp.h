@@ -25,6 +25,9 @@ constexpr int kSplitRows = 256;  // rows per split-K partial of a weight gradien
 
 struct mavi_mlp {
     mavi_ctx *ctx = nullptr;
+    int refs = 0;          // learned propagators bound to this network
+    bool closing = false;  // destroyed while referenced: freed at the last release
+    bool ctx_held = false; // holds a reference to ctx (set once creation succeeded)
     float *params = nullptr, *grad = nullptr, *m1 = nullptr, *m2 = nullptr;  // FP32 master, gradient, Adam moments
     int64_t adam_t = 0;
     uint16_t *w[4] = {nullptr, nullptr, nullptr, nullptr};  // BF16 W_l^T (W3 padded to 16 rows)
@@ -50,6 +53,11 @@ struct mavi_mlp {
     float *loss_hist = nullptr;
     int64_t loss_hist_cap = 0;
 };
+
+extern "C" {
+MAVI_HIDDEN void mlp_retain(mavi_mlp *p);
+MAVI_HIDDEN void mlp_release(mavi_mlp *p);  // frees a destroyed network at its last release
+}
 
 namespace mavi {
 hipError_t ensure_work(mavi_mlp *p, int M);

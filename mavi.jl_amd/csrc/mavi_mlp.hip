@@ -671,12 +671,31 @@ int mavi_mlp_create(mavi_ctx *ctx, uint64_t seed, mavi_mlp **out) {
         mavi_mlp_destroy(p);
         return set_err(ctx, MAVI_E_HIP, m);
     }
+    ctx_retain(ctx);
+    p->ctx_held = true;
     *out = p;
     return MAVI_OK;
 }
 
+static void mlp_free(mavi_mlp *p);
+
+void mlp_retain(mavi_mlp *p) { p->refs++; }
+
+void mlp_release(mavi_mlp *p) {
+    if (--p->refs == 0 && p->closing) mlp_free(p);
+}
+
 int mavi_mlp_destroy(mavi_mlp *p) {
     if (!p) return MAVI_OK;
+    if (p->refs > 0) {  // a learned propagator still runs it
+        p->closing = true;
+        return MAVI_OK;
+    }
+    mlp_free(p);
+    return MAVI_OK;
+}
+
+static void mlp_free(mavi_mlp *p) {
     (void)hipSetDevice(p->ctx->device);
     (void)hipStreamSynchronize(p->ctx->stream);
     dfree(p->params); dfree(p->grad); dfree(p->m1); dfree(p->m2);
@@ -688,8 +707,10 @@ int mavi_mlp_destroy(mavi_mlp *p) {
     }
     dfree(p->dZa); dfree(p->dZb); dfree(p->Yw); dfree(p->dY); dfree(p->loss_part); dfree(p->Xw); dfree(p->Tw);
     dfree(p->dsX); dfree(p->dsT); dfree(p->bX); dfree(p->bT);
+    mavi_ctx *ctx = p->ctx;
+    const bool held = p->ctx_held;
     delete p;
-    return MAVI_OK;
+    if (held) ctx_release(ctx);
 }
 
 int mavi_mlp_num_params(void) { return kMlpParams; }
