@@ -394,6 +394,15 @@ int mavi_mlp_set_params(mavi_mlp *mlp, const float *params);
 /* y[m][2] = MLP(x[m][160]); fused = 1: the inference kernel (weights in
  * LDS), 0: the training forward (GEMM chain). */
 int mavi_mlp_forward(mavi_mlp *mlp, int64_t m, const uint16_t *x, float *y, int32_t fused);
+/* mavi_mlp_forward (fused inference kernel) that allocates nothing: no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (inference in tight serving loops, code that must not call
+ * hipMalloc).  x and y are staged through a fixed device array of the
+ * library, one per GPU, sized for MAVI_MLP_NOALLOC_ROWS rows; more rows fail
+ * with MAVI_E_BADARG.  Calls on one GPU share that array: calls that overlap
+ * on one GPU (two contexts, two host threads) race. */
+#define MAVI_MLP_NOALLOC_ROWS 8192
+int mavi_mlp_forward_noalloc(mavi_mlp *mlp, int64_t m, const uint16_t *x, float *y);
 /* Loss L = 1/(2m) sum (y - t)^2 and its gradient w.r.t. every parameter
  * (MAVI_MLP_NPARAMS floats, parameter layout); either output may be NULL. */
 int mavi_mlp_loss_grad(mavi_mlp *mlp, int64_t m, const uint16_t *x, const float *t, double *loss, float *grad);

@@ -81,6 +81,13 @@ int mavi_ctx_create(int device, mavi_ctx **out) {
         delete ctx;
         return MAVI_E_HIP;
     }
+    // the no-alloc entry point's staging array on this device (first context
+    // of the device only)
+    if (mavi::noalloc_scratch_setup(device) != hipSuccess) {
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return MAVI_E_HIP;
+    }
     *out = ctx;
     return MAVI_OK;
 }

@@ -296,6 +296,9 @@ MAVI_HIDDEN void sys_release(mavi_sys *s);  // frees a destroyed system at its l
 // Host-side launchers (mavi_kernels.hip).  Each works on the state copy
 // s->cur and leaves the result in s->cur (binning flips it).
 namespace mavi {
+// device address of mavi_mlp_forward_noalloc's staging array (mavi_mlp.hip),
+// resolved once per device with that device current
+hipError_t noalloc_scratch_setup(int device);
 hipError_t launch_iota_state(mavi_sys *s, hipStream_t st);          // id-order staging -> state copy cur
 hipError_t launch_unpermute(mavi_sys *s, bool vel, bool force, hipStream_t st);  // state -> staging
 // binning + permutation (cur flips); count_step advances the step counter

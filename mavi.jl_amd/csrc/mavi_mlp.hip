@@ -17,6 +17,7 @@
 // row-major) followed by b_l.  BF16 copies of W_l^T feed the matrix cores.
 #include <math.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -635,6 +636,47 @@ static void init_params(std::vector<float> &p, uint64_t seed) {
     }
 }
 
+namespace {
+
+// Staging array of mavi_mlp_forward_noalloc: MAVI_MLP_NOALLOC_ROWS feature
+// rows (BF16) followed by as many output rows (FP32, padded to kMlpOutPad).
+// Every GPU the library runs on holds its own copy (3.1 MB) until the process
+// ends.  In host code `scratch` only names the symbol: its device address
+// comes from hipGetSymbolAddress, once per device (noalloc_scratch_setup,
+// called by mavi_ctx_create with the device current).
+constexpr int64_t kNoAllocRows = MAVI_MLP_NOALLOC_ROWS;
+constexpr size_t kScratchXBytes = (size_t)kNoAllocRows * kMlpIn * sizeof(uint16_t);
+constexpr size_t kScratchBytes = kScratchXBytes + (size_t)kNoAllocRows * kMlpOutPad * sizeof(float);
+static_assert(kScratchXBytes % 256 == 0, "the output rows start 256-byte aligned");
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+}  // namespace
+
+namespace mavi {
+hipError_t noalloc_scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = ptr;
+    return hipSuccess;
+}
+}  // namespace mavi
+
+namespace {
+void *noalloc_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+}  // namespace
+
 extern "C" {
 
 int mavi_mlp_create(mavi_ctx *ctx, uint64_t seed, mavi_mlp **out) {
@@ -760,6 +802,28 @@ int mavi_mlp_forward(mavi_mlp *p, int64_t m, const uint16_t *x, float *y, int32_
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     dfree(dX);
     dfree(dYo);
+    HIPCHK(ctx, e);
+    return MAVI_OK;
+}
+
+int mavi_mlp_forward_noalloc(mavi_mlp *p, int64_t m, const uint16_t *x, float *y) {
+    if (!p || !x || !y || m <= 0) return MAVI_E_BADARG;
+    mavi_ctx *ctx = p->ctx;
+    if (m > kNoAllocRows)
+        return set_err(ctx, MAVI_E_BADARG, "mavi_mlp_forward_noalloc takes at most MAVI_MLP_NOALLOC_ROWS (" +
+                                               std::to_string(kNoAllocRows) + ") rows, got " + std::to_string(m));
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    char *base = (char *)noalloc_scratch(ctx->device);
+    if (!base) return set_err(ctx, MAVI_E_HIP, "no-alloc staging array not set up on this device");
+    uint16_t *dX = (uint16_t *)base;
+    float *dY = (float *)(base + kScratchXBytes);
+    hipStream_t st = ctx->stream;
+    // every staged row is written before the kernel reads it, and the kernel
+    // writes every output row it is given: nothing needs clearing
+    hipError_t e = hipMemcpyAsync(dX, x, sizeof(uint16_t) * m * kMlpIn, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = mavi::mlp_forward_fused(p, dX, (int)m, dY, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(y, dY, sizeof(float) * m * kMlpOut, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
     HIPCHK(ctx, e);
     return MAVI_OK;
 }

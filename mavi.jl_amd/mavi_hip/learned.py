@@ -19,6 +19,7 @@ from . import _lib
 from ._lib import _DP, _I64P, _P, check, load
 
 N_IN, N_HID, N_OUT, N_PARAMS = 160, 128, 2, 53890
+NOALLOC_ROWS = 8192  # MAVI_MLP_NOALLOC_ROWS
 
 _U16P = C.POINTER(C.c_uint16)
 _FP = C.POINTER(C.c_float)
@@ -30,6 +31,7 @@ SIGNATURES = {
     "mavi_mlp_get_params": (C.c_int, [_P, _FP]),
     "mavi_mlp_set_params": (C.c_int, [_P, _FP]),
     "mavi_mlp_forward": (C.c_int, [_P, C.c_int64, _U16P, _FP, C.c_int32]),
+    "mavi_mlp_forward_noalloc": (C.c_int, [_P, C.c_int64, _U16P, _FP]),
     "mavi_mlp_loss_grad": (C.c_int, [_P, C.c_int64, _U16P, _FP, _DP, _FP]),
     "mavi_mlp_dataset_clear": (C.c_int, [_P]),
     "mavi_mlp_dataset_append": (C.c_int, [_P, C.c_int64, _U16P, _FP]),
@@ -106,6 +108,15 @@ class MLP:
         m = x.size // N_IN
         y = np.zeros((m, N_OUT), dtype=np.float32)
         self._call("mavi_mlp_forward", C.c_int64(m), px, y.ctypes.data_as(_FP), 1 if fused else 0)
+        return y
+
+    def forward_noalloc(self, x_bits) -> np.ndarray:
+        """forward() through mavi_mlp_forward_noalloc: the fused kernel with
+        the library's fixed staging array (at most NOALLOC_ROWS rows)."""
+        x, px = _u16(x_bits)
+        m = x.size // N_IN
+        y = np.zeros((m, N_OUT), dtype=np.float32)
+        self._call("mavi_mlp_forward_noalloc", C.c_int64(m), px, y.ctypes.data_as(_FP))
         return y
 
     def loss_grad(self, x_bits, t):
