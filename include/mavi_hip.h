@@ -309,7 +309,24 @@ int mavi_sys_noise_set(mavi_sys *sys, uint64_t seed, uint64_t counter);
 #define MAVI_OPT_GRAPHS 1
 #define MAVI_OPT_LDS_TILES 2
 #define MAVI_OPT_REUSE_FORCES 3
+/*   MAVI_OPT_NLIST      1 (default): newton_step! of a finite-range plain
+ *                       system (LJ with a cutoff, HarmTrunc) whose chunks are
+ *                       at least the range wide runs on Verlet neighbour
+ *                       lists: every pair within range, the step's chunks
+ *                       kept for the second evaluation (pairs whose
+ *                       step-start chunks are not adjacent left out), forces
+ *                       summed in ascending partner id; lists rebuilt when a
+ *                       particle has moved half the skin.  0: the chunk gather
+ *   MAVI_OPT_NLIST_SKIN skin in thousandths of the range (0: 150)
+ *   MAVI_OPT_NLIST_BLOCK steps between displacement checks (0: 32) */
+#define MAVI_OPT_NLIST 4
+#define MAVI_OPT_NLIST_SKIN 5
+#define MAVI_OPT_NLIST_BLOCK 6
 int mavi_sys_set_option(mavi_sys *sys, int32_t option, int32_t value);
+
+/* Neighbour lists: info[0] the system has them (eligible), [1] they run,
+ * [2] rebuilds and [3] rolled-back blocks so far. */
+int mavi_nlist_info(mavi_sys *sys, int64_t *info);
 
 /* Autotuner: times `trial_steps` steps of the system's own dynamics for every
  * pair-kernel shape (workgroup 64..512 x LDS tiles x two-phase list), keeps

@@ -137,6 +137,7 @@ int mavi_sys_destroy(mavi_sys *s) {
 static void sys_free(mavi_sys *s) {
     (void)hipSetDevice(s->ctx->device);
     (void)hipStreamSynchronize(s->ctx->stream);
+    mavi::nl_free(s);
     if (s->graph2) (void)hipGraphExecDestroy(s->graph2);
     for (int c = 0; c < 2; c++) {
         dfree(s->P_[c]);
@@ -463,6 +464,9 @@ int mavi_sys_create(mavi_ctx *ctx, const mavi_sys_desc *d, mavi_sys **out) {
         s->host_ring_np.resize(P.nrings);
         for (int32_t r = 0; r < P.nrings; r++) s->host_ring_np[r] = host_types[P.typed ? host_rtype[r] : 0].np;
     }
+    // neighbour lists for finite-range plain systems (at most 2^31 list entries)
+    if (e == hipSuccess && mavi::nl_eligible(s) && (int64_t)s->P.na * 64 < (int64_t)INT32_MAX)
+        e = mavi::nl_alloc(s, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     s->seed = h.seed;
     if (e != hipSuccess) {
@@ -494,6 +498,7 @@ int mavi_sys_upload(mavi_sys *s, const double *pos, const double *vel, const dou
     if (pol && s->P.potential == MAVI_POT_RINGS)
         HIPCHK(ctx, hipMemcpyAsync(s->pol, pol, sizeof(double) * s->P.nrings, hipMemcpyHostToDevice, ctx->stream));
     s->cur = 0;
+    if (s->nl) s->nl->valid = false;  // a new state in id order
     HIPCHK(ctx, mavi::drop_keys(s, ctx->stream));
     HIPCHK(ctx, mavi::launch_iota_state(s, ctx->stream));
     if (s->P.potential == MAVI_POT_RINGS) HIPCHK(ctx, mavi::launch_rings_init(s, ctx->stream));
@@ -824,6 +829,114 @@ static bool invasions_due(const mavi_sys *s, int64_t ahead) {
     return s->inv_sched && s->inv_since + ahead >= s->inv_every;
 }
 
+// Neighbour-list stepping of a finite-range plain system: blocks of steps
+// (two-step graphs), after each block one read back of the largest
+// displacement since the last build.  A block in which some particle moved
+// more than skin / 2 may have missed a pair: its start state is restored,
+// the lists are rebuilt there and the block runs again (shorter).  The lists
+// are rebuilt ahead when the displacement would reach skin / 2 within the
+// next block at the rate of the last one.
+static bool nl_active(const mavi_sys *s) { return s->nl && s->nl->on && !s->nl->disabled; }
+
+static hipError_t nl_graph(mavi_sys *s, hipStream_t st) {
+    NList &N = *s->nl;
+    hipGraphExec_t &gx = N.graph[s->cur];  // the kernels hold the state copy's buffers
+    if (gx) return hipSuccess;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    const int kc0 = N.kc;
+    hipError_t e1 = mavi::nl_step(s, st);
+    hipError_t e2 = (e1 == hipSuccess) ? mavi::nl_step(s, st) : e1;
+    e = hipStreamEndCapture(st, &g);
+    N.kc = kc0;
+    if (e1 != hipSuccess) e = e1;
+    if (e == hipSuccess && e2 != hipSuccess) e = e2;
+    if (e == hipSuccess) e = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    return e;
+}
+
+static void nl_drop_graph(mavi_sys *s) {
+    if (!s->nl) return;
+    for (auto &g : s->nl->graph) {
+        if (!g) continue;
+        (void)hipStreamSynchronize(s->ctx->stream);
+        (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+}
+
+static int nl_steps(mavi_sys *s, int64_t nsteps, int64_t *left) {
+    mavi_ctx *ctx = s->ctx;
+    hipStream_t st = ctx->stream;
+    NList &N = *s->nl;
+    const SysParams &P = s->P;
+    const size_t bytes = sizeof(double2) * (size_t)P.na;
+    const double half_skin = sqrt(N.thr2);
+    *left = 0;
+    int64_t k = 0;
+    while (k < nsteps) {
+        const bool fresh = !N.valid;
+        if (fresh) HIPCHK(ctx, mavi::nl_rebuild(s, st));
+        const int64_t blk = std::min<int64_t>(N.block_now, nsteps - k);
+        // the block's start state, to roll back to
+        HIPCHK(ctx, hipMemcpyAsync(N.psnap, s->P_[s->cur], bytes, hipMemcpyDeviceToDevice, st));
+        HIPCHK(ctx, hipMemcpyAsync(N.vsnap, s->V_[s->cur], bytes, hipMemcpyDeviceToDevice, st));
+        HIPCHK(ctx, hipMemcpyAsync(N.fsnap, s->flags, sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+        int64_t done = 0;
+        const bool graphs = s->use_graphs && !s->prof;
+        while (done < blk) {
+            if (graphs && N.kc == 0 && done + 2 <= blk) {
+                HIPCHK(ctx, nl_graph(s, st));
+                HIPCHK(ctx, hipGraphLaunch(N.graph[s->cur], st));
+                done += 2;
+            } else {
+                HIPCHK(ctx, mavi::nl_step(s, st));
+                done += 1;
+            }
+        }
+        HIPCHK(ctx, mavi::nl_reduce_dmax(s, st));
+        HIPCHK(ctx, hipMemcpyAsync(N.host, N.dmax, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        HIPCHK(ctx, hipMemcpyAsync((char *)N.host + 8, N.status, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(ctx, hipStreamSynchronize(st));
+        const double d2 = __builtin_bit_cast(double, N.host[0]);
+        const uint32_t status = (uint32_t)N.host[1];
+        if (status || d2 > N.thr2) {
+            // a step of this block may have used an incomplete list: back to its start
+            HIPCHK(ctx, hipMemcpyAsync(s->P_[s->cur], N.psnap, bytes, hipMemcpyDeviceToDevice, st));
+            HIPCHK(ctx, hipMemcpyAsync(s->V_[s->cur], N.vsnap, bytes, hipMemcpyDeviceToDevice, st));
+            HIPCHK(ctx, hipMemcpyAsync(s->flags, N.fsnap, sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+            N.valid = false;
+            N.rollbacks++;
+            // a list overflowed, or one step from a fresh build moved a
+            // particle more than skin / 2: lists cannot keep up here
+            if (status || (fresh && blk == 1)) {
+                // the chunk gather takes over
+                N.disabled = true;
+                HIPCHK(ctx, hipMemsetAsync(N.status, 0, sizeof(uint32_t), st));
+                HIPCHK(ctx, hipStreamSynchronize(st));
+                s->sorted = false;
+                s->f_valid = false;
+                *left = nsteps - k;  // for the caller's chunk gather
+                return MAVI_OK;
+            }
+            if (N.block_now > 1) N.block_now = std::max(1, N.block_now / 2);
+            continue;
+        }
+        k += blk;
+        // rebuild ahead when the next block would reach skin / 2 at this rate
+        const double d = sqrt(d2);
+        const double rate = (d - N.d_last) / (double)blk;
+        N.d_last = d;
+        if (d + 1.5 * rate * (double)N.block_now > half_skin) N.valid = false;
+        if (N.block_now < N.block) N.block_now = std::min(N.block, 2 * N.block_now);
+    }
+    s->sorted = false;   // not in reference chunk order
+    s->f_valid = false;  // the chunk gather's reuse does not apply to this order
+    return MAVI_OK;
+}
+
 static int steps_impl(mavi_sys *s, int64_t nsteps, const double *noise_host) {
     mavi_ctx *ctx = s->ctx;
     if (nsteps < 0) return set_err(ctx, MAVI_E_BADARG, "nsteps < 0");
@@ -838,6 +951,17 @@ static int steps_impl(mavi_sys *s, int64_t nsteps, const double *noise_host) {
         s->inv_since = due ? 1 : s->inv_since + 1;
         return due ? run_invasions(s) : MAVI_OK;
     };
+    if (!noise_host && nl_active(s) && nsteps > 0) {
+        int64_t left = 0;
+        const int rc = nl_steps(s, nsteps, &left);
+        if (rc != MAVI_OK) return rc;
+        if (left == 0) {
+            int rc2 = sync_staging(s, true);
+            if (rc2 != MAVI_OK) return rc2;
+            return check_flags(s);
+        }
+        nsteps = left;  // lists disabled (a list overflowed): the chunk gather runs the rest
+    }
     if (noise_host) {
         // draws per step: one per ring (rings), one (Szabo) or two (RTP) per particle
         const int64_t nr = s->P.potential == MAVI_POT_RINGS ? s->P.nrings
@@ -1282,6 +1406,16 @@ int mavi_prof_get(mavi_sys *s, int which, double *ms_total, int64_t *launches) {
     return MAVI_OK;
 }
 
+int mavi_nlist_info(mavi_sys *s, int64_t *info) {
+    if (!s || !info) return MAVI_E_BADARG;
+    const NList *N = s->nl;
+    info[0] = N ? 1 : 0;
+    info[1] = (N && N->on && !N->disabled) ? 1 : 0;
+    info[2] = N ? N->rebuilds : 0;
+    info[3] = N ? N->rollbacks : 0;
+    return MAVI_OK;
+}
+
 int mavi_sys_set_tile(mavi_sys *s, int32_t workgroup) {
     if (s) s->f_valid = false;
     if (!s) return MAVI_E_BADARG;
@@ -1342,10 +1476,29 @@ int mavi_sys_set_option(mavi_sys *s, int32_t option, int32_t value) {
             changed = s->reuse != (value != 0);
             s->reuse = value != 0;
             break;
+        case MAVI_OPT_NLIST:
+            if (s->nl) {
+                s->nl->on = value != 0;
+                s->nl->valid = false;
+            }
+            changed = true;
+            break;
+        case MAVI_OPT_NLIST_SKIN:
+            if (value < 0 || value > 2000) return set_err(s->ctx, MAVI_E_BADARG, "skin: 0 (default) .. 2000 thousandths");
+            if (s->nl) {
+                nl_drop_graph(s);
+                mavi::nl_set_skin(s, value == 0 ? 0.15 : value / 1000.0);
+            }
+            break;
+        case MAVI_OPT_NLIST_BLOCK:
+            if (value < 0 || value > 4096) return set_err(s->ctx, MAVI_E_BADARG, "block: 0 (default) .. 4096 steps");
+            if (s->nl) s->nl->block = s->nl->block_now = value == 0 ? 32 : value;
+            break;
         default:
             return set_err(s->ctx, MAVI_E_BADARG, "unknown option");
     }
     if (changed) s->f_valid = false;
+    if (changed) nl_drop_graph(s);
     if (changed && s->graph2) {  // the captured graph holds the old kernels
         (void)hipStreamSynchronize(s->ctx->stream);
         (void)hipGraphExecDestroy(s->graph2);

@@ -37,13 +37,16 @@ __device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
 // rows x cols grid of cl x ch chunks over the chunk space.  A position off
 // the grid raises out_flag and is clamped so that memory accesses stay in
 // bounds.
-__device__ __forceinline__ int32_t cell_in_grid(const SysParams &P, int32_t rows, int32_t cols, double cl,
-                                                double ch, double x, double y, uint32_t *flag, uint32_t out_flag) {
+// Row and column (0-based) of the cell.
+__device__ __forceinline__ void cell_rc(const SysParams &P, int32_t rows, int32_t cols, double cl, double ch, double x,
+                                        double y, uint32_t *flag, uint32_t out_flag, int32_t *row, int32_t *col) {
     double qr = exact_trunc_div((-y + P.bly) + P.H, ch);
     double qc = exact_trunc_div(x - P.blx, cl);
     if (!(fabs(qr) < 1e15) || !(fabs(qc) < 1e15)) {  // NaN / Inf / absurd
         *flag |= MAVI_FLAG_NONFINITE;
-        return 0;
+        *row = 0;
+        *col = 0;
+        return;
     }
     int64_t r = (int64_t)qr;  // 0-based (reference row - 1)
     int64_t c = (int64_t)qc;
@@ -54,7 +57,15 @@ __device__ __forceinline__ int32_t cell_in_grid(const SysParams &P, int32_t rows
         r = r < 0 ? 0 : (r >= rows ? rows - 1 : r);
         c = c < 0 ? 0 : (c >= cols ? cols - 1 : c);
     }
-    return (int32_t)(r * cols + c);
+    *row = (int32_t)r;
+    *col = (int32_t)c;
+}
+
+__device__ __forceinline__ int32_t cell_in_grid(const SysParams &P, int32_t rows, int32_t cols, double cl,
+                                                double ch, double x, double y, uint32_t *flag, uint32_t out_flag) {
+    int32_t r, c;
+    cell_rc(P, rows, cols, cl, ch, x, y, flag, out_flag, &r, &c);
+    return r * cols + c;
 }
 
 __device__ __forceinline__ int32_t cell_of(const SysParams &P, double x, double y, uint32_t *flag) {

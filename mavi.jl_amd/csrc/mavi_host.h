@@ -38,3 +38,25 @@ void dfree(T *&p) {
 inline int nblk(int64_t n, int t) { return (int)((n + t - 1) / t); }
 
 inline bool is_pow2_wg(int32_t w) { return w == 64 || w == 128 || w == 256 || w == 512 || w == 1024; }
+
+// Kernel timing of one launch class with HIP events (mavi_prof_enable): the
+// launches inside the scope are bracketed on the stream they run on.
+struct ProfScope {
+    mavi_sys *s;
+    int which;
+    hipStream_t st;
+    ProfScope(mavi_sys *s_, int w, hipStream_t st_) : s(s_), which(w), st(st_) {
+        if (s->prof) (void)hipEventRecord(s->ev0, st);
+    }
+    ~ProfScope() {
+        if (!s->prof) return;
+        (void)hipEventRecord(s->ev1, st);
+        if (hipEventSynchronize(s->ev1) == hipSuccess) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, s->ev0, s->ev1) == hipSuccess) {
+                s->profs[which].ms += ms;
+                s->profs[which].launches += 1;
+            }
+        }
+    }
+};

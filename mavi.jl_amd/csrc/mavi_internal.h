@@ -160,6 +160,43 @@ struct ProfSlot {
     int64_t launches = 0;
 };
 
+// Verlet neighbour lists of a finite-range plain system (LJ with a cutoff,
+// HarmTrunc) whose chunks are at least the interaction range wide (mavi_nlist.hip).
+struct NList {
+    bool on = true;           // option MAVI_OPT_NLIST
+    bool valid = false;       // built for the current slot order and still complete
+    bool disabled = false;    // a list overflowed: the chunk gather takes over
+    double range = 0.0;       // interaction range (rcut / dist_max)
+    double skin_frac = 0.15;  // skin = skin_frac * range (option, thousandths)
+    double rl2 = 0.0;         // list radius squared (range + skin)^2, with margin
+    double thr2 = 0.0;        // displacement bound squared (skin / 2)^2, with margin
+    double edge = 0.0;        // within this distance of a periodic edge: minimum image
+    int32_t kmax = 48;        // entries per particle
+    int32_t block = 32;       // steps between host checks (option)
+    int32_t block_now = 32;   // current block (halved after a rollback)
+    SysParams G{};            // internal grid for sorting and building (cells >= range + skin)
+    int32_t *nbr = nullptr;   // (kmax / 8) x na groups of 8 16-bit partner offsets (int4), ascending partner id, padded with the particle
+    int32_t *ncount = nullptr;// entries per slot; bit 31: minimum image needed
+    int32_t *key[2] = {nullptr, nullptr};  // reference chunk (row << 16 | col) of each slot (ping-pong: K_n, K_{n+1})
+    int kc = 0;               // key[kc] holds K_n
+    int32_t *ccnt = nullptr;  // particles per reference chunk
+    uint32_t *chg = nullptr;  // this drift: 2 = changed chunk, 1 = a list partner did (stale test)
+    uint8_t *need = nullptr;  // recompute the next first evaluation
+    double2 *xb = nullptr;    // positions at build
+    double2 *psnap = nullptr, *vsnap = nullptr;  // block-start state (rollback)
+    uint32_t *fsnap = nullptr;
+    unsigned long long *dmax = nullptr;  // max displacement^2 since build (ordered bits), reduced per block
+    double *bmax = nullptr;              // per workgroup of the second evaluation: its largest displacement^2 since build
+    int32_t nbmax = 0;
+    uint32_t *status = nullptr;          // bit 0: a list overflowed kmax
+    unsigned long long *scan_ticket = nullptr, *scan_state = nullptr;  // the list grid's scan (cleared per build)
+    int32_t scan_words = 0;
+    unsigned long long *host = nullptr;  // pinned read back: dmax, status
+    double d_last = 0.0;      // sqrt(dmax) at the last block end
+    int64_t rebuilds = 0, rollbacks = 0;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};  // two steps (kc = 0, 1) on state copy 0 / 1 (a rebuild's sort flips it)
+};
+
 struct mavi_sys {
     mavi_ctx *ctx = nullptr;
     int refs = 0;          // learned propagators bound to this system
@@ -250,6 +287,7 @@ struct mavi_sys {
     bool reuse = true;          // option
     bool f_valid = false;       // F holds the last step's second evaluation
     bool keys_ready = false;    // the last second evaluation left key/rank/cnt for the next binning
+    NList *nl = nullptr;        // neighbour lists (finite-range plain systems), null if not eligible
     // graphs of two steps (state starting in copy 0)
     hipGraphExec_t graph2 = nullptr;
     bool use_graphs = true;
@@ -306,8 +344,22 @@ hipError_t launch_unpermute(mavi_sys *s, bool vel, bool force, hipStream_t st); 
 // stamp: mark chunks whose membership changed (force reuse); defer: leave
 // velocities in the old order and write perm (new slot -> old slot) for the
 // next pair launch to move them
+// over: another grid over the same space, with its own scan ticket and
+// states (the one-pass scan's epochs assume one tile count per ticket)
 hipError_t launch_bin(mavi_sys *s, hipStream_t st, bool count_step = false, bool stamp = false,
-                      bool defer = false);
+                      bool defer = false, const SysParams *over = nullptr, unsigned long long *over_ticket = nullptr,
+                      unsigned long long *over_state = nullptr);
+// Neighbour lists (mavi_nlist.hip): eligibility and allocation at creation,
+// the rebuild (sort on the list grid, lists sorted by partner id, reference
+// chunk keys and counts), one newton_step! (two launches), and release.
+bool nl_eligible(const mavi_sys *s);
+hipError_t nl_alloc(mavi_sys *s, hipStream_t st);
+void nl_free(mavi_sys *s);
+hipError_t nl_rebuild(mavi_sys *s, hipStream_t st);
+hipError_t nl_step(mavi_sys *s, hipStream_t st);
+// dmax <- the largest per-workgroup displacement since the build
+hipError_t nl_reduce_dmax(mavi_sys *s, hipStream_t st);
+void nl_set_skin(mavi_sys *s, double skin_frac);
 hipError_t launch_export_chunks(mavi_sys *s, int64_t *d_cp, int64_t *d_counts, hipStream_t st);
 hipError_t launch_forces_only(mavi_sys *s, hipStream_t st);
 // One newton_step!; reuse = the previous call was a Verlet step of this

@@ -1087,26 +1087,6 @@ hipError_t launch_active_t(mavi_sys *s, const double *noise_dev, hipStream_t st)
     return hipGetLastError();
 }
 
-struct ProfScope {
-    mavi_sys *s;
-    int which;
-    hipStream_t st;
-    ProfScope(mavi_sys *s_, int w, hipStream_t st_) : s(s_), which(w), st(st_) {
-        if (s->prof) (void)hipEventRecord(s->ev0, st);
-    }
-    ~ProfScope() {
-        if (!s->prof) return;
-        (void)hipEventRecord(s->ev1, st);
-        if (hipEventSynchronize(s->ev1) == hipSuccess) {
-            float ms = 0.f;
-            if (hipEventElapsedTime(&ms, s->ev0, s->ev1) == hipSuccess) {
-                s->profs[which].ms += ms;
-                s->profs[which].launches += 1;
-            }
-        }
-    }
-};
-
 }  // namespace
 
 namespace mavi {
@@ -1140,9 +1120,13 @@ hipError_t launch_unpermute(mavi_sys *s, bool vel, bool force, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_bin(mavi_sys *s, hipStream_t st, bool count_step, bool stamp, bool defer) {
-    const SysParams &P = s->P;
+hipError_t launch_bin(mavi_sys *s, hipStream_t st, bool count_step, bool stamp, bool defer, const SysParams *over,
+                      unsigned long long *over_ticket, unsigned long long *over_state) {
+    // over: another grid over the same space (the neighbour lists' sort); the
+    // default is the reference chunk grid
+    const SysParams &P = over ? *over : s->P;
     if (P.cols == 0) return hipSuccess;  // chunks = nothing: the state order stays the active id order
+    if (s->nl) s->nl->valid = false;     // a new slot order: lists index slots
     ProfScope ps(s, MAVI_PROF_BIN, st);
     const int c = s->cur, nc = 1 - c;
     // stamps need the previous binning's chunk of every slot (scell)
@@ -1154,7 +1138,8 @@ hipError_t launch_bin(mavi_sys *s, hipStream_t st, bool count_step, bool stamp, 
                            s->cnt, s->flags, s->scell, stamping ? s->stamp : nullptr, s->epoch);
     s->keys_ready = false;
     const int n = P.ncell + 1;
-    launch_scan(s->cnt, n, s->start, s->scan_ticket, s->scan_state, s->flags, st);
+    launch_scan(s->cnt, n, s->start, over ? over_ticket : s->scan_ticket, over ? over_state : s->scan_state, s->flags,
+                st);
     const int grid = nblk(P.na > n ? P.na : n, 256);
     hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(256), 0, st, P, s->key, s->rank, s->start, s->src, s->scell,
                        s->ID_[c], s->sid, s->cnt, count_step ? s->counter : nullptr, stamping ? s->stamp : nullptr, s->epoch, s->needc);

@@ -1,0 +1,594 @@
+// mavi_nlist.hip -- Verlet neighbour lists for newton_step! of finite-range
+// plain systems (LenJonesCfg with a cutoff, HarmTruncCfg) whose chunks are at
+// least the interaction range wide (calc_forces!, src/integration.jl:112-157;
+// update_verlet!, :415-431; newton_step!, :507-515).
+//
+// Which pairs interact.  The reference's calc_forces! visits the pairs of
+// particles in the same or adjacent chunks of the last update_chunks! and the
+// range drops the distant ones.  With chunks at least the range wide, every
+// pair within range at the binned positions is chunk-adjacent, so the first
+// evaluation of a step is "every pair within range".  The second evaluation
+// keeps the step's chunks at the drifted positions: a pair within range whose
+// particles' step-start chunks are not adjacent is left out.  That can only
+// happen to a pair with a particle whose chunk changed in the drift; those
+// particles and their list partners test chunk adjacency pair by pair.
+//
+// Lists.  Every particle lists the others within range + skin, found on a
+// grid of cells at least that wide after sorting the state by those cells.
+// The state keeps that order until the next rebuild, so a step is two
+// launches and no binning.  A list stays complete while no particle has moved
+// more than skin / 2 since the build (no two particles can then close the
+// gap); the second evaluation records the largest displacement, and the host
+// checks it every few steps, rolling the block back and rebuilding when it
+// was exceeded.
+//
+// Order.  A list is sorted by partner id, so a particle's force is the sum of
+// its in-range pairs in ascending partner id whatever the slot order and
+// whenever the list was built.  Runs do not depend on when lists were
+// rebuilt, and the first evaluation of a step takes the previous second
+// evaluation bit for bit wherever the pair set is the same (no chunk change
+// in or around the particle's list).
+//
+// Errors.  The reference chunk of every final position is computed every
+// step (out of box, non-finite), and chunk counts follow the particles that
+// change chunk (the capacity check of update_particle_chunk!,
+// src/chunks.jl:144-146).
+#include <math.h>
+
+#include "mavi_device.h"
+#include "mavi_host.h"
+
+namespace {
+
+constexpr uint32_t kEdgeBit = 0x80000000u;  // ncount: minimum image needed
+constexpr uint32_t kStatusOverflow = 1u;    // a list held more than kmax entries, or a partner more
+                                            // than 32767 slots away (modulo na)
+constexpr int kNlMax = 48;                  // list entries per particle (6 groups of 8)
+constexpr int kNeedBlock = 256;             // workgroup of the evaluation kernels
+
+struct NLArgs {
+    const int4 *nbr;      // groups of 8 partner offsets ((slot - s) mod na in [-32768, 32767]): nbr[g * na + s]
+    const int32_t *ncount;
+    const int32_t *kcur;  // K_n: reference chunk (row << 16 | col) of the step's start positions
+    int32_t *knext;       // K_{n+1}
+    int32_t *ccnt;
+    uint32_t *chg;
+    uint8_t *need;
+    const double2 *xb;
+    double *bmax;
+    double2 *Pst, *V, *F, *Px;
+    uint32_t *flags;
+    int32_t reuse;
+};
+
+__device__ __forceinline__ int32_t pack_rc(int32_t r, int32_t c) { return (r << 16) | c; }
+
+// Chunks of the reference grid adjacent (the same chunk or one of its eight
+// neighbours, across the periodic edges for a periodic box): the pairs
+// calc_forces! visits (src/chunks.jl:61-118 on grids of at least 3 x 3).
+__device__ __forceinline__ bool chunks_adjacent(const SysParams &P, int ri, int ci, int32_t kj) {
+    const int rj = kj >> 16, cj = kj & 0xffff;
+    int dr = abs(ri - rj), dc = abs(ci - cj);
+    if (P.periodic) {
+        dr = min(dr, P.rows - dr);
+        dc = min(dc, P.cols - dc);
+    }
+    return dr <= 1 && dc <= 1;
+}
+
+// One pair.  A list pad is the particle itself (its offsets are 0 and the LJ
+// factor is masked), so it adds nothing.  STALE: with `test`, the pair counts
+// only when the step-start chunks are adjacent.
+template <int POT, bool WRAP, bool STALE>
+__device__ __forceinline__ void nl_pair(const SysParams &P, const double2 pj, const int32_t j, const int s,
+                                        double xi, double yi, bool test, int ri, int ci,
+                                        const int32_t *__restrict__ kcur, PairAcc<POT> &acc) {
+    double dx = xi - pj.x, dy = yi - pj.y;
+    if constexpr (WRAP) {
+        dx = min_image(dx, P.L, P.halfL);
+        dy = min_image(dy, P.H, P.halfH);
+    }
+    bool take = j != s;
+    if constexpr (STALE) take = take && (!test || chunks_adjacent(P, ri, ci, kcur[j]));
+    if constexpr (POT == POT_LJ) {
+        const double f = lj_scale_masked(P, dx * dx + dy * dy, take);
+        acc.fx = __builtin_fma(f, dx, acc.fx);
+        acc.fy = __builtin_fma(f, dy, acc.fy);
+    } else {
+        RingCtx R{};
+        if (take) acc.add(P, R, dx, dy, true, 0);
+    }
+}
+
+// Slots of a group's two entries packed in one word: 16-bit offsets from s
+// taken modulo na (periodic rows put partners about na slots away).
+__device__ __forceinline__ int wrap_slot(int j, int na) { return j < 0 ? j + na : (j >= na ? j - na : j); }
+__device__ __forceinline__ int lo_slot(int s, int32_t w, int na) {
+    return wrap_slot(s + (int32_t)(int16_t)(w & 0xffff), na);
+}
+__device__ __forceinline__ int hi_slot(int s, int32_t w, int na) { return wrap_slot(s + (w >> 16), na); }
+
+// (j - s) mod na folded into [-na/2, na/2]; the caller checks it fits 16 bits.
+__device__ __forceinline__ int32_t slot_offset(int s, int j, int na) {
+    int32_t d = j - s;
+    if (d > na / 2) d -= na;
+    if (d < -(na / 2)) d += na;
+    return d;
+}
+
+// The sum over one particle's list in list order (ascending partner id),
+// eight entries per iteration; the next group is loaded one iteration ahead,
+// so the list and position loads of consecutive groups overlap.
+template <int POT, bool WRAP, bool STALE>
+__device__ __forceinline__ void nl_sum(const SysParams &P, const double2 *__restrict__ pin,
+                                       const int4 *__restrict__ nbr, int s, int groups, double xi, double yi,
+                                       bool test, int ri, int ci, const int32_t *__restrict__ kcur,
+                                       PairAcc<POT> &acc) {
+    if (groups <= 0) return;
+    const int na = P.na;
+    int4 q = nbr[s];
+    for (int g = 0; g < groups; g++) {
+        const int4 c = q;
+        if (g + 1 < groups) q = nbr[(g + 1) * na + s];
+        const int j[8] = {lo_slot(s, c.x, na), hi_slot(s, c.x, na), lo_slot(s, c.y, na), hi_slot(s, c.y, na),
+                          lo_slot(s, c.z, na), hi_slot(s, c.z, na), lo_slot(s, c.w, na), hi_slot(s, c.w, na)};
+        double2 p[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) p[e] = load_slot(pin, j[e]);
+#pragma unroll
+        for (int e = 0; e < 8; e++) nl_pair<POT, WRAP, STALE>(P, p[e], j[e], s, xi, yi, test, ri, ci, kcur, acc);
+    }
+}
+
+__device__ __forceinline__ int32_t pack2(int s, int32_t a, int32_t b, int na) {
+    return (int32_t)(((uint32_t)slot_offset(s, a, na) & 0xffffu) | ((uint32_t)slot_offset(s, b, na) << 16));
+}
+
+// Build, one wave per workgroup: the state is sorted on the list grid G
+// (cells >= range + skin).  Each lane scans the 3x3 cells around its
+// particle's cell and stages the others within the list radius (slot and id)
+// in LDS, ranks them by id and writes them as groups of 8 16-bit offsets in
+// ascending id, the last group padded with the particle itself.  A particle within `edge`
+// of a periodic edge sums with the minimum image (its partners may wrap
+// before the next build).
+__global__ __launch_bounds__(64) void k_nl_build(SysParams G, SysParams P, const double2 *__restrict__ pos,
+                                                 const int32_t *__restrict__ ID, const int32_t *__restrict__ scell,
+                                                 const int32_t *__restrict__ start, int4 *__restrict__ nbr,
+                                                 int32_t *__restrict__ ncount, double rl2, double edge,
+                                                 uint32_t *__restrict__ status) {
+    __shared__ int32_t s_id[kNlMax][64];
+    __shared__ int32_t s_slot[kNlMax][64];
+    __shared__ int32_t s_out[kNlMax][64];
+    const int lane = threadIdx.x;
+    const int s = blockIdx.x * 64 + lane;
+    if (s >= P.na) return;
+    const int na = P.na;
+    const double2 pi = pos[s];
+    const int c = scell[s];
+    const int row = c / G.cols, col = c - row * G.cols;
+    int cnt = 0;
+    for (int dr = -1; dr <= 1; dr++) {
+        int r = row + dr;
+        if (G.periodic) r = r < 0 ? r + G.rows : (r >= G.rows ? r - G.rows : r);
+        else if (r < 0 || r >= G.rows) continue;
+        for (int dc = -1; dc <= 1; dc++) {
+            int cc = col + dc;
+            if (G.periodic) cc = cc < 0 ? cc + G.cols : (cc >= G.cols ? cc - G.cols : cc);
+            else if (cc < 0 || cc >= G.cols) continue;
+            const int cell = r * G.cols + cc;
+            const int b = start[cell], e = start[cell + 1];
+            for (int j = b; j < e; j++) {
+                if (j == s) continue;
+                const double2 pj = pos[j];
+                double dx = pi.x - pj.x, dy = pi.y - pj.y;
+                if (P.periodic) {
+                    dx = min_image(dx, P.L, P.halfL);
+                    dy = min_image(dy, P.H, P.halfH);
+                }
+                if (dx * dx + dy * dy <= rl2) {
+                    const int32_t d = slot_offset(s, j, na);
+                    if (d > 32767 || d < -32768) atomicOr(status, kStatusOverflow);
+                    if (cnt < kNlMax) {
+                        s_id[cnt][lane] = ID[j];
+                        s_slot[cnt][lane] = j;
+                    }
+                    cnt++;
+                }
+            }
+        }
+    }
+    if (cnt > kNlMax) {
+        atomicOr(status, kStatusOverflow);
+        cnt = kNlMax;
+    }
+    for (int a = 0; a < cnt; a++) {
+        const int32_t ida = s_id[a][lane];
+        int rank = 0;
+        for (int b2 = 0; b2 < cnt; b2++) rank += (s_id[b2][lane] < ida) ? 1 : 0;
+        s_out[rank][lane] = s_slot[a][lane];
+    }
+    const int groups = (cnt + 7) >> 3;
+    for (int r = cnt; r < 8 * groups; r++) s_out[r][lane] = s;
+    for (int g = 0; g < groups; g++) {
+        const int b = 8 * g;
+        nbr[g * na + s] = make_int4(pack2(s, s_out[b][lane], s_out[b + 1][lane], na),
+                                    pack2(s, s_out[b + 2][lane], s_out[b + 3][lane], na),
+                                    pack2(s, s_out[b + 4][lane], s_out[b + 5][lane], na),
+                                    pack2(s, s_out[b + 6][lane], s_out[b + 7][lane], na));
+    }
+    bool near_edge = false;
+    if (P.periodic) {
+        const double ex = fmin(pi.x - P.blx, (P.blx + P.L) - pi.x);
+        const double ey = fmin(pi.y - P.bly, (P.bly + P.H) - pi.y);
+        near_edge = !(ex >= edge && ey >= edge);
+    }
+    ncount[s] = (int32_t)((uint32_t)cnt | (near_edge ? kEdgeBit : 0u));
+}
+
+// After a build: the reference chunk of every particle (K_n), chunk counts
+// with the capacity check (one atomic per run of equal chunks in a wave),
+// everyone recomputes its first evaluation, the build positions.
+__global__ void k_nl_keys(SysParams P, const double2 *__restrict__ pos, int32_t *__restrict__ key,
+                          int32_t *__restrict__ ccnt, uint8_t *__restrict__ need, uint32_t *__restrict__ chg,
+                          double2 *__restrict__ xb, uint32_t *__restrict__ flags) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = s < P.na;
+    uint32_t flag = 0;
+    int32_t r = 0, c = 0, k = -1;
+    double2 p = make_double2(0.0, 0.0);
+    if (valid) {
+        p = pos[s];
+        cell_rc(P, P.rows, P.cols, P.cl, P.ch, p.x, p.y, &flag, MAVI_FLAG_OUT_OF_BOX, &r, &c);
+        k = r * P.cols + c;
+    }
+    const int32_t rank = wave_arrival_rank(valid, k, ccnt);
+    if (valid && rank + 1 > P.nc) flag |= MAVI_FLAG_OVERFLOW;
+    if (flag) atomicOr(flags, flag);
+    if (!valid) return;
+    key[s] = pack_rc(r, c);
+    need[s] = 1;
+    chg[s] = 0u;
+    xb[s] = p;
+}
+
+// The drift of one particle (update_verlet!, src/integration.jl:424), the
+// reference chunk of its final position (after walls!'s periodic wrap) and,
+// if the chunk changes, the marks of the particle (2) and of its list
+// partners (1), whose second evaluation tests chunk adjacency, and the chunk
+// counts.
+__device__ __forceinline__ void nl_drift(const SysParams &P, const NLArgs &A, int s, double2 pi, double fx,
+                                         double fy) {
+    const double2 v = A.V[s];
+    const double px = pi.x + (v.x * P.dt + fx * P.term), py = pi.y + (v.y * P.dt + fy * P.term);
+    A.Px[s] = make_double2(px, py);
+    double wx = px, wy = py;
+    if (P.periodic) periodic_wrap(P, wx, wy);  // walls! moves it there after the kick
+    uint32_t flag = 0;
+    int32_t r, c;
+    cell_rc(P, P.rows, P.cols, P.cl, P.ch, wx, wy, &flag, MAVI_FLAG_OUT_OF_BOX, &r, &c);
+    if (flag) atomicOr(A.flags, flag);
+    const int32_t kn = pack_rc(r, c);
+    A.knext[s] = kn;
+    const int32_t ko = A.kcur[s];
+    if (kn != ko) {
+        atomicOr(&A.chg[s], 2u);
+        const int groups = (int)(((uint32_t)A.ncount[s] & ~kEdgeBit) + 7u) >> 3;
+        for (int g = 0; g < groups; g++) {
+            const int4 q = A.nbr[g * P.na + s];
+            const int32_t w[4] = {q.x, q.y, q.z, q.w};
+            for (int e = 0; e < 4; e++) {
+                atomicOr(&A.chg[lo_slot(s, w[e], P.na)], 1u);
+                atomicOr(&A.chg[hi_slot(s, w[e], P.na)], 1u);
+            }
+        }
+        atomicSub(&A.ccnt[(ko >> 16) * P.cols + (ko & 0xffff)], 1);
+        atomicAdd(&A.ccnt[r * P.cols + c], 1);
+    }
+}
+
+// First evaluation of newton_step! (forces at x_n over every pair within
+// range) where the pair set may have changed, the previous second evaluation
+// elsewhere; then the drift.  The particles to recompute are compacted to the
+// first lanes of their workgroup, so waves without one only drift.
+template <int POT>
+__global__ __launch_bounds__(kNeedBlock) void k_nl_first(SysParams P, NLArgs A) {
+    __shared__ int32_t s_redo[kNeedBlock];
+    __shared__ int32_t s_nredo;
+    const int s = xcd_block() * blockDim.x + threadIdx.x;
+    const bool valid = s < P.na;
+    const bool need = valid && (!A.reuse || A.need[s]);
+    if (valid && !need) {
+        const double2 f = A.F[s];
+        nl_drift(P, A, s, A.Pst[s], f.x, f.y);
+    }
+    if (threadIdx.x == 0) s_nredo = 0;
+    __syncthreads();
+    const unsigned long long m = __ballot(need);
+    if (m) {
+        const int leader = __ffsll(m) - 1;
+        int base = 0;
+        if ((int)lane_id() == leader) base = atomicAdd(&s_nredo, __popcll(m));
+        base = __shfl(base, leader, 64);
+        if (need) s_redo[base + __popcll(m & ((1ull << lane_id()) - 1ull))] = s;
+    }
+    __syncthreads();
+    const int nredo = s_nredo;
+    if ((int)(threadIdx.x & ~63u) >= nredo) return;  // whole wave
+    const bool act = (int)threadIdx.x < nredo;
+    const int t = s_redo[act ? threadIdx.x : 0];
+    const double2 pi = A.Pst[t];
+    const uint32_t mc = act ? (uint32_t)A.ncount[t] : 0u;
+    const int groups = (int)((mc & ~kEdgeBit) + 7u) >> 3;
+    PairAcc<POT> acc;
+    if (__any((mc & kEdgeBit) != 0u))
+        nl_sum<POT, true, false>(P, A.Pst, A.nbr, t, groups, pi.x, pi.y, false, 0, 0, nullptr, acc);
+    else
+        nl_sum<POT, false, false>(P, A.Pst, A.nbr, t, groups, pi.x, pi.y, false, 0, 0, nullptr, acc);
+    if (!act) return;
+    A.F[t] = make_double2(acc.fx, acc.fy);
+    nl_drift(P, A, t, pi, acc.fx, acc.fy);
+}
+
+// Second evaluation at the drifted positions with the step-start chunks,
+// then the kick and walls! (src/integration.jl:425-431, 271-285, 309-324).
+// Records whether the next first evaluation must recompute, the capacity
+// check of a chunk change, and the largest displacement since the build.
+template <int POT>
+__global__ __launch_bounds__(kNeedBlock) void k_nl_second(SysParams P, NLArgs A) {
+    const int s = xcd_block() * blockDim.x + threadIdx.x;
+    const bool valid = s < P.na;
+    const int sv = valid ? s : 0;
+    const double2 pi = A.Px[sv];
+    const uint32_t c = valid ? A.chg[s] : 0u;
+    const uint32_t m = valid ? (uint32_t)A.ncount[s] : 0u;
+    const int groups = (int)((m & ~kEdgeBit) + 7u) >> 3;
+    const bool stale = c != 0u;
+    const bool wrap = __any((m & kEdgeBit) != 0u);
+    PairAcc<POT> acc;
+    if (__any(stale)) {
+        const int32_t ki = A.kcur[sv];
+        const int ri = ki >> 16, ci = ki & 0xffff;
+        if (wrap) nl_sum<POT, true, true>(P, A.Px, A.nbr, sv, groups, pi.x, pi.y, stale, ri, ci, A.kcur, acc);
+        else nl_sum<POT, false, true>(P, A.Px, A.nbr, sv, groups, pi.x, pi.y, stale, ri, ci, A.kcur, acc);
+    } else {
+        if (wrap) nl_sum<POT, true, false>(P, A.Px, A.nbr, sv, groups, pi.x, pi.y, false, 0, 0, nullptr, acc);
+        else nl_sum<POT, false, false>(P, A.Px, A.nbr, sv, groups, pi.x, pi.y, false, 0, 0, nullptr, acc);
+    }
+    double d2 = 0.0;
+    if (valid) {
+        const double fx = acc.fx, fy = acc.fy;
+        const double2 v = A.V[s];
+        const double2 fo = A.F[s];
+        // vel += dt/2 * (forces + old_forces) (src/integration.jl:430), then walls!
+        double vx = v.x + P.hdt * (fx + fo.x);
+        double vy = v.y + P.hdt * (fy + fo.y);
+        double x = pi.x, y = pi.y;
+        if (P.periodic) {
+            periodic_wrap(P, x, y);
+        } else {
+            // src/integration.jl:271-285
+            const double rx = x - P.blx, ry = y - P.bly;
+            const bool ox = ((rx + P.radius) > P.L) || ((rx - P.radius) < 0.0);
+            const bool oy = ((ry + P.radius) > P.H) || ((ry - P.radius) < 0.0);
+            if (ox) vx = -vx;
+            if (oy) vy = -vy;
+        }
+        A.Pst[s] = make_double2(x, y);
+        A.V[s] = make_double2(vx, vy);
+        A.F[s] = make_double2(fx, fy);
+        A.need[s] = stale ? 1 : 0;
+        A.chg[s] = 0u;
+        if (c & 2u) {
+            const int32_t kn = A.knext[s];
+            if (A.ccnt[(kn >> 16) * P.cols + (kn & 0xffff)] > P.nc) atomicOr(A.flags, MAVI_FLAG_OVERFLOW);
+        }
+        const double2 b = A.xb[s];
+        double ddx = x - b.x, ddy = y - b.y;
+        if (P.periodic) {
+            ddx = min_image(ddx, P.L, P.halfL);
+            ddy = min_image(ddy, P.H, P.halfH);
+        }
+        d2 = ddx * ddx + ddy * ddy;
+    }
+    // the workgroup's largest displacement since the build, kept per
+    // workgroup across steps (no contended atomic on one word)
+    __shared__ double s_max[kNeedBlock / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) d2 = fmax(d2, __shfl_xor(d2, o, 64));
+    if (lane_id() == 0) s_max[threadIdx.x >> 6] = d2;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mx = s_max[0];
+        for (int w = 1; w < kNeedBlock / 64; w++) mx = fmax(mx, s_max[w]);
+        if (mx > A.bmax[blockIdx.x]) A.bmax[blockIdx.x] = mx;
+    }
+}
+
+// dmax <- max of the per-workgroup maxima (one workgroup of 1024 threads).
+__global__ __launch_bounds__(1024) void k_nl_reduce(const double *__restrict__ bmax, int32_t n,
+                                                    unsigned long long *__restrict__ dmax) {
+    __shared__ double s_max[16];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < n; i += 1024) m = fmax(m, bmax[i]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if (lane_id() == 0) s_max[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 16; w++) m = fmax(m, s_max[w]);
+        m = fmax(m, s_max[0]);
+        dmax[0] = (unsigned long long)__double_as_longlong(m);
+    }
+}
+
+NLArgs nl_args(mavi_sys *s) {
+    NList &N = *s->nl;
+    NLArgs A{};
+    A.nbr = (const int4 *)N.nbr;
+    A.ncount = N.ncount;
+    A.kcur = N.key[N.kc];
+    A.knext = N.key[1 - N.kc];
+    A.ccnt = N.ccnt;
+    A.chg = N.chg;
+    A.need = N.need;
+    A.xb = N.xb;
+    A.bmax = N.bmax;
+    A.Pst = s->P_[s->cur];
+    A.V = s->V_[s->cur];
+    A.F = s->F;
+    A.Px = s->P2;
+    A.flags = s->flags;
+    A.reuse = s->reuse ? 1 : 0;
+    return A;
+}
+
+double nl_range(const mavi_sys *s) {
+    const SysParams &P = s->P;
+    if (P.potential == MAVI_POT_LJ && P.cut2 < INFINITY && P.p2 > 0.0) return P.p2;
+    if (P.potential == MAVI_POT_HARMTRUNC) return P.p3;
+    return 0.0;
+}
+
+}  // namespace
+
+namespace mavi {
+
+bool nl_eligible(const mavi_sys *s) {
+    const SysParams &P = s->P;
+    const double range = nl_range(s);
+    if (!(range > 0.0) || P.nwalls > 0 || s->var || P.na <= 0) return false;
+    if (P.cols < 3 || P.rows < 3) return false;
+    // every pair within range is in adjacent chunks at the binned positions
+    return P.cl >= range * (1.0 + 1e-12) && P.ch >= range * (1.0 + 1e-12);
+}
+
+void nl_set_skin(mavi_sys *s, double skin_frac) {
+    NList &N = *s->nl;
+    const SysParams &P = s->P;
+    N.skin_frac = skin_frac;
+    const double skin = skin_frac * N.range;
+    const double rl = (N.range + skin) * (1.0 + 1e-9);
+    N.rl2 = rl * rl;
+    const double half = 0.5 * skin * (1.0 - 1e-9);
+    N.thr2 = half * half;
+    N.edge = rl + 2.0 * skin;
+    // list grid: cells at least the list radius wide (the reference grid when
+    // its chunks already are)
+    SysParams G = P;
+    G.cols = P.cl >= rl ? P.cols : (int32_t)floor(P.L / rl);
+    G.rows = P.ch >= rl ? P.rows : (int32_t)floor(P.H / rl);
+    if (G.cols < 1) G.cols = 1;
+    if (G.rows < 1) G.rows = 1;
+    G.cl = P.L / (double)G.cols;
+    G.ch = P.H / (double)G.rows;
+    G.ncell = G.cols * G.rows;
+    G.nc = P.na;  // no capacity check on the list grid
+    N.G = G;
+    N.valid = false;
+}
+
+hipError_t nl_alloc(mavi_sys *s, hipStream_t st) {
+    s->nl = new NList();
+    NList &N = *s->nl;
+    N.range = nl_range(s);
+    nl_set_skin(s, N.skin_frac);
+    const int64_t na = s->P.na;
+    N.kmax = kNlMax;
+    hipError_t e = dalloc(&N.nbr, (int64_t)(N.kmax / 2) * na, st);  // 16-bit entries
+    if (e == hipSuccess) e = dalloc(&N.ncount, na, st);
+    if (e == hipSuccess) e = dalloc(&N.key[0], na, st);
+    if (e == hipSuccess) e = dalloc(&N.key[1], na, st);
+    if (e == hipSuccess) e = dalloc(&N.ccnt, (int64_t)s->P.ncell, st);
+    if (e == hipSuccess) e = dalloc(&N.chg, na, st);
+    if (e == hipSuccess) e = dalloc(&N.need, na, st);
+    if (e == hipSuccess) e = dalloc(&N.xb, na, st);
+    if (e == hipSuccess) e = dalloc(&N.psnap, na, st);
+    if (e == hipSuccess) e = dalloc(&N.vsnap, na, st);
+    if (e == hipSuccess) e = dalloc(&N.fsnap, 1, st);
+    if (e == hipSuccess) e = dalloc(&N.dmax, 1, st);
+    N.nbmax = nblk(s->P.na, kNeedBlock);  // workgroups of the second evaluation
+    if (e == hipSuccess) e = dalloc(&N.bmax, N.nbmax, st);
+    if (e == hipSuccess) e = dalloc(&N.status, 1, st);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&N.host, 2 * sizeof(unsigned long long), hipHostMallocDefault);
+    // scan states of the list grid (no more cells than the chunk grid)
+    N.scan_words = (s->P.ncell + 1 + kScanTileMin - 1) / kScanTileMin + 1;
+    if (e == hipSuccess) e = dalloc(&N.scan_ticket, 1, st);
+    if (e == hipSuccess) e = dalloc(&N.scan_state, N.scan_words, st);
+    return e;
+}
+
+void nl_free(mavi_sys *s) {
+    if (!s->nl) return;
+    NList &N = *s->nl;
+    for (auto &g : N.graph)
+        if (g) (void)hipGraphExecDestroy(g);
+    dfree(N.nbr); dfree(N.ncount); dfree(N.key[0]); dfree(N.key[1]); dfree(N.ccnt);
+    dfree(N.chg); dfree(N.need); dfree(N.xb); dfree(N.psnap); dfree(N.vsnap); dfree(N.fsnap);
+    dfree(N.dmax); dfree(N.bmax); dfree(N.status); dfree(N.scan_ticket); dfree(N.scan_state);
+    if (N.host) (void)hipHostFree(N.host);
+    delete s->nl;
+    s->nl = nullptr;
+}
+
+hipError_t nl_rebuild(mavi_sys *s, hipStream_t st) {
+    NList &N = *s->nl;
+    hipError_t e = drop_keys(s, st);  // counts a fused first pass left on the chunk grid
+    // the list grid's scan starts from a cleared ticket and states (its tile
+    // count changes with the skin)
+    if (e == hipSuccess) e = launch_fill_i32((int32_t *)N.scan_ticket, 0, 2, st);
+    if (e == hipSuccess) e = launch_fill_i32((int32_t *)N.scan_state, 0, 2 * (int64_t)N.scan_words, st);
+    if (e == hipSuccess) e = launch_bin(s, st, false, false, false, &N.G, N.scan_ticket, N.scan_state);
+    if (e != hipSuccess) return e;
+    s->sorted = false;   // not in reference chunk order: the chunk gather's stamps start over
+    s->f_valid = false;
+    const SysParams &P = s->P;
+    const int c = s->cur;
+    {
+        ProfScope ps(s, MAVI_PROF_BIN, st);
+        hipLaunchKernelGGL(k_nl_build, dim3(nblk(P.na, 64)), dim3(64), 0, st, N.G, P, s->P_[c], s->ID_[c], s->scell,
+                           s->start, (int4 *)N.nbr, N.ncount, N.rl2, N.edge, N.status);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = launch_fill_i32(N.ccnt, 0, P.ncell, st)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_nl_keys, dim3(nblk(P.na, 256)), dim3(256), 0, st, P, s->P_[c], N.key[0], N.ccnt,
+                           N.need, N.chg, N.xb, s->flags);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = launch_fill_i32((int32_t *)N.dmax, 0, 2, st)) != hipSuccess) return e;
+        if ((e = launch_fill_i32((int32_t *)N.bmax, 0, 2 * (int64_t)N.nbmax, st)) != hipSuccess) return e;
+    }
+    N.kc = 0;
+    N.valid = true;
+    N.d_last = 0.0;
+    N.rebuilds++;
+    return hipSuccess;
+}
+
+hipError_t nl_reduce_dmax(mavi_sys *s, hipStream_t st) {
+    NList &N = *s->nl;
+    hipLaunchKernelGGL(k_nl_reduce, dim3(1), dim3(1024), 0, st, N.bmax, N.nbmax, N.dmax);
+    return hipGetLastError();
+}
+
+hipError_t nl_step(mavi_sys *s, hipStream_t st) {
+    NList &N = *s->nl;
+    const SysParams &P = s->P;
+    const NLArgs A = nl_args(s);
+    const int nb = nblk(P.na, kNeedBlock);
+    {
+        ProfScope ps(s, MAVI_PROF_FORCE_REUSE, st);
+        if (P.potential == MAVI_POT_LJ) hipLaunchKernelGGL(k_nl_first<POT_LJ>, dim3(nb), dim3(kNeedBlock), 0, st, P, A);
+        else hipLaunchKernelGGL(k_nl_first<POT_HT>, dim3(nb), dim3(kNeedBlock), 0, st, P, A);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    {
+        ProfScope ps(s, MAVI_PROF_FORCE, st);
+        if (P.potential == MAVI_POT_LJ) hipLaunchKernelGGL(k_nl_second<POT_LJ>, dim3(nb), dim3(kNeedBlock), 0, st, P, A);
+        else hipLaunchKernelGGL(k_nl_second<POT_HT>, dim3(nb), dim3(kNeedBlock), 0, st, P, A);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    N.kc ^= 1;
+    return hipSuccess;
+}
+
+}  // namespace mavi

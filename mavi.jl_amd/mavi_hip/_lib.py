@@ -166,6 +166,7 @@ SIGNATURES = {
     "mavi_run_record": (C.c_int, [_P, C.c_int64, _DP, _DP]),
     "mavi_prof_enable": (C.c_int, [_P, C.c_int]),
     "mavi_prof_get": (C.c_int, [_P, C.c_int, _DP, _I64P]),
+    "mavi_nlist_info": (C.c_int, [_P, _I64P]),
     "mavi_sys_set_tile": (C.c_int, [_P, C.c_int32]),
     "mavi_sys_set_option": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "mavi_autotune": (C.c_int, [_P, C.c_int32, _DP, C.POINTER(C.c_int32)]),
@@ -175,6 +176,9 @@ MAVI_OPT_TWO_PHASE = 0
 MAVI_OPT_GRAPHS = 1
 MAVI_OPT_LDS_TILES = 2
 MAVI_OPT_REUSE_FORCES = 3
+MAVI_OPT_NLIST = 4
+MAVI_OPT_NLIST_SKIN = 5
+MAVI_OPT_NLIST_BLOCK = 6
 
 # Multi-step calls replay captured hipGraphs unless this is set to False
 # before systems are created (kernel traces are then plain launches).
@@ -415,6 +419,12 @@ class DeviceSystem:
 
     def prof_enable(self, on: bool = True):
         self._call("mavi_prof_enable", 1 if on else 0)
+
+    def nlist_info(self) -> dict:
+        """Neighbour lists: eligible, running, rebuilds, rolled-back blocks."""
+        a = np.zeros(4, dtype=np.int64)
+        self._call("mavi_nlist_info", a.ctypes.data_as(_I64P))
+        return {"eligible": bool(a[0]), "active": bool(a[1]), "rebuilds": int(a[2]), "rollbacks": int(a[3])}
 
     def prof_get(self, which: int):
         ms = C.c_double()

@@ -18,7 +18,7 @@ CSRC = PKG_DIR.parent / "csrc"
 INCLUDE = PKG_DIR.parent.parent / "include"
 OBJ_DIR = PKG_DIR.parent / "_build"
 
-SOURCES = ["mavi_kernels.hip", "mavi_capi.hip", "mavi_batch.hip", "mavi_mlp.hip", "mavi_learned.hip"]
+SOURCES = ["mavi_kernels.hip", "mavi_capi.hip", "mavi_batch.hip", "mavi_mlp.hip", "mavi_learned.hip", "mavi_nlist.hip"]
 
 HIPCC_FLAGS = [
     "-O3",

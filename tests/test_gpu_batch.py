@@ -41,10 +41,15 @@ def single_run(kind, case, seed, steps_list):
     import mavi_hip as mv
     from mavi_hip import rings
 
+    from mavi_hip import _lib
+
     out = []
     if kind == "plain":
         pos, vel, space, dyn, ic = case
         s = mv.System(state=mv.SecondLawState(pos=pos, vel=vel), space_cfg=space, dynamic_cfg=dyn, int_cfg=ic)
+        # replicas run the chunk gather; a system alone would take neighbour
+        # lists (same pairs, another summation order)
+        s.dev.set_option(_lib.MAVI_OPT_NLIST, 0)
         for k in steps_list:
             s.advance(k)
             s.download()

@@ -29,6 +29,17 @@ def lj_system(n_side=24, seed=0):
     return mv.System(state=mv.SecondLawState(pos=pos, vel=vel), space_cfg=space, dynamic_cfg=dyn, int_cfg=ic)
 
 
+def lj_system_gather():
+    """lj_system on the chunk gather, the path of device batch replicas (a
+    system alone takes neighbour lists: the same pairs summed in another
+    order), so that batch and solo runs compare bit for bit."""
+    from mavi_hip import _lib
+
+    s = lj_system()
+    s.dev.set_option(_lib.MAVI_OPT_NLIST, 0)
+    return s
+
+
 def rings_system(seed=5):
     from mavi_hip import rings
     from test_gpu_rings import normal_case
@@ -141,7 +152,7 @@ def test_experiment_resumes_from_checkpoint(gpu_ctx, tmp_path):
     assert np.array_equal(da["pos"], db["pos"])
 
 
-@pytest.mark.parametrize("make", [lj_system, rings_system], ids=["lj", "rings"])
+@pytest.mark.parametrize("make", [lj_system_gather, rings_system], ids=["lj", "rings"])
 def test_experiment_batch_device_equals_solo(gpu_ctx, tmp_path, make):
     from mavi_hip import experiments as ex
 
