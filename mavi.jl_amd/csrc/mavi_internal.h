@@ -175,7 +175,7 @@ struct NList {
     int32_t block = 32;       // steps between host checks (option)
     int32_t block_now = 32;   // current block (halved after a rollback)
     SysParams G{};            // internal grid for sorting and building (cells >= range + skin)
-    int32_t *nbr = nullptr;   // (kmax / 8) x na groups of 8 16-bit partner offsets (int4), ascending partner id, padded with the particle
+    int32_t *nbr = nullptr;   // (kmax / 4) x na groups of 4 16-bit partner offsets (int2), ascending partner id, padded with the particle
     int32_t *ncount = nullptr;// entries per slot; bit 31: minimum image needed
     int32_t *key[2] = {nullptr, nullptr};  // reference chunk (row << 16 | col) of each slot (ping-pong: K_n, K_{n+1})
     int kc = 0;               // key[kc] holds K_n

@@ -43,11 +43,11 @@ namespace {
 constexpr uint32_t kEdgeBit = 0x80000000u;  // ncount: minimum image needed
 constexpr uint32_t kStatusOverflow = 1u;    // a list held more than kmax entries, or a partner more
                                             // than 32767 slots away (modulo na)
-constexpr int kNlMax = 48;                  // list entries per particle (6 groups of 8)
+constexpr int kNlMax = 48;                  // list entries per particle (12 groups of 4)
 constexpr int kNeedBlock = 256;             // workgroup of the evaluation kernels
 
 struct NLArgs {
-    const int4 *nbr;      // groups of 8 partner offsets ((slot - s) mod na in [-32768, 32767]): nbr[g * na + s]
+    const int2 *nbr;      // groups of 4 partner offsets ((slot - s) mod na in [-32768, 32767]): nbr[g * na + s]
     const int32_t *ncount;
     const int32_t *kcur;  // K_n: reference chunk (row << 16 | col) of the step's start positions
     int32_t *knext;       // K_{n+1}
@@ -117,112 +117,247 @@ __device__ __forceinline__ int32_t slot_offset(int s, int j, int na) {
 }
 
 // The sum over one particle's list in list order (ascending partner id),
-// eight entries per iteration; the next group is loaded one iteration ahead,
+// four entries per iteration; the next group is loaded one iteration ahead,
 // so the list and position loads of consecutive groups overlap.
 template <int POT, bool WRAP, bool STALE>
 __device__ __forceinline__ void nl_sum(const SysParams &P, const double2 *__restrict__ pin,
-                                       const int4 *__restrict__ nbr, int s, int groups, double xi, double yi,
+                                       const int2 *__restrict__ nbr, int s, int groups, double xi, double yi,
                                        bool test, int ri, int ci, const int32_t *__restrict__ kcur,
                                        PairAcc<POT> &acc) {
     if (groups <= 0) return;
     const int na = P.na;
-    int4 q = nbr[s];
+    int2 q = nbr[s];
     for (int g = 0; g < groups; g++) {
-        const int4 c = q;
+        const int2 c = q;
         if (g + 1 < groups) q = nbr[(g + 1) * na + s];
-        const int j[8] = {lo_slot(s, c.x, na), hi_slot(s, c.x, na), lo_slot(s, c.y, na), hi_slot(s, c.y, na),
-                          lo_slot(s, c.z, na), hi_slot(s, c.z, na), lo_slot(s, c.w, na), hi_slot(s, c.w, na)};
-        double2 p[8];
+        const int j[4] = {lo_slot(s, c.x, na), hi_slot(s, c.x, na), lo_slot(s, c.y, na), hi_slot(s, c.y, na)};
+        double2 p[4];
 #pragma unroll
-        for (int e = 0; e < 8; e++) p[e] = load_slot(pin, j[e]);
+        for (int e = 0; e < 4; e++) p[e] = load_slot(pin, j[e]);
 #pragma unroll
-        for (int e = 0; e < 8; e++) nl_pair<POT, WRAP, STALE>(P, p[e], j[e], s, xi, yi, test, ri, ci, kcur, acc);
+        for (int e = 0; e < 4; e++) nl_pair<POT, WRAP, STALE>(P, p[e], j[e], s, xi, yi, test, ri, ci, kcur, acc);
     }
 }
 
-__device__ __forceinline__ int32_t pack2(int s, int32_t a, int32_t b, int na) {
-    return (int32_t)(((uint32_t)slot_offset(s, a, na) & 0xffffu) | ((uint32_t)slot_offset(s, b, na) << 16));
+// The columns [ca, cb] of one grid row, as one or two runs of consecutive
+// cells (a periodic grid's stencil wraps at the edges); every column at most
+// once, also on grids of fewer than 3 columns.
+struct ColRuns {
+    int n, a0, b0, a1, b1;
+};
+__device__ __forceinline__ ColRuns stencil_cols(const SysParams &G, int col) {
+    ColRuns R{1, 0, 0, 0, 0};
+    if (!G.periodic) {
+        R.a0 = max(col - 1, 0);
+        R.b0 = min(col + 1, G.cols - 1);
+    } else if (G.cols <= 3) {
+        R.b0 = G.cols - 1;
+    } else if (col == 0) {
+        R = ColRuns{2, 0, 1, G.cols - 1, G.cols - 1};
+    } else if (col == G.cols - 1) {
+        R = ColRuns{2, col - 1, col, 0, 0};
+    } else {
+        R.a0 = col - 1;
+        R.b0 = col + 1;
+    }
+    return R;
 }
 
-// Build, one wave per workgroup: the state is sorted on the list grid G
-// (cells >= range + skin).  Each lane scans the 3x3 cells around its
-// particle's cell and stages the others within the list radius (slot and id)
-// in LDS, ranks them by id and writes them as groups of 8 16-bit offsets in
-// ascending id, the last group padded with the particle itself.  A particle within `edge`
-// of a periodic edge sums with the minimum image (its partners may wrap
-// before the next build).
-__global__ __launch_bounds__(64) void k_nl_build(SysParams G, SysParams P, const double2 *__restrict__ pos,
-                                                 const int32_t *__restrict__ ID, const int32_t *__restrict__ scell,
-                                                 const int32_t *__restrict__ start, int4 *__restrict__ nbr,
-                                                 int32_t *__restrict__ ncount, double rl2, double edge,
-                                                 uint32_t *__restrict__ status) {
-    __shared__ int32_t s_id[kNlMax][64];
-    __shared__ int32_t s_slot[kNlMax][64];
-    __shared__ int32_t s_out[kNlMax][64];
-    const int lane = threadIdx.x;
-    const int s = blockIdx.x * 64 + lane;
-    if (s >= P.na) return;
-    const int na = P.na;
-    const double2 pi = pos[s];
-    const int c = scell[s];
-    const int row = c / G.cols, col = c - row * G.cols;
-    int cnt = 0;
-    for (int dr = -1; dr <= 1; dr++) {
-        int r = row + dr;
-        if (G.periodic) r = r < 0 ? r + G.rows : (r >= G.rows ? r - G.rows : r);
-        else if (r < 0 || r >= G.rows) continue;
-        for (int dc = -1; dc <= 1; dc++) {
-            int cc = col + dc;
-            if (G.periodic) cc = cc < 0 ? cc + G.cols : (cc >= G.cols ? cc - G.cols : cc);
-            else if (cc < 0 || cc >= G.cols) continue;
-            const int cell = r * G.cols + cc;
-            const int b = start[cell], e = start[cell + 1];
-            for (int j = b; j < e; j++) {
-                if (j == s) continue;
-                const double2 pj = pos[j];
-                double dx = pi.x - pj.x, dy = pi.y - pj.y;
-                if (P.periodic) {
-                    dx = min_image(dx, P.L, P.halfL);
-                    dy = min_image(dy, P.H, P.halfH);
-                }
-                if (dx * dx + dy * dy <= rl2) {
-                    const int32_t d = slot_offset(s, j, na);
-                    if (d > 32767 || d < -32768) atomicOr(status, kStatusOverflow);
-                    if (cnt < kNlMax) {
-                        s_id[cnt][lane] = ID[j];
-                        s_slot[cnt][lane] = j;
-                    }
-                    cnt++;
+// Rows of the 3x3 stencil around grid row `row`, each at most once (-1:
+// outside a walled box).
+__device__ __forceinline__ void stencil_rows(const SysParams &G, int row, int rr[3]) {
+    for (int d = 0; d < 3; d++) {
+        int r;
+        if (G.periodic && G.rows <= 3) {
+            r = d < G.rows ? d : -1;
+        } else {
+            r = row + d - 1;
+            if (G.periodic) r = r < 0 ? r + G.rows : (r >= G.rows ? r - G.rows : r);
+            else if (r < 0 || r >= G.rows) r = -1;
+        }
+        rr[d] = r;
+    }
+}
+
+constexpr int kBuildCand = 128;  // stencil candidates one build wave holds (2 per lane)
+
+// Bitonic sort, ascending, of the wave's 64 * H keys, element e = H * lane + h.
+template <int H>
+__device__ __forceinline__ void wave_bitonic(uint32_t (&key)[H]) {
+    const int lane = (int)lane_id();
+#pragma unroll
+    for (int k = 2; k <= 64 * H; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (H == 2 && j == 1) {
+                // partners in one lane
+                const int e = 2 * lane;
+                const bool asc = (e & k) == 0;
+                const uint32_t lo = min(key[0], key[1]), hi = max(key[0], key[1]);
+                key[0] = asc ? lo : hi;
+                key[1] = asc ? hi : lo;
+            } else {
+                const int lj = H == 2 ? j >> 1 : j;
+#pragma unroll
+                for (int h = 0; h < H; h++) {
+                    const uint32_t other = (uint32_t)__shfl_xor((int)key[h], lj, 64);
+                    const int e = H * lane + h;
+                    const bool keep_min = ((e & j) == 0) == ((e & k) == 0);
+                    key[h] = keep_min ? min(key[h], other) : max(key[h], other);
                 }
             }
         }
     }
-    if (cnt > kNlMax) {
-        atomicOr(status, kStatusOverflow);
-        cnt = kNlMax;
+}
+
+// One cell's lists.  The wave's candidates (the cell's 3x3 stencil, at most
+// kBuildCand, staged in LDS by candidate index) are sorted by id; then, for
+// each particle of the cell, every lane tests its candidates against the list
+// radius and the passing ones, ranked by ballot in id order, are written as
+// 16-bit offsets in groups of 4, the last group padded with offset 0 (the
+// particle itself).
+template <int H>
+__device__ __forceinline__ void build_cell(const SysParams &P, const double2 *__restrict__ pos,
+                                           const int32_t *__restrict__ ID, int t0, int t1, int M,
+                                           const int32_t *w_slot, const double2 *w_pos, int16_t *__restrict__ nbr16, int32_t *__restrict__ ncount, double rl2,
+                                           double edge, uint32_t *__restrict__ status) {
+    const int lane = (int)lane_id();
+    const int na = P.na;
+    uint32_t key[H];
+#pragma unroll
+    for (int h = 0; h < H; h++) {
+        const int e = H * lane + h;
+        key[h] = 0xffffffffu;
+        if (e < M) key[h] = ((uint32_t)ID[w_slot[e]] << 7) | (uint32_t)e;
     }
-    for (int a = 0; a < cnt; a++) {
-        const int32_t ida = s_id[a][lane];
-        int rank = 0;
-        for (int b2 = 0; b2 < cnt; b2++) rank += (s_id[b2][lane] < ida) ? 1 : 0;
-        s_out[rank][lane] = s_slot[a][lane];
+    wave_bitonic<H>(key);
+    int js[H];
+    double2 pj[H];
+    bool ok[H];
+#pragma unroll
+    for (int h = 0; h < H; h++) {
+        ok[h] = key[h] != 0xffffffffu;
+        const int e = ok[h] ? (int)(key[h] & 127u) : 0;
+        js[h] = w_slot[e];
+        pj[h] = w_pos[e];
     }
-    const int groups = (cnt + 7) >> 3;
-    for (int r = cnt; r < 8 * groups; r++) s_out[r][lane] = s;
-    for (int g = 0; g < groups; g++) {
-        const int b = 8 * g;
-        nbr[g * na + s] = make_int4(pack2(s, s_out[b][lane], s_out[b + 1][lane], na),
-                                    pack2(s, s_out[b + 2][lane], s_out[b + 3][lane], na),
-                                    pack2(s, s_out[b + 4][lane], s_out[b + 5][lane], na),
-                                    pack2(s, s_out[b + 6][lane], s_out[b + 7][lane], na));
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    for (int i = t0; i < t1; i++) {
+        const double2 pi = pos[i];
+        bool pass[H];
+        unsigned long long m[H];
+#pragma unroll
+        for (int h = 0; h < H; h++) {
+            double dx = pi.x - pj[h].x, dy = pi.y - pj[h].y;
+            if (P.periodic) {
+                dx = min_image(dx, P.L, P.halfL);
+                dy = min_image(dy, P.H, P.halfH);
+            }
+            pass[h] = ok[h] && js[h] != i && dx * dx + dy * dy <= rl2;
+            m[h] = __ballot(pass[h]);
+        }
+        int cnt = 0, rank[H];
+        if (H == 1) {
+            rank[0] = __popcll(m[0] & lt);
+            cnt = __popcll(m[0]);
+        } else {
+            rank[0] = __popcll(m[0] & lt) + __popcll(m[H - 1] & lt);
+            rank[H - 1] = rank[0] + (pass[0] ? 1 : 0);
+            cnt = __popcll(m[0]) + __popcll(m[H - 1]);
+        }
+        bool far = false;
+#pragma unroll
+        for (int h = 0; h < H; h++) {
+            if (pass[h] && rank[h] < kNlMax) {
+                const int32_t d = slot_offset(i, js[h], na);
+                far = far || d > 32767 || d < -32768;
+                nbr16[((int64_t)(rank[h] >> 2) * na + i) * 4 + (rank[h] & 3)] = (int16_t)d;
+            }
+        }
+        const bool over = cnt > kNlMax;
+        if (over) cnt = kNlMax;
+        // pads of the last group: offset 0, the particle itself
+        const int pad = (((cnt + 3) >> 2) << 2) - cnt;
+        if (lane < pad) nbr16[((int64_t)((cnt + lane) >> 2) * na + i) * 4 + ((cnt + lane) & 3)] = 0;
+        if (__any(far) || over) {
+            if (lane == 0) atomicOr(status, kStatusOverflow);
+        }
+        if (lane == 0) {
+            bool near_edge = false;
+            if (P.periodic) {
+                const double ex = fmin(pi.x - P.blx, (P.blx + P.L) - pi.x);
+                const double ey = fmin(pi.y - P.bly, (P.bly + P.H) - pi.y);
+                near_edge = !(ex >= edge && ey >= edge);
+            }
+            ncount[i] = (int32_t)((uint32_t)cnt | (near_edge ? kEdgeBit : 0u));
+        }
     }
-    bool near_edge = false;
-    if (P.periodic) {
-        const double ex = fmin(pi.x - P.blx, (P.blx + P.L) - pi.x);
-        const double ey = fmin(pi.y - P.bly, (P.bly + P.H) - pi.y);
-        near_edge = !(ex >= edge && ey >= edge);
+}
+
+// Build: the state is sorted on the list grid G (cells >= range + skin, a
+// row's cells consecutive slots, each cell in ascending id).  One wave per
+// cell: the slot runs of the 3x3 cell stencil give the candidates (coalesced
+// loads into LDS); more than kBuildCand of them flag an overflow (the lists
+// switch off) and leave the cell's lists empty.  A particle within `edge` of
+// a periodic edge sums with the minimum image (its partners may wrap before
+// the next build).
+__global__ __launch_bounds__(256) void k_nl_build(SysParams G, SysParams P, const double2 *__restrict__ pos,
+                                                  const int32_t *__restrict__ ID,
+                                                  const int32_t *__restrict__ start, int2 *__restrict__ nbr,
+                                                  int32_t *__restrict__ ncount, double rl2, double edge,
+                                                  uint32_t *__restrict__ status) {
+    __shared__ int32_t w_slot[4][kBuildCand];
+    __shared__ double2 w_pos[4][kBuildCand];
+    const int wid = (int)(threadIdx.x >> 6), lane = (int)lane_id();
+    const int cell = blockIdx.x * 4 + wid;
+    if (cell >= G.ncell) return;  // whole wave; no workgroup barrier below
+    const int t0 = start[cell], t1 = start[cell + 1];
+    if (t0 == t1) return;
+    const int row = cell / G.cols, col = cell - row * G.cols;
+    int rr[3];
+    stencil_rows(G, row, rr);
+    const ColRuns R = stencil_cols(G, col);
+    int rb[6], rn[6], M = 0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+#pragma unroll
+        for (int run = 0; run < 2; run++) {
+            const int k = 2 * d + run;
+            rb[k] = 0;
+            rn[k] = 0;
+            if (rr[d] < 0 || run >= R.n) continue;
+            const int ca = run ? R.a1 : R.a0, cb = run ? R.b1 : R.b0;
+            rb[k] = start[rr[d] * G.cols + ca];
+            rn[k] = start[rr[d] * G.cols + cb + 1] - rb[k];
+            M += rn[k];
+        }
     }
-    ncount[s] = (int32_t)((uint32_t)cnt | (near_edge ? kEdgeBit : 0u));
+    if (M > kBuildCand) {
+        for (int i = t0 + lane; i < t1; i += 64) ncount[i] = 0;
+        if (lane == 0) atomicOr(status, kStatusOverflow);
+        return;
+    }
+    // candidate e -> slot, position (runs concatenated in order)
+    for (int e = lane; e < M; e += 64) {
+        int rest = e, j = 0;
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            if (rest >= 0 && rest < rn[k]) j = rb[k] + rest;
+            rest -= rn[k];
+        }
+        w_slot[wid][e] = j;
+        w_pos[wid][e] = pos[j];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    int16_t *nbr16 = (int16_t *)nbr;
+    if (M <= 64)
+        build_cell<1>(P, pos, ID, t0, t1, M, w_slot[wid], w_pos[wid], nbr16, ncount, rl2,
+                      edge, status);
+    else
+        build_cell<2>(P, pos, ID, t0, t1, M, w_slot[wid], w_pos[wid], nbr16, ncount, rl2,
+                      edge, status);
 }
 
 // After a build: the reference chunk of every particle (K_n), chunk counts
@@ -272,11 +407,11 @@ __device__ __forceinline__ void nl_drift(const SysParams &P, const NLArgs &A, in
     const int32_t ko = A.kcur[s];
     if (kn != ko) {
         atomicOr(&A.chg[s], 2u);
-        const int groups = (int)(((uint32_t)A.ncount[s] & ~kEdgeBit) + 7u) >> 3;
+        const int groups = (int)(((uint32_t)A.ncount[s] & ~kEdgeBit) + 3u) >> 2;
         for (int g = 0; g < groups; g++) {
-            const int4 q = A.nbr[g * P.na + s];
-            const int32_t w[4] = {q.x, q.y, q.z, q.w};
-            for (int e = 0; e < 4; e++) {
+            const int2 q = A.nbr[g * P.na + s];
+            const int32_t w[2] = {q.x, q.y};
+            for (int e = 0; e < 2; e++) {
                 atomicOr(&A.chg[lo_slot(s, w[e], P.na)], 1u);
                 atomicOr(&A.chg[hi_slot(s, w[e], P.na)], 1u);
             }
@@ -318,7 +453,7 @@ __global__ __launch_bounds__(kNeedBlock) void k_nl_first(SysParams P, NLArgs A) 
     const int t = s_redo[act ? threadIdx.x : 0];
     const double2 pi = A.Pst[t];
     const uint32_t mc = act ? (uint32_t)A.ncount[t] : 0u;
-    const int groups = (int)((mc & ~kEdgeBit) + 7u) >> 3;
+    const int groups = (int)((mc & ~kEdgeBit) + 3u) >> 2;
     PairAcc<POT> acc;
     if (__any((mc & kEdgeBit) != 0u))
         nl_sum<POT, true, false>(P, A.Pst, A.nbr, t, groups, pi.x, pi.y, false, 0, 0, nullptr, acc);
@@ -341,7 +476,7 @@ __global__ __launch_bounds__(kNeedBlock) void k_nl_second(SysParams P, NLArgs A)
     const double2 pi = A.Px[sv];
     const uint32_t c = valid ? A.chg[s] : 0u;
     const uint32_t m = valid ? (uint32_t)A.ncount[s] : 0u;
-    const int groups = (int)((m & ~kEdgeBit) + 7u) >> 3;
+    const int groups = (int)((m & ~kEdgeBit) + 3u) >> 2;
     const bool stale = c != 0u;
     const bool wrap = __any((m & kEdgeBit) != 0u);
     PairAcc<POT> acc;
@@ -424,7 +559,7 @@ __global__ __launch_bounds__(1024) void k_nl_reduce(const double *__restrict__ b
 NLArgs nl_args(mavi_sys *s) {
     NList &N = *s->nl;
     NLArgs A{};
-    A.nbr = (const int4 *)N.nbr;
+    A.nbr = (const int2 *)N.nbr;
     A.ncount = N.ncount;
     A.kcur = N.key[N.kc];
     A.knext = N.key[1 - N.kc];
@@ -545,8 +680,8 @@ hipError_t nl_rebuild(mavi_sys *s, hipStream_t st) {
     const int c = s->cur;
     {
         ProfScope ps(s, MAVI_PROF_BIN, st);
-        hipLaunchKernelGGL(k_nl_build, dim3(nblk(P.na, 64)), dim3(64), 0, st, N.G, P, s->P_[c], s->ID_[c], s->scell,
-                           s->start, (int4 *)N.nbr, N.ncount, N.rl2, N.edge, N.status);
+        hipLaunchKernelGGL(k_nl_build, dim3(nblk(N.G.ncell, 4)), dim3(256), 0, st, N.G, P, s->P_[c], s->ID_[c],
+                           s->start, (int2 *)N.nbr, N.ncount, N.rl2, N.edge, N.status);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if ((e = launch_fill_i32(N.ccnt, 0, P.ncell, st)) != hipSuccess) return e;
         hipLaunchKernelGGL(k_nl_keys, dim3(nblk(P.na, 256)), dim3(256), 0, st, P, s->P_[c], N.key[0], N.ccnt,

@@ -153,3 +153,35 @@ def test_stale_pair_within_cutoff_left_out(gpu_ctx, oracle_mod):
             assert s.state.vel[1, 0] == -20.0 and np.all(s.forces == 0.0)
         else:
             assert s.state.vel[1, 0] != -20.0  # the next steps bin afresh: the pair acts
+
+
+@pytest.mark.parametrize("skin", [150, 1000])
+def test_small_periodic_list_grid(gpu_ctx, oracle_mod, skin):
+    """A periodic box 3.2 wide on 3 x 3 chunks with LJ cut at 1: the list
+    grid has 2 cells a side at skin 0.15 and 1 at skin 1.0, so the 3 x 3 cell
+    stencil must visit each cell once (the wrapped neighbours coincide)."""
+    import mavi_hip as mv
+    from mavi_hip import _lib
+
+    rng = np.random.default_rng(7)
+    a = 3.2 / 3
+    g = (np.arange(3) + 0.5) * a
+    pos = np.stack(np.meshgrid(g, g, indexing="xy"), axis=-1).reshape(-1, 2)
+    pos = pos + rng.uniform(-0.04, 0.04, size=pos.shape)  # separations > 0.98
+    vel = rng.standard_normal(size=pos.shape) * 0.5
+    dyn = mv.LenJonesCfg(sigma=0.9, epsilon=1.0, cutoff=1.0)
+    space = mv.SpaceCfg(wall_type=mv.PeriodicWalls(), geometry_cfg=mv.RectangleCfg(length=3.2, height=3.2))
+    ic = mv.IntCfg(dt=0.002, chunks_cfg=mv.ChunksCfg(num_cols=3, num_rows=3), device=mv.GPU())
+    s = mv.System(state=mv.SecondLawState(pos=pos.copy(), vel=vel.copy()), space_cfg=space, dynamic_cfg=dyn,
+                  int_cfg=ic)
+    s.dev.set_option(_lib.MAVI_OPT_NLIST_SKIN, skin)
+    s.advance(200)
+    s.download()
+    info = s.dev.nlist_info()
+    say(f"[nlist small grid skin {skin}] {info}")
+    assert info["active"]
+    o = oracle_mod.from_system(len(pos), space, dyn, ic)
+    p_ref, v_ref, f_ref, _, _ = o.newton_steps(pos, vel, 200)
+    ep, ev = traj_err(s.state.pos, s.state.vel, p_ref, v_ref, 3.2, periodic=True)
+    assert ep <= TRAJ_RTOL and ev <= TRAJ_RTOL
+    assert np.abs(s.forces - f_ref).max() <= 1e-9 * max(np.abs(f_ref).max(), 1e-300)
