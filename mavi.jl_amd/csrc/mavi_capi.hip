@@ -830,7 +830,7 @@ static bool invasions_due(const mavi_sys *s, int64_t ahead) {
 }
 
 // Neighbour-list stepping of a finite-range plain system: blocks of steps
-// (two-step graphs), after each block one read back of the largest
+// (six-step graphs), after each block one read back of the largest
 // displacement since the last build.  A block in which some particle moved
 // more than skin / 2 may have missed a pair: its start state is restored,
 // the lists are rebuilt there and the block runs again (shorter).  The lists
@@ -845,13 +845,12 @@ static hipError_t nl_graph(mavi_sys *s, hipStream_t st) {
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return e;
-    const int kc0 = N.kc;
-    hipError_t e1 = mavi::nl_step(s, st);
-    hipError_t e2 = (e1 == hipSuccess) ? mavi::nl_step(s, st) : e1;
+    const int ph0 = N.ph;
+    hipError_t e1 = hipSuccess;
+    for (int k = 0; k < 6 && e1 == hipSuccess; k++) e1 = mavi::nl_step(s, st);
     e = hipStreamEndCapture(st, &g);
-    N.kc = kc0;
+    N.ph = ph0;
     if (e1 != hipSuccess) e = e1;
-    if (e == hipSuccess && e2 != hipSuccess) e = e2;
     if (e == hipSuccess) e = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
     if (g) (void)hipGraphDestroy(g);
     return e;
@@ -887,15 +886,17 @@ static int nl_steps(mavi_sys *s, int64_t nsteps, int64_t *left) {
         int64_t done = 0;
         const bool graphs = s->use_graphs && !s->prof;
         while (done < blk) {
-            if (graphs && N.kc == 0 && done + 2 <= blk) {
+            if (graphs && N.ph == 0 && done + 6 <= blk) {
                 HIPCHK(ctx, nl_graph(s, st));
                 HIPCHK(ctx, hipGraphLaunch(N.graph[s->cur], st));
-                done += 2;
+                N.unsettled = true;
+                done += 6;
             } else {
                 HIPCHK(ctx, mavi::nl_step(s, st));
                 done += 1;
             }
         }
+        HIPCHK(ctx, mavi::nl_settle(s, st));
         HIPCHK(ctx, mavi::nl_reduce_dmax(s, st));
         HIPCHK(ctx, hipMemcpyAsync(N.host, N.dmax, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
         HIPCHK(ctx, hipMemcpyAsync((char *)N.host + 8, N.status, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -908,9 +909,11 @@ static int nl_steps(mavi_sys *s, int64_t nsteps, int64_t *left) {
             HIPCHK(ctx, hipMemcpyAsync(s->V_[s->cur], N.vsnap, bytes, hipMemcpyDeviceToDevice, st));
             HIPCHK(ctx, hipMemcpyAsync(s->flags, N.fsnap, sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
             N.valid = false;
+            N.unsettled = false;
             N.rollbacks++;
-            // a list overflowed, or one step from a fresh build moved a
-            // particle more than skin / 2: lists cannot keep up here
+            // a list overflowed, a chunk count may have passed the capacity
+            // (the chunk gather checks it exactly), or one step from a fresh
+            // build moved a particle more than skin / 2
             if (status || (fresh && blk == 1)) {
                 // the chunk gather takes over
                 N.disabled = true;
@@ -1492,7 +1495,7 @@ int mavi_sys_set_option(mavi_sys *s, int32_t option, int32_t value) {
             break;
         case MAVI_OPT_NLIST_BLOCK:
             if (value < 0 || value > 4096) return set_err(s->ctx, MAVI_E_BADARG, "block: 0 (default) .. 4096 steps");
-            if (s->nl) s->nl->block = s->nl->block_now = value == 0 ? 32 : value;
+            if (s->nl) s->nl->block = s->nl->block_now = value == 0 ? 30 : value;
             break;
         default:
             return set_err(s->ctx, MAVI_E_BADARG, "unknown option");

@@ -172,29 +172,32 @@ struct NList {
     double thr2 = 0.0;        // displacement bound squared (skin / 2)^2, with margin
     double edge = 0.0;        // within this distance of a periodic edge: minimum image
     int32_t kmax = 48;        // entries per particle
-    int32_t block = 32;       // steps between host checks (option)
-    int32_t block_now = 32;   // current block (halved after a rollback)
+    int32_t block = 30;       // steps between host checks (option)
+    int32_t block_now = 30;   // current block (halved after a rollback)
     SysParams G{};            // internal grid for sorting and building (cells >= range + skin)
     int32_t *nbr = nullptr;   // (kmax / 4) x na groups of 4 16-bit partner offsets (int2), ascending partner id, padded with the particle
     int32_t *ncount = nullptr;// entries per slot; bit 31: minimum image needed
-    int32_t *key[2] = {nullptr, nullptr};  // reference chunk (row << 16 | col) of each slot (ping-pong: K_n, K_{n+1})
-    int kc = 0;               // key[kc] holds K_n
+    int32_t *key[3] = {nullptr, nullptr, nullptr};  // reference chunk (row << 16 | col) of each slot: K_n, K_{n+1}, K_{n+2} rotating
+    int ph = 0;               // steps since the build mod 6: key[ph % 3] holds K_n; chg, work, px ping-pong by ph % 2
     int32_t *ccnt = nullptr;  // particles per reference chunk
-    uint32_t *chg = nullptr;  // this drift: 2 = changed chunk, 1 = a list partner did (stale test)
-    uint8_t *need = nullptr;  // recompute the next first evaluation
+    uint8_t *chg[2] = {nullptr, nullptr};   // a step's drift marks: 2 = changed chunk, 1 = a list partner did (stale test)
+    double2 *fk = nullptr;                  // the next first evaluation where it differs from the last second one
+    uint8_t *fk_on = nullptr;               // fk holds it
+    double2 *px[2] = {nullptr, nullptr};    // drifted positions x_{n+1}, x_{n+2} (before the periodic wrap)
+    bool unsettled = false;                 // the state's positions lag the drifted ones (nl_settle)
     double2 *xb = nullptr;    // positions at build
     double2 *psnap = nullptr, *vsnap = nullptr;  // block-start state (rollback)
     uint32_t *fsnap = nullptr;
     unsigned long long *dmax = nullptr;  // max displacement^2 since build (ordered bits), reduced per block
     double *bmax = nullptr;              // per workgroup of the second evaluation: its largest displacement^2 since build
     int32_t nbmax = 0;
-    uint32_t *status = nullptr;          // bit 0: a list overflowed kmax
+    uint32_t *status = nullptr;          // bit 0: a list overflowed kmax; bit 1: a chunk count may have passed the capacity
     unsigned long long *scan_ticket = nullptr, *scan_state = nullptr;  // the list grid's scan (cleared per build)
     int32_t scan_words = 0;
     unsigned long long *host = nullptr;  // pinned read back: dmax, status
     double d_last = 0.0;      // sqrt(dmax) at the last block end
     int64_t rebuilds = 0, rollbacks = 0;
-    hipGraphExec_t graph[2] = {nullptr, nullptr};  // two steps (kc = 0, 1) on state copy 0 / 1 (a rebuild's sort flips it)
+    hipGraphExec_t graph[2] = {nullptr, nullptr};  // six steps from ph = 0 on state copy 0 / 1 (a rebuild's sort flips it)
 };
 
 struct mavi_sys {
@@ -351,12 +354,15 @@ hipError_t launch_bin(mavi_sys *s, hipStream_t st, bool count_step = false, bool
                       unsigned long long *over_state = nullptr);
 // Neighbour lists (mavi_nlist.hip): eligibility and allocation at creation,
 // the rebuild (sort on the list grid, lists sorted by partner id, reference
-// chunk keys and counts), one newton_step! (two launches), and release.
+// chunk keys and counts), one newton_step! (two launches), the block-end
+// settling of the state's positions, and release.
 bool nl_eligible(const mavi_sys *s);
 hipError_t nl_alloc(mavi_sys *s, hipStream_t st);
 void nl_free(mavi_sys *s);
 hipError_t nl_rebuild(mavi_sys *s, hipStream_t st);
 hipError_t nl_step(mavi_sys *s, hipStream_t st);
+// the state's positions <- the wrap of the last drift (after steps)
+hipError_t nl_settle(mavi_sys *s, hipStream_t st);
 // dmax <- the largest per-workgroup displacement since the build
 hipError_t nl_reduce_dmax(mavi_sys *s, hipStream_t st);
 void nl_set_skin(mavi_sys *s, double skin_frac);

@@ -10,13 +10,22 @@
 // evaluation of a step is "every pair within range".  The second evaluation
 // keeps the step's chunks at the drifted positions: a pair within range whose
 // particles' step-start chunks are not adjacent is left out.  That can only
-// happen to a pair with a particle whose chunk changed in the drift; those
-// particles and their list partners test chunk adjacency pair by pair.
+// happen to a pair with a particle whose chunk changed in the drift; such a
+// particle marks the list partners whose step-start chunk is not adjacent to
+// its own, and the marked particles test chunk adjacency pair by pair.
 //
 // Lists.  Every particle lists the others within range + skin, found on a
 // grid of cells at least that wide after sorting the state by those cells.
 // The state keeps that order until the next rebuild, so a step is two
-// launches and no binning.  A list stays complete while no particle has moved
+// launch and no binning: the second evaluation of step n also makes the
+// first evaluation of step n + 1 and its drift.  Where the pair set did not
+// change (no chunk change in or around the particle's list) the next first
+// evaluation is the second one, bit for bit; the marked particles sum their
+// list a second time, every pair within range at the wrapped positions, as
+// the next step's calc_forces! would.  Between the host's checks positions
+// live only as the drifted ones before the periodic wrap (two copies,
+// x_{n+1} and x_{n+2}); the state's positions, their wrap, are written once
+// per block.  A list stays complete while no particle has moved
 // more than skin / 2 since the build (no two particles can then close the
 // gap); the second evaluation records the largest displacement, and the host
 // checks it every few steps, rolling the block back and rebuilding when it
@@ -32,7 +41,9 @@
 // Errors.  The reference chunk of every final position is computed every
 // step (out of box, non-finite), and chunk counts follow the particles that
 // change chunk (the capacity check of update_particle_chunk!,
-// src/chunks.jl:144-146).
+// src/chunks.jl:144-146): a count that passes the capacity while a step's
+// moves are applied (at least the count after them) hands the block to the
+// chunk gather, whose binning checks it exactly.
 #include <math.h>
 
 #include "mavi_device.h"
@@ -43,6 +54,7 @@ namespace {
 constexpr uint32_t kEdgeBit = 0x80000000u;  // ncount: minimum image needed
 constexpr uint32_t kStatusOverflow = 1u;    // a list held more than kmax entries, or a partner more
                                             // than 32767 slots away (modulo na)
+constexpr uint32_t kStatusCapacity = 2u;    // a chunk count passed the capacity during a step's moves
 constexpr int kNlMax = 48;                  // list entries per particle (12 groups of 4)
 constexpr int kNeedBlock = 256;             // workgroup of the evaluation kernels
 
@@ -50,18 +62,37 @@ struct NLArgs {
     const int2 *nbr;      // groups of 4 partner offsets ((slot - s) mod na in [-32768, 32767]): nbr[g * na + s]
     const int32_t *ncount;
     const int32_t *kcur;  // K_n: reference chunk (row << 16 | col) of the step's start positions
-    int32_t *knext;       // K_{n+1}
+    int32_t *knext;       // K_{n+1}: of the drifted positions
+    int32_t *knn;         // K_{n+2}: of the next step's drift
     int32_t *ccnt;
-    uint32_t *chg;
-    uint8_t *need;
+    uint8_t *chg;         // marks of this step's drift: 2 = changed chunk, 1 = a list partner did
+    uint8_t *chg_next;    // marks of the next step's drift
     const double2 *xb;
     double *bmax;
-    double2 *Pst, *V, *F, *Px;
+    const double2 *Pst;   // the state's positions (x_n after a build)
+    double2 *V;
+    double2 *F;           // the last second evaluation (the reference's forces)
+    double2 *Fk;          // where fk_on: the next first evaluation, when it differs from F
+    uint8_t *fk_on;
+    double2 *Px;          // drifted positions x_{n+1} (before the periodic wrap)
+    double2 *Px_next;     // x_{n+2}
     uint32_t *flags;
-    int32_t reuse;
+    uint32_t *status;
+    int32_t reuse;        // 0: every particle sums its next first evaluation afresh
 };
 
 __device__ __forceinline__ int32_t pack_rc(int32_t r, int32_t c) { return (r << 16) | c; }
+
+// A mark bit on slot j's byte (atomic on its 32-bit word).
+__device__ __forceinline__ void mark(uint8_t *chg, int j, uint32_t bit) {
+    atomicOr((uint32_t *)(chg + (j & ~3)), bit << (8 * (j & 3)));
+}
+
+// The state's position of a drifted one: walls! wraps it into a periodic box.
+__device__ __forceinline__ double2 state_pos(const SysParams &P, double2 p) {
+    if (P.periodic) periodic_wrap(P, p.x, p.y);
+    return p;
+}
 
 // Chunks of the reference grid adjacent (the same chunk or one of its eight
 // neighbours, across the periodic edges for a periodic box): the pairs
@@ -118,8 +149,9 @@ __device__ __forceinline__ int32_t slot_offset(int s, int j, int na) {
 
 // The sum over one particle's list in list order (ascending partner id),
 // four entries per iteration; the next group is loaded one iteration ahead,
-// so the list and position loads of consecutive groups overlap.
-template <int POT, bool WRAP, bool STALE>
+// so the list and position loads of consecutive groups overlap.  WRAPIN: pin
+// holds drifted positions, taken through state_pos.
+template <int POT, bool WRAP, bool STALE, bool WRAPIN = false>
 __device__ __forceinline__ void nl_sum(const SysParams &P, const double2 *__restrict__ pin,
                                        const int2 *__restrict__ nbr, int s, int groups, double xi, double yi,
                                        bool test, int ri, int ci, const int32_t *__restrict__ kcur,
@@ -133,7 +165,10 @@ __device__ __forceinline__ void nl_sum(const SysParams &P, const double2 *__rest
         const int j[4] = {lo_slot(s, c.x, na), hi_slot(s, c.x, na), lo_slot(s, c.y, na), hi_slot(s, c.y, na)};
         double2 p[4];
 #pragma unroll
-        for (int e = 0; e < 4; e++) p[e] = load_slot(pin, j[e]);
+        for (int e = 0; e < 4; e++) {
+            p[e] = load_slot(pin, j[e]);
+            if (WRAPIN) p[e] = state_pos(P, p[e]);
+        }
 #pragma unroll
         for (int e = 0; e < 4; e++) nl_pair<POT, WRAP, STALE>(P, p[e], j[e], s, xi, yi, test, ri, ci, kcur, acc);
     }
@@ -361,10 +396,10 @@ __global__ __launch_bounds__(256) void k_nl_build(SysParams G, SysParams P, cons
 }
 
 // After a build: the reference chunk of every particle (K_n), chunk counts
-// with the capacity check (one atomic per run of equal chunks in a wave),
-// everyone recomputes its first evaluation, the build positions.
+// with the capacity check (one atomic per run of equal chunks in a wave), no
+// marks, the build positions.
 __global__ void k_nl_keys(SysParams P, const double2 *__restrict__ pos, int32_t *__restrict__ key,
-                          int32_t *__restrict__ ccnt, uint8_t *__restrict__ need, uint32_t *__restrict__ chg,
+                          int32_t *__restrict__ ccnt, uint8_t *__restrict__ chg0, uint8_t *__restrict__ chg1,
                           double2 *__restrict__ xb, uint32_t *__restrict__ flags) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = s < P.na;
@@ -381,126 +416,136 @@ __global__ void k_nl_keys(SysParams P, const double2 *__restrict__ pos, int32_t 
     if (flag) atomicOr(flags, flag);
     if (!valid) return;
     key[s] = pack_rc(r, c);
-    need[s] = 1;
-    chg[s] = 0u;
+    chg0[s] = 0;
+    chg1[s] = 0;
     xb[s] = p;
 }
 
-// The drift of one particle (update_verlet!, src/integration.jl:424), the
-// reference chunk of its final position (after walls!'s periodic wrap) and,
-// if the chunk changes, the marks of the particle (2) and of its list
-// partners (1), whose second evaluation tests chunk adjacency, and the chunk
-// counts.
-__device__ __forceinline__ void nl_drift(const SysParams &P, const NLArgs &A, int s, double2 pi, double fx,
-                                         double fy) {
-    const double2 v = A.V[s];
+// The drift of one particle (update_verlet!, src/integration.jl:424) from
+// start position pi with velocity v and force f into pout, the reference
+// chunk of its final position (after walls!'s periodic wrap) into knew and,
+// if it differs from kold (the chunks the next second evaluation keeps), the
+// chunk counts and the marks in chg of the pairs that evaluation could treat
+// differently from the first evaluation after it: a list partner whose kold
+// chunk is not adjacent to this particle's (1) and then the particle (2);
+// all partners if the wrap moved the particle (the first evaluation sums the
+// wrapped positions).
+__device__ __forceinline__ void nl_drift(const SysParams &P, const NLArgs &A, int s, double2 pi, double2 v,
+                                         double fx, double fy, double2 *__restrict__ pout,
+                                         const int32_t *__restrict__ kold, int32_t *__restrict__ knew,
+                                         uint8_t *__restrict__ chg) {
     const double px = pi.x + (v.x * P.dt + fx * P.term), py = pi.y + (v.y * P.dt + fy * P.term);
-    A.Px[s] = make_double2(px, py);
+    pout[s] = make_double2(px, py);
     double wx = px, wy = py;
     if (P.periodic) periodic_wrap(P, wx, wy);  // walls! moves it there after the kick
-    uint32_t flag = 0;
-    int32_t r, c;
-    cell_rc(P, P.rows, P.cols, P.cl, P.ch, wx, wy, &flag, MAVI_FLAG_OUT_OF_BOX, &r, &c);
-    if (flag) atomicOr(A.flags, flag);
-    const int32_t kn = pack_rc(r, c);
-    A.knext[s] = kn;
-    const int32_t ko = A.kcur[s];
+    const int32_t ko = kold[s];
+    // still clear inside its chunk (by far more than the rounding of the
+    // chunk edges): the same chunk, without the exact division
+    const int r0 = ko >> 16, c0 = ko & 0xffff;
+    const double ax = wx - P.blx, ay = (-wy + P.bly) + P.H;
+    const double m = 1e-12 * (P.L + P.H);
+    int32_t kn = ko;
+    if (!(ax >= (double)c0 * P.cl + m && ax <= (double)(c0 + 1) * P.cl - m && ay >= (double)r0 * P.ch + m &&
+          ay <= (double)(r0 + 1) * P.ch - m)) {
+        uint32_t flag = 0;
+        int32_t r, c;
+        cell_rc(P, P.rows, P.cols, P.cl, P.ch, wx, wy, &flag, MAVI_FLAG_OUT_OF_BOX, &r, &c);
+        if (flag) atomicOr(A.flags, flag);
+        kn = pack_rc(r, c);
+    }
+    knew[s] = kn;
     if (kn != ko) {
-        atomicOr(&A.chg[s], 2u);
+        const bool wrapped = wx != px || wy != py;
+        bool any = false;
         const int groups = (int)(((uint32_t)A.ncount[s] & ~kEdgeBit) + 3u) >> 2;
         for (int g = 0; g < groups; g++) {
             const int2 q = A.nbr[g * P.na + s];
-            const int32_t w[2] = {q.x, q.y};
-            for (int e = 0; e < 2; e++) {
-                atomicOr(&A.chg[lo_slot(s, w[e], P.na)], 1u);
-                atomicOr(&A.chg[hi_slot(s, w[e], P.na)], 1u);
+            const int j4[4] = {lo_slot(s, q.x, P.na), hi_slot(s, q.x, P.na), lo_slot(s, q.y, P.na),
+                               hi_slot(s, q.y, P.na)};
+            for (int e = 0; e < 4; e++) {
+                const int j = j4[e];
+                if (j == s) continue;  // list pad
+                if (wrapped || !chunks_adjacent(P, r0, c0, kold[j])) {
+                    mark(chg, j, 1u);
+                    any = true;
+                }
             }
         }
-        atomicSub(&A.ccnt[(ko >> 16) * P.cols + (ko & 0xffff)], 1);
-        atomicAdd(&A.ccnt[r * P.cols + c], 1);
+        if (any) mark(chg, s, 2u);
+        atomicSub(&A.ccnt[r0 * P.cols + c0], 1);
+        // while one step's moves are applied a count runs through values at
+        // least as large as its final one
+        if (atomicAdd(&A.ccnt[(kn >> 16) * P.cols + (kn & 0xffff)], 1) + 1 > P.nc)
+            atomicOr(A.status, kStatusCapacity);
     }
 }
 
-// First evaluation of newton_step! (forces at x_n over every pair within
-// range) where the pair set may have changed, the previous second evaluation
-// elsewhere; then the drift.  The particles to recompute are compacted to the
-// first lanes of their workgroup, so waves without one only drift.
+// After a build: the first evaluation of newton_step! (forces at x_n over
+// every pair within range) and the drift, for every particle.
 template <int POT>
-__global__ __launch_bounds__(kNeedBlock) void k_nl_first(SysParams P, NLArgs A) {
-    __shared__ int32_t s_redo[kNeedBlock];
-    __shared__ int32_t s_nredo;
+__global__ __launch_bounds__(kNeedBlock) void k_nl_start(SysParams P, NLArgs A) {
     const int s = xcd_block() * blockDim.x + threadIdx.x;
     const bool valid = s < P.na;
-    const bool need = valid && (!A.reuse || A.need[s]);
-    if (valid && !need) {
-        const double2 f = A.F[s];
-        nl_drift(P, A, s, A.Pst[s], f.x, f.y);
-    }
-    if (threadIdx.x == 0) s_nredo = 0;
-    __syncthreads();
-    const unsigned long long m = __ballot(need);
-    if (m) {
-        const int leader = __ffsll(m) - 1;
-        int base = 0;
-        if ((int)lane_id() == leader) base = atomicAdd(&s_nredo, __popcll(m));
-        base = __shfl(base, leader, 64);
-        if (need) s_redo[base + __popcll(m & ((1ull << lane_id()) - 1ull))] = s;
-    }
-    __syncthreads();
-    const int nredo = s_nredo;
-    if ((int)(threadIdx.x & ~63u) >= nredo) return;  // whole wave
-    const bool act = (int)threadIdx.x < nredo;
-    const int t = s_redo[act ? threadIdx.x : 0];
-    const double2 pi = A.Pst[t];
-    const uint32_t mc = act ? (uint32_t)A.ncount[t] : 0u;
-    const int groups = (int)((mc & ~kEdgeBit) + 3u) >> 2;
+    const int sv = valid ? s : 0;
+    const double2 pi = A.Pst[sv];
+    const uint32_t m = valid ? (uint32_t)A.ncount[s] : 0u;
+    const int groups = (int)((m & ~kEdgeBit) + 3u) >> 2;
     PairAcc<POT> acc;
-    if (__any((mc & kEdgeBit) != 0u))
-        nl_sum<POT, true, false>(P, A.Pst, A.nbr, t, groups, pi.x, pi.y, false, 0, 0, nullptr, acc);
+    if (__any((m & kEdgeBit) != 0u))
+        nl_sum<POT, true, false>(P, A.Pst, A.nbr, sv, groups, pi.x, pi.y, false, 0, 0, nullptr, acc);
     else
-        nl_sum<POT, false, false>(P, A.Pst, A.nbr, t, groups, pi.x, pi.y, false, 0, 0, nullptr, acc);
-    if (!act) return;
-    A.F[t] = make_double2(acc.fx, acc.fy);
-    nl_drift(P, A, t, pi, acc.fx, acc.fy);
+        nl_sum<POT, false, false>(P, A.Pst, A.nbr, sv, groups, pi.x, pi.y, false, 0, 0, nullptr, acc);
+    if (!valid) return;
+    A.F[s] = make_double2(acc.fx, acc.fy);
+    A.fk_on[s] = 0;
+    nl_drift(P, A, s, pi, A.V[s], acc.fx, acc.fy, A.Px, A.kcur, A.knext, A.chg);
 }
 
 // Second evaluation at the drifted positions with the step-start chunks,
-// then the kick and walls! (src/integration.jl:425-431, 271-285, 309-324).
-// Records whether the next first evaluation must recompute, the capacity
-// check of a chunk change, and the largest displacement since the build.
+// then the kick and walls! (src/integration.jl:425-431, 271-285, 309-324),
+// then the next step: its first evaluation (this one, or for the marked
+// particles every pair within range at the wrapped positions) and its drift.
+// Records the largest displacement since the build.
 template <int POT>
 __global__ __launch_bounds__(kNeedBlock) void k_nl_second(SysParams P, NLArgs A) {
     const int s = xcd_block() * blockDim.x + threadIdx.x;
     const bool valid = s < P.na;
     const int sv = valid ? s : 0;
-    const double2 pi = A.Px[sv];
+    const double2 pu = A.Px[sv];
+    const double2 pw = state_pos(P, pu);
     const uint32_t c = valid ? A.chg[s] : 0u;
     const uint32_t m = valid ? (uint32_t)A.ncount[s] : 0u;
     const int groups = (int)((m & ~kEdgeBit) + 3u) >> 2;
     const bool stale = c != 0u;
+    const bool fresh = stale || !A.reuse;  // the next first evaluation is summed afresh
     const bool wrap = __any((m & kEdgeBit) != 0u);
-    PairAcc<POT> acc;
-    if (__any(stale)) {
+    PairAcc<POT> acc, acc1;
+    if (__any(fresh)) {
         const int32_t ki = A.kcur[sv];
         const int ri = ki >> 16, ci = ki & 0xffff;
-        if (wrap) nl_sum<POT, true, true>(P, A.Px, A.nbr, sv, groups, pi.x, pi.y, stale, ri, ci, A.kcur, acc);
-        else nl_sum<POT, false, true>(P, A.Px, A.nbr, sv, groups, pi.x, pi.y, stale, ri, ci, A.kcur, acc);
+        const int g1 = fresh ? groups : 0;
+        if (wrap) {
+            nl_sum<POT, true, true>(P, A.Px, A.nbr, sv, groups, pu.x, pu.y, stale, ri, ci, A.kcur, acc);
+            nl_sum<POT, true, false, true>(P, A.Px, A.nbr, sv, g1, pw.x, pw.y, false, 0, 0, nullptr, acc1);
+        } else {
+            nl_sum<POT, false, true>(P, A.Px, A.nbr, sv, groups, pu.x, pu.y, stale, ri, ci, A.kcur, acc);
+            nl_sum<POT, false, false, true>(P, A.Px, A.nbr, sv, g1, pw.x, pw.y, false, 0, 0, nullptr, acc1);
+        }
     } else {
-        if (wrap) nl_sum<POT, true, false>(P, A.Px, A.nbr, sv, groups, pi.x, pi.y, false, 0, 0, nullptr, acc);
-        else nl_sum<POT, false, false>(P, A.Px, A.nbr, sv, groups, pi.x, pi.y, false, 0, 0, nullptr, acc);
+        if (wrap) nl_sum<POT, true, false>(P, A.Px, A.nbr, sv, groups, pu.x, pu.y, false, 0, 0, nullptr, acc);
+        else nl_sum<POT, false, false>(P, A.Px, A.nbr, sv, groups, pu.x, pu.y, false, 0, 0, nullptr, acc);
     }
     double d2 = 0.0;
     if (valid) {
-        const double fx = acc.fx, fy = acc.fy;
+        const double fx = acc.fx, fy = acc.fy;                     // this second evaluation
+        const double gx = fresh ? acc1.fx : fx, gy = fresh ? acc1.fy : fy;  // the next first one
         const double2 v = A.V[s];
-        const double2 fo = A.F[s];
+        const double2 fo = A.fk_on[s] ? A.Fk[s] : A.F[s];           // this step's first one
         // vel += dt/2 * (forces + old_forces) (src/integration.jl:430), then walls!
         double vx = v.x + P.hdt * (fx + fo.x);
         double vy = v.y + P.hdt * (fy + fo.y);
-        double x = pi.x, y = pi.y;
-        if (P.periodic) {
-            periodic_wrap(P, x, y);
-        } else {
+        const double x = pw.x, y = pw.y;
+        if (!P.periodic) {
             // src/integration.jl:271-285
             const double rx = x - P.blx, ry = y - P.bly;
             const bool ox = ((rx + P.radius) > P.L) || ((rx - P.radius) < 0.0);
@@ -508,15 +553,12 @@ __global__ __launch_bounds__(kNeedBlock) void k_nl_second(SysParams P, NLArgs A)
             if (ox) vx = -vx;
             if (oy) vy = -vy;
         }
-        A.Pst[s] = make_double2(x, y);
         A.V[s] = make_double2(vx, vy);
         A.F[s] = make_double2(fx, fy);
-        A.need[s] = stale ? 1 : 0;
-        A.chg[s] = 0u;
-        if (c & 2u) {
-            const int32_t kn = A.knext[s];
-            if (A.ccnt[(kn >> 16) * P.cols + (kn & 0xffff)] > P.nc) atomicOr(A.flags, MAVI_FLAG_OVERFLOW);
-        }
+        if (fresh) A.Fk[s] = make_double2(gx, gy);
+        A.fk_on[s] = fresh ? 1 : 0;
+        A.chg[s] = 0;
+        nl_drift(P, A, s, pw, make_double2(vx, vy), gx, gy, A.Px_next, A.knext, A.knn, A.chg_next);
         const double2 b = A.xb[s];
         double ddx = x - b.x, ddy = y - b.y;
         if (P.periodic) {
@@ -556,23 +598,35 @@ __global__ __launch_bounds__(1024) void k_nl_reduce(const double *__restrict__ b
     }
 }
 
+// The state's positions at a block end: the wrap of the last drift.
+__global__ void k_nl_settle(SysParams P, const double2 *__restrict__ px, double2 *__restrict__ pst) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < P.na) pst[s] = state_pos(P, px[s]);
+}
+
 NLArgs nl_args(mavi_sys *s) {
     NList &N = *s->nl;
+    const int n = N.ph;
     NLArgs A{};
     A.nbr = (const int2 *)N.nbr;
     A.ncount = N.ncount;
-    A.kcur = N.key[N.kc];
-    A.knext = N.key[1 - N.kc];
+    A.kcur = N.key[n % 3];
+    A.knext = N.key[(n + 1) % 3];
+    A.knn = N.key[(n + 2) % 3];
     A.ccnt = N.ccnt;
-    A.chg = N.chg;
-    A.need = N.need;
+    A.chg = N.chg[n % 2];
+    A.chg_next = N.chg[(n + 1) % 2];
     A.xb = N.xb;
     A.bmax = N.bmax;
     A.Pst = s->P_[s->cur];
     A.V = s->V_[s->cur];
     A.F = s->F;
-    A.Px = s->P2;
+    A.Fk = N.fk;
+    A.fk_on = N.fk_on;
+    A.Px = N.px[n % 2];
+    A.Px_next = N.px[(n + 1) % 2];
     A.flags = s->flags;
+    A.status = N.status;
     A.reuse = s->reuse ? 1 : 0;
     return A;
 }
@@ -631,11 +685,12 @@ hipError_t nl_alloc(mavi_sys *s, hipStream_t st) {
     N.kmax = kNlMax;
     hipError_t e = dalloc(&N.nbr, (int64_t)(N.kmax / 2) * na, st);  // 16-bit entries
     if (e == hipSuccess) e = dalloc(&N.ncount, na, st);
-    if (e == hipSuccess) e = dalloc(&N.key[0], na, st);
-    if (e == hipSuccess) e = dalloc(&N.key[1], na, st);
+    for (int k = 0; k < 3 && e == hipSuccess; k++) e = dalloc(&N.key[k], na, st);
+    for (int k = 0; k < 2 && e == hipSuccess; k++) e = dalloc(&N.chg[k], (na + 3) & ~int64_t(3), st);
+    for (int k = 0; k < 2 && e == hipSuccess; k++) e = dalloc(&N.px[k], na, st);
+    if (e == hipSuccess) e = dalloc(&N.fk, na, st);
+    if (e == hipSuccess) e = dalloc(&N.fk_on, na, st);
     if (e == hipSuccess) e = dalloc(&N.ccnt, (int64_t)s->P.ncell, st);
-    if (e == hipSuccess) e = dalloc(&N.chg, na, st);
-    if (e == hipSuccess) e = dalloc(&N.need, na, st);
     if (e == hipSuccess) e = dalloc(&N.xb, na, st);
     if (e == hipSuccess) e = dalloc(&N.psnap, na, st);
     if (e == hipSuccess) e = dalloc(&N.vsnap, na, st);
@@ -657,8 +712,11 @@ void nl_free(mavi_sys *s) {
     NList &N = *s->nl;
     for (auto &g : N.graph)
         if (g) (void)hipGraphExecDestroy(g);
-    dfree(N.nbr); dfree(N.ncount); dfree(N.key[0]); dfree(N.key[1]); dfree(N.ccnt);
-    dfree(N.chg); dfree(N.need); dfree(N.xb); dfree(N.psnap); dfree(N.vsnap); dfree(N.fsnap);
+    dfree(N.nbr); dfree(N.ncount); dfree(N.ccnt);
+    for (auto &k : N.key) dfree(k);
+    for (auto &c : N.chg) dfree(c);
+    for (auto &p : N.px) dfree(p);
+    dfree(N.fk); dfree(N.fk_on); dfree(N.xb); dfree(N.psnap); dfree(N.vsnap); dfree(N.fsnap);
     dfree(N.dmax); dfree(N.bmax); dfree(N.status); dfree(N.scan_ticket); dfree(N.scan_state);
     if (N.host) (void)hipHostFree(N.host);
     delete s->nl;
@@ -685,16 +743,36 @@ hipError_t nl_rebuild(mavi_sys *s, hipStream_t st) {
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if ((e = launch_fill_i32(N.ccnt, 0, P.ncell, st)) != hipSuccess) return e;
         hipLaunchKernelGGL(k_nl_keys, dim3(nblk(P.na, 256)), dim3(256), 0, st, P, s->P_[c], N.key[0], N.ccnt,
-                           N.need, N.chg, N.xb, s->flags);
+                           N.chg[0], N.chg[1], N.xb, s->flags);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if ((e = launch_fill_i32((int32_t *)N.dmax, 0, 2, st)) != hipSuccess) return e;
         if ((e = launch_fill_i32((int32_t *)N.bmax, 0, 2 * (int64_t)N.nbmax, st)) != hipSuccess) return e;
     }
-    N.kc = 0;
+    // the first evaluation and drift of the first step
+    N.ph = 0;
+    {
+        ProfScope ps(s, MAVI_PROF_FORCE_REUSE, st);
+        const NLArgs A = nl_args(s);
+        const int nb = nblk(P.na, kNeedBlock);
+        if (P.potential == MAVI_POT_LJ) hipLaunchKernelGGL(k_nl_start<POT_LJ>, dim3(nb), dim3(kNeedBlock), 0, st, P, A);
+        else hipLaunchKernelGGL(k_nl_start<POT_HT>, dim3(nb), dim3(kNeedBlock), 0, st, P, A);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    N.unsettled = false;
     N.valid = true;
     N.d_last = 0.0;
     N.rebuilds++;
     return hipSuccess;
+}
+
+hipError_t nl_settle(mavi_sys *s, hipStream_t st) {
+    NList &N = *s->nl;
+    if (!N.unsettled) return hipSuccess;
+    // the last step drifted into px[(ph - 1) % 2]
+    hipLaunchKernelGGL(k_nl_settle, dim3(nblk(s->P.na, 256)), dim3(256), 0, st, s->P, N.px[(N.ph + 1) % 2],
+                       s->P_[s->cur]);
+    N.unsettled = false;
+    return hipGetLastError();
 }
 
 hipError_t nl_reduce_dmax(mavi_sys *s, hipStream_t st) {
@@ -709,20 +787,14 @@ hipError_t nl_step(mavi_sys *s, hipStream_t st) {
     const NLArgs A = nl_args(s);
     const int nb = nblk(P.na, kNeedBlock);
     {
-        ProfScope ps(s, MAVI_PROF_FORCE_REUSE, st);
-        if (P.potential == MAVI_POT_LJ) hipLaunchKernelGGL(k_nl_first<POT_LJ>, dim3(nb), dim3(kNeedBlock), 0, st, P, A);
-        else hipLaunchKernelGGL(k_nl_first<POT_HT>, dim3(nb), dim3(kNeedBlock), 0, st, P, A);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    {
         ProfScope ps(s, MAVI_PROF_FORCE, st);
         if (P.potential == MAVI_POT_LJ) hipLaunchKernelGGL(k_nl_second<POT_LJ>, dim3(nb), dim3(kNeedBlock), 0, st, P, A);
         else hipLaunchKernelGGL(k_nl_second<POT_HT>, dim3(nb), dim3(kNeedBlock), 0, st, P, A);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    N.kc ^= 1;
+    N.ph = (N.ph + 1) % 6;
+    N.unsettled = true;
     return hipSuccess;
 }
 
