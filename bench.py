@@ -7,14 +7,25 @@ density 0.7, jittered lattice (seed 1), Gaussian velocities T = 1, cutoff
 two force evaluations with stale chunks, velocity Verlet, periodic walls).
 The state is resident in HBM when the timed region starts.
 
-Prints ONE JSON line.  `roofline` is for the pair-force kernel's full
-evaluations (the second evaluation of every step): achieved = algorithmic
-bytes per launch / average launch duration measured with HIP events on the
-library's stream.  The first evaluation of a step reuses the previous
-step's second one wherever a wavefront's chunk blocks kept their members;
-its launches are reported apart (step_rates.first_eval_reuse_avg_us).  `cpu_baseline` is the CPU oracle (a restatement of the reference's
-loops, OpenMP over chunk columns like the reference's Threaded mode) timed
-on this host on a bounded sample of the same workload.
+The system runs on Verlet neighbour lists (the library's default for a
+finite-range plain system): one kernel per step, k_nl_second (the second
+force evaluation, the kick and walls, the next step's first evaluation and
+drift), plus a list rebuild every ~30 steps and a position write-back per
+block of 30 steps.
+
+Prints ONE JSON line.  `roofline` is for k_nl_second: achieved =
+algorithmic bytes per launch / average launch duration measured with HIP
+events on the library's stream.  Algorithmic bytes per particle: the state
+read and written (drifted position 16 + 16, velocity 16 + 16, forces 16 +
+16), the packed list (2 B per partner, mean rho pi (rc (1 + skin))^2
+partners) and its count (4 B).  `traffic` is the measured HBM bytes per
+launch from the committed rocprofv3 counter summary.  `cpu_baseline` is the
+CPU oracle (a restatement of the reference's loops, OpenMP over chunk
+columns like the reference's Threaded mode) timed on this host on a bounded
+sample of the same workload.  `forms.lj_gather` times the same workload on
+the chunk gather (lists off) and `forms.lj_reference` the reference's
+LenJonesCfg as it is (no cutoff: chunk-membership pair set, lists do not
+apply).
 """
 from __future__ import annotations
 
@@ -41,8 +52,16 @@ FP64_PEAK_TFLOPS = 78.6
 #          force-phase read of reordered x,y 16                 -> 152 B
 #   flops: 6.87 unique pairs within rc (0.5 rho pi rc^2) x 21 + 14 -> 158
 B_STEP = 152.0
-B_FORCE_LAUNCH = (152.0 - 40.0) / 2.0  # the two pair-force launches carry all but binning
 FLOP_STEP = 158.0
+NL_SKIN = 0.15  # the library's default list skin (fraction of the range)
+
+
+def nl_bytes_per_particle(density: float, rc: float) -> float:
+    """Algorithmic bytes per particle of one k_nl_second launch."""
+    import math
+
+    partners = density * math.pi * (rc * (1.0 + NL_SKIN)) ** 2
+    return 96.0 + 2.0 * partners + 4.0
 
 
 def log(msg):
@@ -234,6 +253,34 @@ def bench_experiments(args):
             "value": n * steps / wall, "unit": "particle-steps/s (files included)", "wall_s": wall,
             "experiments_done": done, "errors": sum(r is not None for r in res),
             "bare_batch_value": n * steps / bare, "runner_overhead": wall / bare}
+
+
+def bench_lj_variant(args, plain_value, kind):
+    """The headline workload on the chunk gather (lists off: one binning and
+    two pair-force launches per step) or as the reference's LenJonesCfg
+    expresses it (no cutoff: every pair in adjacent chunks interacts)."""
+    import mavi_hip as mv
+    from mavi_hip import workloads as wl
+
+    pos, vel, space, dyn, ic = wl.lj_periodic(nside=args.nside, cutoff=(kind == "gather"))
+    n = len(pos)
+    system = mv.System(state=mv.SecondLawState(pos=pos, vel=vel), space_cfg=space, dynamic_cfg=dyn, int_cfg=ic)
+    dev = system.dev
+    if kind == "gather":
+        dev.set_option(mv._lib.MAVI_OPT_NLIST, 0)
+    dev.step(args.warmup)
+    t0 = time.perf_counter()
+    dev.step(args.variant_steps)
+    wall = time.perf_counter() - t0
+    value = n * args.variant_steps / wall
+    info = dev.nlist_info()
+    wk = ("plain LJ as the headline (cutoff 2.5 sigma) on the chunk gather: binning every step, two pair-force "
+          "launches" if kind == "gather" else
+          "plain LJ, reference LenJonesCfg (no cutoff): the pair set is every pair in the same or adjacent "
+          "chunks of the 489x489 grid, as calc_forces! visits them")
+    return {"workload": wk, "particles": n, "steps": args.variant_steps, "value": value,
+            "unit": "particle-steps/s", "ms_per_step": wall / args.variant_steps * 1e3,
+            "ratio_vs_headline": value / plain_value, "neighbour_lists": info["active"]}
 
 
 def bench_walls(args, plain_value):
@@ -515,6 +562,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=48, help="steps of the CPU sample (all host cores)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--prof-steps", type=int, default=100)
+    ap.add_argument("--variant-steps", type=int, default=300,
+                    help="steps of the lj_gather and lj_reference forms (0 = skip)")
     ap.add_argument("--rings", type=int, default=40000, help="rings in the rings form (0 = skip)")
     ap.add_argument("--rings-steps", type=int, default=500)
     ap.add_argument("--rings-cpu-steps", type=int, default=40)
@@ -583,6 +632,7 @@ def main():
     value = n * args.steps / wall
     ms_per_step = wall / args.steps * 1e3
 
+    nl0 = dev.nlist_info()
     log(f"[bench] event-timed pass, {args.prof_steps} steps")
     dev.prof_enable(True)
     dev.step(args.prof_steps)
@@ -590,11 +640,13 @@ def main():
     r_ms, r_n = dev.prof_get(mv._lib.MAVI_PROF_FORCE_REUSE)
     b_ms, b_n = dev.prof_get(mv._lib.MAVI_PROF_BIN)
     dev.prof_enable(False)
+    nl1 = dev.nlist_info()
     f_avg_s = f_ms / max(f_n, 1) / 1e3
     r_avg_s = r_ms / max(r_n, 1) / 1e3
     b_avg_s = b_ms / max(b_n, 1) / 1e3
-    achieved = B_FORCE_LAUNCH * n / f_avg_s / 1e9
-    traffic = read_traffic("pair_force_lj_1M_traffic.txt")
+    b_launch = nl_bytes_per_particle(0.7, 2.5)
+    achieved = b_launch * n / f_avg_s / 1e9
+    traffic = read_traffic("nl_second_lj_1M_traffic.txt")
     step_gbs = B_STEP * n / (wall / args.steps) / 1e9
     fp64_tflops = FLOP_STEP * n / (wall / args.steps) / 1e12
 
@@ -620,24 +672,33 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"plain LJ, {n} particles, periodic square box, density 0.7, cutoff 2.5 sigma, "
+        "config": {"workload": f"plain LJ, {n} particles, periodic square box, density 0.7, cutoff 2.5 sigma "
+                               "(LenJonesCfg cutoff: an extension of this library; the reference's LenJonesCfg has "
+                               "none, see forms.lj_reference), "
                                f"{ic.chunks_cfg.num_cols}x{ic.chunks_cfg.num_rows} chunks, dt 1e-3, "
-                               "newton_step! (velocity Verlet, 2 force evaluations per step)",
+                               "newton_step! (velocity Verlet, 2 force evaluations per step), Verlet neighbour lists",
                    "particles": n, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_pair, full evaluations (pair forces + Verlet/wall epilogue)",
+                     "kernel": "k_nl_second (second evaluation, kick, walls, next first evaluation and drift)",
                      "avg_launch_us": f_avg_s * 1e6, "launches": f_n,
-                     "bytes_per_launch_alg": B_FORCE_LAUNCH * n},
+                     "bytes_per_particle_alg": b_launch, "bytes_per_launch_alg": b_launch * n},
         "step_rates": {"alg_GBs": step_gbs, "fp64_alg_TFLOPs": fp64_tflops,
-                       "fp64_frac": fp64_tflops / FP64_PEAK_TFLOPS, "bin_avg_us": b_avg_s * 1e6,
-                       "first_eval_reuse_avg_us": r_avg_s * 1e6, "first_eval_reuse_launches": r_n},
+                       "fp64_frac": fp64_tflops / FP64_PEAK_TFLOPS,
+                       "rebuild_launch_avg_us": b_avg_s * 1e6, "rebuild_launches": b_n,
+                       "start_eval_avg_us": r_avg_s * 1e6, "start_eval_launches": r_n,
+                       "list_rebuilds_per_1000_steps": 1000.0 * (nl1["rebuilds"] - nl0["rebuilds"]) / args.prof_steps,
+                       "list_rollbacks": nl1["rollbacks"], "neighbour_lists": nl1["active"]},
         "cpu_baseline": cpu,
     }
     if cpu:
         out["speedup_vs_cpu"] = value / cpu["value"]
     forms = {}
     del system, dev
+    if args.variant_steps > 0:
+        log("[bench] chunk-gather and reference (no cutoff) forms")
+        forms["lj_gather"] = bench_lj_variant(args, value, "gather")
+        forms["lj_reference"] = bench_lj_variant(args, value, "reference")
     if args.rings > 0:
         forms["rings"] = bench_rings(args)
         if args.sources_steps > 0:
