@@ -1004,8 +1004,12 @@ double or_rings_area(const or_sys *s, const double *cpos, int32_t ring) {
  * binned (cp/counts NULL without chunks). */
 static void rings_forces_binned(const or_sys *s, const double *pos, const double *cpos, const int64_t *cp,
                                 const int64_t *counts, double *forces, double *areas) {
-    or_calc_forces(s, pos, cp, counts, forces, NULL);
-    /* springs, :205-223 */
+    const int nt = s->nthreads > 1 ? s->nthreads : 1;
+    /* calc_forces!, Threaded mode over chunk columns when nthreads > 1 */
+    if (nt > 1) or_calc_forces_threaded(s, pos, cp, counts, forces, nt);
+    else or_calc_forces(s, pos, cp, counts, forces, NULL);
+    /* springs, :205-223 (each ring writes only its own particles) */
+#pragma omp parallel for num_threads(nt) schedule(static) if (nt > 1)
     for (int32_t r = 1; r <= s->nrings; r++) {
         if (!ring_active(s, r)) continue;
         int32_t np = ring_np(s, r);
@@ -1027,6 +1031,7 @@ static void rings_forces_binned(const or_sys *s, const double *pos, const double
         }
     }
     /* area_forces!, :140-195 */
+#pragma omp parallel for num_threads(nt) schedule(static) if (nt > 1)
     for (int32_t r = 1; r <= s->nrings; r++) {
         if (!ring_active(s, r)) continue;
         int32_t np = ring_np(s, r);
@@ -1233,7 +1238,10 @@ int or_rings_steps(const or_sys *s, double *pos, double *pol, int64_t nsteps, co
             for (int64_t k = 0; k < nids; k++) or_wall_forces(s, pos, ids[k] - 1, forces + 2 * (ids[k] - 1));
             free(ids);
         }
-        /* update!, src/rings/integration.jl:300-351, over get_rings_ids */
+        /* update!, src/rings/integration.jl:300-351, over get_rings_ids (rings
+         * independent: per-ring threads when nthreads > 1) */
+        const int nt = s->nthreads > 1 ? s->nthreads : 1;
+#pragma omp parallel for num_threads(nt) schedule(static) if (nt > 1)
         for (int32_t r = 1; r <= s->nrings; r++) {
             if (!ring_active(s, r)) continue;
             int32_t np = ring_np(s, r);

@@ -101,7 +101,7 @@ typedef struct {
     double dt;
     int32_t cols, rows; /* ChunksCfg; cols == 0 -> chunks = nothing (all pairs) */
     int32_t nc;         /* chunk capacity (src/chunks.jl:32-34) */
-    int32_t pad0;
+    int32_t nthreads;   /* rings steps: threads of calc_forces! (Threaded mode) and the per-ring loops; <= 1: one */
     double cl, ch;      /* chunk_length, chunk_height (src/chunks.jl:29-30) */
     double radius;      /* particle_radius(dynamic_cfg) (src/configs.jl:418-421) */
     /* Rings (kind == OR_KIND_RINGS) */

@@ -88,7 +88,7 @@ class OrSys(C.Structure):
         ("cols", C.c_int32),
         ("rows", C.c_int32),
         ("nc", C.c_int32),
-        ("pad0", C.c_int32),
+        ("nthreads", C.c_int32),
         ("cl", C.c_double),
         ("ch", C.c_double),
         ("radius", C.c_double),
@@ -413,7 +413,11 @@ class Sys:
         return (d["mask"].copy(), d["ids"].copy(), d["uids"].copy(), int(v.num_active), d["cms"].copy(),
                 int(v.pol_used))
 
-    def rings_steps(self, pos, pol, nsteps, noise=None):
+    def rings_steps(self, pos, pol, nsteps, noise=None, nthreads=1):
+        """nthreads > 1: calc_forces! in Threaded mode (per-thread force slices
+        summed in thread order, src/integration.jl:160-193) and per-ring
+        threads for the springs, area forces and update."""
+        self.s.nthreads = int(nthreads)
         pos = np.array(pos, dtype=np.float64, order="C")
         pol = np.array(pol, dtype=np.float64, order="C")
         if noise is not None:
