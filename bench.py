@@ -534,10 +534,32 @@ def bench_learned(args):
     _, g_plain2, seg_plain2, _, _ = run(False, 1002)
     _, g_learn, seg_learn, v_learn, acc = run(True, 1003)
     gate_stats = dict(stats)
+
+    # the gate where it fires: the trained network with its output layer
+    # (the last 2 x 128 weights and 2 biases) scaled up, from the same state
+    fires = {}
+    p0 = net.params
+    for scale in (30.0, 300.0):
+        p = p0.copy()
+        p[-(2 * 128 + 2):] *= scale
+        net.params = p
+        st3, sp3, cfg3, ic3 = wl.rings_mixture(nrings=args.learned_rings, seed=seed, chunk=1.25)
+        s3 = rings.RingsSystem(state=st3, space_cfg=sp3, dynamic_cfg=cfg3, int_cfg=ic3, seed=1004, ctx=ctx)
+        s3.dev.upload(pos0, None, pol0)
+        lp3 = learned.Learned(s3.dev, net, k=k, r_feat=1.25, target_scale=1e-3, seed=1004)
+        nl3 = 40
+        t0 = time.perf_counter()
+        a3 = lp3.run(nl3)
+        w3 = time.perf_counter() - t0
+        fires[f"output_x{scale:g}"] = {"learned_steps": nl3, "accepted": a3, "gate": lp3.stats(),
+                                       "value": n * nl3 * k / w3, "unit": "particle-steps/s (incl. graph capture)"}
+        del lp3, s3
+    net.params = p0
     us_mlp = net.time_forward(n, 20)
     flop = 2.0 * n * (160 * 128 + 128 * 128 * 2 + 128 * 2)
     return {
         "gate": gate_stats,
+        "gate_fires": fires,
         "workload": f"active rings binary mixture, {args.learned_rings} rings x 20 = {n} particles (ring physics of "
                     "the 800k mixture, chunks >= 1.25), k = 10 plain steps per learned step, held-out seed 199",
         "particles": n, "k": k, "value": v_learn, "unit": "particle-steps/s",

@@ -79,23 +79,6 @@ __device__ __forceinline__ bool key_less(double da, int ia, double db, int ib) {
     return da < db || (da == db && ia < ib);
 }
 
-// Overdamped velocity u = v0 (cos th, sin th) + mu F of every particle
-// (update!, src/rings/integration.jl:300-351), in sorted-slot order.
-// Also clears the gate word of the step (no kernel before it sets it).
-__global__ void k_velocity(SysParams P, const int32_t *__restrict__ ID, const double2 *__restrict__ F,
-                           const double *__restrict__ pol, const int32_t *__restrict__ rtype,
-                           const RingType *__restrict__ types, double2 *__restrict__ U, uint32_t *__restrict__ gate) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s == 0) gate[0] = 0u;
-    if (s >= P.na) return;
-    const int ring = ID[s] / P.nmax;
-    const RingType &T = types[P.typed ? rtype[ring] : 0];
-    double sn, cs;
-    sincos(pol[ring], &sn, &cs);
-    const double2 f = F[s];
-    U[s] = make_double2(T.vo * cs + T.mu * f.x, T.vo * sn + T.mu * f.y);
-}
-
 // Sum over the 32 lanes of a half-wave (tree order).
 __device__ __forceinline__ double half_sum(double v) {
 #pragma unroll
@@ -103,15 +86,21 @@ __device__ __forceinline__ double half_sum(double v) {
     return v;
 }
 
-// Per ring (32 lanes, lane i = particle i): the area rate
-// dA/dt = 1/2 sum_i cross(u_i, x_{i+1} - x_{i-1}) over A0, and the alignment
-// term asin(p x v_cm / |v_cm|) of update!.
+// Per ring (32 lanes, lane i = particle i): the overdamped velocity
+// u = v0 (cos th, sin th) + mu F of each particle (update!,
+// src/rings/integration.jl:300-351, kept by sorted slot for the features),
+// the area rate dA/dt = 1/2 sum_i cross(u_i, x_{i+1} - x_{i-1}) over A0, and
+// the alignment term asin(p x v_cm / |v_cm|) of update!.  Every slot is a
+// ring particle (full rings).  Also clears the gate word of the step (no
+// kernel before it sets it).
 __global__ __launch_bounds__(256) void k_ring_rates(SysParams P, const int32_t *__restrict__ slot_of,
-                                                    const double2 *__restrict__ Pst, const double2 *__restrict__ U,
-                                                    const double *__restrict__ pol, const int32_t *__restrict__ rtype,
+                                                    const double2 *__restrict__ Pst, const double2 *__restrict__ F,
+                                                    double2 *__restrict__ U, const double *__restrict__ pol,
+                                                    const int32_t *__restrict__ rtype,
                                                     const RingType *__restrict__ types, double *__restrict__ rate,
-                                                    double *__restrict__ omega) {
+                                                    double *__restrict__ omega, uint32_t *__restrict__ gate) {
     const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gt == 0) gate[0] = 0u;
     const int r = gt >> 5, i = gt & 31;
     const bool ring_ok = r < P.nrings;
     const int rr = ring_ok ? r : 0;
@@ -119,7 +108,14 @@ __global__ __launch_bounds__(256) void k_ring_rates(SysParams P, const int32_t *
     const int np = T.np;
     const bool act = i < np;
     const int slot = act ? slot_of[(int64_t)rr * P.nmax + i] : 0;
-    const double2 u = act ? U[slot] : make_double2(0.0, 0.0);
+    double sn, cs;
+    sincos(pol[rr], &sn, &cs);
+    double2 u = make_double2(0.0, 0.0);
+    if (act) {
+        const double2 f = F[slot];
+        u = make_double2(T.vo * cs + T.mu * f.x, T.vo * sn + T.mu * f.y);
+        if (ring_ok) U[slot] = u;
+    }
     const double2 x = act ? Pst[slot] : make_double2(0.0, 0.0);
     const int inext = (i == np - 1) ? 0 : i + 1, iprev = (i == 0) ? np - 1 : i - 1;
     const double2 xn = make_double2(__shfl(x.x, inext, 32), __shfl(x.y, inext, 32));
@@ -131,8 +127,6 @@ __global__ __launch_bounds__(256) void k_ring_rates(SysParams P, const int32_t *
     double area0 = (double)np * T.lspring / T.p0;
     area0 = area0 * area0;
     rate[rr] = da / area0;
-    double sn, cs;
-    sincos(pol[rr], &sn, &cs);
     const double speed = sqrt(vx * vx + vy * vy);
     double cr = 0.0;
     if (speed != 0.0) {
@@ -375,12 +369,15 @@ __global__ __launch_bounds__(kFeatWG) void k_features(
 }
 
 // Candidate positions x' = x + displacement (then the periodic wrap of
-// walls!, src/integration.jl:309-324).
+// walls!, src/integration.jl:309-324).  Advances the step counter by k ahead
+// of k_commit, which draws at counter - k and takes the advance back when
+// the step is not committed.
 __global__ void k_apply(SysParams P, LearnParams L, const double2 *__restrict__ Pst, const int32_t *__restrict__ ID,
                         const double *__restrict__ pol, const double2 *__restrict__ euler,
                         const float *__restrict__ Y, double2 *__restrict__ P2, double2 *__restrict__ disp,
-                        uint32_t *__restrict__ gate) {
+                        unsigned long long *__restrict__ counter, uint32_t *__restrict__ gate) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s == 0) counter[0] += (unsigned long long)L.k;
     if (s >= P.na) return;
     const int id = ID[s];
     const int ring = id / P.nmax;
@@ -414,10 +411,9 @@ __global__ void k_apply(SysParams P, LearnParams L, const double2 *__restrict__ 
 
 // Overlap gate over the listed pairs (each pair once, from its smaller id;
 // the list of a particle ends at the first -1 or after 32 entries).
-__global__ void k_gate_pairs(SysParams P, LearnParams L, const int32_t *__restrict__ ID,
-                             const int32_t *__restrict__ nbr, const double2 *__restrict__ P2,
-                             uint32_t *__restrict__ gate) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void gate_pairs(const SysParams &P, const LearnParams &L, int s,
+                                           const int32_t *__restrict__ ID, const int32_t *__restrict__ nbr,
+                                           const double2 *__restrict__ P2, uint32_t *__restrict__ gate) {
     if (s >= P.na) return;
     const int id = ID[s];
     const double2 xi = P2[s];
@@ -474,12 +470,20 @@ __device__ __forceinline__ double ring_energy(const SysParams &P, const RingType
     return half_sum(es) + 0.5 * T.karea * (area - area0) * (area - area0);
 }
 
-__global__ __launch_bounds__(256) void k_gate_rings(SysParams P, LearnParams L, const int32_t *__restrict__ rtype,
-                                                    const RingType *__restrict__ types,
-                                                    const int32_t *__restrict__ slot_of,
-                                                    const double2 *__restrict__ Pst, const double2 *__restrict__ P2,
-                                                    uint32_t *__restrict__ gate) {
-    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+// The gates of the candidate positions in one launch: workgroups [0, nbp)
+// test the listed pairs for overlap (one thread per slot), the rest the
+// spring + area energy of each ring (32 lanes per ring).
+__global__ __launch_bounds__(256) void k_gate(SysParams P, LearnParams L, int nbp, const int32_t *__restrict__ ID,
+                                              const int32_t *__restrict__ nbr, const int32_t *__restrict__ rtype,
+                                              const RingType *__restrict__ types,
+                                              const int32_t *__restrict__ slot_of,
+                                              const double2 *__restrict__ Pst, const double2 *__restrict__ P2,
+                                              uint32_t *__restrict__ gate) {
+    if ((int)blockIdx.x < nbp) {
+        gate_pairs(P, L, blockIdx.x * blockDim.x + threadIdx.x, ID, nbr, P2, gate);
+        return;
+    }
+    const int gt = (blockIdx.x - nbp) * blockDim.x + threadIdx.x;
     const int r = gt >> 5;
     const bool ring_ok = r < P.nrings;
     const int rr = ring_ok ? r : 0;
@@ -492,19 +496,16 @@ __global__ __launch_bounds__(256) void k_gate_rings(SysParams P, LearnParams L, 
 
 // Accepted step: positions (blocks [0, nbp), one thread per slot) and
 // polarisations (the remaining blocks, one thread per ring; one normal per
-// ring from the device stream at the current step counter).
-__global__ void k_commit(SysParams P, LearnParams L, int nbp, const int32_t *__restrict__ rtype,
-                         const RingType *__restrict__ types, const double2 *__restrict__ P2,
-                         double2 *__restrict__ Pst, const double2 *__restrict__ disp, double *__restrict__ pol,
-                         const unsigned long long *__restrict__ counter, const uint32_t *__restrict__ gate) {
-    if (gate[0] != 0) return;
-    if ((int)blockIdx.x < nbp) {
-        const int s = blockIdx.x * blockDim.x + threadIdx.x;
-        if (s < P.na) Pst[s] = P2[s];
-        return;
-    }
-    const int r = (blockIdx.x - nbp) * blockDim.x + threadIdx.x;
-    if (r >= P.nrings) return;
+// ring from the device stream at the step counter before k_apply's advance),
+// and ctl[1] (accepted steps) up by one.  A rejected step stores its gate
+// word in ctl[0]; while ctl[0] is set, later steps of the same batch commit
+// nothing (the host runs the plain fallback there).  A step not committed
+// takes k_apply's counter advance back (one thread; no workgroup reads the
+// counter then).
+__device__ __forceinline__ void commit_pol(const SysParams &P, const LearnParams &L, int r,
+                                           const int32_t *__restrict__ rtype, const RingType *__restrict__ types,
+                                           const double2 *__restrict__ disp, double *__restrict__ pol,
+                                           unsigned long long step) {
     const RingType T = types[P.typed ? rtype[r] : 0];
     double vx = 0.0, vy = 0.0;
     for (int i = 0; i < T.np; i++) {
@@ -523,12 +524,37 @@ __global__ void k_commit(SysParams P, LearnParams L, int nbp, const int32_t *__r
         cr = (cs * vy - sn * vx) / speed;
         if (fabs(cr) > 1.0) cr = cr > 0.0 ? 1.0 : -1.0;
     }
-    const double z = T.rot != 0.0 ? philox_normal(L.seed, counter[0], (uint32_t)r) : 0.0;
+    const double z = T.rot != 0.0 ? philox_normal(L.seed, step, (uint32_t)r) : 0.0;
     pol[r] = theta + (L.kdt / T.relax * asin(cr) + sqrt(2.0 * T.rot * L.kdt) * z);
 }
 
-__global__ void k_counter_add(unsigned long long *counter, uint32_t k, const uint32_t *__restrict__ gate) {
-    if (threadIdx.x == 0 && blockIdx.x == 0 && gate[0] == 0) counter[0] += k;
+__global__ __launch_bounds__(256) void k_commit(SysParams P, LearnParams L, int nbp,
+                                                const int32_t *__restrict__ rtype,
+                                                const RingType *__restrict__ types, const double2 *__restrict__ P2,
+                                                double2 *__restrict__ Pst, const double2 *__restrict__ disp,
+                                                double *__restrict__ pol, unsigned long long *__restrict__ counter,
+                                                uint32_t *__restrict__ ctl, const uint32_t *__restrict__ gate) {
+    const bool first = blockIdx.x == 0 && threadIdx.x == 0;
+    if (ctl[0] != 0) {  // an earlier step of this batch was rejected
+        if (first) counter[0] -= (unsigned long long)L.k;
+        return;
+    }
+    const uint32_t g = gate[0];
+    if (g != 0) {  // one word: every workgroup leaves here
+        if (first) {
+            ctl[0] = g;
+            counter[0] -= (unsigned long long)L.k;
+        }
+        return;
+    }
+    if (first) ctl[1] += 1u;
+    if ((int)blockIdx.x < nbp) {
+        const int s = blockIdx.x * blockDim.x + threadIdx.x;
+        if (s < P.na) Pst[s] = P2[s];
+    } else {
+        const int r = (blockIdx.x - nbp) * blockDim.x + threadIdx.x;
+        if (r < P.nrings) commit_pol(P, L, r, rtype, types, disp, pol, counter[0] - (unsigned long long)L.k);
+    }
 }
 
 // Data collection: positions by id at the snapshot, then targets after k steps.
@@ -593,6 +619,9 @@ struct mavi_learned {
     double *rate = nullptr, *omega = nullptr;
     double2 *U = nullptr;
     uint32_t *gate = nullptr;
+    uint32_t *ctl = nullptr;        // per batch of learned steps: [0] first rejection's gate word, [1] accepted
+    uint32_t *ctl_host = nullptr;   // pinned copy of ctl
+    int batch = 8;                  // learned steps per host check (halved after a rejection, up to 16)
     uint32_t *gate_host = nullptr;  // pinned
     hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [start copy][use_net]
     int64_t accepted = 0, rejected = 0;
@@ -606,10 +635,8 @@ hipError_t launch_features(mavi_learned *l, hipStream_t st) {
     hipError_t e = mavi::launch_rings_forces(s, st);
     if (e != hipSuccess) return e;
     const int c = s->cur;
-    hipLaunchKernelGGL(k_velocity, dim3(nblk(s->P.na, 256)), dim3(256), 0, st, s->P, s->ID_[c], s->F, s->pol,
-                       s->rtype, s->types, l->U, l->gate);
     hipLaunchKernelGGL(k_ring_rates, dim3(nblk((int64_t)s->P.nrings * 32, 256)), dim3(256), 0, st, s->P, s->slot_of,
-                       s->P_[c], l->U, s->pol, s->rtype, s->types, l->rate, l->omega);
+                       s->P_[c], s->F, l->U, s->pol, s->rtype, s->types, l->rate, l->omega, l->gate);
     hipLaunchKernelGGL(k_features, dim3(nblk(s->P.na, kFeatWG)), dim3(kFeatWG), 0, st, s->P, l->L, s->P_[c],
                        s->ID_[c], s->slot_of, s->scell, s->start, l->U, s->pol, s->areas, s->rtype, s->types, l->rate,
                        l->omega, l->X, l->nbr, l->euler, l->gate);
@@ -625,13 +652,11 @@ hipError_t launch_learned_step(mavi_learned *l, hipStream_t st) {
     const int c = s->cur;
     const int nb = nblk(s->P.na, 256);
     hipLaunchKernelGGL(k_apply, dim3(nb), dim3(256), 0, st, s->P, l->L, s->P_[c], s->ID_[c], s->pol, l->euler, l->Y,
-                       s->P2, l->disp, l->gate);
-    hipLaunchKernelGGL(k_gate_pairs, dim3(nb), dim3(256), 0, st, s->P, l->L, s->ID_[c], l->nbr, s->P2, l->gate);
-    hipLaunchKernelGGL(k_gate_rings, dim3(nblk((int64_t)s->P.nrings * 32, 256)), dim3(256), 0, st, s->P, l->L,
-                       s->rtype, s->types, s->slot_of, s->P_[c], s->P2, l->gate);
+                       s->P2, l->disp, s->counter, l->gate);
+    hipLaunchKernelGGL(k_gate, dim3(nb + nblk((int64_t)s->P.nrings * 32, 256)), dim3(256), 0, st, s->P, l->L, nb,
+                       s->ID_[c], l->nbr, s->rtype, s->types, s->slot_of, s->P_[c], s->P2, l->gate);
     hipLaunchKernelGGL(k_commit, dim3(nb + nblk(s->P.nrings, 256)), dim3(256), 0, st, s->P, l->L, nb, s->rtype,
-                       s->types, s->P2, s->P_[c], l->disp, s->pol, s->counter, l->gate);
-    hipLaunchKernelGGL(k_counter_add, dim3(1), dim3(64), 0, st, s->counter, (uint32_t)l->L.k, l->gate);
+                       s->types, s->P2, s->P_[c], l->disp, s->pol, s->counter, l->ctl, l->gate);
     return hipGetLastError();
 }
 
@@ -722,6 +747,8 @@ int mavi_learned_create(mavi_sys *s, mavi_mlp *mlp, const mavi_learned_desc *d, 
     if (e == hipSuccess) e = dalloc(&l->U, n, st);
     if (e == hipSuccess) e = dalloc(&l->omega, P.nrings, st);
     if (e == hipSuccess) e = dalloc(&l->gate, 1, st);
+    if (e == hipSuccess) e = dalloc(&l->ctl, 2, st);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&l->ctl_host, 2 * sizeof(uint32_t), hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void **)&l->gate_host, sizeof(uint32_t), hipHostMallocDefault);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
@@ -744,8 +771,9 @@ int mavi_learned_destroy(mavi_learned *l) {
         for (auto &g : gr)
             if (g) (void)hipGraphExecDestroy(g);
     dfree(l->X); dfree(l->Y); dfree(l->T); dfree(l->nbr); dfree(l->euler); dfree(l->disp); dfree(l->pos0);
-    dfree(l->pol0); dfree(l->rate); dfree(l->omega); dfree(l->U); dfree(l->gate);
+    dfree(l->pol0); dfree(l->rate); dfree(l->omega); dfree(l->U); dfree(l->gate); dfree(l->ctl);
     if (l->gate_host) (void)hipHostFree(l->gate_host);
+    if (l->ctl_host) (void)hipHostFree(l->ctl_host);
     mavi_sys *s = l->s;
     mavi_mlp *mlp = l->mlp;
     const bool held = l->held;
@@ -823,29 +851,42 @@ int mavi_learned_run(mavi_learned *l, int64_t nmacro, int32_t use_net, int64_t *
     hipStream_t st = ctx->stream;
     l->L.use_net = use_net != 0;
     const int un = use_net ? 1 : 0;
-    int64_t acc = 0;
-    for (int64_t q = 0; q < nmacro; q++) {
+    // Learned steps run in batches with one read back each: a rejected step
+    // leaves its gate word in ctl[0] and the batch's later steps commit
+    // nothing, so the plain fallback runs exactly where the step-by-step
+    // loop (src/rings/integration.jl step! x k on rejection) would run it.
+    int64_t acc = 0, q = 0;
+    while (q < nmacro) {
+        const int64_t m = std::min<int64_t>(l->batch, nmacro - q);
         // a learned step moves the particles outside the kernels that keep
         // the chunk keys: its binning starts from scratch (also when captured)
         HIPCHK(ctx, mavi::drop_keys(s, st));
-        if (s->use_graphs && !s->prof) {
-            hipGraphExec_t &g = l->graph[s->cur][un];
-            if (!g) HIPCHK(ctx, learned_graph(l, st, &g));
-            HIPCHK(ctx, hipGraphLaunch(g, st));
-            s->cur = 1 - s->cur;  // the step's binning
-        } else {
-            HIPCHK(ctx, launch_learned_step(l, st));
+        HIPCHK(ctx, mavi::launch_fill_i32((int32_t *)l->ctl, 0, 2, st));
+        for (int64_t j = 0; j < m; j++) {
+            if (s->use_graphs && !s->prof) {
+                hipGraphExec_t &g = l->graph[s->cur][un];
+                if (!g) HIPCHK(ctx, learned_graph(l, st, &g));
+                HIPCHK(ctx, hipGraphLaunch(g, st));
+                s->cur = 1 - s->cur;  // the step's binning
+            } else {
+                HIPCHK(ctx, launch_learned_step(l, st));
+            }
         }
-        HIPCHK(ctx, hipMemcpyAsync(l->gate_host, l->gate, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(ctx, hipMemcpyAsync(l->ctl_host, l->ctl, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         HIPCHK(ctx, hipStreamSynchronize(st));
-        const uint32_t gw = *l->gate_host;
+        const uint32_t gw = l->ctl_host[0];
+        const int64_t na = (int64_t)l->ctl_host[1];
+        acc += na;
+        q += na;
         if (gw == 0) {
-            acc++;
-        } else {
-            for (int b = 0; b < 5; b++)
-                if (gw & (1u << b)) l->reasons[b]++;
-            for (int k = 0; k < l->L.k; k++) HIPCHK(ctx, mavi::launch_rings_step(s, nullptr, st));
+            l->batch = std::min(16, 2 * l->batch);
+            continue;
         }
+        for (int b = 0; b < 5; b++)
+            if (gw & (1u << b)) l->reasons[b]++;
+        for (int k = 0; k < l->L.k; k++) HIPCHK(ctx, mavi::launch_rings_step(s, nullptr, st));
+        q += 1;
+        l->batch = std::max(1, l->batch / 2);
     }
     l->accepted += acc;
     l->rejected += nmacro - acc;

@@ -33,12 +33,22 @@ __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint
 
 // Round to nearest even (v_cvt_pk_bf16_f32).
 __device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+// Two values to packed BF16 (a in the low half), one v_cvt_pk_bf16_f32.
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t f2bf2(float a, float b) {
+    typedef float f32x2_t __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2_t{a, b}, bf16x2_t));
+}
 
 // GELU in its sigmoid form, g(z) = z sigma(1.702 z) (Hendrycks & Gimpel):
-// one hardware exp and one reciprocal per activation (a few ulp in FP32, far
-// below the BF16 rounding that follows).  g'(z) = s + 1.702 z s (1 - s).
+// one hardware exp2 (of z times -1.702 log2 e folded into one constant) and
+// one reciprocal per activation (a few ulp in FP32, far below the BF16
+// rounding that follows).  g'(z) = s + 1.702 z s (1 - s).
 constexpr float kGs = 1.702f;
-__device__ __forceinline__ float sig(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-kGs * z)); }
+constexpr float kGsLog2e = -1.702f * 1.44269504088896341f;
+__device__ __forceinline__ float sig(float z) {
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z * kGsLog2e));
+}
 __device__ __forceinline__ float gelu(float z) { return z * sig(z); }
 __device__ __forceinline__ float gelu_grad(float z) {
     const float s = sig(z);
@@ -304,8 +314,8 @@ __device__ __forceinline__ void fused_hidden(const uint16_t *__restrict__ X, int
 #pragma unroll
     for (int f = 0; f < 8; f++) {
         const float4 bb = *reinterpret_cast<const float4 *>(bias + 16 * f + n4);
-        const uint32_t lo = (uint32_t)f2bf(gelu(acc[f][0] + bb.x)) | ((uint32_t)f2bf(gelu(acc[f][1] + bb.y)) << 16);
-        const uint32_t hi = (uint32_t)f2bf(gelu(acc[f][2] + bb.z)) | ((uint32_t)f2bf(gelu(acc[f][3] + bb.w)) << 16);
+        const uint32_t lo = f2bf2(gelu(acc[f][0] + bb.x), gelu(acc[f][1] + bb.y));
+        const uint32_t hi = f2bf2(gelu(acc[f][2] + bb.z), gelu(acc[f][3] + bb.w));
         *reinterpret_cast<uint2 *>(act + fr * kPH + 16 * f + n4) = make_uint2(lo, hi);
     }
     wave_sync();

@@ -131,6 +131,37 @@ def test_rejected_steps_equal_plain_steps(gpu_ctx):
     assert np.array_equal(a.state.pol, b.state.pol)
 
 
+def test_batched_run_equals_step_by_step(gpu_ctx):
+    """Learned steps run in batches with one host check each; a rejection
+    leaves the rest of its batch uncommitted and the plain fallback runs in
+    its place.  One call of 24 steps equals 24 calls of one step bit for
+    bit, with a displacement gate just under the first step's largest
+    displacement (so the steps after it are accepted or rejected as the state
+    moves)."""
+    from mavi_hip import learned
+
+    a, _ = small_rings(nrings=100, seed=11)
+    b, _ = small_rings(nrings=100, seed=11)
+    a.advance(20)
+    b.advance(20)
+    probe = learned.Learned(a.dev, None, k=K, r_feat=RF, seed=5)
+    _, e = probe.features()
+    del probe
+    dmax = 0.98 * float(np.sqrt((e ** 2).sum(axis=1)).max())
+    la = learned.Learned(a.dev, None, k=K, r_feat=RF, max_disp=dmax, seed=5)
+    lb = learned.Learned(b.dev, None, k=K, r_feat=RF, max_disp=dmax, seed=5)
+    acc_a = la.run(24, use_net=False)
+    acc_b = sum(lb.run(1, use_net=False) for _ in range(24))
+    a.download()
+    b.download()
+    say(f"[learned batches] accepted {acc_a}/24 (step by step {acc_b}/24), gate at {dmax:.4g}")
+    assert acc_a == acc_b
+    assert np.array_equal(a.state.pos, b.state.pos)
+    assert np.array_equal(a.state.pol, b.state.pol)
+    if not 0 < acc_a < 24:
+        pytest.skip(f"equal runs, but the gate accepted {acc_a} of 24: no mix of outcomes to compare")
+
+
 def test_learned_steps_stay_close_to_plain(gpu_ctx):
     """Without a network the gated step is the explicit k dt step: accepted,
     and a short trajectory stays within a small distance of the plain one."""
