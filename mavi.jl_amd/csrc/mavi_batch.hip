@@ -13,11 +13,23 @@
 // concatenated too (nbase + local id), and ring arrays (rbase, even) so that
 // a wavefront of the ring kernel holds two rings of one replica.
 //
-// A step is: binning of all replicas (5 launches), one pair launch per kind
-// and evaluation (VERLET1 / VERLET2 for LJ and HarmTrunc replicas, FORCES for
+// A step is: binning of all replicas, one pair launch per kind and
+// evaluation (VERLET1 / VERLET2 for LJ and HarmTrunc replicas, FORCES for
 // rings), and the per-ring kernel for rings replicas.  Each replica runs the
 // same device functions as a single system (mavi_device.h), so its results
 // are bit-identical to mavi_step on the same descriptor.
+//
+// Force reuse, as for a single system: the second evaluation of an LJ or
+// HarmTrunc replica does the next binning's first pass on the positions it
+// writes (chunk, arrival rank) and marks, in the next binning's epoch, the
+// chunks of the 3x3 blocks around a particle's old and new chunk when it
+// changes chunk.  The binning then counts only the rings replicas' slots,
+// writes the permutation instead of moving velocities, and the first
+// evaluation moves velocities through it and takes the previous second
+// evaluation wherever its chunk is not marked (same positions, same
+// candidates, same order: the same bits).  Particles to recompute are
+// handled per wave (a workgroup may hold waves of other replicas).  A
+// replica whose state was replaced recomputes everything once (rvalid).
 #include <math.h>
 
 #include <algorithm>
@@ -49,14 +61,25 @@ enum { KIND_LJ = 0, KIND_HT = 1, KIND_RINGS = 2, NKIND = 3 };
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Chunk and arrival rank of every padded slot.  Padding slots go to their
-// replica's pad chunk.  nslots is a multiple of 64: whole waves leave.
+// Chunk and arrival rank of every padded slot (items == nullptr), or of the
+// slots of the waves items[0, nitems).  Padding slots go to their replica's
+// pad chunk.  nslots is a multiple of 64: whole waves leave.
 __global__ void kb_bin_count(const Replica *__restrict__ reps, const int32_t *__restrict__ wave_rep, int32_t nslots,
+                             const int2 *__restrict__ items, int32_t nitems,
                              const double2 *__restrict__ pos, int32_t *__restrict__ key, int32_t *__restrict__ rank,
                              int32_t *__restrict__ cnt, uint32_t *__restrict__ flags) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nslots) return;
-    const int r = uniform(wave_rep[s >> 6]);
+    int s, r;
+    if (items) {
+        const int wl = blockIdx.x * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);
+        if (wl >= nitems) return;
+        const int2 it = items[wl];
+        s = uniform(it.x) * 64 + (int)lane_id();
+        r = uniform(it.y);
+    } else {
+        s = blockIdx.x * blockDim.x + threadIdx.x;
+        if (s >= nslots) return;
+        r = uniform(wave_rep[s >> 6]);
+    }
     const Replica &Q = reps[r];
     uint32_t flag = 0;
     int32_t k;
@@ -77,11 +100,12 @@ __global__ void kb_bin_count(const Replica *__restrict__ reps, const int32_t *__
 __global__ void kb_finalize(const Replica *__restrict__ reps, const int32_t *__restrict__ wave_rep, int32_t nslots,
                             const int32_t *__restrict__ start, const int32_t *__restrict__ src,
                             const int32_t *__restrict__ scell, const int32_t *__restrict__ sid,
-                            const double2 *__restrict__ Pold,
-                            const double2 *__restrict__ Vold, const int32_t *__restrict__ IDold,
-                            double2 *__restrict__ Pnew, double2 *__restrict__ Vnew, int32_t *__restrict__ IDnew,
-                            int32_t *__restrict__ slot_of, uint32_t *__restrict__ flags) {
+                            const double2 *__restrict__ Pold, const int32_t *__restrict__ IDold,
+                            double2 *__restrict__ Pnew, int32_t *__restrict__ IDnew,
+                            int32_t *__restrict__ slot_of, uint32_t *__restrict__ flags, int32_t *__restrict__ perm,
+                            uint32_t *__restrict__ epoch) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) epoch[0] += 1u;  // binnings so far
     if (t >= nslots) return;
     const int so = src[t];
     const int myid = sid[t];
@@ -92,7 +116,7 @@ __global__ void kb_finalize(const Replica *__restrict__ reps, const int32_t *__r
     for (int q = b; q < e; q++) rk += (sid[q] < myid) ? 1 : 0;
     const int t2 = b + rk;
     Pnew[t2] = Pold[so];
-    Vnew[t2] = Vold[so];
+    perm[t2] = so;  // velocities move in the first evaluation (rings replicas use none)
     IDnew[t2] = myid;
     slot_of[myid] = t2;
     if (t == b) {
@@ -114,9 +138,41 @@ __global__ void kb_unpermute(int32_t nslots, const int32_t *__restrict__ ID, con
     force_id[id] = F[s];
 }
 
+// Reuse state of the Verlet kernels (LJ / HarmTrunc replicas).
+struct BatchReuse {
+    const int32_t *perm;   // new slot -> old slot of the last binning
+    const double2 *Vold;   // velocities in the old slot order
+    const double2 *Fold;   // the previous second evaluation, old slot order
+    int32_t *needc;        // per chunk: the binning epoch in which a chunk of its 3x3 block changed members
+    const uint32_t *epoch; // binnings so far
+    uint32_t *rvalid;      // per replica: 1 once a second evaluation ran on its current state
+    // the next binning's first pass
+    int32_t *key, *rank, *cnt;
+    uint32_t *flags;
+};
+
+// needc <- e on the chunks of the 3x3 block around chunk (row, col) of a
+// replica's grid (periodic wrap or clipped), as the single system's stamps
+// and k_scatter's block test mark them.
+__device__ __forceinline__ void mark_block(const SysParams &P, int32_t *needc, int cbase, int c, int32_t e) {
+    const int row = c / P.cols, col = c - row * P.cols;
+    for (int dr = -1; dr <= 1; dr++) {
+        int r = row + dr;
+        if (P.periodic) r = r < 0 ? r + P.rows : (r >= P.rows ? r - P.rows : r);
+        else if (r < 0 || r >= P.rows) continue;
+        for (int dc = -1; dc <= 1; dc++) {
+            int cc = col + dc;
+            if (P.periodic) cc = cc < 0 ? cc + P.cols : (cc >= P.cols ? cc - P.cols : cc);
+            else if (cc < 0 || cc >= P.cols) continue;
+            needc[cbase + r * P.cols + cc] = e;
+        }
+    }
+}
+
 // Pair forces + integrator epilogue for the waves of one kind: items[w] =
 // (global wave, replica).  The per-particle work is pair_gather_chunks /
-// pair_epilogue with the replica's parameters, as in k_pair.
+// pair_epilogue with the replica's parameters, as in k_pair; with U (Verlet
+// kinds), the force reuse and fused binning pass described at the top.
 template <int MODE, int POT>
 __global__ void kb_pair(const Replica *__restrict__ reps, const int2 *__restrict__ items, int32_t nitems,
                         const int32_t *__restrict__ ids, const int32_t *__restrict__ rtype,
@@ -124,7 +180,7 @@ __global__ void kb_pair(const Replica *__restrict__ reps, const int2 *__restrict
                         const double2 *__restrict__ pin, const int32_t *__restrict__ scell,
                         const int32_t *__restrict__ start, double2 *__restrict__ Pst, double2 *__restrict__ Vst,
                         double2 *__restrict__ Fout, double2 *__restrict__ P2, double2 *__restrict__ F1,
-                        int32_t listed) {
+                        int32_t listed, BatchReuse U) {
     extern __shared__ uint16_t s_list[];
     const int wl = xcd_block() * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);
     if (wl >= nitems) return;
@@ -133,9 +189,25 @@ __global__ void kb_pair(const Replica *__restrict__ reps, const int2 *__restrict
     const Replica &Q = reps[r];
     const SysParams P = Q.P;
     const int s = w * 64 + (int)lane_id();
-    const bool valid = s - Q.pbase < P.na;
+    bool valid = s - Q.pbase < P.na;
     const int sv = valid ? s : Q.pbase;
     const double2 pi = pin[sv];
+    if constexpr (MODE == MODE_VERLET1) {
+        if (U.perm) {
+            const int so = valid ? U.perm[s] : 0;
+            if (valid) Vst[s] = U.Vold[so];
+            // a chunk whose 3x3 block kept its members: the previous second
+            // evaluation, bit for bit
+            const bool need =
+                valid && (uniform((int)U.rvalid[r]) == 0 || U.needc[scell[sv]] == (int32_t)U.epoch[0]);
+            if (valid && !need) {
+                const double2 f = U.Fold[so];
+                pair_epilogue<MODE>(P, s, pi, f.x, f.y, Pst, Vst, Fout, P2, F1);
+            }
+            if (!__any(need)) return;  // whole wave
+            valid = need;
+        }
+    }
     RingCtx R{};
     PairAcc<POT> acc;
     if constexpr (POT == POT_RINGS) {
@@ -152,6 +224,30 @@ __global__ void kb_pair(const Replica *__restrict__ reps, const int2 *__restrict
     }
     pair_gather_chunks<POT>(P, R, acc, pin, scell[sv] - Q.cbase, start + Q.cbase, s, valid, pi,
                             listed ? Q.list2 : INFINITY, s_list);
+    if constexpr (MODE == MODE_VERLET2) {
+        double2 xf = make_double2(0.0, 0.0);
+        if (valid) pair_epilogue<MODE>(P, s, pi, acc.fx, acc.fy, Pst, Vst, Fout, P2, F1, &xf);
+        if (U.key) {  // every lane of the wave takes part in the arrival ranks
+            uint32_t flag = 0;
+            int32_t k = Q.cbase + P.ncell;  // padding slots: the replica's pad chunk
+            if (valid) {
+                const int kl = cell_of(P, xf.x, xf.y, &flag);
+                const int pc = scell[s] - Q.cbase;
+                if (kl != pc) {
+                    const int32_t e = (int32_t)(U.epoch[0] + 1u);
+                    mark_block(P, U.needc, Q.cbase, kl, e);
+                    mark_block(P, U.needc, Q.cbase, pc, e);
+                }
+                k = Q.cbase + kl;
+            }
+            if (flag) atomicOr(&U.flags[r], flag);
+            const int32_t rk = wave_arrival_rank(true, k, U.cnt);
+            U.key[s] = k;
+            U.rank[s] = rk;
+            if (lane_id() == 0) U.rvalid[r] = 1u;
+        }
+        return;
+    }
     if (!valid) return;
     pair_epilogue<MODE>(P, s, pi, acc.fx, acc.fy, Pst, Vst, Fout, P2, F1);
 }
@@ -209,6 +305,10 @@ struct mavi_batch {
     int32_t *cnt = nullptr, *start = nullptr;
     unsigned long long *scan_state = nullptr, *scan_ticket = nullptr;
     uint32_t *flags = nullptr;
+    // force reuse
+    int32_t *perm = nullptr, *needc = nullptr;
+    uint32_t *epoch = nullptr, *rvalid = nullptr;
+    bool keys_ready = false;  // the last second evaluation did the next binning's first pass
     unsigned long long *counter = nullptr;
     int32_t *active_ids = nullptr;
     double2 *pos_id = nullptr, *vel_id = nullptr, *force_id = nullptr;
@@ -257,16 +357,26 @@ hipError_t batch_bin(mavi_batch *b, hipStream_t st) {
     BatchProf ps(b, MAVI_PROF_BIN, st);
     const int c = b->cur, nc = 1 - c;
     const int ns = (int)b->nslots;
-    hipLaunchKernelGGL(kb_bin_count, dim3(nblk(ns, 256)), dim3(256), 0, st, b->d_reps, b->wave_rep, ns, b->P_[c],
-                       b->key, b->rank, b->cnt, b->flags);
     const int n = (int)b->ncells + 1;
+    if (!b->keys_ready) {
+        // counts a fused pass may have left are dropped: every slot afresh
+        hipError_t e = mavi::launch_fill_i32(b->cnt, 0, n, st);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(kb_bin_count, dim3(nblk(ns, 256)), dim3(256), 0, st, b->d_reps, b->wave_rep, ns,
+                           nullptr, 0, b->P_[c], b->key, b->rank, b->cnt, b->flags);
+    } else if (b->nitems[KIND_RINGS] > 0) {
+        // the Verlet replicas' second evaluation counted theirs
+        hipLaunchKernelGGL(kb_bin_count, dim3(nblk(b->nitems[KIND_RINGS], 4)), dim3(256), 0, st, b->d_reps,
+                           b->wave_rep, ns, b->items[KIND_RINGS], b->nitems[KIND_RINGS], b->P_[c], b->key, b->rank,
+                           b->cnt, b->flags);
+    }
     // a stalled scan is reported through replica 0's flags
     launch_scan(b->cnt, n, b->start, b->scan_ticket, b->scan_state, b->flags, st);
     hipLaunchKernelGGL(k_scatter, dim3(nblk(std::max(ns, n), 256)), dim3(256), 0, st, b->G, b->key, b->rank,
                        b->start, b->src, b->scell, b->ID_[c], b->sid, b->cnt, b->counter, nullptr, nullptr, nullptr);
     hipLaunchKernelGGL(kb_finalize, dim3(nblk(ns, 256)), dim3(256), 0, st, b->d_reps, b->wave_rep, ns, b->start,
-                       b->src, b->scell, b->sid, b->P_[c], b->V_[c], b->ID_[c], b->P_[nc], b->V_[nc], b->ID_[nc],
-                       b->slot_of, b->flags);
+                       b->src, b->scell, b->sid, b->P_[c], b->ID_[c], b->P_[nc], b->ID_[nc], b->slot_of, b->flags,
+                       b->perm, b->epoch);
     b->cur = nc;
     return hipGetLastError();
 }
@@ -280,9 +390,26 @@ void batch_pair_kind(mavi_batch *b, int kind, const double2 *pin, hipStream_t st
     const int listed = b->listed[slot];
     const size_t lds = listed ? sizeof(uint16_t) * kListCap * (size_t)wg : 0;
     const int c = b->cur;
+    BatchReuse U{};
+    if (MODE == MODE_VERLET1) {
+        U.perm = b->perm;
+        U.Vold = b->V_[1 - c];
+        U.Fold = b->F;
+        U.needc = b->needc;
+        U.epoch = b->epoch;
+        U.rvalid = b->rvalid;
+    } else if (MODE == MODE_VERLET2) {
+        U.needc = b->needc;
+        U.epoch = b->epoch;
+        U.rvalid = b->rvalid;
+        U.key = b->key;
+        U.rank = b->rank;
+        U.cnt = b->cnt;
+        U.flags = b->flags;
+    }
     hipLaunchKernelGGL((kb_pair<MODE, POT>), dim3(nblk(ni, wg / 64)), dim3(wg), lds, st, b->d_reps, b->items[kind],
                        ni, b->ID_[c], b->rtype, b->types, b->inter, pin, b->scell, b->start, b->P_[c], b->V_[c],
-                       b->F, b->P2, b->F1, listed);
+                       b->F, b->P2, b->F1, listed, U);
 }
 
 hipError_t batch_step(mavi_batch *b, hipStream_t st) {
@@ -305,6 +432,7 @@ hipError_t batch_step(mavi_batch *b, hipStream_t st) {
                            b->nring_waves, b->rtype, b->types, b->slot_of, b->P_[b->cur], b->F, b->pol, b->areas,
                            b->cms, b->cms_next, b->cpos, b->counter);
     }
+    b->keys_ready = true;
     return hipGetLastError();
 }
 
@@ -356,6 +484,10 @@ int batch_check_flags(mavi_batch *b) {
 hipError_t batch_stage_in(mavi_batch *b, int r, hipStream_t st) {
     const Replica &Q = b->reps[r];
     const int c = b->cur;
+    // a replaced state: its forces are recomputed and every slot binned afresh
+    b->keys_ready = false;
+    hipError_t e0 = hipMemsetAsync(b->rvalid + r, 0, sizeof(uint32_t), st);
+    if (e0 != hipSuccess) return e0;
     const bool rings = Q.kind == KIND_RINGS;
     hipLaunchKernelGGL(k_iota_state, dim3(nblk(Q.P.na, 256)), dim3(256), 0, st, Q.P.na,
                        rings ? b->active_ids + Q.abase : nullptr, b->pos_id + Q.nbase, b->vel_id + Q.nbase,
@@ -373,7 +505,9 @@ int run_impl(mavi_batch *b, int64_t nsteps) {
     hipStream_t st = ctx->stream;
     int64_t k = 0;
     const bool graphs = b->use_graphs && !b->prof && nsteps >= b->graph_steps;
-    if (graphs && b->cur != 0) {
+    // the graph holds steps that follow a step of these kernels (fused
+    // binning pass) from state copy 0
+    while (graphs && k < nsteps && (!b->keys_ready || b->cur != 0)) {
         HIPCHK(ctx, batch_step(b, st));
         k++;
     }
@@ -414,6 +548,7 @@ int mavi_batch_destroy(mavi_batch *b) {
     for (int k = 0; k < NKIND; k++) dfree(b->items[k]);
     dfree(b->ring_wave_rep); dfree(b->d_reps); dfree(b->wave_rep);
     dfree(b->F); dfree(b->P2); dfree(b->F1);
+    dfree(b->perm); dfree(b->needc); dfree(b->epoch); dfree(b->rvalid);
     dfree(b->key); dfree(b->rank); dfree(b->src); dfree(b->scell); dfree(b->sid); dfree(b->slot_of);
     dfree(b->cnt); dfree(b->start); dfree(b->scan_state); dfree(b->scan_ticket); dfree(b->flags); dfree(b->counter);
     dfree(b->active_ids); dfree(b->pos_id); dfree(b->vel_id); dfree(b->force_id);
@@ -539,6 +674,10 @@ int mavi_batch_finalize(mavi_batch *b) {
     ALLOC(b->scan_ticket, 1);
     ALLOC(b->flags, nr);
     ALLOC(b->counter, 1);
+    ALLOC(b->perm, nslots);
+    ALLOC(b->needc, ncells + 1);
+    ALLOC(b->epoch, 1);
+    ALLOC(b->rvalid, nr);
     ALLOC(b->pos_id, nids);
     ALLOC(b->vel_id, nids);
     ALLOC(b->force_id, nids);
@@ -679,6 +818,15 @@ int mavi_batch_noise_set_counter(mavi_batch *b, uint64_t counter) {
     return MAVI_OK;
 }
 
+// A changed pair-kernel shape may visit candidates in another order: the
+// reused forces would not be its bits, so every replica recomputes once.
+static int batch_forget_forces(mavi_batch *b) {
+    if (!b->finalized || !b->rvalid) return MAVI_OK;
+    HIPCHK(b->ctx, hipSetDevice(b->ctx->device));
+    HIPCHK(b->ctx, hipMemsetAsync(b->rvalid, 0, sizeof(uint32_t) * b->reps.size(), b->ctx->stream));
+    return MAVI_OK;
+}
+
 int mavi_batch_set_option(mavi_batch *b, int32_t option, int32_t value) {
     if (!b) return MAVI_E_BADARG;
     mavi_ctx *ctx = b->ctx;
@@ -689,12 +837,12 @@ int mavi_batch_set_option(mavi_batch *b, int32_t option, int32_t value) {
             if (!is_pow2_wg(value)) return set_err(ctx, MAVI_E_BADARG, "workgroup must be 64..1024, a power of 2");
             if (b->wg[option - MAVI_BOPT_WG_PLAIN] != value) drop_graph(b);
             b->wg[option - MAVI_BOPT_WG_PLAIN] = value;
-            return MAVI_OK;
+            return batch_forget_forces(b);
         case MAVI_BOPT_LIST_PLAIN:
         case MAVI_BOPT_LIST_RINGS:
             if (b->listed[option - MAVI_BOPT_LIST_PLAIN] != (value != 0)) drop_graph(b);
             b->listed[option - MAVI_BOPT_LIST_PLAIN] = value != 0;
-            return MAVI_OK;
+            return batch_forget_forces(b);
         case MAVI_BOPT_GRAPH_STEPS:
             if (value < 2 || (value & 1)) return set_err(ctx, MAVI_E_BADARG, "graph steps must be even and >= 2");
             if (value != b->graph_steps) drop_graph(b);
