@@ -165,9 +165,10 @@ __device__ __forceinline__ void nl_sum(const SysParams &P, const double2 *__rest
         const int j[4] = {lo_slot(s, c.x, na), hi_slot(s, c.x, na), lo_slot(s, c.y, na), hi_slot(s, c.y, na)};
         double2 p[4];
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-            p[e] = load_slot(pin, j[e]);
-            if (WRAPIN) p[e] = state_pos(P, p[e]);
+        for (int e = 0; e < 4; e++) p[e] = load_slot(pin, j[e]);
+        if constexpr (WRAPIN) {  // after all four loads are in flight
+#pragma unroll
+            for (int e = 0; e < 4; e++) p[e] = state_pos(P, p[e]);
         }
 #pragma unroll
         for (int e = 0; e < 4; e++) nl_pair<POT, WRAP, STALE>(P, p[e], j[e], s, xi, yi, test, ri, ci, kcur, acc);
