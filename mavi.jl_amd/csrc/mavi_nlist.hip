@@ -57,6 +57,7 @@ constexpr uint32_t kStatusOverflow = 1u;    // a list held more than kmax entrie
 constexpr uint32_t kStatusCapacity = 2u;    // a chunk count passed the capacity during a step's moves
 constexpr int kNlMax = 48;                  // list entries per particle (12 groups of 4)
 constexpr int kNeedBlock = 256;             // workgroup of the evaluation kernels
+constexpr int kGhost = 32768;               // drifted-position copies each side of the slots (see nl_sum)
 
 struct NLArgs {
     const int2 *nbr;      // groups of 4 partner offsets ((slot - s) mod na in [-32768, 32767]): nbr[g * na + s]
@@ -110,27 +111,6 @@ __device__ __forceinline__ bool chunks_adjacent(const SysParams &P, int ri, int 
 // One pair.  A list pad is the particle itself (its offsets are 0 and the LJ
 // factor is masked), so it adds nothing.  STALE: with `test`, the pair counts
 // only when the step-start chunks are adjacent.
-template <int POT, bool WRAP, bool STALE>
-__device__ __forceinline__ void nl_pair(const SysParams &P, const double2 pj, const int32_t j, const int s,
-                                        double xi, double yi, bool test, int ri, int ci,
-                                        const int32_t *__restrict__ kcur, PairAcc<POT> &acc) {
-    double dx = xi - pj.x, dy = yi - pj.y;
-    if constexpr (WRAP) {
-        dx = min_image(dx, P.L, P.halfL);
-        dy = min_image(dy, P.H, P.halfH);
-    }
-    bool take = j != s;
-    if constexpr (STALE) take = take && (!test || chunks_adjacent(P, ri, ci, kcur[j]));
-    if constexpr (POT == POT_LJ) {
-        const double f = lj_scale_masked(P, dx * dx + dy * dy, take);
-        acc.fx = __builtin_fma(f, dx, acc.fx);
-        acc.fy = __builtin_fma(f, dy, acc.fy);
-    } else {
-        RingCtx R{};
-        if (take) acc.add(P, R, dx, dy, true, 0);
-    }
-}
-
 // Slots of a group's two entries packed in one word: 16-bit offsets from s
 // taken modulo na (periodic rows put partners about na slots away).
 __device__ __forceinline__ int wrap_slot(int j, int na) { return j < 0 ? j + na : (j >= na ? j - na : j); }
@@ -147,25 +127,62 @@ __device__ __forceinline__ int32_t slot_offset(int s, int j, int na) {
     return d;
 }
 
+template <int POT, bool WRAP, bool STALE>
+__device__ __forceinline__ void nl_pair(const SysParams &P, const double2 pj, const int32_t j, const int s,
+                                        double xi, double yi, bool test, int ri, int ci,
+                                        const int32_t *__restrict__ kcur, PairAcc<POT> &acc) {
+    double dx = xi - pj.x, dy = yi - pj.y;
+    if constexpr (WRAP) {
+        dx = min_image(dx, P.L, P.halfL);
+        dy = min_image(dy, P.H, P.halfH);
+    }
+    bool take = j != s;
+    if constexpr (STALE) take = take && (!test || chunks_adjacent(P, ri, ci, kcur[wrap_slot(j, P.na)]));
+    if constexpr (POT == POT_LJ) {
+        const double f = lj_scale_masked(P, dx * dx + dy * dy, take);
+        acc.fx = __builtin_fma(f, dx, acc.fx);
+        acc.fy = __builtin_fma(f, dy, acc.fy);
+    } else {
+        RingCtx R{};
+        if (take) acc.add(P, R, dx, dy, true, 0);
+    }
+}
+
 // The sum over one particle's list in list order (ascending partner id),
 // four entries per iteration; the next group is loaded one iteration ahead,
 // so the list and position loads of consecutive groups overlap.  WRAPIN: pin
-// holds drifted positions, taken through state_pos.
-template <int POT, bool WRAP, bool STALE, bool WRAPIN = false>
+// holds drifted positions, taken through state_pos.  GHOST: pin holds
+// kGhost copies of the other end's positions on each side of the slots
+// (pin[j - na] = pin[j] for j >= na - kGhost, pin[j + na] = pin[j] for
+// j < kGhost), so a partner is read at s + offset without the wrap modulo na
+// (|offset| <= min(32768, na / 2)).
+template <int POT, bool WRAP, bool STALE, bool WRAPIN = false, bool GHOST = false>
 __device__ __forceinline__ void nl_sum(const SysParams &P, const double2 *__restrict__ pin,
                                        const int2 *__restrict__ nbr, int s, int groups, double xi, double yi,
                                        bool test, int ri, int ci, const int32_t *__restrict__ kcur,
                                        PairAcc<POT> &acc) {
     if (groups <= 0) return;
     const int na = P.na;
+    const double2 *pb = GHOST ? pin - kGhost : pin;  // indexed from 0 with kGhost added
     int2 q = nbr[s];
     for (int g = 0; g < groups; g++) {
         const int2 c = q;
         if (g + 1 < groups) q = nbr[(g + 1) * na + s];
-        const int j[4] = {lo_slot(s, c.x, na), hi_slot(s, c.x, na), lo_slot(s, c.y, na), hi_slot(s, c.y, na)};
+        int j[4];
+        if constexpr (GHOST) {
+            j[0] = s + (int32_t)(int16_t)(c.x & 0xffff);
+            j[1] = s + (c.x >> 16);
+            j[2] = s + (int32_t)(int16_t)(c.y & 0xffff);
+            j[3] = s + (c.y >> 16);
+        } else {
+            j[0] = lo_slot(s, c.x, na);
+            j[1] = hi_slot(s, c.x, na);
+            j[2] = lo_slot(s, c.y, na);
+            j[3] = hi_slot(s, c.y, na);
+        }
         double2 p[4];
 #pragma unroll
-        for (int e = 0; e < 4; e++) p[e] = load_slot(pin, j[e]);
+        for (int e = 0; e < 4; e++) p[e] = load_slot(pb, GHOST ? j[e] + kGhost : j[e]);
         if constexpr (WRAPIN) {  // after all four loads are in flight
 #pragma unroll
             for (int e = 0; e < 4; e++) p[e] = state_pos(P, p[e]);
@@ -437,6 +454,9 @@ __device__ __forceinline__ void nl_drift(const SysParams &P, const NLArgs &A, in
                                          uint8_t *__restrict__ chg) {
     const double px = pi.x + (v.x * P.dt + fx * P.term), py = pi.y + (v.y * P.dt + fy * P.term);
     pout[s] = make_double2(px, py);
+    // the copies the list sums read across the ends of the slot range
+    if (s < kGhost) pout[s + P.na] = make_double2(px, py);
+    if (s >= P.na - kGhost) pout[s - P.na] = make_double2(px, py);
     double wx = px, wy = py;
     if (P.periodic) periodic_wrap(P, wx, wy);  // walls! moves it there after the kick
     const int32_t ko = kold[s];
@@ -526,15 +546,18 @@ __global__ __launch_bounds__(kNeedBlock) void k_nl_second(SysParams P, NLArgs A)
         const int ri = ki >> 16, ci = ki & 0xffff;
         const int g1 = fresh ? groups : 0;
         if (wrap) {
-            nl_sum<POT, true, true>(P, A.Px, A.nbr, sv, groups, pu.x, pu.y, stale, ri, ci, A.kcur, acc);
-            nl_sum<POT, true, false, true>(P, A.Px, A.nbr, sv, g1, pw.x, pw.y, false, 0, 0, nullptr, acc1);
+            nl_sum<POT, true, true, false, true>(P, A.Px, A.nbr, sv, groups, pu.x, pu.y, stale, ri, ci, A.kcur, acc);
+            nl_sum<POT, true, false, true, true>(P, A.Px, A.nbr, sv, g1, pw.x, pw.y, false, 0, 0, nullptr, acc1);
         } else {
-            nl_sum<POT, false, true>(P, A.Px, A.nbr, sv, groups, pu.x, pu.y, stale, ri, ci, A.kcur, acc);
-            nl_sum<POT, false, false, true>(P, A.Px, A.nbr, sv, g1, pw.x, pw.y, false, 0, 0, nullptr, acc1);
+            nl_sum<POT, false, true, false, true>(P, A.Px, A.nbr, sv, groups, pu.x, pu.y, stale, ri, ci, A.kcur,
+                                                  acc);
+            nl_sum<POT, false, false, true, true>(P, A.Px, A.nbr, sv, g1, pw.x, pw.y, false, 0, 0, nullptr, acc1);
         }
     } else {
-        if (wrap) nl_sum<POT, true, false>(P, A.Px, A.nbr, sv, groups, pu.x, pu.y, false, 0, 0, nullptr, acc);
-        else nl_sum<POT, false, false>(P, A.Px, A.nbr, sv, groups, pu.x, pu.y, false, 0, 0, nullptr, acc);
+        if (wrap)
+            nl_sum<POT, true, false, false, true>(P, A.Px, A.nbr, sv, groups, pu.x, pu.y, false, 0, 0, nullptr, acc);
+        else
+            nl_sum<POT, false, false, false, true>(P, A.Px, A.nbr, sv, groups, pu.x, pu.y, false, 0, 0, nullptr, acc);
     }
     double d2 = 0.0;
     if (valid) {
@@ -688,7 +711,11 @@ hipError_t nl_alloc(mavi_sys *s, hipStream_t st) {
     if (e == hipSuccess) e = dalloc(&N.ncount, na, st);
     for (int k = 0; k < 3 && e == hipSuccess; k++) e = dalloc(&N.key[k], na, st);
     for (int k = 0; k < 2 && e == hipSuccess; k++) e = dalloc(&N.chg[k], (na + 3) & ~int64_t(3), st);
-    for (int k = 0; k < 2 && e == hipSuccess; k++) e = dalloc(&N.px[k], na, st);
+    // drifted positions with kGhost copies on each side (nl_sum's GHOST form)
+    for (int k = 0; k < 2 && e == hipSuccess; k++) {
+        e = dalloc(&N.px[k], na + 2 * (int64_t)kGhost, st);
+        if (e == hipSuccess) N.px[k] += kGhost;
+    }
     if (e == hipSuccess) e = dalloc(&N.fk, na, st);
     if (e == hipSuccess) e = dalloc(&N.fk_on, na, st);
     if (e == hipSuccess) e = dalloc(&N.ccnt, (int64_t)s->P.ncell, st);
@@ -716,7 +743,12 @@ void nl_free(mavi_sys *s) {
     dfree(N.nbr); dfree(N.ncount); dfree(N.ccnt);
     for (auto &k : N.key) dfree(k);
     for (auto &c : N.chg) dfree(c);
-    for (auto &p : N.px) dfree(p);
+    for (auto &p : N.px) {
+        if (!p) continue;
+        double2 *base = p - kGhost;  // allocated with kGhost copies before the slots
+        dfree(base);
+        p = nullptr;
+    }
     dfree(N.fk); dfree(N.fk_on); dfree(N.xb); dfree(N.psnap); dfree(N.vsnap); dfree(N.fsnap);
     dfree(N.dmax); dfree(N.bmax); dfree(N.status); dfree(N.scan_ticket); dfree(N.scan_state);
     if (N.host) (void)hipHostFree(N.host);
