@@ -877,7 +877,8 @@ static int nl_steps(mavi_sys *s, int64_t nsteps, int64_t *left) {
     int64_t k = 0;
     while (k < nsteps) {
         const bool fresh = !N.valid;
-        if (fresh) HIPCHK(ctx, mavi::nl_rebuild(s, st));
+        if (fresh) HIPCHK(ctx, mavi::nl_rebuild(s, st));  // resets the step maxima, then the first drift
+        else HIPCHK(ctx, mavi::launch_fill_i32((int32_t *)N.bmax, 0, 2 * (int64_t)N.nbmax, st));
         const int64_t blk = std::min<int64_t>(N.block_now, nsteps - k);
         // the block's start state, to roll back to
         HIPCHK(ctx, hipMemcpyAsync(N.psnap, s->P_[s->cur], bytes, hipMemcpyDeviceToDevice, st));
@@ -898,12 +899,16 @@ static int nl_steps(mavi_sys *s, int64_t nsteps, int64_t *left) {
         }
         HIPCHK(ctx, mavi::nl_settle(s, st));
         HIPCHK(ctx, mavi::nl_reduce_dmax(s, st));
-        HIPCHK(ctx, hipMemcpyAsync(N.host, N.dmax, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-        HIPCHK(ctx, hipMemcpyAsync((char *)N.host + 8, N.status, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(ctx, hipMemcpyAsync(N.host, N.dmax, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        HIPCHK(ctx, hipMemcpyAsync((char *)N.host + 16, N.status, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         HIPCHK(ctx, hipStreamSynchronize(st));
-        const double d2 = __builtin_bit_cast(double, N.host[0]);
-        const uint32_t status = (uint32_t)N.host[1];
-        if (status || d2 > N.thr2) {
+        const double step = sqrt(__builtin_bit_cast(double, N.host[0]));
+        const double d_end = sqrt(__builtin_bit_cast(double, N.host[1]));
+        const uint32_t status = (uint32_t)N.host[2];
+        // every position the block evaluated is within d_start + blk steps
+        // of the largest length from its build position
+        const double bound = N.d_start + (double)blk * step;
+        if (status || bound > half_skin) {
             // a step of this block may have used an incomplete list: back to its start
             HIPCHK(ctx, hipMemcpyAsync(s->P_[s->cur], N.psnap, bytes, hipMemcpyDeviceToDevice, st));
             HIPCHK(ctx, hipMemcpyAsync(s->V_[s->cur], N.vsnap, bytes, hipMemcpyDeviceToDevice, st));
@@ -928,11 +933,10 @@ static int nl_steps(mavi_sys *s, int64_t nsteps, int64_t *left) {
             continue;
         }
         k += blk;
-        // rebuild ahead when the next block would reach skin / 2 at this rate
-        const double d = sqrt(d2);
-        const double rate = (d - N.d_last) / (double)blk;
-        N.d_last = d;
-        if (d + 1.5 * rate * (double)N.block_now > half_skin) N.valid = false;
+        N.d_start = d_end;
+        // rebuild ahead when the next block's bound would pass skin / 2 at
+        // this step length
+        if (d_end + 1.2 * step * (double)N.block_now > half_skin) N.valid = false;
         if (N.block_now < N.block) N.block_now = std::min(N.block, 2 * N.block_now);
     }
     s->sorted = false;   // not in reference chunk order

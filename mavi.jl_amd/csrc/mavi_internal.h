@@ -188,14 +188,15 @@ struct NList {
     double2 *xb = nullptr;    // positions at build
     double2 *psnap = nullptr, *vsnap = nullptr;  // block-start state (rollback)
     uint32_t *fsnap = nullptr;
-    unsigned long long *dmax = nullptr;  // max displacement^2 since build (ordered bits), reduced per block
-    double *bmax = nullptr;              // per workgroup of the second evaluation: its largest displacement^2 since build
+    unsigned long long *dmax = nullptr;  // reduced per block (bits): [0] largest drift step^2, [1] largest pending displacement^2
+    double *bmax = nullptr;              // per workgroup of the evaluations: largest drift step^2 since the block began
+    double *dpend = nullptr;             // per workgroup: largest displacement^2 of the pending positions (block end)
     int32_t nbmax = 0;
     uint32_t *status = nullptr;          // bit 0: a list overflowed kmax; bit 1: a chunk count may have passed the capacity
     unsigned long long *scan_ticket = nullptr, *scan_state = nullptr;  // the list grid's scan (cleared per build)
     int32_t scan_words = 0;
-    unsigned long long *host = nullptr;  // pinned read back: dmax, status
-    double d_last = 0.0;      // sqrt(dmax) at the last block end
+    unsigned long long *host = nullptr;  // pinned read back: dmax[0..1], status
+    double d_start = 0.0;     // largest displacement from the build of the next block's first positions
     int64_t rebuilds = 0, rollbacks = 0;
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // six steps from ph = 0 on state copy 0 / 1 (a rebuild's sort flips it)
 };
@@ -361,9 +362,10 @@ hipError_t nl_alloc(mavi_sys *s, hipStream_t st);
 void nl_free(mavi_sys *s);
 hipError_t nl_rebuild(mavi_sys *s, hipStream_t st);
 hipError_t nl_step(mavi_sys *s, hipStream_t st);
-// the state's positions <- the wrap of the last drift (after steps)
+// the state's positions <- the wrap of the last evaluated drift, and the
+// pending positions' displacement maxima (after steps)
 hipError_t nl_settle(mavi_sys *s, hipStream_t st);
-// dmax <- the largest per-workgroup displacement since the build
+// dmax <- the largest drift step and pending displacement (per-workgroup maxima)
 hipError_t nl_reduce_dmax(mavi_sys *s, hipStream_t st);
 void nl_set_skin(mavi_sys *s, double skin_frac);
 hipError_t launch_export_chunks(mavi_sys *s, int64_t *d_cp, int64_t *d_counts, hipStream_t st);

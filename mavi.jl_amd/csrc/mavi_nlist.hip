@@ -27,9 +27,11 @@
 // x_{n+1} and x_{n+2}); the state's positions, their wrap, are written once
 // per block.  A list stays complete while no particle has moved
 // more than skin / 2 since the build (no two particles can then close the
-// gap); the second evaluation records the largest displacement, and the host
-// checks it every few steps, rolling the block back and rebuilding when it
-// was exceeded.
+// gap).  Each drift records the largest step length of its workgroup; at a
+// block's end the exact largest displacement of the next positions is
+// measured, and the host checks D_start + T * (largest step) against skin /
+// 2 (a bound for every position the block's evaluations used), rolling the
+// block back and rebuilding when it was exceeded.
 //
 // Order.  A list is sorted by partner id, so a particle's force is the sum of
 // its in-range pairs in ascending partner id whatever the slot order and
@@ -68,8 +70,7 @@ struct NLArgs {
     int32_t *ccnt;
     uint8_t *chg;         // marks of this step's drift: 2 = changed chunk, 1 = a list partner did
     uint8_t *chg_next;    // marks of the next step's drift
-    const double2 *xb;
-    double *bmax;
+    double *bmax;         // per workgroup: the largest squared drift step since the last reset
     const double2 *Pst;   // the state's positions (x_n after a build)
     double2 *V;
     double2 *F;           // the last second evaluation (the reference's forces)
@@ -448,10 +449,10 @@ __global__ void k_nl_keys(SysParams P, const double2 *__restrict__ pos, int32_t 
 // chunk is not adjacent to this particle's (1) and then the particle (2);
 // all partners if the wrap moved the particle (the first evaluation sums the
 // wrapped positions).
-__device__ __forceinline__ void nl_drift(const SysParams &P, const NLArgs &A, int s, double2 pi, double2 v,
-                                         double fx, double fy, double2 *__restrict__ pout,
-                                         const int32_t *__restrict__ kold, int32_t *__restrict__ knew,
-                                         uint8_t *__restrict__ chg) {
+__device__ __forceinline__ double nl_drift(const SysParams &P, const NLArgs &A, int s, double2 pi, double2 v,
+                                           double fx, double fy, double2 *__restrict__ pout,
+                                           const int32_t *__restrict__ kold, int32_t *__restrict__ knew,
+                                           uint8_t *__restrict__ chg) {
     const double px = pi.x + (v.x * P.dt + fx * P.term), py = pi.y + (v.y * P.dt + fy * P.term);
     pout[s] = make_double2(px, py);
     // the copies the list sums read across the ends of the slot range
@@ -499,10 +500,28 @@ __device__ __forceinline__ void nl_drift(const SysParams &P, const NLArgs &A, in
         if (atomicAdd(&A.ccnt[(kn >> 16) * P.cols + (kn & 0xffff)], 1) + 1 > P.nc)
             atomicOr(A.status, kStatusCapacity);
     }
+    const double lx = px - pi.x, ly = py - pi.y;  // the step's length, squared
+    return lx * lx + ly * ly;
+}
+
+// max over a kNeedBlock workgroup of v into slot[blockIdx.x] (kept across
+// launches: the larger of the stored and the new value).
+__device__ __forceinline__ void block_max_into(double v, double *__restrict__ slot) {
+    __shared__ double s_max[kNeedBlock / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    if (lane_id() == 0) s_max[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mx = s_max[0];
+        for (int w = 1; w < kNeedBlock / 64; w++) mx = fmax(mx, s_max[w]);
+        if (mx > slot[blockIdx.x]) slot[blockIdx.x] = mx;
+    }
 }
 
 // After a build: the first evaluation of newton_step! (forces at x_n over
-// every pair within range) and the drift, for every particle.
+// every pair within range) and the drift, for every particle; the largest
+// step per workgroup.
 template <int POT>
 __global__ __launch_bounds__(kNeedBlock) void k_nl_start(SysParams P, NLArgs A) {
     const int s = xcd_block() * blockDim.x + threadIdx.x;
@@ -516,17 +535,20 @@ __global__ __launch_bounds__(kNeedBlock) void k_nl_start(SysParams P, NLArgs A) 
         nl_sum<POT, true, false>(P, A.Pst, A.nbr, sv, groups, pi.x, pi.y, false, 0, 0, nullptr, acc);
     else
         nl_sum<POT, false, false>(P, A.Pst, A.nbr, sv, groups, pi.x, pi.y, false, 0, 0, nullptr, acc);
-    if (!valid) return;
-    A.F[s] = make_double2(acc.fx, acc.fy);
-    A.fk_on[s] = 0;
-    nl_drift(P, A, s, pi, A.V[s], acc.fx, acc.fy, A.Px, A.kcur, A.knext, A.chg);
+    double l2 = 0.0;
+    if (valid) {
+        A.F[s] = make_double2(acc.fx, acc.fy);
+        A.fk_on[s] = 0;
+        l2 = nl_drift(P, A, s, pi, A.V[s], acc.fx, acc.fy, A.Px, A.kcur, A.knext, A.chg);
+    }
+    block_max_into(l2, A.bmax);
 }
 
 // Second evaluation at the drifted positions with the step-start chunks,
 // then the kick and walls! (src/integration.jl:425-431, 271-285, 309-324),
 // then the next step: its first evaluation (this one, or for the marked
 // particles every pair within range at the wrapped positions) and its drift.
-// Records the largest displacement since the build.
+// Records the largest drift step.
 template <int POT>
 __global__ __launch_bounds__(kNeedBlock) void k_nl_second(SysParams P, NLArgs A) {
     const int s = xcd_block() * blockDim.x + threadIdx.x;
@@ -559,7 +581,7 @@ __global__ __launch_bounds__(kNeedBlock) void k_nl_second(SysParams P, NLArgs A)
         else
             nl_sum<POT, false, false, false, true>(P, A.Px, A.nbr, sv, groups, pu.x, pu.y, false, 0, 0, nullptr, acc);
     }
-    double d2 = 0.0;
+    double l2 = 0.0;
     if (valid) {
         const double fx = acc.fx, fy = acc.fy;                     // this second evaluation
         const double gx = fresh ? acc1.fx : fx, gy = fresh ? acc1.fy : fy;  // the next first one
@@ -582,17 +604,57 @@ __global__ __launch_bounds__(kNeedBlock) void k_nl_second(SysParams P, NLArgs A)
         if (fresh) A.Fk[s] = make_double2(gx, gy);
         A.fk_on[s] = fresh ? 1 : 0;
         A.chg[s] = 0;
-        nl_drift(P, A, s, pw, make_double2(vx, vy), gx, gy, A.Px_next, A.knext, A.knn, A.chg_next);
-        const double2 b = A.xb[s];
-        double ddx = x - b.x, ddy = y - b.y;
-        if (P.periodic) {
-            ddx = min_image(ddx, P.L, P.halfL);
-            ddy = min_image(ddy, P.H, P.halfH);
-        }
-        d2 = ddx * ddx + ddy * ddy;
+        l2 = nl_drift(P, A, s, pw, make_double2(vx, vy), gx, gy, A.Px_next, A.knext, A.knn, A.chg_next);
     }
-    // the workgroup's largest displacement since the build, kept per
-    // workgroup across steps (no contended atomic on one word)
+    // the workgroup's largest step since the block began (no contended
+    // atomic on one word)
+    block_max_into(l2, A.bmax);
+}
+
+// out[0] <- max of bmax, out[1] <- max of dpend (one workgroup of 1024
+// threads; the doubles' bits).
+__global__ __launch_bounds__(1024) void k_nl_reduce(const double *__restrict__ bmax, const double *__restrict__ dpend,
+                                                    int32_t n, unsigned long long *__restrict__ out) {
+    __shared__ double s_max[2][16];
+    double m[2] = {0.0, 0.0};
+    for (int i = threadIdx.x; i < n; i += 1024) {
+        m[0] = fmax(m[0], bmax[i]);
+        m[1] = fmax(m[1], dpend[i]);
+    }
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) m[k] = fmax(m[k], __shfl_xor(m[k], o, 64));
+        if (lane_id() == 0) s_max[k][threadIdx.x >> 6] = m[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const int k = threadIdx.x;
+        double v = s_max[k][0];
+        for (int w = 1; w < 16; w++) v = fmax(v, s_max[k][w]);
+        out[k] = (unsigned long long)__double_as_longlong(v);
+    }
+}
+
+// At a block end: the state's positions (the wrap of the last evaluated
+// drift, px) and, per workgroup, the largest squared displacement of the
+// next positions (pending, not yet evaluated) from the build positions.
+__global__ __launch_bounds__(kNeedBlock) void k_nl_settle(SysParams P, const double2 *__restrict__ px,
+                                                         const double2 *__restrict__ pending,
+                                                         const double2 *__restrict__ xb, double2 *__restrict__ pst,
+                                                         double *__restrict__ dpend) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    double d2 = 0.0;
+    if (s < P.na) {
+        pst[s] = state_pos(P, px[s]);
+        const double2 q = state_pos(P, pending[s]), b = xb[s];
+        double dx = q.x - b.x, dy = q.y - b.y;
+        if (P.periodic) {
+            dx = min_image(dx, P.L, P.halfL);
+            dy = min_image(dy, P.H, P.halfH);
+        }
+        d2 = dx * dx + dy * dy;
+    }
     __shared__ double s_max[kNeedBlock / 64];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) d2 = fmax(d2, __shfl_xor(d2, o, 64));
@@ -601,31 +663,8 @@ __global__ __launch_bounds__(kNeedBlock) void k_nl_second(SysParams P, NLArgs A)
     if (threadIdx.x == 0) {
         double mx = s_max[0];
         for (int w = 1; w < kNeedBlock / 64; w++) mx = fmax(mx, s_max[w]);
-        if (mx > A.bmax[blockIdx.x]) A.bmax[blockIdx.x] = mx;
+        dpend[blockIdx.x] = mx;
     }
-}
-
-// dmax <- max of the per-workgroup maxima (one workgroup of 1024 threads).
-__global__ __launch_bounds__(1024) void k_nl_reduce(const double *__restrict__ bmax, int32_t n,
-                                                    unsigned long long *__restrict__ dmax) {
-    __shared__ double s_max[16];
-    double m = 0.0;
-    for (int i = threadIdx.x; i < n; i += 1024) m = fmax(m, bmax[i]);
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
-    if (lane_id() == 0) s_max[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < 16; w++) m = fmax(m, s_max[w]);
-        m = fmax(m, s_max[0]);
-        dmax[0] = (unsigned long long)__double_as_longlong(m);
-    }
-}
-
-// The state's positions at a block end: the wrap of the last drift.
-__global__ void k_nl_settle(SysParams P, const double2 *__restrict__ px, double2 *__restrict__ pst) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < P.na) pst[s] = state_pos(P, px[s]);
 }
 
 NLArgs nl_args(mavi_sys *s) {
@@ -640,7 +679,6 @@ NLArgs nl_args(mavi_sys *s) {
     A.ccnt = N.ccnt;
     A.chg = N.chg[n % 2];
     A.chg_next = N.chg[(n + 1) % 2];
-    A.xb = N.xb;
     A.bmax = N.bmax;
     A.Pst = s->P_[s->cur];
     A.V = s->V_[s->cur];
@@ -723,11 +761,12 @@ hipError_t nl_alloc(mavi_sys *s, hipStream_t st) {
     if (e == hipSuccess) e = dalloc(&N.psnap, na, st);
     if (e == hipSuccess) e = dalloc(&N.vsnap, na, st);
     if (e == hipSuccess) e = dalloc(&N.fsnap, 1, st);
-    if (e == hipSuccess) e = dalloc(&N.dmax, 1, st);
+    if (e == hipSuccess) e = dalloc(&N.dmax, 2, st);
     N.nbmax = nblk(s->P.na, kNeedBlock);  // workgroups of the second evaluation
     if (e == hipSuccess) e = dalloc(&N.bmax, N.nbmax, st);
+    if (e == hipSuccess) e = dalloc(&N.dpend, N.nbmax, st);
     if (e == hipSuccess) e = dalloc(&N.status, 1, st);
-    if (e == hipSuccess) e = hipHostMalloc((void **)&N.host, 2 * sizeof(unsigned long long), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&N.host, 3 * sizeof(unsigned long long), hipHostMallocDefault);
     // scan states of the list grid (no more cells than the chunk grid)
     N.scan_words = (s->P.ncell + 1 + kScanTileMin - 1) / kScanTileMin + 1;
     if (e == hipSuccess) e = dalloc(&N.scan_ticket, 1, st);
@@ -750,7 +789,7 @@ void nl_free(mavi_sys *s) {
         p = nullptr;
     }
     dfree(N.fk); dfree(N.fk_on); dfree(N.xb); dfree(N.psnap); dfree(N.vsnap); dfree(N.fsnap);
-    dfree(N.dmax); dfree(N.bmax); dfree(N.status); dfree(N.scan_ticket); dfree(N.scan_state);
+    dfree(N.dmax); dfree(N.bmax); dfree(N.dpend); dfree(N.status); dfree(N.scan_ticket); dfree(N.scan_state);
     if (N.host) (void)hipHostFree(N.host);
     delete s->nl;
     s->nl = nullptr;
@@ -778,7 +817,6 @@ hipError_t nl_rebuild(mavi_sys *s, hipStream_t st) {
         hipLaunchKernelGGL(k_nl_keys, dim3(nblk(P.na, 256)), dim3(256), 0, st, P, s->P_[c], N.key[0], N.ccnt,
                            N.chg[0], N.chg[1], N.xb, s->flags);
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        if ((e = launch_fill_i32((int32_t *)N.dmax, 0, 2, st)) != hipSuccess) return e;
         if ((e = launch_fill_i32((int32_t *)N.bmax, 0, 2 * (int64_t)N.nbmax, st)) != hipSuccess) return e;
     }
     // the first evaluation and drift of the first step
@@ -793,7 +831,7 @@ hipError_t nl_rebuild(mavi_sys *s, hipStream_t st) {
     }
     N.unsettled = false;
     N.valid = true;
-    N.d_last = 0.0;
+    N.d_start = 0.0;  // the first block's positions: the build's plus the drift steps
     N.rebuilds++;
     return hipSuccess;
 }
@@ -802,15 +840,17 @@ hipError_t nl_settle(mavi_sys *s, hipStream_t st) {
     NList &N = *s->nl;
     if (!N.unsettled) return hipSuccess;
     // the last step drifted into px[(ph - 1) % 2]
-    hipLaunchKernelGGL(k_nl_settle, dim3(nblk(s->P.na, 256)), dim3(256), 0, st, s->P, N.px[(N.ph + 1) % 2],
-                       s->P_[s->cur]);
+    // the last step drifted into px[(ph - 1) % 2] what it evaluated and into
+    // px[ph % 2] the next positions
+    hipLaunchKernelGGL(k_nl_settle, dim3(N.nbmax), dim3(kNeedBlock), 0, st, s->P, N.px[(N.ph + 1) % 2],
+                       N.px[N.ph % 2], N.xb, s->P_[s->cur], N.dpend);
     N.unsettled = false;
     return hipGetLastError();
 }
 
 hipError_t nl_reduce_dmax(mavi_sys *s, hipStream_t st) {
     NList &N = *s->nl;
-    hipLaunchKernelGGL(k_nl_reduce, dim3(1), dim3(1024), 0, st, N.bmax, N.nbmax, N.dmax);
+    hipLaunchKernelGGL(k_nl_reduce, dim3(1), dim3(1024), 0, st, N.bmax, N.dpend, N.nbmax, N.dmax);
     return hipGetLastError();
 }
 
