@@ -318,7 +318,9 @@ int mavi_sys_noise_set(mavi_sys *sys, uint64_t seed, uint64_t counter);
  *                       summed in ascending partner id; lists rebuilt when a
  *                       particle has moved half the skin.  0: the chunk gather
  *   MAVI_OPT_NLIST_SKIN skin in thousandths of the range (0: 150)
- *   MAVI_OPT_NLIST_BLOCK steps between displacement checks (0: 30) */
+ *   MAVI_OPT_NLIST_BLOCK steps between displacement checks (0: as many as
+ *                       the displacement bound allows at the last measured
+ *                       step length, at most 60) */
 #define MAVI_OPT_NLIST 4
 #define MAVI_OPT_NLIST_SKIN 5
 #define MAVI_OPT_NLIST_BLOCK 6

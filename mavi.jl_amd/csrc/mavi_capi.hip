@@ -830,12 +830,14 @@ static bool invasions_due(const mavi_sys *s, int64_t ahead) {
 }
 
 // Neighbour-list stepping of a finite-range plain system: blocks of steps
-// (six-step graphs), after each block one read back of the largest
-// displacement since the last build.  A block in which some particle moved
-// more than skin / 2 may have missed a pair: its start state is restored,
-// the lists are rebuilt there and the block runs again (shorter).  The lists
-// are rebuilt ahead when the displacement would reach skin / 2 within the
-// next block at the rate of the last one.
+// (six-step graphs), after each block one read back of the largest drift
+// step and of the largest displacement of the next positions since the last
+// build.  A block in which some particle may have moved more than skin / 2
+// may have missed a pair: its start state is restored, the lists are rebuilt
+// there and the block runs again (shorter).  A block runs as many steps as
+// the bound allows at the last block's step length (block option 0), or a
+// fixed count; the lists are rebuilt ahead when fewer than kNlMinBlock more
+// steps (fixed: the next block at 1.2 times the step length) would fit.
 static bool nl_active(const mavi_sys *s) { return s->nl && s->nl->on && !s->nl->disabled; }
 
 static hipError_t nl_graph(mavi_sys *s, hipStream_t st) {
@@ -866,6 +868,22 @@ static void nl_drop_graph(mavi_sys *s) {
     }
 }
 
+constexpr int64_t kNlMinBlock = 4;  // shorter blocks cost more than the rebuild they put off
+
+// The largest drift step of the last three blocks (0: none measured yet).
+static double nl_step_est(const NList &N) {
+    return std::max(N.step_hist[0], std::max(N.step_hist[1], N.step_hist[2]));
+}
+
+// Steps from a block start (d_start from the build, `fresh`: the block starts
+// with the build's drift) that keep d_start + (steps - 1 + fresh) x step
+// within half_skin, at a step 1.08 times the estimate.
+static int64_t nl_room(const NList &N, double half_skin, bool fresh) {
+    const double r = (half_skin - N.d_start) / (1.08 * nl_step_est(N));
+    if (!(r < 1e9)) return INT64_MAX;
+    return (int64_t)floor(r) + (fresh ? 0 : 1);
+}
+
 static int nl_steps(mavi_sys *s, int64_t nsteps, int64_t *left) {
     mavi_ctx *ctx = s->ctx;
     hipStream_t st = ctx->stream;
@@ -878,12 +896,16 @@ static int nl_steps(mavi_sys *s, int64_t nsteps, int64_t *left) {
     while (k < nsteps) {
         const bool fresh = !N.valid;
         if (fresh) HIPCHK(ctx, mavi::nl_rebuild(s, st));  // resets the step maxima, then the first drift
-        else HIPCHK(ctx, mavi::launch_fill_i32((int32_t *)N.bmax, 0, 2 * (int64_t)N.nbmax, st));
-        const int64_t blk = std::min<int64_t>(N.block_now, nsteps - k);
-        // the block's start state, to roll back to
-        HIPCHK(ctx, hipMemcpyAsync(N.psnap, s->P_[s->cur], bytes, hipMemcpyDeviceToDevice, st));
+        int64_t want = N.block_now;
+        if (N.block == 0) {
+            if (nl_step_est(N) > 0.0) want = std::min(want, std::max<int64_t>(1, nl_room(N, half_skin, fresh)));
+            else want = std::min<int64_t>(want, 8);  // no step length measured yet
+        }
+        const int64_t blk = std::min<int64_t>(want, nsteps - k);
+        // the block's start state, to roll back to (the state's positions are
+        // written only after the check)
+        HIPCHK(ctx, mavi::nl_block_start(s, st, !fresh));
         HIPCHK(ctx, hipMemcpyAsync(N.vsnap, s->V_[s->cur], bytes, hipMemcpyDeviceToDevice, st));
-        HIPCHK(ctx, hipMemcpyAsync(N.fsnap, s->flags, sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
         int64_t done = 0;
         const bool graphs = s->use_graphs && !s->prof;
         while (done < blk) {
@@ -897,24 +919,29 @@ static int nl_steps(mavi_sys *s, int64_t nsteps, int64_t *left) {
                 done += 1;
             }
         }
-        HIPCHK(ctx, mavi::nl_settle(s, st));
-        HIPCHK(ctx, mavi::nl_reduce_dmax(s, st));
-        HIPCHK(ctx, hipMemcpyAsync(N.host, N.dmax, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-        HIPCHK(ctx, hipMemcpyAsync((char *)N.host + 16, N.status, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(ctx, mavi::nl_measure(s, st));
+        HIPCHK(ctx, hipMemcpyAsync(N.host, N.dmax, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
         HIPCHK(ctx, hipStreamSynchronize(st));
         const double step = sqrt(__builtin_bit_cast(double, N.host[0]));
         const double d_end = sqrt(__builtin_bit_cast(double, N.host[1]));
         const uint32_t status = (uint32_t)N.host[2];
-        // every position the block evaluated is within d_start + blk steps
-        // of the largest length from its build position
-        const double bound = N.d_start + (double)blk * step;
+        N.step_hist[N.step_i] = step;
+        N.step_i = (N.step_i + 1) % 3;
+        // The block evaluated positions y_0 .. y_{T-1} and drifted to y_T, each
+        // drift at most `step` long; y_0 is within d_start of the build
+        // positions (a block after a build starts with the build positions
+        // y_{-1}, d_start = 0), y_T within d_end.  So y_t is within
+        // min(d_start + u step, d_end + (U - u) step), u = t (+1 after a
+        // build), U = T (+1): at most the two lines' crossing, and at most
+        // d_start + (U - 1) step.
+        const double U = (double)blk + (fresh ? 1.0 : 0.0);
+        const double bound = std::min(N.d_start + (U - 1.0) * step, 0.5 * (N.d_start + d_end + U * step));
         if (status || bound > half_skin) {
             // a step of this block may have used an incomplete list: back to its start
-            HIPCHK(ctx, hipMemcpyAsync(s->P_[s->cur], N.psnap, bytes, hipMemcpyDeviceToDevice, st));
             HIPCHK(ctx, hipMemcpyAsync(s->V_[s->cur], N.vsnap, bytes, hipMemcpyDeviceToDevice, st));
             HIPCHK(ctx, hipMemcpyAsync(s->flags, N.fsnap, sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
             N.valid = false;
-            N.unsettled = false;
+            N.unsettled = false;  // the state's positions are the block start's
             N.rollbacks++;
             // a list overflowed, a chunk count may have passed the capacity
             // (the chunk gather checks it exactly), or one step from a fresh
@@ -929,15 +956,22 @@ static int nl_steps(mavi_sys *s, int64_t nsteps, int64_t *left) {
                 *left = nsteps - k;  // for the caller's chunk gather
                 return MAVI_OK;
             }
-            if (N.block_now > 1) N.block_now = std::max(1, N.block_now / 2);
+            // (a block sized by the step estimate is shorter next time: the
+            // estimate now holds this block's step)
+            if (N.block != 0 && N.block_now > 1) N.block_now = std::max(1, N.block_now / 2);
             continue;
         }
+        HIPCHK(ctx, mavi::nl_settle(s, st));
         k += blk;
         N.d_start = d_end;
-        // rebuild ahead when the next block's bound would pass skin / 2 at
-        // this step length
-        if (d_end + 1.2 * step * (double)N.block_now > half_skin) N.valid = false;
-        if (N.block_now < N.block) N.block_now = std::min(N.block, 2 * N.block_now);
+        // rebuild ahead when too few steps would fit before the bound
+        if (N.block == 0) {
+            if (nl_room(N, half_skin, false) < kNlMinBlock) N.valid = false;
+        } else if (d_end + 1.2 * step * (double)N.block_now > half_skin) {
+            N.valid = false;
+        }
+        const int32_t cap = N.block == 0 ? kNlBlockCap : N.block;
+        if (N.block_now < cap) N.block_now = std::min(cap, 2 * N.block_now);
     }
     s->sorted = false;   // not in reference chunk order
     s->f_valid = false;  // the chunk gather's reuse does not apply to this order
@@ -1499,7 +1533,10 @@ int mavi_sys_set_option(mavi_sys *s, int32_t option, int32_t value) {
             break;
         case MAVI_OPT_NLIST_BLOCK:
             if (value < 0 || value > 4096) return set_err(s->ctx, MAVI_E_BADARG, "block: 0 (default) .. 4096 steps");
-            if (s->nl) s->nl->block = s->nl->block_now = value == 0 ? 30 : value;
+            if (s->nl) {
+                s->nl->block = (int32_t)value;
+                s->nl->block_now = value == 0 ? kNlBlockCap : (int32_t)value;
+            }
             break;
         default:
             return set_err(s->ctx, MAVI_E_BADARG, "unknown option");

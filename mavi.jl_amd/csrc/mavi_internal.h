@@ -160,6 +160,8 @@ struct ProfSlot {
     int64_t launches = 0;
 };
 
+constexpr int32_t kNlBlockCap = 60;  // most steps between host checks of the list validity
+
 // Verlet neighbour lists of a finite-range plain system (LJ with a cutoff,
 // HarmTrunc) whose chunks are at least the interaction range wide (mavi_nlist.hip).
 struct NList {
@@ -172,8 +174,10 @@ struct NList {
     double thr2 = 0.0;        // displacement bound squared (skin / 2)^2, with margin
     double edge = 0.0;        // within this distance of a periodic edge: minimum image
     int32_t kmax = 48;        // entries per particle
-    int32_t block = 30;       // steps between host checks (option)
-    int32_t block_now = 30;   // current block (halved after a rollback)
+    int32_t block = 0;        // steps between host checks (option; 0: as many as the bound allows, at most kNlBlockCap)
+    int32_t block_now = kNlBlockCap;  // current largest block (halved after a rollback)
+    double step_hist[3] = {0.0, 0.0, 0.0};  // the largest drift step of each of the last three blocks
+    int32_t step_i = 0;
     SysParams G{};            // internal grid for sorting and building (cells >= range + skin)
     int32_t *nbr = nullptr;   // (kmax / 4) x na groups of 4 16-bit partner offsets (int2), ascending partner id, padded with the particle
     int32_t *ncount = nullptr;// entries per slot; bit 31: minimum image needed
@@ -186,9 +190,9 @@ struct NList {
     double2 *px[2] = {nullptr, nullptr};    // drifted positions x_{n+1}, x_{n+2} (before the periodic wrap)
     bool unsettled = false;                 // the state's positions lag the drifted ones (nl_settle)
     double2 *xb = nullptr;    // positions at build
-    double2 *psnap = nullptr, *vsnap = nullptr;  // block-start state (rollback)
+    double2 *vsnap = nullptr;  // block-start velocities (rollback; positions are written after the check)
     uint32_t *fsnap = nullptr;
-    unsigned long long *dmax = nullptr;  // reduced per block (bits): [0] largest drift step^2, [1] largest pending displacement^2
+    unsigned long long *dmax = nullptr;  // reduced per block (bits): [0] largest drift step^2, [1] largest pending displacement^2, [2] status
     double *bmax = nullptr;              // per workgroup of the evaluations: largest drift step^2 since the block began
     double *dpend = nullptr;             // per workgroup: largest displacement^2 of the pending positions (block end)
     int32_t nbmax = 0;
@@ -362,11 +366,14 @@ hipError_t nl_alloc(mavi_sys *s, hipStream_t st);
 void nl_free(mavi_sys *s);
 hipError_t nl_rebuild(mavi_sys *s, hipStream_t st);
 hipError_t nl_step(mavi_sys *s, hipStream_t st);
-// the state's positions <- the wrap of the last evaluated drift, and the
-// pending positions' displacement maxima (after steps)
+// block start: step maxima cleared (clear) and the flags word kept
+hipError_t nl_block_start(mavi_sys *s, hipStream_t st, bool clear);
+// block end: dmax <- the largest drift step, the largest displacement of the
+// pending positions from the build, the status word
+hipError_t nl_measure(mavi_sys *s, hipStream_t st);
+// after an accepted block: the state's positions <- the wrap of the last
+// evaluated drift
 hipError_t nl_settle(mavi_sys *s, hipStream_t st);
-// dmax <- the largest drift step and pending displacement (per-workgroup maxima)
-hipError_t nl_reduce_dmax(mavi_sys *s, hipStream_t st);
 void nl_set_skin(mavi_sys *s, double skin_frac);
 hipError_t launch_export_chunks(mavi_sys *s, int64_t *d_cp, int64_t *d_counts, hipStream_t st);
 hipError_t launch_forces_only(mavi_sys *s, hipStream_t st);

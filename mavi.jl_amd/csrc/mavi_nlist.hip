@@ -264,17 +264,31 @@ __device__ __forceinline__ void wave_bitonic(uint32_t (&key)[H]) {
     }
 }
 
+constexpr int kBuildRows = 16;  // particles whose lists one build wave stages in LDS at a time
+
+// Wave-scope ordering of the wave's own LDS writes before its reads.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // One cell's lists.  The wave's candidates (the cell's 3x3 stencil, at most
-// kBuildCand, staged in LDS by candidate index) are sorted by id; then, for
-// each particle of the cell, every lane tests its candidates against the list
-// radius and the passing ones, ranked by ballot in id order, are written as
-// 16-bit offsets in groups of 4, the last group padded with offset 0 (the
-// particle itself).
-template <int H>
-__device__ __forceinline__ void build_cell(const SysParams &P, const double2 *__restrict__ pos,
-                                           const int32_t *__restrict__ ID, int t0, int t1, int M,
-                                           const int32_t *w_slot, const double2 *w_pos, int16_t *__restrict__ nbr16, int32_t *__restrict__ ncount, double rl2,
-                                           double edge, uint32_t *__restrict__ status) {
+// kBuildCand, staged in LDS by candidate index, the cell's own particles
+// among them at candidate ec0 + (i - t0); NEAR: periodic images already the
+// nearest to every particle of the cell, or a walled box) are sorted by id;
+// then, for each particle of the cell, every lane tests its candidates
+// against the list radius and the passing ones, ranked by ballot in id order,
+// go to the particle's LDS row as 16-bit offsets, the last group of 4 padded
+// with offset 0 (the particle itself).  kBuildRows rows at a time are written
+// out as whole groups (8 bytes, consecutive particles of a group row
+// adjacent).
+template <int H, bool NEAR>
+__device__ __forceinline__ void build_cell(const SysParams &P, const int32_t *__restrict__ ID, int t0, int t1,
+                                           int M, int ec0, const int32_t *w_slot, const double2 *w_pos,
+                                           int16_t (*w_list)[kNlMax], int32_t *w_groups, int2 *__restrict__ nbr,
+                                           int32_t *__restrict__ ncount, double rl2, double edge,
+                                           uint32_t *__restrict__ status) {
     const int lane = (int)lane_id();
     const int na = P.na;
     uint32_t key[H];
@@ -285,66 +299,88 @@ __device__ __forceinline__ void build_cell(const SysParams &P, const double2 *__
         if (e < M) key[h] = ((uint32_t)ID[w_slot[e]] << 7) | (uint32_t)e;
     }
     wave_bitonic<H>(key);
-    int js[H];
+    // per candidate: its slot, the slot's offset from the cell's first slot
+    // (modulo na, in [-na/2, na/2]) and position (no candidate: slot -1,
+    // infinitely far)
+    int js[H], d0[H];
     double2 pj[H];
-    bool ok[H];
 #pragma unroll
     for (int h = 0; h < H; h++) {
-        ok[h] = key[h] != 0xffffffffu;
-        const int e = ok[h] ? (int)(key[h] & 127u) : 0;
-        js[h] = w_slot[e];
-        pj[h] = w_pos[e];
+        const bool ok = key[h] != 0xffffffffu;
+        const int e = ok ? (int)(key[h] & 127u) : 0;
+        js[h] = ok ? w_slot[e] : -1;
+        d0[h] = slot_offset(t0, js[h], na);
+        pj[h] = ok ? w_pos[e] : make_double2(INFINITY, INFINITY);
     }
     const unsigned long long lt = (1ull << lane) - 1ull;
-    for (int i = t0; i < t1; i++) {
-        const double2 pi = pos[i];
-        bool pass[H];
-        unsigned long long m[H];
+    for (int b0 = t0; b0 < t1; b0 += kBuildRows) {
+        const int nb = min(kBuildRows, t1 - b0);
+        int my_count = 0;  // lane p < nb: entries of particle b0 + p
+        int ng_max = 0;    // groups of the longest staged list
+        for (int p = 0; p < nb; p++) {
+            const int i = b0 + p, q = i - t0;
+            const double2 pi = w_pos[ec0 + q];
+            bool pass[H];
+            unsigned long long m[H];
 #pragma unroll
-        for (int h = 0; h < H; h++) {
-            double dx = pi.x - pj[h].x, dy = pi.y - pj[h].y;
-            if (P.periodic) {
-                dx = min_image(dx, P.L, P.halfL);
-                dy = min_image(dy, P.H, P.halfH);
+            for (int h = 0; h < H; h++) {
+                double dx = pi.x - pj[h].x, dy = pi.y - pj[h].y;
+                if (!NEAR) {
+                    dx = min_image(dx, P.L, P.halfL);
+                    dy = min_image(dy, P.H, P.halfH);
+                }
+                pass[h] = js[h] != i && dx * dx + dy * dy <= rl2;
+                m[h] = __ballot(pass[h]);
             }
-            pass[h] = ok[h] && js[h] != i && dx * dx + dy * dy <= rl2;
-            m[h] = __ballot(pass[h]);
-        }
-        int cnt = 0, rank[H];
-        if (H == 1) {
-            rank[0] = __popcll(m[0] & lt);
-            cnt = __popcll(m[0]);
-        } else {
-            rank[0] = __popcll(m[0] & lt) + __popcll(m[H - 1] & lt);
-            rank[H - 1] = rank[0] + (pass[0] ? 1 : 0);
-            cnt = __popcll(m[0]) + __popcll(m[H - 1]);
-        }
-        bool far = false;
+            int cnt = 0, rank[H];
+            if (H == 1) {
+                rank[0] = __popcll(m[0] & lt);
+                cnt = __popcll(m[0]);
+            } else {
+                rank[0] = __popcll(m[0] & lt) + __popcll(m[H - 1] & lt);
+                rank[H - 1] = rank[0] + (pass[0] ? 1 : 0);
+                cnt = __popcll(m[0]) + __popcll(m[H - 1]);
+            }
+            bool far = false;
 #pragma unroll
-        for (int h = 0; h < H; h++) {
-            if (pass[h] && rank[h] < kNlMax) {
-                const int32_t d = slot_offset(i, js[h], na);
-                far = far || d > 32767 || d < -32768;
-                nbr16[((int64_t)(rank[h] >> 2) * na + i) * 4 + (rank[h] & 3)] = (int16_t)d;
+            for (int h = 0; h < H; h++) {
+                if (pass[h] && rank[h] < kNlMax) {
+                    // slot_offset(i, js, na): d0 - q is at most na / 2
+                    int32_t d = d0[h] - q;
+                    if (d < -(na / 2)) d += na;
+                    far = far || d > 32767 || d < -32768;
+                    w_list[p][rank[h]] = (int16_t)d;
+                }
             }
+            const bool over = cnt > kNlMax;
+            if (over) cnt = kNlMax;
+            // pads of the last group: offset 0, the particle itself
+            const int pad = (((cnt + 3) >> 2) << 2) - cnt;
+            if (lane < pad) w_list[p][cnt + lane] = 0;
+            if (__any(far) || over) {
+                if (lane == 0) atomicOr(status, kStatusOverflow);
+            }
+            if (lane == p) my_count = cnt;
+            if (lane == 0) w_groups[p] = (cnt + 3) >> 2;
+            ng_max = max(ng_max, (cnt + 3) >> 2);
         }
-        const bool over = cnt > kNlMax;
-        if (over) cnt = kNlMax;
-        // pads of the last group: offset 0, the particle itself
-        const int pad = (((cnt + 3) >> 2) << 2) - cnt;
-        if (lane < pad) nbr16[((int64_t)((cnt + lane) >> 2) * na + i) * 4 + ((cnt + lane) & 3)] = 0;
-        if (__any(far) || over) {
-            if (lane == 0) atomicOr(status, kStatusOverflow);
-        }
-        if (lane == 0) {
+        wave_lds_sync();
+        if (lane < nb) {
             bool near_edge = false;
             if (P.periodic) {
+                const double2 pi = w_pos[ec0 + (b0 - t0) + lane];
                 const double ex = fmin(pi.x - P.blx, (P.blx + P.L) - pi.x);
                 const double ey = fmin(pi.y - P.bly, (P.bly + P.H) - pi.y);
                 near_edge = !(ex >= edge && ey >= edge);
             }
-            ncount[i] = (int32_t)((uint32_t)cnt | (near_edge ? kEdgeBit : 0u));
+            ncount[b0 + lane] = (int32_t)((uint32_t)my_count | (near_edge ? kEdgeBit : 0u));
         }
+        // group g of particle p from lane g * kBuildRows + p
+        for (int l = lane; l < ng_max * kBuildRows; l += 64) {
+            const int g = l / kBuildRows, p = l % kBuildRows;
+            if (p < nb && g < w_groups[p]) nbr[(int64_t)g * na + b0 + p] = *(const int2 *)&w_list[p][4 * g];
+        }
+        wave_lds_sync();  // the rows are staged again for the next particles
     }
 }
 
@@ -362,28 +398,47 @@ __global__ __launch_bounds__(256) void k_nl_build(SysParams G, SysParams P, cons
                                                   uint32_t *__restrict__ status) {
     __shared__ int32_t w_slot[4][kBuildCand];
     __shared__ double2 w_pos[4][kBuildCand];
-    const int wid = (int)(threadIdx.x >> 6), lane = (int)lane_id();
-    const int cell = blockIdx.x * 4 + wid;
-    if (cell >= G.ncell) return;  // whole wave; no workgroup barrier below
+    __shared__ __align__(16) int16_t w_list[4][kBuildRows][kNlMax];
+    __shared__ int32_t w_groups[4][kBuildRows];
+    // the cell (row blockIdx.y, column 4 blockIdx.x + the wave's index):
+    // uniform, so the stencil is scalar work
+    const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = (int)lane_id();
+    const int row = (int)blockIdx.y, col = (int)blockIdx.x * 4 + wid;
+    if (col >= G.cols) return;  // whole wave; no workgroup barrier below
+    const int cell = row * G.cols + col;
     const int t0 = start[cell], t1 = start[cell + 1];
     if (t0 == t1) return;
-    const int row = cell / G.cols, col = cell - row * G.cols;
-    int rr[3];
-    stencil_rows(G, row, rr);
-    const ColRuns R = stencil_cols(G, col);
-    int rb[6], rn[6], M = 0;
+    int rb[6] = {0, 0, 0, 0, 0, 0}, rn[6] = {0, 0, 0, 0, 0, 0}, M = 0, ec0 = 0;
+    if (row >= 1 && row + 1 < G.rows && col >= 1 && col + 1 < G.cols) {
+        // an interior cell: three runs of three cells, the cell itself in
+        // the middle one
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
+        for (int d = 0; d < 3; d++) {
+            const int32_t *sr = start + (row - 1 + d) * G.cols + (col - 1);
+            rb[2 * d] = sr[0];
+            rn[2 * d] = sr[3] - sr[0];
+        }
+        M = rn[0] + rn[2] + rn[4];
+        ec0 = rn[0] + (t0 - rb[2]);
+    } else {
+        int rr[3];
+        stencil_rows(G, row, rr);
+        const ColRuns R = stencil_cols(G, col);
 #pragma unroll
-        for (int run = 0; run < 2; run++) {
-            const int k = 2 * d + run;
-            rb[k] = 0;
-            rn[k] = 0;
-            if (rr[d] < 0 || run >= R.n) continue;
-            const int ca = run ? R.a1 : R.a0, cb = run ? R.b1 : R.b0;
-            rb[k] = start[rr[d] * G.cols + ca];
-            rn[k] = start[rr[d] * G.cols + cb + 1] - rb[k];
-            M += rn[k];
+        for (int d = 0; d < 3; d++) {
+#pragma unroll
+            for (int run = 0; run < 2; run++) {
+                const int k = 2 * d + run;
+                if (rr[d] < 0 || run >= R.n) continue;
+                const int ca = run ? R.a1 : R.a0, cb = run ? R.b1 : R.b0;
+                rb[k] = start[rr[d] * G.cols + ca];
+                rn[k] = start[rr[d] * G.cols + cb + 1] - rb[k];
+                // the run holding the cell itself (every column and row of
+                // the stencil at most once): its first particle's candidate
+                // index
+                if (t0 >= rb[k] && t0 < rb[k] + rn[k]) ec0 = M + (t0 - rb[k]);
+                M += rn[k];
+            }
         }
     }
     if (M > kBuildCand) {
@@ -391,6 +446,13 @@ __global__ __launch_bounds__(256) void k_nl_build(SysParams G, SysParams P, cons
         if (lane == 0) atomicOr(status, kStatusOverflow);
         return;
     }
+    // On a periodic list grid of at least 7 x 7 cells the image of a
+    // candidate nearest the cell's first particle is the nearest to every
+    // particle of the cell (each is within a cell width of the first, each
+    // candidate within two of it, the box at least 7 wide): the candidates
+    // are staged as those images and the tests need no minimum image.
+    const bool near = !P.periodic || (G.cols >= 7 && G.rows >= 7);
+    const double2 ref = pos[t0];
     // candidate e -> slot, position (runs concatenated in order)
     for (int e = lane; e < M; e += 64) {
         int rest = e, j = 0;
@@ -399,19 +461,29 @@ __global__ __launch_bounds__(256) void k_nl_build(SysParams G, SysParams P, cons
             if (rest >= 0 && rest < rn[k]) j = rb[k] + rest;
             rest -= rn[k];
         }
+        double2 pj = pos[j];
+        if (P.periodic && near) {
+            const double dx = pj.x - ref.x, dy = pj.y - ref.y;
+            if (dx > P.halfL) pj.x -= P.L;
+            else if (dx < -P.halfL) pj.x += P.L;
+            if (dy > P.halfH) pj.y -= P.H;
+            else if (dy < -P.halfH) pj.y += P.H;
+        }
         w_slot[wid][e] = j;
-        w_pos[wid][e] = pos[j];
+        w_pos[wid][e] = pj;
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    int16_t *nbr16 = (int16_t *)nbr;
-    if (M <= 64)
-        build_cell<1>(P, pos, ID, t0, t1, M, w_slot[wid], w_pos[wid], nbr16, ncount, rl2,
-                      edge, status);
-    else
-        build_cell<2>(P, pos, ID, t0, t1, M, w_slot[wid], w_pos[wid], nbr16, ncount, rl2,
-                      edge, status);
+    wave_lds_sync();
+#define MAVI_BUILD_CELL(H, NEAR)                                                                               \
+    build_cell<H, NEAR>(P, ID, t0, t1, M, ec0, w_slot[wid], w_pos[wid], w_list[wid], w_groups[wid], nbr, ncount, \
+                        rl2, edge, status)
+    if (near) {
+        if (M <= 64) MAVI_BUILD_CELL(1, true);
+        else MAVI_BUILD_CELL(2, true);
+    } else {
+        if (M <= 64) MAVI_BUILD_CELL(1, false);
+        else MAVI_BUILD_CELL(2, false);
+    }
+#undef MAVI_BUILD_CELL
 }
 
 // After a build: the reference chunk of every particle (K_n), chunk counts
@@ -611,10 +683,11 @@ __global__ __launch_bounds__(kNeedBlock) void k_nl_second(SysParams P, NLArgs A)
     block_max_into(l2, A.bmax);
 }
 
-// out[0] <- max of bmax, out[1] <- max of dpend (one workgroup of 1024
-// threads; the doubles' bits).
+// out[0] <- max of bmax, out[1] <- max of dpend (the doubles' bits), out[2]
+// <- the status word (one workgroup of 1024 threads).
 __global__ __launch_bounds__(1024) void k_nl_reduce(const double *__restrict__ bmax, const double *__restrict__ dpend,
-                                                    int32_t n, unsigned long long *__restrict__ out) {
+                                                    int32_t n, const uint32_t *__restrict__ status,
+                                                    unsigned long long *__restrict__ out) {
     __shared__ double s_max[2][16];
     double m[2] = {0.0, 0.0};
     for (int i = threadIdx.x; i < n; i += 1024) {
@@ -634,19 +707,26 @@ __global__ __launch_bounds__(1024) void k_nl_reduce(const double *__restrict__ b
         for (int w = 1; w < 16; w++) v = fmax(v, s_max[k][w]);
         out[k] = (unsigned long long)__double_as_longlong(v);
     }
+    if (threadIdx.x == 2) out[2] = *status;
 }
 
-// At a block end: the state's positions (the wrap of the last evaluated
-// drift, px) and, per workgroup, the largest squared displacement of the
+// At a block's start: the step maxima cleared (not after a build, whose drift
+// counts among the block's steps) and the flags word kept for a rollback.
+__global__ void k_nl_block_start(double *__restrict__ bmax, int32_t n, int32_t clear,
+                                 const uint32_t *__restrict__ flags, uint32_t *__restrict__ fsnap) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (clear && i < n) bmax[i] = 0.0;
+    if (i == 0) *fsnap = *flags;
+}
+
+// At a block's end, per workgroup: the largest squared displacement of the
 // next positions (pending, not yet evaluated) from the build positions.
-__global__ __launch_bounds__(kNeedBlock) void k_nl_settle(SysParams P, const double2 *__restrict__ px,
-                                                         const double2 *__restrict__ pending,
-                                                         const double2 *__restrict__ xb, double2 *__restrict__ pst,
-                                                         double *__restrict__ dpend) {
+__global__ __launch_bounds__(kNeedBlock) void k_nl_measure(SysParams P, const double2 *__restrict__ pending,
+                                                          const double2 *__restrict__ xb,
+                                                          double *__restrict__ dpend) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     double d2 = 0.0;
     if (s < P.na) {
-        pst[s] = state_pos(P, px[s]);
         const double2 q = state_pos(P, pending[s]), b = xb[s];
         double dx = q.x - b.x, dy = q.y - b.y;
         if (P.periodic) {
@@ -665,6 +745,13 @@ __global__ __launch_bounds__(kNeedBlock) void k_nl_settle(SysParams P, const dou
         for (int w = 1; w < kNeedBlock / 64; w++) mx = fmax(mx, s_max[w]);
         dpend[blockIdx.x] = mx;
     }
+}
+
+// After an accepted block: the state's positions, the wrap of the last
+// evaluated drift px.
+__global__ void k_nl_settle(SysParams P, const double2 *__restrict__ px, double2 *__restrict__ pst) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < P.na) pst[s] = state_pos(P, px[s]);
 }
 
 NLArgs nl_args(mavi_sys *s) {
@@ -758,10 +845,9 @@ hipError_t nl_alloc(mavi_sys *s, hipStream_t st) {
     if (e == hipSuccess) e = dalloc(&N.fk_on, na, st);
     if (e == hipSuccess) e = dalloc(&N.ccnt, (int64_t)s->P.ncell, st);
     if (e == hipSuccess) e = dalloc(&N.xb, na, st);
-    if (e == hipSuccess) e = dalloc(&N.psnap, na, st);
     if (e == hipSuccess) e = dalloc(&N.vsnap, na, st);
     if (e == hipSuccess) e = dalloc(&N.fsnap, 1, st);
-    if (e == hipSuccess) e = dalloc(&N.dmax, 2, st);
+    if (e == hipSuccess) e = dalloc(&N.dmax, 3, st);
     N.nbmax = nblk(s->P.na, kNeedBlock);  // workgroups of the second evaluation
     if (e == hipSuccess) e = dalloc(&N.bmax, N.nbmax, st);
     if (e == hipSuccess) e = dalloc(&N.dpend, N.nbmax, st);
@@ -788,7 +874,7 @@ void nl_free(mavi_sys *s) {
         dfree(base);
         p = nullptr;
     }
-    dfree(N.fk); dfree(N.fk_on); dfree(N.xb); dfree(N.psnap); dfree(N.vsnap); dfree(N.fsnap);
+    dfree(N.fk); dfree(N.fk_on); dfree(N.xb); dfree(N.vsnap); dfree(N.fsnap);
     dfree(N.dmax); dfree(N.bmax); dfree(N.dpend); dfree(N.status); dfree(N.scan_ticket); dfree(N.scan_state);
     if (N.host) (void)hipHostFree(N.host);
     delete s->nl;
@@ -810,7 +896,7 @@ hipError_t nl_rebuild(mavi_sys *s, hipStream_t st) {
     const int c = s->cur;
     {
         ProfScope ps(s, MAVI_PROF_BIN, st);
-        hipLaunchKernelGGL(k_nl_build, dim3(nblk(N.G.ncell, 4)), dim3(256), 0, st, N.G, P, s->P_[c], s->ID_[c],
+        hipLaunchKernelGGL(k_nl_build, dim3(nblk(N.G.cols, 4), N.G.rows), dim3(256), 0, st, N.G, P, s->P_[c], s->ID_[c],
                            s->start, (int2 *)N.nbr, N.ncount, N.rl2, N.edge, N.status);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if ((e = launch_fill_i32(N.ccnt, 0, P.ncell, st)) != hipSuccess) return e;
@@ -836,21 +922,30 @@ hipError_t nl_rebuild(mavi_sys *s, hipStream_t st) {
     return hipSuccess;
 }
 
-hipError_t nl_settle(mavi_sys *s, hipStream_t st) {
+hipError_t nl_block_start(mavi_sys *s, hipStream_t st, bool clear) {
     NList &N = *s->nl;
-    if (!N.unsettled) return hipSuccess;
-    // the last step drifted into px[(ph - 1) % 2]
-    // the last step drifted into px[(ph - 1) % 2] what it evaluated and into
-    // px[ph % 2] the next positions
-    hipLaunchKernelGGL(k_nl_settle, dim3(N.nbmax), dim3(kNeedBlock), 0, st, s->P, N.px[(N.ph + 1) % 2],
-                       N.px[N.ph % 2], N.xb, s->P_[s->cur], N.dpend);
-    N.unsettled = false;
+    hipLaunchKernelGGL(k_nl_block_start, dim3(nblk(N.nbmax, 256)), dim3(256), 0, st, N.bmax, N.nbmax,
+                       clear ? 1 : 0, s->flags, N.fsnap);
     return hipGetLastError();
 }
 
-hipError_t nl_reduce_dmax(mavi_sys *s, hipStream_t st) {
+hipError_t nl_measure(mavi_sys *s, hipStream_t st) {
     NList &N = *s->nl;
-    hipLaunchKernelGGL(k_nl_reduce, dim3(1), dim3(1024), 0, st, N.bmax, N.dpend, N.nbmax, N.dmax);
+    // the last step drifted into px[(ph - 1) % 2] what it evaluated and into
+    // px[ph % 2] the next positions
+    hipLaunchKernelGGL(k_nl_measure, dim3(N.nbmax), dim3(kNeedBlock), 0, st, s->P, N.px[N.ph % 2], N.xb, N.dpend);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_nl_reduce, dim3(1), dim3(1024), 0, st, N.bmax, N.dpend, N.nbmax, N.status, N.dmax);
+    return hipGetLastError();
+}
+
+hipError_t nl_settle(mavi_sys *s, hipStream_t st) {
+    NList &N = *s->nl;
+    if (!N.unsettled) return hipSuccess;
+    hipLaunchKernelGGL(k_nl_settle, dim3(nblk(s->P.na, 256)), dim3(256), 0, st, s->P, N.px[(N.ph + 1) % 2],
+                       s->P_[s->cur]);
+    N.unsettled = false;
     return hipGetLastError();
 }
 
