@@ -286,7 +286,8 @@ __device__ __forceinline__ void wave_lds_sync() {
 template <int H, bool NEAR>
 __device__ __forceinline__ void build_cell(const SysParams &P, const int32_t *__restrict__ ID, int t0, int t1,
                                            int M, int ec0, const int32_t *w_slot, const double2 *w_pos,
-                                           int16_t (*w_list)[kNlMax], int32_t *w_groups, int2 *__restrict__ nbr,
+                                           int16_t (*w_list)[kNlMax], int16_t *w_dump, int32_t *w_groups,
+                                           int2 *__restrict__ nbr,
                                            int32_t *__restrict__ ncount, double rl2, double edge,
                                            uint32_t *__restrict__ status) {
     const int lane = (int)lane_id();
@@ -341,29 +342,30 @@ __device__ __forceinline__ void build_cell(const SysParams &P, const int32_t *__
                 rank[H - 1] = rank[0] + (pass[0] ? 1 : 0);
                 cnt = __popcll(m[0]) + __popcll(m[H - 1]);
             }
+            const bool over = cnt > kNlMax;
+            const int cw = over ? kNlMax : cnt;
+            // every lane writes (no branches): an entry at its rank, else its
+            // dump slot; then a pad of the last group (offset 0, the particle
+            // itself), else the dump slot
             bool far = false;
 #pragma unroll
             for (int h = 0; h < H; h++) {
-                if (pass[h] && rank[h] < kNlMax) {
-                    // slot_offset(i, js, na): d0 - q is at most na / 2
-                    int32_t d = d0[h] - q;
-                    if (d < -(na / 2)) d += na;
-                    far = far || d > 32767 || d < -32768;
-                    w_list[p][rank[h]] = (int16_t)d;
-                }
+                // slot_offset(i, js, na): d0 - q is at most na / 2
+                int32_t d = d0[h] - q;
+                d = d < -(na / 2) ? d + na : d;
+                const bool put = pass[h] && rank[h] < kNlMax;
+                far = far || (put && (d > 32767 || d < -32768));
+                *(put ? &w_list[p][rank[h]] : &w_dump[lane]) = (int16_t)d;
             }
-            const bool over = cnt > kNlMax;
-            if (over) cnt = kNlMax;
-            // pads of the last group: offset 0, the particle itself
-            const int pad = (((cnt + 3) >> 2) << 2) - cnt;
-            if (lane < pad) w_list[p][cnt + lane] = 0;
+            const int pad = (((cw + 3) >> 2) << 2) - cw;
+            *(lane < pad ? &w_list[p][cw + lane] : &w_dump[lane]) = 0;
             if (__any(far) || over) {
                 if (lane == 0) atomicOr(status, kStatusOverflow);
             }
-            if (lane == p) my_count = cnt;
-            if (lane == 0) w_groups[p] = (cnt + 3) >> 2;
-            ng_max = max(ng_max, (cnt + 3) >> 2);
+            my_count = lane == p ? cw : my_count;
+            ng_max = max(ng_max, (cw + 3) >> 2);
         }
+        if (lane < nb) w_groups[lane] = (my_count + 3) >> 2;
         wave_lds_sync();
         if (lane < nb) {
             bool near_edge = false;
@@ -400,6 +402,7 @@ __global__ __launch_bounds__(256) void k_nl_build(SysParams G, SysParams P, cons
     __shared__ double2 w_pos[4][kBuildCand];
     __shared__ __align__(16) int16_t w_list[4][kBuildRows][kNlMax];
     __shared__ int32_t w_groups[4][kBuildRows];
+    __shared__ int16_t w_dump[4][64];
     // the cell (row blockIdx.y, column 4 blockIdx.x + the wave's index):
     // uniform, so the stencil is scalar work
     const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = (int)lane_id();
@@ -474,8 +477,8 @@ __global__ __launch_bounds__(256) void k_nl_build(SysParams G, SysParams P, cons
     }
     wave_lds_sync();
 #define MAVI_BUILD_CELL(H, NEAR)                                                                               \
-    build_cell<H, NEAR>(P, ID, t0, t1, M, ec0, w_slot[wid], w_pos[wid], w_list[wid], w_groups[wid], nbr, ncount, \
-                        rl2, edge, status)
+    build_cell<H, NEAR>(P, ID, t0, t1, M, ec0, w_slot[wid], w_pos[wid], w_list[wid], w_dump[wid], w_groups[wid], nbr, \
+                        ncount, rl2, edge, status)
     if (near) {
         if (M <= 64) MAVI_BUILD_CELL(1, true);
         else MAVI_BUILD_CELL(2, true);
