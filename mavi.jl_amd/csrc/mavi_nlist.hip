@@ -579,19 +579,15 @@ __device__ __forceinline__ double nl_drift(const SysParams &P, const NLArgs &A, 
     return lx * lx + ly * ly;
 }
 
-// max over a kNeedBlock workgroup of v into slot[blockIdx.x] (kept across
-// launches: the larger of the stored and the new value).
+// max of v over the wave into slot[blockIdx.x] (kept across launches: the
+// larger of the stored and the new value; an atomic per wave, so a wave
+// retires without waiting for the rest of its workgroup; squared lengths are
+// non-negative doubles, ordered as their bit patterns).
 __device__ __forceinline__ void block_max_into(double v, double *__restrict__ slot) {
-    __shared__ double s_max[kNeedBlock / 64];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    if (lane_id() == 0) s_max[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double mx = s_max[0];
-        for (int w = 1; w < kNeedBlock / 64; w++) mx = fmax(mx, s_max[w]);
-        if (mx > slot[blockIdx.x]) slot[blockIdx.x] = mx;
-    }
+    if (lane_id() == 0)
+        atomicMax((unsigned long long *)&slot[blockIdx.x], (unsigned long long)__double_as_longlong(v));
 }
 
 // After a build: the first evaluation of newton_step! (forces at x_n over
@@ -681,8 +677,7 @@ __global__ __launch_bounds__(kNeedBlock) void k_nl_second(SysParams P, NLArgs A)
         A.chg[s] = 0;
         l2 = nl_drift(P, A, s, pw, make_double2(vx, vy), gx, gy, A.Px_next, A.knext, A.knn, A.chg_next);
     }
-    // the workgroup's largest step since the block began (no contended
-    // atomic on one word)
+    // the largest step since the block began, per workgroup slot
     block_max_into(l2, A.bmax);
 }
 
