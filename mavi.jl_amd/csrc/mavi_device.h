@@ -985,28 +985,49 @@ __device__ __forceinline__ void ring_lanes(const SysParams &P, const RingType &T
     const int64_t base = (int64_t)rr * P.nmax;
     const int slot = act ? slot_of[base + i] : 0;
     const double2 x = act ? Pst[slot] : make_double2(0.0, 0.0);
-    auto bc = [lane0](double v, int src) { return __shfl(v, lane0 + src, 64); };
+    (void)lane0;  // the ring's lanes start at thread tb below
 
     if constexpr (UPDATE) {
         if (ring_ok && i == 0) cms[r] = cms_next[r];
     }
+    // The ring's values that other lanes need (neighbours' positions and
+    // spring forces, and each sequential sum's terms) go through LDS: every
+    // lane writes its own, then reads those it needs (broadcast reads for
+    // the sums, in ring order), in place of cross-lane shuffles.  A ring's
+    // lanes lie in one wave, so a wave-scope fence orders the writes before
+    // the reads; each array is written again only after its last reads.
+    __shared__ double2 s_a[256 + 32], s_b[256 + 32], s_c[256 + 32];
+    __shared__ double s_t[256 + 32];
+    const int me = (int)threadIdx.x, tb = me - i;  // tb: the ring's first lane
+    auto lds_sync = []() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+
     // neighbours along the ring
     const int inext = (i == np - 1) ? 0 : i + 1;
     const int iprev = (i == 0) ? np - 1 : i - 1;
-    const double2 xn = make_double2(bc(x.x, inext), bc(x.y, inext));
-    const double2 xp = make_double2(bc(x.x, iprev), bc(x.y, iprev));
+    s_a[me] = x;
+    lds_sync();
+    const double2 xn = s_a[tb + inext];
+    const double2 xp = s_a[tb + iprev];
 
     // update_continuos_pos!: c_0 = x_0, c_i = c_{i-1} + calc_diff(x_i, x_{i-1})
     const double2 d = (i == 0) ? x : cdiff(P, x, xp);
+    double2 *const s_d = s_b;
+    s_d[me] = d;
+    lds_sync();
     double cx = 0.0, cy = 0.0;
+#pragma unroll 4
     for (int k = 0; k < np; k++) {
-        const double dx = bc(d.x, k), dy = bc(d.y, k);
+        const double2 dk = s_d[tb + k];
         if (k == 0) {
-            cx = dx;
-            cy = dy;
+            cx = dk.x;
+            cy = dk.y;
         } else if (k <= i) {
-            cx = cx + dx;
-            cy = cy + dy;
+            cx = cx + dk.x;
+            cy = cy + dk.y;
         }
     }
     if (!P.periodic) {  // update_continuos_pos! is a no-op: continuous = stored positions
@@ -1015,13 +1036,19 @@ __device__ __forceinline__ void ring_lanes(const SysParams &P, const RingType &T
     }
     if (act) cpos[base + i] = make_double2(cx, cy);
     // centre of mass and shoelace area, sums in ring order
-    const double cnx = bc(cx, inext), cny = bc(cy, inext);
-    const double term = cx * cny - cy * cnx;
+    s_c[me] = make_double2(cx, cy);
+    lds_sync();
+    const double2 cn = s_c[tb + inext];
+    const double term = cx * cn.y - cy * cn.x;
+    s_t[me] = term;
+    lds_sync();
     double sx = 0.0, sy = 0.0, area = 0.0;
+#pragma unroll 4
     for (int k = 0; k < np; k++) {
-        sx += bc(cx, k);
-        sy += bc(cy, k);
-        area += bc(term, k);
+        const double2 ck = s_c[tb + k];
+        sx += ck.x;
+        sy += ck.y;
+        area += s_t[tb + k];
     }
     area = area / 2.0;
     if (ring_ok && i == 0) {
@@ -1035,7 +1062,9 @@ __device__ __forceinline__ void ring_lanes(const SysParams &P, const RingType &T
     const double dist = sqrt(ds.x * ds.x + ds.y * ds.y);
     const double q = -T.kspring * (dist - T.lspring) / dist;
     const double2 fi = make_double2(q * ds.x, q * ds.y);
-    const double2 fprev = make_double2(bc(fi.x, iprev), bc(fi.y, iprev));
+    s_b[me] = fi;  // the unwrap's reads of s_b are done
+    lds_sync();
+    const double2 fprev = s_b[tb + iprev];
     double area0 = (double)np * T.lspring / T.p0;
     area0 = area0 * area0;
     const double fa = T.karea * (area - area0);
@@ -1066,10 +1095,17 @@ __device__ __forceinline__ void ring_lanes(const SysParams &P, const RingType &T
     sincos(theta, &sn, &cs);
     const double vx = T.vo * cs + T.mu * Fi.x;
     const double vy = T.vo * sn + T.mu * Fi.y;
+    double2 *const s_v = s_a;  // the neighbour reads of s_a are done
+    s_v[me] = make_double2(vx, vy);
+    lds_sync();
     double vcx = 0.0, vcy = 0.0;
-    for (int k = 0; k < np; k++) {
-        vcx += bc(vx, k);
-        vcy += bc(vy, k);
+    if (ring_ok && i == 0) {  // only the ring's first lane uses the mean velocity
+#pragma unroll 4
+        for (int k = 0; k < np; k++) {
+            const double2 vk = s_v[tb + k];
+            vcx += vk.x;
+            vcy += vk.y;
+        }
     }
     double px = x.x + vx * P.dt, py = x.y + vy * P.dt;
     if constexpr (WALLS) {
