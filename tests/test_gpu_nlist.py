@@ -185,3 +185,52 @@ def test_small_periodic_list_grid(gpu_ctx, oracle_mod, skin):
     ep, ev = traj_err(s.state.pos, s.state.vel, p_ref, v_ref, 3.2, periodic=True)
     assert ep <= TRAJ_RTOL and ev <= TRAJ_RTOL
     assert np.abs(s.forces - f_ref).max() <= 1e-9 * max(np.abs(f_ref).max(), 1e-300)
+
+
+@pytest.mark.parametrize("nside", [12, 15])
+def test_list_grid_without_nearest_image_staging(gpu_ctx, oracle_mod, nside):
+    """Periodic LJ boxes whose list grid is 4 x 4 (nside 12) or 6 x 6 (nside
+    15) cells: under 7 cells a side the build tests every candidate with the
+    minimum image (the nearest image of a candidate to the cell's first
+    particle need not be the nearest to the others), with interior cells on
+    the three-run stencil path; 100 steps at 4x the speed against the oracle."""
+    import mavi_hip as mv
+    from mavi_hip import workloads as wl
+
+    pos, vel, space, dyn, ic = wl.lj_periodic(nside=nside, density=0.7, seed=5, cutoff=True)
+    vel = vel * 4.0
+    s = mv.System(state=mv.SecondLawState(pos=pos.copy(), vel=vel.copy()), space_cfg=space, dynamic_cfg=dyn,
+                  int_cfg=ic)
+    s.advance(100)
+    s.download()
+    info = s.dev.nlist_info()
+    say(f"[nlist grid nside {nside}] {info}")
+    assert info["active"] and info["rebuilds"] >= 1
+    o = oracle_mod.from_system(len(pos), space, dyn, ic)
+    p_ref, v_ref, f_ref, _, _ = o.newton_steps(pos, vel, 100)
+    ep, ev = traj_err(s.state.pos, s.state.vel, p_ref, v_ref, space.geometry_cfg.length, periodic=True)
+    assert ep <= TRAJ_RTOL and ev <= TRAJ_RTOL
+    assert np.abs(s.forces - f_ref).max() <= 1e-9 * max(np.abs(f_ref).max(), 1e-300)
+
+
+def test_block_sized_by_stale_step_rolls_back(gpu_ctx, oracle_mod):
+    """Blocks sized by the displacement bound (block option 0): after 60
+    steps the velocities are raised 6x, so the next block, sized at the old
+    step length, overruns the bound; it rolls back, the step estimate takes
+    the new length, and the run keeps parity with the oracle."""
+    s, (pos, vel, space, dyn, ic) = make("lj_periodic_small_cut")
+    s.advance(60)
+    s.download()
+    info0 = s.dev.nlist_info()
+    pos1, vel1 = s.state.pos.copy(), s.state.vel.copy() * 6.0
+    s.state.vel[...] = vel1
+    s.upload()
+    s.advance(100)
+    s.download()
+    info = s.dev.nlist_info()
+    say(f"[nlist stale step] {info0} -> {info}")
+    assert info["active"] and info["rollbacks"] > info0["rollbacks"]
+    o = oracle_mod.from_system(len(pos), space, dyn, ic)
+    p_ref, v_ref, _, _, _ = o.newton_steps(pos1, vel1, 100)
+    ep, ev = traj_err(s.state.pos, s.state.vel, p_ref, v_ref, space.geometry_cfg.length)
+    assert ep <= TRAJ_RTOL and ev <= TRAJ_RTOL
