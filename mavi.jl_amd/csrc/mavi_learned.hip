@@ -190,7 +190,7 @@ __global__ __launch_bounds__(kFeatWG) void k_features(
     const double2 *__restrict__ U, const double *__restrict__ pol, const double *__restrict__ areas,
     const int32_t *__restrict__ rtype, const RingType *__restrict__ types, const double *__restrict__ rate,
     const double *__restrict__ omega, uint16_t *__restrict__ X, int32_t *__restrict__ nbr,
-    double2 *__restrict__ euler, uint32_t *__restrict__ gate) {
+    uint8_t *__restrict__ qprev, double2 *__restrict__ euler, uint32_t *__restrict__ gate) {
     __shared__ double2 spos[3][kWin];
     __shared__ int32_t sid[3][kWin];
     const int lane = threadIdx.x;
@@ -309,11 +309,12 @@ __global__ __launch_bounds__(kFeatWG) void k_features(
     xrow[0] = make_uint4(pack2(tflag, (areas[ring] - area0) / area0), pack2(dp.x, dp.y), pack2(dn.x, dn.y),
                          pack2(ur.x, ur.y));
     xrow[1] = make_uint4(pack2(dup.x, dup.y), pack2(dun.x, dun.y), pack2(rate[ring], omega[ring]), 0u);
-    xrow[2] = make_uint4(0u, 0u, 0u, 0u);
-    xrow[3] = make_uint4(0u, 0u, 0u, 0u);
+    // self features 16..31 stay the zeros of the allocation
 
-    // neighbour features
-    int32_t *nb = nbr + (int64_t)id * kNb;
+    // neighbour features; the row's slots past the list are zero (those this
+    // row's previous list used past the new one are cleared below)
+    int2 *nb2 = reinterpret_cast<int2 *>(nbr + (int64_t)id * kNb);
+    int jprev = -1;
     auto emit = [&](int q, int j, int idj, double d2, uint32_t &w0, uint32_t &w1) {
         const double2 dl = cdiff(P, Pst[j], xi);
         const double2 dd = rot(cs, sn, dl);
@@ -321,12 +322,14 @@ __global__ __launch_bounds__(kFeatWG) void k_features(
         const double rdot = ((uj.x - u.x) * dl.x + (uj.y - u.y) * dl.y) / sqrt(d2);
         const uint32_t a0 = pack2(dd.x, dd.y);
         const uint32_t a1 = (idj / P.nmax == ring ? 0x3f80u : 0xbf80u) | ((uint32_t)d2bf(rdot) << 16);
-        if (q & 1) xrow[4 + q / 2] = make_uint4(w0, w1, a0, a1);
-        else {
+        if (q & 1) {
+            xrow[4 + q / 2] = make_uint4(w0, w1, a0, a1);
+            nb2[q / 2] = make_int2(jprev, j);
+        } else {
             w0 = a0;
             w1 = a1;
+            jprev = j;
         }
-        nb[q] = j;
     };
     uint32_t w0 = 0u, w1 = 0u;
     const int nreg = cnt < kReg ? cnt : kReg;
@@ -364,8 +367,12 @@ __global__ __launch_bounds__(kFeatWG) void k_features(
         if (cnt > kNb) atomicOr(gate, GATE_NEIGH);
     }
     if (q & 1) xrow[4 + q / 2] = make_uint4(w0, w1, 0u, 0u);
-    for (int h = (q + 1) / 2; h < kNb / 2; h++) xrow[4 + h] = make_uint4(0u, 0u, 0u, 0u);
-    if (q < kNb) nb[q] = -1;  // the list ends at the first -1
+    const int qp = qprev[id];
+    for (int h = (q + 1) / 2; h < (qp + 1) / 2; h++) xrow[4 + h] = make_uint4(0u, 0u, 0u, 0u);
+    qprev[id] = (uint8_t)q;
+    // the list ends at the first -1
+    if (q & 1) nb2[q / 2] = make_int2(jprev, -1);
+    else if (q < kNb) nb2[q / 2] = make_int2(-1, -1);
 }
 
 // Candidate positions x' = x + displacement (then the periodic wrap of
@@ -614,6 +621,7 @@ struct mavi_learned {
     uint16_t *X = nullptr;
     float *Y = nullptr, *T = nullptr;
     int32_t *nbr = nullptr;
+    uint8_t *qprev = nullptr;  // per particle id: neighbour slots its feature row holds (past them: zeros)
     double2 *euler = nullptr, *disp = nullptr, *pos0 = nullptr;
     double *pol0 = nullptr;
     double *rate = nullptr, *omega = nullptr;
@@ -639,7 +647,7 @@ hipError_t launch_features(mavi_learned *l, hipStream_t st) {
                        s->P_[c], s->F, l->U, s->pol, s->rtype, s->types, l->rate, l->omega, l->gate);
     hipLaunchKernelGGL(k_features, dim3(nblk(s->P.na, kFeatWG)), dim3(kFeatWG), 0, st, s->P, l->L, s->P_[c],
                        s->ID_[c], s->slot_of, s->scell, s->start, l->U, s->pol, s->areas, s->rtype, s->types, l->rate,
-                       l->omega, l->X, l->nbr, l->euler, l->gate);
+                       l->omega, l->X, l->nbr, l->qprev, l->euler, l->gate);
     return hipGetLastError();
 }
 
@@ -739,6 +747,7 @@ int mavi_learned_create(mavi_sys *s, mavi_mlp *mlp, const mavi_learned_desc *d, 
     if (e == hipSuccess) e = dalloc(&l->Y, n * kMlpOut, st);
     if (e == hipSuccess) e = dalloc(&l->T, n * kMlpOut, st);
     if (e == hipSuccess) e = dalloc(&l->nbr, n * kNb, st);
+    if (e == hipSuccess) e = dalloc(&l->qprev, n, st);
     if (e == hipSuccess) e = dalloc(&l->euler, n, st);
     if (e == hipSuccess) e = dalloc(&l->disp, n, st);
     if (e == hipSuccess) e = dalloc(&l->pos0, n, st);
@@ -770,7 +779,7 @@ int mavi_learned_destroy(mavi_learned *l) {
     for (auto &gr : l->graph)
         for (auto &g : gr)
             if (g) (void)hipGraphExecDestroy(g);
-    dfree(l->X); dfree(l->Y); dfree(l->T); dfree(l->nbr); dfree(l->euler); dfree(l->disp); dfree(l->pos0);
+    dfree(l->X); dfree(l->Y); dfree(l->T); dfree(l->nbr); dfree(l->qprev); dfree(l->euler); dfree(l->disp); dfree(l->pos0);
     dfree(l->pol0); dfree(l->rate); dfree(l->omega); dfree(l->U); dfree(l->gate); dfree(l->ctl);
     if (l->gate_host) (void)hipHostFree(l->gate_host);
     if (l->ctl_host) (void)hipHostFree(l->ctl_host);
