@@ -793,6 +793,8 @@ bool nl_eligible(const mavi_sys *s) {
     const SysParams &P = s->P;
     const double range = nl_range(s);
     if (!(range > 0.0) || P.nwalls > 0 || s->var || P.na <= 0) return false;
+    // the build sorts 32-bit keys id << 7 | candidate
+    if (P.na >= (1 << 25)) return false;
     if (P.cols < 3 || P.rows < 3) return false;
     // every pair within range is in adjacent chunks at the binned positions
     return P.cl >= range * (1.0 + 1e-12) && P.ch >= range * (1.0 + 1e-12);
