@@ -58,7 +58,8 @@ constexpr uint32_t kStatusOverflow = 1u;    // a list held more than kmax entrie
                                             // than 32767 slots away (modulo na)
 constexpr uint32_t kStatusCapacity = 2u;    // a chunk count passed the capacity during a step's moves
 constexpr int kNlMax = 48;                  // list entries per particle (12 groups of 4)
-constexpr int kNeedBlock = 256;             // workgroup of the evaluation kernels
+constexpr int kNeedBlock = 64;              // workgroup of the evaluation kernels: one wave, so waves of
+                                            // unequal length free and take CU slots one at a time
 constexpr int kGhost = 32768;               // drifted-position copies each side of the slots (see nl_sum)
 
 struct NLArgs {
