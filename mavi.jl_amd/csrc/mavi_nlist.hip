@@ -266,6 +266,7 @@ __device__ __forceinline__ void wave_bitonic(uint32_t (&key)[H]) {
 }
 
 constexpr int kBuildRows = 16;  // particles whose lists one build wave stages in LDS at a time
+constexpr int kBuildWaves = 4;  // cells (waves) per build workgroup
 
 // Wave-scope ordering of the wave's own LDS writes before its reads.
 __device__ __forceinline__ void wave_lds_sync() {
@@ -394,20 +395,20 @@ __device__ __forceinline__ void build_cell(const SysParams &P, const int32_t *__
 // switch off) and leave the cell's lists empty.  A particle within `edge` of
 // a periodic edge sums with the minimum image (its partners may wrap before
 // the next build).
-__global__ __launch_bounds__(256) void k_nl_build(SysParams G, SysParams P, const double2 *__restrict__ pos,
+__global__ __launch_bounds__(64 * kBuildWaves) void k_nl_build(SysParams G, SysParams P, const double2 *__restrict__ pos,
                                                   const int32_t *__restrict__ ID,
                                                   const int32_t *__restrict__ start, int2 *__restrict__ nbr,
                                                   int32_t *__restrict__ ncount, double rl2, double edge,
                                                   uint32_t *__restrict__ status) {
-    __shared__ int32_t w_slot[4][kBuildCand];
-    __shared__ double2 w_pos[4][kBuildCand];
-    __shared__ __align__(16) int16_t w_list[4][kBuildRows][kNlMax];
-    __shared__ int32_t w_groups[4][kBuildRows];
-    __shared__ int16_t w_dump[4][64];
+    __shared__ int32_t w_slot[kBuildWaves][kBuildCand];
+    __shared__ double2 w_pos[kBuildWaves][kBuildCand];
+    __shared__ __align__(16) int16_t w_list[kBuildWaves][kBuildRows][kNlMax];
+    __shared__ int32_t w_groups[kBuildWaves][kBuildRows];
+    __shared__ int16_t w_dump[kBuildWaves][64];
     // the cell (row blockIdx.y, column 4 blockIdx.x + the wave's index):
     // uniform, so the stencil is scalar work
     const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = (int)lane_id();
-    const int row = (int)blockIdx.y, col = (int)blockIdx.x * 4 + wid;
+    const int row = (int)blockIdx.y, col = (int)blockIdx.x * kBuildWaves + wid;
     if (col >= G.cols) return;  // whole wave; no workgroup barrier below
     const int cell = row * G.cols + col;
     const int t0 = start[cell], t1 = start[cell + 1];
@@ -897,7 +898,7 @@ hipError_t nl_rebuild(mavi_sys *s, hipStream_t st) {
     const int c = s->cur;
     {
         ProfScope ps(s, MAVI_PROF_BIN, st);
-        hipLaunchKernelGGL(k_nl_build, dim3(nblk(N.G.cols, 4), N.G.rows), dim3(256), 0, st, N.G, P, s->P_[c], s->ID_[c],
+        hipLaunchKernelGGL(k_nl_build, dim3(nblk(N.G.cols, kBuildWaves), N.G.rows), dim3(64 * kBuildWaves), 0, st, N.G, P, s->P_[c], s->ID_[c],
                            s->start, (int2 *)N.nbr, N.ncount, N.rl2, N.edge, N.status);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if ((e = launch_fill_i32(N.ccnt, 0, P.ncell, st)) != hipSuccess) return e;
