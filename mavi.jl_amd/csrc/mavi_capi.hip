@@ -1460,7 +1460,7 @@ int mavi_nlist_info(mavi_sys *s, int64_t *info) {
 int mavi_sys_set_tile(mavi_sys *s, int32_t workgroup) {
     if (s) s->f_valid = false;
     if (!s) return MAVI_E_BADARG;
-    if (workgroup == 0) workgroup = 256;
+    if (workgroup == 0) workgroup = 64;
     if (!is_pow2_wg(workgroup)) return set_err(s->ctx, MAVI_E_BADARG, "workgroup must be 64..1024, power of two");
     if (workgroup != s->wg && s->graph2) {
         (void)hipStreamSynchronize(s->ctx->stream);

@@ -211,7 +211,7 @@ struct mavi_sys {
     bool closing = false;  // destroyed while referenced: freed at the last release
     bool ctx_held = false; // holds a reference to ctx (set once creation succeeded)
     SysParams P{};
-    int32_t wg = 256;  // pair-force workgroup size
+    int32_t wg = 64;   // pair-force workgroup size (one wave: waves of unequal length free CU slots one at a time)
     bool two_phase = false; // candidate list for finite-range interactions
     bool lds_tiles = false; // LDS-staged chunk-row segments for the pair kernels
     double list2 = INFINITY; // rings: widest interaction range squared
