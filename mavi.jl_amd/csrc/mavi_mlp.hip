@@ -257,43 +257,38 @@ __global__ void k_sync_weights(const float *__restrict__ p, uint16_t *__restrict
 }
 
 // ---------------------------------------------------------------------------
-// Fused inference.  LDS: W0t [128][168], W1t/W2t [128][136], W3t [16][136]
-// (rows k-contiguous, +8 padding), biases, and one 32 x 136 activation tile
-// per wave.  Rows of X: 160 BF16 features; Y: 2 FP32 outputs per row.
+// Fused inference.  LDS holds only the weights: W0t [128][168], W1t/W2t
+// [128][136], W3t [16][136] (rows k-contiguous, +8 padding) and the biases;
+// activations stay in registers.  Each wave computes 16-row tiles, every
+// layer transposed: out^T[n][m] = sum_k W^T[n][k] act[m][k], weights as the
+// first MFMA operand (16 output features per block f), the tile's rows as
+// the second.  A lane (row m = lane % 16, g = lane / 16) then holds
+// features 16 f + 4 g + i (i < 4) of its row: after bias + GELU, blocks 2 ks
+// and 2 ks + 1 give exactly the 8 BF16 values the lane supplies as the next
+// layer's second operand at K step ks, once that layer's weight columns are
+// stored in that order (column 32 ks + 8 g + j holds hidden unit
+// 32 ks + 4 g + j for j < 4, 32 ks + 16 + 4 g + j - 4 for j >= 4; a
+// relabelling of the hidden units, the same network).  Rows of X: 160 BF16
+// features; Y: 2 FP32 outputs per row.
 // ---------------------------------------------------------------------------
 constexpr int kP0 = kMlpIn + 8, kPH = kMlpHid + 8;
 constexpr int kLdsW0 = kMlpHid * kP0, kLdsW1 = kMlpHid * kPH, kLdsW3 = kMlpOutPad * kPH;
-constexpr int kFusedWaves = 8, kTileRows = 16;
+constexpr int kFusedWaves = 12, kTileRows = 16;
 constexpr size_t kFusedLds = sizeof(uint16_t) * (size_t)(kLdsW0 + 2 * kLdsW1 + kLdsW3) +
-                             sizeof(float) * (size_t)(3 * kMlpHid + kMlpOutPad) +
-                             sizeof(uint16_t) * (size_t)kFusedWaves * kTileRows * kPH;
+                             sizeof(float) * (size_t)(3 * kMlpHid + kMlpOutPad);
 
-// One hidden layer for a 16-row tile, computed transposed: out^T[n][m] =
-// sum_k W^T[n][k] act[m][k], weights as the first MFMA operand (16 output
-// features per block), activations as the second (the 16 rows).  A lane's
-// accumulator then holds 4 consecutive features of one row, written to the
-// wave's activation tile as one 8-byte store after bias + GELU.
-// FIRST: the input rows come from X in global memory.
-template <int KIN, int PIN, bool FIRST>
-__device__ __forceinline__ void fused_hidden(const uint16_t *__restrict__ X, int64_t row0, int M, uint16_t *act,
-                                             const uint16_t *W, const float *bias, int lane) {
+// One layer's MFMAs for NT 16-row tiles: acc[t][f] = W^T[16 f .. 16 f + 15][.]
+// x act_t^T, the inputs h[t][ks] (the lane's 8 values at K step ks), each
+// weight fragment read once from LDS for the NT tiles, one K step ahead.
+template <int KIN, int PIN, int NT>
+__device__ __forceinline__ void fused_layer(const bf16x8 (*h)[KIN / 32], const uint16_t *W, int lane,
+                                            f32x4 (&acc)[NT][8]) {
     const int fr = lane & 15, fk = (lane >> 4) * 8;
     constexpr int KS = KIN / 32;
-    f32x4 acc[8];
 #pragma unroll
-    for (int f = 0; f < 8; f++) acc[f] = f32x4{0.f, 0.f, 0.f, 0.f};
-    // the tile's input fragments and the first K step of weights in flight
-    // before the first MFMA; later weight fragments one K step ahead
-    bf16x8 h[KS];
+    for (int t = 0; t < NT; t++)
 #pragma unroll
-    for (int ks = 0; ks < KS; ks++) {
-        if constexpr (FIRST) {
-            const int64_t r = row0 + fr;
-            h[ks] = r < M ? ld8(X + r * kMlpIn + ks * 32 + fk) : __builtin_bit_cast(bf16x8, uint4{0u, 0u, 0u, 0u});
-        } else {
-            h[ks] = ld8(act + fr * kPH + ks * 32 + fk);
-        }
-    }
+        for (int f = 0; f < 8; f++) acc[t][f] = f32x4{0.f, 0.f, 0.f, 0.f};
     bf16x8 wf[2][8];
 #pragma unroll
     for (int f = 0; f < 8; f++) wf[0][f] = ld8(W + (16 * f + fr) * PIN + fk);
@@ -305,26 +300,38 @@ __device__ __forceinline__ void fused_hidden(const uint16_t *__restrict__ X, int
             for (int f = 0; f < 8; f++) wf[cb ^ 1][f] = ld8(W + (16 * f + fr) * PIN + (ks + 1) * 32 + fk);
         }
 #pragma unroll
-        for (int f = 0; f < 8; f++) acc[f] = mfma16(wf[cb][f], h[ks], acc[f]);
+        for (int f = 0; f < 8; f++)
+#pragma unroll
+            for (int t = 0; t < NT; t++) acc[t][f] = mfma16(wf[cb][f], h[t][ks], acc[t][f]);
         if (ks + 1 < KS) __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
-        __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 8 * NT, 0);
     }
-    wave_sync();  // every lane has read the previous tile before it is overwritten
+}
+
+// Bias + GELU of a hidden layer's accumulators into the next layer's inputs
+// (the column order above).
+__device__ __forceinline__ void fused_act(const f32x4 (&acc)[8], const float *bias, int lane,
+                                          bf16x8 (&h)[kMlpHid / 32]) {
     const int n4 = (lane >> 4) * 4;
 #pragma unroll
-    for (int f = 0; f < 8; f++) {
-        const float4 bb = *reinterpret_cast<const float4 *>(bias + 16 * f + n4);
-        const uint32_t lo = f2bf2(gelu(acc[f][0] + bb.x), gelu(acc[f][1] + bb.y));
-        const uint32_t hi = f2bf2(gelu(acc[f][2] + bb.z), gelu(acc[f][3] + bb.w));
-        *reinterpret_cast<uint2 *>(act + fr * kPH + 16 * f + n4) = make_uint2(lo, hi);
+    for (int ks = 0; ks < kMlpHid / 32; ks++) {
+        const float4 b0 = *reinterpret_cast<const float4 *>(bias + 32 * ks + n4);
+        const float4 b1 = *reinterpret_cast<const float4 *>(bias + 32 * ks + 16 + n4);
+        const f32x4 a0 = acc[2 * ks], a1 = acc[2 * ks + 1];
+        const uint4 v = make_uint4(f2bf2(gelu(a0[0] + b0.x), gelu(a0[1] + b0.y)),
+                                   f2bf2(gelu(a0[2] + b0.z), gelu(a0[3] + b0.w)),
+                                   f2bf2(gelu(a1[0] + b1.x), gelu(a1[1] + b1.y)),
+                                   f2bf2(gelu(a1[2] + b1.z), gelu(a1[3] + b1.w)));
+        h[ks] = __builtin_bit_cast(bf16x8, v);
     }
-    wave_sync();
 }
 
 // Copy a [ROWS][COLS] BF16 matrix into LDS rows of PITCH elements, 16 bytes
 // per thread and 8 loads in flight (the chunk index is clamped, so every load
-// is unconditional; the clamped copies store the same value twice).
-template <int ROWS, int COLS, int PITCH>
+// is unconditional; the clamped copies store the same value twice).  PERM:
+// the columns in the hidden-unit order above (each 16-byte chunk from two
+// 8-byte pieces).
+template <int ROWS, int COLS, int PITCH, bool PERM>
 __device__ __forceinline__ void stage_rows(uint16_t *dst, const uint16_t *__restrict__ src, int t) {
     constexpr int kQ = ROWS * COLS / 8, kStride = 64 * kFusedWaves, kU = 8;
     for (int q0 = t; q0 < kQ; q0 += kU * kStride) {
@@ -332,7 +339,15 @@ __device__ __forceinline__ void stage_rows(uint16_t *dst, const uint16_t *__rest
 #pragma unroll
         for (int u = 0; u < kU; u++) {
             const int q = min(q0 + u * kStride, kQ - 1);
-            v[u] = *reinterpret_cast<const uint4 *>(src + (q / (COLS / 8)) * COLS + (q % (COLS / 8)) * 8);
+            const int row = q / (COLS / 8), c8 = (q % (COLS / 8)) * 8;
+            if constexpr (PERM) {
+                const int ks = c8 / 32, g = (c8 % 32) / 8;
+                const uint2 lo = *reinterpret_cast<const uint2 *>(src + row * COLS + 32 * ks + 4 * g);
+                const uint2 hi = *reinterpret_cast<const uint2 *>(src + row * COLS + 32 * ks + 16 + 4 * g);
+                v[u] = make_uint4(lo.x, lo.y, hi.x, hi.y);
+            } else {
+                v[u] = *reinterpret_cast<const uint4 *>(src + row * COLS + c8);
+            }
         }
 #pragma unroll
         for (int u = 0; u < kU; u++) {
@@ -352,13 +367,12 @@ __global__ __launch_bounds__(64 * kFusedWaves) void k_mlp_fused(
     uint16_t *W2 = W1 + kLdsW1;
     uint16_t *W3 = W2 + kLdsW1;
     float *B = reinterpret_cast<float *>(W3 + kLdsW3);  // b0 | b1 | b2 | b3 (16)
-    uint16_t *ACT = reinterpret_cast<uint16_t *>(B + 3 * kMlpHid + kMlpOutPad);
     const int t = threadIdx.x;
     // stage the weights (16-byte chunks, 8 loads in flight per thread) and biases
-    stage_rows<kMlpHid, kMlpIn, kP0>(W0, w0, t);
-    stage_rows<kMlpHid, kMlpHid, kPH>(W1, w1, t);
-    stage_rows<kMlpHid, kMlpHid, kPH>(W2, w2, t);
-    stage_rows<kMlpOutPad, kMlpHid, kPH>(W3, w3, t);
+    stage_rows<kMlpHid, kMlpIn, kP0, false>(W0, w0, t);
+    stage_rows<kMlpHid, kMlpHid, kPH, true>(W1, w1, t);
+    stage_rows<kMlpHid, kMlpHid, kPH, true>(W2, w2, t);
+    stage_rows<kMlpOutPad, kMlpHid, kPH, true>(W3, w3, t);
     for (int q = t; q < kMlpHid; q += blockDim.x) {
         B[q] = params[kOffB0 + q];
         B[kMlpHid + q] = params[kOffB1 + q];
@@ -369,24 +383,40 @@ __global__ __launch_bounds__(64 * kFusedWaves) void k_mlp_fused(
 
     const int lane = t & 63, w = t >> 6;
     const int fr = lane & 15, fk = (lane >> 4) * 8;
-    uint16_t *act = ACT + w * kTileRows * kPH;
+    constexpr int NT = kTileRows / 16;
     const int ntiles = (M + kTileRows - 1) / kTileRows;
     for (int tile = blockIdx.x * kFusedWaves + w; tile < ntiles; tile += gridDim.x * kFusedWaves) {
         const int64_t row0 = (int64_t)tile * kTileRows;
-        fused_hidden<kMlpIn, kP0, true>(X, row0, M, act, W0, B, lane);
-        fused_hidden<kMlpHid, kPH, false>(X, row0, M, act, W1, B + kMlpHid, lane);
-        fused_hidden<kMlpHid, kPH, false>(X, row0, M, act, W2, B + 2 * kMlpHid, lane);
-        // output layer, transposed: 16 padded outputs (2 used) x 16 rows
-        f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+        bf16x8 x[NT][kMlpIn / 32];
 #pragma unroll
-        for (int ks = 0; ks < kMlpHid / 32; ks++)
-            acc = mfma16(ld8(W3 + fr * kPH + ks * 32 + fk), ld8(act + fr * kPH + ks * 32 + fk), acc);
-        wave_sync();  // the next tile's first layer overwrites act
-        if ((lane >> 4) == 0) {  // outputs 0 and 1 of row fr
-            const int64_t m = row0 + fr;
-            if (m < M)
-                *reinterpret_cast<float2 *>(Y + m * kMlpOut) =
-                    make_float2(acc[0] + B[3 * kMlpHid], acc[1] + B[3 * kMlpHid + 1]);
+        for (int u = 0; u < NT; u++) {
+            const int64_t r = row0 + 16 * u + fr;
+#pragma unroll
+            for (int ks = 0; ks < kMlpIn / 32; ks++)
+                x[u][ks] =
+                    r < M ? ld8(X + r * kMlpIn + ks * 32 + fk) : __builtin_bit_cast(bf16x8, uint4{0u, 0u, 0u, 0u});
+        }
+        f32x4 acc[NT][8];
+        bf16x8 h[NT][kMlpHid / 32];
+        fused_layer<kMlpIn, kP0, NT>(x, W0, lane, acc);
+#pragma unroll
+        for (int u = 0; u < NT; u++) fused_act(acc[u], B, lane, h[u]);
+        fused_layer<kMlpHid, kPH, NT>(h, W1, lane, acc);
+#pragma unroll
+        for (int u = 0; u < NT; u++) fused_act(acc[u], B + kMlpHid, lane, h[u]);
+        fused_layer<kMlpHid, kPH, NT>(h, W2, lane, acc);
+#pragma unroll
+        for (int u = 0; u < NT; u++) fused_act(acc[u], B + 2 * kMlpHid, lane, h[u]);
+        // output layer, transposed: 16 padded outputs (2 used) x 16 rows per tile
+#pragma unroll
+        for (int u = 0; u < NT; u++) {
+            f32x4 o = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < kMlpHid / 32; ks++) o = mfma16(ld8(W3 + fr * kPH + ks * 32 + fk), h[u][ks], o);
+            const int64_t r = row0 + 16 * u + fr;
+            if ((lane >> 4) == 0 && r < M)  // outputs 0 and 1 of row r
+                *reinterpret_cast<float2 *>(Y + r * kMlpOut) =
+                    make_float2(o[0] + B[3 * kMlpHid], o[1] + B[3 * kMlpHid + 1]);
         }
     }
 }
