@@ -248,19 +248,35 @@ __global__ __launch_bounds__(kFeatWG) void k_features(
     double sn, cs;
     sincos(pol[ring], &sn, &cs);
 
-    // candidate ranges of the 3x3 chunk block (periodic grid >= 3x3), each
-    // either inside its row's staged window (LDS) or not (global)
-    int cb[9], ce[9];
-    bool inl[9];
+    // candidate ranges of the 3x3 chunk block (periodic grid >= 3x3): per
+    // block row the three chunks' slots are consecutive, one run (two where
+    // the columns wrap), each either inside its row's staged window (LDS) or
+    // not (global).  The kept neighbours are ordered by (distance, id), so
+    // the visiting order does not matter.
+    int rb[6], re[6];
+    bool inl[6];
 #pragma unroll
-    for (int k = 0; k < 9; k++) {
-        const int d = k / 3;
-        int r = row + d - 1, cc = col + k % 3 - 1;
+    for (int d = 0; d < 3; d++) {
+        int r = row + d - 1;
         r = r < 0 ? r + P.rows : (r >= P.rows ? r - P.rows : r);
-        cc = cc < 0 ? cc + P.cols : (cc >= P.cols ? cc - P.cols : cc);
-        cb[k] = start[r * P.cols + cc];
-        ce[k] = start[r * P.cols + cc + 1];
-        inl[k] = cb[k] >= wa[d] && ce[k] <= wb[d];
+        const int *sr = start + r * P.cols;
+        int a0 = col - 1, b0 = col + 1, a1 = 0, b1 = -1;  // columns [a0, b0] and [a1, b1]
+        if (col == 0) {
+            a0 = 0;
+            b0 = 1;
+            a1 = b1 = P.cols - 1;
+        } else if (col == P.cols - 1) {
+            a0 = col - 1;
+            b0 = col;
+            a1 = b1 = 0;
+        }
+        rb[2 * d] = sr[a0];
+        re[2 * d] = sr[b0 + 1];
+        rb[2 * d + 1] = b1 >= a1 ? sr[a1] : 0;
+        re[2 * d + 1] = b1 >= a1 ? sr[b1 + 1] : 0;
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+            inl[2 * d + u] = rb[2 * d + u] >= wa[d] && re[2 * d + u] <= wb[d];
     }
     // pass 1: the kReg nearest within r_feat, sorted
     Cand kn[kReg];
@@ -268,9 +284,9 @@ __global__ __launch_bounds__(kFeatWG) void k_features(
     for (int t = 0; t < kReg; t++) kn[t] = Cand{INFINITY, 0x7fffffff, -1};
     int cnt = 0;
 #pragma unroll
-    for (int k = 0; k < 9; k++) {
-        const int d = k / 3;
-        for (int j = cb[k]; j < ce[k]; j++) {
+    for (int k = 0; k < 6; k++) {
+        const int d = k / 2;
+        for (int j = rb[k]; j < re[k]; j++) {
             if (j == s) continue;
             double2 xj;
             int idj;
@@ -344,8 +360,8 @@ __global__ __launch_bounds__(kFeatWG) void k_features(
         for (; q < cnt && q < kNb; q++) {
             double bd = INFINITY;
             int bid = 0x7fffffff, bj = -1;
-            for (int k = 0; k < 9; k++) {
-                for (int j = cb[k]; j < ce[k]; j++) {
+            for (int k = 0; k < 6; k++) {
+                for (int j = rb[k]; j < re[k]; j++) {
                     if (j == s) continue;
                     const double2 xj = Pst[j];
                     const double dx = min_image(xj.x - xi.x, P.L, P.halfL);
