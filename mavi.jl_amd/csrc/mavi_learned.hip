@@ -432,8 +432,10 @@ __global__ void k_apply(SysParams P, LearnParams L, const double2 *__restrict__ 
     disp[id] = d;
 }
 
-// Overlap gate over the listed pairs (each pair once, from its smaller id;
-// the list of a particle ends at the first -1 or after 32 entries).
+// Overlap gate over the listed pairs (each pair once, from its smaller slot:
+// a list holds every particle within r_feat, so the two particles list each
+// other, and more than 32 fails the gate anyway; a list ends at the first -1
+// or after 32 entries).
 __device__ __forceinline__ void gate_pairs(const SysParams &P, const LearnParams &L, int s,
                                            const int32_t *__restrict__ ID, const int32_t *__restrict__ nbr,
                                            const double2 *__restrict__ P2, uint32_t *__restrict__ gate) {
@@ -450,7 +452,7 @@ __device__ __forceinline__ void gate_pairs(const SysParams &P, const LearnParams
             const int j = js[u];
             done = done || j < 0;
             if (done) continue;
-            if (ID[j] < id) continue;
+            if (j < s) continue;  // the pair is tested from its smaller slot
             const double2 xj = P2[j];
             const double dx = min_image(xj.x - xi.x, P.L, P.halfL);
             const double dy = min_image(xj.y - xi.y, P.H, P.halfH);
