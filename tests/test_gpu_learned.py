@@ -87,6 +87,36 @@ def test_features_vs_oracle(gpu_ctx):
     assert np.any(x[:, 34::4] == 0xBF80)  # some contacts with other rings
 
 
+def test_features_after_state_changes(gpu_ctx):
+    """A feature row clears only the neighbour slots its previous list used
+    past the new one: features taken at successive states of a run (lists
+    of individual particles shrink and grow) match the oracle at each,
+    every slot past a list zero."""
+    from mavi_hip import learned
+
+    system, cfg = small_rings(rot_diff=1.0)
+    system.advance(50)
+    lp = learned.Learned(system.dev, None, k=K, r_feat=RF)
+    prev = None
+    shrunk = 0
+    for it in range(4):
+        if it:
+            system.advance(150)
+        x, _ = lp.features()
+        (xr, _, over), _ = oracle_features(system, cfg)
+        assert not over
+        dmax, frac = bits_close(x, xr)
+        assert dmax <= 1 and frac <= 1e-3, (it, dmax, frac)
+        nn = (x[:, 34::4] != 0).sum(axis=1)  # listed neighbours (membership word +-1)
+        for q in range(32):
+            assert np.all(x[nn <= q, 32 + 4 * q: 36 + 4 * q] == 0), (it, q)
+        if prev is not None:
+            shrunk += int((nn < prev).sum())
+        prev = nn
+    say(f"[features after changes] rows whose list shrank between calls: {shrunk}")
+    assert shrunk > 0
+
+
 def test_targets_vs_oracle(gpu_ctx):
     from mavi_hip import learned
 
