@@ -48,7 +48,7 @@ def plain_system(pos, vel, space, dyn, ic):
 def test_lj_gas_1024_1000_steps_energies(gpu_ctx, oracle_mod, cutoff):
     from mavi_hip import workloads as wl
 
-    say(f"[config 1] 1024 LJ, 1000 steps, energies every step (cutoff {cutoff})")
+    say(f"[lj 1024] 1024 LJ, 1000 steps, energies every step (cutoff {cutoff})")
     pos, vel, space, dyn, ic = wl.lj_gas_1024(cutoff=cutoff)
     s = plain_system(pos, vel, space, dyn, ic)
     ke, pe = s.dev.run_record(1000)
@@ -66,7 +66,7 @@ def test_lj_gas_1024_1000_steps_energies(gpu_ctx, oracle_mod, cutoff):
 def test_lj_1m_10_steps_vs_oracle(gpu_ctx, oracle_mod):
     from mavi_hip import workloads as wl
 
-    say("[config 2] 1M LJ, 10 steps vs oracle")
+    say("[lj 1M] 1M LJ, 10 steps vs oracle")
     pos, vel, space, dyn, ic = wl.lj_periodic()
     s = plain_system(pos, vel, space, dyn, ic)
     s.advance(10)
@@ -90,7 +90,7 @@ def rings_800k():
 
 
 def test_rings_800k_binning_forces_steps(gpu_ctx, oracle_mod):
-    say("[config 3] 40,000 rings x 20: binning, forces")
+    say("[rings 800k] 40,000 rings x 20: binning, forces")
     s, (st, sp, cfg, ic), pos0, pol0 = rings_800k()
     o = oracle_mod.from_rings(st, sp, cfg, ic)
     cp, counts = s.dev.bin(export=True)
@@ -105,7 +105,7 @@ def test_rings_800k_binning_forces_steps(gpu_ctx, oracle_mod):
     e = force_rel_err(f, f_ref, ab + np.abs(f_ref).sum(axis=1))
     say(f"  forces rel err {e:.2e}")
     assert e <= FORCE_RTOL
-    say("[config 3] 5 steps, host noise")
+    say("[rings 800k] 5 steps, host noise")
     noise = np.random.default_rng(33).standard_normal((5, st.num_rings))
     s.upload()
     s.advance(5, noise)
@@ -123,7 +123,7 @@ def test_rings_800k_device_noise_vs_oracle(gpu_ctx, oracle_mod):
     """20 steps with the device's Philox4x32-10 rotational noise; the oracle
     is fed the same draws (oracle/philox.py, pinned by the Random123
     known-answer vectors)."""
-    say("[config 3] 20 steps, device noise")
+    say("[rings 800k] 20 steps, device noise")
     s, (st, sp, cfg, ic), pos0, pol0 = rings_800k()
     seed, counter = s.dev.noise_state()
     assert counter == 0
@@ -146,7 +146,7 @@ def test_batch_512_replicas(gpu_ctx, oracle_mod):
     from mavi_hip import workloads as wl
     from test_gpu_batch import single_run
 
-    say("[config 4] 512 replicas: building")
+    say("[batch 512] 512 replicas: building")
     ens = wl.batch_ensemble()
     assert len(ens) == 512
     b = mb.Batch(ctx=gpu_ctx)
@@ -167,7 +167,7 @@ def test_batch_512_replicas(gpu_ctx, oracle_mod):
         b.upload(i, p, v, pol)
     info = b.info()
     say(f"  {info['particles'] if 'particles' in info else info}")
-    say("[config 4] 100 steps in one batch")
+    say("[batch 512] 100 steps in one batch")
     b.run(100)
     out = [b.download(i) for i in range(len(ens))]
     forces = [b.get_forces(i) for i in range(len(ens))]
@@ -178,7 +178,7 @@ def test_batch_512_replicas(gpu_ctx, oracle_mod):
         if kind == "plain":
             p0 = inits[i][1].sum(axis=0)
             assert np.abs(vel.sum(axis=0) - p0).max() <= 1e-9 * np.abs(inits[i][1]).sum()
-    say("[config 4] every replica against the same system stepped alone")
+    say("[batch 512] every replica against the same system stepped alone")
     for i, (kind, case, seed) in enumerate(ens):
         if kind == "plain":
             pos, vel, space, dyn, ic = case
@@ -197,7 +197,7 @@ def test_batch_512_replicas(gpu_ctx, oracle_mod):
         if pol is not None:
             assert np.array_equal(pol, want[2]), f"replica {i} polarisations"
         assert np.array_equal(forces[i], want[3]), f"replica {i} forces"
-    say("[config 4] three replicas against the oracle")
+    say("[batch 512] three replicas against the oracle")
     plain_ids = [i for i, e in enumerate(ens) if e[0] == "plain"]
     ring_ids = [i for i, e in enumerate(ens) if e[0] == "rings"]
     picks = [max(plain_ids, key=lambda i: sizes[i]), max(ring_ids, key=lambda i: sizes[i]),
@@ -229,7 +229,7 @@ def test_learned_features_targets_200k(gpu_ctx):
     from test_gpu_learned import K, RF, bits_close, host_state, oracle_features
     import learned_oracle as lo
 
-    say("[config 5] 200k features and targets")
+    say("[learned 200k] 200k features and targets")
     st, sp, cfg, ic = wl.rings_mixture(nrings=10000, seed=100, chunk=RF)
     system = rings.RingsSystem(state=st, space_cfg=sp, dynamic_cfg=cfg, int_cfg=ic, seed=100)
     system.advance(40)
