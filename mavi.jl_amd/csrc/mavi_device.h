@@ -969,7 +969,7 @@ __global__ void k_rings_init(SysParams P, const int32_t *__restrict__ rtype, con
 // the one-thread-per-ring form.  slot_of and cpos start at the system's
 // first particle id, pol/areas/cms/cms_next at its first ring; noise (host
 // draws) or the Philox stream (seed, step) give the rotational noise of ring rr.
-template <bool UPDATE, bool WALLS = false>
+template <bool UPDATE, bool WALLS = false, bool RATES = false>
 __device__ __forceinline__ void ring_lanes(const SysParams &P, const RingType &T, const bool ring_ok, const int rr,
                                            const int lane0, const int i, const int32_t *__restrict__ slot_of,
                                            double2 *__restrict__ Pst, double2 *__restrict__ F,
@@ -978,7 +978,8 @@ __device__ __forceinline__ void ring_lanes(const SysParams &P, const RingType &T
                                            double2 *__restrict__ cpos, const double *__restrict__ noise,
                                            const uint64_t seed, const uint64_t step,
                                            int32_t *__restrict__ nkey = nullptr, int32_t *__restrict__ nrank = nullptr,
-                                           int32_t *__restrict__ ncnt = nullptr, uint32_t *__restrict__ nflags = nullptr) {
+                                           int32_t *__restrict__ ncnt = nullptr, uint32_t *__restrict__ nflags = nullptr,
+                                           const RingRates RR = RingRates{}) {
     const int r = rr;
     const int np = T.np;
     const bool act = ring_ok && i < np;
@@ -1087,6 +1088,38 @@ __device__ __forceinline__ void ring_lanes(const SysParams &P, const RingType &T
         if (act) wall_forces(P, x.x, x.y, Fi.x, Fi.y);  // calc_walls_forces!
     }
     if (act) F[slot] = Fi;
+    if constexpr (RATES) {
+        // the learned propagator's inputs: u = vo p + mu F per particle, and
+        // per ring the rate of its area, 1/2 sum u x (x_next - x_prev), over
+        // the target area and the alignment of p with the mean of u
+        const double th = ring_ok ? pol[rr] : 0.0;
+        double sn, cs;
+        sincos(th, &sn, &cs);
+        const double2 u = make_double2(T.vo * cs + T.mu * Fi.x, T.vo * sn + T.mu * Fi.y);
+        if (act && ring_ok) RR.U[slot] = u;
+        double2 *const s_u = s_a;  // the neighbour reads of s_a are done
+        s_u[me] = u;
+        s_t[me] = u.x * da.y - u.y * da.x;  // the area sums' reads of s_t are done
+        lds_sync();
+        if (ring_ok && i == 0) {
+            double ux = 0.0, uy = 0.0, ar = 0.0;
+            for (int k = 0; k < np; k++) {
+                const double2 uk = s_u[tb + k];
+                ux += uk.x;
+                uy += uk.y;
+                ar += s_t[tb + k];
+            }
+            RR.rate[rr] = 0.5 * ar / area0;
+            const double vx = ux / (double)np, vy = uy / (double)np;
+            const double speed = sqrt(vx * vx + vy * vy);
+            double cr = 0.0;
+            if (speed != 0.0) {
+                cr = (cs * vy - sn * vx) / speed;
+                if (fabs(cr) > 1.0) cr = cr > 0.0 ? 1.0 : -1.0;
+            }
+            RR.omega[rr] = asin(cr);
+        }
+    }
     if constexpr (!UPDATE) return;
 
     // update!: v = vo pol + mu F, x += v dt (then walls!), polarisation relaxation
@@ -1143,13 +1176,15 @@ __device__ __forceinline__ void ring_lanes(const SysParams &P, const RingType &T
     }
 }
 
-template <bool UPDATE, bool WALLS = false>
+template <bool UPDATE, bool WALLS = false, bool RATES = false>
 __global__ __launch_bounds__(256) void k_rings_lanes(
     SysParams P, const int32_t *__restrict__ rtype, const RingType *__restrict__ types,
     const int32_t *__restrict__ slot_of, double2 *__restrict__ Pst, double2 *__restrict__ F, double *__restrict__ pol,
     double *__restrict__ areas, double2 *__restrict__ cms, double2 *__restrict__ cms_next, double2 *__restrict__ cpos,
     const double *__restrict__ noise, uint64_t seed, const unsigned long long *__restrict__ counter,
-    int32_t *__restrict__ nkey, int32_t *__restrict__ nrank, int32_t *__restrict__ ncnt, uint32_t *__restrict__ nflags) {
+    int32_t *__restrict__ nkey, int32_t *__restrict__ nrank, int32_t *__restrict__ ncnt, uint32_t *__restrict__ nflags,
+    RingRates RR) {
+    if (RATES && blockIdx.x == 0 && threadIdx.x == 0) RR.gate[0] = 0u;
     // floor(64 / nmax) rings per wave, nmax lanes each (3 rings of 20 per
     // wave: 60 busy lanes instead of 40 with one ring per 32 lanes)
     const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
@@ -1160,8 +1195,8 @@ __global__ __launch_bounds__(256) void k_rings_lanes(
     const bool ring_ok = grp < per && r < P.nrings && (!P.rmask || P.rmask[r]);  // get_rings_ids
     const int rr = ring_ok ? r : 0;
     const RingType T = types[P.typed ? rtype[rr] : 0];
-    ring_lanes<UPDATE, WALLS>(P, T, ring_ok, rr, grp * P.nmax, i, slot_of, Pst, F, pol, areas, cms, cms_next, cpos,
-                              noise, seed, counter[0], nkey, nrank, ncnt, nflags);
+    ring_lanes<UPDATE, WALLS, RATES>(P, T, ring_ok, rr, grp * P.nmax, i, slot_of, Pst, F, pol, areas, cms, cms_next,
+                                     cpos, noise, seed, counter[0], nkey, nrank, ncnt, nflags, RR);
 }
 
 }  // namespace

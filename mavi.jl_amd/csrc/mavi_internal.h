@@ -160,6 +160,14 @@ struct ProfSlot {
     int64_t launches = 0;
 };
 
+// Outputs of a force-only ring pass for the learned propagator's features.
+struct RingRates {
+    double2 *U = nullptr;      // per slot: the overdamped velocity v0 p + mu F
+    double *rate = nullptr;    // per ring: d(area)/dt over the target area
+    double *omega = nullptr;   // per ring: asin of the polarisation x mean-velocity alignment
+    uint32_t *gate = nullptr;  // cleared
+};
+
 constexpr int32_t kNlBlockCap = 60;  // most steps between host checks of the list validity
 
 // Verlet neighbour lists of a finite-range plain system (LJ with a cutoff,
@@ -384,7 +392,10 @@ hipError_t launch_rings_step(mavi_sys *s, const double *noise_dev, hipStream_t s
 // One szabo_step! / rtp_step!; noise_dev = this step's host draws, or null
 // for the device Philox stream.
 hipError_t launch_active_step(mavi_sys *s, const double *noise_dev, hipStream_t st);
-hipError_t launch_rings_forces(mavi_sys *s, hipStream_t st);
+// The ring pass of forces! can also give the learned propagator's per-ring
+// inputs (RingRates): per slot u = v0 p + mu F, per ring the rate of its area
+// (in units of the target area) and the alignment angle; it clears *gate.
+hipError_t launch_rings_forces(mavi_sys *s, hipStream_t st, const RingRates *rates = nullptr);
 hipError_t launch_rings_init(mavi_sys *s, hipStream_t st);
 hipError_t launch_ring_neigh(mavi_sys *s, hipStream_t st);
 // check_invasions!: ring boxes and the ring grid, then the crossing tests;

@@ -1191,7 +1191,8 @@ hipError_t drop_keys(mavi_sys *s, hipStream_t st) {
     return launch_fill_i32(s->cnt, 0, (int64_t)s->P.ncell + 1, st);
 }
 
-static hipError_t launch_rings_kernel(mavi_sys *s, bool update, const double *noise_dev, hipStream_t st) {
+static hipError_t launch_rings_kernel(mavi_sys *s, bool update, const double *noise_dev, hipStream_t st,
+                                      const RingRates *rates = nullptr) {
     ProfScope ps(s, MAVI_PROF_RINGS, st);
     const int c = s->cur;
     const double *nz = update ? noise_dev : nullptr;
@@ -1207,15 +1208,19 @@ static hipError_t launch_rings_kernel(mavi_sys *s, bool update, const double *no
         if (update && s->P.nwalls > 0)
             hipLaunchKernelGGL((k_rings_lanes<true, true>), dim3(nb), dim3(256), 0, st, s->P, s->rtype, s->types,
                                s->slot_of, s->P_[c], s->F, s->pol, s->areas, s->cms, s->cms_next, s->cpos, nz,
-                               s->seed, s->counter, nk, nr, nc, nf);
+                               s->seed, s->counter, nk, nr, nc, nf, RingRates{});
         else if (update)
             hipLaunchKernelGGL(k_rings_lanes<true>, dim3(nb), dim3(256), 0, st, s->P, s->rtype, s->types, s->slot_of,
                                s->P_[c], s->F, s->pol, s->areas, s->cms, s->cms_next, s->cpos, nz, s->seed,
-                               s->counter, nk, nr, nc, nf);
+                               s->counter, nk, nr, nc, nf, RingRates{});
+        else if (rates)
+            hipLaunchKernelGGL((k_rings_lanes<false, false, true>), dim3(nb), dim3(256), 0, st, s->P, s->rtype,
+                               s->types, s->slot_of, s->P_[c], s->F, s->pol, s->areas, s->cms, s->cms_next, s->cpos,
+                               nz, s->seed, s->counter, nk, nr, nc, nf, *rates);
         else
             hipLaunchKernelGGL(k_rings_lanes<false>, dim3(nb), dim3(256), 0, st, s->P, s->rtype, s->types,
                                s->slot_of, s->P_[c], s->F, s->pol, s->areas, s->cms, s->cms_next, s->cpos, nz,
-                               s->seed, s->counter, nk, nr, nc, nf);
+                               s->seed, s->counter, nk, nr, nc, nf, RingRates{});
         return hipGetLastError();
     }
     if (update)
@@ -1276,7 +1281,7 @@ hipError_t launch_var_pre(mavi_sys *s, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_rings_forces(mavi_sys *s, hipStream_t st) {
+hipError_t launch_rings_forces(mavi_sys *s, hipStream_t st, const RingRates *rates) {
     if (s->P.na == 0) return hipSuccess;
     hipError_t e = launch_bin(s, st, false, false, false);
     if (e != hipSuccess) return e;
@@ -1285,7 +1290,7 @@ hipError_t launch_rings_forces(mavi_sys *s, hipStream_t st) {
         if ((e = launch_pair<MODE_FORCES>(s, s->P_[s->cur], st)) != hipSuccess) return e;
     }
     if (s->neigh_on && (e = launch_ring_neigh(s, st)) != hipSuccess) return e;
-    return launch_rings_kernel(s, false, nullptr, st);
+    return launch_rings_kernel(s, false, nullptr, st, rates);
 }
 
 hipError_t launch_rings_step(mavi_sys *s, const double *noise_dev, hipStream_t st) {

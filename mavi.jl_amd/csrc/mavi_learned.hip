@@ -658,11 +658,20 @@ namespace {
 
 hipError_t launch_features(mavi_learned *l, hipStream_t st) {
     mavi_sys *s = l->s;
-    hipError_t e = mavi::launch_rings_forces(s, st);
+    // forces! with the per-ring inputs (u, area rate, alignment) from the
+    // same ring pass (nmax <= 32 packs rings into waves), else k_ring_rates
+    RingRates rr;
+    rr.U = l->U;
+    rr.rate = l->rate;
+    rr.omega = l->omega;
+    rr.gate = l->gate;
+    const bool fused = s->P.nmax <= 32;
+    hipError_t e = mavi::launch_rings_forces(s, st, fused ? &rr : nullptr);
     if (e != hipSuccess) return e;
     const int c = s->cur;
-    hipLaunchKernelGGL(k_ring_rates, dim3(nblk((int64_t)s->P.nrings * 32, 256)), dim3(256), 0, st, s->P, s->slot_of,
-                       s->P_[c], s->F, l->U, s->pol, s->rtype, s->types, l->rate, l->omega, l->gate);
+    if (!fused)
+        hipLaunchKernelGGL(k_ring_rates, dim3(nblk((int64_t)s->P.nrings * 32, 256)), dim3(256), 0, st, s->P,
+                           s->slot_of, s->P_[c], s->F, l->U, s->pol, s->rtype, s->types, l->rate, l->omega, l->gate);
     hipLaunchKernelGGL(k_features, dim3(nblk(s->P.na, kFeatWG)), dim3(kFeatWG), 0, st, s->P, l->L, s->P_[c],
                        s->ID_[c], s->slot_of, s->scell, s->start, l->U, s->pol, s->areas, s->rtype, s->types, l->rate,
                        l->omega, l->X, l->nbr, l->qprev, l->euler, l->gate);
