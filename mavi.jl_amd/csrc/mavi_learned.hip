@@ -4,8 +4,10 @@
 //
 // A learned step from the state x(t) (one hipGraph, then one 4-byte read of
 // the gate word):
-//   1. binning + forces!  (plain kernels: pair, springs, area; FP64)
-//   2. k_ring_rates + k_features: per particle, in the frame of its ring's
+//   1. binning + forces!  (plain kernels: pair, springs, area; FP64; the
+//      ring kernel also gives u = v0 p + mu F and the per-ring rates, or
+//      k_ring_rates for rings of more than 32 particles)
+//   2. k_features: per particle, in the frame of its ring's
 //      polarisation (rotation-canonical), 32 self features (ring type, area
 //      deviation and its rate, the alignment term, the two ring neighbours
 //      and their velocities relative to it, the overdamped velocity
