@@ -733,6 +733,7 @@ constexpr int kScanItemsSmall = 16, kScanItemsLarge = 64;
 constexpr int kScanTileMin = kScanThreads * kScanItemsSmall;  // bounds the tile count from above
 constexpr int kScanLargeFrom = 1 << 20;
 constexpr uint32_t kScanMaxPolls = 1u << 20;
+constexpr int kScanLook = 4;  // tiles per lane in one look-back polling round
 
 __device__ __forceinline__ unsigned long long scan_word(uint32_t epoch, uint32_t kind, int32_t v) {
     return ((unsigned long long)epoch << 34) | ((unsigned long long)kind << 32) | (uint32_t)v;
@@ -779,44 +780,50 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_lookback(const int32_t *_
         __hip_atomic_store(&state[tile], scan_word(epoch, tile == 0 ? 2u : 1u, total), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     if (threadIdx.x < 64) {
-        // one wave looks back over windows of 256 tiles, 4 per lane (lane l
-        // holds tiles j - 4l - q, q = 0..3, nearest first): a tile far from
-        // the start needs a quarter of the polling rounds one tile per lane
-        // would, and each round waits on memory
+        // one wave looks back over windows of 64 * kScanLook tiles (lane l
+        // holds tiles j - kScanLook * l - q, q < kScanLook, nearest first): a
+        // tile far from the start needs 1 / kScanLook of the polling rounds
+        // one tile per lane would, and each round waits on memory once (the
+        // lane's loads are all issued before any result is used; a tile
+        // before tile 0 loads tile 0's word and ignores it)
         const int lane = threadIdx.x;
         int32_t excl = 0;
         int j = tile - 1;  // nearest tile not yet accounted for
         uint32_t polls = 0;
         bool stalled = false;
         while (j >= 0) {
-            uint32_t kind[4];
-            int32_t val[4];
-            int first_pre = 4;  // nearest tile of this lane holding an inclusive prefix
+            unsigned long long w[kScanLook];
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int t = j - 4 * lane - q;
+            for (int q = 0; q < kScanLook; q++) {
+                const int t = j - kScanLook * lane - q;
+                w[q] = __hip_atomic_load(&state[t < 0 ? 0 : t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            uint32_t kind[kScanLook];
+            int32_t val[kScanLook];
+            int first_pre = kScanLook;  // nearest tile of this lane holding an inclusive prefix
+#pragma unroll
+            for (int q = 0; q < kScanLook; q++) {
+                const int t = j - kScanLook * lane - q;
                 kind[q] = 3u;  // before tile 0: contributes nothing
                 val[q] = 0;
                 if (t >= 0) {
-                    const unsigned long long w =
-                        __hip_atomic_load(&state[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     kind[q] = 0u;
-                    if ((uint32_t)(w >> 34) == epoch) {
-                        kind[q] = (uint32_t)(w >> 32) & 3u;
-                        val[q] = (int32_t)(uint32_t)w;
+                    if ((uint32_t)(w[q] >> 34) == epoch) {
+                        kind[q] = (uint32_t)(w[q] >> 32) & 3u;
+                        val[q] = (int32_t)(uint32_t)w[q];
                     }
                 }
-                if (kind[q] == 2u && first_pre == 4) first_pre = q;
+                if (kind[q] == 2u && first_pre == kScanLook) first_pre = q;
             }
             // stop at the nearest inclusive prefix, and never step over a
             // tile that has published nothing yet
-            const unsigned long long pre = __ballot(first_pre < 4);
+            const unsigned long long pre = __ballot(first_pre < kScanLook);
             const int stop = pre ? __ffsll(pre) - 1 : 64;
-            const int last_q = lane < stop ? 3 : (lane == stop ? first_pre : -1);
+            const int last_q = lane < stop ? kScanLook - 1 : (lane == stop ? first_pre : -1);
             bool missing = false;
             int32_t part = 0;
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
+            for (int q = 0; q < kScanLook; q++) {
                 if (q <= last_q) {
                     missing = missing || kind[q] == 0u;
                     part += val[q];
@@ -834,7 +841,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_lookback(const int32_t *_
             for (int d = 32; d > 0; d >>= 1) part += __shfl_xor(part, d, 64);
             excl += part;
             if (pre) break;
-            j -= 256;
+            j -= 64 * kScanLook;
         }
         if (lane == 0) {
             if (tile > 0)
