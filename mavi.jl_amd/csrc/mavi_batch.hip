@@ -61,25 +61,14 @@ enum { KIND_LJ = 0, KIND_HT = 1, KIND_RINGS = 2, NKIND = 3 };
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// Chunk and arrival rank of every padded slot (items == nullptr), or of the
-// slots of the waves items[0, nitems).  Padding slots go to their replica's
-// pad chunk.  nslots is a multiple of 64: whole waves leave.
+// Chunk and arrival rank of every padded slot.  Padding slots go to their
+// replica's pad chunk.  nslots is a multiple of 64: whole waves leave.
 __global__ void kb_bin_count(const Replica *__restrict__ reps, const int32_t *__restrict__ wave_rep, int32_t nslots,
-                             const int2 *__restrict__ items, int32_t nitems,
                              const double2 *__restrict__ pos, int32_t *__restrict__ key, int32_t *__restrict__ rank,
                              int32_t *__restrict__ cnt, uint32_t *__restrict__ flags) {
-    int s, r;
-    if (items) {
-        const int wl = blockIdx.x * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);
-        if (wl >= nitems) return;
-        const int2 it = items[wl];
-        s = uniform(it.x) * 64 + (int)lane_id();
-        r = uniform(it.y);
-    } else {
-        s = blockIdx.x * blockDim.x + threadIdx.x;
-        if (s >= nslots) return;
-        r = uniform(wave_rep[s >> 6]);
-    }
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    const int r = uniform(wave_rep[s >> 6]);
     const Replica &Q = reps[r];
     uint32_t flag = 0;
     int32_t k;
@@ -255,6 +244,9 @@ __global__ void kb_pair(const Replica *__restrict__ reps, const int2 *__restrict
 // Springs, area force and overdamped update of every ring of the rings
 // replicas: wave wl holds rings of replica ring_wave_rep[wl] only, floor(64 /
 // nmax) of them, from ring (wl - rwbase) * floor(64 / nmax) of the replica.
+// Then the next binning's first pass for the replica's slots, as kb_bin_count
+// makes it: the chunk and arrival rank of every final position, and the
+// replica's first ring wave puts its padding slots in its pad chunk.
 __global__ __launch_bounds__(256) void kb_rings(const Replica *__restrict__ reps,
                                                 const int32_t *__restrict__ ring_wave_rep, int32_t nwaves,
                                                 const int32_t *__restrict__ rtype, const RingType *__restrict__ types,
@@ -262,7 +254,9 @@ __global__ __launch_bounds__(256) void kb_rings(const Replica *__restrict__ reps
                                                 double2 *__restrict__ F, double *__restrict__ pol,
                                                 double *__restrict__ areas, double2 *__restrict__ cms,
                                                 double2 *__restrict__ cms_next, double2 *__restrict__ cpos,
-                                                const unsigned long long *__restrict__ counter) {
+                                                const unsigned long long *__restrict__ counter,
+                                                int32_t *__restrict__ key, int32_t *__restrict__ rank,
+                                                int32_t *__restrict__ cnt, uint32_t *__restrict__ flags) {
     const int wl = blockIdx.x * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6);
     if (wl >= nwaves) return;
     const int r = uniform(ring_wave_rep[wl]);
@@ -278,7 +272,20 @@ __global__ __launch_bounds__(256) void kb_rings(const Replica *__restrict__ reps
     const RingType T = types[Q.tbase + (P.typed ? rtype[Q.rbase + rc] : 0)];
     ring_lanes<true>(P, T, ring_ok, rc, grp * P.nmax, i, slot_of + Q.nbase, Pst, F, pol + Q.rbase,
                      areas + Q.rbase, cms + Q.rbase, cms_next + Q.rbase, cpos + Q.cpbase, nullptr, Q.seed,
-                     counter[0]);
+                     counter[0], key, rank, cnt, flags + r, RingRates{}, Q.cbase);
+    if (wl == Q.rwbase) {
+        // padding slots [pbase + na, pbase + npad): all in the last wave of
+        // the replica's slots
+        const int tail = P.na & 63;
+        const bool pad = tail != 0 && w >= tail;
+        const int s = Q.pbase + (P.na - tail) + w;
+        const int32_t k = Q.cbase + P.ncell;
+        const int32_t rk = wave_arrival_rank(pad, k, cnt);
+        if (pad) {
+            key[s] = k;
+            rank[s] = rk;
+        }
+    }
 }
 
 }  // namespace
@@ -363,13 +370,10 @@ hipError_t batch_bin(mavi_batch *b, hipStream_t st) {
         hipError_t e = mavi::launch_fill_i32(b->cnt, 0, n, st);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(kb_bin_count, dim3(nblk(ns, 256)), dim3(256), 0, st, b->d_reps, b->wave_rep, ns,
-                           nullptr, 0, b->P_[c], b->key, b->rank, b->cnt, b->flags);
-    } else if (b->nitems[KIND_RINGS] > 0) {
-        // the Verlet replicas' second evaluation counted theirs
-        hipLaunchKernelGGL(kb_bin_count, dim3(nblk(b->nitems[KIND_RINGS], 4)), dim3(256), 0, st, b->d_reps,
-                           b->wave_rep, ns, b->items[KIND_RINGS], b->nitems[KIND_RINGS], b->P_[c], b->key, b->rank,
-                           b->cnt, b->flags);
+                           b->P_[c], b->key, b->rank, b->cnt, b->flags);
     }
+    // (else the last step did the first pass: the Verlet replicas' second
+    // evaluation and the ring update, padding slots included)
     // a stalled scan is reported through replica 0's flags
     launch_scan(b->cnt, n, b->start, b->scan_ticket, b->scan_state, b->flags, st);
     hipLaunchKernelGGL(k_scatter, dim3(nblk(std::max(ns, n), 256)), dim3(256), 0, st, b->G, b->key, b->rank,
@@ -430,7 +434,7 @@ hipError_t batch_step(mavi_batch *b, hipStream_t st) {
         BatchProf ps(b, MAVI_PROF_RINGS, st);
         hipLaunchKernelGGL(kb_rings, dim3(nblk(b->nring_waves, 4)), dim3(256), 0, st, b->d_reps, b->ring_wave_rep,
                            b->nring_waves, b->rtype, b->types, b->slot_of, b->P_[b->cur], b->F, b->pol, b->areas,
-                           b->cms, b->cms_next, b->cpos, b->counter);
+                           b->cms, b->cms_next, b->cpos, b->counter, b->key, b->rank, b->cnt, b->flags);
     }
     b->keys_ready = true;
     return hipGetLastError();

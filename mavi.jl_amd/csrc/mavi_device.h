@@ -986,7 +986,7 @@ __device__ __forceinline__ void ring_lanes(const SysParams &P, const RingType &T
                                            const uint64_t seed, const uint64_t step,
                                            int32_t *__restrict__ nkey = nullptr, int32_t *__restrict__ nrank = nullptr,
                                            int32_t *__restrict__ ncnt = nullptr, uint32_t *__restrict__ nflags = nullptr,
-                                           const RingRates RR = RingRates{}) {
+                                           const RingRates RR = RingRates{}, const int32_t kbase = 0) {
     const int r = rr;
     const int np = T.np;
     const bool act = ring_ok && i < np;
@@ -1156,11 +1156,11 @@ __device__ __forceinline__ void ring_lanes(const SysParams &P, const RingType &T
     if (act) Pst[slot] = make_double2(px, py);
     if (ncnt) {
         // the next binning's first pass (chunk and arrival rank of the final
-        // position, as k_bin_count makes them); every lane of the wave takes
-        // part in the arrival ranks
+        // position, as k_bin_count makes them; chunks numbered from kbase);
+        // every lane of the wave takes part in the arrival ranks
         uint32_t flag = 0;
         int32_t k = -1;
-        if (act) k = cell_of(P, px, py, &flag);
+        if (act) k = kbase + cell_of(P, px, py, &flag);
         if (flag) atomicOr(nflags, flag);
         const int32_t rk = wave_arrival_rank(act, k, ncnt);
         if (act) {
