@@ -85,7 +85,9 @@ __global__ void kb_bin_count(const Replica *__restrict__ reps, const int32_t *__
 }
 
 // k_finalize over all replicas; padding slots (id -1) never move, so they
-// keep id -1 in both state copies.
+// keep id -1 in both state copies.  slot_of is kept for the rings replicas
+// only (the ring update is its one reader; an LJ replica's id order is
+// scattered over its slots, so those writes would be scattered too).
 __global__ void kb_finalize(const Replica *__restrict__ reps, const int32_t *__restrict__ wave_rep, int32_t nslots,
                             const int32_t *__restrict__ start, const int32_t *__restrict__ src,
                             const int32_t *__restrict__ scell, const int32_t *__restrict__ sid,
@@ -96,9 +98,11 @@ __global__ void kb_finalize(const Replica *__restrict__ reps, const int32_t *__r
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t == 0) epoch[0] += 1u;  // binnings so far
     if (t >= nslots) return;
+    const int r = uniform(wave_rep[t >> 6]);
     const int so = src[t];
     const int myid = sid[t];
     if (myid < 0) return;
+    const Replica &Q = reps[r];
     const int c = scell[t];
     const int b = start[c], e = start[c + 1];
     int rk = 0;
@@ -107,11 +111,8 @@ __global__ void kb_finalize(const Replica *__restrict__ reps, const int32_t *__r
     Pnew[t2] = Pold[so];
     perm[t2] = so;  // velocities move in the first evaluation (rings replicas use none)
     IDnew[t2] = myid;
-    slot_of[myid] = t2;
-    if (t == b) {
-        const int r = wave_rep[t >> 6];
-        if (e - b > reps[r].P.nc) atomicOr(&flags[r], MAVI_FLAG_OVERFLOW);
-    }
+    if (Q.kind == KIND_RINGS) slot_of[myid] = t2;
+    if (t == b && e - b > Q.P.nc) atomicOr(&flags[r], MAVI_FLAG_OVERFLOW);
 }
 
 __global__ void kb_unpermute(int32_t nslots, const int32_t *__restrict__ ID, const double2 *__restrict__ P,
