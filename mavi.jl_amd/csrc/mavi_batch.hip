@@ -10,8 +10,8 @@
 // pad chunk that collects the npad - na padding slots: one counting sort over
 // the concatenated chunk space then keeps every replica inside its slot
 // range, in the same order a single-system binning gives.  Particle ids are
-// concatenated too (nbase + local id), and ring arrays (rbase, even) so that
-// a wavefront of the ring kernel holds two rings of one replica.
+// concatenated too (nbase + local id), and ring arrays (rbase, even); a
+// wavefront of the ring kernel holds floor(64 / nmax) rings of one replica.
 //
 // A step is: binning of all replicas, one pair launch per kind and
 // evaluation (VERLET1 / VERLET2 for LJ and HarmTrunc replicas, FORCES for
@@ -23,8 +23,9 @@
 // HarmTrunc replica does the next binning's first pass on the positions it
 // writes (chunk, arrival rank) and marks, in the next binning's epoch, the
 // chunks of the 3x3 blocks around a particle's old and new chunk when it
-// changes chunk.  The binning then counts only the rings replicas' slots,
-// writes the permutation instead of moving velocities, and the first
+// changes chunk; the ring kernel does the same first pass for the rings
+// replicas (padding slots included).  The binning then starts from the
+// scan, writes the permutation instead of moving velocities, and the first
 // evaluation moves velocities through it and takes the previous second
 // evaluation wherever its chunk is not marked (same positions, same
 // candidates, same order: the same bits).  Particles to recompute are
